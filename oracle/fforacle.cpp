@@ -1,0 +1,1616 @@
+/*
+ * fforacle.cpp -- CPU oracle for the FLIP substep path.
+ *
+ * TEST INFRASTRUCTURE ONLY: a from-scratch, single-threaded restatement of
+ * the reference engine's per-substep algorithms (rlguy/Blender-FLIP-Fluids
+ * 1.8.5).  Only tests, the smoke check and the benchmark's CPU-baseline leg
+ * load it; the shipped HIP engine never does.
+ *
+ * Every function names the reference lines it restates.  The restatement
+ * keeps the reference's float/double promotion points and operation order
+ * (x86-64 SSE2, no FMA contraction: built with -ffp-contract=off) so that,
+ * given the same inputs, it reproduces the reference bit for bit.  Two facts
+ * about the plain configuration are used and stated where they apply:
+ *   - solids are static, so solid face velocities are exactly zero and the
+ *     divergence's solid terms (pressuresolver.cpp:652-657) add +-0.0 to a
+ *     sum that starts at +0.0, which never changes it;
+ *   - the 7-point stencil never gives two rows of one MIC(0) column that are
+ *     neighbours, so the factorisation reduces to its "pull" form over the
+ *     -z, -y, -x neighbours (pcgsolver.h:68-205).
+ * Parity of every function is pinned by the reference build in oracle/_ref
+ * (tests/test_oracle_vs_reference.py) and by committed golden vectors.
+ */
+#include "fforacle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------- vec3 ----
+// vmath::vec3 semantics: three floats, component-wise float arithmetic
+// (vmath.h:37-91, vmath.cpp:51-120).
+struct V3 {
+    float x, y, z;
+};
+inline V3 mk(double x, double y, double z) { return V3{(float)x, (float)y, (float)z}; }
+inline V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float lengthsq(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline float length(V3 a) { return std::sqrt(lengthsq(a)); }
+// vmath::operator/(vec3, float): multiply by the float reciprocal of a double division
+inline V3 divide(V3 a, float s) {
+    float inv = 1.0 / s;
+    return V3{a.x * inv, a.y * inv, a.z * inv};
+}
+inline V3 normalize(V3 a) { return divide(a, length(a)); }
+
+struct GI {
+    int i, j, k;
+};
+
+// Grid3d helpers (grid3d.h:32-124)
+inline GI posToIndex(V3 p, double dx) {
+    double inv = 1.0 / dx;
+    return GI{(int)std::floor(p.x * inv), (int)std::floor(p.y * inv), (int)std::floor(p.z * inv)};
+}
+inline GI posToIndexD(double x, double y, double z, double dx) {
+    double inv = 1.0 / dx;
+    return GI{(int)std::floor(x * inv), (int)std::floor(y * inv), (int)std::floor(z * inv)};
+}
+inline V3 indexToPos(int i, int j, int k, double dx) {
+    return mk((float)i * dx, (float)j * dx, (float)k * dx);
+}
+inline V3 indexToCellCenter(int i, int j, int k, double dx) {
+    double hw = 0.5 * dx;
+    return mk((float)i * dx + hw, (float)j * dx + hw, (float)k * dx + hw);
+}
+inline bool inRange(int i, int j, int k, int ni, int nj, int nk) {
+    return i >= 0 && j >= 0 && k >= 0 && i < ni && j < nj && k < nk;
+}
+inline size_t flat(int i, int j, int k, int ni, int nj) {
+    return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
+}
+
+// ---------------------------------------------------------------- AABB ----
+// aabb.cpp:32-38, 122-133, 489-518
+struct Box {
+    V3 pos;
+    double w, h, d;
+    void expand(double v) {
+        double hh = 0.5 * v;
+        pos = pos - mk(hh, hh, hh);
+        w += v;
+        h += v;
+        d += v;
+    }
+    bool inside(V3 p) const {
+        return p.x >= pos.x && p.y >= pos.y && p.z >= pos.z && p.x < pos.x + w &&
+               p.y < pos.y + h && p.z < pos.z + d;
+    }
+    V3 maxPoint() const { return pos + mk(w, h, d); }
+    V3 nearestInside(V3 p) const {
+        if (inside(p)) return p;
+        double eps = 1e-6;
+        V3 mn = pos, mx = maxPoint();
+        p.x = std::fmax(p.x, mn.x);
+        p.y = std::fmax(p.y, mn.y);
+        p.z = std::fmax(p.z, mn.z);
+        p.x = std::fmin(p.x, mx.x - eps);
+        p.y = std::fmin(p.y, mx.y - eps);
+        p.z = std::fmin(p.z, mx.z - eps);
+        return p;
+    }
+};
+
+// FluidSimulation::_getBoundaryAABB (fluidsimulation.cpp:5175-5180)
+Box boundaryBox(int ni, int nj, int nk, double dx) {
+    Box b{V3{0.0f, 0.0f, 0.0f}, ni * dx, nj * dx, nk * dx};
+    b.expand(-3 * dx - 1e-4);
+    return b;
+}
+
+// ------------------------------------------------------- solid box SDF ----
+// Collision::findClosestPointOnTriangle (collision.cpp:161-268)
+double clampd(double v, double lo, double hi) {
+    if (v < lo) v = lo;
+    if (v > hi) v = hi;
+    return v;
+}
+V3 closestPointOnTriangle(V3 p0, V3 v0, V3 v1, V3 v2) {
+    V3 edge0 = v1 - v0;
+    V3 edge1 = v2 - v0;
+    V3 pv = v0 - p0;
+    double a = dot(edge0, edge0);
+    double b = dot(edge0, edge1);
+    double c = dot(edge1, edge1);
+    double d = dot(edge0, pv);
+    double e = dot(edge1, pv);
+    double det = a * c - b * b;
+    if (std::abs(det) < 1e-9) {
+        float feps = 1e-6;
+        float len1 = length(v0 - v1), len2 = length(v1 - v2), len3 = length(v2 - v0);
+        if (len1 < feps && len2 < feps && len3 < feps) return v0;
+        V3 dp, dq;
+        if (len1 < feps) {
+            dp = v0;
+            dq = v2;
+        } else if (len2 < feps) {
+            dp = v1;
+            dq = v0;
+        } else {
+            dp = v2;
+            dq = v1;
+        }
+        float dotnum = dot(p0 - dp, dq - dp);
+        float dotdem = dot(dq - dp, dq - dp);
+        if (std::abs(dotdem) < feps) return v0;
+        float lambda = dotnum / dotdem;
+        if (lambda <= 0.0f) return dp;
+        if (lambda >= 1.0f) return dq;
+        return dp + lambda * (dq - dp);
+    }
+    double s = b * e - c * d;
+    double t = b * d - a * e;
+    if (s + t < det) {
+        if (s < 0.0) {
+            if (t < 0.0) {
+                if (d < 0.0) {
+                    s = clampd(-d / a, 0.0, 1.0);
+                    t = 0.0;
+                } else {
+                    s = 0.0;
+                    t = clampd(-e / c, 0.0, 1.0);
+                }
+            } else {
+                s = 0.0;
+                t = clampd(-e / c, 0.0, 1.0);
+            }
+        } else if (t < 0.0) {
+            s = clampd(-d / a, 0.0, 1.0);
+            t = 0.0;
+        } else {
+            double invDet = 1.0 / det;
+            s *= invDet;
+            t *= invDet;
+        }
+    } else {
+        if (s < 0.0) {
+            double tmp0 = b + d, tmp1 = c + e;
+            if (tmp1 > tmp0) {
+                s = clampd((tmp1 - tmp0) / (a - 2 * b + c), 0.0, 1.0);
+                t = 1 - s;
+            } else {
+                t = clampd(-e / c, 0.0, 1.0);
+                s = 0.0;
+            }
+        } else if (t < 0.0) {
+            if (a + d > b + e) {
+                s = clampd((c + e - b - d) / (a - 2 * b + c), 0.0, 1.0);
+                t = 1 - s;
+            } else {
+                s = clampd(-e / c, 0.0, 1.0);
+                t = 0.0;
+            }
+        } else {
+            s = clampd((c + e - b - d) / (a - 2 * b + c), 0.0, 1.0);
+            t = 1.0 - s;
+        }
+    }
+    return v0 + (float)s * edge0 + (float)t * edge1;
+}
+
+}  // namespace
+
+/* Nodal solid SDF of the domain boundary.  Restates
+ *   FluidSimulation::_getBoundaryAABB / _getTriangleMeshFromAABB (fluidsimulation.cpp:5148-5186),
+ *   _addStaticObjectsToSDF (5277-5287): fastCalculateSignedDistanceField + negate,
+ *   MeshLevelSet::_computeExactBandDistanceFieldMultiThreaded (meshlevelset.cpp:781-1163):
+ *     per triangle a node band from its AABB (+-3 nodes, _initializeTriangleData 878-918);
+ *     distances evaluated in 10^3-node block-local coordinates (1115-1163);
+ *   _computeDistanceFieldSigns (1298-1318): nodes inside the box are negated.
+ * For the axis-aligned box the inside test is the box test itself: no node lies
+ * on the box faces (they sit 1e-4 off the grid planes). */
+void oracle_solid_box_phi(int ni, int nj, int nk, double dx, float *phi) {
+    const int pi = ni + 1, pj = nj + 1, pk = nk + 1;
+    const int band = 3, bw = 10;
+    Box box = boundaryBox(ni, nj, nk, dx);
+    V3 p = box.pos;
+    V3 verts[8] = {
+        mk(p.x, p.y, p.z),
+        mk(p.x + box.w, p.y, p.z),
+        mk(p.x + box.w, p.y, p.z + box.d),
+        mk(p.x, p.y, p.z + box.d),
+        mk(p.x, p.y + box.h, p.z),
+        mk(p.x + box.w, p.y + box.h, p.z),
+        mk(p.x + box.w, p.y + box.h, p.z + box.d),
+        mk(p.x, p.y + box.h, p.z + box.d),
+    };
+    const int tris[12][3] = {{0, 1, 2}, {0, 2, 3}, {4, 7, 6}, {4, 6, 5}, {0, 3, 7}, {0, 7, 4},
+                             {1, 5, 6}, {1, 6, 2}, {0, 4, 5}, {0, 5, 1}, {3, 2, 6}, {3, 6, 7}};
+    float ub = (float)((pi + pj + pk) * dx);  // getDistanceUpperBound (meshlevelset.cpp:769-771)
+    size_t nn = (size_t)pi * pj * pk;
+    for (size_t n = 0; n < nn; n++) phi[n] = ub;
+
+    for (int t = 0; t < 12; t++) {
+        V3 a = verts[tris[t][0]], b = verts[tris[t][1]], c = verts[tris[t][2]];
+        // AABB(Triangle, vertices) (aabb.cpp:84-115)
+        double minx = a.x, miny = a.y, minz = a.z, maxx = a.x, maxy = a.y, maxz = a.z;
+        V3 pts[3] = {a, b, c};
+        for (int q = 0; q < 3; q++) {
+            minx = std::fmin(pts[q].x, minx);
+            miny = std::fmin(pts[q].y, miny);
+            minz = std::fmin(pts[q].z, minz);
+            maxx = std::fmax(pts[q].x, maxx);
+            maxy = std::fmax(pts[q].y, maxy);
+            maxz = std::fmax(pts[q].z, maxz);
+        }
+        Box tb{mk(minx, miny, minz), maxx - minx, maxy - miny, maxz - minz};
+        GI gmin = posToIndex(tb.pos, dx);
+        GI gmax = posToIndex(tb.maxPoint(), dx);
+        gmin.i -= band; gmin.j -= band; gmin.k -= band;
+        gmax.i += band + 1; gmax.j += band + 1; gmax.k += band + 1;
+        if (gmax.i < 0 || gmin.i >= ni + 1 || gmax.j < 0 || gmin.j >= nj + 1 || gmax.k < 0 ||
+            gmin.k >= nk + 1)
+            continue;
+        gmin.i = std::max(gmin.i, 0); gmin.j = std::max(gmin.j, 0); gmin.k = std::max(gmin.k, 0);
+        gmax.i = std::min(gmax.i, ni); gmax.j = std::min(gmax.j, nj); gmax.k = std::min(gmax.k, nk);
+        for (int k = gmin.k; k <= gmax.k; k++)
+            for (int j = gmin.j; j <= gmax.j; j++)
+                for (int i = gmin.i; i <= gmax.i; i++) {
+                    int bi = i / bw, bj = j / bw, bk = k / bw;
+                    V3 off = indexToPos(bi, bj, bk, bw * dx);
+                    V3 gpos = indexToPos(i - bi * bw, j - bj * bw, k - bk * bw, dx);
+                    V3 cp = closestPointOnTriangle(gpos, a - off, b - off, c - off);
+                    float dist = length(cp - gpos);
+                    size_t n = flat(i, j, k, pi, pj);
+                    if (dist < phi[n]) phi[n] = dist;
+                }
+    }
+    // sign (inside the box mesh -> negative), then MeshLevelSet::negate (meshlevelset.cpp:702-708)
+    for (int k = 0; k < pk; k++)
+        for (int j = 0; j < pj; j++)
+            for (int i = 0; i < pi; i++) {
+                size_t n = flat(i, j, k, pi, pj);
+                V3 g = indexToPos(i, j, k, dx);
+                bool in = box.inside(g);
+                float v = in ? -phi[n] : phi[n];
+                phi[n] = -v;
+            }
+}
+
+/* Counting sort by the cell index the engine computes for a particle
+ * (Grid3d::positionToGridIndex, grid3d.h:53-58; used e.g. at
+ * fluidsimulation.cpp:7832).  Stable: within a cell, ascending particle index. */
+void oracle_cell_sort(int ni, int nj, int nk, double dx, int np, const float *pos, int *keys,
+                      int *counts, int *offsets, int *perm) {
+    size_t N = (size_t)ni * nj * nk;
+    for (size_t c = 0; c < N; c++) counts[c] = 0;
+    for (int p = 0; p < np; p++) {
+        V3 x{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        GI g = posToIndex(x, dx);
+        int key = inRange(g.i, g.j, g.k, ni, nj, nk) ? (int)flat(g.i, g.j, g.k, ni, nj) : -1;
+        keys[p] = key;
+        if (key >= 0) counts[key]++;
+    }
+    offsets[0] = 0;
+    for (size_t c = 0; c < N; c++) offsets[c + 1] = offsets[c] + counts[c];
+    std::vector<int> cur(offsets, offsets + N);
+    for (int p = 0; p < np; p++)
+        if (keys[p] >= 0) perm[cur[keys[p]]++] = p;
+}
+
+namespace {
+
+// GridUtils::featherGrid26 (gridutils.cpp:264-300)
+void feather26(std::vector<char> &g, int ni, int nj, int nk) {
+    std::vector<char> src = g;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                if (!src[flat(i, j, k, ni, nj)]) continue;
+                for (int dk = -1; dk <= 1; dk++)
+                    for (int dj = -1; dj <= 1; dj++)
+                        for (int di = -1; di <= 1; di++)
+                            if (inRange(i + di, j + dj, k + dk, ni, nj, nk))
+                                g[flat(i + di, j + dj, k + dk, ni, nj)] = 1;
+            }
+}
+
+// GridUtils::featherGrid6 (gridutils.cpp:229-262)
+void feather6(std::vector<char> &g, int ni, int nj, int nk) {
+    std::vector<char> src = g;
+    const int d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                if (!src[flat(i, j, k, ni, nj)]) continue;
+                for (int q = 0; q < 6; q++)
+                    if (inRange(i + d[q][0], j + d[q][1], k + d[q][2], ni, nj, nk))
+                        g[flat(i + d[q][0], j + d[q][1], k + d[q][2], ni, nj)] = 1;
+            }
+}
+
+// Block membership of one point, shared by the particle level set
+// (particlelevelset.cpp:466-521) and the velocity advector
+// (velocityadvector.cpp:296-353): a point whose +-sr box lies strictly inside
+// its own block belongs to that block only, otherwise to every active block
+// its box overlaps.  Returns the member block indices.
+int blockMembers(V3 p, float sr, float blockdx, int bi, int bj, int bk,
+                 const std::vector<char> &active, GI *out) {
+    GI b = posToIndex(p, blockdx);
+    V3 bp = indexToPos(b.i, b.j, b.k, blockdx);
+    if (p.x - sr > bp.x && p.y - sr > bp.y && p.z - sr > bp.z && p.x + sr < bp.x + blockdx &&
+        p.y + sr < bp.y + blockdx && p.z + sr < bp.z + blockdx) {
+        if (inRange(b.i, b.j, b.k, bi, bj, bk) && active[flat(b.i, b.j, b.k, bi, bj)]) {
+            out[0] = b;
+            return 1;
+        }
+        return 0;
+    }
+    GI g0 = posToIndexD(p.x - sr, p.y - sr, p.z - sr, blockdx);
+    GI g1 = posToIndexD(p.x + sr, p.y + sr, p.z + sr, blockdx);
+    int n = 0;
+    for (int k = g0.k; k <= g1.k; k++)
+        for (int j = g0.j; j <= g1.j; j++)
+            for (int i = g0.i; i <= g1.i; i++)
+                if (inRange(i, j, k, bi, bj, bk) && active[flat(i, j, k, bi, bj)]) out[n++] = GI{i, j, k};
+    return n;
+}
+
+}  // namespace
+
+/* ParticleLevelSet::_computeSignedDistanceFromParticles (particlelevelset.cpp:335-411)
+ * with _initializeBlockGrid (413-452), _computeGridCountDataThread (484-537) and
+ * _computeExactBandProducerThread (620-666).  Min over the member particles of
+ * |cell centre - p| - r, in block-local coordinates, clipped at 3*dx. */
+void oracle_liquid_sdf(int ni, int nj, int nk, double dx, int np, const float *pos,
+                       double radius, float *phi) {
+    const int bw = 10;
+    float maxd = (float)(3.0 * dx);
+    size_t N = (size_t)ni * nj * nk;
+    for (size_t c = 0; c < N; c++) phi[c] = maxd;
+    if (np == 0) return;
+    int bi = (ni + bw - 1) / bw, bj = (nj + bw - 1) / bw, bk = (nk + bw - 1) / bw;
+    float blockdx = bw * dx;
+    std::vector<char> active((size_t)bi * bj * bk, 0);
+    for (int p = 0; p < np; p++) {
+        V3 x{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        GI g = posToIndex(x, blockdx);
+        if (inRange(g.i, g.j, g.k, bi, bj, bk)) active[flat(g.i, g.j, g.k, bi, bj)] = 1;
+    }
+    feather26(active, bi, bj, bk);
+    float r = radius;
+    float sr = 2.0f * r;
+    GI members[27];
+    for (int p = 0; p < np; p++) {
+        V3 x{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        int nm = blockMembers(x, sr, blockdx, bi, bj, bk, active, members);
+        for (int m = 0; m < nm; m++) {
+            GI b = members[m];
+            V3 off = indexToPos(b.i, b.j, b.k, bw * dx);
+            V3 pl = x - off;
+            GI g0 = posToIndex(V3{pl.x - sr, pl.y - sr, pl.z - sr}, dx);
+            GI g1 = posToIndex(V3{pl.x + sr, pl.y + sr, pl.z + sr}, dx);
+            g0.i = std::max(g0.i, 0); g0.j = std::max(g0.j, 0); g0.k = std::max(g0.k, 0);
+            g1.i = std::min(g1.i, bw - 1); g1.j = std::min(g1.j, bw - 1); g1.k = std::min(g1.k, bw - 1);
+            for (int k = g0.k; k <= g1.k; k++)
+                for (int j = g0.j; j <= g1.j; j++)
+                    for (int i = g0.i; i <= g1.i; i++) {
+                        V3 gpos = indexToCellCenter(i, j, k, dx);
+                        float dist = length(gpos - pl) - r;
+                        int ci = b.i * bw + i, cj = b.j * bw + j, ck = b.k * bw + k;
+                        if (!inRange(ci, cj, ck, ni, nj, nk)) continue;
+                        size_t c = flat(ci, cj, ck, ni, nj);
+                        if (dist < phi[c]) phi[c] = dist;
+                    }
+        }
+    }
+}
+
+/* ParticleLevelSet::postProcessSignedDistanceField (particlelevelset.cpp:170-194)
+ * with MeshLevelSet::getDistanceAtCellCenter (meshlevelset.cpp:152-162). */
+void oracle_liquid_sdf_postprocess(int ni, int nj, int nk, double dx, const float *sp,
+                                   float *phi) {
+    int pi = ni + 1, pj = nj + 1;
+    float eps = 0.005 * dx;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                size_t c = flat(i, j, k, ni, nj);
+                if (phi[c] < 0.5 * dx) {
+                    float sc = 0.125f * (sp[flat(i, j, k, pi, pj)] + sp[flat(i + 1, j, k, pi, pj)] +
+                                         sp[flat(i, j + 1, k, pi, pj)] + sp[flat(i + 1, j + 1, k, pi, pj)] +
+                                         sp[flat(i, j, k + 1, pi, pj)] + sp[flat(i + 1, j, k + 1, pi, pj)] +
+                                         sp[flat(i, j + 1, k + 1, pi, pj)] +
+                                         sp[flat(i + 1, j + 1, k + 1, pi, pj)]);
+                    if (sc < 0) phi[c] = -0.5f * dx;
+                }
+                float val = phi[c];
+                if (std::abs(val) < eps) phi[c] = val > 0 ? eps : -eps;
+            }
+}
+
+/* VelocityAdvector::_advectGrid for one direction with the FLIP producer
+ * (velocityadvector.cpp:74-175, 177-251, 296-353, 467-536).  Contributions to a
+ * face come from the face's own 10^3 block, whose particle list is in ascending
+ * particle index, so the sums here run in ascending particle index too. */
+void oracle_p2g(int ni, int nj, int nk, double dx, int np, const float *pos, const float *vel,
+                double radius, int dir, float *field, unsigned char *valid) {
+    const int bw = 10;
+    int wi = ni + (dir == 0), wj = nj + (dir == 1), wk = nk + (dir == 2);
+    size_t F = (size_t)wi * wj * wk;
+    std::vector<float> sum(F, 0.0f), wsum(F, 0.0f);
+    int bi = (wi + bw - 1) / bw, bj = (wj + bw - 1) / bw, bk = (wk + bw - 1) / bw;
+    double chunkdx = dx * bw;
+    V3 offset = dir == 0 ? mk(0.0, 0.5 * dx, 0.5 * dx)
+                         : (dir == 1 ? mk(0.5 * dx, 0.0, 0.5 * dx) : mk(0.5 * dx, 0.5 * dx, 0.0));
+    std::vector<char> active((size_t)bi * bj * bk, 0);
+    for (int p = 0; p < np; p++) {
+        V3 x = V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]} - offset;
+        GI g = posToIndex(x, chunkdx);
+        if (inRange(g.i, g.j, g.k, bi, bj, bk)) active[flat(g.i, g.j, g.k, bi, bj)] = 1;
+    }
+    feather26(active, bi, bj, bk);
+    float eps = 1e-6;
+    float r = radius;
+    float sr = radius + eps;
+    float blockdx = chunkdx;
+    float rsq = r * r;
+    float coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
+    float coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
+    float coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
+    GI members[27];
+    for (int p = 0; p < np; p++) {
+        V3 x = V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]} - offset;
+        float velocity = vel[3 * p + dir];
+        int nm = blockMembers(x, sr, blockdx, bi, bj, bk, active, members);
+        for (int m = 0; m < nm; m++) {
+            GI b = members[m];
+            V3 off = indexToPos(b.i, b.j, b.k, bw * dx);
+            V3 pl = x - off;
+            GI g0 = posToIndex(V3{pl.x - sr, pl.y - sr, pl.z - sr}, dx);
+            GI g1 = posToIndex(V3{pl.x + sr, pl.y + sr, pl.z + sr}, dx);
+            g0.i = std::max(g0.i, 0); g0.j = std::max(g0.j, 0); g0.k = std::max(g0.k, 0);
+            g1.i = std::min(g1.i, bw - 1); g1.j = std::min(g1.j, bw - 1); g1.k = std::min(g1.k, bw - 1);
+            for (int k = g0.k; k <= g1.k; k++)
+                for (int j = g0.j; j <= g1.j; j++)
+                    for (int i = g0.i; i <= g1.i; i++) {
+                        V3 gpos = indexToPos(i, j, k, dx);
+                        V3 v = gpos - pl;
+                        float d2 = dot(v, v);
+                        if (!(d2 < rsq)) continue;
+                        float weight = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
+                        int fi = b.i * bw + i, fj = b.j * bw + j, fk = b.k * bw + k;
+                        if (!inRange(fi, fj, fk, wi, wj, wk)) continue;
+                        size_t f = flat(fi, fj, fk, wi, wj);
+                        sum[f] += weight * velocity;
+                        wsum[f] += weight;
+                    }
+        }
+    }
+    for (size_t f = 0; f < F; f++) {
+        float s = sum[f];
+        if (wsum[f] > eps) s /= wsum[f];
+        field[f] = s;
+        valid[f] = wsum[f] > eps;
+    }
+}
+
+/* GridUtils::extrapolateGrid (gridutils.h:95-160) with _initializeStatusGridThread
+ * (gridutils.cpp:99-118), _findExtrapolationCells (120-175) and
+ * _extrapolateCellsThread (gridutils.h:43-92). */
+void oracle_extrapolate(int wi, int wj, int wk, float *field, const unsigned char *valid,
+                        int layers) {
+    const char UNKNOWN = 0, WAITING = 1, KNOWN = 2, DONE = 3;
+    size_t F = (size_t)wi * wj * wk;
+    std::vector<char> st(F, UNKNOWN);
+    for (int k = 0; k < wk; k++)
+        for (int j = 0; j < wj; j++)
+            for (int i = 0; i < wi; i++) {
+                size_t f = flat(i, j, k, wi, wj);
+                if (i == 0 || j == 0 || k == 0 || i == wi - 1 || j == wj - 1 || k == wk - 1)
+                    st[f] = DONE;
+                else if (valid[f])
+                    st[f] = KNOWN;
+            }
+    const long long sx = 1, sy = wi, sz = (long long)wi * wj;
+    const long long nb[6] = {sx, -sx, sy, -sy, sz, -sz};
+    std::vector<size_t> cells;
+    for (int layer = 0; layer < layers; layer++) {
+        cells.clear();
+        for (size_t f = 0; f < F; f++) {
+            if (st[f] != KNOWN) continue;
+            for (int q = 0; q < 6; q++) {
+                size_t n = (size_t)((long long)f + nb[q]);
+                if (st[n] == UNKNOWN) {
+                    st[n] = WAITING;
+                    cells.push_back(n);
+                }
+            }
+            st[f] = DONE;
+        }
+        for (size_t c : cells) {
+            float s = 0.0f;
+            int count = 0;
+            for (int q = 0; q < 6; q++) {
+                size_t n = (size_t)((long long)c + nb[q]);
+                if (st[n] == DONE) {
+                    s += field[n];
+                    count++;
+                }
+            }
+            field[c] = s / (float)count;
+        }
+        if (layer != layers - 1)
+            for (size_t c : cells) st[c] = KNOWN;
+    }
+}
+
+namespace {
+
+// LevelsetUtils::fractionInside, two-point form (levelsetutils.cpp:39-51)
+float fractionInside2(float l, float r) {
+    if (l < 0 && r < 0) return 1;
+    if (l < 0 && r >= 0) return l / (l - r);
+    if (l >= 0 && r < 0) return r / (r - l);
+    return 0;
+}
+void cycle4(float *a) {
+    float t = a[0];
+    a[0] = a[1];
+    a[1] = a[2];
+    a[2] = a[3];
+    a[3] = t;
+}
+// LevelsetUtils::fractionInside, four-corner form (levelsetutils.cpp:62-141)
+float fractionInside4(float bl, float br, float tl, float tr) {
+    int in = (bl < 0 ? 1 : 0) + (tl < 0 ? 1 : 0) + (br < 0 ? 1 : 0) + (tr < 0 ? 1 : 0);
+    float L[4] = {bl, br, tr, tl};
+    if (in == 4) return 1;
+    if (in == 3) {
+        while (L[0] < 0) cycle4(L);
+        float s0 = 1 - fractionInside2(L[0], L[3]);
+        float s1 = 1 - fractionInside2(L[0], L[1]);
+        return 1.0f - 0.5f * s0 * s1;
+    }
+    if (in == 2) {
+        while (L[0] >= 0 || !(L[1] < 0 || L[2] < 0)) cycle4(L);
+        if (L[1] < 0) {
+            float sl = fractionInside2(L[0], L[3]);
+            float sr = fractionInside2(L[1], L[2]);
+            return 0.5f * (sl + sr);
+        }
+        float mid = 0.25f * (L[0] + L[1] + L[2] + L[3]);
+        if (mid < 0) {
+            float area = 0;
+            float s1 = 1 - fractionInside2(L[0], L[3]);
+            float s3 = 1 - fractionInside2(L[2], L[3]);
+            area += 0.5f * s1 * s3;
+            float s2 = 1 - fractionInside2(L[2], L[1]);
+            float s0 = 1 - fractionInside2(L[0], L[1]);
+            area += 0.5f * s0 * s2;
+            return 1.0f - area;
+        }
+        float area = 0;
+        float s0 = fractionInside2(L[0], L[1]);
+        float s1 = fractionInside2(L[0], L[3]);
+        area += 0.5f * s0 * s1;
+        float s2 = fractionInside2(L[2], L[1]);
+        float s3 = fractionInside2(L[2], L[3]);
+        area += 0.5f * s2 * s3;
+        return area;
+    }
+    if (in == 1) {
+        while (L[0] >= 0) cycle4(L);
+        float s0 = fractionInside2(L[0], L[3]);
+        float s1 = fractionInside2(L[0], L[1]);
+        return 0.5f * s0 * s1;
+    }
+    return 0;
+}
+float clampf(float v, float lo, float hi) { return std::max(lo, std::min(v, hi)); }
+
+}  // namespace
+
+/* FluidSimulation::_updateWeightGridThread (fluidsimulation.cpp:6110-6150) with
+ * MeshLevelSet::getFaceWeightU/V/W (meshlevelset.cpp:357-391). */
+void oracle_face_weights(int ni, int nj, int nk, const float *sp, float *wu, float *wv, float *ww) {
+    int pi = ni + 1, pj = nj + 1;
+    auto P = [&](int i, int j, int k) { return sp[flat(i, j, k, pi, pj)]; };
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni + 1; i++)
+                wu[flat(i, j, k, ni + 1, nj)] =
+                    clampf(1.0f - fractionInside4(P(i, j, k), P(i, j + 1, k), P(i, j, k + 1), P(i, j + 1, k + 1)), 0.0f, 1.0f);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj + 1; j++)
+            for (int i = 0; i < ni; i++)
+                wv[flat(i, j, k, ni, nj + 1)] =
+                    clampf(1.0f - fractionInside4(P(i, j, k), P(i, j, k + 1), P(i + 1, j, k), P(i + 1, j, k + 1)), 0.0f, 1.0f);
+    for (int k = 0; k < nk + 1; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                ww[flat(i, j, k, ni, nj)] =
+                    clampf(1.0f - fractionInside4(P(i, j, k), P(i, j + 1, k), P(i + 1, j, k), P(i + 1, j + 1, k)), 0.0f, 1.0f);
+}
+
+namespace {
+
+struct PressureSystem {
+    int m = 0;
+    std::vector<int> rows;                 // flat cell id per row, k,j,i order
+    std::vector<int> key;                  // cell -> row or -1
+    std::vector<double> rhs, diag, ox, oy, oz, invdiag;
+    std::vector<int> nbr;                  // 6 per row: -x,+x,-y,+y,-z,+z
+};
+
+/* PressureSolver::_initialize (pressuresolver.cpp:118-159),
+ * _calculateNegativeDivergenceVectorThread (624-713; the solid terms are exact
+ * no-ops for zero solid velocity, see the file header) and
+ * _calculateMatrixCoefficientsThread (767-867) with SparseMatrix::add/set
+ * (sparsematrix.h:72-111; add ignores a -1 column). */
+void buildSystem(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
+                 const float *w, const float *wu, const float *wv, const float *ww,
+                 const float *phi, PressureSystem &S) {
+    size_t N = (size_t)ni * nj * nk;
+    S.key.assign(N, -1);
+    S.rows.clear();
+    for (int k = 1; k < nk - 1; k++)
+        for (int j = 1; j < nj - 1; j++)
+            for (int i = 1; i < ni - 1; i++) {
+                size_t c = flat(i, j, k, ni, nj);
+                if (phi[c] < 0) {
+                    S.key[c] = (int)S.rows.size();
+                    S.rows.push_back((int)c);
+                }
+            }
+    int m = S.m = (int)S.rows.size();
+    S.rhs.assign(m, 0.0);
+    S.diag.assign(m, 0.0);
+    S.ox.assign(m, 0.0);
+    S.oy.assign(m, 0.0);
+    S.oz.assign(m, 0.0);
+    S.nbr.assign((size_t)6 * m, -1);
+    const double factor = 1.0 / dx;
+    const double mfactor = dt / (dx * dx);
+    const double eps = 1e-9, maxtheta = 25;
+    auto U = [&](int i, int j, int k) { return u[flat(i, j, k, ni + 1, nj)]; };
+    auto V = [&](int i, int j, int k) { return v[flat(i, j, k, ni, nj + 1)]; };
+    auto W = [&](int i, int j, int k) { return w[flat(i, j, k, ni, nj)]; };
+    auto WU = [&](int i, int j, int k) { return wu[flat(i, j, k, ni + 1, nj)]; };
+    auto WV = [&](int i, int j, int k) { return wv[flat(i, j, k, ni, nj + 1)]; };
+    auto WW = [&](int i, int j, int k) { return ww[flat(i, j, k, ni, nj)]; };
+    auto PH = [&](int i, int j, int k) { return phi[flat(i, j, k, ni, nj)]; };
+    auto KEY = [&](int i, int j, int k) { return S.key[flat(i, j, k, ni, nj)]; };
+    for (int r = 0; r < m; r++) {
+        int c = S.rows[r];
+        int i = c % ni, j = (c / ni) % nj, k = c / (ni * nj);
+        double volRight = WU(i + 1, j, k), volLeft = WU(i, j, k);
+        double volTop = WV(i, j + 1, k), volBottom = WV(i, j, k);
+        double volFront = WW(i, j, k + 1), volBack = WW(i, j, k);
+        double div = 0.0;
+        div += -factor * volRight * U(i + 1, j, k);
+        div += factor * volLeft * U(i, j, k);
+        div += -factor * volTop * V(i, j + 1, k);
+        div += factor * volBottom * V(i, j, k);
+        div += -factor * volFront * W(i, j, k + 1);
+        div += factor * volBack * W(i, j, k);
+        S.rhs[r] = div;
+
+        // density grid is 1.0 everywhere on the plain path (fluidsimulation.cpp:6214)
+        double rhoC = 1.0f;
+        double rhoR = (rhoC + 1.0f) / 2.0, rhoL = (rhoC + 1.0f) / 2.0, rhoT = (rhoC + 1.0f) / 2.0;
+        double rhoB = (rhoC + 1.0f) / 2.0, rhoF = (rhoC + 1.0f) / 2.0, rhoK = (rhoC + 1.0f) / 2.0;
+        double phiC = PH(i, j, k);
+        double phiR = PH(i + 1, j, k), phiL = PH(i - 1, j, k), phiT = PH(i, j + 1, k);
+        double phiB = PH(i, j - 1, k), phiF = PH(i, j, k + 1), phiK = PH(i, j, k - 1);
+        double diag = (volRight / rhoR + volLeft / rhoL + volTop / rhoT + volBottom / rhoB +
+                       volFront / rhoF + volBack / rhoK) * mfactor;
+        int *nb = &S.nbr[(size_t)6 * r];
+        auto side = [&](double phiN, double vol, double rho, int nkey, int slot, double *store) {
+            if (phiN < 0.0) {
+                if (nkey >= 0) {
+                    nb[slot] = nkey;
+                    if (store) *store = -(vol / rho) * mfactor;
+                }
+            } else {
+                double theta = phiN / (phiC + eps);
+                theta = std::max(-maxtheta, std::min(theta, maxtheta));
+                diag -= (vol / rho) * mfactor * theta;
+            }
+        };
+        side(phiR, volRight, rhoR, KEY(i + 1, j, k), 1, &S.ox[r]);
+        side(phiL, volLeft, rhoL, KEY(i - 1, j, k), 0, nullptr);
+        side(phiT, volTop, rhoT, KEY(i, j + 1, k), 3, &S.oy[r]);
+        side(phiB, volBottom, rhoB, KEY(i, j - 1, k), 2, nullptr);
+        side(phiF, volFront, rhoF, KEY(i, j, k + 1), 5, &S.oz[r]);
+        side(phiK, volBack, rhoK, KEY(i, j, k - 1), 4, nullptr);
+        diag = std::max(diag, 0.0);
+        S.diag[r] = diag;
+    }
+}
+
+/* factorModifiedIncompleteColesky0 (pcgsolver.h:68-205) in pull form: each row
+ * receives its updates from the -z, -y, -x neighbour columns in that (column)
+ * order; within a column the "missing" sum runs over the column's other rows
+ * in ascending row order. */
+void factorMIC0(PressureSystem &S) {
+    const double omega = 0.97, ratio = 0.25, eps = 1e-9;
+    int m = S.m;
+    std::vector<double> adiag(m);
+    S.invdiag.assign(m, 0.0);
+    for (int r = 0; r < m; r++) {
+        double a = S.diag[r];
+        if (a < eps) a = 0;
+        adiag[r] = a;
+        S.invdiag[r] = a;
+    }
+    // coefficient between row r and its +x/+y/+z neighbour, by slot 1/3/5
+    auto up = [&](int r, int slot) { return slot == 1 ? S.ox[r] : (slot == 3 ? S.oy[r] : S.oz[r]); };
+    for (int j = 0; j < m; j++) {
+        const int *nb = &S.nbr[(size_t)6 * j];
+        double d = S.invdiag[j];
+        // columns k < j that hold row j: -z (slot 4), -y (slot 2), -x (slot 0)
+        const int cslots[3] = {4, 2, 0};
+        for (int q = 0; q < 3; q++) {
+            int k = nb[cslots[q]];
+            if (k < 0 || adiag[k] == 0) continue;
+            int jslot = cslots[q] + 1;  // row j is k's +z / +y / +x neighbour
+            const int *kb = &S.nbr[(size_t)6 * k];
+            double Ljk = up(k, jslot) * S.invdiag[k];
+            d -= Ljk * Ljk;
+            double missing = 0;
+            const int order[3] = {1, 3, 5};  // k's rows in ascending order: +x, +y, +z
+            for (int o = 0; o < 3; o++) {
+                int s = order[o];
+                if (s == jslot || kb[s] < 0) continue;
+                missing += up(k, s) * S.invdiag[k];
+            }
+            d -= omega * Ljk * missing;
+        }
+        if (adiag[j] == 0) {
+            S.invdiag[j] = d;
+            continue;
+        }
+        if (d < ratio * adiag[j])
+            S.invdiag[j] = 1 / std::sqrt(adiag[j]);
+        else
+            S.invdiag[j] = 1 / std::sqrt(d);
+    }
+}
+
+// L entry of row (k's neighbour in slot) in column k: scaled unless column k is null
+inline double Lval(const PressureSystem &S, const std::vector<double> &adiagZero, int k, int slot) {
+    double a = slot == 1 ? S.ox[k] : (slot == 3 ? S.oy[k] : S.oz[k]);
+    return adiagZero[k] ? a : a * S.invdiag[k];
+}
+
+/* solveLower + solveLowerTransposeInPlace (pcgsolver.h:211-241) in pull form. */
+void applyMIC0(const PressureSystem &S, const std::vector<double> &nullcol, const double *rv,
+               double *z) {
+    int m = S.m;
+    for (int j = 0; j < m; j++) {
+        const int *nb = &S.nbr[(size_t)6 * j];
+        double y = rv[j];
+        const int cslots[3] = {4, 2, 0};
+        for (int q = 0; q < 3; q++) {
+            int k = nb[cslots[q]];
+            if (k < 0) continue;
+            y -= Lval(S, nullcol, k, cslots[q] + 1) * z[k];
+        }
+        z[j] = y * S.invdiag[j];
+    }
+    for (int i = m - 1; i >= 0; i--) {
+        const int *nb = &S.nbr[(size_t)6 * i];
+        double x = z[i];
+        const int order[3] = {1, 3, 5};
+        for (int o = 0; o < 3; o++) {
+            int rr = nb[order[o]];
+            if (rr < 0) continue;
+            x -= Lval(S, nullcol, i, order[o]) * z[rr];
+        }
+        z[i] = x * S.invdiag[i];
+    }
+}
+
+std::vector<double> nullColumns(const PressureSystem &S) {
+    std::vector<double> nc(S.m);
+    for (int r = 0; r < S.m; r++) nc[r] = (S.diag[r] < 1e-9) ? 1.0 : 0.0;
+    return nc;
+}
+
+// ThreadUtils::splitRangeIntoIntervals (threadutils.cpp:54-72)
+std::vector<int> splitRange(int b, int e, int n) {
+    int size = (int)std::floor((double)(e - b) / (double)n);
+    int rem = (e - b) - size * n;
+    std::vector<int> iv;
+    iv.push_back(b);
+    int cur = b;
+    for (int i = 0; i < n; i++) {
+        int end = cur + size;
+        if (i < rem) end++;
+        iv.push_back(end);
+        cur = end;
+    }
+    return iv;
+}
+
+// BLAS::dot (blaswrapper.h:27-57): serial, or per-thread partial sums added in thread order
+double blasDot(const std::vector<double> &x, const std::vector<double> &y, int ncpu) {
+    int nt = (int)std::fmin(ncpu, std::ceil((float)x.size() / (float)500000));
+    if (nt <= 1) {
+        double s = 0;
+        for (size_t i = 0; i < x.size(); i++) s += x[i] * y[i];
+        return s;
+    }
+    std::vector<int> iv = splitRange(0, (int)x.size(), nt);
+    double s = 0;
+    for (int t = 0; t < nt; t++) {
+        double part = 0;
+        for (int i = iv[t]; i < iv[t + 1]; i++) part += x[i] * y[i];
+        s += part;
+    }
+    return s;
+}
+double absMax(const std::vector<double> &x) {
+    double mx = 0;
+    for (double v : x)
+        if (std::fabs(v) > mx) mx = std::fabs(v);
+    return mx;
+}
+
+/* FixedSparseMatrix multiply (sparsematrix.h:174-215): each row sums its
+ * entries in ascending column order: -z, -y, -x, diag, +x, +y, +z. */
+void spmv(const PressureSystem &S, const std::vector<double> &x, std::vector<double> &out) {
+    for (int r = 0; r < S.m; r++) {
+        const int *nb = &S.nbr[(size_t)6 * r];
+        double s = 0;
+        if (nb[4] >= 0) s += S.oz[nb[4]] * x[nb[4]];
+        if (nb[2] >= 0) s += S.oy[nb[2]] * x[nb[2]];
+        if (nb[0] >= 0) s += S.ox[nb[0]] * x[nb[0]];
+        s += S.diag[r] * x[r];
+        if (nb[1] >= 0) s += S.ox[r] * x[nb[1]];
+        if (nb[3] >= 0) s += S.oy[r] * x[nb[3]];
+        if (nb[5] >= 0) s += S.oz[r] * x[nb[5]];
+        out[r] = s;
+    }
+}
+
+/* PCGSolver<double>::solve (pcgsolver.h:268-322) with BLAS::addScaled taking a
+ * float step length (blaswrapper.h:128-157).  Returns success. */
+bool pcg(PressureSystem &S, double tolFactor, int maxIter, int ncpu, std::vector<double> &x,
+         double &resid, int &iters) {
+    int m = S.m;
+    if (tolFactor < 1e-30) tolFactor = 1e-30;
+    std::vector<double> r = S.rhs, z(m), s(m);
+    x.assign(m, 0.0);
+    resid = absMax(r);
+    if (resid == 0) {
+        iters = 0;
+        return true;
+    }
+    double tol = tolFactor * resid;
+    factorMIC0(S);
+    std::vector<double> nc = nullColumns(S);
+    applyMIC0(S, nc, r.data(), z.data());
+    double rho = blasDot(z, r, ncpu);
+    if (rho == 0 || rho != rho) {
+        iters = 0;
+        return false;
+    }
+    s = z;
+    int it;
+    for (it = 0; it < maxIter; it++) {
+        spmv(S, s, z);
+        double alpha = rho / blasDot(s, z, ncpu);
+        float fa = alpha, fna = -alpha;
+        for (int i = 0; i < m; i++) x[i] += fa * s[i];
+        for (int i = 0; i < m; i++) r[i] += fna * z[i];
+        resid = absMax(r);
+        if (resid <= std::min(tol, 1.0)) {
+            iters = it + 1;
+            return true;
+        }
+        applyMIC0(S, nc, r.data(), z.data());
+        double rhoNew = blasDot(z, r, ncpu);
+        double beta = rhoNew / rho;
+        float fb = beta;
+        for (int i = 0; i < m; i++) z[i] += fb * s[i];
+        s.swap(z);
+        rho = rhoNew;
+    }
+    iters = it;
+    return false;
+}
+
+}  // namespace
+
+int oracle_pressure_system(int ni, int nj, int nk, double dx, double dt, const float *u,
+                           const float *v, const float *w, const float *wu, const float *wv,
+                           const float *ww, const float *phi, int *rows, double *rhs,
+                           double *diag, double *ox, double *oy, double *oz, int *nbr,
+                           double *invdiag) {
+    PressureSystem S;
+    buildSystem(ni, nj, nk, dx, dt, u, v, w, wu, wv, ww, phi, S);
+    factorMIC0(S);
+    int m = S.m;
+    if (rows) std::copy(S.rows.begin(), S.rows.end(), rows);
+    if (rhs) std::copy(S.rhs.begin(), S.rhs.end(), rhs);
+    if (diag) std::copy(S.diag.begin(), S.diag.end(), diag);
+    if (ox) std::copy(S.ox.begin(), S.ox.end(), ox);
+    if (oy) std::copy(S.oy.begin(), S.oy.end(), oy);
+    if (oz) std::copy(S.oz.begin(), S.oz.end(), oz);
+    if (nbr) std::copy(S.nbr.begin(), S.nbr.end(), nbr);
+    if (invdiag) std::copy(S.invdiag.begin(), S.invdiag.end(), invdiag);
+    return m;
+}
+
+void oracle_apply_mic0(int m, const double *diag, const double *ox, const double *oy,
+                       const double *oz, const int *nbr, const double *invdiag, const double *r,
+                       double *z) {
+    PressureSystem S;
+    S.m = m;
+    S.diag.assign(diag, diag + m);
+    S.ox.assign(ox, ox + m);
+    S.oy.assign(oy, oy + m);
+    S.oz.assign(oz, oz + m);
+    S.nbr.assign(nbr, nbr + (size_t)6 * m);
+    S.invdiag.assign(invdiag, invdiag + m);
+    std::vector<double> nc = nullColumns(S);
+    applyMIC0(S, nc, r, z);
+}
+
+/* PressureSolver::solve (pressuresolver.cpp:52-92) and _solveLinearSystem
+ * (869-916): early exit when max|rhs| < tol, PCG, float pressure grid, and the
+ * acceptable-tolerance rule (maxIterations reached with |r| < 1.0). */
+int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt, const float *u,
+                          const float *v, const float *w, const float *wu, const float *wv,
+                          const float *ww, const float *phi, double tol, int maxIter,
+                          int nthreads, float *pressure, int *status, double *err) {
+    PressureSystem S;
+    buildSystem(ni, nj, nk, dx, dt, u, v, w, wu, wv, ww, phi, S);
+    size_t N = (size_t)ni * nj * nk;
+    for (size_t c = 0; c < N; c++) pressure[c] = 0.0f;
+    double maxAbs = 0.0;
+    for (double x : S.rhs)
+        if (std::fabs(x) > maxAbs) maxAbs = std::fabs(x);
+    if (maxAbs < tol) {
+        status[0] = 0;
+        status[1] = 1;
+        err[0] = 0.0;
+        return S.m;
+    }
+    std::vector<double> x;
+    double resid = -1.0;
+    int iters = 0;
+    bool ok = pcg(S, tol, maxIter, nthreads, x, resid, iters);
+    for (int r = 0; r < S.m; r++) pressure[S.rows[r]] = (float)x[r];
+    float e = (float)resid;
+    bool ret = ok || (iters == maxIter && e < 1.0);
+    status[0] = iters;
+    status[1] = ret ? 1 : 0;
+    err[0] = e;
+    return S.m;
+}
+
+/* PressureSolver::applySolutionToVelocityField (pressuresolver.cpp:94-112) and
+ * _applyPressureToVelocityFieldThread (1047-1205), surface tension off, density 1. */
+void oracle_apply_pressure(int ni, int nj, int nk, double dx, double dt, const float *p,
+                           const float *phi, const float *wu, const float *wv, const float *ww,
+                           float *u, float *v, float *w, unsigned char *vu, unsigned char *vv,
+                           unsigned char *vw) {
+    float factor = dt / dx;
+    float eps = 1e-6;
+    const double maxtheta = 25, tension = 0.0;
+    auto fluid = [&](int i, int j, int k) { return phi[flat(i, j, k, ni, nj)] < 0.0; };
+    auto P = [&](int i, int j, int k) { return p[flat(i, j, k, ni, nj)]; };
+    auto PH = [&](int i, int j, int k) { return phi[flat(i, j, k, ni, nj)]; };
+    auto clampT = [&](float t) { return (float)std::max(-maxtheta, std::min((double)t, maxtheta)); };
+    // one face between cells a (low side) and b (high side); "bordering fluid"
+    // is isFaceBorderingMaterialU/V/W (fluidmaterialgrid.cpp:126-146)
+    auto face = [&](float wgt, bool bordering, int ai, int aj, int ak, int bi, int bj, int bk,
+                    float &val, unsigned char &valid) {
+        if (wgt > 0 && bordering) {
+            float p1 = 0.0f, p2 = 0.0f;
+            if (fluid(ai, aj, ak) && fluid(bi, bj, bk)) {
+                p1 = P(ai, aj, ak);
+                p2 = P(bi, bj, bk);
+            } else {
+                float phi1 = PH(ai, aj, ak), phi2 = PH(bi, bj, bk);
+                if (fluid(ai, aj, ak)) {
+                    float thetaP = clampT(phi2 / (phi1 + eps));
+                    float thetaT = clampT((phi1 - phi2) / (phi1 + eps));
+                    p1 = P(ai, aj, ak);
+                    p2 = thetaT * tension + thetaP * p1;
+                } else {
+                    float thetaP = clampT(phi1 / (phi2 + eps));
+                    float thetaT = clampT((phi2 - phi1) / (phi2 + eps));
+                    p2 = P(bi, bj, bk);
+                    p1 = thetaT * tension + thetaP * p2;
+                }
+            }
+            float rho = (1.0f + 1.0f) / 2.0f;
+            val += (float)(-factor * (p2 - p1) / rho);
+            valid = 1;
+        } else {
+            val = 0.0;
+        }
+    };
+    size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+    std::fill(vu, vu + nu, 0);
+    std::fill(vv, vv + nv, 0);
+    std::fill(vw, vw + nw, 0);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni + 1; i++) {
+                if (i == 0 || i == ni - 1) continue;
+                size_t f = flat(i, j, k, ni + 1, nj);
+                // isFaceBorderingMaterialU (fluidmaterialgrid.cpp:126-130)
+                bool lo = i > 0 && fluid(i - 1, j, k);
+                bool hi = i < ni && fluid(i, j, k);
+                face(wu[f], lo || hi, i - 1, j, k, i, j, k, u[f], vu[f]);
+            }
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj + 1; j++)
+            for (int i = 0; i < ni; i++) {
+                if (j == 0 || j == nj - 1) continue;
+                size_t f = flat(i, j, k, ni, nj + 1);
+                bool lo = j > 0 && fluid(i, j - 1, k);
+                bool hi = j < nj && fluid(i, j, k);
+                face(wv[f], lo || hi, i, j - 1, k, i, j, k, v[f], vv[f]);
+            }
+    for (int k = 0; k < nk + 1; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                if (k == 0 || k == nk - 1) continue;
+                size_t f = flat(i, j, k, ni, nj);
+                bool lo = k > 0 && fluid(i, j, k - 1);
+                bool hi = k < nk && fluid(i, j, k);
+                face(ww[f], lo || hi, i, j, k - 1, i, j, k, w[f], vw[f]);
+            }
+}
+
+namespace {
+
+struct Mac {
+    int ni, nj, nk;
+    double dx;
+    const float *u, *v, *w;
+};
+
+bool inGridD(double x, double y, double z, double dx, int ni, int nj, int nk) {
+    return x >= 0 && y >= 0 && z >= 0 && x < dx * ni && y < dx * nj && z < dx * nk;
+}
+
+// Interpolation::trilinearInterpolate(double p[8], x, y, z) (interpolation.cpp:61-70)
+double trilin8(const double *p, double x, double y, double z) {
+    return p[0] * (1 - x) * (1 - y) * (1 - z) + p[1] * x * (1 - y) * (1 - z) +
+           p[2] * (1 - x) * y * (1 - z) + p[3] * (1 - x) * (1 - y) * z + p[4] * x * (1 - y) * z +
+           p[5] * (1 - x) * y * z + p[6] * x * y * (1 - z) + p[7] * x * y * z;
+}
+
+// MACVelocityField::_interpolateLinearU/V/W (macvelocityfield.cpp:519-613) after the
+// in-grid test on the unshifted point; (x, y, z) arrive with the two transverse
+// coordinates already shifted by -0.5*dx.
+double interpFace(const float *a, int wi, int wj, int wk, double dx, double x, double y, double z) {
+    int i, j, k;
+    {
+        double invdx = 1.0 / dx;
+        i = (int)std::floor(x * invdx);
+        j = (int)std::floor(y * invdx);
+        k = (int)std::floor(z * invdx);
+    }
+    double gx = (double)i * dx, gy = (double)j * dx, gz = (double)k * dx;
+    double inv_dx = 1 / dx;
+    double ix = (x - gx) * inv_dx, iy = (y - gy) * inv_dx, iz = (z - gz) * inv_dx;
+    double pts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                           {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+    for (int q = 0; q < 8; q++) {
+        int a0 = i + off[q][0], b0 = j + off[q][1], c0 = k + off[q][2];
+        if (inRange(a0, b0, c0, wi, wj, wk)) pts[q] = a[flat(a0, b0, c0, wi, wj)];
+    }
+    return trilin8(pts, ix, iy, iz);
+}
+
+// MACVelocityField::evaluateVelocityAtPositionLinear (macvelocityfield.cpp:631-645)
+V3 sampleMac(const Mac &M, V3 p) {
+    double x = p.x, y = p.y, z = p.z;
+    if (!inGridD(x, y, z, M.dx, M.ni, M.nj, M.nk)) return V3{0, 0, 0};
+    double h = 0.5 * M.dx;
+    double xu = interpFace(M.u, M.ni + 1, M.nj, M.nk, M.dx, x, y - h, z - h);
+    double yv = interpFace(M.v, M.ni, M.nj + 1, M.nk, M.dx, x - h, y, z - h);
+    double zw = interpFace(M.w, M.ni, M.nj, M.nk + 1, M.dx, x - h, y - h, z);
+    return mk(xu, yv, zw);
+}
+
+}  // namespace
+
+void oracle_sample_velocity(int ni, int nj, int nk, double dx, const float *u, const float *v,
+                            const float *w, int np, const float *pos, float *out) {
+    Mac M{ni, nj, nk, dx, u, v, w};
+    for (int p = 0; p < np; p++) {
+        V3 r = sampleMac(M, V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]});
+        out[3 * p] = r.x;
+        out[3 * p + 1] = r.y;
+        out[3 * p + 2] = r.z;
+    }
+}
+
+/* ====================================================== whole engine ===== */
+namespace {
+
+// Interpolation::trilinearInterpolate(vec3, dx, Array3d<float>) (interpolation.cpp:72-112)
+// on the nodal solid SDF, returned through MeshLevelSet::trilinearInterpolate as float
+// (meshlevelset.cpp:261-263).
+float nodalTrilinear(const float *g, int pi, int pj, int pk, double dx, V3 p) {
+    GI c = posToIndex(p, dx);
+    V3 gp = indexToPos(c.i, c.j, c.k, dx);
+    double inv_dx = 1.0 / dx;
+    double ix = (p.x - gp.x) * inv_dx, iy = (p.y - gp.y) * inv_dx, iz = (p.z - gp.z) * inv_dx;
+    double pts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                           {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+    for (int q = 0; q < 8; q++) {
+        int a = c.i + off[q][0], b = c.j + off[q][1], e = c.k + off[q][2];
+        if (inRange(a, b, e, pi, pj, pk)) pts[q] = g[flat(a, b, e, pi, pj)];
+    }
+    return (float)trilin8(pts, ix, iy, iz);
+}
+
+// Interpolation::bilinearInterpolate + trilinearInterpolateGradient (interpolation.cpp:189-260)
+double bilin(double v00, double v10, double v01, double v11, double ix, double iy) {
+    double l1 = (1 - ix) * v00 + ix * v10;
+    double l2 = (1 - ix) * v01 + ix * v11;
+    return (1 - iy) * l1 + iy * l2;
+}
+V3 nodalGradient(const float *g, int pi, int pj, int pk, double dx, V3 p) {
+    GI c = posToIndex(p, dx);
+    V3 gp = indexToPos(c.i, c.j, c.k, dx);
+    double inv_dx = 1.0 / dx;
+    double ix = (p.x - gp.x) * inv_dx, iy = (p.y - gp.y) * inv_dx, iz = (p.z - gp.z) * inv_dx;
+    auto at = [&](int a, int b, int e) -> float {
+        return inRange(a, b, e, pi, pj, pk) ? g[flat(a, b, e, pi, pj)] : 0.0f;
+    };
+    float v000 = at(c.i, c.j, c.k), v100 = at(c.i + 1, c.j, c.k), v010 = at(c.i, c.j + 1, c.k);
+    float v001 = at(c.i, c.j, c.k + 1), v101 = at(c.i + 1, c.j, c.k + 1);
+    float v011 = at(c.i, c.j + 1, c.k + 1), v110 = at(c.i + 1, c.j + 1, c.k);
+    float v111 = at(c.i + 1, c.j + 1, c.k + 1);
+    float ddx00 = v100 - v000, ddx10 = v110 - v010, ddx01 = v101 - v001, ddx11 = v111 - v011;
+    float dvdx = bilin(ddx00, ddx10, ddx01, ddx11, iy, iz);
+    float ddy00 = v010 - v000, ddy10 = v110 - v100, ddy01 = v011 - v001, ddy11 = v111 - v101;
+    float dvdy = bilin(ddy00, ddy10, ddy01, ddy11, ix, iz);
+    float ddz00 = v001 - v000, ddz10 = v101 - v100, ddz01 = v011 - v010, ddz11 = v111 - v110;
+    float dvdz = bilin(ddz00, ddz10, ddz01, ddz11, ix, iy);
+    return V3{dvdx, dvdy, dvdz};
+}
+
+}  // namespace
+
+struct OracleSim {
+    int ni, nj, nk;
+    double dx;
+    // settings (fluidsimulation.h:2158-2470 defaults)
+    double cfl = 5.0;
+    double picflip = 0.05;
+    int minSteps = 1, maxSteps = 6;
+    double ptol = 1e-9;
+    int maxPIters = 900;
+    int chunks = 1;
+    bool extremeRemoval = true;
+    std::vector<V3> bodyForces;
+    // state
+    bool initialized = false;
+    int frame = 0, stepNumber = 0;
+    double frameDt = 0, remaining = 0;
+    std::vector<V3> pos, vel;
+    std::vector<V3> loadPos, loadVel;
+    bool solidGiven = false;
+    std::vector<float> solid;         // nodal
+    std::vector<float> wu, wv, ww;    // face weights
+    std::vector<char> nearSolid;
+    int nsi = 0, nsj = 0, nsk = 0;
+    double nsCell = 0;
+    std::vector<float> phi, u, v, w, su, sv, sw, pres;
+    std::vector<unsigned char> vu, vv, vw;
+    int lastSubsteps = 0, lastIters = 0, pIters = 0;
+    bool pSuccess = true;
+    double lastErr = 0, pErr = 0;
+
+    size_t nU() const { return (size_t)(ni + 1) * nj * nk; }
+    size_t nV() const { return (size_t)ni * (nj + 1) * nk; }
+    size_t nW() const { return (size_t)ni * nj * (nk + 1); }
+    size_t nN() const { return (size_t)ni * nj * nk; }
+    size_t nNodes() const { return (size_t)(ni + 1) * (nj + 1) * (nk + 1); }
+
+    V3 bodyForce() const {
+        V3 b{0, 0, 0};
+        for (const V3 &f : bodyForces) b = b + f;
+        return b;
+    }
+
+    // FluidSimulation::_initializeSimulation / _loadMarkerParticles (fluidsimulation.cpp:4362-4398, 5010-5016)
+    void initialize() {
+        if (initialized) return;
+        Box bounds{V3{0, 0, 0}, ni * dx, nj * dx, nk * dx};
+        for (size_t p = 0; p < loadPos.size(); p++)
+            if (bounds.inside(loadPos[p])) {
+                pos.push_back(loadPos[p]);
+                vel.push_back(loadVel[p]);
+            }
+        loadPos.clear();
+        loadVel.clear();
+        if (!solidGiven) {
+            solid.resize(nNodes());
+            oracle_solid_box_phi(ni, nj, nk, dx, solid.data());
+        }
+        wu.resize(nU());
+        wv.resize(nV());
+        ww.resize(nW());
+        oracle_face_weights(ni, nj, nk, solid.data(), wu.data(), wv.data(), ww.data());
+        initialized = true;
+    }
+
+    double liquidRadius() const { return 0.5 * 1.0 * dx * std::sqrt(3.0); }  // fluidsimulation.cpp:4351
+
+    // _updateNearSolidGrid (fluidsimulation.cpp:5436-5480)
+    void updateNearSolid() {
+        nsCell = 3 * dx;
+        nsi = (int)std::ceil(ni * dx / nsCell);
+        nsj = (int)std::ceil(nj * dx / nsCell);
+        nsk = (int)std::ceil(nk * dx / nsCell);
+        nearSolid.assign((size_t)nsi * nsj * nsk, 0);
+        float maxd = 3 * dx;
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++)
+                for (int i = 0; i < ni; i++)
+                    if (std::abs(solid[flat(i, j, k, ni + 1, nj + 1)]) < maxd)
+                        nearSolid[flat(i / 3, j / 3, k / 3, nsi, nsj)] = 1;
+        int layers = (int)std::ceil((float)cfl / (float)3);
+        for (int l = 0; l < layers; l++) feather6(nearSolid, nsi, nsj, nsk);
+    }
+
+    // _getMaximumMarkerParticleSpeed / _predictMaximumMarkerParticleSpeed /
+    // _calculateNextTimeStep (fluidsimulation.cpp:10165-10263)
+    double nextTimeStep(double dt) {
+        double maxu = 0.0;
+        if (frame == 0 && stepNumber == 0) {
+            maxu += length(bodyForce()) * dt;
+        } else {
+            double maxsq = 0.0;
+            for (const V3 &v0 : vel) {
+                double d = dot(v0, v0);
+                if (d > maxsq) maxsq = d;
+            }
+            maxu = std::sqrt(maxsq);
+        }
+        if (!pos.empty()) maxu = std::max(0.0, maxu);
+        double eps = 1e-6;
+        double timeStep = cfl * dx / (maxu + eps);
+        int nsub = std::max((int)std::ceil(dt / timeStep), 1);
+        return dt / nsub;
+    }
+
+    // FluidSimulation::update (fluidsimulation.cpp:10410-10501)
+    void update(double dt) {
+        if (!initialized) throw std::runtime_error("Error: FluidSimulation must be initialized before update.\n");
+        if (dt < 0.0) throw std::domain_error("Error: delta time must be greater than or equal to 0.\n");
+        double epsdt = 1e-6;
+        bool zeroLen = dt < epsdt;
+        dt = std::max(dt, epsdt);
+        frameDt = dt;
+        remaining = dt;
+        stepNumber = 0;
+        bool skipped = zeroLen && frame > 0;
+        double substepTime = frameDt / (double)minSteps;
+        pSuccess = true;
+        pIters = 0;
+        pErr = 0.0;
+        double eps = 1e-9;
+        do {
+            double step = std::fmin(nextTimeStep(dt), remaining);
+            double done = frameDt - remaining;
+            double limit = (stepNumber + 1) * substepTime;
+            if (done + step > limit) step = std::fmin(substepTime, remaining);
+            if (stepNumber == maxSteps - 1) step = remaining;
+            remaining -= step;
+            if (!skipped) stepFluid(step);
+            stepNumber++;
+        } while (remaining > eps);
+        lastSubsteps = stepNumber;
+        lastIters = pIters;
+        lastErr = pErr;
+        frame++;
+    }
+
+    // FluidSimulation::_stepFluid (fluidsimulation.cpp:10078-10121), plain configuration
+    void stepFluid(double dt) {
+        bool fluid = !pos.empty();
+        if (fluid) updateNearSolid();
+        // liquid level set (5586-5617)
+        phi.assign(nN(), 0.0f);
+        std::vector<float> P(3 * pos.size()), Vv(3 * pos.size());
+        for (size_t p = 0; p < pos.size(); p++) {
+            P[3 * p] = pos[p].x; P[3 * p + 1] = pos[p].y; P[3 * p + 2] = pos[p].z;
+            Vv[3 * p] = vel[p].x; Vv[3 * p + 1] = vel[p].y; Vv[3 * p + 2] = vel[p].z;
+        }
+        int np = (int)pos.size();
+        if (fluid)
+            oracle_liquid_sdf(ni, nj, nk, dx, np, P.data(), liquidRadius(), phi.data());
+        else
+            std::fill(phi.begin(), phi.end(), (float)(3.0 * dx));
+        oracle_liquid_sdf_postprocess(ni, nj, nk, dx, solid.data(), phi.data());
+        // advect velocity field (5623-5661)
+        u.assign(nU(), 0.0f); v.assign(nV(), 0.0f); w.assign(nW(), 0.0f);
+        vu.assign(nU(), 0); vv.assign(nV(), 0); vw.assign(nW(), 0);
+        int layers = (int)std::ceil(std::sqrt(3) * cfl) + 3;  // _extrapolateFluidVelocities 6282-6286
+        if (fluid) {
+            oracle_p2g(ni, nj, nk, dx, np, P.data(), Vv.data(), liquidRadius(), 0, u.data(), vu.data());
+            oracle_p2g(ni, nj, nk, dx, np, P.data(), Vv.data(), liquidRadius(), 1, v.data(), vv.data());
+            oracle_p2g(ni, nj, nk, dx, np, P.data(), Vv.data(), liquidRadius(), 2, w.data(), vw.data());
+            extrapolateAll(layers);
+        }
+        if (!fluid) return;
+        // save (5671-5685)
+        su = u; sv = v; sw = w;
+        // body forces (5837-5876): every face, no inflow constraints on this path
+        V3 bf = bodyForce();
+        float beps = 1e-6;
+        if (std::fabs(bf.x) > beps) for (float &x : u) x += (float)(bf.x * dt);
+        if (std::fabs(bf.y) > beps) for (float &x : v) x += (float)(bf.y * dt);
+        if (std::fabs(bf.z) > beps) for (float &x : w) x += (float)(bf.z * dt);
+        // pressure (6152-6276)
+        pres.assign(nN(), 0.0f);
+        int status[2];
+        double err = 0;
+        oracle_pressure_solve(ni, nj, nk, dx, dt, u.data(), v.data(), w.data(), wu.data(),
+                              wv.data(), ww.data(), phi.data(), ptol, maxPIters, chunks,
+                              pres.data(), status, &err);
+        bool success = status[1] != 0;
+        if (success)
+            oracle_apply_pressure(ni, nj, nk, dx, dt, pres.data(), phi.data(), wu.data(),
+                                  wv.data(), ww.data(), u.data(), v.data(), w.data(), vu.data(),
+                                  vv.data(), vw.data());
+        if (stepNumber == 0) {
+            pSuccess = success;
+            pIters = status[0];
+            pErr = (float)err;
+        } else if (pSuccess && (!success || status[0] > pIters)) {
+            pSuccess = success;
+            pIters = status[0];
+            pErr = (float)err;
+        }
+        extrapolateAll(layers);
+        // constrain saved then current (6444-6459)
+        constrain(su, sv, sw);
+        constrain(u, v, w);
+        // G2P (6771-6784)
+        Mac cur{ni, nj, nk, dx, u.data(), v.data(), w.data()};
+        Mac sav{ni, nj, nk, dx, su.data(), sv.data(), sw.data()};
+        float r1 = (float)picflip, r2 = (float)(1 - picflip);
+        for (size_t p = 0; p < pos.size(); p++) {
+            V3 vPIC = sampleMac(cur, pos[p]);
+            V3 vFLIP = vel[p] + vPIC - sampleMac(sav, pos[p]);
+            vel[p] = r1 * vPIC + r2 * vFLIP;
+        }
+        advance(dt);
+    }
+
+    void extrapolateAll(int layers) {
+        oracle_extrapolate(ni + 1, nj, nk, u.data(), vu.data(), layers);
+        oracle_extrapolate(ni, nj + 1, nk, v.data(), vv.data(), layers);
+        oracle_extrapolate(ni, nj, nk + 1, w.data(), vw.data(), layers);
+    }
+
+    // _constrainVelocityFieldThread (fluidsimulation.cpp:6388-6442): domain friction 0.0
+    // (fluidsimulation.h:2227), solid face velocity 0 for static solids.
+    void constrain(std::vector<float> &a, std::vector<float> &b, std::vector<float> &c) {
+        float f = 0.0f, sface = 0.0f;
+        auto one = [&](std::vector<float> &x, const std::vector<float> &wt) {
+            for (size_t q = 0; q < x.size(); q++) {
+                if (wt[q] == 0)
+                    x[q] = sface;
+                else if (wt[q] < 1.0f)
+                    x[q] = f * sface + (1.0f - f) * x[q];
+            }
+        };
+        one(a, wu);
+        one(b, wv);
+        one(c, ww);
+    }
+
+    // _RK3 (fluidsimulation.cpp:7616-7623)
+    V3 rk3(const Mac &M, V3 p0, double dt) {
+        V3 k1 = sampleMac(M, p0);
+        V3 k2 = sampleMac(M, p0 + (float)(0.5 * dt) * k1);
+        V3 k3 = sampleMac(M, p0 + (float)(0.75 * dt) * k2);
+        return p0 + (float)(dt / 9.0f) * (2.0f * k1 + 3.0f * k2 + 4.0f * k3);
+    }
+
+    float solidAt(V3 p) const { return nodalTrilinear(solid.data(), ni + 1, nj + 1, nk + 1, dx, p); }
+    bool nearSolidAt(V3 p) const {
+        GI g = posToIndex(p, nsCell);
+        return nearSolid[flat(g.i, g.j, g.k, nsi, nsj)] != 0;
+    }
+
+    // _resolveCollision (fluidsimulation.cpp:7646-7721)
+    V3 resolveCollision(V3 oldp, V3 newp, const Box &boundary) {
+        GI gg = posToIndex(newp, dx);
+        if (!inRange(gg.i, gg.j, gg.k, ni, nj, nk)) newp = boundary.nearestInside(newp);
+        if (!nearSolidAt(oldp) && !nearSolidAt(newp)) return newp;
+        float eps = 1e-6;
+        float stepDistance = 0.1f * (float)dx;
+        float travel = length(newp - oldp);
+        if (travel < eps) return newp;
+        int numSteps = (int)std::ceil(travel / stepDistance);
+        V3 stepdir = normalize(newp - oldp);
+        V3 last = oldp, current{0, 0, 0};
+        bool found = false;
+        float collisionPhi = 0.0f;
+        for (int s = 0; s < numSteps; s++) {
+            if (s == numSteps - 1)
+                current = newp;
+            else
+                current = oldp + (float)(s + 1) * stepDistance * stepdir;
+            float ph = solidAt(current);
+            if (ph < 0.0f || !boundary.inside(current)) {
+                collisionPhi = ph;
+                found = true;
+                break;
+            }
+            last = current;
+        }
+        if (!found) return newp;
+        V3 resolved;
+        float maxResolved = cfl * dx;
+        V3 grad = nodalGradient(solid.data(), ni + 1, nj + 1, nk + 1, dx, current);
+        if (length(grad) > eps) {
+            grad = normalize(grad);
+            resolved = current - (float)(collisionPhi - 0.2f * dx) * grad;
+            float rphi = solidAt(resolved);
+            float rdist = length(resolved - current);
+            if (rphi < 0 || rdist > maxResolved) resolved = last;
+        } else {
+            resolved = last;
+        }
+        if (!boundary.inside(resolved)) {
+            V3 orig = resolved;
+            resolved = boundary.nearestInside(resolved);
+            float rphi = solidAt(resolved);
+            float rdist = length(resolved - orig);
+            if (rphi < 0.0f || rdist > maxResolved) resolved = last;
+        }
+        return resolved;
+    }
+
+    // _getMarkerParticleSpeedLimit (fluidsimulation.cpp:7723-7771)
+    float speedLimit(double dt) {
+        double maxSpeed = 0.0;
+        double stepv = cfl * dx / dt;
+        std::vector<int> counts(maxSteps, 0);
+        for (const V3 &v0 : vel) {
+            double speed = (double)length(v0);
+            int idx = std::fmin(std::floor(speed / stepv), maxSteps - 1);
+            counts[idx]++;
+            maxSpeed = std::max(speed, maxSpeed);
+        }
+        int maxRemoval = std::fmin((int)((double)vel.size() * 0.0005), 35);
+        double maxspeed = maxSteps * stepv;
+        int cur = 0;
+        for (int i = (int)counts.size() - 1; i > 0; i--) {
+            if (cur + counts[i] > maxRemoval) break;
+            cur += counts[i];
+            maxspeed = std::max(i + 4, maxSteps) * stepv;
+        }
+        double lower = 0.90 * maxSpeed, thr = 0.99999 * maxSpeed;
+        int nLower = 0, nOut = 0;
+        for (const V3 &v0 : vel) {
+            double speed = (double)length(v0);
+            if (speed >= lower && speed < thr) nLower++;
+            if (speed >= thr) nOut++;
+        }
+        if (nOut <= 6 && nLower <= 6) maxspeed = std::min(thr, maxspeed);
+        return (float)maxspeed;
+    }
+
+    // _advanceMarkerParticles + _removeMarkerParticles (fluidsimulation.cpp:7626-7900)
+    void advance(double dt) {
+        Mac M{ni, nj, nk, dx, u.data(), v.data(), w.data()};
+        Box boundary = boundaryBox(ni, nj, nk, dx);
+        boundary.expand(-0.2f * dx);
+        std::vector<V3> out(pos.size());
+        for (size_t p = 0; p < pos.size(); p++) out[p] = rk3(M, pos[p], dt);
+        for (size_t p = 0; p < pos.size(); p++) out[p] = resolveCollision(pos[p], out[p], boundary);
+        pos = out;
+        // removal uses the frame delta time (7896)
+        float maxspeed = speedLimit(frameDt);
+        double maxspeedsq = maxspeed * maxspeed;
+        std::vector<int> count(nN(), 0);
+        std::vector<char> removed(pos.size(), 0);
+        for (size_t p = 0; p < pos.size(); p++) removed[p] = solidAt(pos[p]) < 0.0f;
+        for (size_t p = 0; p < pos.size(); p++) {
+            if (removed[p]) continue;
+            GI g = posToIndex(pos[p], dx);
+            int &c = count[flat(g.i, g.j, g.k, ni, nj)];
+            if (c >= 250) {
+                removed[p] = 1;
+                continue;
+            }
+            c++;
+            if (extremeRemoval && dot(vel[p], vel[p]) > maxspeedsq) removed[p] = 1;
+        }
+        size_t q = 0;
+        for (size_t p = 0; p < pos.size(); p++)
+            if (!removed[p]) {
+                pos[q] = pos[p];
+                vel[q] = vel[p];
+                q++;
+            }
+        pos.resize(q);
+        vel.resize(q);
+    }
+};
+
+OracleSim *oracle_sim_new(int ni, int nj, int nk, double dx) {
+    OracleSim *s = new OracleSim();
+    s->ni = ni;
+    s->nj = nj;
+    s->nk = nk;
+    s->dx = dx;
+    return s;
+}
+void oracle_sim_destroy(OracleSim *s) { delete s; }
+void oracle_sim_add_body_force(OracleSim *s, double fx, double fy, double fz) {
+    s->bodyForces.push_back(mk(fx, fy, fz));
+}
+void oracle_sim_set_picflip_ratio(OracleSim *s, double r) { s->picflip = r; }
+void oracle_sim_set_cfl(OracleSim *s, int cfl) { s->cfl = cfl; }
+void oracle_sim_set_pressure_tolerance(OracleSim *s, double tol) { s->ptol = tol; }
+void oracle_sim_set_max_pressure_iterations(OracleSim *s, int n) { s->maxPIters = n; }
+void oracle_sim_set_thread_chunks(OracleSim *s, int n) { s->chunks = n < 1 ? 1 : n; }
+void oracle_sim_set_solid_phi(OracleSim *s, const float *phi) {
+    s->solid.assign(phi, phi + s->nNodes());
+    s->solidGiven = true;
+}
+void oracle_sim_load_particles(OracleSim *s, int np, const float *pos, const float *vel) {
+    for (int p = 0; p < np; p++) {
+        s->loadPos.push_back(V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]});
+        s->loadVel.push_back(V3{vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]});
+    }
+}
+void oracle_sim_initialize(OracleSim *s) { s->initialize(); }
+void oracle_sim_update(OracleSim *s, double dt) { s->update(dt); }
+void oracle_sim_step(OracleSim *s, double dt) {
+    s->frameDt = dt;
+    s->stepFluid(dt);
+    s->stepNumber++;
+}
+int oracle_sim_num_particles(OracleSim *s) { return (int)s->pos.size(); }
+void oracle_sim_get_particles(OracleSim *s, float *pos, float *vel) {
+    for (size_t p = 0; p < s->pos.size(); p++) {
+        if (pos) {
+            pos[3 * p] = s->pos[p].x; pos[3 * p + 1] = s->pos[p].y; pos[3 * p + 2] = s->pos[p].z;
+        }
+        if (vel) {
+            vel[3 * p] = s->vel[p].x; vel[3 * p + 1] = s->vel[p].y; vel[3 * p + 2] = s->vel[p].z;
+        }
+    }
+}
+int oracle_sim_last_substeps(OracleSim *s) { return s->lastSubsteps; }
+int oracle_sim_last_pressure_iterations(OracleSim *s) { return s->lastIters; }
+double oracle_sim_last_pressure_error(OracleSim *s) { return s->lastErr; }
+void oracle_sim_get_grids(OracleSim *s, float *phi, float *u, float *v, float *w, float *p) {
+    if (phi && !s->phi.empty()) std::copy(s->phi.begin(), s->phi.end(), phi);
+    if (u && !s->u.empty()) std::copy(s->u.begin(), s->u.end(), u);
+    if (v && !s->v.empty()) std::copy(s->v.begin(), s->v.end(), v);
+    if (w && !s->w.empty()) std::copy(s->w.begin(), s->w.end(), w);
+    if (p && !s->pres.empty()) std::copy(s->pres.begin(), s->pres.end(), p);
+}

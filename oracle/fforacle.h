@@ -1,0 +1,122 @@
+/*
+ * fforacle.h -- C ABI of the CPU oracle for the FLIP substep path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a from-scratch, single-threaded
+ * restatement of the reference engine's per-substep algorithms
+ * (rlguy/Blender-FLIP-Fluids 1.8.5, src/engine/).  It exists so that tests,
+ * the smoke check and the benchmark's CPU-baseline leg can check the HIP
+ * engine; the shipped library never links, loads or calls it.
+ *
+ * Grids are i-fastest (flat = i + ni*(j + nj*k)), matching Array3d
+ * (array3d.h:774-777).  Face arrays: u is (ni+1)*nj*nk, v is ni*(nj+1)*nk,
+ * w is ni*nj*(nk+1).  Nodal arrays are (ni+1)*(nj+1)*(nk+1).  Particles are
+ * packed float xyz triples, like vmath::vec3 (vmath.h:37-59).
+ */
+#ifndef FFORACLE_H
+#define FFORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- stage functions (each one reference function; see fforacle.cpp) ---- */
+
+/* Domain-boundary solid SDF at the nodes: the negated mesh SDF of the domain
+ * box inset by 3*dx+1e-4 (fluidsimulation.cpp:5175-5186, 5277-5287). */
+void oracle_solid_box_phi(int ni, int nj, int nk, double dx, float *phi_nodes);
+
+/* Counting sort of particles by flat cell id of floor(p/dx).  keys[P],
+ * counts[N], offsets[N+1] (exclusive scan), perm[P] (stable). */
+void oracle_cell_sort(int ni, int nj, int nk, double dx, int np, const float *pos,
+                      int *keys, int *counts, int *offsets, int *perm);
+
+/* Liquid SDF from particles (particlelevelset.cpp:335-666). */
+void oracle_liquid_sdf(int ni, int nj, int nk, double dx, int np, const float *pos,
+                       double radius, float *phi);
+
+/* Post-process of the liquid SDF against the solid SDF (particlelevelset.cpp:170-194). */
+void oracle_liquid_sdf_postprocess(int ni, int nj, int nk, double dx,
+                                   const float *solid_phi_nodes, float *phi);
+
+/* P2G of one velocity component (dir 0/1/2 = U/V/W) (velocityadvector.cpp:38-536).
+ * field and valid have that component's face-array size and are overwritten. */
+void oracle_p2g(int ni, int nj, int nk, double dx, int np, const float *pos,
+                const float *vel, double radius, int dir, float *field,
+                unsigned char *valid);
+
+/* Layered extrapolation of one face array (gridutils.h:43-160). */
+void oracle_extrapolate(int wi, int wj, int wk, float *field,
+                        const unsigned char *valid, int layers);
+
+/* Face weights from the nodal solid SDF (fluidsimulation.cpp:6066-6150). */
+void oracle_face_weights(int ni, int nj, int nk, const float *solid_phi_nodes,
+                         float *wu, float *wv, float *ww);
+
+/* Pressure system: cells (k,j,i order), rhs, matrix, MIC(0), PCG
+ * (pressuresolver.cpp:52-916, pcgsolver.h).  Returns the number of unknowns.
+ * pressure[N] receives the float pressure grid.  status[0] = iterations,
+ * status[1] = success flag (after the acceptable-tolerance rule),
+ * err[0] = |r|_inf at exit.  nthreads mimics the reference's BLAS chunking
+ * for the FP64 dot products (blaswrapper.h:27-57). */
+int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt,
+                          const float *u, const float *v, const float *w,
+                          const float *wu, const float *wv, const float *ww,
+                          const float *liquid_phi, double tol, int max_iter,
+                          int nthreads, float *pressure, int *status, double *err);
+
+/* Pressure system pieces for kernel-level checks (same reference lines).
+ * rows[M] = flat cell ids; rhs[M]; A: diag[M], ox[M], oy[M], oz[M]
+ * (coefficient to the +x/+y/+z neighbour row, 0 when absent), nbr[6M]
+ * (row ids of -x,+x,-y,+y,-z,+z or -1); invdiag[M]. */
+int oracle_pressure_system(int ni, int nj, int nk, double dx, double dt,
+                           const float *u, const float *v, const float *w,
+                           const float *wu, const float *wv, const float *ww,
+                           const float *liquid_phi, int *rows, double *rhs,
+                           double *diag, double *ox, double *oy, double *oz,
+                           int *nbr, double *invdiag);
+
+/* One application of the MIC(0) preconditioner z = (L L^T)^-1 r (pcgsolver.h:211-241). */
+void oracle_apply_mic0(int m, const double *diag, const double *ox, const double *oy,
+                       const double *oz, const int *nbr, const double *invdiag,
+                       const double *r, double *z);
+
+/* Apply the pressure gradient (pressuresolver.cpp:94-112, 1021-1205). */
+void oracle_apply_pressure(int ni, int nj, int nk, double dx, double dt,
+                           const float *pressure, const float *liquid_phi,
+                           const float *wu, const float *wv, const float *ww,
+                           float *u, float *v, float *w,
+                           unsigned char *vu, unsigned char *vv, unsigned char *vw);
+
+/* Velocity at a point, FP64-internal trilinear (macvelocityfield.cpp:519-645). */
+void oracle_sample_velocity(int ni, int nj, int nk, double dx, const float *u,
+                            const float *v, const float *w, int np,
+                            const float *pos, float *out);
+
+/* ---- whole-engine oracle with the FluidSimulation stepping semantics ---- */
+typedef struct OracleSim OracleSim;
+OracleSim *oracle_sim_new(int ni, int nj, int nk, double dx);
+void oracle_sim_destroy(OracleSim *s);
+void oracle_sim_add_body_force(OracleSim *s, double fx, double fy, double fz);
+void oracle_sim_set_picflip_ratio(OracleSim *s, double r);
+void oracle_sim_set_cfl(OracleSim *s, int cfl);
+void oracle_sim_set_pressure_tolerance(OracleSim *s, double tol);
+void oracle_sim_set_max_pressure_iterations(OracleSim *s, int n);
+void oracle_sim_set_thread_chunks(OracleSim *s, int n);
+void oracle_sim_set_solid_phi(OracleSim *s, const float *phi_nodes);
+void oracle_sim_load_particles(OracleSim *s, int np, const float *pos, const float *vel);
+void oracle_sim_initialize(OracleSim *s);
+void oracle_sim_update(OracleSim *s, double frame_dt);
+void oracle_sim_step(OracleSim *s, double dt);
+int oracle_sim_num_particles(OracleSim *s);
+void oracle_sim_get_particles(OracleSim *s, float *pos, float *vel);
+int oracle_sim_last_substeps(OracleSim *s);
+int oracle_sim_last_pressure_iterations(OracleSim *s);
+double oracle_sim_last_pressure_error(OracleSim *s);
+/* copies of the last substep's grids (sizes as above); pass NULL to skip */
+void oracle_sim_get_grids(OracleSim *s, float *liquid_phi, float *u, float *v,
+                          float *w, float *pressure);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,257 @@
+/*
+ * ref_probe.cpp -- drives the reference engine's own classes, stage by stage.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the headers
+ * of the reference sources under $(REF)/src/engine and linked with the
+ * reference engine built from them (oracle/_ref/libffengine_ref.so).  Tests
+ * call these entry points to pin the oracle restatement to the reference bit
+ * for bit.  Layouts and argument meanings match fforacle.h.
+ *
+ * Private members of FluidSimulation and ParticleLevelSet are reached by
+ * compiling their headers with class privacy lifted; the probe only reads
+ * state or calls the private step functions the engine itself calls.
+ */
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <fstream>
+#include <functional>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <queue>
+#include <random>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+#include <utility>
+#include <vector>
+
+// standard headers first: privacy is lifted only for the engine's own headers
+#define private public
+#define protected public
+#include "fluidsimulation.h"
+#include "gridutils.h"
+#include "macvelocityfield.h"
+#include "meshlevelset.h"
+#include "particlelevelset.h"
+#include "particlesystem.h"
+#include "pressuresolver.h"
+#include "velocityadvector.h"
+#undef private
+#undef protected
+
+namespace {
+void fillParticles(ParticleSystem &ps, int np, const float *pos, const float *vel) {
+    ps.addAttributeVector3("POSITION");
+    ps.addAttributeVector3("VELOCITY");
+    std::vector<vmath::vec3> *P, *V;
+    ps.getAttributeValues("POSITION", P);
+    ps.getAttributeValues("VELOCITY", V);
+    for (int p = 0; p < np; p++) {
+        P->push_back(vmath::vec3(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]));
+        V->push_back(vel ? vmath::vec3(vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]) : vmath::vec3());
+    }
+    ps.update();
+}
+template <class T>
+void copyOut(Array3d<T> &a, T *out) {
+    std::memcpy(out, a.getRawArray(), sizeof(T) * (size_t)a.width * a.height * a.depth);
+}
+template <class T>
+void copyIn(Array3d<T> &a, const T *in) {
+    std::memcpy(a.getRawArray(), in, sizeof(T) * (size_t)a.width * a.height * a.depth);
+}
+void boolOut(Array3d<bool> &a, unsigned char *out) {
+    size_t n = (size_t)a.width * a.height * a.depth;
+    bool *b = a.getRawArray();
+    for (size_t i = 0; i < n; i++) out[i] = b[i] ? 1 : 0;
+}
+}  // namespace
+
+extern "C" {
+
+void refprobe_set_threads(int n) { ThreadUtils::setMaxThreadCount(n); }
+
+/* FluidSimulation::_addStaticObjectsToSDF on a fresh level set (fluidsimulation.cpp:5277-5339). */
+void refprobe_solid_box_phi(int ni, int nj, int nk, double dx, float *out) {
+    FluidSimulation sim(ni, nj, nk, dx);
+    sim.initialize();
+    MeshLevelSet sdf(ni, nj, nk, dx);
+    sim._addStaticObjectsToSDF(0.0, sdf);
+    copyOut(*sdf.getPhiArray3d(), out);
+}
+
+void refprobe_liquid_sdf(int ni, int nj, int nk, double dx, int np, const float *pos,
+                         double radius, float *out) {
+    ParticleSystem ps;
+    fillParticles(ps, np, pos, nullptr);
+    ParticleLevelSet pls(ni, nj, nk, dx);
+    pls.calculateSignedDistanceField(ps, radius);
+    copyOut(pls._phi, out);
+}
+
+void refprobe_liquid_sdf_postprocess(int ni, int nj, int nk, double dx, const float *solid,
+                                     float *phi) {
+    MeshLevelSet m(ni, nj, nk, dx);
+    copyIn(*m.getPhiArray3d(), solid);
+    ParticleLevelSet pls(ni, nj, nk, dx);
+    copyIn(pls._phi, (const float *)phi);
+    pls.postProcessSignedDistanceField(m);
+    copyOut(pls._phi, phi);
+}
+
+void refprobe_p2g(int ni, int nj, int nk, double dx, int np, const float *pos, const float *vel,
+                  double radius, float *u, float *v, float *w, unsigned char *vu,
+                  unsigned char *vv, unsigned char *vw) {
+    ParticleSystem ps;
+    fillParticles(ps, np, pos, vel);
+    MACVelocityField mac(ni, nj, nk, dx);
+    ValidVelocityComponentGrid valid(ni, nj, nk);
+    VelocityAdvectorParameters params;
+    params.particles = &ps;
+    params.vfield = &mac;
+    params.validVelocities = &valid;
+    params.particleRadius = radius;
+    VelocityAdvector adv;
+    adv.advect(params);
+    copyOut(*mac.getArray3dU(), u);
+    copyOut(*mac.getArray3dV(), v);
+    copyOut(*mac.getArray3dW(), w);
+    boolOut(valid.validU, vu);
+    boolOut(valid.validV, vv);
+    boolOut(valid.validW, vw);
+}
+
+void refprobe_extrapolate(int wi, int wj, int wk, float *field, const unsigned char *valid,
+                          int layers) {
+    Array3d<float> g(wi, wj, wk, 0.0f);
+    copyIn(g, (const float *)field);
+    Array3d<bool> vg(wi, wj, wk, false);
+    size_t n = (size_t)wi * wj * wk;
+    for (size_t i = 0; i < n; i++) vg.getRawArray()[i] = valid[i] != 0;
+    GridUtils::extrapolateGrid(&g, &vg, layers);
+    copyOut(g, field);
+}
+
+/* FluidSimulation::_updateWeightGridThread (fluidsimulation.cpp:6110-6150). */
+void refprobe_face_weights(int ni, int nj, int nk, double dx, const float *solid, float *wu,
+                           float *wv, float *ww, float *wc) {
+    MeshLevelSet m(ni, nj, nk, dx);
+    copyIn(*m.getPhiArray3d(), solid);
+    auto cl = [](float x) { return std::max(0.0f, std::min(x, 1.0f)); };
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni + 1; i++)
+                wu[i + (ni + 1) * (j + nj * k)] = cl(1.0f - m.getFaceWeightU(i, j, k));
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj + 1; j++)
+            for (int i = 0; i < ni; i++)
+                wv[i + ni * (j + (nj + 1) * k)] = cl(1.0f - m.getFaceWeightV(i, j, k));
+    for (int k = 0; k < nk + 1; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                ww[i + ni * (j + nj * k)] = cl(1.0f - m.getFaceWeightW(i, j, k));
+    if (wc)
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++)
+                for (int i = 0; i < ni; i++)
+                    wc[i + ni * (j + nj * k)] = cl(1.0f - m.getCellWeight(i, j, k));
+}
+
+/* PressureSolver::solve + applySolutionToVelocityField, surface tension off,
+ * density 1, zero solid velocity (fluidsimulation.cpp:6206-6249).  u, v, w are
+ * updated in place when the solve succeeds; returns 1 on success. */
+int refprobe_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u, float *v,
+                            float *w, const float *wu, const float *wv, const float *ww,
+                            const float *wc, const float *phi, double tol, int maxIter,
+                            float *pressure, int *iterations, float *error,
+                            unsigned char *vu, unsigned char *vv, unsigned char *vw) {
+    MACVelocityField fluid(ni, nj, nk, dx), solid(ni, nj, nk, dx);
+    copyIn(*fluid.getArray3dU(), (const float *)u);
+    copyIn(*fluid.getArray3dV(), (const float *)v);
+    copyIn(*fluid.getArray3dW(), (const float *)w);
+    ValidVelocityComponentGrid valid(ni, nj, nk);
+    Array3d<float> liquid(ni, nj, nk, 0.0f);
+    copyIn(liquid, phi);
+    WeightGrid wg(ni, nj, nk);
+    copyIn(wg.U, wu);
+    copyIn(wg.V, wv);
+    copyIn(wg.W, ww);
+    copyIn(wg.center, wc);
+    Array3d<float> pgrid(ni, nj, nk, 0.0f), density(ni, nj, nk, 1.0f);
+    PressureSolverParameters params;
+    params.cellwidth = dx;
+    params.deltaTime = dt;
+    params.tolerance = tol;
+    params.acceptableTolerance = 1.0;
+    params.maxIterations = maxIter;
+    params.velocityFieldFluid = &fluid;
+    params.velocityFieldSolid = &solid;
+    params.validVelocities = &valid;
+    params.liquidSDF = &liquid;
+    params.weightGrid = &wg;
+    params.pressureGrid = &pgrid;
+    params.densityGrid = &density;
+    params.isSurfaceTensionEnabled = false;
+    PressureSolver ps;
+    bool ok = ps.solve(params);
+    if (ok) ps.applySolutionToVelocityField();
+    copyOut(pgrid, pressure);
+    *iterations = ps.getIterations();
+    *error = ps.getError();
+    copyOut(*fluid.getArray3dU(), u);
+    copyOut(*fluid.getArray3dV(), v);
+    copyOut(*fluid.getArray3dW(), w);
+    boolOut(valid.validU, vu);
+    boolOut(valid.validV, vv);
+    boolOut(valid.validW, vw);
+    return ok ? 1 : 0;
+}
+
+void refprobe_sample_velocity(int ni, int nj, int nk, double dx, const float *u, const float *v,
+                              const float *w, int np, const float *pos, float *out) {
+    MACVelocityField mac(ni, nj, nk, dx);
+    copyIn(*mac.getArray3dU(), u);
+    copyIn(*mac.getArray3dV(), v);
+    copyIn(*mac.getArray3dW(), w);
+    for (int p = 0; p < np; p++) {
+        vmath::vec3 r = mac.evaluateVelocityAtPositionLinear(vmath::vec3(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]));
+        out[3 * p] = r.x;
+        out[3 * p + 1] = r.y;
+        out[3 * p + 2] = r.z;
+    }
+}
+
+/* Whole engine through the reference's own object, with the phi grids of the
+ * last substep exposed for comparison. */
+void *refprobe_sim_new(int ni, int nj, int nk, double dx) { return new FluidSimulation(ni, nj, nk, dx); }
+void refprobe_sim_get_liquid_phi(void *s, float *out) {
+    copyOut(*((FluidSimulation *)s)->_liquidSDF.getPhiGrid(), out);
+}
+void refprobe_sim_get_solid_phi(void *s, float *out) {
+    copyOut(*((FluidSimulation *)s)->_solidSDF.getPhiArray3d(), out);
+}
+void refprobe_sim_get_mac(void *s, float *u, float *v, float *w) {
+    FluidSimulation *f = (FluidSimulation *)s;
+    copyOut(*f->_MACVelocity.getArray3dU(), u);
+    copyOut(*f->_MACVelocity.getArray3dV(), v);
+    copyOut(*f->_MACVelocity.getArray3dW(), w);
+}
+int refprobe_sim_last_substeps(void *s) { return ((FluidSimulation *)s)->_outputData.frameData.substeps; }
+int refprobe_sim_pressure_iterations(void *s) { return ((FluidSimulation *)s)->_pressureSolverIterations; }
+
+}  // extern "C"
