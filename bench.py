@@ -1,0 +1,232 @@
+"""Benchmark: FLIP substeps/s of the MI355X engine at 256^3, mixed-precision
+refined PCG vs FP64 PCG at relative residual 1e-8.
+
+Workload (per step): one FluidSimulation.update() of 1/240 s -- one CFL
+substep of the whole per-substep path (cell sort, liquid SDF, P2G,
+extrapolation, body force, pressure assembly + MIC(0)-PCG + projection,
+constraints, G2P + RK3 + collision, removal) -- on the dam break at n^3 with
+the static sphere obstacle: fluid block [0, 0.375) x [0, 0.625) x [0, 1),
+8 particles per cell (seeded jitter, zero initial velocity), sphere r = 0.15 at
+(0.6, 0.2, 0.5).  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line.  `value` is the substeps/s of the mixed-precision run;
+the FP64 run, their ratio, per-kernel GPU times (HIP events on the engine's
+stream) and the roofline of the dominant kernel are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "blender-flip-fluids_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["cell_sort", "liquid_sdf", "p2g", "extrapolate", "pressure_assembly", "mic0_factor", "spmv",
+           "mic0_apply", "unused", "g2p_advect", "removal", "apply_pressure"]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_sim(n, pos, solid, mode, tol):
+    from ffengine import FluidSimulation
+    sim = FluidSimulation(n, n, n, 1.0 / n)
+    sim.add_body_force(0.0, -9.81, 0.0)
+    sim.PICFLIP_ratio = 0.05
+    sim.CFL_condition_number = 5
+    sim.enable_surface_reconstruction = False
+    sim.pressure_solver_tolerance = tol
+    sim.set_pressure_solver_precision(mode, 1e-4)
+    sim.set_solid_phi(solid)
+    sim.load_marker_particle_data(len(pos), pos.tobytes(), np.zeros_like(pos).tobytes())
+    sim.initialize()
+    return sim
+
+
+def kernel_times(sim):
+    from ctypes import byref, c_double, c_int, c_longlong, c_void_p
+    from ffengine import pybindings as pb
+    from ffengine.ffengine import ffengine as lib
+    f = lib.FluidSimulation_get_kernel_timing
+    pb.init_lib_func(f, [c_void_p, c_int, c_void_p, c_void_p, c_void_p], None)
+    out = {}
+    for i, name in enumerate(KERNELS):
+        if name == "unused":
+            continue
+        ms, n, ok = c_double(), c_longlong(), c_int()
+        f(sim(), i, byref(ms), byref(n), byref(ok))
+        out[name] = {"ms": ms.value, "launches": n.value}
+    return out
+
+
+def enable_timing(sim, on):
+    from ctypes import c_int, c_void_p
+    from ffengine import pybindings as pb
+    from ffengine.ffengine import ffengine as lib
+    f = lib.FluidSimulation_enable_kernel_timing
+    pb.init_lib_func(f, [c_void_p, c_int, c_void_p], None)
+    pb.execute_lib_func(f, [sim(), 1 if on else 0])
+
+
+def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt):
+    import torch
+    sim = make_sim(n, pos, solid, mode, tol)
+    for s in range(warmup):
+        sim.update(frame_dt)
+        log("  warmup %s %d/%d" % (mode, s + 1, warmup))
+    enable_timing(sim, True)
+    torch.cuda.synchronize()
+    substeps, iters, unknowns = 0, [], []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        sim.update(frame_dt)
+        st = sim.get_frame_stats_data()
+        ps = sim.get_last_pressure_solve_stats()
+        substeps += st.substeps
+        iters.append(st.pressure_solver_iterations)
+        unknowns.append(ps["unknowns"])
+        log("  %s step %d/%d: substeps=%d iters=%d unknowns=%d" % (mode, s + 1, steps, st.substeps,
+                                                                   st.pressure_solver_iterations, ps["unknowns"]))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = kernel_times(sim)
+    nparticles = sim.get_num_marker_particles()
+    del sim
+    return {"elapsed_s": elapsed, "substeps": substeps, "iterations": iters, "unknowns": unknowns,
+            "kernels": kt, "particles": nparticles}
+
+
+def algorithmic_bytes(name, n, P, M_total, launches_per_sub, substeps, iters_total, mode):
+    """Compulsory bytes per timed region for one kernel group (units below)."""
+    N = n ** 3
+    F = 3 * n * n * (n + 1)
+    Nn = (n + 1) ** 3
+    if name == "p2g":
+        return substeps * (28 * P + 4 * N + 5 * F)          # 28 B/particle, 4 B/cell, 5 B/face
+    if name == "spmv":
+        per = 48 if mode == "fp64" else 24                    # 4 coefficients + x read + y write per row
+        return per * M_total
+    if name == "mic0_apply":
+        per = 96 if mode == "fp64" else 48                    # both sweeps: r, invdiag, 3 L, z per row
+        return per * M_total
+    if name == "g2p_advect":
+        return substeps * (48 * P + 8 * F + 4 * Nn)
+    if name == "extrapolate":
+        return substeps * 2 * 9 * F
+    if name == "liquid_sdf":
+        return substeps * (12 * P + 8 * N)
+    if name == "cell_sort":
+        return substeps * (20 * P + 4 * N)
+    return None
+
+
+def cpu_baseline(n_sample, threads, seconds_budget=30.0):
+    """The reference engine (built from its sources into oracle/_ref) on the
+    same scene at n_sample^3, timed on the host cores."""
+    import oracle_lib
+    from ffengine import scenes
+    if not oracle_lib.reference_available():
+        return None
+    dx = 1.0 / n_sample
+    pos = scenes.dam_break_particles(n_sample, seed=0)
+    sim = oracle_lib.ReferenceSim((n_sample,) * 3, dx, threads=threads)
+    sim.load(pos)
+    sim.initialize()
+    sim.update(1.0 / 240.0)  # warm-up substep (builds the solid SDF)
+    t0 = time.perf_counter()
+    count = 0
+    while True:
+        sim.update(1.0 / 240.0)
+        count += 1
+        if time.perf_counter() - t0 > seconds_budget * 0.5 or count >= 3:
+            break
+    el = time.perf_counter() - t0
+    return {"value": count / el, "unit": "substeps/s", "cores": threads, "kind": "reference",
+            "sample": "reference engine (oracle/_ref, built from its sources) on the %d^3 dam break "
+                      "without the sphere, %d timed substeps of 1/240 s after one warm-up substep" % (n_sample, count)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--cpu-sample-n", type=int, default=96)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the engine keeps a whole simulation on one device "
+              "(multi-GPU slab decomposition is design-only); got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    from ffengine import kernels, scenes
+
+    n = args.n
+    dx = 1.0 / n
+    frame_dt = 1.0 / 240.0
+    log("scene n=%d" % n)
+    pos = scenes.dam_break_particles(n, seed=0)
+    solid = scenes.union_phi(kernels.solid_box_phi((n, n, n), dx), scenes.sphere_phi_nodes(n))
+    results = {}
+    for mode in ("fp64", "mixed"):
+        log("mode %s" % mode)
+        results[mode] = run_mode(n, pos, solid, mode, args.tol, args.steps, args.warmup, frame_dt)
+    mixed, fp64 = results["mixed"], results["fp64"]
+    value = mixed["substeps"] / mixed["elapsed_s"]
+    fp64_rate = fp64["substeps"] / fp64["elapsed_s"]
+
+    # dominant kernel of the mixed run by measured time
+    P = mixed["particles"]
+    kt = mixed["kernels"]
+    dom = max((k for k in kt if algorithmic_bytes(k, n, P, 1, 1, 1, 1, "mixed") is not None),
+              key=lambda k: kt[k]["ms"])
+    M_total = 0
+    iters = mixed["iterations"]
+    for u, it in zip(mixed["unknowns"], iters):
+        M_total += u * it
+    bytes_ = algorithmic_bytes(dom, n, P, M_total, 1, mixed["substeps"], sum(iters), "mixed")
+    achieved = bytes_ / (kt[dom]["ms"] * 1e-3) / 1e9 if kt[dom]["ms"] > 0 else 0.0
+    launches = max(kt[dom]["launches"], 1)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "avg_launch_us": round(kt[dom]["ms"] * 1e3 / launches, 3),
+                "algorithmic_bytes_per_launch": int(bytes_ / launches)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline n=%d threads=%d" % (args.cpu_sample_n, args.cpu_threads))
+        cpu = cpu_baseline(args.cpu_sample_n, args.cpu_threads)
+
+    line = {
+        "metric": "FLIP substeps/s at 256^3, mixed vs FP64 PCG at rel res 1e-8; % HBM peak",
+        "value": round(value, 4), "unit": "substeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * mixed["elapsed_s"] / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "dam break %d^3 + static sphere obstacle, %d particles, frame 1/240 s, "
+                               "PCG rel res %g" % (n, len(pos), args.tol), "grid": n, "particles": len(pos)},
+        "fp64_substeps_per_s": round(fp64_rate, 4),
+        "mixed_over_fp64": round(value / fp64_rate, 4) if fp64_rate > 0 else None,
+        "pressure_iterations": {"fp64": fp64["iterations"], "mixed": mixed["iterations"]},
+        "kernel_ms": {m: {k: round(v["ms"], 3) for k, v in results[m]["kernels"].items()} for m in results},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

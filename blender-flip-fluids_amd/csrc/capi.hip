@@ -1,0 +1,524 @@
+// capi.hip -- the extern "C" boundary of libffengine.so.
+//
+// Every entry point converts exceptions into *err = 0 plus one process-wide
+// message buffer, like the reference's safe_execute_* wrappers
+// (c_bindings/cbindings.h:48-155, cbindings.cpp:35-43).
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace {
+const int SUCCESS = 1, FAIL = 0;
+char g_error[4096];
+
+void setError(const std::exception &ex) {
+    std::string msg = ex.what();
+    size_t n = std::min(msg.size(), sizeof(g_error) - 1);
+    std::memcpy(g_error, msg.data(), n);
+    g_error[n] = '\0';
+}
+
+template <class F>
+auto guarded(int *err, F &&f) -> decltype(f()) {
+    *err = SUCCESS;
+    try {
+        return f();
+    } catch (const std::exception &ex) {
+        setError(ex);
+        *err = FAIL;
+    }
+    return decltype(f())();
+}
+
+ff::FluidSimulation *S(FluidSimulation *o) {
+    if (!o) throw std::runtime_error("Error: null FluidSimulation handle.\n");
+    return reinterpret_cast<ff::FluidSimulation *>(o);
+}
+
+// A scratch engine state on device for the kernel-level entry points.
+struct Scratch {
+    hipStream_t s = nullptr;
+    Scratch() { ff::hipCheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
+    ~Scratch() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    explicit DBuf(size_t n) { ff::hipCheck(hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)), "hipMalloc"); }
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    operator T *() const { return p; }
+};
+template <class T>
+void up(T *d, const T *h, size_t n, hipStream_t s) {
+    if (n) ff::hipCheck(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, s), "H2D");
+}
+template <class T>
+void down(T *h, const T *d, size_t n, hipStream_t s) {
+    if (n) ff::hipCheck(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, s), "D2H");
+}
+void sync(hipStream_t s) { ff::hipCheck(hipStreamSynchronize(s), "sync"); }
+void checkGrid(int i, int j, int k, double dx) {
+    if (i <= 0 || j <= 0 || k <= 0 || !(dx > 0.0))
+        throw std::domain_error("Error: dimensions and cell size must be greater than 0.\n");
+}
+}  // namespace
+
+extern "C" {
+
+char *CBindings_get_error_message(void) { return g_error; }
+
+FluidSimulation *FluidSimulation_new_from_dimensions(int isize, int jsize, int ksize, double dx, int *err) {
+    return guarded(err, [&] {
+        return reinterpret_cast<FluidSimulation *>(new ff::FluidSimulation(isize, jsize, ksize, dx));
+    });
+}
+void FluidSimulation_destroy(FluidSimulation *obj) {
+    try {
+        delete reinterpret_cast<ff::FluidSimulation *>(obj);
+    } catch (...) {
+    }
+}
+void FluidSimulation_initialize(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->initialize(); });
+}
+int FluidSimulation_is_initialized(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return (int)S(o)->isInitialized(); });
+}
+void FluidSimulation_update(FluidSimulation *o, double dt, int *err) {
+    guarded(err, [&] { S(o)->update(dt); });
+}
+int FluidSimulation_get_current_frame(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getCurrentFrame(); });
+}
+void FluidSimulation_set_current_frame(FluidSimulation *o, int f, int *err) {
+    guarded(err, [&] { S(o)->setCurrentFrame(f); });
+}
+int FluidSimulation_is_current_frame_finished(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return (int)S(o)->isCurrentFrameFinished(); });
+}
+double FluidSimulation_get_cell_size(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getCellSize(); });
+}
+void FluidSimulation_get_grid_dimensions(FluidSimulation *o, int *i, int *j, int *k, int *err) {
+    guarded(err, [&] { S(o)->getGridDimensions(i, j, k); });
+}
+double FluidSimulation_get_density(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getDensity(); });
+}
+void FluidSimulation_set_density(FluidSimulation *o, double d, int *err) {
+    guarded(err, [&] { S(o)->setDensity(d); });
+}
+int FluidSimulation_get_pressure_solver_max_iterations(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getPressureSolverMaxIterations(); });
+}
+void FluidSimulation_set_pressure_solver_max_iterations(FluidSimulation *o, int n, int *err) {
+    guarded(err, [&] { S(o)->setPressureSolverMaxIterations(n); });
+}
+void FluidSimulation_enable_surface_reconstruction(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->enableSurfaceReconstruction(); });
+}
+void FluidSimulation_disable_surface_reconstruction(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->disableSurfaceReconstruction(); });
+}
+int FluidSimulation_is_surface_reconstruction_enabled(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return (int)S(o)->isSurfaceReconstructionEnabled(); });
+}
+int FluidSimulation_get_max_thread_count(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getMaxThreadCount(); });
+}
+void FluidSimulation_set_max_thread_count(FluidSimulation *o, int n, int *err) {
+    guarded(err, [&] { S(o)->setMaxThreadCount(n); });
+}
+void FluidSimulation_add_body_force(FluidSimulation *o, double fx, double fy, double fz, int *err) {
+    guarded(err, [&] { S(o)->addBodyForce(fx, fy, fz); });
+}
+FFVector3 FluidSimulation_get_constant_body_force(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getConstantBodyForce(); });
+}
+void FluidSimulation_reset_body_force(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->resetBodyForce(); });
+}
+int FluidSimulation_get_CFL_condition_number(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getCFLConditionNumber(); });
+}
+void FluidSimulation_set_CFL_condition_number(FluidSimulation *o, int n, int *err) {
+    guarded(err, [&] { S(o)->setCFLConditionNumber(n); });
+}
+int FluidSimulation_get_min_time_steps_per_frame(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getMinTimeStepsPerFrame(); });
+}
+void FluidSimulation_set_min_time_steps_per_frame(FluidSimulation *o, int n, int *err) {
+    guarded(err, [&] { S(o)->setMinTimeStepsPerFrame(n); });
+}
+int FluidSimulation_get_max_time_steps_per_frame(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getMaxTimeStepsPerFrame(); });
+}
+void FluidSimulation_set_max_time_steps_per_frame(FluidSimulation *o, int n, int *err) {
+    guarded(err, [&] { S(o)->setMaxTimeStepsPerFrame(n); });
+}
+void FluidSimulation_enable_extreme_velocity_removal(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->enableExtremeVelocityRemoval(); });
+}
+void FluidSimulation_disable_extreme_velocity_removal(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->disableExtremeVelocityRemoval(); });
+}
+int FluidSimulation_is_extreme_velocity_removal_enabled(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return (int)S(o)->isExtremeVelocityRemovalEnabled(); });
+}
+void FluidSimulation_set_velocity_transfer_method_FLIP(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->setVelocityTransferMethodFLIP(); });
+}
+void FluidSimulation_set_velocity_transfer_method_APIC(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->setVelocityTransferMethodAPIC(); });
+}
+int FluidSimulation_is_velocity_transfer_method_FLIP(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return (int)S(o)->isVelocityTransferMethodFLIP(); });
+}
+int FluidSimulation_is_velocity_transfer_method_APIC(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return (int)!S(o)->isVelocityTransferMethodFLIP(); });
+}
+double FluidSimulation_get_PICFLIP_ratio(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getPICFLIPRatio(); });
+}
+void FluidSimulation_set_PICFLIP_ratio(FluidSimulation *o, double r, int *err) {
+    guarded(err, [&] { S(o)->setPICFLIPRatio(r); });
+}
+
+void FluidSimulation_load_marker_particle_data(FluidSimulation *o, FluidSimulationMarkerParticleData d,
+                                               int *err) {
+    guarded(err, [&] { S(o)->loadMarkerParticleData(d.size, d.positions, d.velocities); });
+}
+int FluidSimulation_get_num_marker_particles(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getNumMarkerParticles(); });
+}
+unsigned int FluidSimulation_get_marker_particle_position_data_size(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return (unsigned int)(12u * (unsigned)S(o)->getNumMarkerParticles()); });
+}
+unsigned int FluidSimulation_get_marker_particle_velocity_data_size(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return (unsigned int)(12u * (unsigned)S(o)->getNumMarkerParticles()); });
+}
+void FluidSimulation_get_marker_particle_position_data(FluidSimulation *o, char *data, int *err) {
+    guarded(err, [&] { S(o)->getMarkerParticlePositionData(data); });
+}
+void FluidSimulation_get_marker_particle_velocity_data(FluidSimulation *o, char *data, int *err) {
+    guarded(err, [&] { S(o)->getMarkerParticleVelocityData(data); });
+}
+void FluidSimulation_get_marker_particle_position_data_range(FluidSimulation *o, int a, int b, char *data,
+                                                             int *err) {
+    guarded(err, [&] { S(o)->getMarkerParticlePositionDataRange(a, b, data); });
+}
+void FluidSimulation_get_marker_particle_velocity_data_range(FluidSimulation *o, int a, int b, char *data,
+                                                             int *err) {
+    guarded(err, [&] { S(o)->getMarkerParticleVelocityDataRange(a, b, data); });
+}
+FluidSimulationFrameStats FluidSimulation_get_frame_stats_data(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getFrameStatsData(); });
+}
+
+void FluidSimulation_set_pressure_solver_tolerance(FluidSimulation *o, double tol, int *err) {
+    guarded(err, [&] { S(o)->setPressureSolverTolerance(tol); });
+}
+double FluidSimulation_get_pressure_solver_tolerance(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getPressureSolverTolerance(); });
+}
+void FluidSimulation_set_pressure_solver_precision(FluidSimulation *o, int mode, double inner, int *err) {
+    guarded(err, [&] { S(o)->setPressureSolverPrecision(mode, inner); });
+}
+int FluidSimulation_get_pressure_solver_precision(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getPressureSolverPrecision(); });
+}
+void FluidSimulation_set_solid_phi(FluidSimulation *o, const float *phi, int *err) {
+    guarded(err, [&] { S(o)->setSolidPhi(phi); });
+}
+void FluidSimulation_get_liquid_sdf_data(FluidSimulation *o, float *d, int *err) {
+    guarded(err, [&] { S(o)->getLiquidSDF(d); });
+}
+void FluidSimulation_get_solid_sdf_data(FluidSimulation *o, float *d, int *err) {
+    guarded(err, [&] { S(o)->getSolidSDF(d); });
+}
+void FluidSimulation_get_velocity_field_data(FluidSimulation *o, float *u, float *v, float *w, int *err) {
+    guarded(err, [&] { S(o)->getVelocityField(u, v, w); });
+}
+void FluidSimulation_get_pressure_grid_data(FluidSimulation *o, float *d, int *err) {
+    guarded(err, [&] { S(o)->getPressureGrid(d); });
+}
+void FluidSimulation_get_last_pressure_solve_stats(FluidSimulation *o, int *it, double *e, int *ok, int *m,
+                                                   int *err) {
+    guarded(err, [&] { S(o)->getLastPressureSolveStats(it, e, ok, m); });
+}
+double FluidSimulation_get_last_substep_phase_ms(FluidSimulation *o, int phase, int *err) {
+    return guarded(err, [&] { return S(o)->getLastPhaseMilliseconds(phase); });
+}
+void FluidSimulation_enable_kernel_timing(FluidSimulation *o, int on, int *err) {
+    guarded(err, [&] { S(o)->enableKernelTiming(on != 0); });
+}
+void FluidSimulation_get_kernel_timing(FluidSimulation *o, int id, double *ms, long long *launches, int *err) {
+    guarded(err, [&] { S(o)->getKernelTiming(id, ms, launches); });
+}
+void FluidSimulation_step(FluidSimulation *o, double dt, int *err) {
+    guarded(err, [&] { S(o)->step(dt); });
+}
+
+// ---------------------------------------------------------------------------
+// kernel-level entry points on host arrays
+
+void FFAMD_cell_sort(int ni, int nj, int nk, double dx, int np, const float *pos, int *keys, int *counts,
+                     int *offsets, int *perm, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        ff::Grid g{ni, nj, nk, dx};
+        size_t N = (size_t)ni * nj * nk, P = (size_t)np;
+        Scratch sc;
+        std::vector<float> c(P);
+        DBuf<float> px(P), py(P), pz(P), vx(P), vy(P), vz(P);
+        for (int a = 0; a < 3; a++) {
+            for (size_t q = 0; q < P; q++) c[q] = pos[3 * q + a];
+            up(a == 0 ? px.p : (a == 1 ? py.p : pz.p), c.data(), P, sc.s);
+            sync(sc.s);
+        }
+        DBuf<int> dk(P), ds(P), dc(N + 1), doff(N + 1), dp(P), scan(ff::scanScratchInts(std::max(N + 1, P + 1)));
+        DBuf<float> s0(P), s1(P), s2(P), s3(P), s4(P), s5(P);
+        ff::Particles pp{px, py, pz, vx, vy, vz, np};
+        ff::CellSort cs{dk, ds, dc, doff, dp, s0, s1, s2, s3, s4, s5};
+        ff::cellSort(g, pp, cs, scan, sc.s);
+        down(keys, dk.p, P, sc.s);
+        down(counts, dc.p, N, sc.s);
+        down(offsets, doff.p, N + 1, sc.s);
+        down(perm, dp.p, P, sc.s);
+        sync(sc.s);
+    });
+}
+
+namespace {
+struct SortedParticles {
+    ff::CellSort cs;
+    std::vector<DBuf<float> *> f;
+    std::vector<DBuf<int> *> i;
+    SortedParticles(ff::Grid g, int np, const float *pos, const float *vel, hipStream_t s) {
+        size_t N = (size_t)g.ni * g.nj * g.nk, P = (size_t)np;
+        for (int q = 0; q < 12; q++) f.push_back(new DBuf<float>(P));
+        for (int q = 0; q < 6; q++) i.push_back(new DBuf<int>(q == 2 || q == 3 ? N + 1 : (q == 5 ? ff::scanScratchInts(std::max(N + 1, P + 1)) : P)));
+        std::vector<float> c(P);
+        for (int a = 0; a < 6; a++) {
+            const float *src = a < 3 ? pos : vel;
+            for (size_t q = 0; q < P; q++) c[q] = src ? src[3 * q + a % 3] : 0.0f;
+            up(f[a]->p, c.data(), P, s);
+            sync(s);
+        }
+        ff::Particles pp{*f[0], *f[1], *f[2], *f[3], *f[4], *f[5], np};
+        cs = ff::CellSort{*i[0], *i[1], *i[2], *i[3], *i[4], *f[6], *f[7], *f[8], *f[9], *f[10], *f[11]};
+        ff::cellSort(g, pp, cs, *i[5], s);
+    }
+    ~SortedParticles() {
+        for (auto *p : f) delete p;
+        for (auto *p : i) delete p;
+    }
+};
+}  // namespace
+
+void FFAMD_liquid_sdf(int ni, int nj, int nk, double dx, int np, const float *pos, double radius, float *phi,
+                      int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t N = (size_t)ni * nj * nk;
+        SortedParticles sp(g, np, pos, nullptr, sc.s);
+        DBuf<float> d(N);
+        ff::liquidSDF(g, sp.cs, np, radius, d, sc.s);
+        down(phi, d.p, N, sc.s);
+        sync(sc.s);
+    });
+}
+
+void FFAMD_p2g(int ni, int nj, int nk, double dx, int np, const float *pos, const float *vel, double radius,
+               float *u, float *v, float *w, unsigned char *vu, unsigned char *vv, unsigned char *vw, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        SortedParticles sp(g, np, pos, vel, sc.s);
+        DBuf<float> du(nu), dv(nv), dw(nw);
+        DBuf<uint8_t> bu(nu), bv(nv), bw(nw);
+        ff::p2g(g, sp.cs, radius, du, dv, dw, bu, bv, bw, sc.s);
+        down(u, du.p, nu, sc.s);
+        down(v, dv.p, nv, sc.s);
+        down(w, dw.p, nw, sc.s);
+        down(vu, bu.p, nu, sc.s);
+        down(vv, bv.p, nv, sc.s);
+        down(vw, bw.p, nw, sc.s);
+        sync(sc.s);
+    });
+}
+
+void FFAMD_extrapolate(int wi, int wj, int wk, float *field, const unsigned char *valid, int layers, int *err) {
+    guarded(err, [&] {
+        if (wi < 3 || wj < 3 || wk < 3 || layers < 0 || layers > 250)
+            throw std::domain_error("Error: face array dims must be >= 3 and layers in [0, 250].\n");
+        Scratch sc;
+        size_t n = (size_t)wi * wj * wk;
+        DBuf<float> a(n);
+        DBuf<uint8_t> vb(n), st(n);
+        up(a.p, field, n, sc.s);
+        up(vb.p, valid, n, sc.s);
+        ff::extrapolateInit(wi, wj, wk, vb, st, sc.s);
+        for (int l = 0; l < layers; l++) ff::extrapolateLayer(wi, wj, wk, a, st, l, sc.s);
+        down(field, a.p, n, sc.s);
+        sync(sc.s);
+    });
+}
+
+void FFAMD_solid_box_phi(int ni, int nj, int nk, double dx, float *phi, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        Scratch sc;
+        size_t nn = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
+        DBuf<float> d(nn);
+        ff::solidBoxPhi(ff::Grid{ni, nj, nk, dx}, d, sc.s);
+        down(phi, d.p, nn, sc.s);
+        sync(sc.s);
+    });
+}
+
+void FFAMD_face_weights(int ni, int nj, int nk, const float *phi, float *wu, float *wv, float *ww, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, 1.0);
+        Scratch sc;
+        size_t nn = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> d(nn), a(nu), b(nv), c(nw);
+        up(d.p, phi, nn, sc.s);
+        ff::faceWeights(ff::Grid{ni, nj, nk, 1.0}, d, a, b, c, sc.s);
+        down(wu, a.p, nu, sc.s);
+        down(wv, b.p, nv, sc.s);
+        down(ww, c.p, nw, sc.s);
+        sync(sc.s);
+    });
+}
+
+void FFAMD_sample_velocity(int ni, int nj, int nk, double dx, const float *u, const float *v, const float *w,
+                           int np, const float *pos, float *out, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        // sampling is G2P with PIC ratio 1, zero saved field and no motion:
+        // the new velocity is the sampled field
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        size_t P = (size_t)np;
+        DBuf<float> du(nu), dv(nv), dw(nw), zu(nu), zv(nv), zw(nw);
+        up(du.p, u, nu, sc.s);
+        up(dv.p, v, nv, sc.s);
+        up(dw.p, w, nw, sc.s);
+        ff::hipCheck(hipMemsetAsync(zu, 0, sizeof(float) * nu, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(zv, 0, sizeof(float) * nv, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(zw, 0, sizeof(float) * nw, sc.s), "memset");
+        std::vector<DBuf<float> *> pa;
+        std::vector<float> c(P);
+        for (int a = 0; a < 6; a++) {
+            pa.push_back(new DBuf<float>(P));
+            for (size_t q = 0; q < P; q++) c[q] = a < 3 ? pos[3 * q + a] : 0.0f;
+            up(pa.back()->p, c.data(), P, sc.s);
+            sync(sc.s);
+        }
+        size_t NN = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
+        DBuf<float> solid(NN);
+        // positive solid SDF everywhere and an empty near-solid grid: no collision
+        std::vector<float> one(NN, 1.0f);
+        up(solid.p, one.data(), NN, sc.s);
+        DBuf<uint8_t> near(1);
+        ff::hipCheck(hipMemsetAsync(near, 0, 1, sc.s), "memset");
+        ff::Particles pp{*pa[0], *pa[1], *pa[2], *pa[3], *pa[4], *pa[5], np};
+        ff::g2pAdvect(g, pp, ff::MacView{du, dv, dw}, ff::MacView{zu, zv, zw}, solid, near, 1, 1, 1, 1.0f, 0.0f,
+                      1e-30, 5.0, sc.s);
+        std::vector<float> x(P), y(P), z(P);
+        down(x.data(), pa[3]->p, P, sc.s);
+        down(y.data(), pa[4]->p, P, sc.s);
+        down(z.data(), pa[5]->p, P, sc.s);
+        sync(sc.s);
+        for (size_t q = 0; q < P; q++) {
+            out[3 * q] = x[q];
+            out[3 * q + 1] = y[q];
+            out[3 * q + 2] = z[q];
+        }
+        for (auto *p : pa) delete p;
+    });
+}
+
+int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u, float *v, float *w,
+                         const float *wu, const float *wv, const float *ww, const float *phi, double tol,
+                         int maxIter, int mode, float *pressure, int *iterations, double *error,
+                         unsigned char *vu, unsigned char *vv, unsigned char *vw, int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t N = (size_t)ni * nj * nk;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> du(nu), dv(nv), dw(nw), a(nu), b(nv), c(nw), dphi(N), dp(N);
+        DBuf<uint8_t> bu(nu), bv(nv), bw(nw);
+        DBuf<int> scan(ff::scanScratchInts(N + 1));
+        up(du.p, (const float *)u, nu, sc.s); up(dv.p, (const float *)v, nv, sc.s); up(dw.p, (const float *)w, nw, sc.s);
+        up(a.p, wu, nu, sc.s); up(b.p, wv, nv, sc.s); up(c.p, ww, nw, sc.s);
+        up(dphi.p, phi, N, sc.s);
+        ff::PressureSolver ps;
+        ps.allocate(g);
+        ff::PressureResult r = ps.solve(dt, du, dv, dw, a, b, c, dphi, tol, maxIter, mode, 1e-4, dp, scan, sc.s, nullptr);
+        ff::hipCheck(hipMemsetAsync(bu, 0, nu, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(bv, 0, nv, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(bw, 0, nw, sc.s), "memset");
+        if (r.success) ff::applyPressure(g, dt, dp, dphi, a, b, c, du, dv, dw, bu, bv, bw, sc.s);
+        down(pressure, dp.p, N, sc.s);
+        down(u, du.p, nu, sc.s); down(v, dv.p, nv, sc.s); down(w, dw.p, nw, sc.s);
+        down(vu, bu.p, nu, sc.s); down(vv, bv.p, nv, sc.s); down(vw, bw.p, nw, sc.s);
+        sync(sc.s);
+        *iterations = r.iterations;
+        *error = r.error;
+        return r.success ? 1 : 0;
+    });
+}
+
+int FFAMD_pressure_system(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
+                          const float *w, const float *wu, const float *wv, const float *ww, const float *phi,
+                          int *rows, double *rhs, double *diag, double *ox, double *oy, double *oz, int *nbr,
+                          double *invdiag, int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t N = (size_t)ni * nj * nk;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> du(nu), dv(nv), dw(nw), a(nu), b(nv), c(nw), dphi(N), dp(N);
+        DBuf<int> scan(ff::scanScratchInts(N + 1));
+        up(du.p, u, nu, sc.s); up(dv.p, v, nv, sc.s); up(dw.p, w, nw, sc.s);
+        up(a.p, wu, nu, sc.s); up(b.p, wv, nv, sc.s); up(c.p, ww, nw, sc.s);
+        up(dphi.p, phi, N, sc.s);
+        ff::PressureSolver ps;
+        ps.allocate(g);
+        // one PCG iteration builds the system and the factor; the tolerance is
+        // irrelevant for the system itself
+        ps.solve(dt, du, dv, dw, a, b, c, dphi, 1e-300, 1, 0, 1e-4, dp, scan, sc.s, nullptr);
+        const ff::PressureSys &S = ps.system();
+        size_t m = (size_t)S.m;
+        down(rows, S.rows, m, sc.s);
+        down(rhs, S.rhs, m, sc.s);
+        down(diag, S.diag, m, sc.s);
+        down(ox, S.ox, m, sc.s);
+        down(oy, S.oy, m, sc.s);
+        down(oz, S.oz, m, sc.s);
+        down(nbr, S.nbr, 6 * m, sc.s);
+        down(invdiag, S.invdiag, m, sc.s);
+        sync(sc.s);
+        return S.m;
+    });
+}
+
+}  // extern "C"

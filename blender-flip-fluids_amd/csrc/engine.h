@@ -1,0 +1,190 @@
+// engine.h -- the MI355X FLIP engine behind the FluidSimulation C ABI.
+//
+// FluidSimulation keeps the reference engine's stepping semantics
+// (rlguy/Blender-FLIP-Fluids 1.8.5, fluidsimulation.cpp:10078-10543): a frame
+// is split into CFL substeps on the host, and every substep runs on one HIP
+// stream of the device that was current at initialize().  Particle and grid
+// state stays resident in HBM between calls; particle I/O copies at the
+// C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ffengine_amd.h"
+#include "kernels.h"
+#include "pressure.h"
+#include "timers.h"
+
+namespace ff {
+
+// RAII device allocation
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t count);
+    void release();
+    operator T *() const { return p; }
+};
+
+void hipCheck(hipError_t e, const char *what);
+
+enum class PressurePrecision { FP64 = 0, MixedRefined = 1 };
+
+class FluidSimulation {
+public:
+    FluidSimulation(int isize, int jsize, int ksize, double dx);
+    ~FluidSimulation();
+
+    // lifecycle (fluidsimulation.cpp:84-94, 10410-10543)
+    void initialize();
+    bool isInitialized() const { return _initialized; }
+    void update(double dt);
+    int getCurrentFrame() const { return _currentFrame; }
+    void setCurrentFrame(int f) { _currentFrame = f; }
+    bool isCurrentFrameFinished() const { return _frameFinished; }
+
+    // grid
+    double getCellSize() const { return _g.dx; }
+    void getGridDimensions(int *i, int *j, int *k) const;
+
+    // settings used on the substep path, with the reference's validation
+    void addBodyForce(double fx, double fy, double fz);
+    void resetBodyForce() { _bodyForces.clear(); }
+    FFVector3 getConstantBodyForce() const;
+    void setPICFLIPRatio(double r);
+    double getPICFLIPRatio() const { return _picflip; }
+    void setCFLConditionNumber(int n);
+    int getCFLConditionNumber() const { return (int)_cfl; }
+    void setMinTimeStepsPerFrame(int n);
+    int getMinTimeStepsPerFrame() const { return _minSteps; }
+    void setMaxTimeStepsPerFrame(int n);
+    int getMaxTimeStepsPerFrame() const { return _maxSteps; }
+    void setPressureSolverMaxIterations(int n) { _maxPressureIterations = n; }
+    int getPressureSolverMaxIterations() const { return _maxPressureIterations; }
+    void setMaxThreadCount(int n);
+    int getMaxThreadCount() const { return _maxThreads; }
+    void setDensity(double d);
+    double getDensity() const { return _density; }
+    void enableExtremeVelocityRemoval() { _extremeRemoval = true; }
+    void disableExtremeVelocityRemoval() { _extremeRemoval = false; }
+    bool isExtremeVelocityRemovalEnabled() const { return _extremeRemoval; }
+    void enableSurfaceReconstruction() { _surfaceReconstruction = true; }
+    void disableSurfaceReconstruction() { _surfaceReconstruction = false; }
+    bool isSurfaceReconstructionEnabled() const { return _surfaceReconstruction; }
+    void setVelocityTransferMethodFLIP() {}
+    void setVelocityTransferMethodAPIC();
+    bool isVelocityTransferMethodFLIP() const { return true; }
+
+    // additive settings
+    void setPressureSolverTolerance(double tol);
+    double getPressureSolverTolerance() const { return _pressureTolerance; }
+    void setPressureSolverPrecision(int mode, double innerTol);
+    int getPressureSolverPrecision() const { return (int)_precision; }
+    void setSolidPhi(const float *phiNodes);
+
+    // particle I/O (fluidsimulation.cpp:3426-3839)
+    void loadMarkerParticleData(int n, const char *positions, const char *velocities);
+    int getNumMarkerParticles() const;
+    void getMarkerParticlePositionDataRange(int start, int end, char *out);
+    void getMarkerParticleVelocityDataRange(int start, int end, char *out);
+    void getMarkerParticlePositionData(char *out);
+    void getMarkerParticleVelocityData(char *out);
+
+    FFFrameStats getFrameStatsData() const { return _stats; }
+
+    // grid getters for tests and tools (host copies)
+    void getLiquidSDF(float *out);
+    void getSolidSDF(float *out);
+    void getVelocityField(float *u, float *v, float *w);
+    void getPressureGrid(float *out);
+    void getLastPressureSolveStats(int *iterations, double *error, int *success, int *unknowns);
+    double getLastPhaseMilliseconds(int phase);
+
+    // one substep of dt (for tests and the benchmark; no frame bookkeeping)
+    void step(double dt);
+
+    // HIP-event timing of kernel groups (see timers.h)
+    void enableKernelTiming(bool on) { _timers.enable(on); }
+    void getKernelTiming(int id, double *ms, long long *launches) const;
+
+private:
+    Grid _g;
+    hipStream_t _stream = nullptr;
+    bool _initialized = false;
+    int _currentFrame = 0;
+    int _stepNumber = 0;
+    bool _frameFinished = true;
+    bool _updatedOnce = false;
+    double _frameDt = 0, _remaining = 0;
+
+    double _cfl = 5.0;
+    double _picflip = 0.05;
+    int _minSteps = 1, _maxSteps = 6;
+    int _maxPressureIterations = 900;
+    double _pressureTolerance = 1e-9;
+    PressurePrecision _precision = PressurePrecision::FP64;
+    double _innerTolerance = 1e-4;
+    int _maxThreads = 1;
+    double _density = 20.0;
+    bool _extremeRemoval = true;
+    bool _surfaceReconstruction = true;
+    std::vector<FFVector3> _bodyForces;
+    bool _solidGiven = false;
+    std::vector<float> _solidHost;
+
+    std::vector<std::vector<float>> _loadPos, _loadVel;
+
+    // particles (double-buffered for compaction)
+    int _capacity = 0;
+    DevBuf<float> _pa[6], _pb[6];
+    Particles _P{};
+    Particles _Q{};
+
+    CellSort _cs{};
+    DevBuf<int> _keys, _slot, _perm, _counts, _offsets, _keep, _rank, _scan, _maxCount;
+    DevBuf<float> _sorted[6];
+
+    DevBuf<float> _phi, _solid, _wu, _wv, _ww, _pressure;
+    DevBuf<uint8_t> _near, _nearTmp;
+    int _ci = 0, _cj = 0, _ck = 0;
+    DevBuf<float> _u, _v, _w, _su, _sv, _sw;
+    DevBuf<uint8_t> _vu, _vv, _vw, _stu, _stv, _stw;
+
+    PressureSolver _ps;
+    DevBuf<double> _partial, _scal;
+    DevBuf<int> _bins, _outl;
+
+    // status of the last pressure solve / frame
+    bool _pSuccess = true;
+    int _pIters = 0;
+    float _pErr = 0.0f;
+    int _lastIters = 0, _lastUnknowns = 0, _lastSuccess = 1;
+    double _lastErr = 0.0;
+    FFFrameStats _stats{};
+    double _phaseMs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t _ev[9] = {};
+    KernelTimers _timers;
+
+    void allocate();
+    void uploadParticles();
+    double nextTimeStep(double dt);
+    void stepFluid(double dt);
+    void extrapolate(int layers);
+    bool pressureSolve(double dt);
+    void removeParticles();
+    double hostScalar(const double *d);
+    FFVector3 bodyForce() const;
+    void requireInitialized(const char *what) const;
+};
+
+}  // namespace ff

@@ -1,0 +1,585 @@
+// engine.hip -- host orchestration of the FLIP substep on one HIP stream.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace ff {
+
+void hipCheck(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        std::string msg = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
+        throw std::runtime_error(msg);
+    }
+}
+
+template <class T>
+void DevBuf<T>::alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipCheck(hipMalloc((void **)&p, sizeof(T) * count), "hipMalloc");
+    n = count;
+}
+template <class T>
+void DevBuf<T>::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+template struct DevBuf<float>;
+template struct DevBuf<double>;
+template struct DevBuf<int>;
+template struct DevBuf<uint8_t>;
+
+namespace {
+std::string toString(double v) {
+    std::ostringstream ss;
+    ss << v;
+    return ss.str();
+}
+}  // namespace
+
+FluidSimulation::FluidSimulation(int isize, int jsize, int ksize, double dx) {
+    if (isize <= 0 || jsize <= 0 || ksize <= 0 || dx <= 0.0)
+        throw std::domain_error("Error: dimensions and cell size must be greater than 0.\n");
+    _g = Grid{isize, jsize, ksize, dx};
+}
+
+FluidSimulation::~FluidSimulation() {
+    for (auto &e : _ev)
+        if (e) (void)hipEventDestroy(e);
+    if (_stream) (void)hipStreamDestroy(_stream);
+}
+
+void FluidSimulation::getGridDimensions(int *i, int *j, int *k) const {
+    *i = _g.ni;
+    *j = _g.nj;
+    *k = _g.nk;
+}
+
+// setters (fluidsimulation.cpp:227-238, 2427-2462, 2697-2781, 2863-2876)
+void FluidSimulation::addBodyForce(double fx, double fy, double fz) {
+    _bodyForces.push_back(FFVector3{(float)fx, (float)fy, (float)fz});
+}
+FFVector3 FluidSimulation::bodyForce() const {
+    FFVector3 b{0.0f, 0.0f, 0.0f};
+    for (const FFVector3 &f : _bodyForces) {
+        b.x += f.x;
+        b.y += f.y;
+        b.z += f.z;
+    }
+    return b;
+}
+FFVector3 FluidSimulation::getConstantBodyForce() const { return bodyForce(); }
+void FluidSimulation::setPICFLIPRatio(double r) {
+    if (r < 0.0 || r > 1.0)
+        throw std::domain_error("Error: PICFLIP ratio must be in range [0.0, 1.0].\nratio: " + toString(r) + "\n");
+    _picflip = r;
+}
+void FluidSimulation::setCFLConditionNumber(int n) {
+    if (n < 1)
+        throw std::domain_error("Error: CFL must be greater than or equal to 1.\nCFL: " + toString(n) + "\n");
+    _cfl = n;
+}
+void FluidSimulation::setMinTimeStepsPerFrame(int n) {
+    if (n < 1)
+        throw std::domain_error("Error: min step count must be greater than or equal to 1.\nStep count: " +
+                                toString(n) + "\n");
+    _minSteps = n;
+}
+void FluidSimulation::setMaxTimeStepsPerFrame(int n) {
+    if (n < 1)
+        throw std::domain_error("Error: max step count must be greater than or equal to 1.\nStep count: " +
+                                toString(n) + "\n");
+    _maxSteps = n;
+}
+void FluidSimulation::setMaxThreadCount(int n) {
+    if (n < 1)
+        throw std::domain_error("Error: thread count must be greater than or equal to 1.\nthread count: " +
+                                toString(n) + "\n");
+    _maxThreads = n;
+}
+void FluidSimulation::setDensity(double d) {
+    if (d <= 0.0)
+        throw std::domain_error("Error: density must be greater than 0.\ndensity: " + toString(d) + "\n");
+    _density = d;
+}
+void FluidSimulation::setVelocityTransferMethodAPIC() {
+    throw std::runtime_error("Error: the APIC velocity transfer is not available in this engine.\n");
+}
+void FluidSimulation::setPressureSolverTolerance(double tol) {
+    if (!(tol > 0.0))
+        throw std::domain_error("Error: pressure solver tolerance must be greater than 0.\ntolerance: " +
+                                toString(tol) + "\n");
+    _pressureTolerance = tol;
+}
+void FluidSimulation::setPressureSolverPrecision(int mode, double innerTol) {
+    if (mode != 0 && mode != 1)
+        throw std::domain_error("Error: pressure precision mode must be 0 (FP64) or 1 (mixed).\n");
+    if (mode == 1 && !(innerTol > 0.0 && innerTol < 1.0))
+        throw std::domain_error("Error: inner tolerance must be in (0, 1).\n");
+    _precision = (PressurePrecision)mode;
+    if (mode == 1) _innerTolerance = innerTol;
+}
+void FluidSimulation::setSolidPhi(const float *phi) {
+    if (_initialized)
+        throw std::runtime_error("Error: the solid SDF must be set before initialize.\n");
+    size_t nn = (size_t)(_g.ni + 1) * (_g.nj + 1) * (_g.nk + 1);
+    _solidHost.assign(phi, phi + nn);
+    _solidGiven = true;
+}
+
+void FluidSimulation::requireInitialized(const char *what) const {
+    if (!_initialized)
+        throw std::runtime_error(std::string("Error: FluidSimulation must be initialized before ") + what + ".\n");
+}
+
+// FluidSimulation::loadMarkerParticleData (fluidsimulation.cpp:3818-3839):
+// a synchronous copy into a load queue consumed by initialize()
+void FluidSimulation::loadMarkerParticleData(int n, const char *positions, const char *velocities) {
+    if (n <= 0) return;
+    const float *p = (const float *)positions, *v = (const float *)velocities;
+    _loadPos.emplace_back(p, p + 3 * (size_t)n);
+    _loadVel.emplace_back(v, v + 3 * (size_t)n);
+}
+
+void FluidSimulation::allocate() {
+    const size_t N = (size_t)_g.ni * _g.nj * _g.nk;
+    const size_t NN = (size_t)(_g.ni + 1) * (_g.nj + 1) * (_g.nk + 1);
+    const size_t nu = (size_t)(_g.ni + 1) * _g.nj * _g.nk, nv = (size_t)_g.ni * (_g.nj + 1) * _g.nk,
+                 nw = (size_t)_g.ni * _g.nj * (_g.nk + 1);
+    const size_t P = (size_t)std::max(_capacity, 1);
+    for (int q = 0; q < 6; q++) {
+        _pa[q].alloc(P);
+        _pb[q].alloc(P);
+        _sorted[q].alloc(P);
+    }
+    _keys.alloc(P);
+    _slot.alloc(P);
+    _perm.alloc(P);
+    _keep.alloc(P + 1);
+    _rank.alloc(P + 1);
+    _counts.alloc(N + 1);
+    _offsets.alloc(N + 1);
+    _scan.alloc(scanScratchInts(std::max(N + 1, P + 1)));
+    _maxCount.alloc(1);
+    _phi.alloc(N);
+    _pressure.alloc(N);
+    _solid.alloc(NN);
+    _wu.alloc(nu);
+    _wv.alloc(nv);
+    _ww.alloc(nw);
+    _u.alloc(nu); _v.alloc(nv); _w.alloc(nw);
+    _su.alloc(nu); _sv.alloc(nv); _sw.alloc(nw);
+    _vu.alloc(nu); _vv.alloc(nv); _vw.alloc(nw);
+    _stu.alloc(nu); _stv.alloc(nv); _stw.alloc(nw);
+    _ci = (int)std::ceil(_g.ni * _g.dx / (3 * _g.dx));
+    _cj = (int)std::ceil(_g.nj * _g.dx / (3 * _g.dx));
+    _ck = (int)std::ceil(_g.nk * _g.dx / (3 * _g.dx));
+    _near.alloc((size_t)_ci * _cj * _ck);
+    _nearTmp.alloc((size_t)_ci * _cj * _ck);
+    _partial.alloc(1024);
+    _scal.alloc(16);
+    _bins.alloc(std::max(_maxSteps, 1) + 16);
+    _outl.alloc(2);
+
+    _cs = CellSort{_keys, _slot, _counts, _offsets, _perm, _sorted[0], _sorted[1], _sorted[2],
+                   _sorted[3], _sorted[4], _sorted[5]};
+    _ps.allocate(_g);
+}
+
+// FluidSimulation::initialize / _initializeSimulation (fluidsimulation.cpp:84-90, 4362-4398)
+void FluidSimulation::initialize() {
+    if (_initialized) return;
+    int dev = 0;
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    hipCheck(hipStreamCreateWithFlags(&_stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto &e : _ev) hipCheck(hipEventCreate(&e), "hipEventCreate");
+
+    // _loadMarkerParticles (fluidsimulation.cpp:4903-5016): the load queue is a
+    // FragmentedVector of 5e6-byte fragments (E = 208333 particles); its
+    // operator[] picks the fragment as (int)(i * (1.0 / E)) in double
+    // (fragmentedvector.h:140-151, 294-301), which reads element (c-1)*E at
+    // some exact multiples i = c*E.  Particles outside [0, n*dx) are dropped.
+    std::vector<float> hp, hv;
+    const int E = (int)(5000000u / 24u);
+    const double inv = 1.0 / (double)E;
+    Box bounds{V3{0, 0, 0}, _g.ni * _g.dx, _g.nj * _g.dx, _g.nk * _g.dx};
+    for (size_t b = 0; b < _loadPos.size(); b++) {
+        const std::vector<float> &P = _loadPos[b], &Vl = _loadVel[b];
+        size_t n = P.size() / 3;
+        for (size_t i = 0; i < n; i++) {
+            size_t q = (size_t)((int)((int)i * inv)) * E + (int)i % E;
+            V3 x{P[3 * q], P[3 * q + 1], P[3 * q + 2]};
+            if (!bounds.inside(x)) continue;
+            hp.insert(hp.end(), {x.x, x.y, x.z});
+            hv.insert(hv.end(), {Vl[3 * q], Vl[3 * q + 1], Vl[3 * q + 2]});
+        }
+    }
+    _loadPos.clear();
+    _loadVel.clear();
+    int np = (int)(hp.size() / 3);
+    _capacity = np;
+    allocate();
+    // SoA split on the host, one copy per attribute
+    std::vector<float> col(std::max(np, 1));
+    for (int a = 0; a < 6; a++) {
+        const std::vector<float> &src = a < 3 ? hp : hv;
+        int c = a % 3;
+        for (int q = 0; q < np; q++) col[q] = src[3 * q + c];
+        hipCheck(hipMemcpy(_pa[a], col.data(), sizeof(float) * np, hipMemcpyHostToDevice), "upload particles");
+    }
+    _P = Particles{_pa[0], _pa[1], _pa[2], _pa[3], _pa[4], _pa[5], np};
+    _Q = Particles{_pb[0], _pb[1], _pb[2], _pb[3], _pb[4], _pb[5], 0};
+
+    // static solids: boundary SDF (or the given one), weights, near-solid grid
+    if (_solidGiven)
+        hipCheck(hipMemcpy(_solid, _solidHost.data(), sizeof(float) * _solidHost.size(), hipMemcpyHostToDevice),
+                 "upload solid SDF");
+    else
+        solidBoxPhi(_g, _solid, _stream);
+    faceWeights(_g, _solid, _wu, _wv, _ww, _stream);
+    int layers = (int)std::ceil((float)_cfl / (float)3);
+    nearSolidGrid(_g, _solid, _near, _nearTmp, _ci, _cj, _ck, layers, _stream);
+    hipCheck(hipMemsetAsync(_phi, 0, sizeof(float) * _phi.n, _stream), "memset");
+    hipCheck(hipMemsetAsync(_pressure, 0, sizeof(float) * _pressure.n, _stream), "memset");
+    hipCheck(hipMemsetAsync(_u, 0, sizeof(float) * _u.n, _stream), "memset");
+    hipCheck(hipMemsetAsync(_v, 0, sizeof(float) * _v.n, _stream), "memset");
+    hipCheck(hipMemsetAsync(_w, 0, sizeof(float) * _w.n, _stream), "memset");
+    hipCheck(hipStreamSynchronize(_stream), "initialize");
+    _initialized = true;
+}
+
+double FluidSimulation::hostScalar(const double *d) {
+    double h = 0.0;
+    hipCheck(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, _stream), "D2H scalar");
+    hipCheck(hipStreamSynchronize(_stream), "sync");
+    return h;
+}
+
+// _calculateNextTimeStep (fluidsimulation.cpp:10165-10263)
+double FluidSimulation::nextTimeStep(double dt) {
+    double maxu = 0.0;
+    if (_currentFrame == 0 && _stepNumber == 0) {
+        FFVector3 b = bodyForce();
+        float len = std::sqrt(b.x * b.x + b.y * b.y + b.z * b.z);
+        maxu += len * dt;
+    } else if (_P.n > 0) {
+        maxSpeedSq(_P, _partial, _scal + 0, _stream);
+        maxu = std::sqrt(hostScalar(_scal + 0));
+    }
+    if (_P.n > 0) maxu = std::max(0.0, maxu);
+    double eps = 1e-6;
+    double timeStep = _cfl * _g.dx / (maxu + eps);
+    int nsub = std::max((int)std::ceil(dt / timeStep), 1);
+    return dt / nsub;
+}
+
+// FluidSimulation::update (fluidsimulation.cpp:10410-10543)
+void FluidSimulation::update(double dt) {
+    if (!_initialized) throw std::runtime_error("Error: FluidSimulation must be initialized before update.\n");
+    if (dt < 0.0)
+        throw std::domain_error("Error: delta time must be greater than or equal to 0.\ndelta time: " +
+                                toString(dt) + "\n");
+    hipEvent_t t0, t1;
+    hipCheck(hipEventCreate(&t0), "event");
+    hipCheck(hipEventCreate(&t1), "event");
+    hipCheck(hipEventRecord(t0, _stream), "event");
+    double epsdt = 1e-6;
+    bool zeroLen = dt < epsdt;
+    dt = std::max(dt, epsdt);
+    _frameFinished = false;
+    _frameDt = dt;
+    _remaining = dt;
+    _stepNumber = 0;
+    bool skipped = zeroLen && _updatedOnce;
+    double substepTime = _frameDt / (double)_minSteps;
+    _pSuccess = true;
+    _pIters = 0;
+    _pErr = 0.0f;
+    double timing[4] = {0, 0, 0, 0};
+    size_t processed = 0;
+    double eps = 1e-9;
+    do {
+        double step = std::fmin(nextTimeStep(dt), _remaining);
+        double done = _frameDt - _remaining;
+        double limit = (_stepNumber + 1) * substepTime;
+        if (done + step > limit) step = std::fmin(substepTime, _remaining);
+        if (_stepNumber == _maxSteps - 1) step = _remaining;
+        _remaining -= step;
+        if (!skipped) {
+            stepFluid(step);
+            timing[0] += _phaseMs[0] + _phaseMs[1];
+            timing[1] += _phaseMs[2] + _phaseMs[3] + _phaseMs[4];
+            timing[2] += _phaseMs[5] + _phaseMs[6];
+            timing[3] += _phaseMs[7];
+        }
+        processed += _P.n;
+        _stepNumber++;
+    } while (_remaining > eps);
+    hipCheck(hipEventRecord(t1, _stream), "event");
+    hipCheck(hipEventSynchronize(t1), "event");
+    float total = 0;
+    hipCheck(hipEventElapsedTime(&total, t0, t1), "event");
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+
+    FFFrameStats st{};
+    st.frame = _currentFrame;
+    st.substeps = _stepNumber;
+    st.deltaTime = dt;
+    st.fluidParticles = _P.n;
+    st.diffuseParticles = 0;
+    double secs = timing[3] * 1e-3;
+    st.performanceScore = (processed == 0 || secs < 1e-9) ? -1 : (int)(((double)processed / secs) / 1000.0);
+    st.pressureSolverEnabled = 1;
+    st.pressureSolverSuccess = _pSuccess ? 1 : 0;
+    st.pressureSolverError = (double)_pErr;
+    st.pressureSolverIterations = _pIters;
+    st.pressureSolverMaxIterations = _maxPressureIterations;
+    st.viscositySolverEnabled = 0;
+    st.viscositySolverSuccess = 1;
+    st.viscositySolverMaxIterations = 900;
+    for (auto &m : st.mesh) m = FFMeshStats{0, -1, -1, 0};
+    st.timing.total = total * 1e-3;
+    st.timing.advection = timing[0] * 1e-3;
+    st.timing.pressure = timing[1] * 1e-3;
+    st.timing.particles = timing[2] * 1e-3;
+    _stats = st;
+    _updatedOnce = true;
+    _currentFrame++;
+    _frameFinished = true;
+}
+
+void FluidSimulation::step(double dt) {
+    requireInitialized("step");
+    if (!(dt > 0.0)) throw std::domain_error("Error: substep length must be greater than 0.\n");
+    _frameDt = dt;
+    stepFluid(dt);
+    _stepNumber++;
+}
+
+void FluidSimulation::extrapolate(int layers) {
+    const int ni = _g.ni, nj = _g.nj, nk = _g.nk;
+    _timers.begin(T_EXTRAPOLATE, _stream, 3 * layers + 3);
+    extrapolateInit(ni + 1, nj, nk, _vu, _stu, _stream);
+    extrapolateInit(ni, nj + 1, nk, _vv, _stv, _stream);
+    extrapolateInit(ni, nj, nk + 1, _vw, _stw, _stream);
+    for (int l = 0; l < layers; l++) {
+        extrapolateLayer(ni + 1, nj, nk, _u, _stu, l, _stream);
+        extrapolateLayer(ni, nj + 1, nk, _v, _stv, l, _stream);
+        extrapolateLayer(ni, nj, nk + 1, _w, _stw, l, _stream);
+    }
+    _timers.end(T_EXTRAPOLATE, _stream);
+}
+
+// FluidSimulation::_stepFluid (fluidsimulation.cpp:10078-10121), plain configuration
+void FluidSimulation::stepFluid(double dt) {
+    const size_t nu = _u.n, nv = _v.n, nw = _w.n;
+    const double radius = 0.5 * 1.0 * _g.dx * std::sqrt(3.0);  // fluidsimulation.cpp:4351
+    const int layers = (int)std::ceil(std::sqrt(3.0) * _cfl) + 3;  // :6282-6286
+    if (_P.n == 0) return;
+    hipCheck(hipEventRecord(_ev[0], _stream), "event");
+    // liquid level set (5586-5617)
+    _timers.begin(T_CELLSORT, _stream, 5);
+    cellSort(_g, _P, _cs, _scan, _stream);
+    _timers.end(T_CELLSORT, _stream);
+    _timers.begin(T_LIQUID_SDF, _stream);
+    liquidSDF(_g, _cs, _P.n, radius, _phi, _stream);
+    _timers.end(T_LIQUID_SDF, _stream);
+    liquidSDFPostprocess(_g, _solid, _phi, _stream);
+    hipCheck(hipEventRecord(_ev[1], _stream), "event");
+    // advect velocity field: P2G + extrapolation (5623-5661)
+    _timers.begin(T_P2G, _stream, 3);
+    p2g(_g, _cs, radius, _u, _v, _w, _vu, _vv, _vw, _stream);
+    _timers.end(T_P2G, _stream);
+    extrapolate(layers);
+    // save (5671-5685) and body force (5837-5876)
+    hipCheck(hipMemcpyAsync(_su, _u, sizeof(float) * nu, hipMemcpyDeviceToDevice, _stream), "save");
+    hipCheck(hipMemcpyAsync(_sv, _v, sizeof(float) * nv, hipMemcpyDeviceToDevice, _stream), "save");
+    hipCheck(hipMemcpyAsync(_sw, _w, sizeof(float) * nw, hipMemcpyDeviceToDevice, _stream), "save");
+    FFVector3 bf = bodyForce();
+    const float beps = 1e-6f;
+    if (std::fabs(bf.x) > beps) addToFaces(_u, nu, (float)(bf.x * dt), _stream);
+    if (std::fabs(bf.y) > beps) addToFaces(_v, nv, (float)(bf.y * dt), _stream);
+    if (std::fabs(bf.z) > beps) addToFaces(_w, nw, (float)(bf.z * dt), _stream);
+    hipCheck(hipEventRecord(_ev[2], _stream), "event");
+    // pressure (6152-6276)
+    bool success = pressureSolve(dt);
+    if (success) {
+        _timers.begin(T_APPLY_PRESSURE, _stream, 3);
+        applyPressure(_g, dt, _pressure, _phi, _wu, _wv, _ww, _u, _v, _w, _vu, _vv, _vw, _stream);
+        _timers.end(T_APPLY_PRESSURE, _stream);
+    }
+    if (_stepNumber == 0) {
+        _pSuccess = success;
+        _pIters = _lastIters;
+        _pErr = (float)_lastErr;
+    } else if (_pSuccess && (!success || _lastIters > _pIters)) {
+        _pSuccess = success;
+        _pIters = _lastIters;
+        _pErr = (float)_lastErr;
+    }
+    extrapolate(layers);
+    // constrain saved then current (6444-6459)
+    constrainFaces(_su, _wu, nu, _stream);
+    constrainFaces(_sv, _wv, nv, _stream);
+    constrainFaces(_sw, _ww, nw, _stream);
+    constrainFaces(_u, _wu, nu, _stream);
+    constrainFaces(_v, _wv, nv, _stream);
+    constrainFaces(_w, _ww, nw, _stream);
+    hipCheck(hipEventRecord(_ev[5], _stream), "event");
+    // G2P (6932-6945) + RK3 advection and collision (7853-7890)
+    _timers.begin(T_G2P_ADVECT, _stream);
+    g2pAdvect(_g, _P, MacView{_u, _v, _w}, MacView{_su, _sv, _sw}, _solid, _near, _ci, _cj, _ck,
+              (float)_picflip, (float)(1 - _picflip), dt, _cfl, _stream);
+    _timers.end(T_G2P_ADVECT, _stream);
+    hipCheck(hipEventRecord(_ev[6], _stream), "event");
+    // removal (7773-7851)
+    _timers.begin(T_REMOVAL, _stream);
+    removeParticles();
+    _timers.end(T_REMOVAL, _stream);
+    hipCheck(hipEventRecord(_ev[7], _stream), "event");
+    hipCheck(hipEventSynchronize(_ev[7]), "event");
+    float ms[7];
+    hipEventElapsedTime(&ms[0], _ev[0], _ev[1]);
+    hipEventElapsedTime(&ms[1], _ev[1], _ev[2]);
+    hipEventElapsedTime(&ms[2], _ev[2], _ev[3]);
+    hipEventElapsedTime(&ms[3], _ev[3], _ev[4]);
+    hipEventElapsedTime(&ms[4], _ev[4], _ev[5]);
+    hipEventElapsedTime(&ms[5], _ev[5], _ev[6]);
+    hipEventElapsedTime(&ms[6], _ev[6], _ev[7]);
+    float whole = 0;
+    hipEventElapsedTime(&whole, _ev[0], _ev[7]);
+    for (int q = 0; q < 7; q++) _phaseMs[q] = ms[q];
+    _phaseMs[7] = whole;
+    _timers.collect();
+}
+
+// PressureSolver::solve + applySolutionToVelocityField gate (fluidsimulation.cpp:6206-6249)
+bool FluidSimulation::pressureSolve(double dt) {
+    PressureResult r = _ps.solve(dt, _u, _v, _w, _wu, _wv, _ww, _phi, _pressureTolerance,
+                                 _maxPressureIterations, (int)_precision, _innerTolerance, _pressure,
+                                 _scan, _stream, _ev[3], &_timers);
+    _lastIters = r.iterations;
+    _lastErr = r.error;
+    _lastSuccess = r.success ? 1 : 0;
+    _lastUnknowns = r.unknowns;
+    hipCheck(hipEventRecord(_ev[4], _stream), "event");
+    return r.success;
+}
+
+// _removeMarkerParticles (fluidsimulation.cpp:7773-7851) with the speed limit
+// of _getMarkerParticleSpeedLimit (7723-7771), evaluated with the frame delta
+void FluidSimulation::removeParticles() {
+    const double dt = _frameDt;
+    const int nbins = _maxSteps;
+    if (_bins.n < (size_t)nbins) _bins.alloc(nbins);
+    double stepv = _cfl * _g.dx / dt;
+    speedHistogram(_P, stepv, nbins, _bins, _partial, _scal + 6, _stream);
+    std::vector<int> counts(nbins);
+    hipCheck(hipMemcpyAsync(counts.data(), _bins, sizeof(int) * nbins, hipMemcpyDeviceToHost, _stream), "D2H");
+    double maxParticleSpeed = hostScalar(_scal + 6);
+    int maxRemoval = (int)std::fmin((int)((double)_P.n * 0.0005), 35);
+    double maxspeed = _maxSteps * stepv;
+    int cur = 0;
+    for (int i = nbins - 1; i > 0; i--) {
+        if (cur + counts[i] > maxRemoval) break;
+        cur += counts[i];
+        maxspeed = std::max(i + 4, _maxSteps) * stepv;
+    }
+    double lower = 0.90 * maxParticleSpeed, thr = 0.99999 * maxParticleSpeed;
+    speedOutliers(_P, lower, thr, _outl, _stream);
+    int outl[2];
+    hipCheck(hipMemcpyAsync(outl, _outl, sizeof(int) * 2, hipMemcpyDeviceToHost, _stream), "D2H");
+    hipCheck(hipStreamSynchronize(_stream), "sync");
+    if (outl[1] <= 6 && outl[0] <= 6) maxspeed = std::min(thr, maxspeed);
+    float fmaxspeed = (float)maxspeed;
+    double maxspeedsq = fmaxspeed * fmaxspeed;
+    removalFlags(_g, _P, _solid, _keys, _counts, _slot, _offsets, _perm, _scan, 250, _extremeRemoval,
+                 maxspeedsq, _keep, _maxCount, _stream);
+    compactParticles(_P, _Q, _keep, _rank, _scan, _stream);
+    std::swap(_P, _Q);
+    _Q.n = 0;
+}
+
+int FluidSimulation::getNumMarkerParticles() const { return _initialized ? _P.n : 0; }
+
+namespace {
+void interleave(const float *x, const float *y, const float *z, int start, int end, float *out,
+                hipStream_t s) {
+    int n = end - start;
+    if (n <= 0) return;
+    std::vector<float> a(n), b(n), c(n);
+    hipCheck(hipMemcpyAsync(a.data(), x + start, sizeof(float) * n, hipMemcpyDeviceToHost, s), "D2H");
+    hipCheck(hipMemcpyAsync(b.data(), y + start, sizeof(float) * n, hipMemcpyDeviceToHost, s), "D2H");
+    hipCheck(hipMemcpyAsync(c.data(), z + start, sizeof(float) * n, hipMemcpyDeviceToHost, s), "D2H");
+    hipCheck(hipStreamSynchronize(s), "sync");
+    for (int q = 0; q < n; q++) {
+        out[3 * q] = a[q];
+        out[3 * q + 1] = b[q];
+        out[3 * q + 2] = c[q];
+    }
+}
+}  // namespace
+
+// getMarkerParticle{Position,Velocity}DataRange (fluidsimulation.cpp:3426-3460)
+void FluidSimulation::getMarkerParticlePositionDataRange(int start, int end, char *out) {
+    if (start < 0 || end > getNumMarkerParticles() || start > end)
+        throw std::domain_error("Error: invalid range.\nrange: [" + toString(start) + ", " + toString(end) + "]\n");
+    if (end > start) interleave(_P.px, _P.py, _P.pz, start, end, (float *)out, _stream);
+}
+void FluidSimulation::getMarkerParticleVelocityDataRange(int start, int end, char *out) {
+    if (start < 0 || end > getNumMarkerParticles() || start > end)
+        throw std::domain_error("Error: invalid range.\nrange: [" + toString(start) + ", " + toString(end) + "]\n");
+    if (end > start) interleave(_P.vx, _P.vy, _P.vz, start, end, (float *)out, _stream);
+}
+void FluidSimulation::getMarkerParticlePositionData(char *out) {
+    getMarkerParticlePositionDataRange(0, getNumMarkerParticles(), out);
+}
+// the reference's non-range velocity getter copies positions (fluidsimulation.cpp:3727-3735)
+void FluidSimulation::getMarkerParticleVelocityData(char *out) {
+    getMarkerParticlePositionDataRange(0, getNumMarkerParticles(), out);
+}
+
+void FluidSimulation::getLiquidSDF(float *out) {
+    requireInitialized("reading grids");
+    hipCheck(hipMemcpyAsync(out, _phi, sizeof(float) * _phi.n, hipMemcpyDeviceToHost, _stream), "D2H");
+    hipCheck(hipStreamSynchronize(_stream), "sync");
+}
+void FluidSimulation::getSolidSDF(float *out) {
+    requireInitialized("reading grids");
+    hipCheck(hipMemcpyAsync(out, _solid, sizeof(float) * _solid.n, hipMemcpyDeviceToHost, _stream), "D2H");
+    hipCheck(hipStreamSynchronize(_stream), "sync");
+}
+void FluidSimulation::getVelocityField(float *u, float *v, float *w) {
+    requireInitialized("reading grids");
+    hipCheck(hipMemcpyAsync(u, _u, sizeof(float) * _u.n, hipMemcpyDeviceToHost, _stream), "D2H");
+    hipCheck(hipMemcpyAsync(v, _v, sizeof(float) * _v.n, hipMemcpyDeviceToHost, _stream), "D2H");
+    hipCheck(hipMemcpyAsync(w, _w, sizeof(float) * _w.n, hipMemcpyDeviceToHost, _stream), "D2H");
+    hipCheck(hipStreamSynchronize(_stream), "sync");
+}
+void FluidSimulation::getPressureGrid(float *out) {
+    requireInitialized("reading grids");
+    hipCheck(hipMemcpyAsync(out, _pressure, sizeof(float) * _pressure.n, hipMemcpyDeviceToHost, _stream), "D2H");
+    hipCheck(hipStreamSynchronize(_stream), "sync");
+}
+void FluidSimulation::getLastPressureSolveStats(int *iterations, double *error, int *success, int *unknowns) {
+    *iterations = _lastIters;
+    *error = _lastErr;
+    *success = _lastSuccess;
+    *unknowns = _lastUnknowns;
+}
+void FluidSimulation::getKernelTiming(int id, double *ms, long long *launches) const {
+    if (id < 0 || id >= T_COUNT) throw std::domain_error("Error: kernel timer id out of range.\n");
+    *ms = _timers.ms(id);
+    *launches = _timers.launches(id);
+}
+double FluidSimulation::getLastPhaseMilliseconds(int phase) {
+    if (phase < 0 || phase > 7) throw std::domain_error("Error: phase must be in [0, 7].\n");
+    return _phaseMs[phase];
+}
+
+}  // namespace ff

@@ -1,0 +1,120 @@
+// ffdevice.h -- device-side arithmetic shared by the HIP kernels of the FLIP
+// substep.  Every helper keeps the float/double promotion points of the engine
+// it replaces (rlguy/Blender-FLIP-Fluids 1.8.5, src/engine/): vmath::vec3 is
+// three floats with component-wise float arithmetic (vmath.h:37-91), grid
+// index/position conversions are grid3d.h:32-124.  All HIP sources are
+// compiled with -ffp-contract=off and IEEE-rounded division and square root,
+// so the same inputs give the same bits as the x86-64 (SSE2, no FMA) engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FF_HD __host__ __device__ __forceinline__
+
+namespace ff {
+
+struct V3 {
+    float x, y, z;
+};
+FF_HD V3 mk(double x, double y, double z) { return V3{(float)x, (float)y, (float)z}; }
+FF_HD V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+FF_HD V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+FF_HD V3 scale(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+FF_HD float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+FF_HD float len3(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// vmath::operator/(vec3, float): multiply by the float reciprocal of a double division
+FF_HD V3 divide3(V3 a, float s) {
+    float inv = (float)(1.0 / (double)s);
+    return V3{a.x * inv, a.y * inv, a.z * inv};
+}
+FF_HD V3 normalize3(V3 a) { return divide3(a, len3(a)); }
+
+struct I3 {
+    int i, j, k;
+};
+FF_HD I3 posToIndex(V3 p, double dx) {
+    double inv = 1.0 / dx;
+    return I3{(int)floor((double)p.x * inv), (int)floor((double)p.y * inv),
+              (int)floor((double)p.z * inv)};
+}
+FF_HD I3 posToIndexD(double x, double y, double z, double dx) {
+    double inv = 1.0 / dx;
+    return I3{(int)floor(x * inv), (int)floor(y * inv), (int)floor(z * inv)};
+}
+FF_HD V3 indexToPos(int i, int j, int k, double dx) {
+    return mk((double)(float)i * dx, (double)(float)j * dx, (double)(float)k * dx);
+}
+FF_HD V3 indexToCellCenter(int i, int j, int k, double dx) {
+    double hw = 0.5 * dx;
+    return mk((double)(float)i * dx + hw, (double)(float)j * dx + hw, (double)(float)k * dx + hw);
+}
+FF_HD bool inRange(int i, int j, int k, int ni, int nj, int nk) {
+    return i >= 0 && j >= 0 && k >= 0 && i < ni && j < nj && k < nk;
+}
+FF_HD size_t flat(int i, int j, int k, int ni, int nj) {
+    return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
+}
+
+// AABB with float position and double extents (aabb.cpp:32-38, 122-133, 489-518)
+struct Box {
+    V3 pos;
+    double w, h, d;
+    FF_HD void expand(double v) {
+        double hh = 0.5 * v;
+        pos = sub(pos, mk(hh, hh, hh));
+        w += v;
+        h += v;
+        d += v;
+    }
+    FF_HD bool inside(V3 p) const {
+        return p.x >= pos.x && p.y >= pos.y && p.z >= pos.z && (double)p.x < (double)pos.x + w &&
+               (double)p.y < (double)pos.y + h && (double)p.z < (double)pos.z + d;
+    }
+    FF_HD V3 maxPoint() const { return add(pos, mk(w, h, d)); }
+    FF_HD V3 nearestInside(V3 p) const {
+        if (inside(p)) return p;
+        double eps = 1e-6;
+        V3 mn = pos, mx = maxPoint();
+        p.x = (float)fmax((double)p.x, (double)mn.x);
+        p.y = (float)fmax((double)p.y, (double)mn.y);
+        p.z = (float)fmax((double)p.z, (double)mn.z);
+        p.x = (float)fmin((double)p.x, (double)mx.x - eps);
+        p.y = (float)fmin((double)p.y, (double)mx.y - eps);
+        p.z = (float)fmin((double)p.z, (double)mx.z - eps);
+        return p;
+    }
+};
+
+// FluidSimulation::_getBoundaryAABB (fluidsimulation.cpp:5175-5180)
+FF_HD Box boundaryBox(int ni, int nj, int nk, double dx) {
+    Box b{V3{0.0f, 0.0f, 0.0f}, ni * dx, nj * dx, nk * dx};
+    b.expand(-3 * dx - 1e-4);
+    return b;
+}
+
+// Order-preserving map of a float onto an unsigned key, so that an unsigned
+// min over keys is the float min (used for deterministic LDS min-reductions).
+FF_HD uint32_t floatKey(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+FF_HD float keyFloat(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+
+// Interpolation::trilinearInterpolate(double p[8], x, y, z) (interpolation.cpp:61-70)
+FF_HD double trilin8(const double *p, double x, double y, double z) {
+    return p[0] * (1 - x) * (1 - y) * (1 - z) + p[1] * x * (1 - y) * (1 - z) +
+           p[2] * (1 - x) * y * (1 - z) + p[3] * (1 - x) * (1 - y) * z + p[4] * x * (1 - y) * z +
+           p[5] * (1 - x) * y * z + p[6] * x * y * (1 - z) + p[7] * x * y * z;
+}
+
+// Grid shape of one simulation (cells ni x nj x nk, cell size dx).
+struct Grid {
+    int ni, nj, nk;
+    double dx;
+};
+
+}  // namespace ff

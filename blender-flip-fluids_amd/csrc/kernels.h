@@ -1,0 +1,128 @@
+// kernels.h -- host-side launchers of the HIP kernels of the FLIP substep.
+// Every launcher enqueues on the given stream and returns without waiting.
+// Device pointers only; sizes in elements.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ffdevice.h"
+
+namespace ff {
+
+// ---- scan / reductions (kernels_scan.hip) ----
+// Exclusive prefix sum of n int32 values; out may alias in; out[n] = total.
+// scratch must hold scanScratchInts(n) ints.
+size_t scanScratchInts(size_t n);
+void exclusiveScan(const int *in, int *out, size_t n, int *scratch, hipStream_t s);
+// Deterministic FP64 dot product: fixed two-stage tree.  partial must hold
+// reduceBlocks(n) doubles; result is written to *out (device).
+int reduceBlocks(size_t n);
+void dot(const double *a, const double *b, size_t n, double *partial, double *out, hipStream_t s);
+void dotf(const float *a, const float *b, size_t n, double *partial, double *out, hipStream_t s);
+void absMax(const double *a, size_t n, double *partial, double *out, hipStream_t s);
+void absMaxf(const float *a, size_t n, double *partial, double *out, hipStream_t s);
+
+// ---- setup (kernels_setup.hip) ----
+void solidBoxPhi(Grid g, float *phiNodes, hipStream_t s);
+void faceWeights(Grid g, const float *phiNodes, float *wu, float *wv, float *ww, hipStream_t s);
+// near-solid coarse grid (cells of 3dx), marked then feathered `layers` times
+void nearSolidGrid(Grid g, const float *phiNodes, uint8_t *near, uint8_t *tmp, int ci, int cj,
+                   int ck, int layers, hipStream_t s);
+
+// ---- particles (kernels_particles.hip) ----
+struct Particles {
+    float *px, *py, *pz, *vx, *vy, *vz;
+    int n;
+};
+struct CellSort {
+    int *keys;     // P: flat cell of floor(p/dx) or -1
+    int *slot;     // P: arrival slot inside the cell (scratch)
+    int *counts;   // N+1
+    int *offsets;  // N+1 (exclusive scan of counts)
+    int *perm;     // P: sorted position -> particle index (stable)
+    // sorted copies
+    float *spx, *spy, *spz, *svx, *svy, *svz;
+};
+void cellSort(Grid g, const Particles &p, CellSort &cs, int *scanScratch, hipStream_t s);
+void liquidSDF(Grid g, const CellSort &cs, int np, double radius, float *phi, hipStream_t s);
+void liquidSDFPostprocess(Grid g, const float *solidNodes, float *phi, hipStream_t s);
+// P2G of all three components into u, v, w with validity flags
+void p2g(Grid g, const CellSort &cs, double radius, float *u, float *v, float *w, uint8_t *vu,
+         uint8_t *vv, uint8_t *vw, hipStream_t s);
+// max over particles of dot(v, v) (float) as double
+void maxSpeedSq(const Particles &p, double *partial, double *out, hipStream_t s);
+
+// G2P blend + RK3 + collision; writes new positions in place.
+struct MacView {
+    const float *u, *v, *w;
+};
+void g2pAdvect(Grid g, Particles p, MacView cur, MacView saved, const float *solidNodes,
+               const uint8_t *nearSolid, int ci, int cj, int ck, float picRatio, float flipRatio,
+               double dt, double cfl, hipStream_t s);
+// speed statistics for the extreme-velocity rule: counts[nbins] and max speed
+void speedHistogram(const Particles &p, double stepv, int nbins, int *counts, double *partial,
+                    double *maxSpeed, hipStream_t s);
+void speedOutliers(const Particles &p, double lower, double thr, int *counts2, hipStream_t s);
+// removal flags: solid test, per-cell cap, extreme speed.  keep[P] = 1 to keep.
+void removalFlags(Grid g, const Particles &p, const float *solidNodes, int *keys, int *counts,
+                  int *slot, int *offsets, int *perm, int *scanScratch, int maxPerCell,
+                  bool extremeRemoval, double maxSpeedSq, int *keep, int *maxCount,
+                  hipStream_t s);
+// stable compaction of the six particle arrays by keep flags (scan in rank)
+void compactParticles(Particles &src, Particles &dst, const int *keep, int *rank,
+                      int *scanScratch, hipStream_t s);
+
+// ---- grid stencils (kernels_grid.hip) ----
+void extrapolateInit(int wi, int wj, int wk, const uint8_t *valid, uint8_t *stat, hipStream_t s);
+void extrapolateLayer(int wi, int wj, int wk, float *field, uint8_t *stat, int layer, hipStream_t s);
+void addToFaces(float *a, size_t n, float inc, hipStream_t s);
+void constrainFaces(float *a, const float *w, size_t n, hipStream_t s);
+void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,
+                   const float *wv, const float *ww, float *u, float *v, float *w, uint8_t *vu,
+                   uint8_t *vv, uint8_t *vw, hipStream_t s);
+
+// ---- pressure (kernels_pressure.hip) ----
+struct PressureSys {
+    int *mask;      // N+1: pressure-cell flag, then scanned into row ids
+    int *rowOf;     // N+1 (exclusive scan of mask)
+    int *rows;      // M: flat cell of each row (k,j,i order)
+    double *rhs, *diag, *ox, *oy, *oz, *mx, *my, *mz;  // M
+    int *nbr;       // 6M: -x,+x,-y,+y,-z,+z row ids or -1
+    double *invdiag, *lx, *ly, *lz;  // M: MIC(0) factor (L entries toward +x/+y/+z)
+    float *finvdiag, *flx, *fly, *flz;  // FP32 copies for the mixed-precision inner solve
+    float *fdiag, *fox, *foy, *foz, *fmx, *fmy, *fmz;
+    int *levelCount;  // L+1
+    int *levelStart;  // L+1
+    int *levelFill;   // L
+    int *levelRows;   // M: rows grouped by i+j+k
+    int m;
+    int nLevels;
+};
+void pressureMask(Grid g, const float *phi, int *mask, hipStream_t s);
+void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, hipStream_t s);
+void pressureAssemble(Grid g, double dt, const float *u, const float *v, const float *w,
+                      const float *wu, const float *wv, const float *ww, const float *phi,
+                      PressureSys &S, hipStream_t s);
+void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s);
+// MIC(0) factor; level-by-level launches with host level table
+void mic0Factor(PressureSys &S, const int *hLevelStart, hipStream_t s);
+void mic0Apply(const PressureSys &S, const int *hLevelStart, const double *r, double *z,
+               hipStream_t s);
+void mic0ApplyF(const PressureSys &S, const int *hLevelStart, const float *r, float *z,
+                hipStream_t s);
+void spmv(const PressureSys &S, const double *x, double *y, hipStream_t s);
+void spmvF(const PressureSys &S, const float *x, float *y, hipStream_t s);
+// y += (float)(sign * num/den) * x   (device scalars; den may be null -> scalar = num)
+void axpyFloatStep(double *y, const double *x, size_t n, const double *num, const double *den,
+                   double sign, hipStream_t s);
+void axpyFloatStepF(float *y, const float *x, size_t n, const double *num, const double *den,
+                    double sign, hipStream_t s);
+// r = b - A x (FP64 true residual)
+void residual(const PressureSys &S, const double *b, const double *x, double *r, hipStream_t s);
+void pressureToGrid(Grid g, const PressureSys &S, const double *x, float *p, hipStream_t s);
+void castToFloat(const double *a, float *b, size_t n, hipStream_t s);
+void addFloatToDouble(double *x, const float *d, size_t n, hipStream_t s);
+void mic0ToFloat(PressureSys &S, hipStream_t s);
+
+}  // namespace ff

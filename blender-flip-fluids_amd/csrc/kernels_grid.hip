@@ -1,0 +1,165 @@
+// kernels_grid.hip -- face-array stencils of the FLIP substep: layered
+// velocity extrapolation, body force, solid constraint and the pressure
+// gradient update.
+#include "kernels.h"
+
+namespace ff {
+
+namespace {
+
+unsigned blocksFor(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+constexpr int TPB = 256;
+constexpr uint8_t ST_BORDER = 254, ST_UNKNOWN = 255;
+
+// GridUtils::extrapolateGrid (gridutils.h:95-160).  The status byte holds the
+// layer at which a face became known: 0 for valid faces, L+1 for faces filled
+// in layer L, ST_BORDER for the array border (always DONE, never a source of
+// new work) and ST_UNKNOWN.  In layer L the reference turns the KNOWN front
+// (status L) into DONE and fills every UNKNOWN face next to it with the mean
+// of its DONE neighbours in the order +i,-i,+j,-j,+k,-k (gridutils.h:43-92,
+// gridutils.cpp:120-175).  Faces written in this layer get L+1, which neither
+// the front test (== L) nor the DONE test (<= L) accepts, so one in-place pass
+// per layer is race free.
+__global__ void extrapInitKernel(int wi, int wj, int wk, const uint8_t *valid, uint8_t *st) {
+    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)wi * wj * wk;
+    if (f >= n) return;
+    int i = (int)(f % wi), j = (int)((f / wi) % wj), k = (int)(f / ((size_t)wi * wj));
+    uint8_t s;
+    if (i == 0 || j == 0 || k == 0 || i == wi - 1 || j == wj - 1 || k == wk - 1)
+        s = ST_BORDER;
+    else
+        s = valid[f] ? 0 : ST_UNKNOWN;
+    st[f] = s;
+}
+
+__global__ void extrapLayerKernel(int wi, int wj, int wk, float *a, uint8_t *st, int L) {
+    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)wi * wj * wk;
+    if (f >= n || st[f] != ST_UNKNOWN) return;
+    const long long nb[6] = {1, -1, wi, -(long long)wi, (long long)wi * wj, -(long long)wi * wj};
+    uint8_t s[6];
+    bool front = false;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        s[q] = st[(long long)f + nb[q]];
+        front |= s[q] == (uint8_t)L;
+    }
+    if (!front) return;
+    float sum = 0.0f;
+    int count = 0;
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+        if (s[q] == ST_BORDER || s[q] <= (uint8_t)L) {
+            sum += a[(long long)f + nb[q]];
+            count++;
+        }
+    a[f] = sum / (float)count;
+    st[f] = (uint8_t)(L + 1);
+}
+
+// _applyConstantBodyForces (fluidsimulation.cpp:5837-5876): no inflow
+// constraints on this path, so every face gets the increment
+__global__ void addKernel(float *a, size_t n, float inc) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) a[q] += inc;
+}
+
+// _constrainVelocityFieldThread (fluidsimulation.cpp:6388-6442): domain
+// friction 0 and zero face velocity of static solids
+__global__ void constrainKernel(float *a, const float *w, size_t n) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float f = 0.0f, sface = 0.0f;
+    float wt = w[q];
+    if (wt == 0)
+        a[q] = sface;
+    else if (wt < 1.0f)
+        a[q] = f * sface + (1.0f - f) * a[q];
+}
+
+// PressureSolver::_applyPressureToVelocityFieldThread (pressuresolver.cpp:1047-1205),
+// surface tension off and density 1; faces with index 0 or n-1 along their
+// axis are skipped (1058, 1109, 1159) and keep their value.
+template <int DIR>
+__global__ void applyPressureKernel(Grid g, double dt, const float *p, const float *phi,
+                                    const float *wgt, float *a, uint8_t *valid) {
+    const int wi = g.ni + (DIR == 0), wj = g.nj + (DIR == 1), wk = g.nk + (DIR == 2);
+    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)wi * wj * wk;
+    if (f >= n) return;
+    int i = (int)(f % wi), j = (int)((f / wi) % wj), k = (int)(f / ((size_t)wi * wj));
+    const int c = DIR == 0 ? i : (DIR == 1 ? j : k);
+    const int nc = DIR == 0 ? g.ni : (DIR == 1 ? g.nj : g.nk);
+    valid[f] = 0;
+    if (c == 0 || c == nc - 1) return;
+    const float factor = (float)(dt / g.dx);
+    const float eps = 1e-6f;
+    const double maxtheta = 25, tension = 0.0;
+    int ai = i - (DIR == 0), aj = j - (DIR == 1), ak = k - (DIR == 2);  // low-side cell
+    // isFaceBorderingMaterial (fluidmaterialgrid.cpp:126-146)
+    // The reference reads the cell past the last face (c == nc) when that face
+    // is open; with the domain walls it never is.  Reads stay in bounds here by
+    // using the low-side cell in its place.
+    const size_t la = flat(ai, aj, ak, g.ni, g.nj);
+    const size_t hb = c < nc ? flat(i, j, k, g.ni, g.nj) : la;
+    bool lo = c > 0 && phi[la] < 0.0f;
+    bool hi = c < nc && phi[hb] < 0.0f;
+    if (wgt[f] > 0 && (lo || hi)) {
+        float p1 = 0.0f, p2 = 0.0f;
+        if (lo && hi) {
+            p1 = p[la];
+            p2 = p[hb];
+        } else {
+            float phi1 = phi[la], phi2 = phi[hb];
+            if (lo) {
+                float thetaP = (float)fmax(-maxtheta, fmin((double)(phi2 / (phi1 + eps)), maxtheta));
+                float thetaT = (float)fmax(-maxtheta, fmin((double)((phi1 - phi2) / (phi1 + eps)), maxtheta));
+                p1 = p[la];
+                p2 = (float)((double)thetaT * tension + (double)(thetaP * p1));
+            } else {
+                float thetaP = (float)fmax(-maxtheta, fmin((double)(phi1 / (phi2 + eps)), maxtheta));
+                float thetaT = (float)fmax(-maxtheta, fmin((double)((phi2 - phi1) / (phi2 + eps)), maxtheta));
+                p2 = p[hb];
+                p1 = (float)((double)thetaT * tension + (double)(thetaP * p2));
+            }
+        }
+        float rho = (1.0f + 1.0f) / 2.0f;
+        a[f] += -factor * (p2 - p1) / rho;
+        valid[f] = 1;
+    } else {
+        a[f] = 0.0f;
+    }
+}
+
+}  // namespace
+
+void extrapolateInit(int wi, int wj, int wk, const uint8_t *valid, uint8_t *st, hipStream_t s) {
+    size_t n = (size_t)wi * wj * wk;
+    hipLaunchKernelGGL(extrapInitKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, wi, wj, wk, valid, st);
+}
+
+void extrapolateLayer(int wi, int wj, int wk, float *a, uint8_t *st, int layer, hipStream_t s) {
+    size_t n = (size_t)wi * wj * wk;
+    hipLaunchKernelGGL(extrapLayerKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, wi, wj, wk, a, st, layer);
+}
+
+void addToFaces(float *a, size_t n, float inc, hipStream_t s) {
+    hipLaunchKernelGGL(addKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, a, n, inc);
+}
+
+void constrainFaces(float *a, const float *w, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(constrainKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, a, w, n);
+}
+
+void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,
+                   const float *wv, const float *ww, float *u, float *v, float *w, uint8_t *vu,
+                   uint8_t *vv, uint8_t *vw, hipStream_t s) {
+    size_t nu = (size_t)(g.ni + 1) * g.nj * g.nk, nv = (size_t)g.ni * (g.nj + 1) * g.nk,
+           nw = (size_t)g.ni * g.nj * (g.nk + 1);
+    hipLaunchKernelGGL(applyPressureKernel<0>, dim3(blocksFor(nu, TPB)), dim3(TPB), 0, s, g, dt, p, phi, wu, u, vu);
+    hipLaunchKernelGGL(applyPressureKernel<1>, dim3(blocksFor(nv, TPB)), dim3(TPB), 0, s, g, dt, p, phi, wv, v, vv);
+    hipLaunchKernelGGL(applyPressureKernel<2>, dim3(blocksFor(nw, TPB)), dim3(TPB), 0, s, g, dt, p, phi, ww, w, vw);
+}
+
+}  // namespace ff

@@ -1,0 +1,653 @@
+// kernels_particles.hip -- particle-side kernels of the FLIP substep:
+// cell-key counting sort, liquid SDF, P2G gather, G2P + RK3 + collision,
+// removal flags and stable compaction.
+#include "kernels.h"
+
+namespace ff {
+
+namespace {
+
+unsigned blocksFor(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+constexpr int TPB = 256;
+
+// ------------------------------------------------------------ cell sort ----
+// Key = flat cell of Grid3d::positionToGridIndex(p, dx) (grid3d.h:53-58).
+// The arrival slot from the atomic counter gives an unordered placement;
+// sorting every cell's segment by particle index afterwards makes the whole
+// permutation the stable counting sort (ascending index inside a cell).
+__global__ void cellKeysKernel(Grid g, const float *px, const float *py, const float *pz, int np,
+                               int *keys, int *slot, int *counts) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    I3 c = posToIndex(V3{px[p], py[p], pz[p]}, g.dx);
+    int key = inRange(c.i, c.j, c.k, g.ni, g.nj, g.nk) ? (int)flat(c.i, c.j, c.k, g.ni, g.nj) : -1;
+    keys[p] = key;
+    if (key >= 0) slot[p] = atomicAdd(&counts[key], 1);
+}
+
+__global__ void scatterKernel(const int *keys, const int *slot, const int *offsets, int np, int *perm) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int key = keys[p];
+    if (key >= 0) perm[offsets[key] + slot[p]] = p;
+}
+
+__global__ void segmentSortKernel(const int *offsets, size_t ncells, int *perm) {
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int b = offsets[c], e = offsets[c + 1];
+    for (int a = b + 1; a < e; a++) {
+        int v = perm[a];
+        int q = a - 1;
+        while (q >= b && perm[q] > v) {
+            perm[q + 1] = perm[q];
+            q--;
+        }
+        perm[q + 1] = v;
+    }
+}
+
+// sorted copies; particles outside the grid have no key and no sorted slot,
+// so only the first offsets[N] entries of perm are filled
+__global__ void gatherSortedKernel(const int *perm, const int *total, Particles p, float *spx, float *spy,
+                                   float *spz, float *svx, float *svy, float *svz) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= p.n || s >= *total) return;
+    int q = perm[s];
+    spx[s] = p.px[q];
+    spy[s] = p.py[q];
+    spz[s] = p.pz[q];
+    svx[s] = p.vx[q];
+    svy[s] = p.vy[q];
+    svz[s] = p.vz[q];
+}
+
+// --------------------------------------------------------------- SDF ------
+// ParticleLevelSet::_computeSignedDistanceFromParticles (particlelevelset.cpp:335-666)
+// as one workgroup per 10^3 block: the block's min-distance tile lives in LDS
+// and every particle that belongs to the block (the count-data membership rule,
+// 484-537) lowers the cells of its block-local search range with an unsigned
+// atomic min on an order-preserving float key -- a min is order independent,
+// so the result is exact and deterministic.
+constexpr int BW = 10;
+__global__ __launch_bounds__(256) void liquidSDFKernel(Grid g, const int *offsets, const float *spx,
+                                                       const float *spy, const float *spz,
+                                                       double radius, float *phi, int nbi, int nbj) {
+    __shared__ uint32_t tile[BW * BW * BW];
+    const int b = blockIdx.x;
+    const int bi = b % nbi, bj = (b / nbi) % nbj, bk = b / (nbi * nbj);
+    const float maxd = (float)(3.0 * g.dx);
+    for (int c = threadIdx.x; c < BW * BW * BW; c += blockDim.x) tile[c] = floatKey(maxd);
+    __syncthreads();
+    const float blockdx = (float)(BW * g.dx);
+    const float r = (float)radius;
+    const float sr = 2.0f * r;
+    const V3 off = indexToPos(bi, bj, bk, BW * g.dx);
+    const int i0 = max(bi * BW - 2, 0), i1 = min(bi * BW + BW + 1, g.ni - 1);
+    const int j0 = max(bj * BW - 2, 0), j1 = min(bj * BW + BW + 1, g.nj - 1);
+    const int k0 = max(bk * BW - 2, 0), k1 = min(bk * BW + BW + 1, g.nk - 1);
+    const int nrowj = j1 - j0 + 1, nrows = nrowj * (k1 - k0 + 1);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nwaves = blockDim.x / 64;
+    for (int row = wave; row < nrows; row += nwaves) {
+        int j = j0 + row % nrowj, k = k0 + row / nrowj;
+        int s0 = offsets[flat(i0, j, k, g.ni, g.nj)];
+        int s1 = offsets[flat(i1, j, k, g.ni, g.nj) + 1];
+        for (int s = s0 + lane; s < s1; s += 64) {
+            V3 p{spx[s], spy[s], spz[s]};
+            // membership of block (bi, bj, bk)
+            I3 ob = posToIndex(p, blockdx);
+            V3 bp = indexToPos(ob.i, ob.j, ob.k, blockdx);
+            bool member;
+            if (p.x - sr > bp.x && p.y - sr > bp.y && p.z - sr > bp.z && p.x + sr < bp.x + blockdx &&
+                p.y + sr < bp.y + blockdx && p.z + sr < bp.z + blockdx) {
+                member = ob.i == bi && ob.j == bj && ob.k == bk;
+            } else {
+                I3 a = posToIndexD(p.x - sr, p.y - sr, p.z - sr, blockdx);
+                I3 c = posToIndexD(p.x + sr, p.y + sr, p.z + sr, blockdx);
+                member = bi >= a.i && bi <= c.i && bj >= a.j && bj <= c.j && bk >= a.k && bk <= c.k;
+            }
+            if (!member) continue;
+            V3 pl = sub(p, off);
+            I3 a = posToIndex(V3{pl.x - sr, pl.y - sr, pl.z - sr}, g.dx);
+            I3 c = posToIndex(V3{pl.x + sr, pl.y + sr, pl.z + sr}, g.dx);
+            a.i = max(a.i, 0); a.j = max(a.j, 0); a.k = max(a.k, 0);
+            c.i = min(c.i, BW - 1); c.j = min(c.j, BW - 1); c.k = min(c.k, BW - 1);
+            for (int kk = a.k; kk <= c.k; kk++)
+                for (int jj = a.j; jj <= c.j; jj++)
+                    for (int ii = a.i; ii <= c.i; ii++) {
+                        V3 gpos = indexToCellCenter(ii, jj, kk, g.dx);
+                        float dist = len3(sub(gpos, pl)) - r;
+                        atomicMin(&tile[ii + BW * (jj + BW * kk)], floatKey(dist));
+                    }
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < BW * BW * BW; c += blockDim.x) {
+        int ii = bi * BW + c % BW, jj = bj * BW + (c / BW) % BW, kk = bk * BW + c / (BW * BW);
+        if (ii < g.ni && jj < g.nj && kk < g.nk) phi[flat(ii, jj, kk, g.ni, g.nj)] = keyFloat(tile[c]);
+    }
+}
+
+// ParticleLevelSet::postProcessSignedDistanceField (particlelevelset.cpp:170-194)
+__global__ void sdfPostKernel(Grid g, const float *sp, float *phi) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)g.ni * g.nj * g.nk;
+    if (idx >= n) return;
+    int i = (int)(idx % g.ni), j = (int)((idx / g.ni) % g.nj), k = (int)(idx / ((size_t)g.ni * g.nj));
+    const int pi = g.ni + 1, pj = g.nj + 1;
+    float eps = (float)(0.005 * g.dx);
+    float val = phi[idx];
+    if ((double)val < 0.5 * g.dx) {
+        float sc = 0.125f * (sp[flat(i, j, k, pi, pj)] + sp[flat(i + 1, j, k, pi, pj)] +
+                             sp[flat(i, j + 1, k, pi, pj)] + sp[flat(i + 1, j + 1, k, pi, pj)] +
+                             sp[flat(i, j, k + 1, pi, pj)] + sp[flat(i + 1, j, k + 1, pi, pj)] +
+                             sp[flat(i, j + 1, k + 1, pi, pj)] + sp[flat(i + 1, j + 1, k + 1, pi, pj)]);
+        if (sc < 0) val = (float)(-0.5f * g.dx);
+    }
+    if (fabsf(val) < eps) val = val > 0 ? eps : -eps;
+    phi[idx] = val;
+}
+
+// --------------------------------------------------------------- P2G ------
+// VelocityAdvector FLIP transfer (velocityadvector.cpp:467-536) as a gather:
+// a face sums w*v and w over the particles within r of it, with the distance
+// taken in the local coordinates of the face's own 10^3 block (the block that
+// holds the face in the reference, whose particle list is in ascending
+// particle index).  The candidate cells' sorted lists are merged by particle
+// index, so the float sums run in the reference's order and no atomics are
+// used.
+template <int DIR>
+__global__ __launch_bounds__(256) void p2gKernel(Grid g, const int *offsets, const int *perm,
+                                                 const float *spx, const float *spy,
+                                                 const float *spz, const float *svel,
+                                                 double radius, float *field, uint8_t *valid) {
+    const int wi = g.ni + (DIR == 0), wj = g.nj + (DIR == 1), wk = g.nk + (DIR == 2);
+    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t nf = (size_t)wi * wj * wk;
+    if (f >= nf) return;
+    int fi = (int)(f % wi), fj = (int)((f / wi) % wj), fk = (int)(f / ((size_t)wi * wj));
+    const float eps = 1e-6f;
+    const float r = (float)radius;
+    const float rsq = r * r;
+    const float coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
+    const float coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
+    const float coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
+    const double h = 0.5 * g.dx;
+    const V3 offd = DIR == 0 ? mk(0.0, h, h) : (DIR == 1 ? mk(h, 0.0, h) : mk(h, h, 0.0));
+    const int bi = fi / BW, bj = fj / BW, bk = fk / BW;
+    const V3 offb = indexToPos(bi, bj, bk, BW * g.dx);
+    const V3 gpos = indexToPos(fi - bi * BW, fj - bj * BW, fk - bk * BW, g.dx);
+    // candidate cells: 2 along the face normal, 3 across
+    const int ci0 = DIR == 0 ? fi - 1 : fi - 1, ci1 = DIR == 0 ? fi : fi + 1;
+    const int cj0 = fj - 1, cj1 = DIR == 1 ? fj : fj + 1;
+    const int ck0 = fk - 1, ck1 = DIR == 2 ? fk : fk + 1;
+    int cur[18], end[18], head[18];
+    int nc = 0;
+    for (int k = ck0; k <= ck1; k++)
+        for (int j = cj0; j <= cj1; j++)
+            for (int i = ci0; i <= ci1; i++) {
+                if (!inRange(i, j, k, g.ni, g.nj, g.nk)) continue;
+                size_t c = flat(i, j, k, g.ni, g.nj);
+                cur[nc] = offsets[c];
+                end[nc] = offsets[c + 1];
+                nc++;
+            }
+    auto d2of = [&](int s) {
+        V3 p = sub(sub(V3{spx[s], spy[s], spz[s]}, offd), offb);
+        V3 v = sub(gpos, p);
+        return dot3(v, v);
+    };
+    auto advance = [&](int c) {
+        while (cur[c] < end[c]) {
+            if (d2of(cur[c]) < rsq) {
+                head[c] = perm[cur[c]];
+                return;
+            }
+            cur[c]++;
+        }
+        head[c] = 0x7fffffff;
+    };
+    for (int c = 0; c < nc; c++) advance(c);
+    float sum = 0.0f, wsum = 0.0f;
+    while (true) {
+        int best = -1, bidx = 0x7fffffff;
+        for (int c = 0; c < nc; c++)
+            if (head[c] < bidx) {
+                bidx = head[c];
+                best = c;
+            }
+        if (best < 0) break;
+        int s = cur[best];
+        float d2 = d2of(s);
+        float weight = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
+        sum += weight * svel[s];
+        wsum += weight;
+        cur[best]++;
+        advance(best);
+    }
+    if (wsum > eps) sum /= wsum;
+    field[f] = sum;
+    valid[f] = wsum > eps ? 1 : 0;
+}
+
+// ------------------------------------------------- max |v|^2 over particles
+__global__ __launch_bounds__(256) void maxSpeedStage1(Particles p, double *partial) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < p.n; q += gridDim.x * blockDim.x) {
+        V3 v{p.vx[q], p.vy[q], p.vz[q]};
+        acc = fmax(acc, (double)dot3(v, v));
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+__global__ __launch_bounds__(256) void maxStage2(const double *partial, int nb, double *out) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < nb; q += 256) acc = fmax(acc, partial[q]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// ------------------------------------------------- G2P + RK3 + collision ---
+struct FaceArr {
+    const float *a;
+    int wi, wj, wk;
+};
+// MACVelocityField::_interpolateLinearU/V/W after the in-grid test
+// (macvelocityfield.cpp:519-613); transverse coordinates already shifted.
+__device__ double interpFace(FaceArr F, double dx, double x, double y, double z) {
+    double invdx = 1.0 / dx;
+    int i = (int)floor(x * invdx), j = (int)floor(y * invdx), k = (int)floor(z * invdx);
+    double gx = (double)i * dx, gy = (double)j * dx, gz = (double)k * dx;
+    double inv_dx = 1 / dx;
+    double ix = (x - gx) * inv_dx, iy = (y - gy) * inv_dx, iz = (z - gz) * inv_dx;
+    double pts[8];
+    const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        int a = i + o[q][0], b = j + o[q][1], c = k + o[q][2];
+        pts[q] = inRange(a, b, c, F.wi, F.wj, F.wk) ? (double)F.a[flat(a, b, c, F.wi, F.wj)] : 0.0;
+    }
+    return trilin8(pts, ix, iy, iz);
+}
+// MACVelocityField::evaluateVelocityAtPositionLinear (macvelocityfield.cpp:631-645)
+__device__ V3 sampleMac(Grid g, MacView M, V3 p) {
+    double x = p.x, y = p.y, z = p.z;
+    if (!(x >= 0 && y >= 0 && z >= 0 && x < g.dx * g.ni && y < g.dx * g.nj && z < g.dx * g.nk))
+        return V3{0.0f, 0.0f, 0.0f};
+    double h = 0.5 * g.dx;
+    double xu = interpFace(FaceArr{M.u, g.ni + 1, g.nj, g.nk}, g.dx, x, y - h, z - h);
+    double yv = interpFace(FaceArr{M.v, g.ni, g.nj + 1, g.nk}, g.dx, x - h, y, z - h);
+    double zw = interpFace(FaceArr{M.w, g.ni, g.nj, g.nk + 1}, g.dx, x - h, y - h, z);
+    return mk(xu, yv, zw);
+}
+
+// Interpolation::trilinearInterpolate on the nodal solid SDF (interpolation.cpp:72-112)
+__device__ float nodalTrilinear(Grid g, const float *sp, V3 p) {
+    const int pi = g.ni + 1, pj = g.nj + 1, pk = g.nk + 1;
+    I3 c = posToIndex(p, g.dx);
+    V3 gp = indexToPos(c.i, c.j, c.k, g.dx);
+    double inv_dx = 1.0 / g.dx;
+    double ix = (double)(p.x - gp.x) * inv_dx, iy = (double)(p.y - gp.y) * inv_dx,
+           iz = (double)(p.z - gp.z) * inv_dx;
+    double pts[8];
+    const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        int a = c.i + o[q][0], b = c.j + o[q][1], e = c.k + o[q][2];
+        pts[q] = inRange(a, b, e, pi, pj, pk) ? (double)sp[flat(a, b, e, pi, pj)] : 0.0;
+    }
+    return (float)trilin8(pts, ix, iy, iz);
+}
+// Interpolation::trilinearInterpolateGradient (interpolation.cpp:189-260)
+__device__ double bilin(double v00, double v10, double v01, double v11, double ix, double iy) {
+    double l1 = (1 - ix) * v00 + ix * v10;
+    double l2 = (1 - ix) * v01 + ix * v11;
+    return (1 - iy) * l1 + iy * l2;
+}
+__device__ V3 nodalGradient(Grid g, const float *sp, V3 p) {
+    const int pi = g.ni + 1, pj = g.nj + 1, pk = g.nk + 1;
+    I3 c = posToIndex(p, g.dx);
+    V3 gp = indexToPos(c.i, c.j, c.k, g.dx);
+    double inv_dx = 1.0 / g.dx;
+    double ix = (double)(p.x - gp.x) * inv_dx, iy = (double)(p.y - gp.y) * inv_dx,
+           iz = (double)(p.z - gp.z) * inv_dx;
+    auto at = [&](int a, int b, int e) -> float {
+        return inRange(a, b, e, pi, pj, pk) ? sp[flat(a, b, e, pi, pj)] : 0.0f;
+    };
+    float v000 = at(c.i, c.j, c.k), v100 = at(c.i + 1, c.j, c.k), v010 = at(c.i, c.j + 1, c.k);
+    float v001 = at(c.i, c.j, c.k + 1), v101 = at(c.i + 1, c.j, c.k + 1);
+    float v011 = at(c.i, c.j + 1, c.k + 1), v110 = at(c.i + 1, c.j + 1, c.k);
+    float v111 = at(c.i + 1, c.j + 1, c.k + 1);
+    float dvdx = (float)bilin(v100 - v000, v110 - v010, v101 - v001, v111 - v011, iy, iz);
+    float dvdy = (float)bilin(v010 - v000, v110 - v100, v011 - v001, v111 - v101, ix, iz);
+    float dvdz = (float)bilin(v001 - v000, v101 - v100, v011 - v010, v111 - v110, ix, iy);
+    return V3{dvdx, dvdy, dvdz};
+}
+
+struct NearGrid {
+    const uint8_t *a;
+    int ci, cj, ck;
+    double cell;
+};
+__device__ bool nearAt(NearGrid N, V3 p) {
+    I3 c = posToIndex(p, N.cell);
+    if (!inRange(c.i, c.j, c.k, N.ci, N.cj, N.ck)) return false;
+    return N.a[flat(c.i, c.j, c.k, N.ci, N.cj)] != 0;
+}
+
+// FluidSimulation::_resolveCollision (fluidsimulation.cpp:7646-7721)
+__device__ V3 resolveCollision(Grid g, const float *sp, NearGrid N, double cfl, V3 oldp, V3 newp,
+                               const Box &boundary) {
+    I3 gg = posToIndex(newp, g.dx);
+    if (!inRange(gg.i, gg.j, gg.k, g.ni, g.nj, g.nk)) newp = boundary.nearestInside(newp);
+    if (!nearAt(N, oldp) && !nearAt(N, newp)) return newp;
+    const float eps = 1e-6f;
+    float stepDistance = 0.1f * (float)g.dx;
+    float travel = len3(sub(newp, oldp));
+    if (travel < eps) return newp;
+    // A non-finite travel converts to INT_MIN steps on x86 and the reference
+    // returns newp; the step count is also capped so that a runaway particle
+    // cannot stall the wave (the cap is 100 grid widths of march).
+    if (!(travel < 3.0e38f)) return newp;
+    float fsteps = ceilf(travel / stepDistance);
+    int numSteps = fsteps > 1.0e7f ? 10000000 : (int)fsteps;
+    V3 stepdir = normalize3(sub(newp, oldp));
+    V3 last = oldp, current{0, 0, 0};
+    bool found = false;
+    float collisionPhi = 0.0f;
+    for (int st = 0; st < numSteps; st++) {
+        if (st == numSteps - 1)
+            current = newp;
+        else
+            current = add(oldp, scale((float)(st + 1) * stepDistance, stepdir));
+        float ph = nodalTrilinear(g, sp, current);
+        if (ph < 0.0f || !boundary.inside(current)) {
+            collisionPhi = ph;
+            found = true;
+            break;
+        }
+        last = current;
+    }
+    if (!found) return newp;
+    V3 resolved;
+    float maxResolved = (float)(cfl * g.dx);
+    V3 grad = nodalGradient(g, sp, current);
+    if (len3(grad) > eps) {
+        grad = normalize3(grad);
+        resolved = sub(current, scale((float)((double)collisionPhi - (double)0.2f * g.dx), grad));
+        float rphi = nodalTrilinear(g, sp, resolved);
+        float rdist = len3(sub(resolved, current));
+        if (rphi < 0 || rdist > maxResolved) resolved = last;
+    } else {
+        resolved = last;
+    }
+    if (!boundary.inside(resolved)) {
+        V3 orig = resolved;
+        resolved = boundary.nearestInside(resolved);
+        float rphi = nodalTrilinear(g, sp, resolved);
+        float rdist = len3(sub(resolved, orig));
+        if (rphi < 0.0f || rdist > maxResolved) resolved = last;
+    }
+    return resolved;
+}
+
+// G2P (fluidsimulation.cpp:6771-6784), then _RK3 (7616-7623) on the unchanged
+// current field and the collision of the old/new pair (7626-7644).
+__global__ __launch_bounds__(256) void g2pAdvectKernel(Grid g, Particles p, MacView cur, MacView sav,
+                                                       const float *sp, NearGrid N, float r1,
+                                                       float r2, double dt, double cfl) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p.n) return;
+    V3 p0{p.px[q], p.py[q], p.pz[q]};
+    V3 v0{p.vx[q], p.vy[q], p.vz[q]};
+    V3 vPIC = sampleMac(g, cur, p0);
+    V3 vFLIP = sub(add(v0, vPIC), sampleMac(g, sav, p0));
+    V3 vn = add(scale(r1, vPIC), scale(r2, vFLIP));
+    p.vx[q] = vn.x;
+    p.vy[q] = vn.y;
+    p.vz[q] = vn.z;
+    V3 k1 = vPIC;
+    V3 k2 = sampleMac(g, cur, add(p0, scale((float)(0.5 * dt), k1)));
+    V3 k3 = sampleMac(g, cur, add(p0, scale((float)(0.75 * dt), k2)));
+    V3 p1 = add(p0, scale((float)(dt / (double)9.0f), add(add(scale(2.0f, k1), scale(3.0f, k2)), scale(4.0f, k3))));
+    Box boundary = boundaryBox(g.ni, g.nj, g.nk, g.dx);
+    boundary.expand((double)(-0.2f) * g.dx);
+    V3 pn = resolveCollision(g, sp, N, cfl, p0, p1, boundary);
+    p.px[q] = pn.x;
+    p.py[q] = pn.y;
+    p.pz[q] = pn.z;
+}
+
+// ------------------------------------------------------------ removal -----
+// _getMarkerParticleSpeedLimit (fluidsimulation.cpp:7723-7771): speed bins and max speed
+__global__ __launch_bounds__(256) void speedHistKernel(Particles p, double stepv, int nbins,
+                                                       int *counts, double *partial) {
+    __shared__ double s[256];
+    __shared__ int bins[1024];
+    const bool local = nbins <= 1024;
+    if (local)
+        for (int b = threadIdx.x; b < nbins; b += blockDim.x) bins[b] = 0;
+    __syncthreads();
+    double acc = 0.0;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < p.n; q += gridDim.x * blockDim.x) {
+        double speed = (double)len3(V3{p.vx[q], p.vy[q], p.vz[q]});
+        int idx = (int)fmin(floor(speed / stepv), (double)(nbins - 1));
+        atomicAdd(local ? &bins[idx] : &counts[idx], 1);
+        acc = fmax(acc, speed);
+    }
+    __syncthreads();
+    if (local)
+        for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+            if (bins[b]) atomicAdd(&counts[b], bins[b]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+__global__ void speedOutlierKernel(Particles p, double lower, double thr, int *counts2) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p.n) return;
+    double speed = (double)len3(V3{p.vx[q], p.vy[q], p.vz[q]});
+    if (speed >= lower && speed < thr) atomicAdd(&counts2[0], 1);
+    if (speed >= thr) atomicAdd(&counts2[1], 1);
+}
+
+// _removeMarkerParticles (fluidsimulation.cpp:7773-7851), first pass: solid
+// test (meshlevelset.h:333-339) and the cell key of every surviving particle
+__global__ void removalKeysKernel(Grid g, Particles p, const float *sp, int *keys, int *slot,
+                                  int *counts, int *maxCount) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    int myCount = 0;
+    if (q < p.n) {
+        V3 x{p.px[q], p.py[q], p.pz[q]};
+        int key = -1;
+        if (!(nodalTrilinear(g, sp, x) < 0.0f)) {
+            I3 c = posToIndex(x, g.dx);
+            if (inRange(c.i, c.j, c.k, g.ni, g.nj, g.nk)) key = (int)flat(c.i, c.j, c.k, g.ni, g.nj);
+        }
+        keys[q] = key;
+        if (key >= 0) {
+            slot[q] = atomicAdd(&counts[key], 1);
+            myCount = slot[q] + 1;
+        }
+    }
+    // block max of the counts seen, then one atomic per block
+    __shared__ int s[256];
+    s[threadIdx.x] = myCount;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) s[threadIdx.x] = max(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(maxCount, s[0]);
+}
+// second pass: per-cell cap on the rank in ascending particle index (rank is
+// null when no cell exceeds the cap), then the extreme-speed rule
+__global__ void removalFlagsKernel(Particles p, const int *keys, const int *rank, int maxPerCell,
+                                   int extreme, double maxSpeedSq, int *keep) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= p.n) return;
+    int k = 1;
+    if (keys[q] < 0)
+        k = 0;
+    else if (rank && rank[q] >= maxPerCell)
+        k = 0;
+    else if (extreme) {
+        V3 v{p.vx[q], p.vy[q], p.vz[q]};
+        if ((double)dot3(v, v) > maxSpeedSq) k = 0;
+    }
+    keep[q] = k;
+}
+__global__ void rankInCellKernel(const int *perm, const int *offsets, const int *keys, int ns, int *rank) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    int q = perm[s];
+    rank[q] = s - offsets[keys[q]];
+}
+
+// ParticleSystem::removeParticles (particlesystem.cpp:88-99): stable compaction
+__global__ void compactKernel(Particles src, Particles dst, const int *keep, const int *rank) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= src.n || !keep[q]) return;
+    int d = rank[q];
+    dst.px[d] = src.px[q];
+    dst.py[d] = src.py[q];
+    dst.pz[d] = src.pz[q];
+    dst.vx[d] = src.vx[q];
+    dst.vy[d] = src.vy[q];
+    dst.vz[d] = src.vz[q];
+}
+
+}  // namespace
+
+void cellSort(Grid g, const Particles &p, CellSort &cs, int *scanScratch, hipStream_t s) {
+    size_t N = (size_t)g.ni * g.nj * g.nk;
+    hipMemsetAsync(cs.counts, 0, sizeof(int) * (N + 1), s);
+    if (p.n > 0)
+        hipLaunchKernelGGL(cellKeysKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, g, p.px, p.py,
+                           p.pz, p.n, cs.keys, cs.slot, cs.counts);
+    exclusiveScan(cs.counts, cs.offsets, N, scanScratch, s);
+    if (p.n > 0) {
+        hipLaunchKernelGGL(scatterKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, cs.keys, cs.slot,
+                           cs.offsets, p.n, cs.perm);
+        hipLaunchKernelGGL(segmentSortKernel, dim3(blocksFor(N, TPB)), dim3(TPB), 0, s, cs.offsets, N,
+                           cs.perm);
+        hipLaunchKernelGGL(gatherSortedKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, cs.perm,
+                           cs.offsets + N, p, cs.spx, cs.spy, cs.spz, cs.svx, cs.svy, cs.svz);
+    }
+}
+
+void liquidSDF(Grid g, const CellSort &cs, int np, double radius, float *phi, hipStream_t s) {
+    int nbi = (g.ni + BW - 1) / BW, nbj = (g.nj + BW - 1) / BW, nbk = (g.nk + BW - 1) / BW;
+    (void)np;
+    hipLaunchKernelGGL(liquidSDFKernel, dim3(nbi * nbj * nbk), dim3(256), 0, s, g, cs.offsets, cs.spx,
+                       cs.spy, cs.spz, radius, phi, nbi, nbj);
+}
+
+void liquidSDFPostprocess(Grid g, const float *sp, float *phi, hipStream_t s) {
+    size_t n = (size_t)g.ni * g.nj * g.nk;
+    hipLaunchKernelGGL(sdfPostKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, sp, phi);
+}
+
+void p2g(Grid g, const CellSort &cs, double radius, float *u, float *v, float *w, uint8_t *vu,
+         uint8_t *vv, uint8_t *vw, hipStream_t s) {
+    size_t nu = (size_t)(g.ni + 1) * g.nj * g.nk, nv = (size_t)g.ni * (g.nj + 1) * g.nk,
+           nw = (size_t)g.ni * g.nj * (g.nk + 1);
+    hipLaunchKernelGGL(p2gKernel<0>, dim3(blocksFor(nu, TPB)), dim3(TPB), 0, s, g, cs.offsets, cs.perm,
+                       cs.spx, cs.spy, cs.spz, cs.svx, radius, u, vu);
+    hipLaunchKernelGGL(p2gKernel<1>, dim3(blocksFor(nv, TPB)), dim3(TPB), 0, s, g, cs.offsets, cs.perm,
+                       cs.spx, cs.spy, cs.spz, cs.svy, radius, v, vv);
+    hipLaunchKernelGGL(p2gKernel<2>, dim3(blocksFor(nw, TPB)), dim3(TPB), 0, s, g, cs.offsets, cs.perm,
+                       cs.spx, cs.spy, cs.spz, cs.svz, radius, w, vw);
+}
+
+void maxSpeedSq(const Particles &p, double *partial, double *out, hipStream_t s) {
+    int nb = (int)blocksFor(p.n > 0 ? p.n : 1, TPB * 8);
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(maxSpeedStage1, dim3(nb), dim3(256), 0, s, p, partial);
+    hipLaunchKernelGGL(maxStage2, dim3(1), dim3(256), 0, s, partial, nb, out);
+}
+
+void g2pAdvect(Grid g, Particles p, MacView cur, MacView saved, const float *solidNodes,
+               const uint8_t *nearSolid, int ci, int cj, int ck, float r1, float r2, double dt,
+               double cfl, hipStream_t s) {
+    if (p.n == 0) return;
+    NearGrid N{nearSolid, ci, cj, ck, 3 * g.dx};
+    hipLaunchKernelGGL(g2pAdvectKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, g, p, cur, saved,
+                       solidNodes, N, r1, r2, dt, cfl);
+}
+
+void speedHistogram(const Particles &p, double stepv, int nbins, int *counts, double *partial,
+                    double *maxSpeed, hipStream_t s) {
+    hipMemsetAsync(counts, 0, sizeof(int) * nbins, s);
+    int nb = (int)blocksFor(p.n > 0 ? p.n : 1, TPB * 8);
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(speedHistKernel, dim3(nb), dim3(256), 0, s, p, stepv, nbins, counts, partial);
+    hipLaunchKernelGGL(maxStage2, dim3(1), dim3(256), 0, s, partial, nb, maxSpeed);
+}
+
+void speedOutliers(const Particles &p, double lower, double thr, int *counts2, hipStream_t s) {
+    hipMemsetAsync(counts2, 0, sizeof(int) * 2, s);
+    if (p.n > 0)
+        hipLaunchKernelGGL(speedOutlierKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, p, lower, thr,
+                           counts2);
+}
+
+void removalFlags(Grid g, const Particles &p, const float *sp, int *keys, int *counts, int *slot,
+                  int *offsets, int *perm, int *scanScratch, int maxPerCell, bool extreme,
+                  double maxSpeedSq, int *keep, int *maxCount, hipStream_t s) {
+    if (p.n == 0) return;
+    size_t N = (size_t)g.ni * g.nj * g.nk;
+    hipMemsetAsync(counts, 0, sizeof(int) * (N + 1), s);
+    hipMemsetAsync(maxCount, 0, sizeof(int), s);
+    hipLaunchKernelGGL(removalKeysKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, g, p, sp, keys,
+                       slot, counts, maxCount);
+    int hmax = 0;
+    hipMemcpyAsync(&hmax, maxCount, sizeof(int), hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    const int *rank = nullptr;
+    if (hmax > maxPerCell) {
+        // exact cap: stable sort of the survivors by cell, rank inside the cell
+        exclusiveScan(counts, offsets, N, scanScratch, s);
+        hipLaunchKernelGGL(scatterKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, keys, slot, offsets,
+                           p.n, perm);
+        hipLaunchKernelGGL(segmentSortKernel, dim3(blocksFor(N, TPB)), dim3(TPB), 0, s, offsets, N, perm);
+        int ns = 0;
+        hipMemcpyAsync(&ns, offsets + N, sizeof(int), hipMemcpyDeviceToHost, s);
+        hipStreamSynchronize(s);
+        if (ns > 0)
+            hipLaunchKernelGGL(rankInCellKernel, dim3(blocksFor(ns, TPB)), dim3(TPB), 0, s, perm, offsets,
+                               keys, ns, slot);
+        rank = slot;
+    }
+    hipLaunchKernelGGL(removalFlagsKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, p, keys, rank,
+                       maxPerCell, extreme ? 1 : 0, maxSpeedSq, keep);
+}
+
+void compactParticles(Particles &src, Particles &dst, const int *keep, int *rank, int *scanScratch,
+                      hipStream_t s) {
+    exclusiveScan(keep, rank, src.n, scanScratch, s);
+    if (src.n > 0)
+        hipLaunchKernelGGL(compactKernel, dim3(blocksFor(src.n, TPB)), dim3(TPB), 0, s, src, dst, keep, rank);
+    int total = 0;
+    hipMemcpyAsync(&total, rank + src.n, sizeof(int), hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    dst.n = total;
+}
+
+}  // namespace ff

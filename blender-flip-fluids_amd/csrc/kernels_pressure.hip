@@ -1,0 +1,395 @@
+// kernels_pressure.hip -- the variational pressure solve of the FLIP substep:
+// pressure-cell rows, divergence and 7-point ghost-fluid assembly, the MIC(0)
+// factor, its level-scheduled triangular solves, SpMV and the PCG vector ops.
+//
+// Rows keep the reference's order (interior liquid cells in k,j,i order,
+// pressuresolver.cpp:118-159).  The factor and both triangular sweeps depend
+// only on the -x/-y/-z (or +x/+y/+z) neighbours, which sit on the previous
+// (next) i+j+k hyperplane, so running the rows level by level performs the
+// reference's own serial recurrences (pcgsolver.h:68-241) with identical
+// operations per row.
+#include "kernels.h"
+
+namespace ff {
+
+namespace {
+
+unsigned blocksFor(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+constexpr int TPB = 256;
+
+__global__ void maskKernel(Grid g, const float *phi, int *mask) {
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)g.ni * g.nj * g.nk;
+    if (c > n) return;
+    if (c == n) {
+        mask[n] = 0;
+        return;
+    }
+    int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / ((size_t)g.ni * g.nj));
+    bool interior = i >= 1 && j >= 1 && k >= 1 && i < g.ni - 1 && j < g.nj - 1 && k < g.nk - 1;
+    mask[c] = (interior && phi[c] < 0) ? 1 : 0;
+}
+
+__global__ void rowsKernel(size_t n, const int *mask, const int *rowOf, int *rows) {
+    size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n && mask[c]) rows[rowOf[c]] = (int)c;
+}
+
+// _calculateNegativeDivergenceVectorThread (pressuresolver.cpp:624-713) and
+// _calculateMatrixCoefficientsThread (767-867).  Static solids have zero face
+// velocity, so the divergence's solid terms add +-0.0 to a sum that is never
+// -0.0 and are left out.  Density is 1 (fluidsimulation.cpp:6214).  An
+// off-diagonal entry exists when the neighbour is liquid and a pressure row
+// (SparseMatrix::add ignores -1, sparsematrix.h:94-96).
+__global__ void assembleKernel(Grid g, double dt, const float *u, const float *v, const float *w,
+                               const float *wu, const float *wv, const float *ww, const float *phi,
+                               const int *mask, const int *rowOf, PressureSys S) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= S.m) return;
+    const int ni = g.ni, nj = g.nj;
+    int c = S.rows[r];
+    int i = c % ni, j = (c / ni) % nj, k = c / (ni * nj);
+    const double factor = 1.0 / g.dx;
+    const double mfactor = dt / (g.dx * g.dx);
+    const double eps = 1e-9, maxtheta = 25;
+    double volRight = wu[flat(i + 1, j, k, ni + 1, nj)], volLeft = wu[flat(i, j, k, ni + 1, nj)];
+    double volTop = wv[flat(i, j + 1, k, ni, nj + 1)], volBottom = wv[flat(i, j, k, ni, nj + 1)];
+    double volFront = ww[flat(i, j, k + 1, ni, nj)], volBack = ww[flat(i, j, k, ni, nj)];
+    double div = 0.0;
+    div += -factor * volRight * (double)u[flat(i + 1, j, k, ni + 1, nj)];
+    div += factor * volLeft * (double)u[flat(i, j, k, ni + 1, nj)];
+    div += -factor * volTop * (double)v[flat(i, j + 1, k, ni, nj + 1)];
+    div += factor * volBottom * (double)v[flat(i, j, k, ni, nj + 1)];
+    div += -factor * volFront * (double)w[flat(i, j, k + 1, ni, nj)];
+    div += factor * volBack * (double)w[flat(i, j, k, ni, nj)];
+    S.rhs[r] = div;
+
+    const double rho = (1.0 + 1.0) / 2.0;
+    double phiC = phi[c];
+    const size_t sx = 1, sy = (size_t)ni, sz = (size_t)ni * nj;
+    double diag = (volRight / rho + volLeft / rho + volTop / rho + volBottom / rho + volFront / rho +
+                   volBack / rho) * mfactor;
+    int *nb = S.nbr + (size_t)6 * r;
+    double off[6] = {0, 0, 0, 0, 0, 0};
+    const double vols[6] = {volLeft, volRight, volBottom, volTop, volBack, volFront};
+    const long long steps[6] = {-(long long)sx, (long long)sx, -(long long)sy, (long long)sy,
+                                -(long long)sz, (long long)sz};
+    // reference neighbour order: +x, -x, +y, -y, +z, -z
+    const int order[6] = {1, 0, 3, 2, 5, 4};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        int sl = order[q];
+        size_t nc = (size_t)((long long)c + steps[sl]);
+        double phiN = phi[nc];
+        nb[sl] = -1;
+        if (phiN < 0.0) {
+            if (mask[nc]) {
+                nb[sl] = rowOf[nc];
+                off[sl] = -(vols[sl] / rho) * mfactor;
+            }
+        } else {
+            double theta = phiN / (phiC + eps);
+            theta = fmax(-maxtheta, fmin(theta, maxtheta));
+            diag -= (vols[sl] / rho) * mfactor * theta;
+        }
+    }
+    diag = fmax(diag, 0.0);
+    S.diag[r] = diag;
+    S.mx[r] = off[0];
+    S.ox[r] = off[1];
+    S.my[r] = off[2];
+    S.oy[r] = off[3];
+    S.mz[r] = off[4];
+    S.oz[r] = off[5];
+}
+
+__global__ void levelCountKernel(Grid g, PressureSys S) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= S.m) return;
+    int c = S.rows[r];
+    int l = c % g.ni + (c / g.ni) % g.nj + c / (g.ni * g.nj);
+    atomicAdd(&S.levelCount[l], 1);
+}
+__global__ void levelScatterKernel(Grid g, PressureSys S) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= S.m) return;
+    int c = S.rows[r];
+    int l = c % g.ni + (c / g.ni) % g.nj + c / (g.ni * g.nj);
+    int slot = atomicAdd(&S.levelFill[l], 1);
+    S.levelRows[S.levelStart[l] + slot] = r;
+}
+
+// factorModifiedIncompleteColesky0 (pcgsolver.h:68-205), omega 0.97, min
+// diagonal ratio 0.25, one hyperplane of rows: every row pulls its updates
+// from the -z, -y, -x neighbour columns (ascending column order); within a
+// column the "missing" sum runs over the column's other rows in ascending row
+// order (+x, +y, +z).  A row whose diagonal is below 1e-9 is null: it keeps
+// its updated diagonal and contributes nothing as a column.
+__global__ void mic0LevelKernel(PressureSys S, const int *rows, int count) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int j = rows[t];
+    const int *nb = S.nbr + (size_t)6 * j;
+    double a = S.diag[j];
+    if (a < 1e-9) a = 0;
+    double d = a;
+    const int cslots[3] = {4, 2, 0};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        int k = nb[cslots[q]];
+        if (k < 0) continue;
+        if (S.diag[k] < 1e-9) continue;
+        int jslot = cslots[q] + 1;
+        const int *kb = S.nbr + (size_t)6 * k;
+        double ik = S.invdiag[k];
+        double okx = S.ox[k], oky = S.oy[k], okz = S.oz[k];
+        double Ljk = (jslot == 1 ? okx : (jslot == 3 ? oky : okz)) * ik;
+        d -= Ljk * Ljk;
+        double missing = 0;
+        if (jslot != 1 && kb[1] >= 0) missing += okx * ik;
+        if (jslot != 3 && kb[3] >= 0) missing += oky * ik;
+        if (jslot != 5 && kb[5] >= 0) missing += okz * ik;
+        d -= 0.97 * Ljk * missing;
+    }
+    double inv;
+    if (a == 0)
+        inv = d;
+    else if (d < 0.25 * a)
+        inv = 1 / sqrt(a);
+    else
+        inv = 1 / sqrt(d);
+    S.invdiag[j] = inv;
+}
+
+// L entries toward +x/+y/+z: A * invdiag, or A itself for a null column
+__global__ void lfactorKernel(PressureSys S) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= S.m) return;
+    bool null = S.diag[r] < 1e-9;
+    double iv = S.invdiag[r];
+    S.lx[r] = null ? S.ox[r] : S.ox[r] * iv;
+    S.ly[r] = null ? S.oy[r] : S.oy[r] * iv;
+    S.lz[r] = null ? S.oz[r] : S.oz[r] * iv;
+}
+
+// solveLower (pcgsolver.h:211-224), one hyperplane: subtractions in column
+// order -z, -y, -x, then the scaling by invdiag
+template <class T>
+__global__ void forwardLevelKernel(const int *nbr, const T *invdiag, const T *lx, const T *ly,
+                                   const T *lz, const int *rows, int count, const T *rv, T *z) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int j = rows[t];
+    const int *nb = nbr + (size_t)6 * j;
+    T y = rv[j];
+    int k = nb[4];
+    if (k >= 0) y -= lz[k] * z[k];
+    k = nb[2];
+    if (k >= 0) y -= ly[k] * z[k];
+    k = nb[0];
+    if (k >= 0) y -= lx[k] * z[k];
+    z[j] = y * invdiag[j];
+}
+// solveLowerTransposeInPlace (pcgsolver.h:227-241), one hyperplane
+template <class T>
+__global__ void backwardLevelKernel(const int *nbr, const T *invdiag, const T *lx, const T *ly,
+                                    const T *lz, const int *rows, int count, T *z) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    int i = rows[t];
+    const int *nb = nbr + (size_t)6 * i;
+    T x = z[i];
+    int r = nb[1];
+    if (r >= 0) x -= lx[i] * z[r];
+    r = nb[3];
+    if (r >= 0) x -= ly[i] * z[r];
+    r = nb[5];
+    if (r >= 0) x -= lz[i] * z[r];
+    z[i] = x * invdiag[i];
+}
+
+// FixedSparseMatrix multiply (sparsematrix.h:174-215): ascending column order
+template <class T>
+__global__ void spmvKernel(int m, const int *nbr, const T *diag, const T *ox, const T *oy,
+                           const T *oz, const T *mx, const T *my, const T *mz, const T *x, T *y) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const int *nb = nbr + (size_t)6 * r;
+    T s = 0;
+    if (nb[4] >= 0) s += mz[r] * x[nb[4]];
+    if (nb[2] >= 0) s += my[r] * x[nb[2]];
+    if (nb[0] >= 0) s += mx[r] * x[nb[0]];
+    s += diag[r] * x[r];
+    if (nb[1] >= 0) s += ox[r] * x[nb[1]];
+    if (nb[3] >= 0) s += oy[r] * x[nb[3]];
+    if (nb[5] >= 0) s += oz[r] * x[nb[5]];
+    y[r] = s;
+}
+
+// r = b - A x
+__global__ void residualKernel(int m, const int *nbr, const double *diag, const double *ox,
+                               const double *oy, const double *oz, const double *mx,
+                               const double *my, const double *mz, const double *b,
+                               const double *x, double *r) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const int *nb = nbr + (size_t)6 * q;
+    double s = 0;
+    if (nb[4] >= 0) s += mz[q] * x[nb[4]];
+    if (nb[2] >= 0) s += my[q] * x[nb[2]];
+    if (nb[0] >= 0) s += mx[q] * x[nb[0]];
+    s += diag[q] * x[q];
+    if (nb[1] >= 0) s += ox[q] * x[nb[1]];
+    if (nb[3] >= 0) s += oy[q] * x[nb[3]];
+    if (nb[5] >= 0) s += oz[q] * x[nb[5]];
+    r[q] = b[q] - s;
+}
+
+// BLAS::addScaled(float alpha, ...) (blaswrapper.h:128-157): the step length
+// is rounded to float before it scales the FP64 vector
+template <class T>
+__global__ void axpyStepKernel(T *y, const T *x, size_t n, const double *num, const double *den,
+                               double sign) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double a = den ? *num / *den : *num;
+    float fa = (float)(sign * a);
+    y[q] += (T)fa * x[q];
+}
+
+__global__ void toGridKernel(int m, const int *rows, const double *x, float *p) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m) p[rows[r]] = (float)x[r];
+}
+__global__ void castKernel(const double *a, float *b, size_t n) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) b[q] = (float)a[q];
+}
+__global__ void addFloatKernel(double *x, const float *d, size_t n) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] += (double)d[q];
+}
+
+}  // namespace
+
+void pressureMask(Grid g, const float *phi, int *mask, hipStream_t s) {
+    size_t n = (size_t)g.ni * g.nj * g.nk;
+    hipLaunchKernelGGL(maskKernel, dim3(blocksFor(n + 1, TPB)), dim3(TPB), 0, s, g, phi, mask);
+}
+
+void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, hipStream_t s) {
+    size_t n = (size_t)g.ni * g.nj * g.nk;
+    hipLaunchKernelGGL(rowsKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, n, mask, rowOf, rows);
+}
+
+void pressureAssemble(Grid g, double dt, const float *u, const float *v, const float *w,
+                      const float *wu, const float *wv, const float *ww, const float *phi,
+                      PressureSys &S, hipStream_t s) {
+    if (S.m == 0) return;
+    hipLaunchKernelGGL(assembleKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, g, dt, u, v, w, wu,
+                       wv, ww, phi, S.mask, S.rowOf, S);
+}
+
+void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s) {
+    S.nLevels = g.ni + g.nj + g.nk - 2;
+    hipMemsetAsync(S.levelCount, 0, sizeof(int) * (S.nLevels + 1), s);
+    hipMemsetAsync(S.levelFill, 0, sizeof(int) * S.nLevels, s);
+    if (S.m > 0)
+        hipLaunchKernelGGL(levelCountKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, g, S);
+    exclusiveScan(S.levelCount, S.levelStart, S.nLevels, scanScratch, s);
+    if (S.m > 0)
+        hipLaunchKernelGGL(levelScatterKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, g, S);
+}
+
+void mic0Factor(PressureSys &S, const int *hs, hipStream_t s) {
+    for (int l = 0; l < S.nLevels; l++) {
+        int cnt = hs[l + 1] - hs[l];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(mic0LevelKernel, dim3(blocksFor(cnt, TPB)), dim3(TPB), 0, s, S,
+                           S.levelRows + hs[l], cnt);
+    }
+    if (S.m > 0) hipLaunchKernelGGL(lfactorKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S);
+}
+
+template <class T>
+static void micApply(const PressureSys &S, const int *hs, const T *invdiag, const T *lx, const T *ly,
+                     const T *lz, const T *r, T *z, hipStream_t s) {
+    for (int l = 0; l < S.nLevels; l++) {
+        int cnt = hs[l + 1] - hs[l];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(forwardLevelKernel<T>, dim3(blocksFor(cnt, TPB)), dim3(TPB), 0, s, S.nbr,
+                           invdiag, lx, ly, lz, S.levelRows + hs[l], cnt, r, z);
+    }
+    for (int l = S.nLevels - 1; l >= 0; l--) {
+        int cnt = hs[l + 1] - hs[l];
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(backwardLevelKernel<T>, dim3(blocksFor(cnt, TPB)), dim3(TPB), 0, s, S.nbr,
+                           invdiag, lx, ly, lz, S.levelRows + hs[l], cnt, z);
+    }
+}
+
+void mic0Apply(const PressureSys &S, const int *hs, const double *r, double *z, hipStream_t s) {
+    micApply<double>(S, hs, S.invdiag, S.lx, S.ly, S.lz, r, z, s);
+}
+void mic0ApplyF(const PressureSys &S, const int *hs, const float *r, float *z, hipStream_t s) {
+    micApply<float>(S, hs, S.finvdiag, S.flx, S.fly, S.flz, r, z, s);
+}
+
+void spmv(const PressureSys &S, const double *x, double *y, hipStream_t s) {
+    if (S.m == 0) return;
+    hipLaunchKernelGGL(spmvKernel<double>, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.nbr, S.diag,
+                       S.ox, S.oy, S.oz, S.mx, S.my, S.mz, x, y);
+}
+void spmvF(const PressureSys &S, const float *x, float *y, hipStream_t s) {
+    if (S.m == 0) return;
+    hipLaunchKernelGGL(spmvKernel<float>, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.nbr, S.fdiag,
+                       S.fox, S.foy, S.foz, S.fmx, S.fmy, S.fmz, x, y);
+}
+
+void axpyFloatStep(double *y, const double *x, size_t n, const double *num, const double *den,
+                   double sign, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(axpyStepKernel<double>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, y, x, n, num, den, sign);
+}
+void axpyFloatStepF(float *y, const float *x, size_t n, const double *num, const double *den,
+                    double sign, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(axpyStepKernel<float>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, y, x, n, num, den, sign);
+}
+
+void residual(const PressureSys &S, const double *b, const double *x, double *r, hipStream_t s) {
+    if (S.m == 0) return;
+    hipLaunchKernelGGL(residualKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.nbr, S.diag, S.ox,
+                       S.oy, S.oz, S.mx, S.my, S.mz, b, x, r);
+}
+
+void pressureToGrid(Grid g, const PressureSys &S, const double *x, float *p, hipStream_t s) {
+    size_t n = (size_t)g.ni * g.nj * g.nk;
+    hipMemsetAsync(p, 0, sizeof(float) * n, s);
+    if (S.m > 0) hipLaunchKernelGGL(toGridKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.rows, x, p);
+}
+
+void castToFloat(const double *a, float *b, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(castKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, a, b, n);
+}
+void addFloatToDouble(double *x, const float *d, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(addFloatKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, x, d, n);
+}
+void mic0ToFloat(PressureSys &S, hipStream_t s) {
+    size_t m = S.m;
+    castToFloat(S.invdiag, S.finvdiag, m, s);
+    castToFloat(S.lx, S.flx, m, s);
+    castToFloat(S.ly, S.fly, m, s);
+    castToFloat(S.lz, S.flz, m, s);
+    castToFloat(S.diag, S.fdiag, m, s);
+    castToFloat(S.ox, S.fox, m, s);
+    castToFloat(S.oy, S.foy, m, s);
+    castToFloat(S.oz, S.foz, m, s);
+    castToFloat(S.mx, S.fmx, m, s);
+    castToFloat(S.my, S.fmy, m, s);
+    castToFloat(S.mz, S.fmz, m, s);
+}
+
+}  // namespace ff

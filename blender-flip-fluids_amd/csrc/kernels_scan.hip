@@ -1,0 +1,177 @@
+// kernels_scan.hip -- prefix sums and deterministic reductions.
+//
+// The scan orders pressure rows and compacts particles; the reductions are the
+// FP64 dot products and inf-norms of the pressure solve.  Reductions run a
+// fixed two-stage tree (fixed grid, fixed per-thread stride, fixed LDS tree),
+// so a given vector always reduces to the same bits.
+#include "kernels.h"
+
+namespace ff {
+
+namespace {
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAX_BLOCKS = 1024;
+
+__global__ __launch_bounds__(SCAN_THREADS) void scanTile(const int *in, int *out, size_t n,
+                                                         int *blockSums) {
+    __shared__ int s[SCAN_THREADS];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) v[q] = (base + q < n) ? in[base + q] : 0;
+    int run = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        int t = v[q];
+        v[q] = run;
+        run += t;
+    }
+    s[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        int add = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int excl = s[threadIdx.x] - run;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++)
+        if (base + q < n) out[base + q] = v[q] + excl;
+    if (threadIdx.x == SCAN_THREADS - 1) blockSums[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void scanAdd(int *out, size_t n, const int *blockOffsets) {
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    int add = blockOffsets[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++)
+        if (base + q < n) out[base + q] += add;
+}
+
+__global__ void writeTotal(int *out, size_t n, const int *blockSums, const int *blockOffsets,
+                           int nb) {
+    out[n] = blockOffsets[nb - 1] + blockSums[nb - 1];
+}
+
+size_t tiles(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+}  // namespace
+
+size_t scanScratchInts(size_t n) {
+    size_t total = 0;
+    while (true) {
+        size_t nb = tiles(n < 1 ? 1 : n);
+        total += 2 * nb + 2;
+        if (nb <= 1) break;
+        n = nb;
+    }
+    return total;
+}
+
+// out[0..n) = exclusive scan of in, out[n] = total.  Recursive over tile sums.
+void exclusiveScan(const int *in, int *out, size_t n, int *scratch, hipStream_t s) {
+    if (n == 0) {
+        hipMemsetAsync(out, 0, sizeof(int), s);
+        return;
+    }
+    size_t nb = tiles(n);
+    int *sums = scratch;
+    int *offs = scratch + nb + 1;
+    hipLaunchKernelGGL(scanTile, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, in, out, n, sums);
+    if (nb > 1) {
+        exclusiveScan(sums, offs, nb, scratch + 2 * nb + 2, s);
+        hipLaunchKernelGGL(scanAdd, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, out, n, offs);
+        hipLaunchKernelGGL(writeTotal, dim3(1), dim3(1), 0, s, out, n, sums, offs, (int)nb);
+    } else {
+        hipMemsetAsync(offs, 0, sizeof(int), s);
+        hipLaunchKernelGGL(writeTotal, dim3(1), dim3(1), 0, s, out, n, sums, offs, 1);
+    }
+}
+
+int reduceBlocks(size_t n) {
+    size_t b = (n + RED_THREADS * 8 - 1) / (RED_THREADS * 8);
+    if (b < 1) b = 1;
+    if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
+    return (int)b;
+}
+
+namespace {
+
+template <class T, int OP>  // OP 0: sum of a*b, 1: max |a|
+__global__ __launch_bounds__(RED_THREADS) void reduceStage1(const T *a, const T *b, size_t n,
+                                                            double *partial) {
+    __shared__ double s[RED_THREADS];
+    double acc = 0.0;
+    size_t stride = (size_t)gridDim.x * RED_THREADS;
+    for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += stride) {
+        if (OP == 0)
+            acc += (double)a[i] * (double)b[i];
+        else
+            acc = fmax(acc, fabs((double)a[i]));
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off) {
+            if (OP == 0)
+                s[threadIdx.x] += s[threadIdx.x + off];
+            else
+                s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+template <int OP>
+__global__ __launch_bounds__(RED_THREADS) void reduceStage2(const double *partial, int nb,
+                                                            double *out) {
+    __shared__ double s[RED_THREADS];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RED_THREADS) {
+        if (OP == 0)
+            acc += partial[i];
+        else
+            acc = fmax(acc, partial[i]);
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off) {
+            if (OP == 0)
+                s[threadIdx.x] += s[threadIdx.x + off];
+            else
+                s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+template <class T, int OP>
+void reduce(const T *a, const T *b, size_t n, double *partial, double *out, hipStream_t s) {
+    int nb = reduceBlocks(n);
+    hipLaunchKernelGGL((reduceStage1<T, OP>), dim3(nb), dim3(RED_THREADS), 0, s, a, b, n, partial);
+    hipLaunchKernelGGL((reduceStage2<OP>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+}
+
+}  // namespace
+
+void dot(const double *a, const double *b, size_t n, double *partial, double *out, hipStream_t s) {
+    reduce<double, 0>(a, b, n, partial, out, s);
+}
+void dotf(const float *a, const float *b, size_t n, double *partial, double *out, hipStream_t s) {
+    reduce<float, 0>(a, b, n, partial, out, s);
+}
+void absMax(const double *a, size_t n, double *partial, double *out, hipStream_t s) {
+    reduce<double, 1>(a, a, n, partial, out, s);
+}
+void absMaxf(const float *a, size_t n, double *partial, double *out, hipStream_t s) {
+    reduce<float, 1>(a, a, n, partial, out, s);
+}
+
+}  // namespace ff

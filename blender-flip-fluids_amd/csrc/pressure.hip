@@ -1,0 +1,242 @@
+// pressure.hip -- host side of the GPU pressure solve.
+#include "pressure.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+namespace ff {
+
+void hipCheck(hipError_t e, const char *what);
+
+PressureSolver::~PressureSolver() {
+    if (_block) (void)hipFree(_block);
+}
+
+void PressureSolver::allocate(Grid g) {
+    _g = g;
+    if (_block) (void)hipFree(_block);
+    const size_t N = (size_t)g.ni * g.nj * g.nk;
+    // pressure rows live on interior cells only
+    const size_t M = (size_t)std::max(g.ni - 2, 1) * std::max(g.nj - 2, 1) * std::max(g.nk - 2, 1);
+    const size_t L = (size_t)g.ni + g.nj + g.nk;
+    size_t bytes = 0;
+    auto take = [&](size_t nbytes) {
+        size_t off = bytes;
+        bytes += (nbytes + 255) / 256 * 256;
+        return off;
+    };
+    size_t oMask = take(4 * (N + 1)), oRowOf = take(4 * (N + 1)), oRows = take(4 * M), oNbr = take(24 * M);
+    size_t oLc = take(4 * (L + 1)), oLs = take(4 * (L + 1)), oLf = take(4 * L), oLr = take(4 * M);
+    size_t oD[12], oF[11];
+    for (auto &o : oD) o = take(8 * M);
+    for (auto &o : oF) o = take(4 * M);
+    size_t oV[4], oFV[5];
+    for (auto &o : oV) o = take(8 * M);
+    for (auto &o : oFV) o = take(4 * M);
+    size_t oPart = take(8 * 1024), oScal = take(8 * 16);
+    hipCheck(hipMalloc(&_block, bytes), "hipMalloc pressure");
+    char *b = (char *)_block;
+    _S.mask = (int *)(b + oMask);
+    _S.rowOf = (int *)(b + oRowOf);
+    _S.rows = (int *)(b + oRows);
+    _S.nbr = (int *)(b + oNbr);
+    _S.levelCount = (int *)(b + oLc);
+    _S.levelStart = (int *)(b + oLs);
+    _S.levelFill = (int *)(b + oLf);
+    _S.levelRows = (int *)(b + oLr);
+    double *d[12];
+    for (int q = 0; q < 12; q++) d[q] = (double *)(b + oD[q]);
+    _S.rhs = d[0]; _S.diag = d[1]; _S.ox = d[2]; _S.oy = d[3]; _S.oz = d[4];
+    _S.mx = d[5]; _S.my = d[6]; _S.mz = d[7]; _S.invdiag = d[8];
+    _S.lx = d[9]; _S.ly = d[10]; _S.lz = d[11];
+    float *f[11];
+    for (int q = 0; q < 11; q++) f[q] = (float *)(b + oF[q]);
+    _S.finvdiag = f[0]; _S.flx = f[1]; _S.fly = f[2]; _S.flz = f[3];
+    _S.fdiag = f[4]; _S.fox = f[5]; _S.foy = f[6]; _S.foz = f[7];
+    _S.fmx = f[8]; _S.fmy = f[9]; _S.fmz = f[10];
+    _x = (double *)(b + oV[0]); _r = (double *)(b + oV[1]); _z = (double *)(b + oV[2]); _s = (double *)(b + oV[3]);
+    _fr = (float *)(b + oFV[0]); _fz = (float *)(b + oFV[1]); _fs = (float *)(b + oFV[2]);
+    _fd = (float *)(b + oFV[3]); _fq = (float *)(b + oFV[4]);
+    _partial = (double *)(b + oPart);
+    _scal = (double *)(b + oScal);
+    _S.m = 0;
+    _S.nLevels = g.ni + g.nj + g.nk - 2;
+}
+
+double PressureSolver::hostScalar(const double *d, hipStream_t s) {
+    double h = 0.0;
+    hipCheck(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s), "D2H scalar");
+    hipCheck(hipStreamSynchronize(s), "sync");
+    return h;
+}
+
+// PressureSolver::solve (pressuresolver.cpp:52-92) and _solveLinearSystem (869-916)
+PressureResult PressureSolver::solve(double dt, const float *u, const float *v, const float *w,
+                                     const float *wu, const float *wv, const float *ww, const float *phi,
+                                     double tol, int maxIter, int mode, double innerTol, float *pressure,
+                                     int *scanScratch, hipStream_t s, hipEvent_t afterAssembly,
+                                     KernelTimers *timers) {
+    PressureResult res;
+    _t = timers ? timers : &_noTimers;
+    _t->begin(T_ASSEMBLE, s);
+    const size_t N = (size_t)_g.ni * _g.nj * _g.nk;
+    pressureMask(_g, phi, _S.mask, s);
+    exclusiveScan(_S.mask, _S.rowOf, N, scanScratch, s);
+    int m = 0;
+    hipCheck(hipMemcpyAsync(&m, _S.rowOf + N, sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+    hipCheck(hipStreamSynchronize(s), "sync");
+    _S.m = m;
+    res.unknowns = m;
+    pressureRows(_g, _S.mask, _S.rowOf, _S.rows, s);
+    pressureAssemble(_g, dt, u, v, w, wu, wv, ww, phi, _S, s);
+    pressureLevels(_g, _S, scanScratch, s);
+    _t->end(T_ASSEMBLE, s);
+    _hLevelStart.resize(_S.nLevels + 1);
+    hipCheck(hipMemcpyAsync(_hLevelStart.data(), _S.levelStart, sizeof(int) * (_S.nLevels + 1),
+                            hipMemcpyDeviceToHost, s), "D2H levels");
+    double maxAbs = 0.0;
+    if (m > 0) {
+        absMax(_S.rhs, m, _partial, _scal + 1, s);
+        maxAbs = hostScalar(_scal + 1, s);  // also waits for the level table
+    } else {
+        hipCheck(hipStreamSynchronize(s), "sync");
+    }
+    if (afterAssembly) hipCheck(hipEventRecord(afterAssembly, s), "event");
+    _levelLaunches = 0;
+    for (int l = 0; l < _S.nLevels; l++) _levelLaunches += _hLevelStart[l + 1] > _hLevelStart[l];
+    if (maxAbs < tol) {
+        hipCheck(hipMemsetAsync(pressure, 0, sizeof(float) * N, s), "memset");
+        res.success = true;
+        res.iterations = 0;
+        res.error = 0.0;
+        return res;
+    }
+    int iters = 0;
+    double resid = -1.0;
+    bool conv = mode == 0 ? pcgFP64(tol, maxIter, maxAbs, iters, resid, s)
+                          : pcgMixed(tol, maxIter, maxAbs, innerTol, iters, resid, s);
+    pressureToGrid(_g, _S, _x, pressure, s);
+    float e = (float)resid;
+    res.success = conv || (iters == maxIter && e < 1.0);
+    res.iterations = iters;
+    res.error = e;
+    return res;
+}
+
+// PCGSolver<double>::solve (pcgsolver.h:268-322): x0 = 0, MIC(0), float step
+// lengths, |r|_inf stop test against min(tol * |b|_inf, 1.0)
+bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &iters, double &resid,
+                             hipStream_t st) {
+    const int m = _S.m;
+    const int *hs = _hLevelStart.data();
+    if (tolFactor < 1e-30) tolFactor = 1e-30;
+    hipCheck(hipMemcpyAsync(_r, _S.rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
+    hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
+    resid = resid0;
+    double tol = tolFactor * resid;
+    _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 1);
+    mic0Factor(_S, hs, st);
+    _t->end(T_MIC0_FACTOR, st);
+    _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
+    mic0Apply(_S, hs, _r, _z, st);
+    _t->end(T_MIC0_APPLY, st);
+    double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
+    dot(_z, _r, m, _partial, rho, st);
+    double hrho = hostScalar(rho, st);
+    if (hrho == 0 || hrho != hrho) {
+        iters = 0;
+        return false;
+    }
+    hipCheck(hipMemcpyAsync(_s, _z, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
+    double *s = _s, *z = _z;
+    int it;
+    for (it = 0; it < maxIter; it++) {
+        _t->begin(T_SPMV, st);
+        spmv(_S, s, z, st);
+        _t->end(T_SPMV, st);
+        dot(s, z, m, _partial, sz, st);
+        axpyFloatStep(_x, s, m, rho, sz, 1.0, st);
+        axpyFloatStep(_r, z, m, rho, sz, -1.0, st);
+        absMax(_r, m, _partial, rn, st);
+        resid = hostScalar(rn, st);
+        if (resid <= std::min(tol, 1.0)) {
+            iters = it + 1;
+            return true;
+        }
+        _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
+        mic0Apply(_S, hs, _r, z, st);
+        _t->end(T_MIC0_APPLY, st);
+        dot(z, _r, m, _partial, rhoNew, st);
+        // beta = rhoNew / rho; z += (float)beta * s; then s <-> z
+        axpyFloatStep(z, s, m, rhoNew, rho, 1.0, st);
+        std::swap(s, z);
+        std::swap(rho, rhoNew);
+    }
+    iters = it;
+    return false;
+}
+
+// FP32 PCG inside FP64 iterative refinement.  The outer loop keeps the FP64
+// solution and its true residual r = b - A x; each inner solve runs PCG with
+// the same MIC(0) factor rounded to FP32 on FP32 vectors (FP64 dot products)
+// to a relative tolerance on its own right-hand side.  The stop test is the
+// reference's |r|_inf <= min(tol * |b|_inf, 1.0), on the true residual.
+bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, double innerRel, int &iters,
+                              double &resid, hipStream_t st) {
+    const int m = _S.m;
+    const int *hs = _hLevelStart.data();
+    if (tolFactor < 1e-30) tolFactor = 1e-30;
+    double tol = std::min(tolFactor * resid0, 1.0);
+    hipCheck(hipMemcpyAsync(_r, _S.rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
+    hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
+    _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 1);
+    mic0Factor(_S, hs, st);
+    _t->end(T_MIC0_FACTOR, st);
+    mic0ToFloat(_S, st);
+    double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
+    resid = resid0;
+    iters = 0;
+    int outer = 0;
+    while (resid > tol && iters < maxIter && outer < 100) {
+        outer++;
+        castToFloat(_r, _fr, m, st);
+        absMaxf(_fr, m, _partial, rn, st);
+        double innerTol = innerRel * hostScalar(rn, st);
+        hipCheck(hipMemsetAsync(_fd, 0, sizeof(float) * m, st), "memset");
+        _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
+        mic0ApplyF(_S, hs, _fr, _fz, st);
+        _t->end(T_MIC0_APPLY, st);
+        dotf(_fz, _fr, m, _partial, rho, st);
+        double hrho = hostScalar(rho, st);
+        if (hrho == 0 || hrho != hrho) return false;
+        hipCheck(hipMemcpyAsync(_fs, _fz, sizeof(float) * m, hipMemcpyDeviceToDevice, st), "copy");
+        float *s = _fs, *z = _fz;
+        while (iters < maxIter) {
+            _t->begin(T_SPMV, st);
+            spmvF(_S, s, _fq, st);
+            _t->end(T_SPMV, st);
+            dotf(s, _fq, m, _partial, sz, st);
+            axpyFloatStepF(_fd, s, m, rho, sz, 1.0, st);
+            axpyFloatStepF(_fr, _fq, m, rho, sz, -1.0, st);
+            absMaxf(_fr, m, _partial, rn, st);
+            iters++;
+            if (hostScalar(rn, st) <= innerTol) break;
+            _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
+            mic0ApplyF(_S, hs, _fr, z, st);
+            _t->end(T_MIC0_APPLY, st);
+            dotf(z, _fr, m, _partial, rhoNew, st);
+            axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);
+            std::swap(s, z);
+            std::swap(rho, rhoNew);
+        }
+        addFloatToDouble(_x, _fd, m, st);
+        residual(_S, _S.rhs, _x, _r, st);
+        absMax(_r, m, _partial, rn, st);
+        resid = hostScalar(rn, st);
+    }
+    return resid <= tol;
+}
+
+}  // namespace ff

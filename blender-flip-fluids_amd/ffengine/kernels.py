@@ -1,0 +1,133 @@
+"""numpy wrappers over the kernel-level entry points of libffengine.so.
+
+Each runs one HIP kernel group of the FLIP substep on host arrays (copied to
+and from the device) -- the parity tests compare them with the CPU oracle on
+identical inputs.  Grids are returned shaped (k, j, i).
+"""
+from ctypes import byref, c_double, c_int, c_void_p
+
+import numpy as np
+
+from . import pybindings as pb
+from .ffengine import ffengine as lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def _run(name, argtypes, args, restype=None):
+    f = getattr(lib, name)
+    pb.init_lib_func(f, argtypes + [c_void_p], restype)
+    ok = c_int()
+    r = f(*args, byref(ok))
+    pb.check_success(ok.value, name + " - ")
+    return r
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def cell_sort(n3, dx, pos):
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    npart, ncell = len(pos), ni * nj * nk
+    keys = np.zeros(npart, np.int32)
+    counts = np.zeros(ncell, np.int32)
+    offsets = np.zeros(ncell + 1, np.int32)
+    perm = np.zeros(npart, np.int32)
+    _run("FFAMD_cell_sort", [c_int] * 3 + [c_double, c_int] + [c_void_p] * 5,
+         [ni, nj, nk, dx, npart, _fp(pos), _fp(keys), _fp(counts), _fp(offsets), _fp(perm)])
+    return keys, counts, offsets, perm
+
+
+def liquid_sdf(n3, dx, pos, radius):
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    phi = np.zeros((nk, nj, ni), np.float32)
+    _run("FFAMD_liquid_sdf", [c_int] * 3 + [c_double, c_int, c_void_p, c_double, c_void_p],
+         [ni, nj, nk, dx, len(pos), _fp(pos), radius, _fp(phi)])
+    return phi
+
+
+def p2g(n3, dx, pos, vel, radius):
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    vel = _f32(vel).reshape(-1, 3)
+    u = np.zeros((nk, nj, ni + 1), np.float32)
+    v = np.zeros((nk, nj + 1, ni), np.float32)
+    w = np.zeros((nk + 1, nj, ni), np.float32)
+    vu, vv, vw = (np.zeros(a.shape, np.uint8) for a in (u, v, w))
+    _run("FFAMD_p2g", [c_int] * 3 + [c_double, c_int, c_void_p, c_void_p, c_double] + [c_void_p] * 6,
+         [ni, nj, nk, dx, len(pos), _fp(pos), _fp(vel), radius, _fp(u), _fp(v), _fp(w), _fp(vu), _fp(vv), _fp(vw)])
+    return (u, v, w), (vu, vv, vw)
+
+
+def extrapolate(field, valid, layers):
+    a = np.array(field, np.float32, copy=True, order="C")
+    vb = np.ascontiguousarray(valid, dtype=np.uint8)
+    wk, wj, wi = a.shape
+    _run("FFAMD_extrapolate", [c_int] * 3 + [c_void_p, c_void_p, c_int], [wi, wj, wk, _fp(a), _fp(vb), layers])
+    return a
+
+
+def solid_box_phi(n3, dx):
+    ni, nj, nk = n3
+    phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
+    _run("FFAMD_solid_box_phi", [c_int] * 3 + [c_double, c_void_p], [ni, nj, nk, dx, _fp(phi)])
+    return phi
+
+
+def face_weights(n3, phi_nodes):
+    ni, nj, nk = n3
+    p = _f32(phi_nodes)
+    wu = np.zeros((nk, nj, ni + 1), np.float32)
+    wv = np.zeros((nk, nj + 1, ni), np.float32)
+    ww = np.zeros((nk + 1, nj, ni), np.float32)
+    _run("FFAMD_face_weights", [c_int] * 3 + [c_void_p] * 4, [ni, nj, nk, _fp(p), _fp(wu), _fp(wv), _fp(ww)])
+    return wu, wv, ww
+
+
+def pressure_solve(n3, dx, dt, uvw, weights, liquid_phi, tol=1e-9, max_iter=900, mode=0):
+    """Pressure solve + projection; returns (pressure, (u, v, w), valid, iterations, error, success)."""
+    ni, nj, nk = n3
+    u, v, w = (np.array(a, np.float32, copy=True, order="C") for a in uvw)
+    wu, wv, ww = (_f32(a) for a in weights)
+    phi = _f32(liquid_phi)
+    p = np.zeros((nk, nj, ni), np.float32)
+    vu, vv, vw = (np.zeros(a.shape, np.uint8) for a in (u, v, w))
+    it, err = c_int(), c_double()
+    ok = _run("FFAMD_pressure_solve",
+              [c_int] * 3 + [c_double, c_double] + [c_void_p] * 7 + [c_double, c_int, c_int] + [c_void_p] * 6,
+              [ni, nj, nk, dx, dt, _fp(u), _fp(v), _fp(w), _fp(wu), _fp(wv), _fp(ww), _fp(phi), tol, max_iter,
+               mode, _fp(p), byref(it), byref(err), _fp(vu), _fp(vv), _fp(vw)], c_int)
+    return p, (u, v, w), (vu, vv, vw), it.value, err.value, bool(ok)
+
+
+def pressure_system(n3, dx, dt, uvw, weights, liquid_phi):
+    ni, nj, nk = n3
+    m_cap = max(ni - 2, 1) * max(nj - 2, 1) * max(nk - 2, 1)
+    u, v, w = (_f32(a) for a in uvw)
+    wu, wv, ww = (_f32(a) for a in weights)
+    phi = _f32(liquid_phi)
+    rows = np.zeros(m_cap, np.int32)
+    arr = [np.zeros(m_cap, np.float64) for _ in range(5)]
+    nbr = np.zeros(6 * m_cap, np.int32)
+    inv = np.zeros(m_cap, np.float64)
+    m = _run("FFAMD_pressure_system", [c_int] * 3 + [c_double, c_double] + [c_void_p] * 15,
+             [ni, nj, nk, dx, dt, _fp(u), _fp(v), _fp(w), _fp(wu), _fp(wv), _fp(ww), _fp(phi), _fp(rows)]
+             + [_fp(a) for a in arr] + [_fp(nbr), _fp(inv)], c_int)
+    rhs, diag, ox, oy, oz = (a[:m] for a in arr)
+    return {"rows": rows[:m], "rhs": rhs, "diag": diag, "ox": ox, "oy": oy, "oz": oz,
+            "nbr": nbr[:6 * m].reshape(m, 6), "invdiag": inv[:m]}
+
+
+def sample_velocity(n3, dx, uvw, pos):
+    ni, nj, nk = n3
+    u, v, w = (_f32(a) for a in uvw)
+    pos = _f32(pos).reshape(-1, 3)
+    out = np.zeros_like(pos)
+    _run("FFAMD_sample_velocity", [c_int] * 3 + [c_double] + [c_void_p] * 3 + [c_int, c_void_p, c_void_p],
+         [ni, nj, nk, dx, _fp(u), _fp(v), _fp(w), len(pos), _fp(pos), _fp(out)])
+    return out

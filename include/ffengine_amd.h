@@ -1,0 +1,227 @@
+/*
+ * ffengine_amd.h -- C ABI of the MI355X FLIP engine (libffengine.so).
+ *
+ * The entry points carry the names, argument meanings and error convention of
+ * the reference engine's C bindings (rlguy/Blender-FLIP-Fluids 1.8.5,
+ * src/engine/c_bindings/fluidsimulation_c.cpp and cbindings.cpp), so the
+ * reference's ctypes layer (src/engine/ffengine/) can load this library in
+ * place of libffengine.so for the per-substep path.  Citations are
+ * file:line into that tree.
+ *
+ * Error convention (cbindings.h:48-155, cbindings.cpp:35-43): every call
+ * except FluidSimulation_destroy and CBindings_get_error_message ends with
+ * int *err; *err = 1 on success and 0 on failure, in which case
+ * CBindings_get_error_message() returns the message of the exception (one
+ * process-wide buffer, like the reference).  No C++ exception crosses the ABI.
+ *
+ * Threading: one caller thread per FluidSimulation; calls are not re-entrant.
+ */
+#ifndef FFENGINE_AMD_H
+#define FFENGINE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Opaque handle (the reference returns FluidSimulation*). */
+typedef struct FluidSimulation FluidSimulation;
+
+/* Vector3_t (c_bindings/vector3_c.h) */
+typedef struct FFVector3 {
+    float x, y, z;
+} FFVector3;
+
+/* FluidSimulationMarkerParticleData (fluidsimulation.h:153-157): packed float
+ * xyz positions and velocities, passed by value. */
+typedef struct FluidSimulationMarkerParticleData {
+    int size;
+    char *positions;
+    char *velocities;
+} FluidSimulationMarkerParticleData;
+
+/* FluidSimulationMeshStats / TimingStats / FrameStats (fluidsimulation.h:62-151),
+ * same field order and types, returned by value. */
+typedef struct FFMeshStats {
+    int enabled;
+    int vertices;
+    int triangles;
+    unsigned int bytes;
+} FFMeshStats;
+typedef struct FFTimingStats {
+    double total, mesh, advection, particles, pressure, diffuse, viscosity, objects;
+} FFTimingStats;
+typedef struct FFFrameStats {
+    int frame;
+    int substeps;
+    double deltaTime;
+    int fluidParticles;
+    int diffuseParticles;
+    int performanceScore;
+    int pressureSolverEnabled;
+    int pressureSolverSuccess;
+    double pressureSolverError;
+    int pressureSolverIterations;
+    int pressureSolverMaxIterations;
+    int viscositySolverEnabled;
+    int viscositySolverSuccess;
+    double viscositySolverError;
+    int viscositySolverIterations;
+    int viscositySolverMaxIterations;
+    FFMeshStats mesh[50]; /* surface ... forcefield, in the reference's order */
+    FFTimingStats timing;
+} FFFrameStats;
+typedef FFFrameStats FluidSimulationFrameStats;
+
+/* ---- lifecycle ---- */
+FluidSimulation *FluidSimulation_new_from_dimensions(int isize, int jsize, int ksize, double dx,
+                                                     int *err); /* fluidsimulation_c.cpp:56-69 */
+void FluidSimulation_destroy(FluidSimulation *obj);             /* fluidsimulation_c.cpp:71-73 */
+void FluidSimulation_initialize(FluidSimulation *obj, int *err); /* fluidsimulation_c.cpp:97-99 */
+int FluidSimulation_is_initialized(FluidSimulation *obj, int *err); /* fluidsimulation_c.cpp:101-105 */
+void FluidSimulation_update(FluidSimulation *obj, double dt, int *err); /* fluidsimulation_c.cpp:115-119 */
+int FluidSimulation_get_current_frame(FluidSimulation *obj, int *err); /* fluidsimulation_c.cpp:121-125 */
+void FluidSimulation_set_current_frame(FluidSimulation *obj, int frameno, int *err); /* :127-131 */
+int FluidSimulation_is_current_frame_finished(FluidSimulation *obj, int *err); /* :148-153 */
+double FluidSimulation_get_cell_size(FluidSimulation *obj, int *err);          /* :155-159 */
+void FluidSimulation_get_grid_dimensions(FluidSimulation *obj, int *i, int *j, int *k,
+                                         int *err);                            /* :161-166 */
+
+/* ---- settings on the substep path ---- */
+double FluidSimulation_get_density(FluidSimulation *obj, int *err);            /* :253-257 */
+void FluidSimulation_set_density(FluidSimulation *obj, double density, int *err); /* :259-265 */
+int FluidSimulation_get_pressure_solver_max_iterations(FluidSimulation *obj, int *err); /* :318-323 */
+void FluidSimulation_set_pressure_solver_max_iterations(FluidSimulation *obj, int n, int *err); /* :325-331 */
+void FluidSimulation_enable_surface_reconstruction(FluidSimulation *obj, int *err);  /* :607-612 */
+void FluidSimulation_disable_surface_reconstruction(FluidSimulation *obj, int *err); /* :614-619 */
+int FluidSimulation_is_surface_reconstruction_enabled(FluidSimulation *obj, int *err); /* :621-626 */
+int FluidSimulation_get_max_thread_count(FluidSimulation *obj, int *err);       /* :2583-2589 */
+void FluidSimulation_set_max_thread_count(FluidSimulation *obj, int n, int *err); /* :2591-2597 */
+void FluidSimulation_add_body_force(FluidSimulation *obj, double fx, double fy, double fz,
+                                    int *err);                                 /* :2599-2605 */
+FFVector3 FluidSimulation_get_constant_body_force(FluidSimulation *obj, int *err); /* :2607-2613 */
+void FluidSimulation_reset_body_force(FluidSimulation *obj, int *err);         /* :2615-2619 */
+int FluidSimulation_get_CFL_condition_number(FluidSimulation *obj, int *err);  /* :2825-2831 */
+void FluidSimulation_set_CFL_condition_number(FluidSimulation *obj, int n, int *err); /* :2833-2839 */
+int FluidSimulation_get_min_time_steps_per_frame(FluidSimulation *obj, int *err); /* :2878-2884 */
+void FluidSimulation_set_min_time_steps_per_frame(FluidSimulation *obj, int n, int *err); /* :2886-2892 */
+int FluidSimulation_get_max_time_steps_per_frame(FluidSimulation *obj, int *err); /* :2894-2900 */
+void FluidSimulation_set_max_time_steps_per_frame(FluidSimulation *obj, int n, int *err); /* :2902-2908 */
+void FluidSimulation_enable_extreme_velocity_removal(FluidSimulation *obj, int *err); /* :2952-2957 */
+void FluidSimulation_disable_extreme_velocity_removal(FluidSimulation *obj, int *err); /* :2959-2964 */
+int FluidSimulation_is_extreme_velocity_removal_enabled(FluidSimulation *obj, int *err); /* :2966-2971 */
+void FluidSimulation_set_velocity_transfer_method_FLIP(FluidSimulation *obj, int *err); /* :2973-2978 */
+void FluidSimulation_set_velocity_transfer_method_APIC(FluidSimulation *obj, int *err); /* :2980-2985; fails: APIC is not on this path yet */
+int FluidSimulation_is_velocity_transfer_method_FLIP(FluidSimulation *obj, int *err); /* :2987-2992 */
+int FluidSimulation_is_velocity_transfer_method_APIC(FluidSimulation *obj, int *err); /* :2994-2999 */
+double FluidSimulation_get_PICFLIP_ratio(FluidSimulation *obj, int *err);      /* :3001-3005 */
+void FluidSimulation_set_PICFLIP_ratio(FluidSimulation *obj, double ratio, int *err); /* :3007-3013 */
+
+/* ---- particle I/O ---- */
+void FluidSimulation_load_marker_particle_data(FluidSimulation *obj,
+                                               FluidSimulationMarkerParticleData data,
+                                               int *err);                      /* :4872-4878 */
+int FluidSimulation_get_num_marker_particles(FluidSimulation *obj, int *err);  /* :3151-3156 */
+unsigned int FluidSimulation_get_marker_particle_position_data_size(FluidSimulation *obj,
+                                                                   int *err); /* :3952-3956 */
+unsigned int FluidSimulation_get_marker_particle_velocity_data_size(FluidSimulation *obj,
+                                                                   int *err); /* :3958-3962 */
+void FluidSimulation_get_marker_particle_position_data(FluidSimulation *obj, char *data,
+                                                       int *err);              /* :4823-4828 */
+/* Returns positions, like the reference (fluidsimulation.cpp:3727-3735). */
+void FluidSimulation_get_marker_particle_velocity_data(FluidSimulation *obj, char *data,
+                                                       int *err);              /* :4830-4835 */
+void FluidSimulation_get_marker_particle_position_data_range(FluidSimulation *obj, int start_idx,
+                                                             int end_idx, char *data,
+                                                             int *err);        /* :4625-4634 */
+void FluidSimulation_get_marker_particle_velocity_data_range(FluidSimulation *obj, int start_idx,
+                                                             int end_idx, char *data,
+                                                             int *err);        /* :4636-4645 */
+
+/* ---- stats ---- */
+FluidSimulationFrameStats FluidSimulation_get_frame_stats_data(FluidSimulation *obj,
+                                                               int *err);      /* :4618-4623 */
+char *CBindings_get_error_message(void);                                       /* cbindings.cpp:101-104 */
+
+/* ---- additive entry points (no counterpart in the reference) ---- */
+
+/* Relative pressure tolerance of the PCG stop test |r|_inf <= tol*|b|_inf.
+ * The reference hard-codes 1e-9 (fluidsimulation.h:2378). */
+void FluidSimulation_set_pressure_solver_tolerance(FluidSimulation *obj, double tol, int *err);
+double FluidSimulation_get_pressure_solver_tolerance(FluidSimulation *obj, int *err);
+
+/* Pressure precision: 0 = FP64 PCG with the reference's MIC(0) preconditioner
+ * and float-rounded step lengths; 1 = FP32 PCG inside FP64 iterative
+ * refinement (the FP64 true residual decides convergence), inner relative
+ * tolerance inner_tol. */
+void FluidSimulation_set_pressure_solver_precision(FluidSimulation *obj, int mode,
+                                                   double inner_tol, int *err);
+int FluidSimulation_get_pressure_solver_precision(FluidSimulation *obj, int *err);
+
+/* Nodal solid SDF ((isize+1)*(jsize+1)*(ksize+1) floats, i fastest, negative
+ * inside solids) replacing the domain-boundary SDF the engine builds itself.
+ * Solids are static; their face velocity is zero.  Call before initialize. */
+void FluidSimulation_set_solid_phi(FluidSimulation *obj, const float *phi_nodes, int *err);
+
+/* Grid getters (host copies of the last substep's grids).  Sizes: liquid SDF
+ * and pressure isize*jsize*ksize; solid SDF nodal; u (isize+1)*jsize*ksize,
+ * v isize*(jsize+1)*ksize, w isize*jsize*(ksize+1). */
+void FluidSimulation_get_liquid_sdf_data(FluidSimulation *obj, float *data, int *err);
+void FluidSimulation_get_solid_sdf_data(FluidSimulation *obj, float *data, int *err);
+void FluidSimulation_get_velocity_field_data(FluidSimulation *obj, float *u, float *v, float *w,
+                                             int *err);
+void FluidSimulation_get_pressure_grid_data(FluidSimulation *obj, float *data, int *err);
+/* iterations, |r|_inf, success flag and unknown count of the last pressure solve */
+void FluidSimulation_get_last_pressure_solve_stats(FluidSimulation *obj, int *iterations,
+                                                   double *error, int *success, int *unknowns,
+                                                   int *err);
+/* GPU milliseconds of the last substep's phases: 0 level sets, 1 P2G and
+ * extrapolation, 2 pressure system, 3 PCG, 4 projection and constraints,
+ * 5 G2P and advection, 6 removal, 7 whole substep. */
+double FluidSimulation_get_last_substep_phase_ms(FluidSimulation *obj, int phase, int *err);
+/* HIP-event timing of kernel groups on the engine's stream (off by default).
+ * Enabling resets the counters; ids: 0 cell sort, 1 liquid SDF, 2 P2G,
+ * 3 extrapolation, 4 pressure assembly, 5 MIC(0) factor, 6 SpMV, 7 MIC(0)
+ * apply (both sweeps), 8 (unused), 9 G2P + advection, 10 removal,
+ * 11 pressure gradient.  ms = summed kernel time, launches = kernel count. */
+void FluidSimulation_enable_kernel_timing(FluidSimulation *obj, int on, int *err);
+void FluidSimulation_get_kernel_timing(FluidSimulation *obj, int id, double *ms, long long *launches,
+                                       int *err);
+/* Run one substep of length dt with the current state (no frame split). */
+void FluidSimulation_step(FluidSimulation *obj, double dt, int *err);
+
+/* ---- kernel-level entry points on host arrays (parity tests) ---- */
+/* Stable cell-key counting sort: keys[np], counts[N], offsets[N+1], perm[np]. */
+void FFAMD_cell_sort(int isize, int jsize, int ksize, double dx, int np, const float *pos,
+                     int *keys, int *counts, int *offsets, int *perm, int *err);
+void FFAMD_liquid_sdf(int isize, int jsize, int ksize, double dx, int np, const float *pos,
+                      double radius, float *phi, int *err);
+void FFAMD_p2g(int isize, int jsize, int ksize, double dx, int np, const float *pos,
+               const float *vel, double radius, float *u, float *v, float *w,
+               unsigned char *vu, unsigned char *vv, unsigned char *vw, int *err);
+void FFAMD_extrapolate(int wi, int wj, int wk, float *field, const unsigned char *valid,
+                       int layers, int *err);
+void FFAMD_solid_box_phi(int isize, int jsize, int ksize, double dx, float *phi_nodes, int *err);
+void FFAMD_face_weights(int isize, int jsize, int ksize, const float *phi_nodes, float *wu,
+                        float *wv, float *ww, int *err);
+/* Pressure solve + projection on host grids; mode as in
+ * FluidSimulation_set_pressure_solver_precision.  u, v, w updated in place. */
+int FFAMD_pressure_solve(int isize, int jsize, int ksize, double dx, double dt, float *u,
+                         float *v, float *w, const float *wu, const float *wv, const float *ww,
+                         const float *liquid_phi, double tol, int max_iter, int mode,
+                         float *pressure, int *iterations, double *error, unsigned char *vu,
+                         unsigned char *vv, unsigned char *vw, int *err);
+/* MIC(0) system of the pressure solve: rows[M], rhs, diag, ox, oy, oz,
+ * nbr[6M], invdiag; returns M. Arrays must hold the interior cell count. */
+int FFAMD_pressure_system(int isize, int jsize, int ksize, double dx, double dt,
+                          const float *u, const float *v, const float *w, const float *wu,
+                          const float *wv, const float *ww, const float *liquid_phi, int *rows,
+                          double *rhs, double *diag, double *ox, double *oy, double *oz,
+                          int *nbr, double *invdiag, int *err);
+void FFAMD_sample_velocity(int isize, int jsize, int ksize, double dx, const float *u,
+                           const float *v, const float *w, int np, const float *pos, float *out,
+                           int *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1145,6 +1145,20 @@ void oracle_sample_velocity(int ni, int nj, int nk, double dx, const float *u, c
     }
 }
 
+/* Element actually read at position i of one marker-particle load batch.
+ * The batch is a FragmentedVector<MarkerParticle> (fluidsimulation.h:1628-1630)
+ * of 5e6-byte fragments, E = 5e6 / 24 = 208333 elements each
+ * (fragmentedvector.h:294-301, 320).  operator[] picks the fragment as
+ * (int)(i * (1.0 / E)) in double (fragmentedvector.h:140-151), which rounds
+ * below c at some exact multiples i = c*E and then reads element (c-1)*E. */
+size_t oracle_load_queue_index(size_t i) {
+    const int E = (int)(5000000u / 24u);
+    const double inv = 1.0 / (double)E;
+    int node = (int)((int)i * inv);
+    int item = (int)i % E;
+    return (size_t)node * E + item;
+}
+
 /* ====================================================== whole engine ===== */
 namespace {
 
@@ -1212,7 +1226,7 @@ struct OracleSim {
     int frame = 0, stepNumber = 0;
     double frameDt = 0, remaining = 0;
     std::vector<V3> pos, vel;
-    std::vector<V3> loadPos, loadVel;
+    std::vector<std::vector<V3>> loadPos, loadVel;  // one batch per load call
     bool solidGiven = false;
     std::vector<float> solid;         // nodal
     std::vector<float> wu, wv, ww;    // face weights
@@ -1241,11 +1255,16 @@ struct OracleSim {
     void initialize() {
         if (initialized) return;
         Box bounds{V3{0, 0, 0}, ni * dx, nj * dx, nk * dx};
-        for (size_t p = 0; p < loadPos.size(); p++)
-            if (bounds.inside(loadPos[p])) {
-                pos.push_back(loadPos[p]);
-                vel.push_back(loadVel[p]);
+        for (size_t b = 0; b < loadPos.size(); b++) {
+            const std::vector<V3> &P = loadPos[b], &Vl = loadVel[b];
+            for (size_t p = 0; p < P.size(); p++) {
+                size_t q = oracle_load_queue_index(p);
+                if (bounds.inside(P[q])) {
+                    pos.push_back(P[q]);
+                    vel.push_back(Vl[q]);
+                }
             }
+        }
         loadPos.clear();
         loadVel.clear();
         if (!solidGiven) {
@@ -1581,9 +1600,12 @@ void oracle_sim_set_solid_phi(OracleSim *s, const float *phi) {
     s->solidGiven = true;
 }
 void oracle_sim_load_particles(OracleSim *s, int np, const float *pos, const float *vel) {
+    if (np <= 0) return;
+    s->loadPos.emplace_back();
+    s->loadVel.emplace_back();
     for (int p = 0; p < np; p++) {
-        s->loadPos.push_back(V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]});
-        s->loadVel.push_back(V3{vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]});
+        s->loadPos.back().push_back(V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]});
+        s->loadVel.back().push_back(V3{vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]});
     }
 }
 void oracle_sim_initialize(OracleSim *s) { s->initialize(); }

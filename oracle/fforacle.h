@@ -15,6 +15,8 @@
 #ifndef FFORACLE_H
 #define FFORACLE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -91,6 +93,10 @@ void oracle_apply_pressure(int ni, int nj, int nk, double dx, double dt,
 void oracle_sample_velocity(int ni, int nj, int nk, double dx, const float *u,
                             const float *v, const float *w, int np,
                             const float *pos, float *out);
+
+/* Index of the element the reference's load queue returns for position i of
+ * one load_marker_particle_data batch (see fforacle.cpp). */
+size_t oracle_load_queue_index(size_t i);
 
 /* ---- whole-engine oracle with the FluidSimulation stepping semantics ---- */
 typedef struct OracleSim OracleSim;

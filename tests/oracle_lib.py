@@ -1,0 +1,316 @@
+"""ctypes access to the CPU checkers (test infrastructure only).
+
+* the oracle restatement, oracle/liboracle.so (built by oracle/Makefile);
+* the reference engine built from its sources, oracle/_ref/libffengine_ref.so,
+  and its stage probe oracle/_ref/libref_probe.so -- present only where the
+  reference sources were available to build them.
+
+Array conventions match ffengine.kernels: grids shaped (k, j, i).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+REF_ENGINE = os.path.join(REF_DIR, "libffengine_ref.so")
+REF_PROBE = os.path.join(REF_DIR, "libref_probe.so")
+
+_oracle = None
+_probe = None
+
+
+def fp(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.isfile(ORACLE_PATH):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle oracle`")
+        L = C.CDLL(ORACLE_PATH)
+        L.oracle_sim_new.restype = C.c_void_p
+        L.oracle_sim_new.argtypes = [C.c_int] * 3 + [C.c_double]
+        for name in ("oracle_sim_destroy", "oracle_sim_initialize", "oracle_sim_num_particles",
+                     "oracle_sim_last_substeps", "oracle_sim_last_pressure_iterations"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.oracle_sim_last_pressure_error.argtypes = [C.c_void_p]
+        L.oracle_sim_last_pressure_error.restype = C.c_double
+        L.oracle_sim_add_body_force.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.oracle_sim_set_picflip_ratio.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_sim_set_cfl.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_sim_set_pressure_tolerance.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_sim_set_max_pressure_iterations.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_sim_set_thread_chunks.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_sim_set_solid_phi.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_sim_load_particles.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.oracle_sim_update.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_sim_step.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_sim_get_particles.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_sim_get_grids.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.oracle_load_queue_index.restype = C.c_size_t
+        L.oracle_load_queue_index.argtypes = [C.c_size_t]
+        _oracle = L
+    return _oracle
+
+
+def reference_available():
+    return os.path.isfile(REF_ENGINE) and os.path.isfile(REF_PROBE)
+
+
+def probe():
+    global _probe
+    if _probe is None:
+        _probe = C.CDLL(REF_PROBE)
+        _probe.refprobe_set_threads(1)
+        _probe.refprobe_sim_new.restype = C.c_void_p
+    return _probe
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ---------------------------------------------------------------- oracle ----
+def cell_sort(n3, dx, pos):
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    N = ni * nj * nk
+    keys = np.zeros(len(pos), np.int32)
+    counts = np.zeros(N, np.int32)
+    offsets = np.zeros(N + 1, np.int32)
+    perm = np.zeros(len(pos), np.int32)
+    oracle().oracle_cell_sort(ni, nj, nk, C.c_double(dx), len(pos), fp(pos), fp(keys), fp(counts),
+                              fp(offsets), fp(perm))
+    return keys, counts, offsets, perm
+
+
+def solid_box_phi(n3, dx):
+    ni, nj, nk = n3
+    phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
+    oracle().oracle_solid_box_phi(ni, nj, nk, C.c_double(dx), fp(phi))
+    return phi
+
+
+def liquid_sdf(n3, dx, pos, radius):
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    phi = np.zeros((nk, nj, ni), np.float32)
+    oracle().oracle_liquid_sdf(ni, nj, nk, C.c_double(dx), len(pos), fp(pos), C.c_double(radius), fp(phi))
+    return phi
+
+
+def liquid_sdf_postprocess(n3, dx, solid, phi):
+    ni, nj, nk = n3
+    out = np.array(phi, np.float32, copy=True, order="C")
+    oracle().oracle_liquid_sdf_postprocess(ni, nj, nk, C.c_double(dx), fp(_f32(solid)), fp(out))
+    return out
+
+
+def p2g(n3, dx, pos, vel, radius):
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    vel = _f32(vel).reshape(-1, 3)
+    shapes = [(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]
+    fields, valids = [], []
+    for d, s in enumerate(shapes):
+        f = np.zeros(s, np.float32)
+        v = np.zeros(s, np.uint8)
+        oracle().oracle_p2g(ni, nj, nk, C.c_double(dx), len(pos), fp(pos), fp(vel), C.c_double(radius), d,
+                            fp(f), fp(v))
+        fields.append(f)
+        valids.append(v)
+    return tuple(fields), tuple(valids)
+
+
+def extrapolate(field, valid, layers):
+    a = np.array(field, np.float32, copy=True, order="C")
+    wk, wj, wi = a.shape
+    oracle().oracle_extrapolate(wi, wj, wk, fp(a), fp(np.ascontiguousarray(valid, np.uint8)), layers)
+    return a
+
+
+def face_weights(n3, phi_nodes):
+    ni, nj, nk = n3
+    wu = np.zeros((nk, nj, ni + 1), np.float32)
+    wv = np.zeros((nk, nj + 1, ni), np.float32)
+    ww = np.zeros((nk + 1, nj, ni), np.float32)
+    oracle().oracle_face_weights(ni, nj, nk, fp(_f32(phi_nodes)), fp(wu), fp(wv), fp(ww))
+    return wu, wv, ww
+
+
+def pressure_solve(n3, dx, dt, uvw, weights, liquid_phi, tol=1e-9, max_iter=900, nthreads=1):
+    ni, nj, nk = n3
+    u, v, w = (_f32(a) for a in uvw)
+    p = np.zeros((nk, nj, ni), np.float32)
+    st = (C.c_int * 2)()
+    err = C.c_double()
+    m = oracle().oracle_pressure_solve(ni, nj, nk, C.c_double(dx), C.c_double(dt), fp(u), fp(v), fp(w),
+                                       *[fp(_f32(x)) for x in weights], fp(_f32(liquid_phi)),
+                                       C.c_double(tol), max_iter, nthreads, fp(p), st, C.byref(err))
+    return p, st[0], err.value, bool(st[1]), m
+
+
+def pressure_system(n3, dx, dt, uvw, weights, liquid_phi):
+    ni, nj, nk = n3
+    cap = max(ni - 2, 1) * max(nj - 2, 1) * max(nk - 2, 1)
+    rows = np.zeros(cap, np.int32)
+    arr = [np.zeros(cap, np.float64) for _ in range(5)]
+    nbr = np.zeros(6 * cap, np.int32)
+    inv = np.zeros(cap, np.float64)
+    m = oracle().oracle_pressure_system(ni, nj, nk, C.c_double(dx), C.c_double(dt),
+                                        *[fp(_f32(x)) for x in uvw], *[fp(_f32(x)) for x in weights],
+                                        fp(_f32(liquid_phi)), fp(rows), *[fp(a) for a in arr], fp(nbr), fp(inv))
+    rhs, diag, ox, oy, oz = (a[:m] for a in arr)
+    return {"rows": rows[:m], "rhs": rhs, "diag": diag, "ox": ox, "oy": oy, "oz": oz,
+            "nbr": nbr[:6 * m].reshape(m, 6), "invdiag": inv[:m]}
+
+
+def apply_mic0(system, r):
+    m = len(system["rows"])
+    z = np.zeros(m, np.float64)
+    r = np.ascontiguousarray(r, np.float64)
+    oracle().oracle_apply_mic0(m, *[fp(np.ascontiguousarray(system[k])) for k in ("diag", "ox", "oy", "oz")],
+                               fp(np.ascontiguousarray(system["nbr"].reshape(-1), np.int32)),
+                               fp(np.ascontiguousarray(system["invdiag"])), fp(r), fp(z))
+    return z
+
+
+def apply_pressure(n3, dx, dt, pressure, liquid_phi, weights, uvw):
+    ni, nj, nk = n3
+    u, v, w = (np.array(a, np.float32, copy=True, order="C") for a in uvw)
+    vu, vv, vw = (np.zeros(a.shape, np.uint8) for a in (u, v, w))
+    oracle().oracle_apply_pressure(ni, nj, nk, C.c_double(dx), C.c_double(dt), fp(_f32(pressure)),
+                                   fp(_f32(liquid_phi)), *[fp(_f32(x)) for x in weights], fp(u), fp(v), fp(w),
+                                   fp(vu), fp(vv), fp(vw))
+    return (u, v, w), (vu, vv, vw)
+
+
+def sample_velocity(n3, dx, uvw, pos):
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    out = np.zeros_like(pos)
+    oracle().oracle_sample_velocity(ni, nj, nk, C.c_double(dx), *[fp(_f32(x)) for x in uvw], len(pos), fp(pos),
+                                    fp(out))
+    return out
+
+
+class OracleSim:
+    """The whole-engine oracle with the FluidSimulation stepping semantics."""
+
+    def __init__(self, n3, dx, gravity=(0.0, -9.81, 0.0), picflip=0.05, cfl=5, tol=1e-9, chunks=1):
+        L = oracle()
+        self.n3, self.dx = n3, dx
+        self.h = C.c_void_p(L.oracle_sim_new(*n3, dx))
+        L.oracle_sim_add_body_force(self.h, *gravity)
+        L.oracle_sim_set_picflip_ratio(self.h, picflip)
+        L.oracle_sim_set_cfl(self.h, cfl)
+        L.oracle_sim_set_pressure_tolerance(self.h, tol)
+        L.oracle_sim_set_thread_chunks(self.h, chunks)
+
+    def set_solid_phi(self, phi):
+        oracle().oracle_sim_set_solid_phi(self.h, fp(_f32(phi)))
+
+    def load(self, pos, vel=None):
+        pos = _f32(pos).reshape(-1, 3)
+        vel = np.zeros_like(pos) if vel is None else _f32(vel).reshape(-1, 3)
+        oracle().oracle_sim_load_particles(self.h, len(pos), fp(pos), fp(vel))
+
+    def initialize(self):
+        oracle().oracle_sim_initialize(self.h)
+
+    def update(self, dt):
+        oracle().oracle_sim_update(self.h, dt)
+
+    def step(self, dt):
+        oracle().oracle_sim_step(self.h, dt)
+
+    def particles(self):
+        n = oracle().oracle_sim_num_particles(self.h)
+        p = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        oracle().oracle_sim_get_particles(self.h, fp(p), fp(v))
+        return p, v
+
+    def substeps(self):
+        return oracle().oracle_sim_last_substeps(self.h)
+
+    def pressure_iterations(self):
+        return oracle().oracle_sim_last_pressure_iterations(self.h)
+
+    def grids(self):
+        ni, nj, nk = self.n3
+        phi = np.zeros((nk, nj, ni), np.float32)
+        u = np.zeros((nk, nj, ni + 1), np.float32)
+        v = np.zeros((nk, nj + 1, ni), np.float32)
+        w = np.zeros((nk + 1, nj, ni), np.float32)
+        p = np.zeros((nk, nj, ni), np.float32)
+        oracle().oracle_sim_get_grids(self.h, fp(phi), fp(u), fp(v), fp(w), fp(p))
+        return phi, (u, v, w), p
+
+    def __del__(self):
+        try:
+            oracle().oracle_sim_destroy(self.h)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------ reference engine (C ABI) ---
+class MarkerParticleData(C.Structure):
+    _fields_ = [("size", C.c_int), ("positions", C.c_void_p), ("velocities", C.c_void_p)]
+
+
+class ReferenceSim:
+    """The reference engine built from its sources, driven through its C ABI
+    (fluidsimulation_c.cpp) exactly like the reference's ctypes layer does."""
+
+    def __init__(self, n3, dx, threads=1, gravity=(0.0, -9.81, 0.0), picflip=0.05, cfl=5):
+        L = C.CDLL(REF_ENGINE)
+        self.L = L
+        err = C.c_int()
+        L.FluidSimulation_new_from_dimensions.restype = C.c_void_p
+        L.FluidSimulation_new_from_dimensions.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p]
+        self.h = C.c_void_p(L.FluidSimulation_new_from_dimensions(*n3, dx, C.byref(err)))
+        L.FluidSimulation_set_max_thread_count(self.h, threads, C.byref(err))
+        L.FluidSimulation_disable_surface_reconstruction(self.h, C.byref(err))
+        L.FluidSimulation_add_body_force(self.h, *[C.c_double(g) for g in gravity], C.byref(err))
+        L.FluidSimulation_set_PICFLIP_ratio(self.h, C.c_double(picflip), C.byref(err))
+        L.FluidSimulation_set_CFL_condition_number(self.h, cfl, C.byref(err))
+        L.FluidSimulation_load_marker_particle_data.argtypes = [C.c_void_p, MarkerParticleData, C.c_void_p]
+        L.FluidSimulation_get_num_marker_particles.restype = C.c_int
+        self._keep = []
+
+    def load(self, pos, vel=None):
+        pos = _f32(pos).reshape(-1, 3)
+        vel = np.zeros_like(pos) if vel is None else _f32(vel).reshape(-1, 3)
+        self._keep += [pos, vel]
+        err = C.c_int()
+        self.L.FluidSimulation_load_marker_particle_data(self.h, MarkerParticleData(len(pos), fp(pos), fp(vel)),
+                                                         C.byref(err))
+
+    def initialize(self):
+        err = C.c_int()
+        self.L.FluidSimulation_initialize(self.h, C.byref(err))
+
+    def update(self, dt):
+        err = C.c_int()
+        self.L.FluidSimulation_update(self.h, C.c_double(dt), C.byref(err))
+        assert err.value == 1
+
+    def particles(self):
+        err = C.c_int()
+        n = self.L.FluidSimulation_get_num_marker_particles(self.h, C.byref(err))
+        p = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        self.L.FluidSimulation_get_marker_particle_position_data_range(self.h, 0, n, fp(p), C.byref(err))
+        self.L.FluidSimulation_get_marker_particle_velocity_data_range(self.h, 0, n, fp(v), C.byref(err))
+        return p, v
+
+    def solid_phi(self, n3):
+        ni, nj, nk = n3
+        out = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
+        probe().refprobe_sim_get_solid_phi(self.h, fp(out))
+        return out
