@@ -1,0 +1,195 @@
+"""Writes the golden fixtures under tests/golden/ from the reference engine.
+
+The reference engine (rlguy/Blender-FLIP-Fluids 1.8.5, src/engine/) is built
+from its sources by `make -C oracle ref` into oracle/_ref/ (the engine library
+and a stage probe that calls its internal functions).  This script drives
+those builds on seeded synthetic inputs and stores inputs and outputs as
+compressed .npz files (no pickled objects) plus one JSON file of checksums:
+
+  stages_n16.npz   every stage of one substep at 16^3 with a box + sphere
+                   solid: solid box SDF, liquid SDF (raw and post-processed),
+                   P2G (u, v, w and valid masks), 12-layer extrapolation,
+                   face weights, the pressure solve (pressure grid,
+                   iterations, |r|_inf), apply-pressure and trilinear velocity
+                   sampling.  Probe calls: oracle/ref_probe.cpp.
+  e2e_n16.npz      the reference engine through its C ABI on the 16^3 dam
+                   break (8 particles per cell, g = -9.81 y, PIC/FLIP 0.05,
+                   CFL 5), particles after frames 1, 3 and 10 of 1/30 s.
+  e2e_c1_n64.json  the 64^3 dam break of the same recipe for 60 frames at
+                   1/30 s: per frame the particle count and the SHA-256 of the
+                   position and velocity bytes (the full snapshots would be
+                   5.6 MB each).
+
+Usage:  python tests/golden/make_golden.py [--threads 8] [--skip-c1]
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import ctypes as C  # noqa: E402
+
+import oracle_lib  # noqa: E402
+
+fp = oracle_lib.fp
+
+
+def dam_break(n, seed, frac=(0.375, 0.625, 1.0)):
+    """8 particles per cell in the lower-left block: one per octant of each
+    cell, uniformly jittered inside the octant (fluidsimulation.cpp:7994-8004
+    places the same eight sub-cell points, without jitter)."""
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / n
+    ci, cj, ck = int(frac[0] * n), int(frac[1] * n), int(frac[2] * n)
+    k, j, i = np.meshgrid(np.arange(ck), np.arange(cj), np.arange(ci), indexing="ij")
+    base = np.stack([i, j, k], -1).reshape(-1, 1, 3).astype(np.float64)
+    sub = np.array([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)], np.float64)[None] * 0.5
+    jit = rng.random((base.shape[0], 8, 3)) * 0.5
+    return ((base + sub + jit) * dx).reshape(-1, 3).astype(np.float32)
+
+
+def sphere_phi(n, center, radius):
+    """Nodal SDF of a solid sphere, negative inside (float32)."""
+    dx = 1.0 / n
+    g = np.arange(n + 1, dtype=np.float64) * dx
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    d = np.sqrt((x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2) - radius
+    return d.astype(np.float32)
+
+
+def make_stages(path):
+    P = oracle_lib.probe()
+    n = 16
+    dx = 1.0 / n
+    N, NN = n ** 3, (n + 1) ** 3
+    pos = dam_break(n, seed=11)
+    rng = np.random.default_rng(12)
+    vel = (rng.standard_normal(pos.shape) * 0.3).astype(np.float32)
+    radius = 0.5 * dx * np.sqrt(3.0)
+    out = {"n": np.int32(n), "dx": np.float64(dx), "pos": pos, "vel": vel, "radius": np.float64(radius)}
+
+    box = np.zeros(NN, np.float32)
+    P.refprobe_solid_box_phi(n, n, n, C.c_double(dx), fp(box))
+    out["solid_box_phi"] = box
+    solid = np.minimum(box, sphere_phi(n, (0.6, 0.3, 0.5), 0.2).reshape(-1))
+    out["solid_phi"] = solid
+
+    sdf = np.zeros(N, np.float32)
+    P.refprobe_liquid_sdf(n, n, n, C.c_double(dx), len(pos), fp(pos), C.c_double(radius), fp(sdf))
+    out["liquid_sdf"] = sdf.copy()
+    P.refprobe_liquid_sdf_postprocess(n, n, n, C.c_double(dx), fp(solid), fp(sdf))
+    out["liquid_phi"] = sdf
+
+    fsize = (n + 1) * n * n
+    uvw = [np.zeros(fsize, np.float32) for _ in range(3)]
+    valid = [np.zeros(fsize, np.uint8) for _ in range(3)]
+    P.refprobe_p2g(n, n, n, C.c_double(dx), len(pos), fp(pos), fp(vel), C.c_double(radius),
+                   *[fp(a) for a in uvw], *[fp(a) for a in valid])
+    dims = [(n + 1, n, n), (n, n + 1, n), (n, n, n + 1)]
+    for d, name in enumerate("uvw"):
+        out["p2g_" + name] = uvw[d].copy()
+        out["p2g_valid_" + name] = valid[d]
+        P.refprobe_extrapolate(*dims[d], fp(uvw[d]), fp(valid[d]), 12)
+        out["extrap_" + name] = uvw[d].copy()
+
+    wts = [np.zeros(fsize, np.float32) for _ in range(3)]
+    wc = np.zeros(N, np.float32)
+    P.refprobe_face_weights(n, n, n, C.c_double(dx), fp(solid), *[fp(a) for a in wts], fp(wc))
+    for d, name in enumerate("uvw"):
+        out["weight_" + name] = wts[d]
+
+    dt = 0.005
+    u, v, w = [a.copy() for a in uvw]
+    v -= np.float32(9.81 * dt)
+    out["dt"] = np.float64(dt)
+    for name, a in zip("uvw", (u, v, w)):
+        out["pre_" + name] = a.copy()
+    pres = np.zeros(N, np.float32)
+    it, err = C.c_int(), C.c_float()
+    vv = [np.zeros(fsize, np.uint8) for _ in range(3)]
+    tol = 1e-9
+    P.refprobe_pressure_solve(n, n, n, C.c_double(dx), C.c_double(dt), fp(u), fp(v), fp(w),
+                              *[fp(a) for a in wts], fp(wc), fp(sdf), C.c_double(tol), 900, fp(pres),
+                              C.byref(it), C.byref(err), *[fp(a) for a in vv])
+    out["tol"] = np.float64(tol)
+    out["pressure"] = pres
+    out["pressure_iterations"] = np.int32(it.value)
+    out["pressure_error"] = np.float32(err.value)
+    for d, (name, a) in enumerate(zip("uvw", (u, v, w))):
+        out["post_" + name] = a
+        out["post_valid_" + name] = vv[d]
+
+    q = (rng.random((2000, 3)) * (n * dx)).astype(np.float32)
+    s = np.zeros_like(q)
+    P.refprobe_sample_velocity(n, n, n, C.c_double(dx), fp(u), fp(v), fp(w), len(q), fp(q), fp(s))
+    out["sample_points"] = q
+    out["sample_velocity"] = s
+    np.savez_compressed(path, **out)
+    print("wrote", path, "pressure iterations", it.value)
+
+
+def make_e2e16(path, threads):
+    n = 16
+    dx = 1.0 / n
+    pos = dam_break(n, seed=0)
+    sim = oracle_lib.ReferenceSim((n, n, n), dx, threads=threads)
+    sim.load(pos)
+    sim.initialize()
+    out = {"n": np.int32(n), "dx": np.float64(dx), "pos0": pos, "frame_dt": np.float64(1.0 / 30.0)}
+    for f in range(1, 11):
+        sim.update(1.0 / 30.0)
+        if f in (1, 3, 10):
+            p, v = sim.particles()
+            out["pos_f%d" % f] = p
+            out["vel_f%d" % f] = v
+    np.savez_compressed(path, **out)
+    print("wrote", path)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def make_c1(path, threads, frames=60):
+    n = 64
+    dx = 1.0 / n
+    pos = dam_break(n, seed=0)
+    sim = oracle_lib.ReferenceSim((n, n, n), dx, threads=threads)
+    sim.load(pos)
+    sim.initialize()
+    rec = {"scene": "dam break 64^3, cells [0,24)x[0,40)x[0,64), 8 particles per cell, seed 0, "
+                    "g=(0,-9.81,0), PIC/FLIP 0.05, CFL 5, frames of 1/30 s",
+           "initial_positions_sha256": sha(pos), "initial_count": int(len(pos)), "frames": []}
+    for f in range(1, frames + 1):
+        t0 = time.time()
+        sim.update(1.0 / 30.0)
+        p, v = sim.particles()
+        rec["frames"].append({"frame": f, "count": int(len(p)), "positions_sha256": sha(p),
+                              "velocities_sha256": sha(v)})
+        print("frame %d count %d (%.1fs)" % (f, len(p), time.time() - t0), flush=True)
+    with open(path, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    print("wrote", path)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--skip-c1", action="store_true")
+    args = ap.parse_args()
+    if not oracle_lib.reference_available():
+        sys.exit("the reference build is missing: run `make -C oracle ref` first")
+    make_stages(os.path.join(HERE, "stages_n16.npz"))
+    make_e2e16(os.path.join(HERE, "e2e_n16.npz"), args.threads)
+    if not args.skip_c1:
+        make_c1(os.path.join(HERE, "e2e_c1_n64.json"), args.threads)
+
+
+if __name__ == "__main__":
+    main()
