@@ -486,6 +486,49 @@ int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u,
     });
 }
 
+int FFAMD_mic0_apply(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
+                     const float *w, const float *wu, const float *wv, const float *ww, const float *phi,
+                     int precision, int schedule, int n, const double *r, double *z, int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (precision != 0 && precision != 1) throw std::domain_error("precision must be 0 (FP64) or 1 (FP32)");
+        if (schedule != 0 && schedule != 1) throw std::domain_error("schedule must be 0 (hyperplanes) or 1 (tiles)");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t N = (size_t)ni * nj * nk;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> du(nu), dv(nv), dw(nw), a(nu), b(nv), c(nw), dphi(N), dp(N);
+        DBuf<int> scan(ff::scanScratchInts(N + 1));
+        up(du.p, u, nu, sc.s); up(dv.p, v, nv, sc.s); up(dw.p, w, nw, sc.s);
+        up(a.p, wu, nu, sc.s); up(b.p, wv, nv, sc.s); up(c.p, ww, nw, sc.s);
+        up(dphi.p, phi, N, sc.s);
+        ff::PressureSolver ps;
+        ps.allocate(g);
+        // one PCG iteration builds the system and the factor
+        ps.solve(dt, du, dv, dw, a, b, c, dphi, 1e-300, 1, precision, 1e-4, dp, scan, sc.s, nullptr);
+        size_t m = (size_t)ps.system().m;
+        if ((size_t)n != m)
+            throw std::domain_error("r and z hold " + std::to_string(n) + " values; the system has " +
+                                    std::to_string(m) + " rows");
+        DBuf<double> dr(m), dz(m);
+        up(dr.p, r, m, sc.s);
+        if (precision == 0) {
+            ps.applyPreconditioner(dr, dz, schedule == 1, sc.s);
+        } else {
+            DBuf<float> fr(m), fz(m);
+            ff::castToFloat(dr, fr, m, sc.s);
+            ps.applyPreconditionerF(fr, fz, schedule == 1, sc.s);
+            ff::hipCheck(hipMemsetAsync(dz, 0, sizeof(double) * m, sc.s), "memset");
+            ff::addFloatToDouble(dz, fz, m, sc.s);
+            sync(sc.s);
+        }
+        down(z, dz.p, m, sc.s);
+        sync(sc.s);
+        ps.checkError(sc.s);
+        return (int)m;
+    });
+}
+
 int FFAMD_pressure_system(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
                           const float *w, const float *wu, const float *wv, const float *ww, const float *phi,
                           int *rows, double *rhs, double *diag, double *ox, double *oy, double *oz, int *nbr,

@@ -444,15 +444,15 @@ void FluidSimulation::stepFluid(double dt) {
     hipCheck(hipEventRecord(_ev[7], _stream), "event");
     hipCheck(hipEventSynchronize(_ev[7]), "event");
     float ms[7];
-    hipEventElapsedTime(&ms[0], _ev[0], _ev[1]);
-    hipEventElapsedTime(&ms[1], _ev[1], _ev[2]);
-    hipEventElapsedTime(&ms[2], _ev[2], _ev[3]);
-    hipEventElapsedTime(&ms[3], _ev[3], _ev[4]);
-    hipEventElapsedTime(&ms[4], _ev[4], _ev[5]);
-    hipEventElapsedTime(&ms[5], _ev[5], _ev[6]);
-    hipEventElapsedTime(&ms[6], _ev[6], _ev[7]);
+    hipCheck(hipEventElapsedTime(&ms[0], _ev[0], _ev[1]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&ms[1], _ev[1], _ev[2]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&ms[2], _ev[2], _ev[3]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&ms[3], _ev[3], _ev[4]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&ms[4], _ev[4], _ev[5]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&ms[5], _ev[5], _ev[6]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&ms[6], _ev[6], _ev[7]), "hipEventElapsedTime");
     float whole = 0;
-    hipEventElapsedTime(&whole, _ev[0], _ev[7]);
+    hipCheck(hipEventElapsedTime(&whole, _ev[0], _ev[7]), "hipEventElapsedTime");
     for (int q = 0; q < 7; q++) _phaseMs[q] = ms[q];
     _phaseMs[7] = whole;
     _timers.collect();

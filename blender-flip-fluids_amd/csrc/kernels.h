@@ -10,6 +10,9 @@
 
 namespace ff {
 
+// throws std::runtime_error naming `what` when e is not hipSuccess
+void hipCheck(hipError_t e, const char *what);
+
 // ---- scan / reductions (kernels_scan.hip) ----
 // Exclusive prefix sum of n int32 values; out may alias in; out[n] = total.
 // scratch must hold scanScratchInts(n) ints.
@@ -87,10 +90,13 @@ struct PressureSys {
     int *mask;      // N+1: pressure-cell flag, then scanned into row ids
     int *rowOf;     // N+1 (exclusive scan of mask)
     int *rows;      // M: flat cell of each row (k,j,i order)
+    int *cellRow;   // N: row of each cell, or -1
     double *rhs, *diag, *ox, *oy, *oz, *mx, *my, *mz;  // M
     int *nbr;       // 6M: -x,+x,-y,+y,-z,+z row ids or -1
     double *invdiag, *lx, *ly, *lz;  // M: MIC(0) factor (L entries toward +x/+y/+z)
+    double *lmx, *lmy, *lmz;         // M: L entries of the -x/-y/-z neighbour rows toward the row
     float *finvdiag, *flx, *fly, *flz;  // FP32 copies for the mixed-precision inner solve
+    float *flmx, *flmy, *flmz;
     float *fdiag, *fox, *foy, *foz, *fmx, *fmy, *fmz;
     int *levelCount;  // L+1
     int *levelStart;  // L+1
@@ -100,12 +106,13 @@ struct PressureSys {
     int nLevels;
 };
 void pressureMask(Grid g, const float *phi, int *mask, hipStream_t s);
-void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, hipStream_t s);
+void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, int *cellRow, hipStream_t s);
 void pressureAssemble(Grid g, double dt, const float *u, const float *v, const float *w,
                       const float *wu, const float *wv, const float *ww, const float *phi,
                       PressureSys &S, hipStream_t s);
 void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s);
-// MIC(0) factor; level-by-level launches with host level table
+// MIC(0) factor and application z = (L L^T)^-1 r; hyperplane-by-hyperplane
+// launches driven by the host copy of the level table
 void mic0Factor(PressureSys &S, const int *hLevelStart, hipStream_t s);
 void mic0Apply(const PressureSys &S, const int *hLevelStart, const double *r, double *z,
                hipStream_t s);
@@ -124,5 +131,31 @@ void pressureToGrid(Grid g, const PressureSys &S, const double *x, float *p, hip
 void castToFloat(const double *a, float *b, size_t n, hipStream_t s);
 void addFloatToDouble(double *x, const float *d, size_t n, hipStream_t s);
 void mic0ToFloat(PressureSys &S, hipStream_t s);
+
+// ---- MIC(0) application, persistent tile wavefront (kernels_mic.hip) ----
+// Tiles are 16 x 16 x depth cells; the schedule lists the non-empty tiles by
+// tile level i/16 + j/16 + k/depth.  base and epoch advance with every launch.
+struct MicSchedule {
+    Grid g{};
+    int depth = 0;               // cells per tile along k
+    int ti = 0, tj = 0, tk = 0;  // tiles per axis
+    int nTiles = 0, nActive = 0;
+    int *order = nullptr;        // nActive tile ids (device)
+    int *tileRows = nullptr;     // nTiles: non-zero when the tile holds rows (device)
+    int *flags = nullptr;        // nTiles (device)
+    int *counter = nullptr;      // ticket counter (device)
+    int *error = nullptr;        // set when a wait exceeded spinLimit (device)
+    int base = 0, epoch = 0;
+    int grid = 0;                // persistent workgroups per launch
+    long long waitTicks = 0;     // longest tile wait in wall-clock ticks (set by the owner)
+};
+int micTileDepth(bool fp64);
+size_t micTileLds(bool fp64);  // bytes of LDS per workgroup
+void micTileCounts(Grid g, int depth, int &ti, int &tj, int &tk);
+void micMarkTiles(Grid g, const PressureSys &S, int depth, int *tileRows, hipStream_t s);
+// L entries of the lower neighbours (lmx.. from lx..), FP64 or the FP32 copies
+void micLowerEntries(const PressureSys &S, bool fp64, hipStream_t s);
+void micApplyTiles(const PressureSys &S, MicSchedule &ms, const double *r, double *z, hipStream_t s);
+void micApplyTilesF(const PressureSys &S, MicSchedule &ms, const float *r, float *z, hipStream_t s);
 
 }  // namespace ff

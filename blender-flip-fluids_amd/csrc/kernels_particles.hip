@@ -537,7 +537,7 @@ __global__ void compactKernel(Particles src, Particles dst, const int *keep, con
 
 void cellSort(Grid g, const Particles &p, CellSort &cs, int *scanScratch, hipStream_t s) {
     size_t N = (size_t)g.ni * g.nj * g.nk;
-    hipMemsetAsync(cs.counts, 0, sizeof(int) * (N + 1), s);
+    hipCheck(hipMemsetAsync(cs.counts, 0, sizeof(int) * (N + 1), s), "hipMemsetAsync");
     if (p.n > 0)
         hipLaunchKernelGGL(cellKeysKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, g, p.px, p.py,
                            p.pz, p.n, cs.keys, cs.slot, cs.counts);
@@ -594,7 +594,7 @@ void g2pAdvect(Grid g, Particles p, MacView cur, MacView saved, const float *sol
 
 void speedHistogram(const Particles &p, double stepv, int nbins, int *counts, double *partial,
                     double *maxSpeed, hipStream_t s) {
-    hipMemsetAsync(counts, 0, sizeof(int) * nbins, s);
+    hipCheck(hipMemsetAsync(counts, 0, sizeof(int) * nbins, s), "hipMemsetAsync");
     int nb = (int)blocksFor(p.n > 0 ? p.n : 1, TPB * 8);
     if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(speedHistKernel, dim3(nb), dim3(256), 0, s, p, stepv, nbins, counts, partial);
@@ -602,7 +602,7 @@ void speedHistogram(const Particles &p, double stepv, int nbins, int *counts, do
 }
 
 void speedOutliers(const Particles &p, double lower, double thr, int *counts2, hipStream_t s) {
-    hipMemsetAsync(counts2, 0, sizeof(int) * 2, s);
+    hipCheck(hipMemsetAsync(counts2, 0, sizeof(int) * 2, s), "hipMemsetAsync");
     if (p.n > 0)
         hipLaunchKernelGGL(speedOutlierKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, p, lower, thr,
                            counts2);
@@ -613,13 +613,13 @@ void removalFlags(Grid g, const Particles &p, const float *sp, int *keys, int *c
                   double maxSpeedSq, int *keep, int *maxCount, hipStream_t s) {
     if (p.n == 0) return;
     size_t N = (size_t)g.ni * g.nj * g.nk;
-    hipMemsetAsync(counts, 0, sizeof(int) * (N + 1), s);
-    hipMemsetAsync(maxCount, 0, sizeof(int), s);
+    hipCheck(hipMemsetAsync(counts, 0, sizeof(int) * (N + 1), s), "hipMemsetAsync");
+    hipCheck(hipMemsetAsync(maxCount, 0, sizeof(int), s), "hipMemsetAsync");
     hipLaunchKernelGGL(removalKeysKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, g, p, sp, keys,
                        slot, counts, maxCount);
     int hmax = 0;
-    hipMemcpyAsync(&hmax, maxCount, sizeof(int), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    hipCheck(hipMemcpyAsync(&hmax, maxCount, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
     const int *rank = nullptr;
     if (hmax > maxPerCell) {
         // exact cap: stable sort of the survivors by cell, rank inside the cell
@@ -628,8 +628,8 @@ void removalFlags(Grid g, const Particles &p, const float *sp, int *keys, int *c
                            p.n, perm);
         hipLaunchKernelGGL(segmentSortKernel, dim3(blocksFor(N, TPB)), dim3(TPB), 0, s, offsets, N, perm);
         int ns = 0;
-        hipMemcpyAsync(&ns, offsets + N, sizeof(int), hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
+        hipCheck(hipMemcpyAsync(&ns, offsets + N, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
         if (ns > 0)
             hipLaunchKernelGGL(rankInCellKernel, dim3(blocksFor(ns, TPB)), dim3(TPB), 0, s, perm, offsets,
                                keys, ns, slot);
@@ -645,8 +645,8 @@ void compactParticles(Particles &src, Particles &dst, const int *keep, int *rank
     if (src.n > 0)
         hipLaunchKernelGGL(compactKernel, dim3(blocksFor(src.n, TPB)), dim3(TPB), 0, s, src, dst, keep, rank);
     int total = 0;
-    hipMemcpyAsync(&total, rank + src.n, sizeof(int), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    hipCheck(hipMemcpyAsync(&total, rank + src.n, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
     dst.n = total;
 }
 

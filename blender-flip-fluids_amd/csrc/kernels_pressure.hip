@@ -1,13 +1,14 @@
 // kernels_pressure.hip -- the variational pressure solve of the FLIP substep:
 // pressure-cell rows, divergence and 7-point ghost-fluid assembly, the MIC(0)
-// factor, its level-scheduled triangular solves, SpMV and the PCG vector ops.
+// factor, its hyperplane-scheduled triangular solves, SpMV and the PCG vector
+// ops.
 //
 // Rows keep the reference's order (interior liquid cells in k,j,i order,
 // pressuresolver.cpp:118-159).  The factor and both triangular sweeps depend
 // only on the -x/-y/-z (or +x/+y/+z) neighbours, which sit on the previous
-// (next) i+j+k hyperplane, so running the rows level by level performs the
-// reference's own serial recurrences (pcgsolver.h:68-241) with identical
-// operations per row.
+// (next) i+j+k hyperplane, so running the rows hyperplane by hyperplane
+// performs the reference's own serial recurrences (pcgsolver.h:68-241) with
+// identical operations per row.
 #include "kernels.h"
 
 namespace ff {
@@ -30,9 +31,12 @@ __global__ void maskKernel(Grid g, const float *phi, int *mask) {
     mask[c] = (interior && phi[c] < 0) ? 1 : 0;
 }
 
-__global__ void rowsKernel(size_t n, const int *mask, const int *rowOf, int *rows) {
+__global__ void rowsKernel(size_t n, const int *mask, const int *rowOf, int *rows, int *cellRow) {
     size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n && mask[c]) rows[rowOf[c]] = (int)c;
+    if (c >= n) return;
+    int row = mask[c] ? rowOf[c] : -1;
+    cellRow[c] = row;
+    if (row >= 0) rows[row] = (int)c;
 }
 
 // _calculateNegativeDivergenceVectorThread (pressuresolver.cpp:624-713) and
@@ -277,9 +281,9 @@ void pressureMask(Grid g, const float *phi, int *mask, hipStream_t s) {
     hipLaunchKernelGGL(maskKernel, dim3(blocksFor(n + 1, TPB)), dim3(TPB), 0, s, g, phi, mask);
 }
 
-void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, hipStream_t s) {
+void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, int *cellRow, hipStream_t s) {
     size_t n = (size_t)g.ni * g.nj * g.nk;
-    hipLaunchKernelGGL(rowsKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, n, mask, rowOf, rows);
+    hipLaunchKernelGGL(rowsKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, n, mask, rowOf, rows, cellRow);
 }
 
 void pressureAssemble(Grid g, double dt, const float *u, const float *v, const float *w,
@@ -292,8 +296,8 @@ void pressureAssemble(Grid g, double dt, const float *u, const float *v, const f
 
 void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s) {
     S.nLevels = g.ni + g.nj + g.nk - 2;
-    hipMemsetAsync(S.levelCount, 0, sizeof(int) * (S.nLevels + 1), s);
-    hipMemsetAsync(S.levelFill, 0, sizeof(int) * S.nLevels, s);
+    hipCheck(hipMemsetAsync(S.levelCount, 0, sizeof(int) * (S.nLevels + 1), s), "hipMemsetAsync");
+    hipCheck(hipMemsetAsync(S.levelFill, 0, sizeof(int) * S.nLevels, s), "hipMemsetAsync");
     if (S.m > 0)
         hipLaunchKernelGGL(levelCountKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, g, S);
     exclusiveScan(S.levelCount, S.levelStart, S.nLevels, scanScratch, s);
@@ -365,7 +369,7 @@ void residual(const PressureSys &S, const double *b, const double *x, double *r,
 
 void pressureToGrid(Grid g, const PressureSys &S, const double *x, float *p, hipStream_t s) {
     size_t n = (size_t)g.ni * g.nj * g.nk;
-    hipMemsetAsync(p, 0, sizeof(float) * n, s);
+    hipCheck(hipMemsetAsync(p, 0, sizeof(float) * n, s), "hipMemsetAsync");
     if (S.m > 0) hipLaunchKernelGGL(toGridKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.rows, x, p);
 }
 

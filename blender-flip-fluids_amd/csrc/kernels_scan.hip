@@ -75,7 +75,7 @@ size_t scanScratchInts(size_t n) {
 // out[0..n) = exclusive scan of in, out[n] = total.  Recursive over tile sums.
 void exclusiveScan(const int *in, int *out, size_t n, int *scratch, hipStream_t s) {
     if (n == 0) {
-        hipMemsetAsync(out, 0, sizeof(int), s);
+        hipCheck(hipMemsetAsync(out, 0, sizeof(int), s), "hipMemsetAsync");
         return;
     }
     size_t nb = tiles(n);
@@ -87,7 +87,7 @@ void exclusiveScan(const int *in, int *out, size_t n, int *scratch, hipStream_t 
         hipLaunchKernelGGL(scanAdd, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, out, n, offs);
         hipLaunchKernelGGL(writeTotal, dim3(1), dim3(1), 0, s, out, n, sums, offs, (int)nb);
     } else {
-        hipMemsetAsync(offs, 0, sizeof(int), s);
+        hipCheck(hipMemsetAsync(offs, 0, sizeof(int), s), "hipMemsetAsync");
         hipLaunchKernelGGL(writeTotal, dim3(1), dim3(1), 0, s, out, n, sums, offs, 1);
     }
 }

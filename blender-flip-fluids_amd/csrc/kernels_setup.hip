@@ -303,12 +303,12 @@ void faceWeights(Grid g, const float *sp, float *wu, float *wv, float *ww, hipSt
 void nearSolidGrid(Grid g, const float *sp, uint8_t *near, uint8_t *tmp, int ci, int cj, int ck,
                    int layers, hipStream_t s) {
     size_t nc = (size_t)ci * cj * ck;
-    hipMemsetAsync(near, 0, nc, s);
+    hipCheck(hipMemsetAsync(near, 0, nc, s), "hipMemsetAsync");
     size_t n = (size_t)g.ni * g.nj * g.nk;
     hipLaunchKernelGGL(nearSolidMark, dim3(blocksFor(n, 256)), dim3(256), 0, s, g, sp, near, ci, cj);
     for (int l = 0; l < layers; l++) {
         hipLaunchKernelGGL(feather6Kernel, dim3(blocksFor(nc, 256)), dim3(256), 0, s, near, tmp, ci, cj, ck);
-        hipMemcpyAsync(near, tmp, nc, hipMemcpyDeviceToDevice, s);
+        hipCheck(hipMemcpyAsync(near, tmp, nc, hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
     }
 }
 

@@ -36,6 +36,13 @@ public:
                          hipEvent_t afterAssembly, KernelTimers *timers = nullptr);
     const PressureSys &system() const { return _S; }
     const std::vector<int> &levelTable() const { return _hLevelStart; }
+    // one preconditioner application z = (L L^T)^-1 r on the current factor
+    // (after solve()); FP64 or the FP32 copy.  tiles selects the tile-wavefront
+    // kernel, which is switched off and raises an error.
+    void applyPreconditioner(const double *r, double *z, bool tiles, hipStream_t s);
+    void applyPreconditionerF(const float *r, float *z, bool tiles, hipStream_t s);
+    // throws when a preconditioner launch gave up waiting (waits for the stream)
+    void checkError(hipStream_t s);
 
 private:
     Grid _g{};
@@ -45,11 +52,16 @@ private:
     float *_fr = nullptr, *_fz = nullptr, *_fs = nullptr, *_fd = nullptr, *_fq = nullptr;
     double *_partial = nullptr, *_scal = nullptr;
     std::vector<int> _hLevelStart;
+    MicSchedule _ms;
+    int _maxTiles = 0;
+    std::vector<int> _hTileRows, _hOrder;
     KernelTimers *_t = nullptr;
     KernelTimers _noTimers;
     int _levelLaunches = 0;
 
     double hostScalar(const double *d, hipStream_t s);
+    // tile schedule of the preconditioner application for this system
+    void buildSchedule(bool fp64, hipStream_t s);
     bool pcgFP64(double tolFactor, int maxIter, double resid0, int &iters, double &resid, hipStream_t s);
     bool pcgMixed(double tolFactor, int maxIter, double resid0, double innerTol, int &iters,
                   double &resid, hipStream_t s);

@@ -28,8 +28,16 @@ void PressureSolver::allocate(Grid g) {
         return off;
     };
     size_t oMask = take(4 * (N + 1)), oRowOf = take(4 * (N + 1)), oRows = take(4 * M), oNbr = take(24 * M);
+    size_t oCellRow = take(4 * N);
+    {
+        int ti, tj, tk;
+        micTileCounts(g, std::min(micTileDepth(true), micTileDepth(false)), ti, tj, tk);
+        _maxTiles = ti * tj * tk;
+    }
+    size_t oOrder = take(4 * (size_t)_maxTiles), oTileRows = take(4 * (size_t)_maxTiles);
+    size_t oFlags = take(4 * (size_t)_maxTiles), oCounter = take(4), oError = take(4);
     size_t oLc = take(4 * (L + 1)), oLs = take(4 * (L + 1)), oLf = take(4 * L), oLr = take(4 * M);
-    size_t oD[12], oF[11];
+    size_t oD[15], oF[14];
     for (auto &o : oD) o = take(8 * M);
     for (auto &o : oF) o = take(4 * M);
     size_t oV[4], oFV[5];
@@ -46,16 +54,24 @@ void PressureSolver::allocate(Grid g) {
     _S.levelStart = (int *)(b + oLs);
     _S.levelFill = (int *)(b + oLf);
     _S.levelRows = (int *)(b + oLr);
-    double *d[12];
-    for (int q = 0; q < 12; q++) d[q] = (double *)(b + oD[q]);
+    _S.cellRow = (int *)(b + oCellRow);
+    _ms.order = (int *)(b + oOrder);
+    _ms.tileRows = (int *)(b + oTileRows);
+    _ms.flags = (int *)(b + oFlags);
+    _ms.counter = (int *)(b + oCounter);
+    _ms.error = (int *)(b + oError);
+    double *d[15];
+    for (int q = 0; q < 15; q++) d[q] = (double *)(b + oD[q]);
     _S.rhs = d[0]; _S.diag = d[1]; _S.ox = d[2]; _S.oy = d[3]; _S.oz = d[4];
     _S.mx = d[5]; _S.my = d[6]; _S.mz = d[7]; _S.invdiag = d[8];
     _S.lx = d[9]; _S.ly = d[10]; _S.lz = d[11];
-    float *f[11];
-    for (int q = 0; q < 11; q++) f[q] = (float *)(b + oF[q]);
+    _S.lmx = d[12]; _S.lmy = d[13]; _S.lmz = d[14];
+    float *f[14];
+    for (int q = 0; q < 14; q++) f[q] = (float *)(b + oF[q]);
     _S.finvdiag = f[0]; _S.flx = f[1]; _S.fly = f[2]; _S.flz = f[3];
     _S.fdiag = f[4]; _S.fox = f[5]; _S.foy = f[6]; _S.foz = f[7];
     _S.fmx = f[8]; _S.fmy = f[9]; _S.fmz = f[10];
+    _S.flmx = f[11]; _S.flmy = f[12]; _S.flmz = f[13];
     _x = (double *)(b + oV[0]); _r = (double *)(b + oV[1]); _z = (double *)(b + oV[2]); _s = (double *)(b + oV[3]);
     _fr = (float *)(b + oFV[0]); _fz = (float *)(b + oFV[1]); _fs = (float *)(b + oFV[2]);
     _fd = (float *)(b + oFV[3]); _fq = (float *)(b + oFV[4]);
@@ -63,12 +79,84 @@ void PressureSolver::allocate(Grid g) {
     _scal = (double *)(b + oScal);
     _S.m = 0;
     _S.nLevels = g.ni + g.nj + g.nk - 2;
+    int dev = 0, cus = 0;
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    hipCheck(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+    _ms.grid = std::max(cus, 1);
+    int wallKHz = 0;
+    hipCheck(hipDeviceGetAttribute(&wallKHz, hipDeviceAttributeWallClockRate, dev), "device attribute");
+    _ms.waitTicks = 2000LL * std::max(wallKHz, 1);  // 2 s
+    int lds = 0;
+    hipCheck(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "device attribute");
+    if ((size_t)lds < micTileLds(true))
+        throw std::runtime_error("the MIC(0) tile kernel needs " + std::to_string(micTileLds(true)) +
+                                 " bytes of LDS per workgroup; the device offers " + std::to_string(lds));
+    _ms.g = g;
 }
 
+// Non-empty tiles of the preconditioner's tile wavefront, listed by tile
+// level (any order inside a level), with the flags, ticket counter and error
+// word cleared for a new system.
+void PressureSolver::buildSchedule(bool fp64, hipStream_t s) {
+    _ms.depth = micTileDepth(fp64);
+    micTileCounts(_g, _ms.depth, _ms.ti, _ms.tj, _ms.tk);
+    _ms.nTiles = _ms.ti * _ms.tj * _ms.tk;
+    micMarkTiles(_g, _S, _ms.depth, _ms.tileRows, s);
+    _hTileRows.resize(_ms.nTiles);
+    hipCheck(hipMemcpyAsync(_hTileRows.data(), _ms.tileRows, sizeof(int) * _ms.nTiles, hipMemcpyDeviceToHost, s),
+             "D2H tiles");
+    hipCheck(hipStreamSynchronize(s), "sync");
+    const int nl = _ms.ti + _ms.tj + _ms.tk;
+    std::vector<int> count(nl + 1, 0);
+    for (int t = 0; t < _ms.nTiles; t++)
+        if (_hTileRows[t]) count[t % _ms.ti + (t / _ms.ti) % _ms.tj + t / (_ms.ti * _ms.tj) + 1]++;
+    for (int l = 0; l < nl; l++) count[l + 1] += count[l];
+    _hOrder.assign(count[nl], 0);
+    for (int t = 0; t < _ms.nTiles; t++)
+        if (_hTileRows[t]) _hOrder[count[t % _ms.ti + (t / _ms.ti) % _ms.tj + t / (_ms.ti * _ms.tj)]++] = t;
+    _ms.nActive = (int)_hOrder.size();
+    if (_ms.nActive > 0)
+        hipCheck(hipMemcpyAsync(_ms.order, _hOrder.data(), sizeof(int) * _ms.nActive, hipMemcpyHostToDevice, s),
+                 "H2D tile order");
+    hipCheck(hipMemsetAsync(_ms.flags, 0, sizeof(int) * _ms.nTiles, s), "memset flags");
+    hipCheck(hipMemsetAsync(_ms.counter, 0, sizeof(int), s), "memset counter");
+    _ms.base = 0;
+    _ms.epoch = 0;
+}
+
+void PressureSolver::checkError(hipStream_t s) {
+    int e = 0;
+    hipCheck(hipMemcpyAsync(&e, _ms.error, sizeof(int), hipMemcpyDeviceToHost, s), "D2H error word");
+    hipCheck(hipStreamSynchronize(s), "sync");
+    if (e) throw std::runtime_error("MIC(0) preconditioner: a tile wait exceeded its time limit");
+}
+
+void PressureSolver::applyPreconditioner(const double *r, double *z, bool tiles, hipStream_t s) {
+    if (tiles) {
+        buildSchedule(true, s);
+        micApplyTiles(_S, _ms, r, z, s);
+    } else {
+        mic0Apply(_S, _hLevelStart.data(), r, z, s);
+    }
+}
+void PressureSolver::applyPreconditionerF(const float *r, float *z, bool tiles, hipStream_t s) {
+    if (tiles) {
+        buildSchedule(false, s);
+        micApplyTilesF(_S, _ms, r, z, s);
+    } else {
+        mic0ApplyF(_S, _hLevelStart.data(), r, z, s);
+    }
+}
+
+// Reads one device scalar; also fails as soon as a preconditioner launch has
+// raised the error word.
 double PressureSolver::hostScalar(const double *d, hipStream_t s) {
     double h = 0.0;
+    int e = 0;
     hipCheck(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s), "D2H scalar");
+    hipCheck(hipMemcpyAsync(&e, _ms.error, sizeof(int), hipMemcpyDeviceToHost, s), "D2H error word");
     hipCheck(hipStreamSynchronize(s), "sync");
+    if (e) throw std::runtime_error("MIC(0) preconditioner: a tile wait exceeded its time limit");
     return h;
 }
 
@@ -80,6 +168,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
                                      KernelTimers *timers) {
     PressureResult res;
     _t = timers ? timers : &_noTimers;
+    hipCheck(hipMemsetAsync(_ms.error, 0, sizeof(int), s), "memset error word");
     _t->begin(T_ASSEMBLE, s);
     const size_t N = (size_t)_g.ni * _g.nj * _g.nk;
     pressureMask(_g, phi, _S.mask, s);
@@ -89,7 +178,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     hipCheck(hipStreamSynchronize(s), "sync");
     _S.m = m;
     res.unknowns = m;
-    pressureRows(_g, _S.mask, _S.rowOf, _S.rows, s);
+    pressureRows(_g, _S.mask, _S.rowOf, _S.rows, _S.cellRow, s);
     pressureAssemble(_g, dt, u, v, w, wu, wv, ww, phi, _S, s);
     pressureLevels(_g, _S, scanScratch, s);
     _t->end(T_ASSEMBLE, s);
@@ -117,6 +206,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     double resid = -1.0;
     bool conv = mode == 0 ? pcgFP64(tol, maxIter, maxAbs, iters, resid, s)
                           : pcgMixed(tol, maxIter, maxAbs, innerTol, iters, resid, s);
+    checkError(s);
     pressureToGrid(_g, _S, _x, pressure, s);
     float e = (float)resid;
     res.success = conv || (iters == maxIter && e < 1.0);
@@ -136,8 +226,9 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
     resid = resid0;
     double tol = tolFactor * resid;
-    _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 1);
+    _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
     mic0Factor(_S, hs, st);
+    micLowerEntries(_S, true, st);
     _t->end(T_MIC0_FACTOR, st);
     _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
     mic0Apply(_S, hs, _r, _z, st);
@@ -191,10 +282,11 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
     double tol = std::min(tolFactor * resid0, 1.0);
     hipCheck(hipMemcpyAsync(_r, _S.rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
     hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
-    _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 1);
+    _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
     mic0Factor(_S, hs, st);
     _t->end(T_MIC0_FACTOR, st);
     mic0ToFloat(_S, st);
+    micLowerEntries(_S, false, st);
     double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
     resid = resid0;
     iters = 0;

@@ -123,6 +123,22 @@ def pressure_system(n3, dx, dt, uvw, weights, liquid_phi):
             "nbr": nbr[:6 * m].reshape(m, 6), "invdiag": inv[:m]}
 
 
+def mic0_apply(n3, dx, dt, uvw, weights, liquid_phi, r, precision=0, schedule=0):
+    """One MIC(0) application z = (L L^T)^-1 r on the pressure system of these
+    inputs (row order as pressure_system); precision 0 = FP64, 1 = FP32;
+    schedule 0 = hyperplane launches, 1 = tile wavefront (switched off)."""
+    ni, nj, nk = n3
+    u, v, w = (_f32(a) for a in uvw)
+    wu, wv, ww = (_f32(a) for a in weights)
+    phi = _f32(liquid_phi)
+    r = np.ascontiguousarray(r, np.float64)
+    z = np.zeros_like(r)
+    _run("FFAMD_mic0_apply", [c_int] * 3 + [c_double, c_double] + [c_void_p] * 7 + [c_int] * 3 + [c_void_p] * 2,
+         [ni, nj, nk, dx, dt, _fp(u), _fp(v), _fp(w), _fp(wu), _fp(wv), _fp(ww), _fp(phi), precision, schedule,
+          len(r), _fp(r), _fp(z)], c_int)
+    return z
+
+
 def sample_velocity(n3, dx, uvw, pos):
     ni, nj, nk = n3
     u, v, w = (_f32(a) for a in uvw)

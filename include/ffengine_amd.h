@@ -217,6 +217,18 @@ int FFAMD_pressure_system(int isize, int jsize, int ksize, double dx, double dt,
                           const float *wv, const float *ww, const float *liquid_phi, int *rows,
                           double *rhs, double *diag, double *ox, double *oy, double *oz,
                           int *nbr, double *invdiag, int *err);
+/* One MIC(0) preconditioner application z = (L L^T)^-1 r on the system and
+ * factor FFAMD_pressure_system builds from the same inputs
+ * (solveLower + solveLowerTransposeInPlace, pcgsolver.h:211-241).  r and z
+ * hold n values in row order and n must equal M (else an error); precision
+ * 0 = FP64, 1 = the FP32 copy of the factor used by the mixed-precision solve
+ * (r rounded to float, z widened).  schedule 0 = hyperplane launches (the
+ * solver's), 1 = the tile-wavefront kernel, which is switched off and fails
+ * with an error.  The inputs must give a non-zero divergence.  Returns M. */
+int FFAMD_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, const float *u,
+                     const float *v, const float *w, const float *wu, const float *wv,
+                     const float *ww, const float *liquid_phi, int precision, int schedule,
+                     int n, const double *r, double *z, int *err);
 void FFAMD_sample_velocity(int isize, int jsize, int ksize, double dx, const float *u,
                            const float *v, const float *w, int np, const float *pos, float *out,
                            int *err);
