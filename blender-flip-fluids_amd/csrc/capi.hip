@@ -471,7 +471,9 @@ int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u,
         up(dphi.p, phi, N, sc.s);
         ff::PressureSolver ps;
         ps.allocate(g);
-        ff::PressureResult r = ps.solve(dt, du, dv, dw, a, b, c, dphi, tol, maxIter, mode, 1e-4, dp, scan, sc.s, nullptr);
+        if (mode < 0 || mode > 2) throw std::domain_error("mode must be 0, 1 or 2");
+        ff::PressureResult r = ps.solve(dt, du, dv, dw, a, b, c, dphi, tol, maxIter, mode, mode == 2 ? 0.1 : 1e-4,
+                                        dp, scan, sc.s, nullptr);
         ff::hipCheck(hipMemsetAsync(bu, 0, nu, sc.s), "memset");
         ff::hipCheck(hipMemsetAsync(bv, 0, nv, sc.s), "memset");
         ff::hipCheck(hipMemsetAsync(bw, 0, nw, sc.s), "memset");
@@ -492,7 +494,8 @@ int FFAMD_mic0_apply(int ni, int nj, int nk, double dx, double dt, const float *
     return guarded(err, [&] {
         checkGrid(ni, nj, nk, dx);
         if (precision != 0 && precision != 1) throw std::domain_error("precision must be 0 (FP64) or 1 (FP32)");
-        if (schedule != 0 && schedule != 1) throw std::domain_error("schedule must be 0 (hyperplanes) or 1 (tiles)");
+        if (schedule < 0 || schedule > 2)
+            throw std::domain_error("schedule must be 0 (hyperplanes), 1 (tiles) or 2 (bands)");
         ff::Grid g{ni, nj, nk, dx};
         Scratch sc;
         size_t N = (size_t)ni * nj * nk;
@@ -513,11 +516,11 @@ int FFAMD_mic0_apply(int ni, int nj, int nk, double dx, double dt, const float *
         DBuf<double> dr(m), dz(m);
         up(dr.p, r, m, sc.s);
         if (precision == 0) {
-            ps.applyPreconditioner(dr, dz, schedule == 1, sc.s);
+            ps.applyPreconditioner(dr, dz, schedule, sc.s);
         } else {
             DBuf<float> fr(m), fz(m);
             ff::castToFloat(dr, fr, m, sc.s);
-            ps.applyPreconditionerF(fr, fz, schedule == 1, sc.s);
+            ps.applyPreconditionerF(fr, fz, schedule, sc.s);
             ff::hipCheck(hipMemsetAsync(dz, 0, sizeof(double) * m, sc.s), "memset");
             ff::addFloatToDouble(dz, fz, m, sc.s);
             sync(sc.s);

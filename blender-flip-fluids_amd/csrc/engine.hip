@@ -116,12 +116,13 @@ void FluidSimulation::setPressureSolverTolerance(double tol) {
     _pressureTolerance = tol;
 }
 void FluidSimulation::setPressureSolverPrecision(int mode, double innerTol) {
-    if (mode != 0 && mode != 1)
-        throw std::domain_error("Error: pressure precision mode must be 0 (FP64) or 1 (mixed).\n");
-    if (mode == 1 && !(innerTol > 0.0 && innerTol < 1.0))
+    if (mode < 0 || mode > 2)
+        throw std::domain_error("Error: pressure precision mode must be 0 (FP64), 1 (mixed, restarted "
+                                "refinement) or 2 (mixed, reliable updates).\n");
+    if (mode != 0 && !(innerTol > 0.0 && innerTol < 1.0))
         throw std::domain_error("Error: inner tolerance must be in (0, 1).\n");
     _precision = (PressurePrecision)mode;
-    if (mode == 1) _innerTolerance = innerTol;
+    if (mode != 0) _innerTolerance = innerTol;
 }
 void FluidSimulation::setSolidPhi(const float *phi) {
     if (_initialized)

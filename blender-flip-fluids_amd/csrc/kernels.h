@@ -25,6 +25,11 @@ void dot(const double *a, const double *b, size_t n, double *partial, double *ou
 void dotf(const float *a, const float *b, size_t n, double *partial, double *out, hipStream_t s);
 void absMax(const double *a, size_t n, double *partial, double *out, hipStream_t s);
 void absMaxf(const float *a, size_t n, double *partial, double *out, hipStream_t s);
+// The reductions above run reduceBlocks(n) blocks of REDUCE_THREADS threads
+// over a grid-stride loop, then one block folds the partials in a fixed tree;
+// fused kernels that use the same grid produce the same bits.
+constexpr int REDUCE_THREADS = 256;
+void reduceFinish(const double *partial, int nb, double *out, bool isMax, hipStream_t s);
 
 // ---- setup (kernels_setup.hip) ----
 void solidBoxPhi(Grid g, float *phiNodes, hipStream_t s);
@@ -118,6 +123,16 @@ void mic0Apply(const PressureSys &S, const int *hLevelStart, const double *r, do
                hipStream_t s);
 void mic0ApplyF(const PressureSys &S, const int *hLevelStart, const float *r, float *z,
                 hipStream_t s);
+
+// ---- MIC(0) application in bands of hyperplanes (kernels_band.hip) ----
+// L entries of the lower neighbour rows (lmx.. from lx..; FP64 or the FP32 copies)
+void mic0LowerEntries(const PressureSys &S, bool fp64, hipStream_t s);
+// z = (L L^T)^-1 r with y as the forward result (M values, distinct from r and z)
+void mic0ApplyBands(Grid g, const PressureSys &S, const int *hLevelStart, const double *r, double *y,
+                    double *z, hipStream_t s);
+void mic0ApplyBandsF(Grid g, const PressureSys &S, const int *hLevelStart, const float *r, float *y,
+                     float *z, hipStream_t s);
+int mic0BandLaunches(const PressureSys &S, const int *hLevelStart, bool fp64);
 void spmv(const PressureSys &S, const double *x, double *y, hipStream_t s);
 void spmvF(const PressureSys &S, const float *x, float *y, hipStream_t s);
 // y += (float)(sign * num/den) * x   (device scalars; den may be null -> scalar = num)
@@ -125,6 +140,19 @@ void axpyFloatStep(double *y, const double *x, size_t n, const double *num, cons
                    double sign, hipStream_t s);
 void axpyFloatStepF(float *y, const float *x, size_t n, const double *num, const double *den,
                     double sign, hipStream_t s);
+// q = A s and *sq = s . q (the dot in the dot()/dotf() order)
+void spmvDot(const PressureSys &S, const double *s, double *q, double *partial, double *sq, hipStream_t st);
+void spmvDotF(const PressureSys &S, const float *s, float *q, double *partial, double *sq, hipStream_t st);
+// a = (float)(num/den); x += a * s; r -= a * q; *rmax = |r|_inf (the absMax order)
+void pcgStep(double *x, const double *s, double *r, const double *q, size_t n, const double *num,
+             const double *den, double *partial, double *rmax, hipStream_t st);
+void pcgStepF(float *x, const float *s, float *r, const float *q, size_t n, const double *num,
+              const double *den, double *partial, double *rmax, hipStream_t st);
+void pcgStepMixed(double *x, const float *s, float *r, const float *q, size_t n, const double *num,
+                  const double *den, double *partial, double *rmax, hipStream_t st);
+// x += (double)(float)(sign * num/den) * (double)p   (FP64 x, FP32 p)
+void axpyMixed(double *x, const float *p, size_t n, const double *num, const double *den, double sign,
+               hipStream_t s);
 // r = b - A x (FP64 true residual)
 void residual(const PressureSys &S, const double *b, const double *x, double *r, hipStream_t s);
 void pressureToGrid(Grid g, const PressureSys &S, const double *x, float *p, hipStream_t s);
@@ -153,8 +181,6 @@ int micTileDepth(bool fp64);
 size_t micTileLds(bool fp64);  // bytes of LDS per workgroup
 void micTileCounts(Grid g, int depth, int &ti, int &tj, int &tk);
 void micMarkTiles(Grid g, const PressureSys &S, int depth, int *tileRows, hipStream_t s);
-// L entries of the lower neighbours (lmx.. from lx..), FP64 or the FP32 copies
-void micLowerEntries(const PressureSys &S, bool fp64, hipStream_t s);
 void micApplyTiles(const PressureSys &S, MicSchedule &ms, const double *r, double *z, hipStream_t s);
 void micApplyTilesF(const PressureSys &S, MicSchedule &ms, const float *r, float *z, hipStream_t s);
 

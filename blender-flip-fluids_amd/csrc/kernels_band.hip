@@ -1,0 +1,255 @@
+// kernels_band.hip -- the MIC(0) preconditioner application z = (L L^T)^-1 r
+// (solveLower + solveLowerTransposeInPlace, pcgsolver.h:211-241) in bands of
+// K consecutive i+j+k hyperplanes per launch.
+//
+// The forward solve of row (i,j,k) needs the results of (i,j,k-1), (i,j-1,k)
+// and (i-1,j,k), all on the previous hyperplane.  A launch handles the band
+// of hyperplanes [l0, l0+K).  Each workgroup owns a 16 x 16 tile of (j, k)
+// lines along i and also recomputes the K-1 lines before the tile in -y and
+// -z (its halo): a row on band level q depends, through the band, only on
+// rows at most K-1-q lines away in -y/-z, and on rows of hyperplane l0-1,
+// which earlier launches have finished.  So the workgroup needs no other
+// workgroup's results from the same launch: every band level is one barrier,
+// the levels exchange results through LDS, and only the tile's own rows are
+// written.  Rows recomputed in the halo perform the same operations on the
+// same inputs as in their own tile, so they are bit-identical.
+//
+// The band's cells are staged in LDS first, numbered line by line so that
+// consecutive threads read consecutive cells along i: cell ids, row ids and
+// the row-indexed operands (rows are in k,j,i order) are read as contiguous
+// runs.  Then a thread follows one line along i through the band.
+//
+// The backward solve mirrors this (halo in +y/+z, hyperplanes descending).
+// It reads the forward result from one array and writes to another, so no
+// workgroup reads a value another one has already overwritten.
+//
+// Every row performs the reference's operations in the reference's order
+// (forward: subtract the -z, -y, -x terms, then scale by invdiag; backward:
+// +x, +y, +z, then scale), so the result equals the serial sweeps bit for bit.
+#include "kernels.h"
+
+namespace ff {
+
+namespace {
+
+constexpr int BT = 16;  // owned lines per tile side
+
+template <class T, int K>
+struct Band {
+    static constexpr int E = BT + K - 1;                // lines per side, halo included
+    static constexpr int LINES = E * E;
+    static constexpr int THREADS = (LINES + 63) / 64 * 64;
+    static constexpr int P = K + 1;                      // LDS stride per line (odd: no bank conflicts)
+    static constexpr int CELLS = LINES * K;
+    static constexpr int PER_THREAD = (CELLS + THREADS - 1) / THREADS;
+};
+
+// FWD: levels l0 .. l0+K-1, in = r, out = y, (ca, cb, cc) = L entries of the
+// row's -x/-y/-z neighbour rows toward it.  !FWD: levels l0 .. l0-K+1, in = y,
+// out = x, (ca, cb, cc) = the row's own L entries toward +x/+y/+z.
+//
+// Two phases.  Staging: the workgroup's band cells are numbered line by line
+// (K consecutive cells of one line along i), so consecutive threads read
+// consecutive cells and rows; every load of the phase is issued before any is
+// used.  Sweep: one thread per line runs the K band levels from LDS.
+template <class T, int K, bool FWD>
+__global__ __launch_bounds__((Band<T, K>::THREADS)) void micBandKernel(Grid g, const int *cellRow, const T *in,
+                                                                     T *out, const T *inv, const T *ca,
+                                                                     const T *cb, const T *cc, int l0,
+                                                                     int tilesJ) {
+    using B = Band<T, K>;
+    constexpr int E = B::E, LINES = B::LINES, P = B::P;
+    __shared__ T sIn[LINES * P], sInv[LINES * P], sA[LINES * P], sB[LINES * P], sC[LINES * P];
+    __shared__ int sRow[LINES * P];
+    __shared__ T val[2][LINES];
+    __shared__ int has[2][LINES];
+    const int ni = g.ni, nj = g.nj, nk = g.nk;
+    const int J0 = (blockIdx.x % tilesJ) * BT, K0 = (blockIdx.x / tilesJ) * BT;
+    // workgroup-uniform exit when no owned line reaches the band
+    const int lo = FWD ? l0 : l0 - (K - 1), hi = FWD ? l0 + (K - 1) : l0;
+    if (J0 + K0 > hi || J0 + K0 + 2 * (BT - 1) < lo - (ni - 1)) return;
+    const int s = FWD ? 1 : -1;  // direction of the sweep along the hyperplanes
+    const int jBase = FWD ? J0 - (K - 1) : J0, kBase = FWD ? K0 - (K - 1) : K0;
+
+    // ---- staging: rows, then operands, of every band cell of the workgroup ----
+    int crow[B::PER_THREAD];
+#pragma unroll
+    for (int t = 0; t < B::PER_THREAD; t++) {
+        const int c = threadIdx.x + t * B::THREADS;
+        const int line = c / K, q = c % K;
+        const int j = jBase + line % E, k = kBase + line / E;
+        const int i = (l0 + s * q) - j - k;
+        const bool ok = c < B::CELLS && j >= 0 && j < nj && k >= 0 && k < nk && i >= 0 && i < ni;
+        const int r = cellRow[ok ? i + (size_t)ni * (j + (size_t)nj * k) : 0];
+        crow[t] = ok ? r : -1;
+    }
+    // the line's neighbours on the hyperplane just outside the band, for its
+    // first band level (finished by earlier launches)
+    const int tid = threadIdx.x;
+    const bool active = tid < LINES;
+    const int ej = tid % E, ek = tid / E;
+    const int j = jBase + ej, k = kBase + ek;
+    const bool lineIn = active && j >= 0 && j < nj && k >= 0 && k < nk;
+    const int i0 = l0 - j - k;
+    const bool ok0 = lineIn && i0 >= 0 && i0 < ni;
+    const size_t c0 = ok0 ? i0 + (size_t)ni * (j + (size_t)nj * k) : 0;
+    const int ii = i0 - s, jj = j - s, kk = k - s;
+    const bool okx = ok0 && ii >= 0 && ii < ni, oky = ok0 && jj >= 0 && jj < nj, okz = ok0 && kk >= 0 && kk < nk;
+    const int r0 = cellRow[c0];
+    const int rx = cellRow[okx ? c0 - s : 0];
+    const int ry = cellRow[oky ? c0 - s * (size_t)ni : 0];
+    const int rz = cellRow[okz ? c0 - s * (size_t)ni * nj : 0];
+    const bool first = ok0 && r0 >= 0;
+    const bool hx = first && okx && rx >= 0, hy = first && oky && ry >= 0, hz = first && okz && rz >= 0;
+    T opIn[B::PER_THREAD], opInv[B::PER_THREAD], opA[B::PER_THREAD], opB[B::PER_THREAD], opC[B::PER_THREAD];
+#pragma unroll
+    for (int t = 0; t < B::PER_THREAD; t++) {
+        const size_t r = crow[t] >= 0 ? (size_t)crow[t] : 0;  // row 0 stands in for an absent row
+        opIn[t] = in[r];
+        opInv[t] = inv[r];
+        opA[t] = ca[r];
+        opB[t] = cb[r];
+        opC[t] = cc[r];
+    }
+    const T nx = out[hx ? rx : 0], ny = out[hy ? ry : 0], nz = out[hz ? rz : 0];
+#pragma unroll
+    for (int t = 0; t < B::PER_THREAD; t++) {
+        const int c = threadIdx.x + t * B::THREADS;
+        if (c < B::CELLS) {
+            const int a = (c / K) * P + c % K;
+            sRow[a] = crow[t];
+            sIn[a] = opIn[t];
+            sInv[a] = opInv[t];
+            sA[a] = opA[t];
+            sB[a] = opB[t];
+            sC[a] = opC[t];
+        }
+    }
+    __syncthreads();
+
+    // ---- sweep: one thread per line, one barrier per band level ----
+    const bool owned = FWD ? (ej >= K - 1 && ek >= K - 1) : (ej < BT && ek < BT);
+    const bool yin = FWD ? ej > 0 : ej < E - 1;
+    const bool zin = FWD ? ek > 0 : ek < E - 1;
+    const int cy = tid - s, cz = tid - s * E;
+    const int base = tid * P;
+    T prev = T(0);  // this line's result on the previous band level (the x neighbour)
+    bool prevHas = false;
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        const int buf = q & 1, pb = buf ^ 1;
+        const int row = active ? sRow[base + q] : -1;
+        T v = T(0);
+        if (row >= 0) {
+            T y = sIn[base + q];
+            const T a = sA[base + q], b = sB[base + q], c = sC[base + q];
+            if (FWD) {
+                if (q == 0) {
+                    if (hz) y -= c * nz;
+                    if (hy) y -= b * ny;
+                    if (hx) y -= a * nx;
+                } else {
+                    if (zin && has[pb][cz]) y -= c * val[pb][cz];
+                    if (yin && has[pb][cy]) y -= b * val[pb][cy];
+                    if (prevHas) y -= a * prev;
+                }
+            } else {
+                if (q == 0) {
+                    if (hx) y -= a * nx;
+                    if (hy) y -= b * ny;
+                    if (hz) y -= c * nz;
+                } else {
+                    if (prevHas) y -= a * prev;
+                    if (yin && has[pb][cy]) y -= b * val[pb][cy];
+                    if (zin && has[pb][cz]) y -= c * val[pb][cz];
+                }
+            }
+            v = y * sInv[base + q];
+            if (owned) out[row] = v;
+        }
+        if (active) {
+            val[buf][tid] = v;
+            has[buf][tid] = row >= 0;
+        }
+        prev = v;
+        prevHas = row >= 0;
+        __syncthreads();
+    }
+}
+
+// L entry of each row's -x/-y/-z neighbour row toward the row (0 when absent)
+template <class T>
+__global__ void lowerEntriesKernel(int m, const int *nbr, const T *lx, const T *ly, const T *lz, T *lmx,
+                                   T *lmy, T *lmz) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const int *nb = nbr + (size_t)6 * r;
+    lmx[r] = nb[0] >= 0 ? lx[nb[0]] : T(0);
+    lmy[r] = nb[2] >= 0 ? ly[nb[2]] : T(0);
+    lmz[r] = nb[4] >= 0 ? lz[nb[4]] : T(0);
+}
+
+template <class T, int K>
+void bandSweeps(Grid g, const PressureSys &S, const int *hs, const T *r, T *y, T *z, const T *inv,
+                const T *lx, const T *ly, const T *lz, const T *lmx, const T *lmy, const T *lmz,
+                hipStream_t st) {
+    const int tilesJ = (g.nj + BT - 1) / BT, tilesK = (g.nk + BT - 1) / BT;
+    const dim3 grid(tilesJ * tilesK), block(Band<T, K>::THREADS);
+    const int nl = S.nLevels;
+    // forward, bands [l0, l0+K) that hold rows
+    for (int l0 = 0; l0 < nl; l0 += K) {
+        const int l1 = l0 + K < nl ? l0 + K : nl;
+        if (hs[l1] == hs[l0]) continue;
+        hipLaunchKernelGGL((micBandKernel<T, K, true>), grid, block, 0, st, g, S.cellRow, r, y, inv, lmx, lmy,
+                           lmz, l0, tilesJ);
+    }
+    // backward, bands (l0-K, l0] from the top
+    for (int l0 = nl - 1; l0 >= 0; l0 -= K) {
+        const int lb = l0 - K + 1 > 0 ? l0 - K + 1 : 0;
+        if (hs[l0 + 1] == hs[lb]) continue;
+        hipLaunchKernelGGL((micBandKernel<T, K, false>), grid, block, 0, st, g, S.cellRow, y, z, inv, lx, ly, lz,
+                           l0, tilesJ);
+    }
+    hipCheck(hipGetLastError(), "MIC(0) band kernel launch");
+}
+
+template <int K>
+int bandLaunches(const PressureSys &S, const int *hs) {
+    int n = 0;
+    for (int l0 = 0; l0 < S.nLevels; l0 += K) n += hs[(l0 + K < S.nLevels ? l0 + K : S.nLevels)] > hs[l0];
+    return 2 * n;
+}
+
+// band depth: the staged band must fit in LDS (FP64: 6 levels, FP32: 8)
+constexpr int K64 = 6, K32 = 8;
+
+}  // namespace
+
+void mic0LowerEntries(const PressureSys &S, bool fp64, hipStream_t s) {
+    if (S.m == 0) return;
+    dim3 grid((S.m + 255) / 256), block(256);
+    if (fp64)
+        hipLaunchKernelGGL(lowerEntriesKernel<double>, grid, block, 0, s, S.m, S.nbr, S.lx, S.ly, S.lz, S.lmx,
+                           S.lmy, S.lmz);
+    else
+        hipLaunchKernelGGL(lowerEntriesKernel<float>, grid, block, 0, s, S.m, S.nbr, S.flx, S.fly, S.flz,
+                           S.flmx, S.flmy, S.flmz);
+}
+
+int mic0BandLaunches(const PressureSys &S, const int *hs, bool fp64) {
+    return fp64 ? bandLaunches<K64>(S, hs) : bandLaunches<K32>(S, hs);
+}
+
+void mic0ApplyBands(Grid g, const PressureSys &S, const int *hs, const double *r, double *y, double *z,
+                    hipStream_t s) {
+    if (S.m == 0) return;
+    bandSweeps<double, K64>(g, S, hs, r, y, z, S.invdiag, S.lx, S.ly, S.lz, S.lmx, S.lmy, S.lmz, s);
+}
+
+void mic0ApplyBandsF(Grid g, const PressureSys &S, const int *hs, const float *r, float *y, float *z,
+                     hipStream_t s) {
+    if (S.m == 0) return;
+    bandSweeps<float, K32>(g, S, hs, r, y, z, S.finvdiag, S.flx, S.fly, S.flz, S.flmx, S.flmy, S.flmz, s);
+}
+
+}  // namespace ff

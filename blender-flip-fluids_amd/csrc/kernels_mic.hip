@@ -266,18 +266,6 @@ __global__ __launch_bounds__(TPT) void micTileKernel(MicArgs<T> a) {
     }
 }
 
-// L entry of each row's -x/-y/-z neighbour row toward the row (0 when absent)
-template <class T>
-__global__ void lowerEntriesKernel(int m, const int *nbr, const T *lx, const T *ly, const T *lz, T *lmx,
-                                   T *lmy, T *lmz) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= m) return;
-    const int *nb = nbr + (size_t)6 * r;
-    lmx[r] = nb[0] >= 0 ? lx[nb[0]] : T(0);
-    lmy[r] = nb[2] >= 0 ? ly[nb[2]] : T(0);
-    lmz[r] = nb[4] >= 0 ? lz[nb[4]] : T(0);
-}
-
 __global__ void tileMarkKernel(Grid g, int m, const int *rows, int tk, int ti, int tj, int *tileRows) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= m) return;
@@ -310,17 +298,6 @@ void micMarkTiles(Grid g, const PressureSys &S, int tk, int *tileRows, hipStream
     if (S.m > 0)
         hipLaunchKernelGGL(tileMarkKernel, dim3((S.m + 255) / 256), dim3(256), 0, s, g, S.m, S.rows, tk, ti, tj,
                            tileRows);
-}
-
-void micLowerEntries(const PressureSys &S, bool fp64, hipStream_t s) {
-    if (S.m == 0) return;
-    dim3 grid((S.m + 255) / 256), block(256);
-    if (fp64)
-        hipLaunchKernelGGL(lowerEntriesKernel<double>, grid, block, 0, s, S.m, S.nbr, S.lx, S.ly, S.lz, S.lmx,
-                           S.lmy, S.lmz);
-    else
-        hipLaunchKernelGGL(lowerEntriesKernel<float>, grid, block, 0, s, S.m, S.nbr, S.flx, S.fly, S.flz,
-                           S.flmx, S.flmy, S.flmz);
 }
 
 template <class T, int TK>

@@ -213,21 +213,89 @@ __global__ void backwardLevelKernel(const int *nbr, const T *invdiag, const T *l
 }
 
 // FixedSparseMatrix multiply (sparsematrix.h:174-215): ascending column order
+// (A x)[r] with the terms added in ascending column order: -z, -y, -x, diag,
+// +x, +y, +z.  Every load is issued up front (an absent neighbour reads x[r]
+// and its term is skipped by a select), so the row costs one round trip.
+template <class T>
+__device__ inline T rowProduct(size_t r, const int *nbr, const T *diag, const T *ox, const T *oy, const T *oz,
+                               const T *mx, const T *my, const T *mz, const T *x) {
+    const int *nb = nbr + 6 * r;
+    const int n0 = nb[0], n1 = nb[1], n2 = nb[2], n3 = nb[3], n4 = nb[4], n5 = nb[5];
+    const T x0 = x[n0 >= 0 ? (size_t)n0 : r], x1 = x[n1 >= 0 ? (size_t)n1 : r];
+    const T x2 = x[n2 >= 0 ? (size_t)n2 : r], x3 = x[n3 >= 0 ? (size_t)n3 : r];
+    const T x4 = x[n4 >= 0 ? (size_t)n4 : r], x5 = x[n5 >= 0 ? (size_t)n5 : r];
+    const T xr = x[r];
+    const T cmz = mz[r], cmy = my[r], cmx = mx[r], cd = diag[r], cox = ox[r], coy = oy[r], coz = oz[r];
+    T s = 0;
+    if (n4 >= 0) s += cmz * x4;
+    if (n2 >= 0) s += cmy * x2;
+    if (n0 >= 0) s += cmx * x0;
+    s += cd * xr;
+    if (n1 >= 0) s += cox * x1;
+    if (n3 >= 0) s += coy * x3;
+    if (n5 >= 0) s += coz * x5;
+    return s;
+}
+
 template <class T>
 __global__ void spmvKernel(int m, const int *nbr, const T *diag, const T *ox, const T *oy,
                            const T *oz, const T *mx, const T *my, const T *mz, const T *x, T *y) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= m) return;
-    const int *nb = nbr + (size_t)6 * r;
-    T s = 0;
-    if (nb[4] >= 0) s += mz[r] * x[nb[4]];
-    if (nb[2] >= 0) s += my[r] * x[nb[2]];
-    if (nb[0] >= 0) s += mx[r] * x[nb[0]];
-    s += diag[r] * x[r];
-    if (nb[1] >= 0) s += ox[r] * x[nb[1]];
-    if (nb[3] >= 0) s += oy[r] * x[nb[3]];
-    if (nb[5] >= 0) s += oz[r] * x[nb[5]];
-    y[r] = s;
+    y[r] = rowProduct<T>(r, nbr, diag, ox, oy, oz, mx, my, mz, x);
+}
+
+// block sum (OP 0) or max (OP 1) of one double per thread, into partial[block]
+template <int OP>
+__device__ void blockFold(double v, double *partial) {
+    __shared__ double sh[REDUCE_THREADS];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = REDUCE_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off) {
+            if (OP == 0)
+                sh[threadIdx.x] += sh[threadIdx.x + off];
+            else
+                sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// SpMV as spmvKernel, grid-stride, with the per-thread partial of s . q in
+// the order of the standalone dot product
+template <class T>
+__global__ __launch_bounds__(REDUCE_THREADS) void spmvDotKernel(int m, const int *nbr, const T *diag,
+                                                                const T *ox, const T *oy, const T *oz,
+                                                                const T *mx, const T *my, const T *mz,
+                                                                const T *x, T *y, double *partial) {
+    double acc = 0.0;
+    const size_t stride = (size_t)gridDim.x * REDUCE_THREADS;
+    for (size_t r = (size_t)blockIdx.x * REDUCE_THREADS + threadIdx.x; r < (size_t)m; r += stride) {
+        const T s = rowProduct<T>(r, nbr, diag, ox, oy, oz, mx, my, mz, x);
+        y[r] = s;
+        acc += (double)x[r] * (double)s;
+    }
+    blockFold<0>(acc, partial);
+}
+
+// x += a s; r -= a q with a the float-rounded step, and the partial max |r|
+template <class TX, class T>
+__global__ __launch_bounds__(REDUCE_THREADS) void pcgStepKernel(TX *x, const T *s, T *r, const T *q, size_t n,
+                                                                const double *num, const double *den,
+                                                                double *partial) {
+    const double a = *num / *den;
+    const float fa = (float)a, fna = (float)(-a);
+    double acc = 0.0;
+    const size_t stride = (size_t)gridDim.x * REDUCE_THREADS;
+    for (size_t i = (size_t)blockIdx.x * REDUCE_THREADS + threadIdx.x; i < n; i += stride) {
+        x[i] += (TX)fa * (TX)s[i];
+        const T ri = r[i] + (T)fna * q[i];
+        r[i] = ri;
+        acc = fmax(acc, fabs((double)ri));
+    }
+    blockFold<1>(acc, partial);
 }
 
 // r = b - A x
@@ -237,16 +305,7 @@ __global__ void residualKernel(int m, const int *nbr, const double *diag, const 
                                const double *x, double *r) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m) return;
-    const int *nb = nbr + (size_t)6 * q;
-    double s = 0;
-    if (nb[4] >= 0) s += mz[q] * x[nb[4]];
-    if (nb[2] >= 0) s += my[q] * x[nb[2]];
-    if (nb[0] >= 0) s += mx[q] * x[nb[0]];
-    s += diag[q] * x[q];
-    if (nb[1] >= 0) s += ox[q] * x[nb[1]];
-    if (nb[3] >= 0) s += oy[q] * x[nb[3]];
-    if (nb[5] >= 0) s += oz[q] * x[nb[5]];
-    r[q] = b[q] - s;
+    r[q] = b[q] - rowProduct<double>(q, nbr, diag, ox, oy, oz, mx, my, mz, x);
 }
 
 // BLAS::addScaled(float alpha, ...) (blaswrapper.h:128-157): the step length
@@ -259,6 +318,17 @@ __global__ void axpyStepKernel(T *y, const T *x, size_t n, const double *num, co
     double a = den ? *num / *den : *num;
     float fa = (float)(sign * a);
     y[q] += (T)fa * x[q];
+}
+
+// x += (double)(float)(sign * num/den) * (double)p: the FP64 solution takes
+// the FP32 direction with the same float-rounded step length
+__global__ void axpyMixedKernel(double *x, const float *p, size_t n, const double *num, const double *den,
+                                double sign) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double a = den ? *num / *den : *num;
+    float fa = (float)(sign * a);
+    x[q] += (double)fa * (double)p[q];
 }
 
 __global__ void toGridKernel(int m, const int *rows, const double *x, float *p) {
@@ -359,6 +429,51 @@ void axpyFloatStepF(float *y, const float *x, size_t n, const double *num, const
                     double sign, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(axpyStepKernel<float>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, y, x, n, num, den, sign);
+}
+
+template <class T>
+static void launchSpmvDot(const PressureSys &S, const T *diag, const T *ox, const T *oy, const T *oz, const T *mx,
+                          const T *my, const T *mz, const T *x, T *y, double *partial, double *sq, hipStream_t st) {
+    const int nb = reduceBlocks(S.m);
+    hipLaunchKernelGGL(spmvDotKernel<T>, dim3(nb), dim3(REDUCE_THREADS), 0, st, S.m, S.nbr, diag, ox, oy, oz, mx,
+                       my, mz, x, y, partial);
+    reduceFinish(partial, nb, sq, false, st);
+}
+void spmvDot(const PressureSys &S, const double *x, double *y, double *partial, double *sq, hipStream_t st) {
+    if (S.m == 0) return;
+    launchSpmvDot<double>(S, S.diag, S.ox, S.oy, S.oz, S.mx, S.my, S.mz, x, y, partial, sq, st);
+}
+void spmvDotF(const PressureSys &S, const float *x, float *y, double *partial, double *sq, hipStream_t st) {
+    if (S.m == 0) return;
+    launchSpmvDot<float>(S, S.fdiag, S.fox, S.foy, S.foz, S.fmx, S.fmy, S.fmz, x, y, partial, sq, st);
+}
+
+template <class TX, class T>
+static void launchPcgStep(TX *x, const T *s, T *r, const T *q, size_t n, const double *num, const double *den,
+                          double *partial, double *rmax, hipStream_t st) {
+    if (n == 0) return;
+    const int nb = reduceBlocks(n);
+    hipLaunchKernelGGL((pcgStepKernel<TX, T>), dim3(nb), dim3(REDUCE_THREADS), 0, st, x, s, r, q, n, num, den,
+                       partial);
+    reduceFinish(partial, nb, rmax, true, st);
+}
+void pcgStep(double *x, const double *s, double *r, const double *q, size_t n, const double *num,
+             const double *den, double *partial, double *rmax, hipStream_t st) {
+    launchPcgStep<double, double>(x, s, r, q, n, num, den, partial, rmax, st);
+}
+void pcgStepF(float *x, const float *s, float *r, const float *q, size_t n, const double *num,
+              const double *den, double *partial, double *rmax, hipStream_t st) {
+    launchPcgStep<float, float>(x, s, r, q, n, num, den, partial, rmax, st);
+}
+void pcgStepMixed(double *x, const float *s, float *r, const float *q, size_t n, const double *num,
+                  const double *den, double *partial, double *rmax, hipStream_t st) {
+    launchPcgStep<double, float>(x, s, r, q, n, num, den, partial, rmax, st);
+}
+
+void axpyMixed(double *x, const float *p, size_t n, const double *num, const double *den, double sign,
+               hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(axpyMixedKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, x, p, n, num, den, sign);
 }
 
 void residual(const PressureSys &S, const double *b, const double *x, double *r, hipStream_t s) {

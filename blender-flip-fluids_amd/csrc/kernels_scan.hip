@@ -161,6 +161,13 @@ void reduce(const T *a, const T *b, size_t n, double *partial, double *out, hipS
 
 }  // namespace
 
+void reduceFinish(const double *partial, int nb, double *out, bool isMax, hipStream_t s) {
+    if (isMax)
+        hipLaunchKernelGGL((reduceStage2<1>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+    else
+        hipLaunchKernelGGL((reduceStage2<0>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+}
+
 void dot(const double *a, const double *b, size_t n, double *partial, double *out, hipStream_t s) {
     reduce<double, 0>(a, b, n, partial, out, s);
 }

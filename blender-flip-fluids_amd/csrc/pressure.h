@@ -29,7 +29,10 @@ public:
 
     void allocate(Grid g);
     // Builds the system from the current fields, solves it and writes the
-    // float pressure grid.  mode 0 = FP64 PCG, 1 = FP32 PCG in FP64 refinement.
+    // float pressure grid.  mode 0 = FP64 PCG; 1 = FP32 PCG restarted inside
+    // FP64 iterative refinement (innerTol = relative tolerance of each inner
+    // solve); 2 = FP32 PCG with FP64 reliable updates (innerTol = drop of the
+    // FP32 residual between true-residual updates).
     PressureResult solve(double dt, const float *u, const float *v, const float *w, const float *wu,
                          const float *wv, const float *ww, const float *phi, double tol, int maxIter,
                          int mode, double innerTol, float *pressure, int *scanScratch, hipStream_t s,
@@ -37,10 +40,11 @@ public:
     const PressureSys &system() const { return _S; }
     const std::vector<int> &levelTable() const { return _hLevelStart; }
     // one preconditioner application z = (L L^T)^-1 r on the current factor
-    // (after solve()); FP64 or the FP32 copy.  tiles selects the tile-wavefront
-    // kernel, which is switched off and raises an error.
-    void applyPreconditioner(const double *r, double *z, bool tiles, hipStream_t s);
-    void applyPreconditionerF(const float *r, float *z, bool tiles, hipStream_t s);
+    // (after solve()); FP64 or the FP32 copy.  schedule 0 = one launch per
+    // hyperplane, 1 = tile wavefront (switched off: raises an error),
+    // 2 = bands of hyperplanes (the solver's).
+    void applyPreconditioner(const double *r, double *z, int schedule, hipStream_t s);
+    void applyPreconditionerF(const float *r, float *z, int schedule, hipStream_t s);
     // throws when a preconditioner launch gave up waiting (waits for the stream)
     void checkError(hipStream_t s);
 
@@ -48,8 +52,8 @@ private:
     Grid _g{};
     PressureSys _S{};
     void *_block = nullptr;  // one allocation for everything
-    double *_x = nullptr, *_r = nullptr, *_z = nullptr, *_s = nullptr;
-    float *_fr = nullptr, *_fz = nullptr, *_fs = nullptr, *_fd = nullptr, *_fq = nullptr;
+    double *_x = nullptr, *_r = nullptr, *_z = nullptr, *_s = nullptr, *_y = nullptr;
+    float *_fr = nullptr, *_fz = nullptr, *_fs = nullptr, *_fd = nullptr, *_fq = nullptr, *_fy = nullptr;
     double *_partial = nullptr, *_scal = nullptr;
     std::vector<int> _hLevelStart;
     MicSchedule _ms;
@@ -57,7 +61,7 @@ private:
     std::vector<int> _hTileRows, _hOrder;
     KernelTimers *_t = nullptr;
     KernelTimers _noTimers;
-    int _levelLaunches = 0;
+    int _levelLaunches = 0, _bandLaunches = 0;
 
     double hostScalar(const double *d, hipStream_t s);
     // tile schedule of the preconditioner application for this system
@@ -65,6 +69,8 @@ private:
     bool pcgFP64(double tolFactor, int maxIter, double resid0, int &iters, double &resid, hipStream_t s);
     bool pcgMixed(double tolFactor, int maxIter, double resid0, double innerTol, int &iters,
                   double &resid, hipStream_t s);
+    bool pcgReliable(double tolFactor, int maxIter, double resid0, double delta, int &iters, double &resid,
+                     hipStream_t s);
 };
 
 }  // namespace ff

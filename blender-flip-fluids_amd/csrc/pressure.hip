@@ -40,7 +40,7 @@ void PressureSolver::allocate(Grid g) {
     size_t oD[15], oF[14];
     for (auto &o : oD) o = take(8 * M);
     for (auto &o : oF) o = take(4 * M);
-    size_t oV[4], oFV[5];
+    size_t oV[5], oFV[6];
     for (auto &o : oV) o = take(8 * M);
     for (auto &o : oFV) o = take(4 * M);
     size_t oPart = take(8 * 1024), oScal = take(8 * 16);
@@ -73,8 +73,9 @@ void PressureSolver::allocate(Grid g) {
     _S.fmx = f[8]; _S.fmy = f[9]; _S.fmz = f[10];
     _S.flmx = f[11]; _S.flmy = f[12]; _S.flmz = f[13];
     _x = (double *)(b + oV[0]); _r = (double *)(b + oV[1]); _z = (double *)(b + oV[2]); _s = (double *)(b + oV[3]);
+    _y = (double *)(b + oV[4]);
     _fr = (float *)(b + oFV[0]); _fz = (float *)(b + oFV[1]); _fs = (float *)(b + oFV[2]);
-    _fd = (float *)(b + oFV[3]); _fq = (float *)(b + oFV[4]);
+    _fd = (float *)(b + oFV[3]); _fq = (float *)(b + oFV[4]); _fy = (float *)(b + oFV[5]);
     _partial = (double *)(b + oPart);
     _scal = (double *)(b + oScal);
     _S.m = 0;
@@ -131,18 +132,22 @@ void PressureSolver::checkError(hipStream_t s) {
     if (e) throw std::runtime_error("MIC(0) preconditioner: a tile wait exceeded its time limit");
 }
 
-void PressureSolver::applyPreconditioner(const double *r, double *z, bool tiles, hipStream_t s) {
-    if (tiles) {
+void PressureSolver::applyPreconditioner(const double *r, double *z, int schedule, hipStream_t s) {
+    if (schedule == 1) {
         buildSchedule(true, s);
         micApplyTiles(_S, _ms, r, z, s);
+    } else if (schedule == 2) {
+        mic0ApplyBands(_g, _S, _hLevelStart.data(), r, _y, z, s);
     } else {
         mic0Apply(_S, _hLevelStart.data(), r, z, s);
     }
 }
-void PressureSolver::applyPreconditionerF(const float *r, float *z, bool tiles, hipStream_t s) {
-    if (tiles) {
+void PressureSolver::applyPreconditionerF(const float *r, float *z, int schedule, hipStream_t s) {
+    if (schedule == 1) {
         buildSchedule(false, s);
         micApplyTilesF(_S, _ms, r, z, s);
+    } else if (schedule == 2) {
+        mic0ApplyBandsF(_g, _S, _hLevelStart.data(), r, _fy, z, s);
     } else {
         mic0ApplyF(_S, _hLevelStart.data(), r, z, s);
     }
@@ -195,6 +200,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     if (afterAssembly) hipCheck(hipEventRecord(afterAssembly, s), "event");
     _levelLaunches = 0;
     for (int l = 0; l < _S.nLevels; l++) _levelLaunches += _hLevelStart[l + 1] > _hLevelStart[l];
+    _bandLaunches = mic0BandLaunches(_S, _hLevelStart.data(), mode == 0);
     if (maxAbs < tol) {
         hipCheck(hipMemsetAsync(pressure, 0, sizeof(float) * N, s), "memset");
         res.success = true;
@@ -204,8 +210,9 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     }
     int iters = 0;
     double resid = -1.0;
-    bool conv = mode == 0 ? pcgFP64(tol, maxIter, maxAbs, iters, resid, s)
-                          : pcgMixed(tol, maxIter, maxAbs, innerTol, iters, resid, s);
+    bool conv = mode == 0   ? pcgFP64(tol, maxIter, maxAbs, iters, resid, s)
+                : mode == 1 ? pcgMixed(tol, maxIter, maxAbs, innerTol, iters, resid, s)
+                            : pcgReliable(tol, maxIter, maxAbs, innerTol, iters, resid, s);
     checkError(s);
     pressureToGrid(_g, _S, _x, pressure, s);
     float e = (float)resid;
@@ -228,10 +235,10 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     double tol = tolFactor * resid;
     _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
     mic0Factor(_S, hs, st);
-    micLowerEntries(_S, true, st);
+    mic0LowerEntries(_S, true, st);
     _t->end(T_MIC0_FACTOR, st);
-    _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
-    mic0Apply(_S, hs, _r, _z, st);
+    _t->begin(T_MIC0_APPLY, st, _bandLaunches);
+    mic0ApplyBands(_g, _S, hs, _r, _y, _z, st);
     _t->end(T_MIC0_APPLY, st);
     double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
     dot(_z, _r, m, _partial, rho, st);
@@ -245,19 +252,16 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     int it;
     for (it = 0; it < maxIter; it++) {
         _t->begin(T_SPMV, st);
-        spmv(_S, s, z, st);
+        spmvDot(_S, s, z, _partial, sz, st);
         _t->end(T_SPMV, st);
-        dot(s, z, m, _partial, sz, st);
-        axpyFloatStep(_x, s, m, rho, sz, 1.0, st);
-        axpyFloatStep(_r, z, m, rho, sz, -1.0, st);
-        absMax(_r, m, _partial, rn, st);
+        pcgStep(_x, s, _r, z, m, rho, sz, _partial, rn, st);
         resid = hostScalar(rn, st);
         if (resid <= std::min(tol, 1.0)) {
             iters = it + 1;
             return true;
         }
-        _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
-        mic0Apply(_S, hs, _r, z, st);
+        _t->begin(T_MIC0_APPLY, st, _bandLaunches);
+        mic0ApplyBands(_g, _S, hs, _r, _y, z, st);
         _t->end(T_MIC0_APPLY, st);
         dot(z, _r, m, _partial, rhoNew, st);
         // beta = rhoNew / rho; z += (float)beta * s; then s <-> z
@@ -286,7 +290,7 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
     mic0Factor(_S, hs, st);
     _t->end(T_MIC0_FACTOR, st);
     mic0ToFloat(_S, st);
-    micLowerEntries(_S, false, st);
+    mic0LowerEntries(_S, false, st);
     double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
     resid = resid0;
     iters = 0;
@@ -297,8 +301,8 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
         absMaxf(_fr, m, _partial, rn, st);
         double innerTol = innerRel * hostScalar(rn, st);
         hipCheck(hipMemsetAsync(_fd, 0, sizeof(float) * m, st), "memset");
-        _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
-        mic0ApplyF(_S, hs, _fr, _fz, st);
+        _t->begin(T_MIC0_APPLY, st, _bandLaunches);
+        mic0ApplyBandsF(_g, _S, hs, _fr, _fy, _fz, st);
         _t->end(T_MIC0_APPLY, st);
         dotf(_fz, _fr, m, _partial, rho, st);
         double hrho = hostScalar(rho, st);
@@ -307,16 +311,13 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
         float *s = _fs, *z = _fz;
         while (iters < maxIter) {
             _t->begin(T_SPMV, st);
-            spmvF(_S, s, _fq, st);
+            spmvDotF(_S, s, _fq, _partial, sz, st);
             _t->end(T_SPMV, st);
-            dotf(s, _fq, m, _partial, sz, st);
-            axpyFloatStepF(_fd, s, m, rho, sz, 1.0, st);
-            axpyFloatStepF(_fr, _fq, m, rho, sz, -1.0, st);
-            absMaxf(_fr, m, _partial, rn, st);
+            pcgStepF(_fd, s, _fr, _fq, m, rho, sz, _partial, rn, st);
             iters++;
             if (hostScalar(rn, st) <= innerTol) break;
-            _t->begin(T_MIC0_APPLY, st, 2 * _levelLaunches);
-            mic0ApplyF(_S, hs, _fr, z, st);
+            _t->begin(T_MIC0_APPLY, st, _bandLaunches);
+            mic0ApplyBandsF(_g, _S, hs, _fr, _fy, z, st);
             _t->end(T_MIC0_APPLY, st);
             dotf(z, _fr, m, _partial, rhoNew, st);
             axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);
@@ -324,6 +325,74 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
             std::swap(rho, rhoNew);
         }
         addFloatToDouble(_x, _fd, m, st);
+        residual(_S, _S.rhs, _x, _r, st);
+        absMax(_r, m, _partial, rn, st);
+        resid = hostScalar(rn, st);
+    }
+    return resid <= tol;
+}
+
+// FP32 PCG whose FP64 solution is kept exact by reliable updates (Sleijpen &
+// van der Vorst, 1996).  The search directions, the FP32 residual, SpMV and the
+// preconditioner run in FP32; x accumulates in FP64 with the float-rounded
+// step length; dot products accumulate in FP64.  Whenever the FP32 residual
+// has dropped by `delta` since the last update (or meets the tolerance), the
+// true residual r = b - A x is recomputed in FP64, tested against the
+// reference's stop rule |r|_inf <= min(tol * |b|_inf, 1.0), and replaces the
+// FP32 residual; the search direction is kept, so the Krylov space is not
+// restarted.  Iterations counts every FP32 PCG step.
+bool PressureSolver::pcgReliable(double tolFactor, int maxIter, double resid0, double delta, int &iters,
+                                 double &resid, hipStream_t st) {
+    const int m = _S.m;
+    const int *hs = _hLevelStart.data();
+    if (tolFactor < 1e-30) tolFactor = 1e-30;
+    const double tol = std::min(tolFactor * resid0, 1.0);
+    hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
+    _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
+    mic0Factor(_S, hs, st);
+    _t->end(T_MIC0_FACTOR, st);
+    mic0ToFloat(_S, st);
+    mic0LowerEntries(_S, false, st);
+    double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
+    castToFloat(_S.rhs, _fr, m, st);
+    double sinceUpdate = resid0;
+    resid = resid0;
+    iters = 0;
+    _t->begin(T_MIC0_APPLY, st, _bandLaunches);
+    mic0ApplyBandsF(_g, _S, hs, _fr, _fy, _fz, st);
+    _t->end(T_MIC0_APPLY, st);
+    dotf(_fz, _fr, m, _partial, rho, st);
+    double hrho = hostScalar(rho, st);
+    if (hrho == 0 || hrho != hrho) return false;
+    hipCheck(hipMemcpyAsync(_fs, _fz, sizeof(float) * m, hipMemcpyDeviceToDevice, st), "copy");
+    float *s = _fs, *z = _fz;
+    bool trueKnown = true;  // resid holds the true residual of the current x
+    while (iters < maxIter) {
+        _t->begin(T_SPMV, st);
+        spmvDotF(_S, s, _fq, _partial, sz, st);
+        _t->end(T_SPMV, st);
+        pcgStepMixed(_x, s, _fr, _fq, m, rho, sz, _partial, rn, st);
+        iters++;
+        trueKnown = false;
+        const double rf = hostScalar(rn, st);
+        if (rf <= delta * sinceUpdate || rf <= tol) {
+            residual(_S, _S.rhs, _x, _r, st);
+            absMax(_r, m, _partial, rn, st);
+            resid = hostScalar(rn, st);
+            trueKnown = true;
+            if (resid <= tol) return true;
+            castToFloat(_r, _fr, m, st);
+            sinceUpdate = resid;
+        }
+        _t->begin(T_MIC0_APPLY, st, _bandLaunches);
+        mic0ApplyBandsF(_g, _S, hs, _fr, _fy, z, st);
+        _t->end(T_MIC0_APPLY, st);
+        dotf(z, _fr, m, _partial, rhoNew, st);
+        axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);
+        std::swap(s, z);
+        std::swap(rho, rhoNew);
+    }
+    if (!trueKnown) {
         residual(_S, _S.rhs, _x, _r, st);
         absMax(_r, m, _partial, rn, st);
         resid = hostScalar(rn, st);

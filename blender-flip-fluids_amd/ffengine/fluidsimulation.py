@@ -236,15 +236,21 @@ class FluidSimulation(object):
         lambda s: _call0(s, "FluidSimulation_get_pressure_solver_tolerance", c_double),
         _setter("FluidSimulation_set_pressure_solver_tolerance", c_double))
 
-    def set_pressure_solver_precision(self, mode, inner_tolerance=1e-4):
-        """mode 'fp64' (reference PCG) or 'mixed' (FP32 PCG in FP64 refinement)."""
-        m = {"fp64": 0, "mixed": 1}[mode] if isinstance(mode, str) else int(mode)
+    def set_pressure_solver_precision(self, mode, inner_tolerance=None):
+        """mode 'fp64' (the reference's PCG), 'mixed' (FP32 PCG restarted inside
+        FP64 iterative refinement; inner_tolerance = relative tolerance of each
+        inner solve, default 1e-4) or 'mixed-reliable' (FP32 PCG with FP64
+        reliable updates; inner_tolerance = FP32 residual drop between true
+        residual updates, default 0.1)."""
+        m = {"fp64": 0, "mixed": 1, "mixed-reliable": 2}[mode] if isinstance(mode, str) else int(mode)
+        if inner_tolerance is None:
+            inner_tolerance = 0.1 if m == 2 else 1e-4
         f = lib.FluidSimulation_set_pressure_solver_precision
         pb.init_lib_func(f, [c_void_p, c_int, c_double, c_void_p], None)
         pb.execute_lib_func(f, [self(), m, inner_tolerance])
 
     def get_pressure_solver_precision(self):
-        return ("fp64", "mixed")[_call0(self, "FluidSimulation_get_pressure_solver_precision", c_int)]
+        return ("fp64", "mixed", "mixed-reliable")[_call0(self, "FluidSimulation_get_pressure_solver_precision", c_int)]
 
     @_check_simulation_not_initialized
     def set_solid_phi(self, phi_nodes):

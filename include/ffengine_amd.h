@@ -149,10 +149,13 @@ char *CBindings_get_error_message(void);                                       /
 void FluidSimulation_set_pressure_solver_tolerance(FluidSimulation *obj, double tol, int *err);
 double FluidSimulation_get_pressure_solver_tolerance(FluidSimulation *obj, int *err);
 
-/* Pressure precision: 0 = FP64 PCG with the reference's MIC(0) preconditioner
- * and float-rounded step lengths; 1 = FP32 PCG inside FP64 iterative
- * refinement (the FP64 true residual decides convergence), inner relative
- * tolerance inner_tol. */
+/* Pressure precision (no counterpart in the reference, which always solves
+ * in FP64): 0 = FP64 PCG with the reference's MIC(0) preconditioner and
+ * float-rounded step lengths; 1 = FP32 PCG restarted inside FP64 iterative
+ * refinement, inner relative tolerance inner_tol; 2 = FP32 PCG with FP64
+ * reliable updates, the true residual recomputed whenever the FP32 residual
+ * has dropped by inner_tol.  In modes 1 and 2 the FP64 true residual decides
+ * convergence with the reference's rule. */
 void FluidSimulation_set_pressure_solver_precision(FluidSimulation *obj, int mode,
                                                    double inner_tol, int *err);
 int FluidSimulation_get_pressure_solver_precision(FluidSimulation *obj, int *err);
@@ -222,9 +225,10 @@ int FFAMD_pressure_system(int isize, int jsize, int ksize, double dx, double dt,
  * (solveLower + solveLowerTransposeInPlace, pcgsolver.h:211-241).  r and z
  * hold n values in row order and n must equal M (else an error); precision
  * 0 = FP64, 1 = the FP32 copy of the factor used by the mixed-precision solve
- * (r rounded to float, z widened).  schedule 0 = hyperplane launches (the
- * solver's), 1 = the tile-wavefront kernel, which is switched off and fails
- * with an error.  The inputs must give a non-zero divergence.  Returns M. */
+ * (r rounded to float, z widened).  schedule 0 = one launch per hyperplane,
+ * 1 = the tile-wavefront kernel, which is switched off and fails with an
+ * error, 2 = bands of hyperplanes (the solver's).  The inputs must give a
+ * non-zero divergence.  Returns M. */
 int FFAMD_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, const float *u,
                      const float *v, const float *w, const float *wu, const float *wv,
                      const float *ww, const float *liquid_phi, int precision, int schedule,

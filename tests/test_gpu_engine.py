@@ -79,15 +79,24 @@ def test_solid_phi_matches_oracle():
                           O.solid_box_phi((n, n, n), 1.0 / n).view(np.uint32))
 
 
-def test_mixed_precision_engine_runs():
+@pytest.mark.parametrize("precision", ["mixed", "mixed-reliable"])
+def test_mixed_precision_engine_runs(precision):
     n = 24
-    sim = make_gpu(n, scenes.dam_break_particles(n, seed=2), precision="mixed", tol=1e-8)
-    say("engine mixed precision")
+    pos = scenes.dam_break_particles(n, seed=2)
+    sim = make_gpu(n, pos, precision=precision, tol=1e-8)
+    ref = make_gpu(n, pos, precision="fp64", tol=1e-8)
+    say("engine %s precision" % precision)
     sim.update(1.0 / 30.0)
+    ref.update(1.0 / 30.0)
     stats = sim.get_last_pressure_solve_stats()
     assert stats["success"] and stats["unknowns"] > 0
     p, v = gpu_particles(sim)
     assert np.isfinite(p).all() and np.isfinite(v).all()
+    # both solves meet the same FP64 residual test, so the first frame's
+    # positions agree with the FP64 run far below a cell
+    pr, _ = gpu_particles(ref)
+    assert len(p) == len(pr)
+    assert np.abs(p - pr).max() <= 1e-4 / n
 
 
 def test_error_paths():

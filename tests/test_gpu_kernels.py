@@ -215,12 +215,13 @@ def _check_mic0_apply(n, schedule):
     assert same_bits(g32.astype(np.float32), _mic_sweeps(sysm, n3, r, np.float32))
 
 
+@pytest.mark.parametrize("schedule", [2, 0], ids=["bands", "hyperplanes"])
 @pytest.mark.parametrize("n", CASES + [70])
-def test_mic0_apply_bitexact(n):
-    """The preconditioner application the solver uses (hyperplane launches)
-    against the oracle's serial sweeps (FP64) and a hyperplane-vectorised FP32
-    restatement (mixed mode)."""
-    _check_mic0_apply(n, schedule=0)
+def test_mic0_apply_bitexact(n, schedule):
+    """The preconditioner application in bands of hyperplanes (the solver's)
+    and one launch per hyperplane, against the oracle's serial sweeps (FP64)
+    and a hyperplane-vectorised FP32 restatement (mixed mode)."""
+    _check_mic0_apply(n, schedule=schedule)
 
 
 @pytest.mark.skip(reason="tile-wavefront MIC(0) kernel switched off: under test it never finished (the compiled "
@@ -253,11 +254,12 @@ def test_pressure_solve_fp64(n):
         assert np.abs(g - o).max() <= 1e-5 * max(np.abs(o).max(), 1e-6)
 
 
+@pytest.mark.parametrize("mode", [1, 2], ids=["restarted", "reliable"])
 @pytest.mark.parametrize("n", CASES)
-def test_pressure_solve_mixed(n):
+def test_pressure_solve_mixed(n, mode):
     n3, dx, dt, uvw, weights, phi = _pressure_inputs(n)
-    say(f"pressure_solve mixed n={n}")
-    gp, _, _, git, gerr, gok = K.pressure_solve(n3, dx, dt, uvw, weights, phi, tol=1e-9, mode=1)
+    say(f"pressure_solve mixed mode={mode} n={n}")
+    gp, _, _, git, gerr, gok = K.pressure_solve(n3, dx, dt, uvw, weights, phi, tol=1e-9, mode=mode)
     op, oit, oerr, ook, m = O.pressure_solve(n3, dx, dt, uvw, weights, phi, tol=1e-9)
     assert gok
     sysm = O.pressure_system(n3, dx, dt, uvw, weights, phi)
