@@ -127,11 +127,15 @@ void mic0ApplyF(const PressureSys &S, const int *hLevelStart, const float *r, fl
 // ---- MIC(0) application in bands of hyperplanes (kernels_band.hip) ----
 // L entries of the lower neighbour rows (lmx.. from lx..; FP64 or the FP32 copies)
 void mic0LowerEntries(const PressureSys &S, bool fp64, hipStream_t s);
+// tiles of the band schedule and, per tile, the lowest and highest i+j+k of
+// its rows (tileLo/tileHi hold mic0BandTiles ints; once per system)
+int mic0BandTiles(Grid g, bool fp64);
+void mic0BandTileRanges(Grid g, const PressureSys &S, bool fp64, int *tileLo, int *tileHi, hipStream_t s);
 // z = (L L^T)^-1 r with y as the forward result (M values, distinct from r and z)
-void mic0ApplyBands(Grid g, const PressureSys &S, const int *hLevelStart, const double *r, double *y,
-                    double *z, hipStream_t s);
-void mic0ApplyBandsF(Grid g, const PressureSys &S, const int *hLevelStart, const float *r, float *y,
-                     float *z, hipStream_t s);
+void mic0ApplyBands(Grid g, const PressureSys &S, const int *hLevelStart, const int *tileLo,
+                    const int *tileHi, const double *r, double *y, double *z, hipStream_t s);
+void mic0ApplyBandsF(Grid g, const PressureSys &S, const int *hLevelStart, const int *tileLo,
+                     const int *tileHi, const float *r, float *y, float *z, hipStream_t s);
 int mic0BandLaunches(const PressureSys &S, const int *hLevelStart, bool fp64);
 void spmv(const PressureSys &S, const double *x, double *y, hipStream_t s);
 void spmvF(const PressureSys &S, const float *x, float *y, hipStream_t s);

@@ -4,7 +4,7 @@
 //
 // The forward solve of row (i,j,k) needs the results of (i,j,k-1), (i,j-1,k)
 // and (i-1,j,k), all on the previous hyperplane.  A launch handles the band
-// of hyperplanes [l0, l0+K).  Each workgroup owns a 16 x 16 tile of (j, k)
+// of hyperplanes [l0, l0+K).  Each workgroup owns a BT x BT tile of (j, k)
 // lines along i and also recomputes the K-1 lines before the tile in -y and
 // -z (its halo): a row on band level q depends, through the band, only on
 // rows at most K-1-q lines away in -y/-z, and on rows of hyperplane l0-1,
@@ -32,9 +32,7 @@ namespace ff {
 
 namespace {
 
-constexpr int BT = 16;  // owned lines per tile side
-
-template <class T, int K>
+template <class T, int K, int BT>
 struct Band {
     static constexpr int E = BT + K - 1;                // lines per side, halo included
     static constexpr int LINES = E * E;
@@ -52,12 +50,13 @@ struct Band {
 // (K consecutive cells of one line along i), so consecutive threads read
 // consecutive cells and rows; every load of the phase is issued before any is
 // used.  Sweep: one thread per line runs the K band levels from LDS.
-template <class T, int K, bool FWD>
-__global__ __launch_bounds__((Band<T, K>::THREADS)) void micBandKernel(Grid g, const int *cellRow, const T *in,
-                                                                     T *out, const T *inv, const T *ca,
-                                                                     const T *cb, const T *cc, int l0,
-                                                                     int tilesJ) {
-    using B = Band<T, K>;
+template <class T, int K, int BT, bool FWD>
+__global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid g, const int *cellRow,
+                                                                         const T *in, T *out, const T *inv,
+                                                                         const T *ca, const T *cb, const T *cc,
+                                                                         const int *tileLo, const int *tileHi,
+                                                                         int l0, int tilesJ) {
+    using B = Band<T, K, BT>;
     constexpr int E = B::E, LINES = B::LINES, P = B::P;
     __shared__ T sIn[LINES * P], sInv[LINES * P], sA[LINES * P], sB[LINES * P], sC[LINES * P];
     __shared__ int sRow[LINES * P];
@@ -65,21 +64,28 @@ __global__ __launch_bounds__((Band<T, K>::THREADS)) void micBandKernel(Grid g, c
     __shared__ int has[2][LINES];
     const int ni = g.ni, nj = g.nj, nk = g.nk;
     const int J0 = (blockIdx.x % tilesJ) * BT, K0 = (blockIdx.x / tilesJ) * BT;
-    // workgroup-uniform exit when no owned line reaches the band
+    // workgroup-uniform exit when none of the tile's own rows lies in the band
     const int lo = FWD ? l0 : l0 - (K - 1), hi = FWD ? l0 + (K - 1) : l0;
-    if (J0 + K0 > hi || J0 + K0 + 2 * (BT - 1) < lo - (ni - 1)) return;
+    if (tileHi[blockIdx.x] < lo || tileLo[blockIdx.x] > hi) return;
     const int s = FWD ? 1 : -1;  // direction of the sweep along the hyperplanes
     const int jBase = FWD ? J0 - (K - 1) : J0, kBase = FWD ? K0 - (K - 1) : K0;
 
     // ---- staging: rows, then operands, of every band cell of the workgroup ----
+    // A halo line dj + dk lines away from the tile feeds the tile's rows only
+    // through band levels q <= K-1-dj-dk; its later cells are left unloaded
+    // (absent) and are never read by a row that is.
     int crow[B::PER_THREAD];
 #pragma unroll
     for (int t = 0; t < B::PER_THREAD; t++) {
         const int c = threadIdx.x + t * B::THREADS;
         const int line = c / K, q = c % K;
-        const int j = jBase + line % E, k = kBase + line / E;
+        const int lj = line % E, lk = line / E;
+        const int j = jBase + lj, k = kBase + lk;
         const int i = (l0 + s * q) - j - k;
-        const bool ok = c < B::CELLS && j >= 0 && j < nj && k >= 0 && k < nk && i >= 0 && i < ni;
+        const int dj = FWD ? (K - 1 - lj > 0 ? K - 1 - lj : 0) : (lj - (BT - 1) > 0 ? lj - (BT - 1) : 0);
+        const int dk = FWD ? (K - 1 - lk > 0 ? K - 1 - lk : 0) : (lk - (BT - 1) > 0 ? lk - (BT - 1) : 0);
+        const bool ok = c < B::CELLS && q <= K - 1 - dj - dk && j >= 0 && j < nj && k >= 0 && k < nk &&
+                        i >= 0 && i < ni;
         const int r = cellRow[ok ? i + (size_t)ni * (j + (size_t)nj * k) : 0];
         crow[t] = ok ? r : -1;
     }
@@ -133,16 +139,29 @@ __global__ __launch_bounds__((Band<T, K>::THREADS)) void micBandKernel(Grid g, c
     const bool zin = FWD ? ek > 0 : ek < E - 1;
     const int cy = tid - s, cz = tid - s * E;
     const int base = tid * P;
+    // the line's own operands, read before the first level: a barrier orders
+    // LDS accesses, so reads inside the level loop would sit on its critical path
+    int lRow[K];
+    T lIn[K], lInv[K], lA[K], lB[K], lC[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        lRow[q] = active ? sRow[base + q] : -1;
+        lIn[q] = sIn[base + q];
+        lInv[q] = sInv[base + q];
+        lA[q] = sA[base + q];
+        lB[q] = sB[base + q];
+        lC[q] = sC[base + q];
+    }
     T prev = T(0);  // this line's result on the previous band level (the x neighbour)
     bool prevHas = false;
 #pragma unroll
     for (int q = 0; q < K; q++) {
         const int buf = q & 1, pb = buf ^ 1;
-        const int row = active ? sRow[base + q] : -1;
+        const int row = lRow[q];
         T v = T(0);
         if (row >= 0) {
-            T y = sIn[base + q];
-            const T a = sA[base + q], b = sB[base + q], c = sC[base + q];
+            T y = lIn[q];
+            const T a = lA[q], b = lB[q], c = lC[q];
             if (FWD) {
                 if (q == 0) {
                     if (hz) y -= c * nz;
@@ -164,7 +183,7 @@ __global__ __launch_bounds__((Band<T, K>::THREADS)) void micBandKernel(Grid g, c
                     if (zin && has[pb][cz]) y -= c * val[pb][cz];
                 }
             }
-            v = y * sInv[base + q];
+            v = y * lInv[q];
             if (owned) out[row] = v;
         }
         if (active) {
@@ -189,26 +208,42 @@ __global__ void lowerEntriesKernel(int m, const int *nbr, const T *lx, const T *
     lmz[r] = nb[4] >= 0 ? lz[nb[4]] : T(0);
 }
 
-template <class T, int K>
+// lowest and highest hyperplane of the rows in each tile of BT x BT lines
+__global__ void tileRangeKernel(Grid g, int m, const int *rows, int bt, int tilesJ, int *lo, int *hi) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const int c = rows[r];
+    const int i = c % g.ni, j = (c / g.ni) % g.nj, k = c / (g.ni * g.nj);
+    const int t = j / bt + tilesJ * (k / bt);
+    atomicMin(&lo[t], i + j + k);
+    atomicMax(&hi[t], i + j + k);
+}
+
+__global__ void fillKernel(int *a, int n, int v) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = v;
+}
+
+template <class T, int K, int BT>
 void bandSweeps(Grid g, const PressureSys &S, const int *hs, const T *r, T *y, T *z, const T *inv,
                 const T *lx, const T *ly, const T *lz, const T *lmx, const T *lmy, const T *lmz,
-                hipStream_t st) {
+                const int *tileLo, const int *tileHi, hipStream_t st) {
     const int tilesJ = (g.nj + BT - 1) / BT, tilesK = (g.nk + BT - 1) / BT;
-    const dim3 grid(tilesJ * tilesK), block(Band<T, K>::THREADS);
+    const dim3 grid(tilesJ * tilesK), block(Band<T, K, BT>::THREADS);
     const int nl = S.nLevels;
     // forward, bands [l0, l0+K) that hold rows
     for (int l0 = 0; l0 < nl; l0 += K) {
         const int l1 = l0 + K < nl ? l0 + K : nl;
         if (hs[l1] == hs[l0]) continue;
-        hipLaunchKernelGGL((micBandKernel<T, K, true>), grid, block, 0, st, g, S.cellRow, r, y, inv, lmx, lmy,
-                           lmz, l0, tilesJ);
+        hipLaunchKernelGGL((micBandKernel<T, K, BT, true>), grid, block, 0, st, g, S.cellRow, r, y, inv, lmx,
+                           lmy, lmz, tileLo, tileHi, l0, tilesJ);
     }
     // backward, bands (l0-K, l0] from the top
     for (int l0 = nl - 1; l0 >= 0; l0 -= K) {
         const int lb = l0 - K + 1 > 0 ? l0 - K + 1 : 0;
         if (hs[l0 + 1] == hs[lb]) continue;
-        hipLaunchKernelGGL((micBandKernel<T, K, false>), grid, block, 0, st, g, S.cellRow, y, z, inv, lx, ly, lz,
-                           l0, tilesJ);
+        hipLaunchKernelGGL((micBandKernel<T, K, BT, false>), grid, block, 0, st, g, S.cellRow, y, z, inv, lx, ly,
+                           lz, tileLo, tileHi, l0, tilesJ);
     }
     hipCheck(hipGetLastError(), "MIC(0) band kernel launch");
 }
@@ -220,8 +255,21 @@ int bandLaunches(const PressureSys &S, const int *hs) {
     return 2 * n;
 }
 
-// band depth: the staged band must fit in LDS (FP64: 6 levels, FP32: 8)
-constexpr int K64 = 6, K32 = 8;
+// band depth K and tile side BT: the staged band, (BT+K-1)^2 lines of K
+// cells, must fit in LDS
+#ifndef FF_BAND64_K
+#define FF_BAND64_K 8
+#endif
+#ifndef FF_BAND64_BT
+#define FF_BAND64_BT 12
+#endif
+#ifndef FF_BAND32_K
+#define FF_BAND32_K 10
+#endif
+#ifndef FF_BAND32_BT
+#define FF_BAND32_BT 14
+#endif
+constexpr int K64 = FF_BAND64_K, BT64 = FF_BAND64_BT, K32 = FF_BAND32_K, BT32 = FF_BAND32_BT;
 
 }  // namespace
 
@@ -240,16 +288,33 @@ int mic0BandLaunches(const PressureSys &S, const int *hs, bool fp64) {
     return fp64 ? bandLaunches<K64>(S, hs) : bandLaunches<K32>(S, hs);
 }
 
-void mic0ApplyBands(Grid g, const PressureSys &S, const int *hs, const double *r, double *y, double *z,
-                    hipStream_t s) {
-    if (S.m == 0) return;
-    bandSweeps<double, K64>(g, S, hs, r, y, z, S.invdiag, S.lx, S.ly, S.lz, S.lmx, S.lmy, S.lmz, s);
+int mic0BandTiles(Grid g, bool fp64) {
+    const int bt = fp64 ? BT64 : BT32;
+    return ((g.nj + bt - 1) / bt) * ((g.nk + bt - 1) / bt);
 }
 
-void mic0ApplyBandsF(Grid g, const PressureSys &S, const int *hs, const float *r, float *y, float *z,
-                     hipStream_t s) {
+void mic0BandTileRanges(Grid g, const PressureSys &S, bool fp64, int *tileLo, int *tileHi, hipStream_t s) {
+    const int bt = fp64 ? BT64 : BT32, tilesJ = (g.nj + bt - 1) / bt;
+    const int nt = mic0BandTiles(g, fp64);
+    hipLaunchKernelGGL(fillKernel, dim3((nt + 255) / 256), dim3(256), 0, s, tileLo, nt, 0x7fffffff);
+    hipLaunchKernelGGL(fillKernel, dim3((nt + 255) / 256), dim3(256), 0, s, tileHi, nt, -1);
+    if (S.m > 0)
+        hipLaunchKernelGGL(tileRangeKernel, dim3((S.m + 255) / 256), dim3(256), 0, s, g, S.m, S.rows, bt, tilesJ,
+                           tileLo, tileHi);
+}
+
+void mic0ApplyBands(Grid g, const PressureSys &S, const int *hs, const int *tileLo, const int *tileHi,
+                    const double *r, double *y, double *z, hipStream_t s) {
     if (S.m == 0) return;
-    bandSweeps<float, K32>(g, S, hs, r, y, z, S.finvdiag, S.flx, S.fly, S.flz, S.flmx, S.flmy, S.flmz, s);
+    bandSweeps<double, K64, BT64>(g, S, hs, r, y, z, S.invdiag, S.lx, S.ly, S.lz, S.lmx, S.lmy, S.lmz, tileLo,
+                                  tileHi, s);
+}
+
+void mic0ApplyBandsF(Grid g, const PressureSys &S, const int *hs, const int *tileLo, const int *tileHi,
+                     const float *r, float *y, float *z, hipStream_t s) {
+    if (S.m == 0) return;
+    bandSweeps<float, K32, BT32>(g, S, hs, r, y, z, S.finvdiag, S.flx, S.fly, S.flz, S.flmx, S.flmy, S.flmz,
+                                 tileLo, tileHi, s);
 }
 
 }  // namespace ff

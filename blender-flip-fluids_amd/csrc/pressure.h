@@ -58,6 +58,8 @@ private:
     std::vector<int> _hLevelStart;
     MicSchedule _ms;
     int _maxTiles = 0;
+    int *_tileLo = nullptr, *_tileHi = nullptr;  // band schedule: hyperplane range of each tile's rows
+    int _bandTilesMax = 0;
     std::vector<int> _hTileRows, _hOrder;
     KernelTimers *_t = nullptr;
     KernelTimers _noTimers;

@@ -36,6 +36,8 @@ void PressureSolver::allocate(Grid g) {
     }
     size_t oOrder = take(4 * (size_t)_maxTiles), oTileRows = take(4 * (size_t)_maxTiles);
     size_t oFlags = take(4 * (size_t)_maxTiles), oCounter = take(4), oError = take(4);
+    _bandTilesMax = std::max(mic0BandTiles(g, true), mic0BandTiles(g, false));
+    size_t oTileLo = take(4 * (size_t)_bandTilesMax), oTileHi = take(4 * (size_t)_bandTilesMax);
     size_t oLc = take(4 * (L + 1)), oLs = take(4 * (L + 1)), oLf = take(4 * L), oLr = take(4 * M);
     size_t oD[15], oF[14];
     for (auto &o : oD) o = take(8 * M);
@@ -60,6 +62,8 @@ void PressureSolver::allocate(Grid g) {
     _ms.flags = (int *)(b + oFlags);
     _ms.counter = (int *)(b + oCounter);
     _ms.error = (int *)(b + oError);
+    _tileLo = (int *)(b + oTileLo);
+    _tileHi = (int *)(b + oTileHi);
     double *d[15];
     for (int q = 0; q < 15; q++) d[q] = (double *)(b + oD[q]);
     _S.rhs = d[0]; _S.diag = d[1]; _S.ox = d[2]; _S.oy = d[3]; _S.oz = d[4];
@@ -137,7 +141,8 @@ void PressureSolver::applyPreconditioner(const double *r, double *z, int schedul
         buildSchedule(true, s);
         micApplyTiles(_S, _ms, r, z, s);
     } else if (schedule == 2) {
-        mic0ApplyBands(_g, _S, _hLevelStart.data(), r, _y, z, s);
+        mic0BandTileRanges(_g, _S, true, _tileLo, _tileHi, s);
+        mic0ApplyBands(_g, _S, _hLevelStart.data(), _tileLo, _tileHi, r, _y, z, s);
     } else {
         mic0Apply(_S, _hLevelStart.data(), r, z, s);
     }
@@ -147,7 +152,8 @@ void PressureSolver::applyPreconditionerF(const float *r, float *z, int schedule
         buildSchedule(false, s);
         micApplyTilesF(_S, _ms, r, z, s);
     } else if (schedule == 2) {
-        mic0ApplyBandsF(_g, _S, _hLevelStart.data(), r, _fy, z, s);
+        mic0BandTileRanges(_g, _S, false, _tileLo, _tileHi, s);
+        mic0ApplyBandsF(_g, _S, _hLevelStart.data(), _tileLo, _tileHi, r, _fy, z, s);
     } else {
         mic0ApplyF(_S, _hLevelStart.data(), r, z, s);
     }
@@ -201,6 +207,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     _levelLaunches = 0;
     for (int l = 0; l < _S.nLevels; l++) _levelLaunches += _hLevelStart[l + 1] > _hLevelStart[l];
     _bandLaunches = mic0BandLaunches(_S, _hLevelStart.data(), mode == 0);
+    mic0BandTileRanges(_g, _S, mode == 0, _tileLo, _tileHi, s);
     if (maxAbs < tol) {
         hipCheck(hipMemsetAsync(pressure, 0, sizeof(float) * N, s), "memset");
         res.success = true;
@@ -238,7 +245,7 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     mic0LowerEntries(_S, true, st);
     _t->end(T_MIC0_FACTOR, st);
     _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-    mic0ApplyBands(_g, _S, hs, _r, _y, _z, st);
+    mic0ApplyBands(_g, _S, hs, _tileLo, _tileHi, _r, _y, _z, st);
     _t->end(T_MIC0_APPLY, st);
     double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
     dot(_z, _r, m, _partial, rho, st);
@@ -261,7 +268,7 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
             return true;
         }
         _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBands(_g, _S, hs, _r, _y, z, st);
+        mic0ApplyBands(_g, _S, hs, _tileLo, _tileHi, _r, _y, z, st);
         _t->end(T_MIC0_APPLY, st);
         dot(z, _r, m, _partial, rhoNew, st);
         // beta = rhoNew / rho; z += (float)beta * s; then s <-> z
@@ -302,7 +309,7 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
         double innerTol = innerRel * hostScalar(rn, st);
         hipCheck(hipMemsetAsync(_fd, 0, sizeof(float) * m, st), "memset");
         _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBandsF(_g, _S, hs, _fr, _fy, _fz, st);
+        mic0ApplyBandsF(_g, _S, hs, _tileLo, _tileHi, _fr, _fy, _fz, st);
         _t->end(T_MIC0_APPLY, st);
         dotf(_fz, _fr, m, _partial, rho, st);
         double hrho = hostScalar(rho, st);
@@ -317,7 +324,7 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
             iters++;
             if (hostScalar(rn, st) <= innerTol) break;
             _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-            mic0ApplyBandsF(_g, _S, hs, _fr, _fy, z, st);
+            mic0ApplyBandsF(_g, _S, hs, _tileLo, _tileHi, _fr, _fy, z, st);
             _t->end(T_MIC0_APPLY, st);
             dotf(z, _fr, m, _partial, rhoNew, st);
             axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);
@@ -359,7 +366,7 @@ bool PressureSolver::pcgReliable(double tolFactor, int maxIter, double resid0, d
     resid = resid0;
     iters = 0;
     _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-    mic0ApplyBandsF(_g, _S, hs, _fr, _fy, _fz, st);
+    mic0ApplyBandsF(_g, _S, hs, _tileLo, _tileHi, _fr, _fy, _fz, st);
     _t->end(T_MIC0_APPLY, st);
     dotf(_fz, _fr, m, _partial, rho, st);
     double hrho = hostScalar(rho, st);
@@ -385,7 +392,7 @@ bool PressureSolver::pcgReliable(double tolFactor, int maxIter, double resid0, d
             sinceUpdate = resid;
         }
         _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBandsF(_g, _S, hs, _fr, _fy, z, st);
+        mic0ApplyBandsF(_g, _S, hs, _tileLo, _tileHi, _fr, _fy, z, st);
         _t->end(T_MIC0_APPLY, st);
         dotf(z, _fr, m, _partial, rhoNew, st);
         axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);
