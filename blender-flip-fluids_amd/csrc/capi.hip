@@ -494,8 +494,9 @@ int FFAMD_mic0_apply(int ni, int nj, int nk, double dx, double dt, const float *
     return guarded(err, [&] {
         checkGrid(ni, nj, nk, dx);
         if (precision != 0 && precision != 1) throw std::domain_error("precision must be 0 (FP64) or 1 (FP32)");
-        if (schedule < 0 || schedule > 2)
-            throw std::domain_error("schedule must be 0 (hyperplanes), 1 (tiles) or 2 (bands)");
+        if (schedule < 0 || schedule > 3)
+            throw std::domain_error("schedule must be 0 (hyperplanes), 1 (tiles), 2 (bands) or 3 (bands, device "
+                                    "tile check)");
         ff::Grid g{ni, nj, nk, dx};
         Scratch sc;
         size_t N = (size_t)ni * nj * nk;

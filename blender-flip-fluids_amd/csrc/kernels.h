@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "ffdevice.h"
 
 namespace ff {
@@ -91,6 +93,14 @@ void applyPressure(Grid g, double dt, const float *p, const float *phi, const fl
                    uint8_t *vv, uint8_t *vw, hipStream_t s);
 
 // ---- pressure (kernels_pressure.hip) ----
+// MIC(0) operands of one row as the band sweeps read them: invdiag and the
+// three L entries the sweep multiplies neighbour results by (forward: those of
+// the -x/-y/-z neighbour rows toward the row; backward: the row's own toward
+// +x/+y/+z)
+template <class T>
+struct alignas(4 * sizeof(T)) MicRec {
+    T inv, a, b, c;
+};
 struct PressureSys {
     int *mask;      // N+1: pressure-cell flag, then scanned into row ids
     int *rowOf;     // N+1 (exclusive scan of mask)
@@ -102,6 +112,8 @@ struct PressureSys {
     double *lmx, *lmy, *lmz;         // M: L entries of the -x/-y/-z neighbour rows toward the row
     float *finvdiag, *flx, *fly, *flz;  // FP32 copies for the mixed-precision inner solve
     float *flmx, *flmy, *flmz;
+    MicRec<double> *recFwd, *recBwd;  // M: band-sweep records (FP64)
+    MicRec<float> *frecFwd, *frecBwd;  // M: band-sweep records (FP32 copies)
     float *fdiag, *fox, *foy, *foz, *fmx, *fmy, *fmz;
     int *levelCount;  // L+1
     int *levelStart;  // L+1
@@ -125,18 +137,36 @@ void mic0ApplyF(const PressureSys &S, const int *hLevelStart, const float *r, fl
                 hipStream_t s);
 
 // ---- MIC(0) application in bands of hyperplanes (kernels_band.hip) ----
-// L entries of the lower neighbour rows (lmx.. from lx..; FP64 or the FP32 copies)
+// L entries of the lower neighbour rows (lmx.. from lx..) and the band-sweep
+// records (FP64 or the FP32 copies)
 void mic0LowerEntries(const PressureSys &S, bool fp64, hipStream_t s);
 // tiles of the band schedule and, per tile, the lowest and highest i+j+k of
 // its rows (tileLo/tileHi hold mic0BandTiles ints; once per system)
 int mic0BandTiles(Grid g, bool fp64);
 void mic0BandTileRanges(Grid g, const PressureSys &S, bool fp64, int *tileLo, int *tileHi, hipStream_t s);
+// tiles that hold rows of a launch's band, passed as a launch argument
+struct TileMask {
+    static constexpr int BITS = 2048;
+    int use;  // 0: each workgroup reads its tile's range from tileLo/tileHi instead
+    unsigned w[BITS / 32];
+};
+// the launches of one application: the first nFwd run the forward sweep
+struct BandPlan {
+    bool fp64 = true;
+    int nFwd = 0;
+    std::vector<int> l0;         // first hyperplane of each launch's band
+    std::vector<TileMask> mask;  // tiles with rows in the band
+};
+// builds the plan from the host copies of the level table and the tile ranges
+// (hTileLo/hTileHi may be null: then every band that holds rows is launched
+// and the workgroups check their tile's range on the device)
+void mic0BandPlan(Grid g, const PressureSys &S, const int *hLevelStart, bool fp64, const int *hTileLo,
+                  const int *hTileHi, BandPlan &plan);
 // z = (L L^T)^-1 r with y as the forward result (M values, distinct from r and z)
-void mic0ApplyBands(Grid g, const PressureSys &S, const int *hLevelStart, const int *tileLo,
-                    const int *tileHi, const double *r, double *y, double *z, hipStream_t s);
-void mic0ApplyBandsF(Grid g, const PressureSys &S, const int *hLevelStart, const int *tileLo,
-                     const int *tileHi, const float *r, float *y, float *z, hipStream_t s);
-int mic0BandLaunches(const PressureSys &S, const int *hLevelStart, bool fp64);
+void mic0ApplyBands(Grid g, const PressureSys &S, const BandPlan &plan, const int *tileLo, const int *tileHi,
+                    const double *r, double *y, double *z, hipStream_t s);
+void mic0ApplyBandsF(Grid g, const PressureSys &S, const BandPlan &plan, const int *tileLo, const int *tileHi,
+                     const float *r, float *y, float *z, hipStream_t s);
 void spmv(const PressureSys &S, const double *x, double *y, hipStream_t s);
 void spmvF(const PressureSys &S, const float *x, float *y, hipStream_t s);
 // y += (float)(sign * num/den) * x   (device scalars; den may be null -> scalar = num)

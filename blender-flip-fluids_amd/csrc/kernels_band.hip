@@ -28,6 +28,8 @@
 // +x, +y, +z, then scale), so the result equals the serial sweeps bit for bit.
 #include "kernels.h"
 
+#include <stdexcept>
+
 namespace ff {
 
 namespace {
@@ -42,9 +44,11 @@ struct Band {
     static constexpr int PER_THREAD = (CELLS + THREADS - 1) / THREADS;
 };
 
-// FWD: levels l0 .. l0+K-1, in = r, out = y, (ca, cb, cc) = L entries of the
-// row's -x/-y/-z neighbour rows toward it.  !FWD: levels l0 .. l0-K+1, in = y,
-// out = x, (ca, cb, cc) = the row's own L entries toward +x/+y/+z.
+// FWD: levels l0 .. l0+K-1, in = r, out = y, rec = invdiag and the L entries
+// of the row's -x/-y/-z neighbour rows toward it.  !FWD: levels l0 .. l0-K+1,
+// in = y, out = x, rec = invdiag and the row's own L entries toward +x/+y/+z.
+// The four operands of a row sit in one record, so a row costs one 16-byte
+// (FP32) or two 16-byte (FP64) loads besides its input value.
 //
 // Two phases.  Staging: the workgroup's band cells are numbered line by line
 // (K consecutive cells of one line along i), so consecutive threads read
@@ -52,10 +56,10 @@ struct Band {
 // used.  Sweep: one thread per line runs the K band levels from LDS.
 template <class T, int K, int BT, bool FWD>
 __global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid g, const int *cellRow,
-                                                                         const T *in, T *out, const T *inv,
-                                                                         const T *ca, const T *cb, const T *cc,
+                                                                         const T *in, T *out,
+                                                                         const MicRec<T> *rec,
                                                                          const int *tileLo, const int *tileHi,
-                                                                         int l0, int tilesJ) {
+                                                                         int l0, int tilesJ, TileMask mask) {
     using B = Band<T, K, BT>;
     constexpr int E = B::E, LINES = B::LINES, P = B::P;
     __shared__ T sIn[LINES * P], sInv[LINES * P], sA[LINES * P], sB[LINES * P], sC[LINES * P];
@@ -65,8 +69,13 @@ __global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid 
     const int ni = g.ni, nj = g.nj, nk = g.nk;
     const int J0 = (blockIdx.x % tilesJ) * BT, K0 = (blockIdx.x / tilesJ) * BT;
     // workgroup-uniform exit when none of the tile's own rows lies in the band
+    // (from the launch's tile mask, or from the tile's hyperplane range)
     const int lo = FWD ? l0 : l0 - (K - 1), hi = FWD ? l0 + (K - 1) : l0;
-    if (tileHi[blockIdx.x] < lo || tileLo[blockIdx.x] > hi) return;
+    if (mask.use) {
+        if (!((mask.w[blockIdx.x >> 5] >> (blockIdx.x & 31)) & 1u)) return;
+    } else if (tileHi[blockIdx.x] < lo || tileLo[blockIdx.x] > hi) {
+        return;
+    }
     const int s = FWD ? 1 : -1;  // direction of the sweep along the hyperplanes
     const int jBase = FWD ? J0 - (K - 1) : J0, kBase = FWD ? K0 - (K - 1) : K0;
 
@@ -107,15 +116,13 @@ __global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid 
     const int rz = cellRow[okz ? c0 - s * (size_t)ni * nj : 0];
     const bool first = ok0 && r0 >= 0;
     const bool hx = first && okx && rx >= 0, hy = first && oky && ry >= 0, hz = first && okz && rz >= 0;
-    T opIn[B::PER_THREAD], opInv[B::PER_THREAD], opA[B::PER_THREAD], opB[B::PER_THREAD], opC[B::PER_THREAD];
+    T opIn[B::PER_THREAD];
+    MicRec<T> op[B::PER_THREAD];
 #pragma unroll
     for (int t = 0; t < B::PER_THREAD; t++) {
         const size_t r = crow[t] >= 0 ? (size_t)crow[t] : 0;  // row 0 stands in for an absent row
         opIn[t] = in[r];
-        opInv[t] = inv[r];
-        opA[t] = ca[r];
-        opB[t] = cb[r];
-        opC[t] = cc[r];
+        op[t] = rec[r];
     }
     const T nx = out[hx ? rx : 0], ny = out[hy ? ry : 0], nz = out[hz ? rz : 0];
 #pragma unroll
@@ -125,10 +132,10 @@ __global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid 
             const int a = (c / K) * P + c % K;
             sRow[a] = crow[t];
             sIn[a] = opIn[t];
-            sInv[a] = opInv[t];
-            sA[a] = opA[t];
-            sB[a] = opB[t];
-            sC[a] = opC[t];
+            sInv[a] = op[t].inv;
+            sA[a] = op[t].a;
+            sB[a] = op[t].b;
+            sC[a] = op[t].c;
         }
     }
     __syncthreads();
@@ -196,63 +203,113 @@ __global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid 
     }
 }
 
-// L entry of each row's -x/-y/-z neighbour row toward the row (0 when absent)
+// L entry of each row's -x/-y/-z neighbour row toward the row (0 when
+// absent), and the band sweeps' records: forward {invdiag, those three},
+// backward {invdiag, the row's own lx, ly, lz}
 template <class T>
-__global__ void lowerEntriesKernel(int m, const int *nbr, const T *lx, const T *ly, const T *lz, T *lmx,
-                                   T *lmy, T *lmz) {
+__global__ void lowerEntriesKernel(int m, const int *nbr, const T *inv, const T *lx, const T *ly, const T *lz,
+                                   T *lmx, T *lmy, T *lmz, MicRec<T> *recFwd, MicRec<T> *recBwd) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= m) return;
     const int *nb = nbr + (size_t)6 * r;
-    lmx[r] = nb[0] >= 0 ? lx[nb[0]] : T(0);
-    lmy[r] = nb[2] >= 0 ? ly[nb[2]] : T(0);
-    lmz[r] = nb[4] >= 0 ? lz[nb[4]] : T(0);
+    const T a = nb[0] >= 0 ? lx[nb[0]] : T(0);
+    const T b = nb[2] >= 0 ? ly[nb[2]] : T(0);
+    const T c = nb[4] >= 0 ? lz[nb[4]] : T(0);
+    lmx[r] = a;
+    lmy[r] = b;
+    lmz[r] = c;
+    const T d = inv[r];
+    recFwd[r] = MicRec<T>{d, a, b, c};
+    recBwd[r] = MicRec<T>{d, lx[r], ly[r], lz[r]};
 }
 
-// lowest and highest hyperplane of the rows in each tile of BT x BT lines
-__global__ void tileRangeKernel(Grid g, int m, const int *rows, int bt, int tilesJ, int *lo, int *hi) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= m) return;
-    const int c = rows[r];
-    const int i = c % g.ni, j = (c / g.ni) % g.nj, k = c / (g.ni * g.nj);
-    const int t = j / bt + tilesJ * (k / bt);
-    atomicMin(&lo[t], i + j + k);
-    atomicMax(&hi[t], i + j + k);
-}
-
-__global__ void fillKernel(int *a, int n, int v) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) a[q] = v;
+// lowest and highest hyperplane of the rows in each tile of BT x BT lines:
+// one workgroup per tile reads the tile's cells line by line (no atomics)
+__global__ __launch_bounds__(256) void tileRangeKernel(Grid g, const int *cellRow, int bt, int tilesJ, int *tileLo,
+                                                       int *tileHi) {
+    __shared__ int sLo[256], sHi[256];
+    const int ni = g.ni, nj = g.nj, nk = g.nk;
+    const int J0 = (blockIdx.x % tilesJ) * bt, K0 = (blockIdx.x / tilesJ) * bt;
+    const int lj = nj - J0 < bt ? nj - J0 : bt, lk = nk - K0 < bt ? nk - K0 : bt;
+    const long long cells = (long long)lj * lk * ni;
+    int lo = 0x7fffffff, hi = -1;
+    for (long long q = threadIdx.x; q < cells; q += blockDim.x) {
+        const int i = (int)(q % ni);
+        const int line = (int)(q / ni);
+        const int j = J0 + line % lj, k = K0 + line / lj;
+        if (cellRow[i + (size_t)ni * (j + (size_t)nj * k)] >= 0) {
+            lo = min(lo, i + j + k);
+            hi = max(hi, i + j + k);
+        }
+    }
+    sLo[threadIdx.x] = lo;
+    sHi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) {
+            sLo[threadIdx.x] = min(sLo[threadIdx.x], sLo[threadIdx.x + o]);
+            sHi[threadIdx.x] = max(sHi[threadIdx.x], sHi[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tileLo[blockIdx.x] = sLo[0];
+        tileHi[blockIdx.x] = sHi[0];
+    }
 }
 
 template <class T, int K, int BT>
-void bandSweeps(Grid g, const PressureSys &S, const int *hs, const T *r, T *y, T *z, const T *inv,
-                const T *lx, const T *ly, const T *lz, const T *lmx, const T *lmy, const T *lmz,
-                const int *tileLo, const int *tileHi, hipStream_t st) {
+void bandSweeps(Grid g, const PressureSys &S, const BandPlan &plan, const T *r, T *y, T *z,
+                const MicRec<T> *recFwd, const MicRec<T> *recBwd, const int *tileLo, const int *tileHi,
+                hipStream_t st) {
     const int tilesJ = (g.nj + BT - 1) / BT, tilesK = (g.nk + BT - 1) / BT;
     const dim3 grid(tilesJ * tilesK), block(Band<T, K, BT>::THREADS);
-    const int nl = S.nLevels;
-    // forward, bands [l0, l0+K) that hold rows
-    for (int l0 = 0; l0 < nl; l0 += K) {
-        const int l1 = l0 + K < nl ? l0 + K : nl;
-        if (hs[l1] == hs[l0]) continue;
-        hipLaunchKernelGGL((micBandKernel<T, K, BT, true>), grid, block, 0, st, g, S.cellRow, r, y, inv, lmx,
-                           lmy, lmz, tileLo, tileHi, l0, tilesJ);
-    }
-    // backward, bands (l0-K, l0] from the top
-    for (int l0 = nl - 1; l0 >= 0; l0 -= K) {
-        const int lb = l0 - K + 1 > 0 ? l0 - K + 1 : 0;
-        if (hs[l0 + 1] == hs[lb]) continue;
-        hipLaunchKernelGGL((micBandKernel<T, K, BT, false>), grid, block, 0, st, g, S.cellRow, y, z, inv, lx, ly,
-                           lz, tileLo, tileHi, l0, tilesJ);
+    for (size_t q = 0; q < plan.l0.size(); q++) {
+        if ((int)q < plan.nFwd)
+            hipLaunchKernelGGL((micBandKernel<T, K, BT, true>), grid, block, 0, st, g, S.cellRow, r, y, recFwd,
+                               tileLo, tileHi, plan.l0[q], tilesJ, plan.mask[q]);
+        else
+            hipLaunchKernelGGL((micBandKernel<T, K, BT, false>), grid, block, 0, st, g, S.cellRow, y, z, recBwd,
+                               tileLo, tileHi, plan.l0[q], tilesJ, plan.mask[q]);
     }
     hipCheck(hipGetLastError(), "MIC(0) band kernel launch");
 }
 
-template <int K>
-int bandLaunches(const PressureSys &S, const int *hs) {
-    int n = 0;
-    for (int l0 = 0; l0 < S.nLevels; l0 += K) n += hs[(l0 + K < S.nLevels ? l0 + K : S.nLevels)] > hs[l0];
-    return 2 * n;
+// launches of the two sweeps: the bands of K hyperplanes that hold rows of a
+// tile (all bands that hold rows when the tile ranges are unknown)
+template <int K, int BT>
+void bandPlan(Grid g, const PressureSys &S, const int *hs, const int *hLo, const int *hHi, BandPlan &plan) {
+    const int tilesJ = (g.nj + BT - 1) / BT, tilesK = (g.nk + BT - 1) / BT, nt = tilesJ * tilesK;
+    const bool useMask = hLo && hHi && nt <= TileMask::BITS;
+    const int nl = S.nLevels;
+    plan.l0.clear();
+    plan.mask.clear();
+    auto add = [&](int l0, int lo, int hi) {
+        TileMask m{};
+        if (useMask) {
+            m.use = 1;
+            bool any = false;
+            for (int t = 0; t < nt; t++)
+                if (!(hHi[t] < lo || hLo[t] > hi)) {
+                    m.w[t >> 5] |= 1u << (t & 31);
+                    any = true;
+                }
+            if (!any) return;
+        }
+        plan.l0.push_back(l0);
+        plan.mask.push_back(m);
+    };
+    // forward, bands [l0, l0+K) that hold rows
+    for (int l0 = 0; l0 < nl; l0 += K) {
+        const int l1 = l0 + K < nl ? l0 + K : nl;
+        if (hs[l1] > hs[l0]) add(l0, l0, l0 + K - 1);
+    }
+    plan.nFwd = (int)plan.l0.size();
+    // backward, bands (l0-K, l0] from the top
+    for (int l0 = nl - 1; l0 >= 0; l0 -= K) {
+        const int lb = l0 - K + 1 > 0 ? l0 - K + 1 : 0;
+        if (hs[l0 + 1] > hs[lb]) add(l0, l0 - K + 1, l0);
+    }
 }
 
 // band depth K and tile side BT: the staged band, (BT+K-1)^2 lines of K
@@ -277,15 +334,20 @@ void mic0LowerEntries(const PressureSys &S, bool fp64, hipStream_t s) {
     if (S.m == 0) return;
     dim3 grid((S.m + 255) / 256), block(256);
     if (fp64)
-        hipLaunchKernelGGL(lowerEntriesKernel<double>, grid, block, 0, s, S.m, S.nbr, S.lx, S.ly, S.lz, S.lmx,
-                           S.lmy, S.lmz);
+        hipLaunchKernelGGL(lowerEntriesKernel<double>, grid, block, 0, s, S.m, S.nbr, S.invdiag, S.lx, S.ly, S.lz,
+                           S.lmx, S.lmy, S.lmz, S.recFwd, S.recBwd);
     else
-        hipLaunchKernelGGL(lowerEntriesKernel<float>, grid, block, 0, s, S.m, S.nbr, S.flx, S.fly, S.flz,
-                           S.flmx, S.flmy, S.flmz);
+        hipLaunchKernelGGL(lowerEntriesKernel<float>, grid, block, 0, s, S.m, S.nbr, S.finvdiag, S.flx, S.fly,
+                           S.flz, S.flmx, S.flmy, S.flmz, S.frecFwd, S.frecBwd);
 }
 
-int mic0BandLaunches(const PressureSys &S, const int *hs, bool fp64) {
-    return fp64 ? bandLaunches<K64>(S, hs) : bandLaunches<K32>(S, hs);
+void mic0BandPlan(Grid g, const PressureSys &S, const int *hLevelStart, bool fp64, const int *hTileLo,
+                  const int *hTileHi, BandPlan &plan) {
+    plan.fp64 = fp64;
+    if (fp64)
+        bandPlan<K64, BT64>(g, S, hLevelStart, hTileLo, hTileHi, plan);
+    else
+        bandPlan<K32, BT32>(g, S, hLevelStart, hTileLo, hTileHi, plan);
 }
 
 int mic0BandTiles(Grid g, bool fp64) {
@@ -296,25 +358,21 @@ int mic0BandTiles(Grid g, bool fp64) {
 void mic0BandTileRanges(Grid g, const PressureSys &S, bool fp64, int *tileLo, int *tileHi, hipStream_t s) {
     const int bt = fp64 ? BT64 : BT32, tilesJ = (g.nj + bt - 1) / bt;
     const int nt = mic0BandTiles(g, fp64);
-    hipLaunchKernelGGL(fillKernel, dim3((nt + 255) / 256), dim3(256), 0, s, tileLo, nt, 0x7fffffff);
-    hipLaunchKernelGGL(fillKernel, dim3((nt + 255) / 256), dim3(256), 0, s, tileHi, nt, -1);
-    if (S.m > 0)
-        hipLaunchKernelGGL(tileRangeKernel, dim3((S.m + 255) / 256), dim3(256), 0, s, g, S.m, S.rows, bt, tilesJ,
-                           tileLo, tileHi);
+    hipLaunchKernelGGL(tileRangeKernel, dim3(nt), dim3(256), 0, s, g, S.cellRow, bt, tilesJ, tileLo, tileHi);
 }
 
-void mic0ApplyBands(Grid g, const PressureSys &S, const int *hs, const int *tileLo, const int *tileHi,
+void mic0ApplyBands(Grid g, const PressureSys &S, const BandPlan &plan, const int *tileLo, const int *tileHi,
                     const double *r, double *y, double *z, hipStream_t s) {
     if (S.m == 0) return;
-    bandSweeps<double, K64, BT64>(g, S, hs, r, y, z, S.invdiag, S.lx, S.ly, S.lz, S.lmx, S.lmy, S.lmz, tileLo,
-                                  tileHi, s);
+    if (!plan.fp64) throw std::runtime_error("MIC(0) bands: the launch plan was built for FP32");
+    bandSweeps<double, K64, BT64>(g, S, plan, r, y, z, S.recFwd, S.recBwd, tileLo, tileHi, s);
 }
 
-void mic0ApplyBandsF(Grid g, const PressureSys &S, const int *hs, const int *tileLo, const int *tileHi,
+void mic0ApplyBandsF(Grid g, const PressureSys &S, const BandPlan &plan, const int *tileLo, const int *tileHi,
                      const float *r, float *y, float *z, hipStream_t s) {
     if (S.m == 0) return;
-    bandSweeps<float, K32, BT32>(g, S, hs, r, y, z, S.finvdiag, S.flx, S.fly, S.flz, S.flmx, S.flmy, S.flmz,
-                                 tileLo, tileHi, s);
+    if (plan.fp64) throw std::runtime_error("MIC(0) bands: the launch plan was built for FP64");
+    bandSweeps<float, K32, BT32>(g, S, plan, r, y, z, S.frecFwd, S.frecBwd, tileLo, tileHi, s);
 }
 
 }  // namespace ff

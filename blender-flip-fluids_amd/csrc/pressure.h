@@ -42,7 +42,8 @@ public:
     // one preconditioner application z = (L L^T)^-1 r on the current factor
     // (after solve()); FP64 or the FP32 copy.  schedule 0 = one launch per
     // hyperplane, 1 = tile wavefront (switched off: raises an error),
-    // 2 = bands of hyperplanes (the solver's).
+    // 2 = bands of hyperplanes (the solver's), 3 = the same bands with the tile
+    // ranges checked on the device instead of passed in the launches.
     void applyPreconditioner(const double *r, double *z, int schedule, hipStream_t s);
     void applyPreconditionerF(const float *r, float *z, int schedule, hipStream_t s);
     // throws when a preconditioner launch gave up waiting (waits for the stream)
@@ -60,12 +61,15 @@ private:
     int _maxTiles = 0;
     int *_tileLo = nullptr, *_tileHi = nullptr;  // band schedule: hyperplane range of each tile's rows
     int _bandTilesMax = 0;
+    std::vector<int> _hTileLo, _hTileHi;  // host copies of the tile ranges
+    BandPlan _plan;                       // band launches of the current system
     std::vector<int> _hTileRows, _hOrder;
     KernelTimers *_t = nullptr;
     KernelTimers _noTimers;
     int _levelLaunches = 0, _bandLaunches = 0;
 
     double hostScalar(const double *d, hipStream_t s);
+    void enqueueTileRanges(bool fp64, hipStream_t s);
     // tile schedule of the preconditioner application for this system
     void buildSchedule(bool fp64, hipStream_t s);
     bool pcgFP64(double tolFactor, int maxIter, double resid0, int &iters, double &resid, hipStream_t s);

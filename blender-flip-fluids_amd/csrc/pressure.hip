@@ -45,6 +45,9 @@ void PressureSolver::allocate(Grid g) {
     size_t oV[5], oFV[6];
     for (auto &o : oV) o = take(8 * M);
     for (auto &o : oFV) o = take(4 * M);
+    size_t oRec64[2], oRec32[2];
+    for (auto &o : oRec64) o = take(sizeof(MicRec<double>) * M);
+    for (auto &o : oRec32) o = take(sizeof(MicRec<float>) * M);
     size_t oPart = take(8 * 1024), oScal = take(8 * 16);
     hipCheck(hipMalloc(&_block, bytes), "hipMalloc pressure");
     char *b = (char *)_block;
@@ -64,6 +67,8 @@ void PressureSolver::allocate(Grid g) {
     _ms.error = (int *)(b + oError);
     _tileLo = (int *)(b + oTileLo);
     _tileHi = (int *)(b + oTileHi);
+    _hTileLo.assign(_bandTilesMax, 0);
+    _hTileHi.assign(_bandTilesMax, 0);
     double *d[15];
     for (int q = 0; q < 15; q++) d[q] = (double *)(b + oD[q]);
     _S.rhs = d[0]; _S.diag = d[1]; _S.ox = d[2]; _S.oy = d[3]; _S.oz = d[4];
@@ -76,6 +81,10 @@ void PressureSolver::allocate(Grid g) {
     _S.fdiag = f[4]; _S.fox = f[5]; _S.foy = f[6]; _S.foz = f[7];
     _S.fmx = f[8]; _S.fmy = f[9]; _S.fmz = f[10];
     _S.flmx = f[11]; _S.flmy = f[12]; _S.flmz = f[13];
+    _S.recFwd = (MicRec<double> *)(b + oRec64[0]);
+    _S.recBwd = (MicRec<double> *)(b + oRec64[1]);
+    _S.frecFwd = (MicRec<float> *)(b + oRec32[0]);
+    _S.frecBwd = (MicRec<float> *)(b + oRec32[1]);
     _x = (double *)(b + oV[0]); _r = (double *)(b + oV[1]); _z = (double *)(b + oV[2]); _s = (double *)(b + oV[3]);
     _y = (double *)(b + oV[4]);
     _fr = (float *)(b + oFV[0]); _fz = (float *)(b + oFV[1]); _fs = (float *)(b + oFV[2]);
@@ -129,6 +138,15 @@ void PressureSolver::buildSchedule(bool fp64, hipStream_t s) {
     _ms.epoch = 0;
 }
 
+// hyperplane range of each band tile's rows, on the device and (after the
+// stream's next synchronisation) in _hTileLo/_hTileHi
+void PressureSolver::enqueueTileRanges(bool fp64, hipStream_t s) {
+    const int nt = mic0BandTiles(_g, fp64);
+    mic0BandTileRanges(_g, _S, fp64, _tileLo, _tileHi, s);
+    hipCheck(hipMemcpyAsync(_hTileLo.data(), _tileLo, sizeof(int) * nt, hipMemcpyDeviceToHost, s), "D2H tiles");
+    hipCheck(hipMemcpyAsync(_hTileHi.data(), _tileHi, sizeof(int) * nt, hipMemcpyDeviceToHost, s), "D2H tiles");
+}
+
 void PressureSolver::checkError(hipStream_t s) {
     int e = 0;
     hipCheck(hipMemcpyAsync(&e, _ms.error, sizeof(int), hipMemcpyDeviceToHost, s), "D2H error word");
@@ -140,9 +158,13 @@ void PressureSolver::applyPreconditioner(const double *r, double *z, int schedul
     if (schedule == 1) {
         buildSchedule(true, s);
         micApplyTiles(_S, _ms, r, z, s);
-    } else if (schedule == 2) {
-        mic0BandTileRanges(_g, _S, true, _tileLo, _tileHi, s);
-        mic0ApplyBands(_g, _S, _hLevelStart.data(), _tileLo, _tileHi, r, _y, z, s);
+    } else if (schedule == 2 || schedule == 3) {
+        enqueueTileRanges(true, s);
+        hipCheck(hipStreamSynchronize(s), "sync");
+        const bool host = schedule == 2;
+        mic0BandPlan(_g, _S, _hLevelStart.data(), true, host ? _hTileLo.data() : nullptr,
+                     host ? _hTileHi.data() : nullptr, _plan);
+        mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, r, _y, z, s);
     } else {
         mic0Apply(_S, _hLevelStart.data(), r, z, s);
     }
@@ -151,9 +173,13 @@ void PressureSolver::applyPreconditionerF(const float *r, float *z, int schedule
     if (schedule == 1) {
         buildSchedule(false, s);
         micApplyTilesF(_S, _ms, r, z, s);
-    } else if (schedule == 2) {
-        mic0BandTileRanges(_g, _S, false, _tileLo, _tileHi, s);
-        mic0ApplyBandsF(_g, _S, _hLevelStart.data(), _tileLo, _tileHi, r, _fy, z, s);
+    } else if (schedule == 2 || schedule == 3) {
+        enqueueTileRanges(false, s);
+        hipCheck(hipStreamSynchronize(s), "sync");
+        const bool host = schedule == 2;
+        mic0BandPlan(_g, _S, _hLevelStart.data(), false, host ? _hTileLo.data() : nullptr,
+                     host ? _hTileHi.data() : nullptr, _plan);
+        mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, r, _fy, z, s);
     } else {
         mic0ApplyF(_S, _hLevelStart.data(), r, z, s);
     }
@@ -196,18 +222,19 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     _hLevelStart.resize(_S.nLevels + 1);
     hipCheck(hipMemcpyAsync(_hLevelStart.data(), _S.levelStart, sizeof(int) * (_S.nLevels + 1),
                             hipMemcpyDeviceToHost, s), "D2H levels");
+    enqueueTileRanges(mode == 0, s);
     double maxAbs = 0.0;
     if (m > 0) {
         absMax(_S.rhs, m, _partial, _scal + 1, s);
-        maxAbs = hostScalar(_scal + 1, s);  // also waits for the level table
+        maxAbs = hostScalar(_scal + 1, s);  // also waits for the level table and the tile ranges
     } else {
         hipCheck(hipStreamSynchronize(s), "sync");
     }
     if (afterAssembly) hipCheck(hipEventRecord(afterAssembly, s), "event");
     _levelLaunches = 0;
     for (int l = 0; l < _S.nLevels; l++) _levelLaunches += _hLevelStart[l + 1] > _hLevelStart[l];
-    _bandLaunches = mic0BandLaunches(_S, _hLevelStart.data(), mode == 0);
-    mic0BandTileRanges(_g, _S, mode == 0, _tileLo, _tileHi, s);
+    mic0BandPlan(_g, _S, _hLevelStart.data(), mode == 0, _hTileLo.data(), _hTileHi.data(), _plan);
+    _bandLaunches = (int)_plan.l0.size();
     if (maxAbs < tol) {
         hipCheck(hipMemsetAsync(pressure, 0, sizeof(float) * N, s), "memset");
         res.success = true;
@@ -245,7 +272,7 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     mic0LowerEntries(_S, true, st);
     _t->end(T_MIC0_FACTOR, st);
     _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-    mic0ApplyBands(_g, _S, hs, _tileLo, _tileHi, _r, _y, _z, st);
+    mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, _r, _y, _z, st);
     _t->end(T_MIC0_APPLY, st);
     double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
     dot(_z, _r, m, _partial, rho, st);
@@ -268,7 +295,7 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
             return true;
         }
         _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBands(_g, _S, hs, _tileLo, _tileHi, _r, _y, z, st);
+        mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, _r, _y, z, st);
         _t->end(T_MIC0_APPLY, st);
         dot(z, _r, m, _partial, rhoNew, st);
         // beta = rhoNew / rho; z += (float)beta * s; then s <-> z
@@ -309,7 +336,7 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
         double innerTol = innerRel * hostScalar(rn, st);
         hipCheck(hipMemsetAsync(_fd, 0, sizeof(float) * m, st), "memset");
         _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBandsF(_g, _S, hs, _tileLo, _tileHi, _fr, _fy, _fz, st);
+        mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, _fz, st);
         _t->end(T_MIC0_APPLY, st);
         dotf(_fz, _fr, m, _partial, rho, st);
         double hrho = hostScalar(rho, st);
@@ -324,7 +351,7 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
             iters++;
             if (hostScalar(rn, st) <= innerTol) break;
             _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-            mic0ApplyBandsF(_g, _S, hs, _tileLo, _tileHi, _fr, _fy, z, st);
+            mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, z, st);
             _t->end(T_MIC0_APPLY, st);
             dotf(z, _fr, m, _partial, rhoNew, st);
             axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);
@@ -366,7 +393,7 @@ bool PressureSolver::pcgReliable(double tolFactor, int maxIter, double resid0, d
     resid = resid0;
     iters = 0;
     _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-    mic0ApplyBandsF(_g, _S, hs, _tileLo, _tileHi, _fr, _fy, _fz, st);
+    mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, _fz, st);
     _t->end(T_MIC0_APPLY, st);
     dotf(_fz, _fr, m, _partial, rho, st);
     double hrho = hostScalar(rho, st);
@@ -392,7 +419,7 @@ bool PressureSolver::pcgReliable(double tolFactor, int maxIter, double resid0, d
             sinceUpdate = resid;
         }
         _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBandsF(_g, _S, hs, _tileLo, _tileHi, _fr, _fy, z, st);
+        mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, z, st);
         _t->end(T_MIC0_APPLY, st);
         dotf(z, _fr, m, _partial, rhoNew, st);
         axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);

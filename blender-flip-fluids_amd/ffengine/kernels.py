@@ -127,7 +127,8 @@ def mic0_apply(n3, dx, dt, uvw, weights, liquid_phi, r, precision=0, schedule=0)
     """One MIC(0) application z = (L L^T)^-1 r on the pressure system of these
     inputs (row order as pressure_system); precision 0 = FP64, 1 = FP32;
     schedule 0 = one launch per hyperplane, 1 = tile wavefront (switched off),
-    2 = bands of hyperplanes (the solver's)."""
+    2 = bands of hyperplanes (the solver's), 3 = bands with the tile check on
+    the device."""
     ni, nj, nk = n3
     u, v, w = (_f32(a) for a in uvw)
     wu, wv, ww = (_f32(a) for a in weights)

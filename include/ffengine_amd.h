@@ -227,8 +227,11 @@ int FFAMD_pressure_system(int isize, int jsize, int ksize, double dx, double dt,
  * 0 = FP64, 1 = the FP32 copy of the factor used by the mixed-precision solve
  * (r rounded to float, z widened).  schedule 0 = one launch per hyperplane,
  * 1 = the tile-wavefront kernel, which is switched off and fails with an
- * error, 2 = bands of hyperplanes (the solver's).  The inputs must give a
- * non-zero divergence.  Returns M. */
+ * error, 2 = bands of hyperplanes (the solver's: the launches name the tiles
+ * that hold rows of their band), 3 = the same bands with each workgroup
+ * reading its tile's hyperplane range on the device (used when the tiles
+ * outnumber a launch's tile mask).  The inputs must give a non-zero
+ * divergence.  Returns M. */
 int FFAMD_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, const float *u,
                      const float *v, const float *w, const float *wu, const float *wv,
                      const float *ww, const float *liquid_phi, int precision, int schedule,
