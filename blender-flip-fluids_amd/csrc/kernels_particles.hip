@@ -156,16 +156,33 @@ __global__ void sdfPostKernel(Grid g, const float *sp, float *phi) {
 // particle index).  The candidate cells' sorted lists are merged by particle
 // index, so the float sums run in the reference's order and no atomics are
 // used.
-template <int DIR>
-__global__ __launch_bounds__(256) void p2gKernel(Grid g, const int *offsets, const int *perm,
-                                                 const float *spx, const float *spy,
-                                                 const float *spz, const float *svel,
-                                                 double radius, float *field, uint8_t *valid) {
-    const int wi = g.ni + (DIR == 0), wj = g.nj + (DIR == 1), wk = g.nk + (DIR == 2);
-    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t nf = (size_t)wi * wj * wk;
-    if (f >= nf) return;
-    int fi = (int)(f % wi), fj = (int)((f / wi) % wj), fk = (int)(f / ((size_t)wi * wj));
+//
+// One workgroup handles a tile of 8 x 8 x 4 faces of each of the three
+// directions.  The particles of the 10 x 10 x 6 cells those faces can reach
+// (the tile grown by one cell on each side) are staged in LDS once, with
+// their positions, three velocity components and particle index; then one
+// thread per face merges its 18 candidate cells from LDS.  A tile whose cells
+// hold more particles than LDS does reads them from the sorted arrays instead
+// (same merge, same order, same bits).
+constexpr int P2T_I = 8, P2T_J = 8, P2T_K = 4;  // faces of a tile per direction
+constexpr int P2R_I = P2T_I + 2, P2R_J = P2T_J + 2, P2R_K = P2T_K + 2;
+constexpr int P2R_CELLS = P2R_I * P2R_J * P2R_K;  // staged cells
+constexpr int P2_FACES = P2T_I * P2T_J * P2T_K;   // faces per direction and tile
+constexpr int P2_THREADS = 768;                   // one face per thread
+constexpr int P2G_CAP = 5500;                     // staged particles (LDS: 28 bytes each)
+
+// Face (fi, fj, fk) of direction dir: sum and weight sum over the particles
+// of its 18 candidate cells (range(c, b, e) gives cell c's particles [b, e)
+// as indices into x/y/z/vel/pidx), merged in ascending particle index.
+// The merge takes a full minimum over the 18 list heads only when the list it
+// draws from stops being the smallest: it then keeps drawing from that list
+// while its next head is below the second smallest head, which gives the same
+// order with one scan per run of a list.  The 18-entry cursor and head arrays
+// are only addressed by unrolled constant indices (selects), so the merge
+// uses no dynamically indexed registers or private memory.
+template <int dir, class F, class I, class R>
+__device__ __forceinline__ void p2gFace(Grid g, int fi, int fj, int fk, R range, F x, F y, F z, F vel,
+                                        I pidx, double radius, float *field, uint8_t *valid, size_t f) {
     const float eps = 1e-6f;
     const float r = (float)radius;
     const float rsq = r * r;
@@ -173,61 +190,189 @@ __global__ __launch_bounds__(256) void p2gKernel(Grid g, const int *offsets, con
     const float coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
     const float coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
     const double h = 0.5 * g.dx;
-    const V3 offd = DIR == 0 ? mk(0.0, h, h) : (DIR == 1 ? mk(h, 0.0, h) : mk(h, h, 0.0));
+    const V3 offd = dir == 0 ? mk(0.0, h, h) : (dir == 1 ? mk(h, 0.0, h) : mk(h, h, 0.0));
     const int bi = fi / BW, bj = fj / BW, bk = fk / BW;
     const V3 offb = indexToPos(bi, bj, bk, BW * g.dx);
     const V3 gpos = indexToPos(fi - bi * BW, fj - bj * BW, fk - bk * BW, g.dx);
-    // candidate cells: 2 along the face normal, 3 across
-    const int ci0 = DIR == 0 ? fi - 1 : fi - 1, ci1 = DIR == 0 ? fi : fi + 1;
-    const int cj0 = fj - 1, cj1 = DIR == 1 ? fj : fj + 1;
-    const int ck0 = fk - 1, ck1 = DIR == 2 ? fk : fk + 1;
-    int cur[18], end[18], head[18];
-    int nc = 0;
-    for (int k = ck0; k <= ck1; k++)
-        for (int j = cj0; j <= cj1; j++)
-            for (int i = ci0; i <= ci1; i++) {
-                if (!inRange(i, j, k, g.ni, g.nj, g.nk)) continue;
-                size_t c = flat(i, j, k, g.ni, g.nj);
-                cur[nc] = offsets[c];
-                end[nc] = offsets[c + 1];
-                nc++;
-            }
     auto d2of = [&](int s) {
-        V3 p = sub(sub(V3{spx[s], spy[s], spz[s]}, offd), offb);
+        V3 p = sub(sub(V3{x[s], y[s], z[s]}, offd), offb);
         V3 v = sub(gpos, p);
         return dot3(v, v);
     };
-    auto advance = [&](int c) {
-        while (cur[c] < end[c]) {
-            if (d2of(cur[c]) < rsq) {
-                head[c] = perm[cur[c]];
-                return;
-            }
-            cur[c]++;
+    // first particle at or after c within r: its particle index (or none) and distance
+    auto next = [&](int &c, int e, float &d2) {
+        while (c < e) {
+            d2 = d2of(c);
+            if (d2 < rsq) return pidx[c];
+            c++;
         }
-        head[c] = 0x7fffffff;
+        return 0x7fffffff;
     };
-    for (int c = 0; c < nc; c++) advance(c);
+    auto put = [](int(&a)[18], int c, int v) {
+#pragma unroll
+        for (int q = 0; q < 18; q++) a[q] = q == c ? v : a[q];
+    };
+    int cur[18], end[18], head[18];
+#pragma unroll 1
+    for (int c = 0; c < 18; c++) {
+        int b, e;
+        range(c, b, e);
+        float d2;
+        const int hh = next(b, e, d2);
+        put(cur, c, b);
+        put(end, c, e);
+        put(head, c, hh);
+    }
     float sum = 0.0f, wsum = 0.0f;
     while (true) {
-        int best = -1, bidx = 0x7fffffff;
-        for (int c = 0; c < nc; c++)
-            if (head[c] < bidx) {
-                bidx = head[c];
+        int best = -1, bidx = 0x7fffffff, second = 0x7fffffff, bcur = 0, bend = 0;
+#pragma unroll
+        for (int c = 0; c < 18; c++) {
+            const int hc = head[c];
+            if (hc < bidx) {
+                second = bidx;
+                bidx = hc;
                 best = c;
+                bcur = cur[c];
+                bend = end[c];
+            } else if (hc < second) {
+                second = hc;
             }
+        }
         if (best < 0) break;
-        int s = cur[best];
-        float d2 = d2of(s);
-        float weight = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
-        sum += weight * svel[s];
-        wsum += weight;
-        cur[best]++;
-        advance(best);
+        float d2 = d2of(bcur);
+        do {
+            const float weight = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
+            sum += weight * vel[bcur];
+            wsum += weight;
+            bcur++;
+            bidx = next(bcur, bend, d2);
+        } while (bidx < second);
+        put(cur, best, bcur);
+        put(head, best, bidx);
     }
     if (wsum > eps) sum /= wsum;
     field[f] = sum;
     valid[f] = wsum > eps ? 1 : 0;
+}
+
+struct P2GOut {
+    float *f[3];
+    uint8_t *valid[3];
+};
+
+// face a of the tile at (I0, J0, K0) in direction dir, from the staged
+// particles (or the sorted arrays when the tile's particles overflowed LDS)
+template <int dir>
+__device__ __forceinline__ void p2gTileFace(Grid g, int a, int I0, int J0, int K0, int total, bool staged,
+                                            const int *sStart, const int *sGlob, const float *sx, const float *sy,
+                                            const float *sz, const float *sv, const int *sp, const float *spx,
+                                            const float *spy, const float *spz, const float *svel, const int *perm,
+                                            double radius, P2GOut out) {
+    const int fi = I0 + a % P2T_I, fj = J0 + (a / P2T_I) % P2T_J, fk = K0 + a / (P2T_I * P2T_J);
+    const int wi = g.ni + (dir == 0), wj = g.nj + (dir == 1), wk = g.nk + (dir == 2);
+    if (fi >= wi || fj >= wj || fk >= wk) return;
+    const size_t f = flat(fi, fj, fk, wi, wj);
+    if (total == 0) {  // no particle can reach the tile's faces
+        out.f[dir][f] = 0.0f;
+        out.valid[dir][f] = 0;
+        return;
+    }
+    // candidate cells: 2 along the face normal, 3 across; candidate c of the
+    // box, i fastest
+    constexpr int NI = dir == 0 ? 2 : 3, NJ = dir == 1 ? 2 : 3;
+    const int ci0 = fi - 1, cj0 = fj - 1, ck0 = fk - 1;
+    auto range = [&](int c, int &b, int &e) {
+        const int i = ci0 + c % NI, j = cj0 + (c / NI) % NJ, k = ck0 + c / (NI * NJ);
+        b = 0;
+        e = 0;
+        if (inRange(i, j, k, g.ni, g.nj, g.nk)) {
+            const int rc = (i - (I0 - 1)) + P2R_I * ((j - (J0 - 1)) + P2R_J * (k - (K0 - 1)));
+            b = staged ? sStart[rc] : sGlob[rc];
+            e = b + (sStart[rc + 1] - sStart[rc]);
+        }
+    };
+    if (staged)
+        p2gFace<dir>(g, fi, fj, fk, range, sx, sy, sz, sv, sp, radius, out.f[dir], out.valid[dir], f);
+    else
+        p2gFace<dir>(g, fi, fj, fk, range, spx, spy, spz, svel, perm, radius, out.f[dir], out.valid[dir], f);
+}
+
+__global__ __launch_bounds__(P2_THREADS) void p2gTileKernel(Grid g, const int *offsets, const int *perm,
+                                                            const float *spx, const float *spy, const float *spz,
+                                                            const float *svx, const float *svy, const float *svz,
+                                                            double radius, P2GOut out, int tilesI, int tilesJ) {
+    __shared__ int sStart[P2R_CELLS + 1], sGlob[P2R_CELLS], sWave[P2_THREADS / 64];
+    __shared__ float sx[P2G_CAP], sy[P2G_CAP], sz[P2G_CAP], sv[3][P2G_CAP];
+    __shared__ int sp[P2G_CAP];
+    const int t = threadIdx.x;
+    const int I0 = (blockIdx.x % tilesI) * P2T_I, J0 = ((blockIdx.x / tilesI) % tilesJ) * P2T_J,
+              K0 = (blockIdx.x / (tilesI * tilesJ)) * P2T_K;
+    // particle counts of the staged cells (cells 2t and 2t+1 for thread t),
+    // then an exclusive scan over them in cell order
+    static_assert(2 * P2_THREADS >= P2R_CELLS, "two staged cells per thread");
+    int cnt[2] = {0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int rc = 2 * t + h;
+        if (rc < P2R_CELLS) {
+            const int i = I0 - 1 + rc % P2R_I, j = J0 - 1 + (rc / P2R_I) % P2R_J, k = K0 - 1 + rc / (P2R_I * P2R_J);
+            int b = 0;
+            if (inRange(i, j, k, g.ni, g.nj, g.nk)) {
+                const size_t c = flat(i, j, k, g.ni, g.nj);
+                b = offsets[c];
+                cnt[h] = offsets[c + 1] - b;
+            }
+            sGlob[rc] = b;
+        }
+    }
+    const int pair = cnt[0] + cnt[1];
+    int incl = pair;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if ((t & 63) >= o) incl += v;
+    }
+    if ((t & 63) == 63) sWave[t >> 6] = incl;
+    __syncthreads();
+    int waveBase = 0;
+    for (int w = 0; w < (t >> 6); w++) waveBase += sWave[w];
+    const int excl = waveBase + incl - pair;
+    if (2 * t < P2R_CELLS) sStart[2 * t] = excl;
+    if (2 * t + 1 < P2R_CELLS) sStart[2 * t + 1] = excl + cnt[0];
+    if (2 * t + 2 == P2R_CELLS || 2 * t + 1 == P2R_CELLS) sStart[P2R_CELLS] = excl + pair;
+    __syncthreads();
+    const int total = sStart[P2R_CELLS];
+    const bool staged = total <= P2G_CAP;
+    if (staged) {
+        for (int q = t; q < total; q += P2_THREADS) {
+            int lo = 0, hi = P2R_CELLS - 1;  // last cell whose start is <= q
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sStart[mid] <= q) lo = mid;
+                else hi = mid - 1;
+            }
+            const int gs = sGlob[lo] + (q - sStart[lo]);
+            sx[q] = spx[gs];
+            sy[q] = spy[gs];
+            sz[q] = spz[gs];
+            sv[0][q] = svx[gs];
+            sv[1][q] = svy[gs];
+            sv[2][q] = svz[gs];
+            sp[q] = perm[gs];
+        }
+    }
+    __syncthreads();
+    // one face per thread (direction = face / P2_FACES, wave-uniform)
+    for (int task = t; task < 3 * P2_FACES; task += P2_THREADS) {
+        const int dir = task / P2_FACES;
+        if (dir == 0) p2gTileFace<0>(g, task % P2_FACES, I0, J0, K0, total, staged, sStart, sGlob, sx, sy, sz, sv[0], sp,
+                                     spx, spy, spz, svx, perm, radius, out);
+        else if (dir == 1) p2gTileFace<1>(g, task % P2_FACES, I0, J0, K0, total, staged, sStart, sGlob, sx, sy, sz, sv[1],
+                                          sp, spx, spy, spz, svy, perm, radius, out);
+        else p2gTileFace<2>(g, task % P2_FACES, I0, J0, K0, total, staged, sStart, sGlob, sx, sy, sz, sv[2], sp, spx, spy,
+                            spz, svz, perm, radius, out);
+    }
 }
 
 // ------------------------------------------------- max |v|^2 over particles
@@ -566,14 +711,12 @@ void liquidSDFPostprocess(Grid g, const float *sp, float *phi, hipStream_t s) {
 
 void p2g(Grid g, const CellSort &cs, double radius, float *u, float *v, float *w, uint8_t *vu,
          uint8_t *vv, uint8_t *vw, hipStream_t s) {
-    size_t nu = (size_t)(g.ni + 1) * g.nj * g.nk, nv = (size_t)g.ni * (g.nj + 1) * g.nk,
-           nw = (size_t)g.ni * g.nj * (g.nk + 1);
-    hipLaunchKernelGGL(p2gKernel<0>, dim3(blocksFor(nu, TPB)), dim3(TPB), 0, s, g, cs.offsets, cs.perm,
-                       cs.spx, cs.spy, cs.spz, cs.svx, radius, u, vu);
-    hipLaunchKernelGGL(p2gKernel<1>, dim3(blocksFor(nv, TPB)), dim3(TPB), 0, s, g, cs.offsets, cs.perm,
-                       cs.spx, cs.spy, cs.spz, cs.svy, radius, v, vv);
-    hipLaunchKernelGGL(p2gKernel<2>, dim3(blocksFor(nw, TPB)), dim3(TPB), 0, s, g, cs.offsets, cs.perm,
-                       cs.spx, cs.spy, cs.spz, cs.svz, radius, w, vw);
+    const int tilesI = (g.ni + 1 + P2T_I - 1) / P2T_I, tilesJ = (g.nj + 1 + P2T_J - 1) / P2T_J,
+              tilesK = (g.nk + 1 + P2T_K - 1) / P2T_K;
+    P2GOut out{{u, v, w}, {vu, vv, vw}};
+    hipLaunchKernelGGL(p2gTileKernel, dim3((unsigned)tilesI * tilesJ * tilesK), dim3(P2_THREADS), 0, s, g,
+                       cs.offsets, cs.perm, cs.spx, cs.spy, cs.spz, cs.svx, cs.svy, cs.svz, radius, out, tilesI,
+                       tilesJ);
 }
 
 void maxSpeedSq(const Particles &p, double *partial, double *out, hipStream_t s) {

@@ -119,6 +119,30 @@ def test_p2g_bitexact(n):
         assert np.array_equal(g, o)
 
 
+@pytest.mark.parametrize("per_cell", [3, 40])
+def test_p2g_dense_cells_bitexact(per_cell):
+    """P2G on a non-cubic grid with a dense cluster: at 40 particles per cell
+    the tiles over the cluster hold more particles than their LDS stage and
+    read the sorted arrays instead; both paths must give the oracle's bits."""
+    n3 = (24, 17, 13)
+    dx = 1.0 / 24
+    rng = np.random.default_rng(per_cell)
+    lo, hi = np.array([4, 3, 2]), np.array([13, 11, 9])
+    cells = np.stack(np.meshgrid(*[np.arange(a, b) for a, b in zip(lo, hi)], indexing="ij"), -1).reshape(-1, 3)
+    cluster = (np.repeat(cells, per_cell, axis=0) + rng.random((len(cells) * per_cell, 3))) * dx
+    sparse = rng.random((500, 3)) * np.array(n3) * dx
+    pos = np.concatenate([cluster, sparse]).astype(np.float32)
+    vel = (rng.standard_normal(pos.shape) * 0.4).astype(np.float32)
+    r = 0.5 * dx * np.sqrt(3.0)
+    say(f"p2g dense per_cell={per_cell} P={len(pos)}")
+    (gu, gv, gw), (gvu, gvv, gvw) = K.p2g(n3, dx, pos, vel, r)
+    (ou, ov, ow), (ovu, ovv, ovw) = O.p2g(n3, dx, pos, vel, r)
+    for g, o, name in ((gu, ou, "u"), (gv, ov, "v"), (gw, ow, "w")):
+        assert same_bits(g, o), name
+    for g, o in ((gvu, ovu), (gvv, ovv), (gvw, ovw)):
+        assert np.array_equal(g, o)
+
+
 @pytest.mark.parametrize("n", CASES)
 def test_extrapolate_bitexact(n):
     n3, dx, pos, vel = scene(n)
