@@ -9,11 +9,12 @@ the static sphere obstacle: fluid block [0, 0.375) x [0, 0.625) x [0, 1),
 8 particles per cell (seeded jitter, zero initial velocity), sphere r = 0.15 at
 (0.6, 0.2, 0.5).  Inputs are resident in HBM before the timed region.
 
-Prints one JSON line.  `value` is the substeps/s of the faster mixed-precision
-mode (FP32 PCG restarted inside FP64 iterative refinement, or FP32 PCG with
-FP64 reliable updates; `mode` names it); the FP64 run, the ratio, per-kernel
-GPU times (HIP events on the engine's stream) and the roofline of the
-dominant kernel are reported beside it.
+Prints one JSON line.  `value` is the substeps/s of the fastest
+mixed-precision mode (FP32 PCG restarted inside FP64 iterative refinement,
+FP32 PCG with FP64 reliable updates, or FP64 PCG with the MIC(0) application
+in FP32; `mode` names it); the FP64 run, the ratio, per-kernel GPU times (HIP
+events on the engine's stream) and the roofline of the dominant kernel are
+reported beside it.
 """
 import argparse
 import json
@@ -112,7 +113,7 @@ def algorithmic_bytes(name, n, P, M_total, launches_per_sub, substeps, iters_tot
     if name == "p2g":
         return substeps * (28 * P + 4 * N + 5 * F)          # 28 B/particle, 4 B/cell, 5 B/face
     if name == "spmv":
-        per = 48 if mode == "fp64" else 24                    # 4 coefficients + x read + y write per row
+        per = 24 if mode in ("mixed", "mixed-reliable") else 48  # 4 coefficients + x read + y write per row
         return per * M_total
     if name == "mic0_apply":
         per = 96 if mode == "fp64" else 48                    # both sweeps: r, invdiag, 3 L, z per row
@@ -164,9 +165,10 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=96)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--modes", default="fp64,mixed,mixed-reliable",
+    ap.add_argument("--modes", default="fp64,mixed,mixed-reliable,fp32-preconditioner",
                     help="comma list of PCG modes: fp64, mixed (FP32 PCG restarted in FP64 refinement), "
-                         "mixed-reliable (FP32 PCG with FP64 reliable updates)")
+                         "mixed-reliable (FP32 PCG with FP64 reliable updates), fp32-preconditioner (FP64 "
+                         "PCG with the MIC(0) application in FP32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the engine keeps a whole simulation on one device "
@@ -193,14 +195,14 @@ def main():
     pos = scenes.dam_break_particles(n, seed=0)
     solid = scenes.union_phi(kernels.solid_box_phi((n, n, n), dx), scenes.sphere_phi_nodes(n))
     modes = [m for m in args.modes.split(",") if m]
-    if not set(modes) <= {"fp64", "mixed", "mixed-reliable"} or not modes:
-        sys.exit("--modes takes fp64, mixed and/or mixed-reliable")
+    if not set(modes) <= {"fp64", "mixed", "mixed-reliable", "fp32-preconditioner"} or not modes:
+        sys.exit("--modes takes fp64, mixed, mixed-reliable and/or fp32-preconditioner")
     results = {}
     for mode in modes:
         log("mode %s" % mode)
         results[mode] = run_mode(n, pos, solid, mode, args.tol, args.steps, args.warmup, frame_dt)
     # headline: the faster mixed-precision mode, else FP64
-    mixed_modes = [m for m in ("mixed-reliable", "mixed") if m in results]
+    mixed_modes = [m for m in ("mixed-reliable", "mixed", "fp32-preconditioner") if m in results]
     rate = lambda m: results[m]["substeps"] / results[m]["elapsed_s"]  # noqa: E731
     main_mode = max(mixed_modes, key=rate) if mixed_modes else "fp64"
     mixed, fp64 = results.get(main_mode) if mixed_modes else None, results.get("fp64")

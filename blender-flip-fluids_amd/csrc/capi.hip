@@ -471,7 +471,7 @@ int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u,
         up(dphi.p, phi, N, sc.s);
         ff::PressureSolver ps;
         ps.allocate(g);
-        if (mode < 0 || mode > 2) throw std::domain_error("mode must be 0, 1 or 2");
+        if (mode < 0 || mode > 3) throw std::domain_error("mode must be 0, 1, 2 or 3");
         ff::PressureResult r = ps.solve(dt, du, dv, dw, a, b, c, dphi, tol, maxIter, mode, mode == 2 ? 0.1 : 1e-4,
                                         dp, scan, sc.s, nullptr);
         ff::hipCheck(hipMemsetAsync(bu, 0, nu, sc.s), "memset");

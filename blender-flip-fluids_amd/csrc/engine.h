@@ -38,7 +38,7 @@ struct DevBuf {
 
 void hipCheck(hipError_t e, const char *what);
 
-enum class PressurePrecision { FP64 = 0, MixedRefined = 1, MixedReliable = 2 };
+enum class PressurePrecision { FP64 = 0, MixedRefined = 1, MixedReliable = 2, FP32Preconditioner = 3 };
 
 class FluidSimulation {
 public:

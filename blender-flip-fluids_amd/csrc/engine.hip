@@ -116,13 +116,14 @@ void FluidSimulation::setPressureSolverTolerance(double tol) {
     _pressureTolerance = tol;
 }
 void FluidSimulation::setPressureSolverPrecision(int mode, double innerTol) {
-    if (mode < 0 || mode > 2)
+    if (mode < 0 || mode > 3)
         throw std::domain_error("Error: pressure precision mode must be 0 (FP64), 1 (mixed, restarted "
-                                "refinement) or 2 (mixed, reliable updates).\n");
-    if (mode != 0 && !(innerTol > 0.0 && innerTol < 1.0))
+                                "refinement), 2 (mixed, reliable updates) or 3 (FP64 with an FP32 "
+                                "preconditioner).\n");
+    if ((mode == 1 || mode == 2) && !(innerTol > 0.0 && innerTol < 1.0))
         throw std::domain_error("Error: inner tolerance must be in (0, 1).\n");
     _precision = (PressurePrecision)mode;
-    if (mode != 0) _innerTolerance = innerTol;
+    if (mode == 1 || mode == 2) _innerTolerance = innerTol;
 }
 void FluidSimulation::setSolidPhi(const float *phi) {
     if (_initialized)

@@ -25,6 +25,8 @@ void exclusiveScan(const int *in, int *out, size_t n, int *scratch, hipStream_t 
 int reduceBlocks(size_t n);
 void dot(const double *a, const double *b, size_t n, double *partial, double *out, hipStream_t s);
 void dotf(const float *a, const float *b, size_t n, double *partial, double *out, hipStream_t s);
+// sum of (double)a[i] * b[i] (FP32 a, FP64 b), same tree
+void dotfd(const float *a, const double *b, size_t n, double *partial, double *out, hipStream_t s);
 void absMax(const double *a, size_t n, double *partial, double *out, hipStream_t s);
 void absMaxf(const float *a, size_t n, double *partial, double *out, hipStream_t s);
 // The reductions above run reduceBlocks(n) blocks of REDUCE_THREADS threads
@@ -144,18 +146,19 @@ void mic0LowerEntries(const PressureSys &S, bool fp64, hipStream_t s);
 // its rows (tileLo/tileHi hold mic0BandTiles ints; once per system)
 int mic0BandTiles(Grid g, bool fp64);
 void mic0BandTileRanges(Grid g, const PressureSys &S, bool fp64, int *tileLo, int *tileHi, hipStream_t s);
-// tiles that hold rows of a launch's band, passed as a launch argument
-struct TileMask {
-    static constexpr int BITS = 2048;
-    int use;  // 0: each workgroup reads its tile's range from tileLo/tileHi instead
-    unsigned w[BITS / 32];
+// the tiles that hold rows of a launch's band, passed as a launch argument:
+// workgroup b works on tile id[b]
+struct TileList {
+    static constexpr int MAX = 1900;  // keeps the launch arguments under 4 KB
+    int n;  // 0: one workgroup per tile, each reading its tile's range from tileLo/tileHi
+    uint16_t id[MAX];
 };
 // the launches of one application: the first nFwd run the forward sweep
 struct BandPlan {
     bool fp64 = true;
     int nFwd = 0;
     std::vector<int> l0;         // first hyperplane of each launch's band
-    std::vector<TileMask> mask;  // tiles with rows in the band
+    std::vector<TileList> list;  // tiles with rows in the band
 };
 // builds the plan from the host copies of the level table and the tile ranges
 // (hTileLo/hTileHi may be null: then every band that holds rows is launched
@@ -177,13 +180,17 @@ void axpyFloatStepF(float *y, const float *x, size_t n, const double *num, const
 // q = A s and *sq = s . q (the dot in the dot()/dotf() order)
 void spmvDot(const PressureSys &S, const double *s, double *q, double *partial, double *sq, hipStream_t st);
 void spmvDotF(const PressureSys &S, const float *s, float *q, double *partial, double *sq, hipStream_t st);
-// a = (float)(num/den); x += a * s; r -= a * q; *rmax = |r|_inf (the absMax order)
+// a = (float)(num/den); x += a * s; r -= a * q; *rmax = |r|_inf (the absMax order);
+// rf (optional) receives (float)r
 void pcgStep(double *x, const double *s, double *r, const double *q, size_t n, const double *num,
-             const double *den, double *partial, double *rmax, hipStream_t st);
+             const double *den, double *partial, double *rmax, hipStream_t st, float *rf = nullptr);
 void pcgStepF(float *x, const float *s, float *r, const float *q, size_t n, const double *num,
               const double *den, double *partial, double *rmax, hipStream_t st);
 void pcgStepMixed(double *x, const float *s, float *r, const float *q, size_t n, const double *num,
                   const double *den, double *partial, double *rmax, hipStream_t st);
+// z = (double)zf + (double)(float)(num/den) * s   (FP32 zf, FP64 z and s)
+void widenAxpyStep(double *z, const float *zf, const double *s, size_t n, const double *num, const double *den,
+                   hipStream_t st);
 // x += (double)(float)(sign * num/den) * (double)p   (FP64 x, FP32 p)
 void axpyMixed(double *x, const float *p, size_t n, const double *num, const double *den, double sign,
                hipStream_t s);

@@ -59,7 +59,7 @@ __global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid 
                                                                          const T *in, T *out,
                                                                          const MicRec<T> *rec,
                                                                          const int *tileLo, const int *tileHi,
-                                                                         int l0, int tilesJ, TileMask mask) {
+                                                                         int l0, int tilesJ, TileList list) {
     using B = Band<T, K, BT>;
     constexpr int E = B::E, LINES = B::LINES, P = B::P;
     __shared__ T sIn[LINES * P], sInv[LINES * P], sA[LINES * P], sB[LINES * P], sC[LINES * P];
@@ -67,15 +67,17 @@ __global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid 
     __shared__ T val[2][LINES];
     __shared__ int has[2][LINES];
     const int ni = g.ni, nj = g.nj, nk = g.nk;
-    const int J0 = (blockIdx.x % tilesJ) * BT, K0 = (blockIdx.x / tilesJ) * BT;
-    // workgroup-uniform exit when none of the tile's own rows lies in the band
-    // (from the launch's tile mask, or from the tile's hyperplane range)
+    // the launch names the tiles with rows in the band; without a list every
+    // tile has a workgroup, which exits (uniformly) when none of the tile's own
+    // rows lies in the band
     const int lo = FWD ? l0 : l0 - (K - 1), hi = FWD ? l0 + (K - 1) : l0;
-    if (mask.use) {
-        if (!((mask.w[blockIdx.x >> 5] >> (blockIdx.x & 31)) & 1u)) return;
-    } else if (tileHi[blockIdx.x] < lo || tileLo[blockIdx.x] > hi) {
+    int tile = blockIdx.x;
+    if (list.n > 0) {
+        tile = list.id[blockIdx.x];
+    } else if (tileHi[tile] < lo || tileLo[tile] > hi) {
         return;
     }
+    const int J0 = (tile % tilesJ) * BT, K0 = (tile / tilesJ) * BT;
     const int s = FWD ? 1 : -1;  // direction of the sweep along the hyperplanes
     const int jBase = FWD ? J0 - (K - 1) : J0, kBase = FWD ? K0 - (K - 1) : K0;
 
@@ -263,14 +265,16 @@ void bandSweeps(Grid g, const PressureSys &S, const BandPlan &plan, const T *r, 
                 const MicRec<T> *recFwd, const MicRec<T> *recBwd, const int *tileLo, const int *tileHi,
                 hipStream_t st) {
     const int tilesJ = (g.nj + BT - 1) / BT, tilesK = (g.nk + BT - 1) / BT;
-    const dim3 grid(tilesJ * tilesK), block(Band<T, K, BT>::THREADS);
+    const dim3 block(Band<T, K, BT>::THREADS);
     for (size_t q = 0; q < plan.l0.size(); q++) {
+        const TileList &tl = plan.list[q];
+        const dim3 grid(tl.n > 0 ? tl.n : tilesJ * tilesK);
         if ((int)q < plan.nFwd)
             hipLaunchKernelGGL((micBandKernel<T, K, BT, true>), grid, block, 0, st, g, S.cellRow, r, y, recFwd,
-                               tileLo, tileHi, plan.l0[q], tilesJ, plan.mask[q]);
+                               tileLo, tileHi, plan.l0[q], tilesJ, tl);
         else
             hipLaunchKernelGGL((micBandKernel<T, K, BT, false>), grid, block, 0, st, g, S.cellRow, y, z, recBwd,
-                               tileLo, tileHi, plan.l0[q], tilesJ, plan.mask[q]);
+                               tileLo, tileHi, plan.l0[q], tilesJ, tl);
     }
     hipCheck(hipGetLastError(), "MIC(0) band kernel launch");
 }
@@ -280,24 +284,23 @@ void bandSweeps(Grid g, const PressureSys &S, const BandPlan &plan, const T *r, 
 template <int K, int BT>
 void bandPlan(Grid g, const PressureSys &S, const int *hs, const int *hLo, const int *hHi, BandPlan &plan) {
     const int tilesJ = (g.nj + BT - 1) / BT, tilesK = (g.nk + BT - 1) / BT, nt = tilesJ * tilesK;
-    const bool useMask = hLo && hHi && nt <= TileMask::BITS;
+    const bool ranges = hLo && hHi && nt <= 65535;
     const int nl = S.nLevels;
     plan.l0.clear();
-    plan.mask.clear();
+    plan.list.clear();
     auto add = [&](int l0, int lo, int hi) {
-        TileMask m{};
-        if (useMask) {
-            m.use = 1;
-            bool any = false;
-            for (int t = 0; t < nt; t++)
-                if (!(hHi[t] < lo || hLo[t] > hi)) {
-                    m.w[t >> 5] |= 1u << (t & 31);
-                    any = true;
-                }
-            if (!any) return;
+        TileList tl;
+        tl.n = 0;
+        if (ranges) {
+            int count = 0;
+            for (int t = 0; t < nt; t++) count += !(hHi[t] < lo || hLo[t] > hi);
+            if (count == 0) return;
+            if (count <= TileList::MAX)
+                for (int t = 0; t < nt; t++)
+                    if (!(hHi[t] < lo || hLo[t] > hi)) tl.id[tl.n++] = (uint16_t)t;
         }
         plan.l0.push_back(l0);
-        plan.mask.push_back(m);
+        plan.list.push_back(tl);
     };
     // forward, bands [l0, l0+K) that hold rows
     for (int l0 = 0; l0 < nl; l0 += K) {
@@ -318,13 +321,13 @@ void bandPlan(Grid g, const PressureSys &S, const int *hs, const int *hLo, const
 #define FF_BAND64_K 8
 #endif
 #ifndef FF_BAND64_BT
-#define FF_BAND64_BT 12
+#define FF_BAND64_BT 11
 #endif
 #ifndef FF_BAND32_K
-#define FF_BAND32_K 10
+#define FF_BAND32_K 12
 #endif
 #ifndef FF_BAND32_BT
-#define FF_BAND32_BT 14
+#define FF_BAND32_BT 11
 #endif
 constexpr int K64 = FF_BAND64_K, BT64 = FF_BAND64_BT, K32 = FF_BAND32_K, BT32 = FF_BAND32_BT;
 

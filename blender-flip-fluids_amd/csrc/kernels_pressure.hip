@@ -284,7 +284,7 @@ __global__ __launch_bounds__(REDUCE_THREADS) void spmvDotKernel(int m, const int
 template <class TX, class T>
 __global__ __launch_bounds__(REDUCE_THREADS) void pcgStepKernel(TX *x, const T *s, T *r, const T *q, size_t n,
                                                                 const double *num, const double *den,
-                                                                double *partial) {
+                                                                double *partial, float *rf) {
     const double a = *num / *den;
     const float fa = (float)a, fna = (float)(-a);
     double acc = 0.0;
@@ -293,6 +293,7 @@ __global__ __launch_bounds__(REDUCE_THREADS) void pcgStepKernel(TX *x, const T *
         x[i] += (TX)fa * (TX)s[i];
         const T ri = r[i] + (T)fna * q[i];
         r[i] = ri;
+        if (rf) rf[i] = (float)ri;  // FP32 copy for an FP32 preconditioner
         acc = fmax(acc, fabs((double)ri));
     }
     blockFold<1>(acc, partial);
@@ -318,6 +319,18 @@ __global__ void axpyStepKernel(T *y, const T *x, size_t n, const double *num, co
     double a = den ? *num / *den : *num;
     float fa = (float)(sign * a);
     y[q] += (T)fa * x[q];
+}
+
+// z = (double)zf + (double)(float)(num/den) * s: the FP32 preconditioner's
+// result widened, then the search-direction update of axpyStepKernel
+__global__ void widenAxpyKernel(double *z, const float *zf, const double *s, size_t n, const double *num,
+                                const double *den) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float fa = (float)(*num / *den);
+    double zq = (double)zf[q];
+    zq += (double)fa * s[q];
+    z[q] = zq;
 }
 
 // x += (double)(float)(sign * num/den) * (double)p: the FP64 solution takes
@@ -450,16 +463,16 @@ void spmvDotF(const PressureSys &S, const float *x, float *y, double *partial, d
 
 template <class TX, class T>
 static void launchPcgStep(TX *x, const T *s, T *r, const T *q, size_t n, const double *num, const double *den,
-                          double *partial, double *rmax, hipStream_t st) {
+                          double *partial, double *rmax, hipStream_t st, float *rf = nullptr) {
     if (n == 0) return;
     const int nb = reduceBlocks(n);
     hipLaunchKernelGGL((pcgStepKernel<TX, T>), dim3(nb), dim3(REDUCE_THREADS), 0, st, x, s, r, q, n, num, den,
-                       partial);
+                       partial, rf);
     reduceFinish(partial, nb, rmax, true, st);
 }
 void pcgStep(double *x, const double *s, double *r, const double *q, size_t n, const double *num,
-             const double *den, double *partial, double *rmax, hipStream_t st) {
-    launchPcgStep<double, double>(x, s, r, q, n, num, den, partial, rmax, st);
+             const double *den, double *partial, double *rmax, hipStream_t st, float *rf) {
+    launchPcgStep<double, double>(x, s, r, q, n, num, den, partial, rmax, st, rf);
 }
 void pcgStepF(float *x, const float *s, float *r, const float *q, size_t n, const double *num,
               const double *den, double *partial, double *rmax, hipStream_t st) {
@@ -468,6 +481,12 @@ void pcgStepF(float *x, const float *s, float *r, const float *q, size_t n, cons
 void pcgStepMixed(double *x, const float *s, float *r, const float *q, size_t n, const double *num,
                   const double *den, double *partial, double *rmax, hipStream_t st) {
     launchPcgStep<double, float>(x, s, r, q, n, num, den, partial, rmax, st);
+}
+
+void widenAxpyStep(double *z, const float *zf, const double *s, size_t n, const double *num, const double *den,
+                   hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(widenAxpyKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, st, z, zf, s, n, num, den);
 }
 
 void axpyMixed(double *x, const float *p, size_t n, const double *num, const double *den, double sign,

@@ -101,8 +101,8 @@ int reduceBlocks(size_t n) {
 
 namespace {
 
-template <class T, int OP>  // OP 0: sum of a*b, 1: max |a|
-__global__ __launch_bounds__(RED_THREADS) void reduceStage1(const T *a, const T *b, size_t n,
+template <class T, int OP, class TB = T>  // OP 0: sum of a*b, 1: max |a|
+__global__ __launch_bounds__(RED_THREADS) void reduceStage1(const T *a, const TB *b, size_t n,
                                                             double *partial) {
     __shared__ double s[RED_THREADS];
     double acc = 0.0;
@@ -152,10 +152,10 @@ __global__ __launch_bounds__(RED_THREADS) void reduceStage2(const double *partia
     if (threadIdx.x == 0) *out = s[0];
 }
 
-template <class T, int OP>
-void reduce(const T *a, const T *b, size_t n, double *partial, double *out, hipStream_t s) {
+template <class T, int OP, class TB = T>
+void reduce(const T *a, const TB *b, size_t n, double *partial, double *out, hipStream_t s) {
     int nb = reduceBlocks(n);
-    hipLaunchKernelGGL((reduceStage1<T, OP>), dim3(nb), dim3(RED_THREADS), 0, s, a, b, n, partial);
+    hipLaunchKernelGGL((reduceStage1<T, OP, TB>), dim3(nb), dim3(RED_THREADS), 0, s, a, b, n, partial);
     hipLaunchKernelGGL((reduceStage2<OP>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
 }
 
@@ -173,6 +173,9 @@ void dot(const double *a, const double *b, size_t n, double *partial, double *ou
 }
 void dotf(const float *a, const float *b, size_t n, double *partial, double *out, hipStream_t s) {
     reduce<float, 0>(a, b, n, partial, out, s);
+}
+void dotfd(const float *a, const double *b, size_t n, double *partial, double *out, hipStream_t s) {
+    reduce<float, 0, double>(a, b, n, partial, out, s);
 }
 void absMax(const double *a, size_t n, double *partial, double *out, hipStream_t s) {
     reduce<double, 1>(a, a, n, partial, out, s);

@@ -32,7 +32,8 @@ public:
     // float pressure grid.  mode 0 = FP64 PCG; 1 = FP32 PCG restarted inside
     // FP64 iterative refinement (innerTol = relative tolerance of each inner
     // solve); 2 = FP32 PCG with FP64 reliable updates (innerTol = drop of the
-    // FP32 residual between true-residual updates).
+    // FP32 residual between true-residual updates); 3 = FP64 PCG with the
+    // MIC(0) application in FP32 (innerTol unused).
     PressureResult solve(double dt, const float *u, const float *v, const float *w, const float *wu,
                          const float *wv, const float *ww, const float *phi, double tol, int maxIter,
                          int mode, double innerTol, float *pressure, int *scanScratch, hipStream_t s,
@@ -77,6 +78,7 @@ private:
                   double &resid, hipStream_t s);
     bool pcgReliable(double tolFactor, int maxIter, double resid0, double delta, int &iters, double &resid,
                      hipStream_t s);
+    bool pcgPrecond32(double tolFactor, int maxIter, double resid0, int &iters, double &resid, hipStream_t s);
 };
 
 }  // namespace ff

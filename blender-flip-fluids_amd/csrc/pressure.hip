@@ -246,7 +246,8 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     double resid = -1.0;
     bool conv = mode == 0   ? pcgFP64(tol, maxIter, maxAbs, iters, resid, s)
                 : mode == 1 ? pcgMixed(tol, maxIter, maxAbs, innerTol, iters, resid, s)
-                            : pcgReliable(tol, maxIter, maxAbs, innerTol, iters, resid, s);
+                : mode == 2 ? pcgReliable(tol, maxIter, maxAbs, innerTol, iters, resid, s)
+                            : pcgPrecond32(tol, maxIter, maxAbs, iters, resid, s);
     checkError(s);
     pressureToGrid(_g, _S, _x, pressure, s);
     float e = (float)resid;
@@ -300,6 +301,63 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
         dot(z, _r, m, _partial, rhoNew, st);
         // beta = rhoNew / rho; z += (float)beta * s; then s <-> z
         axpyFloatStep(z, s, m, rhoNew, rho, 1.0, st);
+        std::swap(s, z);
+        std::swap(rho, rhoNew);
+    }
+    iters = it;
+    return false;
+}
+
+// FP64 PCG (pcgsolver.h:268-322) whose MIC(0) application runs on the FP32
+// copy of the factor: the step that updates r also rounds it to float for the
+// sweeps, and their result is widened into the search-direction update.  The
+// SpMV, the dot products, the step lengths and every vector stay FP64, so the
+// iteration follows the FP64 solve except for the preconditioner's rounding.
+bool PressureSolver::pcgPrecond32(double tolFactor, int maxIter, double resid0, int &iters, double &resid,
+                                  hipStream_t st) {
+    const int m = _S.m;
+    const int *hs = _hLevelStart.data();
+    if (tolFactor < 1e-30) tolFactor = 1e-30;
+    hipCheck(hipMemcpyAsync(_r, _S.rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
+    hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
+    resid = resid0;
+    double tol = tolFactor * resid;
+    _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
+    mic0Factor(_S, hs, st);
+    _t->end(T_MIC0_FACTOR, st);
+    mic0ToFloat(_S, st);
+    mic0LowerEntries(_S, false, st);
+    castToFloat(_r, _fr, m, st);
+    _t->begin(T_MIC0_APPLY, st, _bandLaunches);
+    mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, _fz, st);
+    _t->end(T_MIC0_APPLY, st);
+    double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
+    dotfd(_fz, _r, m, _partial, rho, st);
+    double hrho = hostScalar(rho, st);
+    if (hrho == 0 || hrho != hrho) {
+        iters = 0;
+        return false;
+    }
+    hipCheck(hipMemsetAsync(_s, 0, sizeof(double) * m, st), "memset");
+    addFloatToDouble(_s, _fz, m, st);
+    double *s = _s, *z = _z;
+    int it;
+    for (it = 0; it < maxIter; it++) {
+        _t->begin(T_SPMV, st);
+        spmvDot(_S, s, z, _partial, sz, st);
+        _t->end(T_SPMV, st);
+        pcgStep(_x, s, _r, z, m, rho, sz, _partial, rn, st, _fr);
+        resid = hostScalar(rn, st);
+        if (resid <= std::min(tol, 1.0)) {
+            iters = it + 1;
+            return true;
+        }
+        _t->begin(T_MIC0_APPLY, st, _bandLaunches);
+        mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, _fz, st);
+        _t->end(T_MIC0_APPLY, st);
+        dotfd(_fz, _r, m, _partial, rhoNew, st);
+        // beta = rhoNew / rho; z = (double)fz + (float)beta * s; then s <-> z
+        widenAxpyStep(z, _fz, s, m, rhoNew, rho, st);
         std::swap(s, z);
         std::swap(rho, rhoNew);
     }

@@ -241,8 +241,10 @@ class FluidSimulation(object):
         FP64 iterative refinement; inner_tolerance = relative tolerance of each
         inner solve, default 1e-4) or 'mixed-reliable' (FP32 PCG with FP64
         reliable updates; inner_tolerance = FP32 residual drop between true
-        residual updates, default 0.1)."""
-        m = {"fp64": 0, "mixed": 1, "mixed-reliable": 2}[mode] if isinstance(mode, str) else int(mode)
+        residual updates, default 0.1) or 'fp32-preconditioner' (FP64 PCG
+        whose MIC(0) application runs in FP32)."""
+        m = {"fp64": 0, "mixed": 1, "mixed-reliable": 2, "fp32-preconditioner": 3}[mode] \
+            if isinstance(mode, str) else int(mode)
         if inner_tolerance is None:
             inner_tolerance = 0.1 if m == 2 else 1e-4
         f = lib.FluidSimulation_set_pressure_solver_precision
@@ -250,7 +252,8 @@ class FluidSimulation(object):
         pb.execute_lib_func(f, [self(), m, inner_tolerance])
 
     def get_pressure_solver_precision(self):
-        return ("fp64", "mixed", "mixed-reliable")[_call0(self, "FluidSimulation_get_pressure_solver_precision", c_int)]
+        return ("fp64", "mixed", "mixed-reliable", "fp32-preconditioner")[
+            _call0(self, "FluidSimulation_get_pressure_solver_precision", c_int)]
 
     @_check_simulation_not_initialized
     def set_solid_phi(self, phi_nodes):

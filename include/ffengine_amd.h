@@ -154,8 +154,9 @@ double FluidSimulation_get_pressure_solver_tolerance(FluidSimulation *obj, int *
  * float-rounded step lengths; 1 = FP32 PCG restarted inside FP64 iterative
  * refinement, inner relative tolerance inner_tol; 2 = FP32 PCG with FP64
  * reliable updates, the true residual recomputed whenever the FP32 residual
- * has dropped by inner_tol.  In modes 1 and 2 the FP64 true residual decides
- * convergence with the reference's rule. */
+ * has dropped by inner_tol; 3 = FP64 PCG whose MIC(0) application runs on
+ * the FP32 copy of the factor (inner_tol unused).  In modes 1 and 2 the FP64
+ * true residual decides convergence with the reference's rule. */
 void FluidSimulation_set_pressure_solver_precision(FluidSimulation *obj, int mode,
                                                    double inner_tol, int *err);
 int FluidSimulation_get_pressure_solver_precision(FluidSimulation *obj, int *err);

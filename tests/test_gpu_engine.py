@@ -79,7 +79,7 @@ def test_solid_phi_matches_oracle():
                           O.solid_box_phi((n, n, n), 1.0 / n).view(np.uint32))
 
 
-@pytest.mark.parametrize("precision", ["mixed", "mixed-reliable"])
+@pytest.mark.parametrize("precision", ["mixed", "mixed-reliable", "fp32-preconditioner"])
 def test_mixed_precision_engine_runs(precision):
     n = 24
     pos = scenes.dam_break_particles(n, seed=2)

@@ -280,7 +280,7 @@ def test_pressure_solve_fp64(n):
         assert np.abs(g - o).max() <= 1e-5 * max(np.abs(o).max(), 1e-6)
 
 
-@pytest.mark.parametrize("mode", [1, 2], ids=["restarted", "reliable"])
+@pytest.mark.parametrize("mode", [1, 2, 3], ids=["restarted", "reliable", "fp32-preconditioner"])
 @pytest.mark.parametrize("n", CASES)
 def test_pressure_solve_mixed(n, mode):
     n3, dx, dt, uvw, weights, phi = _pressure_inputs(n)
