@@ -129,6 +129,32 @@ def algorithmic_bytes(name, n, P, M_total, launches_per_sub, substeps, iters_tot
     return None
 
 
+def band_kernel_name(fp64):
+    """Template name of the MIC(0) band kernel the library runs for one precision."""
+    from ctypes import byref, c_int, c_void_p
+    from ffengine import pybindings as pb
+    from ffengine.ffengine import ffengine as lib
+    f = lib.FFAMD_mic0_band_shape
+    pb.init_lib_func(f, [c_int, c_void_p, c_void_p, c_void_p], None)
+    k, bt = c_int(), c_int()
+    pb.execute_lib_func(f, [1 if fp64 else 0, byref(k), byref(bt)])
+    return "micBandKernel<%s, %d, %d>" % ("double" if fp64 else "float", k.value, bt.value)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed PMC summaries
+    (lines 'traffic <kernel> <bytes>' in profiles/*.md), or None when no
+    summary covers this kernel build."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                parts = line.strip().strip("`").split()
+                if len(parts) >= 2 and parts[0] == "traffic" and " ".join(parts[1:-1]) == kernel:
+                    return int(parts[-1])
+    return None
+
+
 def cpu_baseline(n_sample, threads, seconds_budget=30.0):
     """The reference engine (built from its sources into oracle/_ref) on the
     same scene at n_sample^3, timed on the host cores."""
@@ -222,8 +248,12 @@ def main():
     bytes_ = algorithmic_bytes(dom, n, P, M_total, 1, head["substeps"], sum(iters), main_mode)
     achieved = bytes_ / (kt[dom]["ms"] * 1e-3) / 1e9 if kt[dom]["ms"] > 0 else 0.0
     launches = max(kt[dom]["launches"], 1)
+    traffic, kname = None, dom
+    if dom == "mic0_apply":
+        kname = band_kernel_name(main_mode == "fp64")
+        traffic = measured_traffic(kname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "avg_launch_us": round(kt[dom]["ms"] * 1e3 / launches, 3),
                 "algorithmic_bytes_per_launch": int(bytes_ / launches)}
 

@@ -488,6 +488,15 @@ int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u,
     });
 }
 
+void FFAMD_mic0_band_shape(int fp64, int *levels, int *tile, int *err) {
+    guarded(err, [&] {
+        int k = 0, bt = 0;
+        ff::mic0BandShape(fp64 != 0, k, bt);
+        *levels = k;
+        *tile = bt;
+    });
+}
+
 int FFAMD_mic0_apply(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
                      const float *w, const float *wu, const float *wv, const float *ww, const float *phi,
                      int precision, int schedule, int n, const double *r, double *z, int *err) {

@@ -145,6 +145,8 @@ void mic0LowerEntries(const PressureSys &S, bool fp64, hipStream_t s);
 // tiles of the band schedule and, per tile, the lowest and highest i+j+k of
 // its rows (tileLo/tileHi hold mic0BandTiles ints; once per system)
 int mic0BandTiles(Grid g, bool fp64);
+// hyperplanes per launch and tile side of the band schedule
+void mic0BandShape(bool fp64, int &levels, int &tile);
 void mic0BandTileRanges(Grid g, const PressureSys &S, bool fp64, int *tileLo, int *tileHi, hipStream_t s);
 // the tiles that hold rows of a launch's band, passed as a launch argument:
 // workgroup b works on tile id[b]

@@ -353,6 +353,11 @@ void mic0BandPlan(Grid g, const PressureSys &S, const int *hLevelStart, bool fp6
         bandPlan<K32, BT32>(g, S, hLevelStart, hTileLo, hTileHi, plan);
 }
 
+void mic0BandShape(bool fp64, int &levels, int &tile) {
+    levels = fp64 ? K64 : K32;
+    tile = fp64 ? BT64 : BT32;
+}
+
 int mic0BandTiles(Grid g, bool fp64) {
     const int bt = fp64 ? BT64 : BT32;
     return ((g.nj + bt - 1) / bt) * ((g.nk + bt - 1) / bt);

@@ -231,12 +231,16 @@ int FFAMD_pressure_system(int isize, int jsize, int ksize, double dx, double dt,
  * error, 2 = bands of hyperplanes (the solver's: the launches name the tiles
  * that hold rows of their band), 3 = the same bands with each workgroup
  * reading its tile's hyperplane range on the device (used when the tiles
- * outnumber a launch's tile mask).  The inputs must give a non-zero
+ * outnumber what a launch can list).  The inputs must give a non-zero
  * divergence.  Returns M. */
 int FFAMD_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, const float *u,
                      const float *v, const float *w, const float *wu, const float *wv,
                      const float *ww, const float *liquid_phi, int precision, int schedule,
                      int n, const double *r, double *z, int *err);
+/* Shape of the band schedule of the MIC(0) application (no counterpart in
+ * the reference): hyperplanes per launch and tile side in lines, for the FP64
+ * (fp64 = 1) or the FP32 factor. */
+void FFAMD_mic0_band_shape(int fp64, int *levels, int *tile, int *err);
 void FFAMD_sample_velocity(int isize, int jsize, int ksize, double dx, const float *u,
                            const float *v, const float *w, int np, const float *pos, float *out,
                            int *err);
