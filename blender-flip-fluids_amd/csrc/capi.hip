@@ -234,6 +234,12 @@ void FluidSimulation_set_pressure_solver_precision(FluidSimulation *o, int mode,
 int FluidSimulation_get_pressure_solver_precision(FluidSimulation *o, int *err) {
     return guarded(err, [&] { return S(o)->getPressureSolverPrecision(); });
 }
+void FluidSimulation_set_pressure_solver_reference_summation(FluidSimulation *o, int on, int *err) {
+    guarded(err, [&] { S(o)->setPressureSolverReferenceSummation(on != 0); });
+}
+int FluidSimulation_get_pressure_solver_reference_summation(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getPressureSolverReferenceSummation() ? 1 : 0; });
+}
 void FluidSimulation_set_solid_phi(FluidSimulation *o, const float *phi, int *err) {
     guarded(err, [&] { S(o)->setSolidPhi(phi); });
 }

@@ -90,6 +90,9 @@ public:
     double getPressureSolverTolerance() const { return _pressureTolerance; }
     void setPressureSolverPrecision(int mode, double innerTol);
     int getPressureSolverPrecision() const { return (int)_precision; }
+    // FP64 solve: dot products in the reference's order for the thread count
+    void setPressureSolverReferenceSummation(bool on) { _referenceSummation = on; }
+    bool getPressureSolverReferenceSummation() const { return _referenceSummation; }
     void setSolidPhi(const float *phiNodes);
 
     // particle I/O (fluidsimulation.cpp:3426-3839)
@@ -134,6 +137,7 @@ private:
     double _pressureTolerance = 1e-9;
     PressurePrecision _precision = PressurePrecision::FP64;
     double _innerTolerance = 1e-4;
+    bool _referenceSummation = false;
     int _maxThreads = 1;
     double _density = 20.0;
     bool _extremeRemoval = true;

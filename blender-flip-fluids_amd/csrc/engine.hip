@@ -464,7 +464,7 @@ void FluidSimulation::stepFluid(double dt) {
 bool FluidSimulation::pressureSolve(double dt) {
     PressureResult r = _ps.solve(dt, _u, _v, _w, _wu, _wv, _ww, _phi, _pressureTolerance,
                                  _maxPressureIterations, (int)_precision, _innerTolerance, _pressure,
-                                 _scan, _stream, _ev[3], &_timers);
+                                 _scan, _stream, _ev[3], &_timers, _referenceSummation ? _maxThreads : 0);
     _lastIters = r.iterations;
     _lastErr = r.error;
     _lastSuccess = r.success ? 1 : 0;

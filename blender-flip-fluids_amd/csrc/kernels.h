@@ -28,6 +28,13 @@ void dotf(const float *a, const float *b, size_t n, double *partial, double *out
 // sum of (double)a[i] * b[i] (FP32 a, FP64 b), same tree
 void dotfd(const float *a, const double *b, size_t n, double *partial, double *out, hipStream_t s);
 void absMax(const double *a, size_t n, double *partial, double *out, hipStream_t s);
+// BLAS::dot in the reference's order (blaswrapper.h:27-63): `threads` (the
+// engine's thread count) decides the number of intervals, each summed from
+// zero in index order, then the interval sums in order.  partial must hold
+// referenceDotChunks(n, threads) doubles.
+int referenceDotChunks(size_t n, int threads);
+void dotReference(const double *a, const double *b, size_t n, int threads, double *partial, double *out,
+                  hipStream_t s);
 void absMaxf(const float *a, size_t n, double *partial, double *out, hipStream_t s);
 // The reductions above run reduceBlocks(n) blocks of REDUCE_THREADS threads
 // over a grid-stride loop, then one block folds the partials in a fixed tree;

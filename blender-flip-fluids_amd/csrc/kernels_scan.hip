@@ -6,6 +6,9 @@
 // so a given vector always reduces to the same bits.
 #include "kernels.h"
 
+#include <cmath>
+#include <stdexcept>
+
 namespace ff {
 
 namespace {
@@ -182,6 +185,59 @@ void absMax(const double *a, size_t n, double *partial, double *out, hipStream_t
 }
 void absMaxf(const float *a, size_t n, double *partial, double *out, hipStream_t s) {
     reduce<float, 1>(a, a, n, partial, out, s);
+}
+
+// ---- dot products in the reference's summation order ----
+namespace {
+
+// v of lane l (l the same in every lane), through two scalar reads
+__device__ __forceinline__ double laneValue(double v, int l) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// interval c of [0, n) split into `chunks` (ThreadUtils::splitRangeIntoIntervals,
+// threadutils.cpp:54-72), summed from zero in index order by one wave: the
+// lanes form 64 products at a time, lane 0's running sum takes them in order
+__global__ __launch_bounds__(64) void dotChunkKernel(const double *a, const double *b, int n, int chunks,
+                                                     double *partial) {
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int size = (int)floor((double)n / (double)chunks);
+    const int rem = n - size * chunks;
+    const int lo = c * size + (c < rem ? c : rem), hi = lo + size + (c < rem ? 1 : 0);
+    double s = 0.0;
+    for (int base = lo; base < hi; base += 64) {
+        const int i = base + lane;
+        const double p = i < hi ? a[i] * b[i] : 0.0;
+        const int cnt = hi - base < 64 ? hi - base : 64;
+        for (int l = 0; l < cnt; l++) s += laneValue(p, l);
+    }
+    if (lane == 0) partial[c] = s;
+}
+
+__global__ void dotChunkSumKernel(const double *partial, int chunks, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int c = 0; c < chunks; c++) s += partial[c];
+    *out = s;
+}
+
+}  // namespace
+
+int referenceDotChunks(size_t n, int threads) {
+    // BLAS::dot: numthreads = min(numCPU, ceil((float)size / 500000)) (blaswrapper.h:39-40)
+    const int c = (int)fmin((double)threads, std::ceil((float)n / (float)500000));
+    return c < 1 ? 1 : c;
+}
+
+void dotReference(const double *a, const double *b, size_t n, int threads, double *partial, double *out,
+                  hipStream_t s) {
+    const int chunks = referenceDotChunks(n, threads);
+    if (chunks > RED_MAX_BLOCKS) throw std::runtime_error("reference-order dot: too many intervals");
+    hipLaunchKernelGGL(dotChunkKernel, dim3(chunks), dim3(64), 0, s, a, b, (int)n, chunks, partial);
+    hipLaunchKernelGGL(dotChunkSumKernel, dim3(1), dim3(1), 0, s, partial, chunks, out);
 }
 
 }  // namespace ff

@@ -29,7 +29,9 @@ public:
 
     void allocate(Grid g);
     // Builds the system from the current fields, solves it and writes the
-    // float pressure grid.  mode 0 = FP64 PCG; 1 = FP32 PCG restarted inside
+    // float pressure grid.  referenceDots > 0 makes the FP64 solve (mode 0)
+    // take its dot products in the reference's order for that thread count.
+    // mode 0 = FP64 PCG; 1 = FP32 PCG restarted inside
     // FP64 iterative refinement (innerTol = relative tolerance of each inner
     // solve); 2 = FP32 PCG with FP64 reliable updates (innerTol = drop of the
     // FP32 residual between true-residual updates); 3 = FP64 PCG with the
@@ -37,7 +39,7 @@ public:
     PressureResult solve(double dt, const float *u, const float *v, const float *w, const float *wu,
                          const float *wv, const float *ww, const float *phi, double tol, int maxIter,
                          int mode, double innerTol, float *pressure, int *scanScratch, hipStream_t s,
-                         hipEvent_t afterAssembly, KernelTimers *timers = nullptr);
+                         hipEvent_t afterAssembly, KernelTimers *timers = nullptr, int referenceDots = 0);
     const PressureSys &system() const { return _S; }
     const std::vector<int> &levelTable() const { return _hLevelStart; }
     // one preconditioner application z = (L L^T)^-1 r on the current factor
@@ -68,6 +70,7 @@ private:
     KernelTimers *_t = nullptr;
     KernelTimers _noTimers;
     int _levelLaunches = 0, _bandLaunches = 0;
+    int _refDots = 0;  // thread count of the reference's dot order, or 0 for the tree reductions
 
     double hostScalar(const double *d, hipStream_t s);
     void enqueueTileRanges(bool fp64, hipStream_t s);

@@ -202,8 +202,9 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
                                      const float *wu, const float *wv, const float *ww, const float *phi,
                                      double tol, int maxIter, int mode, double innerTol, float *pressure,
                                      int *scanScratch, hipStream_t s, hipEvent_t afterAssembly,
-                                     KernelTimers *timers) {
+                                     KernelTimers *timers, int referenceDots) {
     PressureResult res;
+    _refDots = referenceDots;
     _t = timers ? timers : &_noTimers;
     hipCheck(hipMemsetAsync(_ms.error, 0, sizeof(int), s), "memset error word");
     _t->begin(T_ASSEMBLE, s);
@@ -276,7 +277,14 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, _r, _y, _z, st);
     _t->end(T_MIC0_APPLY, st);
     double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
-    dot(_z, _r, m, _partial, rho, st);
+    // dot products: the reference's order, or the deterministic tree
+    auto dotp = [&](const double *a, const double *b, double *out) {
+        if (_refDots > 0)
+            dotReference(a, b, m, _refDots, _partial, out, st);
+        else
+            dot(a, b, m, _partial, out, st);
+    };
+    dotp(_z, _r, rho);
     double hrho = hostScalar(rho, st);
     if (hrho == 0 || hrho != hrho) {
         iters = 0;
@@ -287,7 +295,12 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     int it;
     for (it = 0; it < maxIter; it++) {
         _t->begin(T_SPMV, st);
-        spmvDot(_S, s, z, _partial, sz, st);
+        if (_refDots > 0) {
+            spmv(_S, s, z, st);
+            dotReference(s, z, m, _refDots, _partial, sz, st);
+        } else {
+            spmvDot(_S, s, z, _partial, sz, st);
+        }
         _t->end(T_SPMV, st);
         pcgStep(_x, s, _r, z, m, rho, sz, _partial, rn, st);
         resid = hostScalar(rn, st);
@@ -298,7 +311,7 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
         _t->begin(T_MIC0_APPLY, st, _bandLaunches);
         mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, _r, _y, z, st);
         _t->end(T_MIC0_APPLY, st);
-        dot(z, _r, m, _partial, rhoNew, st);
+        dotp(z, _r, rhoNew);
         // beta = rhoNew / rho; z += (float)beta * s; then s <-> z
         axpyFloatStep(z, s, m, rhoNew, rho, 1.0, st);
         std::swap(s, z);

@@ -251,6 +251,16 @@ class FluidSimulation(object):
         pb.init_lib_func(f, [c_void_p, c_int, c_double, c_void_p], None)
         pb.execute_lib_func(f, [self(), m, inner_tolerance])
 
+    def set_pressure_solver_reference_summation(self, enabled):
+        """FP64 solve: dot products in the reference's summation order (its
+        BLAS::dot split by the max thread count) instead of the parallel tree."""
+        f = lib.FluidSimulation_set_pressure_solver_reference_summation
+        pb.init_lib_func(f, [c_void_p, c_int, c_void_p], None)
+        pb.execute_lib_func(f, [self(), 1 if enabled else 0])
+
+    def get_pressure_solver_reference_summation(self):
+        return bool(_call0(self, "FluidSimulation_get_pressure_solver_reference_summation", c_int))
+
     def get_pressure_solver_precision(self):
         return ("fp64", "mixed", "mixed-reliable", "fp32-preconditioner")[
             _call0(self, "FluidSimulation_get_pressure_solver_precision", c_int)]

@@ -161,6 +161,17 @@ void FluidSimulation_set_pressure_solver_precision(FluidSimulation *obj, int mod
                                                    double inner_tol, int *err);
 int FluidSimulation_get_pressure_solver_precision(FluidSimulation *obj, int *err);
 
+/* Dot products of the FP64 solve in the reference's summation order (no
+ * counterpart in the reference, which always sums that way): BLAS::dot
+ * (pcgsolver/blaswrapper.h:27-63) splits the rows into
+ * min(max thread count, ceil(M / 500000)) intervals, sums each from zero in
+ * index order and adds the interval sums in order.  With it on (and the same
+ * thread count as the reference run) the FP64 solve takes the reference's
+ * iterates bit for bit; off (the default), the sums use a fixed parallel tree. */
+void FluidSimulation_set_pressure_solver_reference_summation(FluidSimulation *obj, int enabled,
+                                                            int *err);
+int FluidSimulation_get_pressure_solver_reference_summation(FluidSimulation *obj, int *err);
+
 /* Nodal solid SDF ((isize+1)*(jsize+1)*(ksize+1) floats, i fastest, negative
  * inside solids) replacing the domain-boundary SDF the engine builds itself.
  * Solids are static; their face velocity is zero.  Call before initialize. */

@@ -2,11 +2,13 @@
 with the CPU oracle stepping the same dam break, plus the engine's error
 behaviour through the C ABI.
 
-Tolerances: within a substep every kernel except the PCG dot products keeps
-the reference's operation order, so the two runs stay identical until a
-float pressure value rounds differently; after that, differences grow with
-the flow.  The tests bound positions to 1e-5 dx after one frame and count the
-particles exactly.
+Tolerances: every kernel keeps the reference's operation order.  With the
+PCG dot products in the reference's summation order the engine takes the
+reference's bits: the 64^3 dam break matches the reference engine's
+checksums for all 60 frames.  With the default parallel-tree dot products
+the two runs stay identical until a float pressure value rounds differently;
+after that, differences grow with the flow, so those tests bound positions
+to 1e-5 dx after one frame and count the particles exactly.
 """
 import sys
 
@@ -69,6 +71,50 @@ def test_frames_match_oracle(n):
             f"oracle_iters={orc.pressure_iterations()} max|dp|/dx={err:.3g}")
         if frame == 0:
             assert err <= 1e-5
+
+
+def _golden_dam_break(n, seed):
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "make_golden.py")
+    spec = importlib.util.spec_from_file_location("make_golden", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.dam_break(n, seed)
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.timeout(900)
+def test_c1_sixty_frames_match_reference_bits():
+    """The 64^3 dam break for 60 frames of 1/30 s, FP64 PCG with
+    its dot products in the reference's summation order: every frame's
+    particle count, positions and velocities hash to what the reference
+    engine (built from its sources) produced (tests/golden/e2e_c1_n64.json)."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "e2e_c1_n64.json")
+    with open(path) as fh:
+        rec = json.load(fh)
+    n = 64
+    pos = _golden_dam_break(n, 0)
+    assert _sha(pos) == rec["initial_positions_sha256"]
+    sim = make_gpu(n, pos)
+    sim.max_thread_count = 8
+    sim.set_pressure_solver_reference_summation(True)
+    for fr in rec["frames"]:
+        sim.update(1.0 / 30.0)
+        p, v = gpu_particles(sim)
+        st = sim.get_frame_stats_data()
+        same = len(p) == fr["count"] and _sha(p) == fr["positions_sha256"] and _sha(v) == fr["velocities_sha256"]
+        say("C1 frame %d: particles %d substeps %d iters %d bit-identical %s" % (
+            fr["frame"], len(p), st.substeps, st.pressure_solver_iterations, same))
+        assert len(p) == fr["count"], fr["frame"]
+        assert _sha(p) == fr["positions_sha256"], fr["frame"]
+        assert _sha(v) == fr["velocities_sha256"], fr["frame"]
 
 
 def test_solid_phi_matches_oracle():
