@@ -46,7 +46,8 @@ def make_sim(n, pos, solid, mode, tol):
     sim.enable_surface_reconstruction = False
     sim.pressure_solver_tolerance = tol
     sim.set_pressure_solver_precision(mode)
-    sim.set_solid_phi(solid)
+    if solid is not None:
+        sim.set_solid_phi(solid)
     sim.load_marker_particle_data(len(pos), pos.tobytes(), np.zeros_like(pos).tobytes())
     sim.initialize()
     return sim
@@ -181,12 +182,105 @@ def cpu_baseline(n_sample, threads, seconds_budget=30.0):
                       "without the sphere, %d timed substeps of 1/240 s after one warm-up substep" % (n_sample, count)}
 
 
+def precision_config(n, tol):
+    """The precision configuration: one pressure solve + projection of a liquid
+    slab over 40% of the height carrying smoothed Gaussian-noise velocities
+    (0.5 m/s, 8-cell correlation), in every PCG mode.  Reports each mode's
+    pressure error (relative L2) against the FP64 GPU solve, its iterations and
+    the largest divergence left after the projection (max |rhs| of the
+    projected field).  The FP64 GPU solve itself is checked against the FP64
+    CPU oracle at 40^3 by the test suite."""
+    import numpy as np
+    from ffengine import kernels, scenes
+    n3, dx, dt = (n, n, n), 1.0 / n, 1.0 / 240.0
+    log("precision slab n=%d" % n)
+    phi, uvw = scenes.precision_slab(n, seed=0)
+    weights = kernels.face_weights(n3, kernels.solid_box_phi(n3, dx))
+    div0 = float(np.abs(kernels.pressure_system(n3, dx, dt, uvw, weights, phi)["rhs"]).max())
+    res, ref = {}, None
+    for mode, name in ((0, "fp64"), (1, "mixed"), (2, "mixed-reliable"), (3, "fp32-preconditioner")):
+        t0 = time.perf_counter()
+        p, uvw2, _, it, err, ok = kernels.pressure_solve(n3, dx, dt, uvw, weights, phi, tol=tol, max_iter=20000,
+                                                         mode=mode)
+        wall = time.perf_counter() - t0
+        div = float(np.abs(kernels.pressure_system(n3, dx, dt, uvw2, weights, phi)["rhs"]).max())
+        if ref is None:
+            ref = p.astype(np.float64)
+        l2 = float(np.linalg.norm(p.astype(np.float64) - ref) / max(np.linalg.norm(ref), 1e-300))
+        res[name] = {"iterations": int(it), "residual_inf": float(err), "success": bool(ok),
+                     "pressure_rel_l2_vs_fp64": l2, "divergence_max_after": div,
+                     "wall_s_with_host_copies": round(wall, 3)}
+        log("  %s: iters %d residual %.3g rel L2 %.3g div %.3g (%.1fs)" % (name, it, err, l2, div, wall))
+    line = {"metric": "pressure rel L2 vs FP64 PCG and max post-projection divergence, precision slab %d^3" % n,
+            "value": res["mixed"]["pressure_rel_l2_vs_fp64"], "unit": "relative L2", "n_gpus": 1,
+            "higher_is_better": False, "data": "synthetic",
+            "config": {"workload": "liquid slab over 40%% of %d^3, smoothed Gaussian-noise MAC velocity "
+                                   "(0.5 m/s, 8-cell correlation, seed 0), dt 1/240 s, PCG rel res %g" % (n, tol),
+                       "grid": n},
+            "divergence_max_before": div0, "modes": res}
+    print(json.dumps(line), flush=True)
+
+
+def splash_particles(n, seed=0, pool=0.3, center=(0.5, 0.65, 0.5), radius=0.2):
+    """8 jittered particles per cell in the pool (y < pool) and in the cells
+    whose centre lies inside the falling sphere of liquid."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / n
+    c = (np.arange(n) + 0.5) * dx
+    chunks = []
+    for k in range(n):
+        zz, yy, xx = c[k], c[:, None], c[None, :]
+        inside = (yy < pool) | ((xx - center[0]) ** 2 + (yy - center[1]) ** 2 + (zz - center[2]) ** 2 < radius ** 2)
+        jj, ii = np.nonzero(np.broadcast_to(inside, (n, n)))
+        if len(ii) == 0:
+            continue
+        base = np.stack([ii, jj, np.full_like(ii, k)], -1).reshape(-1, 1, 3).astype(np.float32)
+        sub = np.array([[a, b, cc] for cc in (0, 1) for b in (0, 1) for a in (0, 1)], np.float32)[None] * 0.5
+        jit = rng.random((base.shape[0], 8, 3), dtype=np.float32) * 0.5
+        chunks.append(((base + sub + jit) * np.float32(dx)).reshape(-1, 3))
+    return np.concatenate(chunks).astype(np.float32)
+
+
+def splash_config(n, steps, warmup, tol, mode):
+    """The large single-GPU configuration: a sphere of liquid (r = 0.2) falling
+    into a pool over 30% of the height, 8 particles per liquid cell."""
+    import torch
+    log("splash n=%d: seeding particles" % n)
+    pos = splash_particles(n)
+    log("  %d particles" % len(pos))
+    res = run_mode(n, pos, None, mode, tol, steps, warmup, 1.0 / 240.0)
+    rate = res["substeps"] / res["elapsed_s"]
+    P = res["particles"]
+    kt = res["kernels"]
+    M_total = sum(u * it for u, it in zip(res["unknowns"], res["iterations"]))
+    gbs = {}
+    for k, v in kt.items():
+        b = algorithmic_bytes(k, n, P, M_total, 1, res["substeps"], sum(res["iterations"]), mode)
+        if b is not None and v["ms"] > 0:
+            gbs[k] = round(b / (v["ms"] * 1e-3) / 1e9, 1)
+    line = {"metric": "FLIP substeps/s, splash %d^3" % n, "value": round(rate, 4), "unit": "substeps/s",
+            "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "data": "synthetic",
+            "dtype": "f32+f64" if mode != "fp64" else "f64", "mode": mode,
+            "config": {"workload": "sphere of liquid r=0.2 at (0.5, 0.65, 0.5) falling into a pool over 30%% of "
+                                   "%d^3, %d particles, frame 1/240 s, PCG rel res %g" % (n, len(pos), tol),
+                       "grid": n, "particles": len(pos)},
+            "particle_updates_per_s": round(rate * P, 1),
+            "pressure_iterations": res["iterations"], "unknowns": res["unknowns"],
+            "kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items()}, "kernel_gb_per_s": gbs,
+            "gpu_memory_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="dam-break", choices=["dam-break", "precision", "splash"],
+                    help="dam-break: the 256^3 headline benchmark (default); precision: the 384^3 slab "
+                         "pressure-accuracy comparison; splash: the 512^3 splash")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--n", type=int, default=None, help="grid edge (default 256, 384 or 512 by --config)")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-sample-n", type=int, default=96)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -207,7 +301,14 @@ def main():
     torch.cuda.set_device(0)
     from ffengine import kernels, scenes
 
-    n = args.n
+    if args.config == "precision":
+        precision_config(args.n or 384, args.tol)
+        return
+    if args.config == "splash":
+        modes = [m for m in args.modes.split(",") if m]
+        splash_config(args.n or 512, args.steps, args.warmup, args.tol, modes[0] if len(modes) == 1 else "mixed")
+        return
+    n = args.n or 256
     dx = 1.0 / n
     frame_dt = 1.0 / 240.0
     # one plain device-to-device copy of exactly 512 MiB before the timed

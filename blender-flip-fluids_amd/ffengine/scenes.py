@@ -5,6 +5,10 @@ one per half-cell sub-cube, each jittered uniformly inside its sub-cube from a
 seeded numpy generator (the stratified layout of
 FluidSimulation::_addMarkerParticlesToCell, fluidsimulation.cpp:7994-8004,
 with the jitter made reproducible).  Velocities are zero.
+
+precision_slab() builds the grid state of the precision configuration: a
+liquid slab over the bottom 40% of the domain whose MAC velocity is seeded
+Gaussian noise smoothed to an 8-cell correlation length.
 """
 import numpy as np
 
@@ -43,3 +47,33 @@ def union_phi(*phis):
     for p in phis[1:]:
         np.minimum(out, p, out=out)
     return out
+
+
+def noise_mac_field(n, seed=0, amplitude=0.5, correlation=8.0):
+    """u, v, w face arrays ((k, j, i) shapes (n, n, n+1), (n, n+1, n),
+    (n+1, n, n)), float32: independent Gaussian noise per component, smoothed
+    by a Gaussian filter of standard deviation correlation/2 cells and scaled
+    to a standard deviation of `amplitude` (m/s)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    out = []
+    for shape in ((n, n, n + 1), (n, n + 1, n), (n + 1, n, n)):
+        f = gaussian_filter(rng.standard_normal(shape, dtype=np.float32), sigma=correlation / 2.0, mode="wrap")
+        f *= np.float32(amplitude / max(float(f.std()), 1e-30))
+        out.append(f.astype(np.float32))
+    return tuple(out)
+
+
+def slab_phi(n, height=0.4):
+    """Liquid signed distance at cell centres ((k, j, i), float32): negative
+    below y = height (the slab), the distance to its top face elsewhere."""
+    dx = 1.0 / n
+    y = (np.arange(n, dtype=np.float64) + 0.5) * dx
+    phi = np.empty((n, n, n), np.float32)
+    phi[:] = (y - height).astype(np.float32)[None, :, None]
+    return phi
+
+
+def precision_slab(n, seed=0, height=0.4, amplitude=0.5, correlation=8.0):
+    """Grid state of the precision configuration: (liquid phi, (u, v, w))."""
+    return slab_phi(n, height), noise_mac_field(n, seed, amplitude, correlation)
