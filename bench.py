@@ -99,11 +99,12 @@ def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt):
                                                                    st.pressure_solver_iterations, ps["unknowns"]))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
     kt = kernel_times(sim)
     nparticles = sim.get_num_marker_particles()
     del sim
     return {"elapsed_s": elapsed, "substeps": substeps, "iterations": iters, "unknowns": unknowns,
-            "kernels": kt, "particles": nparticles}
+            "kernels": kt, "particles": nparticles, "device_bytes_used": total - free}
 
 
 def algorithmic_bytes(name, n, P, M_total, launches_per_sub, substeps, iters_total, mode):
@@ -246,6 +247,7 @@ def splash_config(n, steps, warmup, tol, mode):
     """The large single-GPU configuration: a sphere of liquid (r = 0.2) falling
     into a pool over 30% of the height, 8 particles per liquid cell."""
     import torch
+    _, total = torch.cuda.mem_get_info()
     log("splash n=%d: seeding particles" % n)
     pos = splash_particles(n)
     log("  %d particles" % len(pos))
@@ -254,6 +256,7 @@ def splash_config(n, steps, warmup, tol, mode):
     P = res["particles"]
     kt = res["kernels"]
     M_total = sum(u * it for u, it in zip(res["unknowns"], res["iterations"]))
+    peak_used = res.get("device_bytes_used", 0)
     gbs = {}
     for k, v in kt.items():
         b = algorithmic_bytes(k, n, P, M_total, 1, res["substeps"], sum(res["iterations"]), mode)
@@ -268,7 +271,7 @@ def splash_config(n, steps, warmup, tol, mode):
             "particle_updates_per_s": round(rate * P, 1),
             "pressure_iterations": res["iterations"], "unknowns": res["unknowns"],
             "kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items()}, "kernel_gb_per_s": gbs,
-            "gpu_memory_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)}
+            "device_memory_gb": round(peak_used / 1e9, 2), "device_memory_total_gb": round(total / 1e9, 1)}
     print(json.dumps(line), flush=True)
 
 
