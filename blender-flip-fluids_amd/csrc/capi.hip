@@ -494,6 +494,39 @@ int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u,
     });
 }
 
+int FFAMD_projection_vjp(int ni, int nj, int nk, double dx, double dt, const float *wu, const float *wv,
+                         const float *ww, const float *phi, const float *gu, const float *gv, const float *gw,
+                         double tol, int maxIter, float *ou, float *ov, float *ow, int *iterations, int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t N = (size_t)ni * nj * nk;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> zu(nu), zv(nv), zw(nw), a(nu), b(nv), c(nw), dphi(N), dp(N), gu_(nu), gv_(nv), gw_(nw);
+        DBuf<float> du(nu), dv(nv), dw(nw);
+        DBuf<double> pbar(N), lambda(N);
+        DBuf<int> scan(ff::scanScratchInts(N + 1));
+        up(a.p, wu, nu, sc.s); up(b.p, wv, nv, sc.s); up(c.p, ww, nw, sc.s);
+        up(dphi.p, phi, N, sc.s);
+        up(gu_.p, gu, nu, sc.s); up(gv_.p, gv, nv, sc.s); up(gw_.p, gw, nw, sc.s);
+        ff::hipCheck(hipMemsetAsync(zu, 0, sizeof(float) * nu, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(zv, 0, sizeof(float) * nv, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(zw, 0, sizeof(float) * nw, sc.s), "memset");
+        // C^T u'bar on the cells, then A lambda = that on the rows (FP64 PCG)
+        ff::projectionAdjointCells(g, dt, dphi, a, b, c, gu_, gv_, gw_, pbar, sc.s);
+        ff::PressureSolver ps;
+        ps.allocate(g);
+        ff::PressureResult r = ps.solve(dt, zu, zv, zw, a, b, c, dphi, tol, maxIter, 0, 1e-4, dp, scan, sc.s, nullptr,
+                                        nullptr, 0, pbar, lambda);
+        ff::projectionAdjointFaces(g, dt, dphi, a, b, c, gu_, gv_, gw_, lambda, du, dv, dw, sc.s);
+        down(ou, du.p, nu, sc.s); down(ov, dv.p, nv, sc.s); down(ow, dw.p, nw, sc.s);
+        sync(sc.s);
+        *iterations = r.iterations;
+        return r.success ? 1 : 0;
+    });
+}
+
 void FFAMD_mic0_band_shape(int fp64, int *levels, int *tile, int *err) {
     guarded(err, [&] {
         int k = 0, bt = 0;

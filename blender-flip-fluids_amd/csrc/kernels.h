@@ -210,6 +210,18 @@ void castToFloat(const double *a, float *b, size_t n, hipStream_t s);
 void addFloatToDouble(double *x, const float *d, size_t n, hipStream_t s);
 void mic0ToFloat(PressureSys &S, hipStream_t s);
 
+// ---- adjoint of the pressure projection (kernels_adjoint.hip) ----
+// pbar (N cells) = C^T u'bar: the pressure-gradient update's transpose
+void projectionAdjointCells(Grid g, double dt, const float *phi, const float *wu, const float *wv, const float *ww,
+                            const float *gu, const float *gv, const float *gw, double *pbar, hipStream_t s);
+// ubar = K u'bar + B^T lambda (lambda on cells, 0 off the pressure rows)
+void projectionAdjointFaces(Grid g, double dt, const float *phi, const float *wu, const float *wv, const float *ww,
+                            const float *gu, const float *gv, const float *gw, const double *lambda, float *ou,
+                            float *ov, float *ow, hipStream_t s);
+// rhs[r] = cells[rows[r]];  cells[rows[r]] = x[r] (other cells untouched)
+void gatherRows(const PressureSys &S, const double *cells, double *rhs, hipStream_t s);
+void scatterRows(const PressureSys &S, const double *x, double *cells, hipStream_t s);
+
 // ---- MIC(0) application, persistent tile wavefront (kernels_mic.hip) ----
 // Tiles are 16 x 16 x depth cells; the schedule lists the non-empty tiles by
 // tile level i/16 + j/16 + k/depth.  base and epoch advance with every launch.

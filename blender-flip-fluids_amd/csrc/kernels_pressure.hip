@@ -344,6 +344,14 @@ __global__ void axpyMixedKernel(double *x, const float *p, size_t n, const doubl
     x[q] += (double)fa * (double)p[q];
 }
 
+__global__ void gatherRowsKernel(int m, const int *rows, const double *cells, double *rhs) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m) rhs[r] = cells[rows[r]];
+}
+__global__ void scatterRowsKernel(int m, const int *rows, const double *x, double *cells) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m) cells[rows[r]] = x[r];
+}
 __global__ void toGridKernel(int m, const int *rows, const double *x, float *p) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < m) p[rows[r]] = (float)x[r];
@@ -507,6 +515,14 @@ void pressureToGrid(Grid g, const PressureSys &S, const double *x, float *p, hip
     if (S.m > 0) hipLaunchKernelGGL(toGridKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.rows, x, p);
 }
 
+void gatherRows(const PressureSys &S, const double *cells, double *rhs, hipStream_t s) {
+    if (S.m == 0) return;
+    hipLaunchKernelGGL(gatherRowsKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.rows, cells, rhs);
+}
+void scatterRows(const PressureSys &S, const double *x, double *cells, hipStream_t s) {
+    if (S.m == 0) return;
+    hipLaunchKernelGGL(scatterRowsKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.rows, x, cells);
+}
 void castToFloat(const double *a, float *b, size_t n, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(castKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, a, b, n);

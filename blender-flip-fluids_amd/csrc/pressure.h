@@ -29,7 +29,11 @@ public:
 
     void allocate(Grid g);
     // Builds the system from the current fields, solves it and writes the
-    // float pressure grid.  referenceDots > 0 makes the FP64 solve (mode 0)
+    // float pressure grid.  cellRhs (N doubles, optional) replaces the
+    // assembled right-hand side by its values at the rows' cells (the matrix
+    // still comes from phi and the weights); cellSolution (N doubles,
+    // optional) receives the FP64 solution on the rows' cells and 0 elsewhere.
+    // referenceDots > 0 makes the FP64 solve (mode 0)
     // take its dot products in the reference's order for that thread count.
     // mode 0 = FP64 PCG; 1 = FP32 PCG restarted inside
     // FP64 iterative refinement (innerTol = relative tolerance of each inner
@@ -39,7 +43,8 @@ public:
     PressureResult solve(double dt, const float *u, const float *v, const float *w, const float *wu,
                          const float *wv, const float *ww, const float *phi, double tol, int maxIter,
                          int mode, double innerTol, float *pressure, int *scanScratch, hipStream_t s,
-                         hipEvent_t afterAssembly, KernelTimers *timers = nullptr, int referenceDots = 0);
+                         hipEvent_t afterAssembly, KernelTimers *timers = nullptr, int referenceDots = 0,
+                         const double *cellRhs = nullptr, double *cellSolution = nullptr);
     const PressureSys &system() const { return _S; }
     const std::vector<int> &levelTable() const { return _hLevelStart; }
     // one preconditioner application z = (L L^T)^-1 r on the current factor

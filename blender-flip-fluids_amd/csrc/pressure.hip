@@ -202,9 +202,12 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
                                      const float *wu, const float *wv, const float *ww, const float *phi,
                                      double tol, int maxIter, int mode, double innerTol, float *pressure,
                                      int *scanScratch, hipStream_t s, hipEvent_t afterAssembly,
-                                     KernelTimers *timers, int referenceDots) {
+                                     KernelTimers *timers, int referenceDots, const double *cellRhs,
+                                     double *cellSolution) {
     PressureResult res;
     _refDots = referenceDots;
+    if (cellSolution)
+        hipCheck(hipMemsetAsync(cellSolution, 0, sizeof(double) * (size_t)_g.ni * _g.nj * _g.nk, s), "memset");
     _t = timers ? timers : &_noTimers;
     hipCheck(hipMemsetAsync(_ms.error, 0, sizeof(int), s), "memset error word");
     _t->begin(T_ASSEMBLE, s);
@@ -218,6 +221,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     res.unknowns = m;
     pressureRows(_g, _S.mask, _S.rowOf, _S.rows, _S.cellRow, s);
     pressureAssemble(_g, dt, u, v, w, wu, wv, ww, phi, _S, s);
+    if (cellRhs) gatherRows(_S, cellRhs, _S.rhs, s);
     pressureLevels(_g, _S, scanScratch, s);
     _t->end(T_ASSEMBLE, s);
     _hLevelStart.resize(_S.nLevels + 1);
@@ -251,6 +255,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
                             : pcgPrecond32(tol, maxIter, maxAbs, iters, resid, s);
     checkError(s);
     pressureToGrid(_g, _S, _x, pressure, s);
+    if (cellSolution) scatterRows(_S, _x, cellSolution, s);
     float e = (float)resid;
     res.success = conv || (iters == maxIter && e < 1.0);
     res.iterations = iters;

@@ -149,3 +149,19 @@ def sample_velocity(n3, dx, uvw, pos):
     _run("FFAMD_sample_velocity", [c_int] * 3 + [c_double] + [c_void_p] * 3 + [c_int, c_void_p, c_void_p],
          [ni, nj, nk, dx, _fp(u), _fp(v), _fp(w), len(pos), _fp(pos), _fp(out)])
     return out
+
+
+def projection_vjp(n3, dx, dt, weights, liquid_phi, grads, tol=1e-12, max_iter=5000):
+    """Adjoint of the pressure projection (pressure_solve's velocity update)
+    for these weights and liquid SDF: returns (u, v, w) = P^T (grads)."""
+    ni, nj, nk = n3
+    wu, wv, ww = (_f32(a) for a in weights)
+    phi = _f32(liquid_phi)
+    gu, gv, gw = (_f32(a) for a in grads)
+    ou, ov, ow = (np.zeros(a.shape, np.float32) for a in (gu, gv, gw))
+    it = c_int()
+    ok = _run("FFAMD_projection_vjp",
+              [c_int] * 3 + [c_double, c_double] + [c_void_p] * 7 + [c_double, c_int] + [c_void_p] * 4,
+              [ni, nj, nk, dx, dt, _fp(wu), _fp(wv), _fp(ww), _fp(phi), _fp(gu), _fp(gv), _fp(gw), tol, max_iter,
+               _fp(ou), _fp(ov), _fp(ow), byref(it)], c_int)
+    return (ou, ov, ow), it.value, bool(ok)

@@ -248,6 +248,19 @@ int FFAMD_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, cons
                      const float *v, const float *w, const float *wu, const float *wv,
                      const float *ww, const float *liquid_phi, int precision, int schedule,
                      int n, const double *r, double *z, int *err);
+/* Adjoint of the pressure projection (no counterpart in the reference):
+ * for the liquid SDF and face weights of a FFAMD_pressure_solve call, the
+ * projection u' = P(u) is linear, and this returns ubar = P^T u'bar on the
+ * three face grids.  The pressure system's matrix is symmetric, so the
+ * adjoint solve is the same FP64 MIC(0)-PCG (tolerance tol relative to its
+ * right-hand side).  The forward's rounding of the pressure to float is not
+ * differentiated.  Returns 1 when the solve converged. */
+int FFAMD_projection_vjp(int isize, int jsize, int ksize, double dx, double dt, const float *wu,
+                         const float *wv, const float *ww, const float *liquid_phi,
+                         const float *grad_u, const float *grad_v, const float *grad_w, double tol,
+                         int max_iter, float *out_u, float *out_v, float *out_w, int *iterations,
+                         int *err);
+
 /* Shape of the band schedule of the MIC(0) application (no counterpart in
  * the reference): hyperplanes per launch and tile side in lines, for the FP64
  * (fp64 = 1) or the FP32 factor. */
