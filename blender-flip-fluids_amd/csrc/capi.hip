@@ -494,6 +494,65 @@ int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u,
     });
 }
 
+void FFAMD_p2g_vjp(int ni, int nj, int nk, double dx, int np, const float *pos, double radius, const float *gu,
+                   const float *gv, const float *gw, float *vbar, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (np < 0) throw std::domain_error("negative particle count");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        // the forward's weight sums (velocities do not enter them)
+        SortedParticles sp(g, np, pos, nullptr, sc.s);
+        DBuf<float> fu(nu), fv(nv), fw(nw), wsu(nu), wsv(nv), wsw(nw), gu_(nu), gv_(nv), gw_(nw);
+        DBuf<uint8_t> bu(nu), bv(nv), bw(nw);
+        ff::p2g(g, sp.cs, radius, fu, fv, fw, bu, bv, bw, sc.s, wsu, wsv, wsw);
+        up(gu_.p, gu, nu, sc.s); up(gv_.p, gv, nv, sc.s); up(gw_.p, gw, nw, sc.s);
+        DBuf<float> vx(np), vy(np), vz(np);
+        ff::p2gAdjoint(g, np, *sp.f[0], *sp.f[1], *sp.f[2], radius, gu_, gv_, gw_, wsu, wsv, wsw, vx, vy, vz, sc.s);
+        std::vector<float> hx(np), hy(np), hz(np);
+        down(hx.data(), vx.p, np, sc.s); down(hy.data(), vy.p, np, sc.s); down(hz.data(), vz.p, np, sc.s);
+        sync(sc.s);
+        for (int q = 0; q < np; q++) {
+            vbar[3 * (size_t)q] = hx[q];
+            vbar[3 * (size_t)q + 1] = hy[q];
+            vbar[3 * (size_t)q + 2] = hz[q];
+        }
+    });
+}
+
+void FFAMD_sample_velocity_vjp(int ni, int nj, int nk, double dx, int np, const float *pos, const float *vbar,
+                               float *u, float *v, float *w, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (np < 0) throw std::domain_error("negative particle count");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        const size_t P = (size_t)np;
+        std::vector<float> c(P);
+        DBuf<float> px(P), py(P), pz(P), vb(3 * P);
+        float *dst[3] = {px.p, py.p, pz.p};
+        for (int a = 0; a < 3; a++) {
+            for (size_t q = 0; q < P; q++) c[q] = pos[3 * q + a];
+            up(dst[a], c.data(), P, sc.s);
+            sync(sc.s);
+        }
+        up(vb.p, vbar, 3 * P, sc.s);
+        size_t nbMax = 0;
+        for (int d = 0; d < 3; d++) nbMax = std::max(nbMax, ff::sampleBuckets(g, d));
+        DBuf<int> key(P), slot(P), perm(P), counts(nbMax + 1), offsets(nbMax + 1),
+            scan(ff::scanScratchInts(nbMax + 1));
+        float *outs[3] = {u, v, w};
+        for (int d = 0; d < 3; d++) {
+            const size_t nf = (size_t)(ni + (d == 0)) * (nj + (d == 1)) * (nk + (d == 2));
+            DBuf<float> o(nf);
+            ff::sampleAdjoint(g, d, np, px, py, pz, vb, key, slot, counts, offsets, perm, scan, o, sc.s);
+            down(outs[d], o.p, nf, sc.s);
+            sync(sc.s);
+        }
+    });
+}
+
 int FFAMD_projection_vjp(int ni, int nj, int nk, double dx, double dt, const float *wu, const float *wv,
                          const float *ww, const float *phi, const float *gu, const float *gv, const float *gw,
                          double tol, int maxIter, float *ou, float *ov, float *ow, int *iterations, int *err) {

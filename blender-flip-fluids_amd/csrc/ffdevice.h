@@ -117,4 +117,38 @@ struct Grid {
     double dx;
 };
 
+// The FLIP transfer's frame of face (fi, fj, fk) of direction dir
+// (velocityadvector.cpp:177-188, 467-536): particles are shifted by the
+// direction's half-cell offset and taken relative to the face's 10^3 block,
+// in which the face sits at gpos.
+constexpr int P2G_BLOCK = 10;
+struct FaceFrame {
+    V3 offd, offb, gpos;
+};
+FF_HD FaceFrame p2gFrame(Grid g, int dir, int fi, int fj, int fk) {
+    const double h = 0.5 * g.dx;
+    const int bi = fi / P2G_BLOCK, bj = fj / P2G_BLOCK, bk = fk / P2G_BLOCK;
+    return FaceFrame{dir == 0 ? mk(0.0, h, h) : (dir == 1 ? mk(h, 0.0, h) : mk(h, h, 0.0)),
+                     indexToPos(bi, bj, bk, P2G_BLOCK * g.dx),
+                     indexToPos(fi - bi * P2G_BLOCK, fj - bj * P2G_BLOCK, fk - bk * P2G_BLOCK, g.dx)};
+}
+// squared distance of a particle from the face in that frame
+FF_HD float p2gDist2(const FaceFrame &F, float x, float y, float z) {
+    V3 p = sub(sub(V3{x, y, z}, F.offd), F.offb);
+    V3 v = sub(F.gpos, p);
+    return dot3(v, v);
+}
+// the transfer kernel's weight at squared distance d2 < r^2
+struct P2GWeight {
+    float rsq, coef1, coef2, coef3;
+};
+FF_HD P2GWeight p2gWeight(double radius) {
+    const float r = (float)radius;
+    return P2GWeight{r * r, (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r)), (17.0f / 9.0f) * (1.0f / (r * r * r * r)),
+                     (22.0f / 9.0f) * (1.0f / (r * r))};
+}
+FF_HD float p2gWeightAt(const P2GWeight &W, float d2) {
+    return 1.0f - W.coef1 * d2 * d2 * d2 + W.coef2 * d2 * d2 - W.coef3 * d2;
+}
+
 }  // namespace ff

@@ -66,9 +66,11 @@ struct CellSort {
 void cellSort(Grid g, const Particles &p, CellSort &cs, int *scanScratch, hipStream_t s);
 void liquidSDF(Grid g, const CellSort &cs, int np, double radius, float *phi, hipStream_t s);
 void liquidSDFPostprocess(Grid g, const float *solidNodes, float *phi, hipStream_t s);
-// P2G of all three components into u, v, w with validity flags
+// P2G of all three components into u, v, w with validity flags; wsu/wsv/wsw
+// (optional) receive each face's weight sum
 void p2g(Grid g, const CellSort &cs, double radius, float *u, float *v, float *w, uint8_t *vu,
-         uint8_t *vv, uint8_t *vw, hipStream_t s);
+         uint8_t *vv, uint8_t *vw, hipStream_t s, float *wsu = nullptr, float *wsv = nullptr,
+         float *wsw = nullptr);
 // max over particles of dot(v, v) (float) as double
 void maxSpeedSq(const Particles &p, double *partial, double *out, hipStream_t s);
 
@@ -218,6 +220,18 @@ void projectionAdjointCells(Grid g, double dt, const float *phi, const float *wu
 void projectionAdjointFaces(Grid g, double dt, const float *phi, const float *wu, const float *wv, const float *ww,
                             const float *gu, const float *gv, const float *gw, const double *lambda, float *ou,
                             float *ov, float *ow, hipStream_t s);
+// adjoint of P2G for fixed positions: vbar (per particle, SoA) from ubar on
+// the faces and the forward's per-face weight sums
+void p2gAdjoint(Grid g, int np, const float *px, const float *py, const float *pz, double radius, const float *gu,
+                const float *gv, const float *gw, const float *wsu, const float *wsv, const float *wsw, float *vbx,
+                float *vby, float *vbz, hipStream_t s);
+// adjoint of trilinear sampling of component dir: ubar on that face grid from
+// vbar (P x 3, interleaved).  key/slot/perm hold P ints, counts/offsets
+// sampleBuckets(g, dir) + 1 ints, scanScratch scanScratchInts of that.
+size_t sampleBuckets(Grid g, int dir);
+void sampleAdjoint(Grid g, int dir, int np, const float *px, const float *py, const float *pz, const float *vbar,
+                   int *key, int *slot, int *counts, int *offsets, int *perm, int *scanScratch, float *out,
+                   hipStream_t s);
 // rhs[r] = cells[rows[r]];  cells[rows[r]] = x[r] (other cells untouched)
 void gatherRows(const PressureSys &S, const double *cells, double *rhs, hipStream_t s);
 void scatterRows(const PressureSys &S, const double *x, double *cells, hipStream_t s);

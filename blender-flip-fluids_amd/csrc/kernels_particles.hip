@@ -182,23 +182,13 @@ constexpr int P2G_CAP = 5500;                     // staged particles (LDS: 28 b
 // uses no dynamically indexed registers or private memory.
 template <int dir, class F, class I, class R>
 __device__ __forceinline__ void p2gFace(Grid g, int fi, int fj, int fk, R range, F x, F y, F z, F vel,
-                                        I pidx, double radius, float *field, uint8_t *valid, size_t f) {
+                                        I pidx, double radius, float *field, uint8_t *valid, float *wsumOut,
+                                        size_t f) {
     const float eps = 1e-6f;
-    const float r = (float)radius;
-    const float rsq = r * r;
-    const float coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
-    const float coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
-    const float coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
-    const double h = 0.5 * g.dx;
-    const V3 offd = dir == 0 ? mk(0.0, h, h) : (dir == 1 ? mk(h, 0.0, h) : mk(h, h, 0.0));
-    const int bi = fi / BW, bj = fj / BW, bk = fk / BW;
-    const V3 offb = indexToPos(bi, bj, bk, BW * g.dx);
-    const V3 gpos = indexToPos(fi - bi * BW, fj - bj * BW, fk - bk * BW, g.dx);
-    auto d2of = [&](int s) {
-        V3 p = sub(sub(V3{x[s], y[s], z[s]}, offd), offb);
-        V3 v = sub(gpos, p);
-        return dot3(v, v);
-    };
+    const P2GWeight W = p2gWeight(radius);
+    const float rsq = W.rsq;
+    const FaceFrame frame = p2gFrame(g, dir, fi, fj, fk);
+    auto d2of = [&](int s) { return p2gDist2(frame, x[s], y[s], z[s]); };
     // first particle at or after c within r: its particle index (or none) and distance
     auto next = [&](int &c, int e, float &d2) {
         while (c < e) {
@@ -242,7 +232,7 @@ __device__ __forceinline__ void p2gFace(Grid g, int fi, int fj, int fk, R range,
         if (best < 0) break;
         float d2 = d2of(bcur);
         do {
-            const float weight = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
+            const float weight = p2gWeightAt(W, d2);
             sum += weight * vel[bcur];
             wsum += weight;
             bcur++;
@@ -251,6 +241,7 @@ __device__ __forceinline__ void p2gFace(Grid g, int fi, int fj, int fk, R range,
         put(cur, best, bcur);
         put(head, best, bidx);
     }
+    if (wsumOut) wsumOut[f] = wsum;
     if (wsum > eps) sum /= wsum;
     field[f] = sum;
     valid[f] = wsum > eps ? 1 : 0;
@@ -259,6 +250,7 @@ __device__ __forceinline__ void p2gFace(Grid g, int fi, int fj, int fk, R range,
 struct P2GOut {
     float *f[3];
     uint8_t *valid[3];
+    float *wsum[3];  // optional: each face's weight sum
 };
 
 // face a of the tile at (I0, J0, K0) in direction dir, from the staged
@@ -276,6 +268,7 @@ __device__ __forceinline__ void p2gTileFace(Grid g, int a, int I0, int J0, int K
     if (total == 0) {  // no particle can reach the tile's faces
         out.f[dir][f] = 0.0f;
         out.valid[dir][f] = 0;
+        if (out.wsum[dir]) out.wsum[dir][f] = 0.0f;
         return;
     }
     // candidate cells: 2 along the face normal, 3 across; candidate c of the
@@ -293,9 +286,10 @@ __device__ __forceinline__ void p2gTileFace(Grid g, int a, int I0, int J0, int K
         }
     };
     if (staged)
-        p2gFace<dir>(g, fi, fj, fk, range, sx, sy, sz, sv, sp, radius, out.f[dir], out.valid[dir], f);
+        p2gFace<dir>(g, fi, fj, fk, range, sx, sy, sz, sv, sp, radius, out.f[dir], out.valid[dir], out.wsum[dir], f);
     else
-        p2gFace<dir>(g, fi, fj, fk, range, spx, spy, spz, svel, perm, radius, out.f[dir], out.valid[dir], f);
+        p2gFace<dir>(g, fi, fj, fk, range, spx, spy, spz, svel, perm, radius, out.f[dir], out.valid[dir],
+                     out.wsum[dir], f);
 }
 
 __global__ __launch_bounds__(P2_THREADS) void p2gTileKernel(Grid g, const int *offsets, const int *perm,
@@ -710,10 +704,10 @@ void liquidSDFPostprocess(Grid g, const float *sp, float *phi, hipStream_t s) {
 }
 
 void p2g(Grid g, const CellSort &cs, double radius, float *u, float *v, float *w, uint8_t *vu,
-         uint8_t *vv, uint8_t *vw, hipStream_t s) {
+         uint8_t *vv, uint8_t *vw, hipStream_t s, float *wsu, float *wsv, float *wsw) {
     const int tilesI = (g.ni + 1 + P2T_I - 1) / P2T_I, tilesJ = (g.nj + 1 + P2T_J - 1) / P2T_J,
               tilesK = (g.nk + 1 + P2T_K - 1) / P2T_K;
-    P2GOut out{{u, v, w}, {vu, vv, vw}};
+    P2GOut out{{u, v, w}, {vu, vv, vw}, {wsu, wsv, wsw}};
     hipLaunchKernelGGL(p2gTileKernel, dim3((unsigned)tilesI * tilesJ * tilesK), dim3(P2_THREADS), 0, s, g,
                        cs.offsets, cs.perm, cs.spx, cs.spy, cs.spz, cs.svx, cs.svy, cs.svz, radius, out, tilesI,
                        tilesJ);

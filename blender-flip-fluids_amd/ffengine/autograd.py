@@ -5,6 +5,9 @@ PCG solve, velocity update; pressuresolver.cpp:52-104, 1047-1205) as a
 function of the three MAC face grids, for a fixed liquid SDF and fixed face
 weights.  Its forward runs FFAMD_pressure_solve, its backward the
 hand-written adjoint FFAMD_projection_vjp (the same symmetric Poisson solve).
+P2GTransfer and SampleVelocity: the particle-to-grid transfer and the
+grid-to-particle interpolation for fixed particle positions, backed by
+FFAMD_p2g_vjp and FFAMD_sample_velocity_vjp.
 Tensors live on the host: the entry points take and return host arrays, and
 the device work happens inside the library calls.
 """
@@ -43,3 +46,38 @@ class PressureProjection(torch.autograd.Function):
 
 def pressure_projection(u, v, w, dx, dt, weights, liquid_phi, tol=1e-9):
     return PressureProjection.apply(u, v, w, dx, dt, weights, liquid_phi, tol)
+
+
+class P2GTransfer(torch.autograd.Function):
+    """Particle velocities (P, 3) -> the three MAC face grids of the FLIP
+    transfer (velocityadvector.cpp:467-536), for fixed positions (P, 3)."""
+
+    @staticmethod
+    def forward(ctx, vel, pos, n3, dx, radius):
+        (u, v, w), _ = kernels.p2g(n3, dx, pos, vel.detach().cpu().numpy().astype(np.float32), radius)
+        ctx.static = (pos, n3, dx, radius)
+        return torch.from_numpy(u), torch.from_numpy(v), torch.from_numpy(w)
+
+    @staticmethod
+    def backward(ctx, gu, gv, gw):
+        pos, n3, dx, radius = ctx.static
+        grads = [g.detach().cpu().numpy().astype(np.float32) for g in (gu, gv, gw)]
+        return torch.from_numpy(kernels.p2g_vjp(n3, dx, pos, radius, grads)), None, None, None, None
+
+
+class SampleVelocity(torch.autograd.Function):
+    """The three MAC face grids -> trilinear velocity samples (P, 3) at fixed
+    positions (macvelocityfield.cpp:519-645), the G2P interpolation."""
+
+    @staticmethod
+    def forward(ctx, u, v, w, pos, dx):
+        n3 = (u.shape[2] - 1, u.shape[1], u.shape[0])
+        arrays = [t.detach().cpu().numpy().astype(np.float32) for t in (u, v, w)]
+        ctx.static = (n3, dx, pos)
+        return torch.from_numpy(kernels.sample_velocity(n3, dx, arrays, pos))
+
+    @staticmethod
+    def backward(ctx, gvel):
+        n3, dx, pos = ctx.static
+        u, v, w = kernels.sample_velocity_vjp(n3, dx, pos, gvel.detach().cpu().numpy().astype(np.float32))
+        return torch.from_numpy(u), torch.from_numpy(v), torch.from_numpy(w), None, None

@@ -165,3 +165,27 @@ def projection_vjp(n3, dx, dt, weights, liquid_phi, grads, tol=1e-12, max_iter=5
               [ni, nj, nk, dx, dt, _fp(wu), _fp(wv), _fp(ww), _fp(phi), _fp(gu), _fp(gv), _fp(gw), tol, max_iter,
                _fp(ou), _fp(ov), _fp(ow), byref(it)], c_int)
     return (ou, ov, ow), it.value, bool(ok)
+
+
+def p2g_vjp(n3, dx, pos, radius, grads):
+    """Adjoint of p2g in the particle velocities (positions fixed): vbar (P, 3)."""
+    ni, nj, nk = n3
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    gu, gv, gw = (_f32(a) for a in grads)
+    vbar = np.zeros_like(pos)
+    _run("FFAMD_p2g_vjp", [c_int] * 3 + [c_double, c_int, c_void_p, c_double] + [c_void_p] * 4,
+         [ni, nj, nk, dx, len(pos), _fp(pos), radius, _fp(gu), _fp(gv), _fp(gw), _fp(vbar)])
+    return vbar
+
+
+def sample_velocity_vjp(n3, dx, pos, vbar):
+    """Adjoint of sample_velocity in the face grids (positions fixed): (u, v, w)."""
+    ni, nj, nk = n3
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vb = np.ascontiguousarray(vbar, np.float32).reshape(-1, 3)
+    u = np.zeros((nk, nj, ni + 1), np.float32)
+    v = np.zeros((nk, nj + 1, ni), np.float32)
+    w = np.zeros((nk + 1, nj, ni), np.float32)
+    _run("FFAMD_sample_velocity_vjp", [c_int] * 3 + [c_double, c_int] + [c_void_p] * 5,
+         [ni, nj, nk, dx, len(pos), _fp(pos), _fp(vb), _fp(u), _fp(v), _fp(w)])
+    return u, v, w

@@ -248,6 +248,18 @@ int FFAMD_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, cons
                      const float *v, const float *w, const float *wu, const float *wv,
                      const float *ww, const float *liquid_phi, int precision, int schedule,
                      int n, const double *r, double *z, int *err);
+/* Adjoints of the particle/grid transfers for fixed positions (no
+ * counterpart in the reference).  FFAMD_p2g_vjp: P2G (as FFAMD_p2g, radius r)
+ * is linear in the particle velocities; returns vbar (np x 3) = J^T of the
+ * three face-grid cotangents grad_u/v/w.  FFAMD_sample_velocity_vjp:
+ * trilinear sampling (as FFAMD_sample_velocity) is linear in the face grids;
+ * returns the face-grid cotangents u, v, w of the per-point cotangents vbar
+ * (np x 3).  Both gather in a fixed order (no float atomics). */
+void FFAMD_p2g_vjp(int isize, int jsize, int ksize, double dx, int np, const float *pos, double radius,
+                   const float *grad_u, const float *grad_v, const float *grad_w, float *vbar, int *err);
+void FFAMD_sample_velocity_vjp(int isize, int jsize, int ksize, double dx, int np, const float *pos,
+                               const float *vbar, float *u, float *v, float *w, int *err);
+
 /* Adjoint of the pressure projection (no counterpart in the reference):
  * for the liquid SDF and face weights of a FFAMD_pressure_solve call, the
  * projection u' = P(u) is linear, and this returns ubar = P^T u'bar on the

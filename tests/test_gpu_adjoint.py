@@ -128,3 +128,133 @@ def test_projection_autograd():
     say("projection autograd vs adjoint entry point")
     for f, g in zip(fields, (gu, gv, gw)):
         assert np.array_equal(f.grad.numpy(), g)
+
+
+def _points(n, count, seed, margin=0.02):
+    rng = np.random.default_rng(seed)
+    return (rng.random((count, 3)) * (1 + 2 * margin) - margin).astype(np.float32)
+
+
+def test_p2g_vjp_is_oracle_jacobian_transposed():
+    n = 8
+    n3, dx = (n, n, n), 1.0 / n
+    r = 0.5 * dx * np.sqrt(3.0)
+    pos = _points(n, 300, 3, margin=0.0)
+    P = len(pos)
+    shapes = _shapes(n3)
+    say(f"p2g Jacobian: {3 * P} oracle transfers")
+    cols = []
+    for q in range(P):
+        for d in range(3):
+            vel = np.zeros((P, 3), np.float32)
+            vel[q, d] = 1.0
+            (u, v, w), _ = O.p2g(n3, dx, pos, vel, r)
+            cols.append(np.concatenate([u.ravel(), v.ravel(), w.ravel()]))
+    J = np.stack(cols, 1).astype(np.float64)  # faces x (P*3)
+    rng = np.random.default_rng(4)
+    sizes = [int(np.prod(s)) for s in shapes]
+    for trial in range(3):
+        ub = rng.standard_normal(J.shape[0]).astype(np.float32)
+        want = (J.T @ ub.astype(np.float64)).reshape(P, 3)
+        parts = [p.reshape(s) for p, s in zip(np.split(ub, np.cumsum(sizes)[:-1]), shapes)]
+        got = K.p2g_vjp(n3, dx, pos, r, parts).astype(np.float64)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        say(f"  trial {trial}: max |error| / max |J^T u| = {err:.3g}")
+        assert err <= 1e-6
+
+
+def test_sample_vjp_is_oracle_jacobian_transposed():
+    n = 6
+    n3, dx = (n, n, n), 1.0 / n
+    pos = _points(n, 200, 5)
+    shapes = _shapes(n3)
+    sizes = [int(np.prod(s)) for s in shapes]
+    total = sum(sizes)
+    say(f"sampling Jacobian: {total} oracle samplings")
+    J = np.zeros((3 * len(pos), total))
+    for f in range(total):
+        e = np.zeros(total, np.float32)
+        e[f] = 1.0
+        parts = [p.reshape(s) for p, s in zip(np.split(e, np.cumsum(sizes)[:-1]), shapes)]
+        J[:, f] = O.sample_velocity(n3, dx, parts, pos).astype(np.float64).ravel()
+    rng = np.random.default_rng(6)
+    for trial in range(3):
+        vb = rng.standard_normal((len(pos), 3)).astype(np.float32)
+        want = J.T @ vb.astype(np.float64).ravel()
+        got = np.concatenate([a.ravel() for a in K.sample_velocity_vjp(n3, dx, pos, vb)]).astype(np.float64)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        say(f"  trial {trial}: max |error| / max |J^T v| = {err:.3g}")
+        assert err <= 1e-6
+
+
+def test_transfers_central_differences():
+    n = 24
+    n3, dx = (n, n, n), 1.0 / n
+    r = 0.5 * dx * np.sqrt(3.0)
+    pos = scenes.dam_break_particles(n, seed=8)
+    rng = np.random.default_rng(9)
+    shapes = _shapes(n3)
+    eps = 1e-3
+    # P2G: L(vel) = <ubar, P2G(vel)>, derivative in 64 random particle velocities
+    vel = (rng.standard_normal(pos.shape) * 0.01).astype(np.float32)
+    ub = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+
+    def lp(v):
+        out, _ = K.p2g(n3, dx, pos, v, r)
+        return sum(float(np.dot(o.ravel().astype(np.float64), b.ravel().astype(np.float64))) for o, b in zip(out, ub))
+
+    g = K.p2g_vjp(n3, dx, pos, r, ub)
+    worst = 0.0
+    for t in range(64):
+        q, d = int(rng.integers(len(pos))), int(rng.integers(3))
+        a, b = vel.copy(), vel.copy()
+        a[q, d] += eps
+        b[q, d] -= eps
+        h = float(a[q, d]) - float(b[q, d])
+        worst = max(worst, abs((lp(a) - lp(b)) / h - float(g[q, d])) / float(np.abs(g).max()))
+    say(f"p2g central differences: max |fd - adjoint| / max |adjoint| = {worst:.3g}")
+    assert worst <= 1e-3
+    # sampling: L(u) = <vbar, sample(u)>, derivative in 64 random faces
+    uvw = [(rng.standard_normal(s) * 0.01).astype(np.float32) for s in shapes]
+    q = _points(n, 5000, 10)
+    vb = rng.standard_normal((len(q), 3)).astype(np.float32)
+
+    def ls(fields):
+        return float(np.dot(K.sample_velocity(n3, dx, fields, q).astype(np.float64).ravel(),
+                            vb.astype(np.float64).ravel()))
+
+    gs = K.sample_velocity_vjp(n3, dx, q, vb)
+    scale = max(float(np.abs(a).max()) for a in gs)
+    worst = 0.0
+    for t in range(64):
+        d = int(rng.integers(3))
+        idx = tuple(int(rng.integers(s)) for s in shapes[d])
+        plus = [a.copy() for a in uvw]
+        minus = [a.copy() for a in uvw]
+        plus[d][idx] += eps
+        minus[d][idx] -= eps
+        h = float(plus[d][idx]) - float(minus[d][idx])
+        worst = max(worst, abs((ls(plus) - ls(minus)) / h - float(gs[d][idx])) / scale)
+    say(f"sampling central differences: max |fd - adjoint| / max |adjoint| = {worst:.3g}")
+    assert worst <= 1e-3
+
+
+def test_transfer_autograd():
+    torch = pytest.importorskip("torch")
+    from ffengine import autograd as AG
+    n = 16
+    n3, dx = (n, n, n), 1.0 / n
+    r = 0.5 * dx * np.sqrt(3.0)
+    pos = scenes.dam_break_particles(n, seed=11)
+    rng = np.random.default_rng(12)
+    vel = torch.tensor((rng.standard_normal(pos.shape) * 0.3).astype(np.float32), requires_grad=True)
+    u, v, w = AG.P2GTransfer.apply(vel, pos, n3, dx, r)
+    samples = AG.SampleVelocity.apply(u, v, w, pos, dx)
+    target = torch.from_numpy(rng.standard_normal(pos.shape).astype(np.float32))
+    loss = ((samples - target) ** 2).sum()
+    loss.backward()
+    say("transfer autograd: P2G then sampling, gradient against the chained entry points")
+    gvel = 2.0 * (samples.detach().numpy() - target.numpy())
+    gu, gv, gw = K.sample_velocity_vjp(n3, dx, pos, gvel.astype(np.float32))
+    want = K.p2g_vjp(n3, dx, pos, r, [gu, gv, gw])
+    assert np.array_equal(vel.grad.numpy(), want)
