@@ -553,6 +553,79 @@ void FFAMD_sample_velocity_vjp(int ni, int nj, int nk, double dx, int np, const 
     });
 }
 
+namespace {
+struct HostSoA {  // device SoA copies of a host (P x 3) array
+    DBuf<float> x, y, z;
+    HostSoA(const float *a, size_t P, hipStream_t s) : x(P), y(P), z(P) {
+        std::vector<float> c(P);
+        float *d[3] = {x.p, y.p, z.p};
+        for (int e = 0; e < 3; e++) {
+            for (size_t q = 0; q < P; q++) c[q] = a[3 * q + e];
+            up(d[e], c.data(), P, s);
+            sync(s);
+        }
+    }
+};
+}  // namespace
+
+void FFAMD_advect_rk3(int ni, int nj, int nk, double dx, const float *u, const float *v, const float *w, int np,
+                      const float *pos, double dt, float *out, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (np < 0) throw std::domain_error("negative particle count");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> du(nu), dv(nv), dw(nw);
+        up(du.p, u, nu, sc.s); up(dv.p, v, nv, sc.s); up(dw.p, w, nw, sc.s);
+        HostSoA p(pos, np, sc.s);
+        DBuf<float> ox(np), oy(np), oz(np);
+        ff::advectRK3(g, np, p.x, p.y, p.z, ff::MacView{du, dv, dw}, dt, ox, oy, oz, sc.s);
+        std::vector<float> hx(np), hy(np), hz(np);
+        down(hx.data(), ox.p, np, sc.s); down(hy.data(), oy.p, np, sc.s); down(hz.data(), oz.p, np, sc.s);
+        sync(sc.s);
+        for (int q = 0; q < np; q++) {
+            out[3 * (size_t)q] = hx[q];
+            out[3 * (size_t)q + 1] = hy[q];
+            out[3 * (size_t)q + 2] = hz[q];
+        }
+    });
+}
+
+void FFAMD_advect_rk3_vjp(int ni, int nj, int nk, double dx, const float *u, const float *v, const float *w, int np,
+                          const float *pos, double dt, const float *xbar, float *pbar, float *gu, float *gv,
+                          float *gw, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (np < 0) throw std::domain_error("negative particle count");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> du(nu), dv(nv), dw(nw);
+        up(du.p, u, nu, sc.s); up(dv.p, v, nv, sc.s); up(dw.p, w, nw, sc.s);
+        HostSoA p(pos, np, sc.s);
+        const size_t P = (size_t)np, P3 = 3 * P;
+        DBuf<float> xb(P3), pb(P3), pts(3 * P3), cot(3 * P3);
+        up(xb.p, xbar, P3, sc.s);
+        ff::advectRK3Adjoint(g, np, p.x, p.y, p.z, ff::MacView{du, dv, dw}, dt, xb, pb, pts, cot, sc.s);
+        down(pbar, pb.p, P3, sc.s);
+        // the face grids' cotangent: sampling adjoints of the three stages
+        size_t nbMax = 0;
+        for (int d = 0; d < 3; d++) nbMax = std::max(nbMax, ff::sampleBuckets(g, d));
+        DBuf<int> key(P3), slot(P3), perm(P3), counts(nbMax + 1), offsets(nbMax + 1),
+            scan(ff::scanScratchInts(nbMax + 1));
+        float *outs[3] = {gu, gv, gw};
+        for (int d = 0; d < 3; d++) {
+            const size_t nf = (size_t)(ni + (d == 0)) * (nj + (d == 1)) * (nk + (d == 2));
+            DBuf<float> o(nf);
+            ff::sampleAdjoint(g, d, (int)P3, pts.p, pts.p + P3, pts.p + 2 * P3, cot, key, slot, counts, offsets, perm,
+                              scan, o, sc.s);
+            down(outs[d], o.p, nf, sc.s);
+            sync(sc.s);
+        }
+    });
+}
+
 int FFAMD_projection_vjp(int ni, int nj, int nk, double dx, double dt, const float *wu, const float *wv,
                          const float *ww, const float *phi, const float *gu, const float *gv, const float *gw,
                          double tol, int maxIter, float *ou, float *ov, float *ow, int *iterations, int *err) {

@@ -81,6 +81,14 @@ struct MacView {
 void g2pAdvect(Grid g, Particles p, MacView cur, MacView saved, const float *solidNodes,
                const uint8_t *nearSolid, int ci, int cj, int ck, float picRatio, float flipRatio,
                double dt, double cfl, hipStream_t s);
+// RK3 advection through the MAC field without collision (positions out), and
+// its adjoint: pbar (P x 3) for xbar (P x 3), plus the 3P sample points
+// (pts: their x, then y, then z coordinates) with their velocity cotangents
+// (cot: 3P x 3) for the sampling adjoint
+void advectRK3(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur, double dt, float *ox,
+               float *oy, float *oz, hipStream_t s);
+void advectRK3Adjoint(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur, double dt,
+                      const float *xbar, float *pbar, float *pts, float *cot, hipStream_t s);
 // speed statistics for the extreme-velocity rule: counts[nbins] and max speed
 void speedHistogram(const Particles &p, double stepv, int nbins, int *counts, double *partial,
                     double *maxSpeed, hipStream_t s);

@@ -432,6 +432,102 @@ __device__ V3 sampleMac(Grid g, MacView M, V3 p) {
     return mk(xu, yv, zw);
 }
 
+// Gradient of the trilinear sample of each component at p: grad[d][e] =
+// d V_d / d p_e inside the cell the forward picks (floor of the shifted
+// coordinate), 0 outside the domain, where the forward returns 0.
+__device__ void sampleMacGrad(Grid g, MacView M, V3 p, double grad[3][3]) {
+    for (int d = 0; d < 3; d++)
+        for (int e = 0; e < 3; e++) grad[d][e] = 0.0;
+    double x = p.x, y = p.y, z = p.z;
+    if (!(x >= 0 && y >= 0 && z >= 0 && x < g.dx * g.ni && y < g.dx * g.nj && z < g.dx * g.nk)) return;
+    const double h = 0.5 * g.dx;
+    const double invdx = 1.0 / g.dx;
+    for (int d = 0; d < 3; d++) {
+        const FaceArr F = d == 0 ? FaceArr{M.u, g.ni + 1, g.nj, g.nk}
+                                 : (d == 1 ? FaceArr{M.v, g.ni, g.nj + 1, g.nk} : FaceArr{M.w, g.ni, g.nj, g.nk + 1});
+        const double xs = d == 0 ? x : x - h, ys = d == 1 ? y : y - h, zs = d == 2 ? z : z - h;
+        const int i = (int)floor(xs * invdx), j = (int)floor(ys * invdx), k = (int)floor(zs * invdx);
+        const double ix = (xs - (double)i * g.dx) * invdx, iy = (ys - (double)j * g.dx) * invdx,
+                     iz = (zs - (double)k * g.dx) * invdx;
+        for (int q = 0; q < 8; q++) {
+            const int ox = q & 1, oy = (q >> 1) & 1, oz = q >> 2;
+            const int a = i + ox, b = j + oy, c = k + oz;
+            if (!inRange(a, b, c, F.wi, F.wj, F.wk)) continue;
+            const double val = (double)F.a[flat(a, b, c, F.wi, F.wj)];
+            const double ax = ox ? ix : 1 - ix, ay = oy ? iy : 1 - iy, az = oz ? iz : 1 - iz;
+            grad[d][0] += val * (ox ? 1.0 : -1.0) * ay * az * invdx;
+            grad[d][1] += val * ax * (oy ? 1.0 : -1.0) * az * invdx;
+            grad[d][2] += val * ax * ay * (oz ? 1.0 : -1.0) * invdx;
+        }
+    }
+}
+
+// _RK3 (fluidsimulation.cpp:7616-7623) without the collision step
+__global__ void advectRK3Kernel(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur,
+                                double dt, float *ox, float *oy, float *oz) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const V3 p0{px[q], py[q], pz[q]};
+    const V3 k1 = sampleMac(g, cur, p0);
+    const V3 k2 = sampleMac(g, cur, add(p0, scale((float)(0.5 * dt), k1)));
+    const V3 k3 = sampleMac(g, cur, add(p0, scale((float)(0.75 * dt), k2)));
+    const V3 p1 = add(p0, scale((float)(dt / (double)9.0f), add(add(scale(2.0f, k1), scale(3.0f, k2)), scale(4.0f, k3))));
+    ox[q] = p1.x;
+    oy[q] = p1.y;
+    oz[q] = p1.z;
+}
+
+// Adjoint of advectRK3Kernel: pbar (P x 3) for xbar (P x 3), and the three
+// sample points and their velocity cotangents (k1 at p0, k2 at y2, k3 at y3)
+// whose sampling adjoints give the face-grid cotangent.
+__global__ void advectRK3AdjKernel(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur,
+                                   double dt, const float *xbar, float *pbar, float *pts, float *cot) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const V3 p0{px[q], py[q], pz[q]};
+    const float c2 = (float)(0.5 * dt), c3 = (float)(0.75 * dt), a = (float)(dt / (double)9.0f);
+    const V3 k1 = sampleMac(g, cur, p0);
+    const V3 y2 = add(p0, scale(c2, k1));
+    const V3 k2 = sampleMac(g, cur, y2);
+    const V3 y3 = add(p0, scale(c3, k2));
+    double xb[3] = {xbar[3 * (size_t)q], xbar[3 * (size_t)q + 1], xbar[3 * (size_t)q + 2]};
+    double kb1[3], kb2[3], kb3[3], pb[3], G[3][3];
+    for (int e = 0; e < 3; e++) {
+        kb1[e] = 2.0 * (double)a * xb[e];
+        kb2[e] = 3.0 * (double)a * xb[e];
+        kb3[e] = 4.0 * (double)a * xb[e];
+        pb[e] = xb[e];
+    }
+    // k3 = V(y3), y3 = p0 + c3 k2
+    sampleMacGrad(g, cur, y3, G);
+    for (int e = 0; e < 3; e++) {
+        const double yb = G[0][e] * kb3[0] + G[1][e] * kb3[1] + G[2][e] * kb3[2];
+        pb[e] += yb;
+        kb2[e] += (double)c3 * yb;
+    }
+    // k2 = V(y2), y2 = p0 + c2 k1
+    sampleMacGrad(g, cur, y2, G);
+    for (int e = 0; e < 3; e++) {
+        const double yb = G[0][e] * kb2[0] + G[1][e] * kb2[1] + G[2][e] * kb2[2];
+        pb[e] += yb;
+        kb1[e] += (double)c2 * yb;
+    }
+    // k1 = V(p0)
+    sampleMacGrad(g, cur, p0, G);
+    for (int e = 0; e < 3; e++) pb[e] += G[0][e] * kb1[0] + G[1][e] * kb1[1] + G[2][e] * kb1[2];
+    const V3 at[3] = {p0, y2, y3};
+    const double *kb[3] = {kb1, kb2, kb3};
+    for (int e = 0; e < 3; e++) pbar[3 * (size_t)q + e] = (float)pb[e];
+    const size_t n3 = 3 * (size_t)np;  // pts: x of all 3P points, then y, then z
+    for (int s = 0; s < 3; s++) {
+        const size_t o = (size_t)s * np + q;
+        pts[o] = at[s].x;
+        pts[n3 + o] = at[s].y;
+        pts[2 * n3 + o] = at[s].z;
+        for (int e = 0; e < 3; e++) cot[3 * o + e] = (float)kb[s][e];
+    }
+}
+
 // Interpolation::trilinearInterpolate on the nodal solid SDF (interpolation.cpp:72-112)
 __device__ float nodalTrilinear(Grid g, const float *sp, V3 p) {
     const int pi = g.ni + 1, pj = g.nj + 1, pk = g.nk + 1;
@@ -785,6 +881,20 @@ void compactParticles(Particles &src, Particles &dst, const int *keep, int *rank
     hipCheck(hipMemcpyAsync(&total, rank + src.n, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
     dst.n = total;
+}
+
+void advectRK3(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur, double dt, float *ox,
+               float *oy, float *oz, hipStream_t s) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(advectRK3Kernel, dim3(blocksFor(np, TPB)), dim3(TPB), 0, s, g, np, px, py, pz, cur, dt, ox, oy,
+                       oz);
+}
+
+void advectRK3Adjoint(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur, double dt,
+                      const float *xbar, float *pbar, float *pts, float *cot, hipStream_t s) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(advectRK3AdjKernel, dim3(blocksFor(np, TPB)), dim3(TPB), 0, s, g, np, px, py, pz, cur, dt, xbar,
+                       pbar, pts, cot);
 }
 
 }  // namespace ff

@@ -7,7 +7,8 @@ weights.  Its forward runs FFAMD_pressure_solve, its backward the
 hand-written adjoint FFAMD_projection_vjp (the same symmetric Poisson solve).
 P2GTransfer and SampleVelocity: the particle-to-grid transfer and the
 grid-to-particle interpolation for fixed particle positions, backed by
-FFAMD_p2g_vjp and FFAMD_sample_velocity_vjp.
+FFAMD_p2g_vjp and FFAMD_sample_velocity_vjp.  AdvectRK3: the RK3 position
+update through the MAC field, backed by FFAMD_advect_rk3_vjp.
 Tensors live on the host: the entry points take and return host arrays, and
 the device work happens inside the library calls.
 """
@@ -81,3 +82,25 @@ class SampleVelocity(torch.autograd.Function):
         n3, dx, pos = ctx.static
         u, v, w = kernels.sample_velocity_vjp(n3, dx, pos, gvel.detach().cpu().numpy().astype(np.float32))
         return torch.from_numpy(u), torch.from_numpy(v), torch.from_numpy(w), None, None
+
+
+class AdvectRK3(torch.autograd.Function):
+    """Positions (P, 3) and the three MAC face grids -> positions after the
+    RK3 advection step (fluidsimulation.cpp:7616-7623), without the collision
+    step; differentiable in both the positions and the grids."""
+
+    @staticmethod
+    def forward(ctx, pos, u, v, w, dx, dt):
+        n3 = (u.shape[2] - 1, u.shape[1], u.shape[0])
+        arrays = [t.detach().cpu().numpy().astype(np.float32) for t in (u, v, w)]
+        p = pos.detach().cpu().numpy().astype(np.float32)
+        ctx.static = (n3, dx, dt, arrays, p)
+        return torch.from_numpy(kernels.advect_rk3(n3, dx, arrays, p, dt))
+
+    @staticmethod
+    def backward(ctx, gpos):
+        n3, dx, dt, arrays, p = ctx.static
+        pbar, (gu, gv, gw) = kernels.advect_rk3_vjp(n3, dx, arrays, p, dt,
+                                                    gpos.detach().cpu().numpy().astype(np.float32))
+        return (torch.from_numpy(pbar), torch.from_numpy(gu), torch.from_numpy(gv), torch.from_numpy(gw), None,
+                None)

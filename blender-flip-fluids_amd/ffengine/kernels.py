@@ -189,3 +189,29 @@ def sample_velocity_vjp(n3, dx, pos, vbar):
     _run("FFAMD_sample_velocity_vjp", [c_int] * 3 + [c_double, c_int] + [c_void_p] * 5,
          [ni, nj, nk, dx, len(pos), _fp(pos), _fp(vb), _fp(u), _fp(v), _fp(w)])
     return u, v, w
+
+
+def advect_rk3(n3, dx, uvw, pos, dt):
+    """RK3 advection of positions through the MAC field, no collision."""
+    ni, nj, nk = n3
+    u, v, w = (_f32(a) for a in uvw)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    out = np.zeros_like(pos)
+    _run("FFAMD_advect_rk3", [c_int] * 3 + [c_double] + [c_void_p] * 3 + [c_int, c_void_p, c_double, c_void_p],
+         [ni, nj, nk, dx, _fp(u), _fp(v), _fp(w), len(pos), _fp(pos), dt, _fp(out)])
+    return out
+
+
+def advect_rk3_vjp(n3, dx, uvw, pos, dt, xbar):
+    """Adjoint of advect_rk3: (pbar (P, 3), (u, v, w) cotangents)."""
+    ni, nj, nk = n3
+    u, v, w = (_f32(a) for a in uvw)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    xb = np.ascontiguousarray(xbar, np.float32).reshape(-1, 3)
+    pbar = np.zeros_like(pos)
+    gu, gv, gw = (np.zeros(a.shape, np.float32) for a in (u, v, w))
+    _run("FFAMD_advect_rk3_vjp",
+         [c_int] * 3 + [c_double] + [c_void_p] * 3 + [c_int, c_void_p, c_double] + [c_void_p] * 5,
+         [ni, nj, nk, dx, _fp(u), _fp(v), _fp(w), len(pos), _fp(pos), dt, _fp(xb), _fp(pbar), _fp(gu), _fp(gv),
+          _fp(gw)])
+    return pbar, (gu, gv, gw)

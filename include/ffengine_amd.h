@@ -260,6 +260,18 @@ void FFAMD_p2g_vjp(int isize, int jsize, int ksize, double dx, int np, const flo
 void FFAMD_sample_velocity_vjp(int isize, int jsize, int ksize, double dx, int np, const float *pos,
                                const float *vbar, float *u, float *v, float *w, int *err);
 
+/* RK3 advection of positions through a MAC field (_RK3,
+ * fluidsimulation.cpp:7616-7623) without the collision step, and its adjoint
+ * (no counterpart in the reference): for the cotangent xbar (np x 3) of the
+ * new positions, pbar (np x 3) is the cotangent of the old positions (the
+ * trilinear field's gradient inside the cell the forward samples) and
+ * grad_u/v/w that of the face grids. */
+void FFAMD_advect_rk3(int isize, int jsize, int ksize, double dx, const float *u, const float *v,
+                      const float *w, int np, const float *pos, double dt, float *out, int *err);
+void FFAMD_advect_rk3_vjp(int isize, int jsize, int ksize, double dx, const float *u, const float *v,
+                          const float *w, int np, const float *pos, double dt, const float *xbar,
+                          float *pbar, float *grad_u, float *grad_v, float *grad_w, int *err);
+
 /* Adjoint of the pressure projection (no counterpart in the reference):
  * for the liquid SDF and face weights of a FFAMD_pressure_solve call, the
  * projection u' = P(u) is linear, and this returns ubar = P^T u'bar on the

@@ -258,3 +258,117 @@ def test_transfer_autograd():
     gu, gv, gw = K.sample_velocity_vjp(n3, dx, pos, gvel.astype(np.float32))
     want = K.p2g_vjp(n3, dx, pos, r, [gu, gv, gw])
     assert np.array_equal(vel.grad.numpy(), want)
+
+
+def _rk3_restated(n3, dx, uvw, p0, dt):
+    """_RK3 (fluidsimulation.cpp:7616-7623) in float32 over the oracle's
+    trilinear sampling: k1 = V(p0), k2 = V(p0 + 0.5 dt k1),
+    k3 = V(p0 + 0.75 dt k2), p1 = p0 + dt/9 (2 k1 + 3 k2 + 4 k3)."""
+    f = np.float32
+    k1 = O.sample_velocity(n3, dx, uvw, p0)
+    k2 = O.sample_velocity(n3, dx, uvw, p0 + f(0.5 * dt) * k1)
+    k3 = O.sample_velocity(n3, dx, uvw, p0 + f(0.75 * dt) * k2)
+    return p0 + f(dt / float(np.float32(9.0))) * ((f(2.0) * k1 + f(3.0) * k2) + f(4.0) * k3)
+
+
+def _linear_field(n3, dx, A, b):
+    """MAC face grids sampling the affine velocity V(x) = A x + b exactly."""
+    ni, nj, nk = n3
+    out = []
+    for d, shape in enumerate(_shapes(n3)):
+        k, j, i = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
+        x = [i * dx, j * dx, k * dx]
+        for e in range(3):
+            if e != d:
+                x[e] = x[e] + 0.5 * dx
+        out.append((A[d, 0] * x[0] + A[d, 1] * x[1] + A[d, 2] * x[2] + b[d]).astype(np.float32))
+    return out
+
+
+def test_advect_rk3_matches_restatement():
+    n = 24
+    n3, dx, dt = (n, n, n), 1.0 / n, 1.0 / 240.0
+    rng = np.random.default_rng(13)
+    uvw = [(rng.standard_normal(s) * 1.5).astype(np.float32) for s in _shapes(n3)]
+    pos = _points(n, 20000, 14)
+    got = K.advect_rk3(n3, dx, uvw, pos, dt)
+    want = _rk3_restated(n3, dx, uvw, pos, dt)
+    say("advect RK3 forward against the float32 restatement over the oracle's sampling")
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_advect_rk3_vjp_central_differences():
+    n = 24
+    n3, dx, dt = (n, n, n), 1.0 / n, 1.0 / 240.0
+    rng = np.random.default_rng(15)
+    # positions: an affine field has no kinks, so differences across cell
+    # faces stay exact; points keep 3 cells from the walls
+    A = rng.standard_normal((3, 3)) * 2.0
+    b = rng.standard_normal(3) * 0.5
+    lin = _linear_field(n3, dx, A, b)
+    pos = (3 * dx + rng.random((4000, 3)) * (1 - 6 * dx)).astype(np.float32)
+    xb = rng.standard_normal(pos.shape).astype(np.float32)
+    pbar, _ = K.advect_rk3_vjp(n3, dx, lin, pos, dt, xb)
+
+    def lpos(p):
+        return float(np.dot(K.advect_rk3(n3, dx, lin, p, dt).astype(np.float64).ravel(), xb.astype(np.float64).ravel()))
+
+    eps = 1e-3
+    worst = 0.0
+    for t in range(64):
+        q, e = int(rng.integers(len(pos))), int(rng.integers(3))
+        a, c = pos.copy(), pos.copy()
+        a[q, e] += eps
+        c[q, e] -= eps
+        h = float(a[q, e]) - float(c[q, e])
+        worst = max(worst, abs((lpos(a) - lpos(c)) / h - float(pbar[q, e])) / float(np.abs(pbar).max()))
+    say(f"advect d/dpositions: max |fd - adjoint| / max |adjoint| = {worst:.3g}")
+    assert worst <= 1e-3
+    # grids: a random field, 64 random faces.  A face moves the new positions
+    # by about dt * eps * (trilinear weight); the step and eps are sized so
+    # that this stays far above the float32 spacing of the positions (6e-8)
+    dtg, eps = 1.0 / 24.0, 1e-2
+    uvw = [(rng.standard_normal(s) * 1.5).astype(np.float32) for s in _shapes(n3)]
+    pos = _points(n, 4000, 16)
+    _, grads = K.advect_rk3_vjp(n3, dx, uvw, pos, dtg, xb)
+
+    def lgrid(fields):
+        return float(np.dot(K.advect_rk3(n3, dx, fields, pos, dtg).astype(np.float64).ravel(),
+                            xb.astype(np.float64).ravel()))
+
+    scale = max(float(np.abs(gg).max()) for gg in grads)
+    worst = 0.0
+    shapes = _shapes(n3)
+    for t in range(64):
+        d = int(rng.integers(3))
+        idx = tuple(int(rng.integers(s)) for s in shapes[d])
+        plus = [x.copy() for x in uvw]
+        minus = [x.copy() for x in uvw]
+        plus[d][idx] += eps
+        minus[d][idx] -= eps
+        h = float(plus[d][idx]) - float(minus[d][idx])
+        worst = max(worst, abs((lgrid(plus) - lgrid(minus)) / h - float(grads[d][idx])) / scale)
+    say(f"advect d/dgrid: max |fd - adjoint| / max |adjoint| = {worst:.3g}")
+    assert worst <= 1e-3
+
+
+def test_advect_autograd():
+    torch = pytest.importorskip("torch")
+    from ffengine import autograd as AG
+    n = 16
+    n3, dx, dt = (n, n, n), 1.0 / n, 1.0 / 240.0
+    rng = np.random.default_rng(17)
+    fields = [torch.tensor((rng.standard_normal(s) * 1.5).astype(np.float32), requires_grad=True)
+              for s in _shapes(n3)]
+    pos0 = torch.tensor(_points(n, 3000, 18), requires_grad=True)
+    p1 = AG.AdvectRK3.apply(pos0, *fields, dx, dt)
+    p2 = AG.AdvectRK3.apply(p1, *fields, dx, dt)
+    wb = torch.from_numpy(rng.standard_normal(tuple(p2.shape)).astype(np.float32))
+    (p2 * wb).sum().backward()
+    say("advect autograd: two steps against the chained entry points")
+    uvw = [f.detach().numpy() for f in fields]
+    pb2, g2 = K.advect_rk3_vjp(n3, dx, uvw, p1.detach().numpy(), dt, wb.numpy())
+    pb1, g1 = K.advect_rk3_vjp(n3, dx, uvw, pos0.detach().numpy(), dt, pb2)
+    assert np.array_equal(pos0.grad.numpy(), pb1)
+    for f, a, c in zip(fields, g1, g2):
+        assert np.allclose(f.grad.numpy(), a + c, rtol=0, atol=1e-6 * max(np.abs(a + c).max(), 1e-30))
