@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "rollout.h"
 
 namespace {
 const int SUCCESS = 1, FAIL = 0;
@@ -382,6 +383,27 @@ void FFAMD_extrapolate(int wi, int wj, int wk, float *field, const unsigned char
     });
 }
 
+void FFAMD_extrapolate_vjp(int wi, int wj, int wk, float *bar, const unsigned char *valid, int layers, int *err) {
+    guarded(err, [&] {
+        if (wi < 3 || wj < 3 || wk < 3 || layers < 0 || layers > 250)
+            throw std::domain_error("Error: face array dims must be >= 3 and layers in [0, 250].\n");
+        Scratch sc;
+        size_t n = (size_t)wi * wj * wk;
+        DBuf<float> a(n), dummy(n);
+        DBuf<uint8_t> vb(n), st(n);
+        up(a.p, bar, n, sc.s);
+        up(vb.p, valid, n, sc.s);
+        // the status bytes depend on the flags only: run the forward's layers
+        // on a scratch field to get them
+        ff::hipCheck(hipMemsetAsync(dummy, 0, sizeof(float) * n, sc.s), "memset");
+        ff::extrapolateInit(wi, wj, wk, vb, st, sc.s);
+        for (int l = 0; l < layers; l++) ff::extrapolateLayer(wi, wj, wk, dummy, st, l, sc.s);
+        ff::extrapolateAdjoint(wi, wj, wk, a, st, layers, sc.s);
+        down(bar, a.p, n, sc.s);
+        sync(sc.s);
+    });
+}
+
 void FFAMD_solid_box_phi(int ni, int nj, int nk, double dx, float *phi, int *err) {
     guarded(err, [&] {
         checkGrid(ni, nj, nk, dx);
@@ -494,24 +516,28 @@ int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u,
     });
 }
 
-void FFAMD_p2g_vjp(int ni, int nj, int nk, double dx, int np, const float *pos, double radius, const float *gu,
-                   const float *gv, const float *gw, float *vbar, int *err) {
+void FFAMD_p2g_vjp(int ni, int nj, int nk, double dx, int np, const float *pos, const float *vel, double radius,
+                   const float *gu, const float *gv, const float *gw, float *vbar, float *xbar, int *err) {
     guarded(err, [&] {
         checkGrid(ni, nj, nk, dx);
         if (np < 0) throw std::domain_error("negative particle count");
         ff::Grid g{ni, nj, nk, dx};
         Scratch sc;
         size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
-        // the forward's weight sums (velocities do not enter them)
-        SortedParticles sp(g, np, pos, nullptr, sc.s);
+        // the forward: face values and weight sums
+        if (xbar && !vel) throw std::domain_error("the position cotangent needs the particle velocities");
+        SortedParticles sp(g, np, pos, vel, sc.s);
         DBuf<float> fu(nu), fv(nv), fw(nw), wsu(nu), wsv(nv), wsw(nw), gu_(nu), gv_(nv), gw_(nw);
         DBuf<uint8_t> bu(nu), bv(nv), bw(nw);
         ff::p2g(g, sp.cs, radius, fu, fv, fw, bu, bv, bw, sc.s, wsu, wsv, wsw);
         up(gu_.p, gu, nu, sc.s); up(gv_.p, gv, nv, sc.s); up(gw_.p, gw, nw, sc.s);
-        DBuf<float> vx(np), vy(np), vz(np);
-        ff::p2gAdjoint(g, np, *sp.f[0], *sp.f[1], *sp.f[2], radius, gu_, gv_, gw_, wsu, wsv, wsw, vx, vy, vz, sc.s);
+        DBuf<float> vx(np), vy(np), vz(np), xb(3 * (size_t)np);
+        ff::p2gAdjoint(g, np, *sp.f[0], *sp.f[1], *sp.f[2], vel ? (const float *)*sp.f[3] : nullptr, *sp.f[4],
+                       *sp.f[5], radius, gu_, gv_, gw_, wsu, wsv, wsw, fu, fv, fw, vx, vy, vz, xbar ? xb.p : nullptr,
+                       sc.s);
         std::vector<float> hx(np), hy(np), hz(np);
         down(hx.data(), vx.p, np, sc.s); down(hy.data(), vy.p, np, sc.s); down(hz.data(), vz.p, np, sc.s);
+        if (xbar) down(xbar, xb.p, 3 * (size_t)np, sc.s);
         sync(sc.s);
         for (int q = 0; q < np; q++) {
             vbar[3 * (size_t)q] = hx[q];
@@ -745,6 +771,96 @@ int FFAMD_pressure_system(int ni, int nj, int nk, double dx, double dt, const fl
         down(invdiag, S.invdiag, m, sc.s);
         sync(sc.s);
         return S.m;
+    });
+}
+
+void FFAMD_sample_velocity_position_vjp(int ni, int nj, int nk, double dx, const float *u, const float *v,
+                                        const float *w, int np, const float *pos, const float *vbar, float *xbar,
+                                        int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (np < 0) throw std::domain_error("negative particle count");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> du(nu), dv(nv), dw(nw), vb(3 * (size_t)np), xb(3 * (size_t)np), pts(9 * (size_t)np),
+            cot(9 * (size_t)np), zu(nu), zv(nv), zw(nw);
+        up(du.p, u, nu, sc.s); up(dv.p, v, nv, sc.s); up(dw.p, w, nw, sc.s);
+        HostSoA p(pos, np, sc.s);
+        up(vb.p, vbar, 3 * (size_t)np, sc.s);
+        ff::hipCheck(hipMemsetAsync(xb, 0, sizeof(float) * 3 * np, sc.s), "memset");
+        // the G2P blend's adjoint with PIC share 1, a zero saved field, no
+        // motion (dt = 0) and a zero position cotangent leaves exactly
+        // grad V(pos)^T vbar
+        ff::hipCheck(hipMemsetAsync(zu, 0, sizeof(float) * nu, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(zv, 0, sizeof(float) * nv, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(zw, 0, sizeof(float) * nw, sc.s), "memset");
+        DBuf<float> vdummy(3 * (size_t)np), sdummy(3 * (size_t)np), out(3 * (size_t)np);
+        ff::BlendAdj B;
+        B.sav = ff::MacView{zu, zv, zw};
+        B.r1 = 1.0f;
+        B.r2 = 0.0f;
+        B.vbarOut = vb;
+        B.vbar = vdummy;
+        B.cotSav = sdummy;
+        ff::advectRK3Adjoint(g, np, p.x, p.y, p.z, ff::MacView{du, dv, dw}, 0.0, xb, out, pts, cot, sc.s, &B);
+        down(xbar, out.p, 3 * (size_t)np, sc.s);
+        sync(sc.s);
+    });
+}
+
+FFAMD_Rollout *FFAMD_rollout_new(int ni, int nj, int nk, double dx, int np, const float *pos, const float *solidNodes,
+                                 double dt, int substeps, int checkpointEvery, double picflip, double gx, double gy,
+                                 double gz, double cfl, double tol, int maxIter, int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        ff::RolloutSettings st;
+        st.dt = dt;
+        st.substeps = substeps;
+        st.checkpointEvery = checkpointEvery;
+        st.picflip = picflip;
+        st.gx = gx;
+        st.gy = gy;
+        st.gz = gz;
+        st.cfl = cfl;
+        st.tol = tol;
+        st.maxIter = maxIter;
+        return reinterpret_cast<FFAMD_Rollout *>(new ff::Rollout(ff::Grid{ni, nj, nk, dx}, np, pos, solidNodes, st));
+    });
+}
+
+void FFAMD_rollout_destroy(FFAMD_Rollout *r) {
+    try {
+        delete reinterpret_cast<ff::Rollout *>(r);
+    } catch (...) {
+    }
+}
+
+namespace {
+ff::Rollout *R(FFAMD_Rollout *r) {
+    if (!r) throw std::runtime_error("null rollout handle");
+    return reinterpret_cast<ff::Rollout *>(r);
+}
+}  // namespace
+
+void FFAMD_rollout_forward(FFAMD_Rollout *r, const float *u0, const float *v0, const float *w0, int freeze,
+                           float *density, int *err) {
+    guarded(err, [&] { R(r)->forward(u0, v0, w0, density, freeze != 0); });
+}
+
+void FFAMD_rollout_backward(FFAMD_Rollout *r, const float *densityBar, float *gu, float *gv, float *gw, int *err) {
+    guarded(err, [&] { R(r)->backward(densityBar, gu, gv, gw); });
+}
+
+void FFAMD_rollout_positions(FFAMD_Rollout *r, float *pos, int *err) {
+    guarded(err, [&] { R(r)->positions(pos); });
+}
+
+void FFAMD_rollout_stats(FFAMD_Rollout *r, long long *collided, long long *fwd, long long *bwd, int *err) {
+    guarded(err, [&] {
+        *collided = R(r)->collided();
+        *fwd = R(r)->forwardIterations();
+        *bwd = R(r)->backwardIterations();
     });
 }
 

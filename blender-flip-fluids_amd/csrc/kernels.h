@@ -74,21 +74,42 @@ void p2g(Grid g, const CellSort &cs, double radius, float *u, float *v, float *w
 // max over particles of dot(v, v) (float) as double
 void maxSpeedSq(const Particles &p, double *partial, double *out, hipStream_t s);
 
-// G2P blend + RK3 + collision; writes new positions in place.
+// G2P blend + RK3 + collision; writes new positions in place.  moved
+// (optional, P bytes) flags the particles whose position the collision
+// response changed.
 struct MacView {
     const float *u, *v, *w;
 };
 void g2pAdvect(Grid g, Particles p, MacView cur, MacView saved, const float *solidNodes,
                const uint8_t *nearSolid, int ci, int cj, int ck, float picRatio, float flipRatio,
-               double dt, double cfl, hipStream_t s);
+               double dt, double cfl, hipStream_t s, uint8_t *moved = nullptr);
+// v = trilinear sample of the MAC field at the positions (SoA)
+void sampleParticles(Grid g, int np, const float *px, const float *py, const float *pz, MacView M, float *vx,
+                     float *vy, float *vz, hipStream_t s);
+// particle density on the cells: the transfer kernel's weights summed at the
+// cell centres; and its adjoint in the positions (xbar: P x 3)
+void densityGrid(Grid g, const CellSort &cs, double radius, float *rho, hipStream_t s);
+void densityAdjoint(Grid g, int np, const float *px, const float *py, const float *pz, double radius,
+                    const float *rhobar, float *xbar, hipStream_t s);
 // RK3 advection through the MAC field without collision (positions out), and
 // its adjoint: pbar (P x 3) for xbar (P x 3), plus the 3P sample points
 // (pts: their x, then y, then z coordinates) with their velocity cotangents
 // (cot: 3P x 3) for the sampling adjoint
 void advectRK3(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur, double dt, float *ox,
                float *oy, float *oz, hipStream_t s);
+// blend: the G2P blend's terms (vbarOut, P x 3: the new velocities'
+// cotangent; vbar, P x 3: the old velocities'; cotSav, P x 3: the saved
+// field's samples at the positions); with it the adjoint is that of
+// g2pAdvect with the collision response taken as the identity
+struct BlendAdj {
+    MacView sav{nullptr, nullptr, nullptr};
+    float r1 = 0.0f, r2 = 0.0f;
+    const float *vbarOut = nullptr;
+    float *vbar = nullptr, *cotSav = nullptr;
+};
 void advectRK3Adjoint(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur, double dt,
-                      const float *xbar, float *pbar, float *pts, float *cot, hipStream_t s);
+                      const float *xbar, float *pbar, float *pts, float *cot, hipStream_t s,
+                      const BlendAdj *blend = nullptr);
 // speed statistics for the extreme-velocity rule: counts[nbins] and max speed
 void speedHistogram(const Particles &p, double stepv, int nbins, int *counts, double *partial,
                     double *maxSpeed, hipStream_t s);
@@ -105,6 +126,9 @@ void compactParticles(Particles &src, Particles &dst, const int *keep, int *rank
 // ---- grid stencils (kernels_grid.hip) ----
 void extrapolateInit(int wi, int wj, int wk, const uint8_t *valid, uint8_t *stat, hipStream_t s);
 void extrapolateLayer(int wi, int wj, int wk, float *field, uint8_t *stat, int layer, hipStream_t s);
+// adjoint of the layered extrapolation (for the status the forward's
+// `layers` layers left), in place on the face cotangent bar
+void extrapolateAdjoint(int wi, int wj, int wk, float *bar, const uint8_t *stat, int layers, hipStream_t s);
 void addToFaces(float *a, size_t n, float inc, hipStream_t s);
 void constrainFaces(float *a, const float *w, size_t n, hipStream_t s);
 void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,
@@ -228,11 +252,14 @@ void projectionAdjointCells(Grid g, double dt, const float *phi, const float *wu
 void projectionAdjointFaces(Grid g, double dt, const float *phi, const float *wu, const float *wv, const float *ww,
                             const float *gu, const float *gv, const float *gw, const double *lambda, float *ou,
                             float *ov, float *ow, hipStream_t s);
-// adjoint of P2G for fixed positions: vbar (per particle, SoA) from ubar on
-// the faces and the forward's per-face weight sums
-void p2gAdjoint(Grid g, int np, const float *px, const float *py, const float *pz, double radius, const float *gu,
-                const float *gv, const float *gw, const float *wsu, const float *wsv, const float *wsw, float *vbx,
-                float *vby, float *vbz, hipStream_t s);
+// adjoint of P2G: vbar (per particle, SoA) from ubar on the faces and the
+// forward's per-face weight sums; with the velocities vx/vy/vz and the
+// forward's face values fu/fv/fw also xbar (P x 3), the positions' cotangent
+// (vx and xbar may be null)
+void p2gAdjoint(Grid g, int np, const float *px, const float *py, const float *pz, const float *vx, const float *vy,
+                const float *vz, double radius, const float *gu, const float *gv, const float *gw, const float *wsu,
+                const float *wsv, const float *wsw, const float *fu, const float *fv, const float *fw, float *vbx,
+                float *vby, float *vbz, float *xbar, hipStream_t s);
 // adjoint of trilinear sampling of component dir: ubar on that face grid from
 // vbar (P x 3, interleaved).  key/slot/perm hold P ints, counts/offsets
 // sampleBuckets(g, dir) + 1 ints, scanScratch scanScratchInts of that.

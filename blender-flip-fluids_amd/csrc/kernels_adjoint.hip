@@ -153,15 +153,21 @@ void projectionAdjointFaces(Grid g, double dt, const float *phi, const float *wu
 // ---- transfer adjoints ----
 namespace {
 
-// P2G adjoint: vbar[p] for direction dir
-__global__ void p2gAdjKernel(Grid g, int np, const float *px, const float *py, const float *pz, double radius,
-                             const float *gu, const float *gv, const float *gw, const float *wsu, const float *wsv,
-                             const float *wsw, float *vbx, float *vby, float *vbz) {
+// P2G adjoint of particle q: vbar (the velocity cotangent) and, when vel is
+// given, xbar (the position cotangent): with u_f = S_f / W_f,
+// du_f/dx_p = (dw_pf/dx_p) (v_p - u_f) / W_f  (S_f alone when W_f <= 1e-6),
+// dw/dx_e = w'(d2) * (-2) (gpos_e - p'_e) in the face's frame.  The kernel
+// weight and its slope vanish at d2 = r^2, so the derivative is continuous
+// as particles enter or leave a face's support.
+__global__ void p2gAdjKernel(Grid g, int np, const float *px, const float *py, const float *pz, const float *vx,
+                             const float *vy, const float *vz, double radius, const float *gu, const float *gv,
+                             const float *gw, const float *wsu, const float *wsv, const float *wsw, const float *fu,
+                             const float *fv, const float *fw, float *vbx, float *vby, float *vbz, float *xbar) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     const float x = px[q], y = py[q], z = pz[q];
     const I3 c = posToIndex(V3{x, y, z}, g.dx);
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0}, xb[3] = {0.0, 0.0, 0.0};
     if (inRange(c.i, c.j, c.k, g.ni, g.nj, g.nk)) {
         const P2GWeight W = p2gWeight(radius);
         const float eps = 1e-6f;
@@ -170,6 +176,8 @@ __global__ void p2gAdjKernel(Grid g, int np, const float *px, const float *py, c
             const int wi = g.ni + (dir == 0), wj = g.nj + (dir == 1), wk = g.nk + (dir == 2);
             const float *gb = dir == 0 ? gu : (dir == 1 ? gv : gw);
             const float *ws = dir == 0 ? wsu : (dir == 1 ? wsv : wsw);
+            const float *fo = dir == 0 ? fu : (dir == 1 ? fv : fw);
+            const double vd = vx ? (double)(dir == 0 ? vx[q] : (dir == 1 ? vy[q] : vz[q])) : 0.0;
             // faces whose candidate cells (2 along the normal, 3 across) hold cell c
             const int i0 = dir == 0 ? c.i : c.i - 1, i1 = c.i + 1;
             const int j0 = dir == 1 ? c.j : c.j - 1, j1 = c.j + 1;
@@ -184,13 +192,26 @@ __global__ void p2gAdjKernel(Grid g, int np, const float *px, const float *py, c
                         const size_t f = flat(fi, fj, fk, wi, wj);
                         const double w = (double)p2gWeightAt(W, d2);
                         const double wsum = (double)ws[f];
-                        acc[dir] += (ws[f] > eps ? w / wsum : w) * (double)gb[f];
+                        const bool norm = ws[f] > eps;
+                        acc[dir] += (norm ? w / wsum : w) * (double)gb[f];
+                        if (xbar) {
+                            const double dd = (double)d2;
+                            const double dw = -3.0 * (double)W.coef1 * dd * dd + 2.0 * (double)W.coef2 * dd -
+                                              (double)W.coef3;
+                            const double s = (norm ? (vd - (double)fo[f]) / wsum : vd) * (double)gb[f] * dw * -2.0;
+                            const V3 pl = sub(sub(V3{x, y, z}, F.offd), F.offb);
+                            xb[0] += s * ((double)F.gpos.x - (double)pl.x);
+                            xb[1] += s * ((double)F.gpos.y - (double)pl.y);
+                            xb[2] += s * ((double)F.gpos.z - (double)pl.z);
+                        }
                     }
         }
     }
     vbx[q] = (float)acc[0];
     vby[q] = (float)acc[1];
     vbz[q] = (float)acc[2];
+    if (xbar)
+        for (int e = 0; e < 3; e++) xbar[3 * (size_t)q + e] = (float)xb[e];
 }
 
 // stencil of point q for component dir of the MAC field (interpFace after the
@@ -281,12 +302,13 @@ __global__ void sampleAdjFaceKernel(Grid g, int dir, const float *px, const floa
 
 }  // namespace
 
-void p2gAdjoint(Grid g, int np, const float *px, const float *py, const float *pz, double radius, const float *gu,
-                const float *gv, const float *gw, const float *wsu, const float *wsv, const float *wsw, float *vbx,
-                float *vby, float *vbz, hipStream_t s) {
+void p2gAdjoint(Grid g, int np, const float *px, const float *py, const float *pz, const float *vx, const float *vy,
+                const float *vz, double radius, const float *gu, const float *gv, const float *gw, const float *wsu,
+                const float *wsv, const float *wsw, const float *fu, const float *fv, const float *fw, float *vbx,
+                float *vby, float *vbz, float *xbar, hipStream_t s) {
     if (np == 0) return;
-    hipLaunchKernelGGL(p2gAdjKernel, dim3(blocksFor(np, TPB)), dim3(TPB), 0, s, g, np, px, py, pz, radius, gu, gv,
-                       gw, wsu, wsv, wsw, vbx, vby, vbz);
+    hipLaunchKernelGGL(p2gAdjKernel, dim3(blocksFor(np, TPB)), dim3(TPB), 0, s, g, np, px, py, pz, vx, vy, vz, radius,
+                       gu, gv, gw, wsu, wsv, wsw, fu, fv, fw, vbx, vby, vbz, xbar);
 }
 
 size_t sampleBuckets(Grid g, int dir) {

@@ -58,6 +58,45 @@ __global__ void extrapLayerKernel(int wi, int wj, int wk, float *a, uint8_t *st,
     st[f] = (uint8_t)(L + 1);
 }
 
+// Adjoint of layer L of extrapLayerKernel, given the status bytes the
+// forward left.  Face f filled in layer L (status L+1) holds the mean of
+// its neighbours that were DONE then (status <= L or the border), so each
+// such neighbour g gathers bar[f] / count_f from its filled neighbours.
+// Writes go to faces with status <= L or the border and reads to faces with
+// status L+1, so one pass per layer is race free.
+__global__ void extrapAdjLayerKernel(int wi, int wj, int wk, float *bar, const uint8_t *st, int L) {
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)wi * wj * wk;
+    if (g >= n) return;
+    const uint8_t sg = st[g];
+    if (!(sg == ST_BORDER || sg <= (uint8_t)L)) return;
+    const int i = (int)(g % wi), j = (int)((g / wi) % wj), k = (int)(g / ((size_t)wi * wj));
+    const long long nb[6] = {1, -1, wi, -(long long)wi, (long long)wi * wj, -(long long)wi * wj};
+    const bool in[6] = {i + 1 < wi, i > 0, j + 1 < wj, j > 0, k + 1 < wk, k > 0};
+    double acc = 0.0;
+    for (int q = 0; q < 6; q++) {
+        if (!in[q]) continue;
+        const long long f = (long long)g + nb[q];
+        if (st[f] != (uint8_t)(L + 1)) continue;
+        // a filled face is never on the border, so its six neighbours exist
+        int count = 0;
+        for (int r = 0; r < 6; r++) {
+            const uint8_t s = st[f + nb[r]];
+            count += (s == ST_BORDER || s <= (uint8_t)L) ? 1 : 0;
+        }
+        acc += (double)(bar[f] / (float)count);
+    }
+    if (acc != 0.0) bar[g] = (float)((double)bar[g] + acc);
+}
+
+// after the layers: the filled faces' own input values were overwritten
+__global__ void extrapAdjClearKernel(size_t n, float *bar, const uint8_t *st) {
+    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const uint8_t s = st[f];
+    if (s != 0 && s != ST_BORDER && s != ST_UNKNOWN) bar[f] = 0.0f;
+}
+
 // _applyConstantBodyForces (fluidsimulation.cpp:5837-5876): no inflow
 // constraints on this path, so every face gets the increment
 __global__ void addKernel(float *a, size_t n, float inc) {
@@ -142,6 +181,13 @@ void extrapolateInit(int wi, int wj, int wk, const uint8_t *valid, uint8_t *st, 
 void extrapolateLayer(int wi, int wj, int wk, float *a, uint8_t *st, int layer, hipStream_t s) {
     size_t n = (size_t)wi * wj * wk;
     hipLaunchKernelGGL(extrapLayerKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, wi, wj, wk, a, st, layer);
+}
+
+void extrapolateAdjoint(int wi, int wj, int wk, float *bar, const uint8_t *st, int layers, hipStream_t s) {
+    size_t n = (size_t)wi * wj * wk;
+    for (int L = layers - 1; L >= 0; L--)
+        hipLaunchKernelGGL(extrapAdjLayerKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, wi, wj, wk, bar, st, L);
+    hipLaunchKernelGGL(extrapAdjClearKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, n, bar, st);
 }
 
 void addToFaces(float *a, size_t n, float inc, hipStream_t s) {

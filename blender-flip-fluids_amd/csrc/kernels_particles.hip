@@ -479,9 +479,14 @@ __global__ void advectRK3Kernel(Grid g, int np, const float *px, const float *py
 
 // Adjoint of advectRK3Kernel: pbar (P x 3) for xbar (P x 3), and the three
 // sample points and their velocity cotangents (k1 at p0, k2 at y2, k3 at y3)
-// whose sampling adjoints give the face-grid cotangent.
+// whose sampling adjoints give the face-grid cotangent.  With B.vbarOut set
+// it is the adjoint of g2pAdvectKernel with the collision response taken as
+// the identity: the new velocity vn = r1 k1 + r2 ((v0 + k1) - V_sav(p0))
+// adds (r1 + r2) vbarOut to k1's cotangent, r2 vbarOut to v0's (B.vbar) and
+// -r2 vbarOut to the saved field's sample at p0 (B.cotSav), whose position
+// gradient enters pbar.
 __global__ void advectRK3AdjKernel(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur,
-                                   double dt, const float *xbar, float *pbar, float *pts, float *cot) {
+                                   double dt, const float *xbar, float *pbar, float *pts, float *cot, BlendAdj B) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     const V3 p0{px[q], py[q], pz[q]};
@@ -497,6 +502,18 @@ __global__ void advectRK3AdjKernel(Grid g, int np, const float *px, const float 
         kb2[e] = 3.0 * (double)a * xb[e];
         kb3[e] = 4.0 * (double)a * xb[e];
         pb[e] = xb[e];
+    }
+    if (B.vbarOut) {
+        double sb[3];
+        for (int e = 0; e < 3; e++) {
+            const double vb = (double)B.vbarOut[3 * (size_t)q + e];
+            kb1[e] += ((double)B.r1 + (double)B.r2) * vb;
+            sb[e] = -(double)B.r2 * vb;
+            B.vbar[3 * (size_t)q + e] = (float)((double)B.r2 * vb);
+            B.cotSav[3 * (size_t)q + e] = (float)sb[e];
+        }
+        sampleMacGrad(g, B.sav, p0, G);
+        for (int e = 0; e < 3; e++) pb[e] += G[0][e] * sb[0] + G[1][e] * sb[1] + G[2][e] * sb[2];
     }
     // k3 = V(y3), y3 = p0 + c3 k2
     sampleMacGrad(g, cur, y3, G);
@@ -642,7 +659,7 @@ __device__ V3 resolveCollision(Grid g, const float *sp, NearGrid N, double cfl, 
 // current field and the collision of the old/new pair (7626-7644).
 __global__ __launch_bounds__(256) void g2pAdvectKernel(Grid g, Particles p, MacView cur, MacView sav,
                                                        const float *sp, NearGrid N, float r1,
-                                                       float r2, double dt, double cfl) {
+                                                       float r2, double dt, double cfl, uint8_t *moved) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= p.n) return;
     V3 p0{p.px[q], p.py[q], p.pz[q]};
@@ -660,6 +677,7 @@ __global__ __launch_bounds__(256) void g2pAdvectKernel(Grid g, Particles p, MacV
     Box boundary = boundaryBox(g.ni, g.nj, g.nk, g.dx);
     boundary.expand((double)(-0.2f) * g.dx);
     V3 pn = resolveCollision(g, sp, N, cfl, p0, p1, boundary);
+    if (moved) moved[q] = pn.x != p1.x || pn.y != p1.y || pn.z != p1.z;
     p.px[q] = pn.x;
     p.py[q] = pn.y;
     p.pz[q] = pn.z;
@@ -818,11 +836,11 @@ void maxSpeedSq(const Particles &p, double *partial, double *out, hipStream_t s)
 
 void g2pAdvect(Grid g, Particles p, MacView cur, MacView saved, const float *solidNodes,
                const uint8_t *nearSolid, int ci, int cj, int ck, float r1, float r2, double dt,
-               double cfl, hipStream_t s) {
+               double cfl, hipStream_t s, uint8_t *moved) {
     if (p.n == 0) return;
     NearGrid N{nearSolid, ci, cj, ck, 3 * g.dx};
     hipLaunchKernelGGL(g2pAdvectKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, g, p, cur, saved,
-                       solidNodes, N, r1, r2, dt, cfl);
+                       solidNodes, N, r1, r2, dt, cfl, moved);
 }
 
 void speedHistogram(const Particles &p, double stepv, int nbins, int *counts, double *partial,
@@ -891,10 +909,98 @@ void advectRK3(Grid g, int np, const float *px, const float *py, const float *pz
 }
 
 void advectRK3Adjoint(Grid g, int np, const float *px, const float *py, const float *pz, MacView cur, double dt,
-                      const float *xbar, float *pbar, float *pts, float *cot, hipStream_t s) {
+                      const float *xbar, float *pbar, float *pts, float *cot, hipStream_t s, const BlendAdj *blend) {
     if (np == 0) return;
+    const BlendAdj B = blend ? *blend : BlendAdj{};
     hipLaunchKernelGGL(advectRK3AdjKernel, dim3(blocksFor(np, TPB)), dim3(TPB), 0, s, g, np, px, py, pz, cur, dt, xbar,
-                       pbar, pts, cot);
+                       pbar, pts, cot, B);
+}
+
+// G2P of the MAC field at the particle positions (the sample the G2P blend
+// calls vPIC): v = V(p)
+__global__ void sampleParticlesKernel(Grid g, int np, const float *px, const float *py, const float *pz, MacView M,
+                                      float *vx, float *vy, float *vz) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const V3 v = sampleMac(g, M, V3{px[q], py[q], pz[q]});
+    vx[q] = v.x;
+    vy[q] = v.y;
+    vz[q] = v.z;
+}
+
+void sampleParticles(Grid g, int np, const float *px, const float *py, const float *pz, MacView M, float *vx,
+                     float *vy, float *vz, hipStream_t s) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(sampleParticlesKernel, dim3(blocksFor(np, TPB)), dim3(TPB), 0, s, g, np, px, py, pz, M, vx, vy,
+                       vz);
+}
+
+// Particle density on the cells: rho_c = sum over particles within r of the
+// cell centre of the transfer kernel's weight w(|x_p - c|^2), gathered from
+// the 27 cells around c in cell order, then the cells' sorted order.
+__global__ void densityKernel(Grid g, const int *offsets, const float *spx, const float *spy, const float *spz,
+                              double radius, float *rho) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N = (size_t)g.ni * g.nj * g.nk;
+    if (c >= N) return;
+    const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / ((size_t)g.ni * g.nj));
+    const P2GWeight W = p2gWeight(radius);
+    const double h = 0.5 * g.dx;
+    const double cx = i * g.dx + h, cy = j * g.dx + h, cz = k * g.dx + h;
+    double acc = 0.0;
+    for (int dk = -1; dk <= 1; dk++)
+        for (int dj = -1; dj <= 1; dj++)
+            for (int di = -1; di <= 1; di++) {
+                const int a = i + di, b = j + dj, e = k + dk;
+                if (!inRange(a, b, e, g.ni, g.nj, g.nk)) continue;
+                const size_t nc = flat(a, b, e, g.ni, g.nj);
+                for (int s = offsets[nc]; s < offsets[nc + 1]; s++) {
+                    const double dx = cx - spx[s], dy = cy - spy[s], dz = cz - spz[s];
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 < (double)W.rsq) acc += 1.0 - W.coef1 * d2 * d2 * d2 + W.coef2 * d2 * d2 - W.coef3 * d2;
+                }
+            }
+    rho[c] = (float)acc;
+}
+
+// xbar_p = sum_c rhobar_c w'(d2) (-2) (c - x_p) over the cells within r
+__global__ void densityAdjKernel(Grid g, int np, const float *px, const float *py, const float *pz, double radius,
+                                 const float *rhobar, float *xbar) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const P2GWeight W = p2gWeight(radius);
+    const double x = px[q], y = py[q], z = pz[q];
+    const I3 c0 = posToIndex(V3{px[q], py[q], pz[q]}, g.dx);
+    const double h = 0.5 * g.dx;
+    double xb[3] = {0.0, 0.0, 0.0};
+    for (int dk = -1; dk <= 1; dk++)
+        for (int dj = -1; dj <= 1; dj++)
+            for (int di = -1; di <= 1; di++) {
+                const int a = c0.i + di, b = c0.j + dj, e = c0.k + dk;
+                if (!inRange(a, b, e, g.ni, g.nj, g.nk)) continue;
+                const double ex = a * g.dx + h - x, ey = b * g.dx + h - y, ez = e * g.dx + h - z;
+                const double d2 = ex * ex + ey * ey + ez * ez;
+                if (!(d2 < (double)W.rsq)) continue;
+                const double dw = -3.0 * W.coef1 * d2 * d2 + 2.0 * W.coef2 * d2 - W.coef3;
+                const double s = (double)rhobar[flat(a, b, e, g.ni, g.nj)] * dw * -2.0;
+                xb[0] += s * ex;
+                xb[1] += s * ey;
+                xb[2] += s * ez;
+            }
+    for (int e = 0; e < 3; e++) xbar[3 * (size_t)q + e] = (float)xb[e];
+}
+
+void densityGrid(Grid g, const CellSort &cs, double radius, float *rho, hipStream_t s) {
+    const size_t N = (size_t)g.ni * g.nj * g.nk;
+    hipLaunchKernelGGL(densityKernel, dim3(blocksFor(N, TPB)), dim3(TPB), 0, s, g, cs.offsets, cs.spx, cs.spy, cs.spz,
+                       radius, rho);
+}
+
+void densityAdjoint(Grid g, int np, const float *px, const float *py, const float *pz, double radius,
+                    const float *rhobar, float *xbar, hipStream_t s) {
+    if (np == 0) return;
+    hipLaunchKernelGGL(densityAdjKernel, dim3(blocksFor(np, TPB)), dim3(TPB), 0, s, g, np, px, py, pz, radius, rhobar,
+                       xbar);
 }
 
 }  // namespace ff

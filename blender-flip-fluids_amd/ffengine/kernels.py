@@ -72,6 +72,15 @@ def extrapolate(field, valid, layers):
     return a
 
 
+def extrapolate_vjp(bar, valid, layers):
+    """Adjoint of extrapolate for the same flags and layer count."""
+    a = np.array(bar, np.float32, copy=True, order="C")
+    vb = np.ascontiguousarray(valid, dtype=np.uint8)
+    wk, wj, wi = a.shape
+    _run("FFAMD_extrapolate_vjp", [c_int] * 3 + [c_void_p, c_void_p, c_int], [wi, wj, wk, _fp(a), _fp(vb), layers])
+    return a
+
+
 def solid_box_phi(n3, dx):
     ni, nj, nk = n3
     phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
@@ -167,15 +176,19 @@ def projection_vjp(n3, dx, dt, weights, liquid_phi, grads, tol=1e-12, max_iter=5
     return (ou, ov, ow), it.value, bool(ok)
 
 
-def p2g_vjp(n3, dx, pos, radius, grads):
-    """Adjoint of p2g in the particle velocities (positions fixed): vbar (P, 3)."""
+def p2g_vjp(n3, dx, pos, radius, grads, vel=None):
+    """Adjoint of p2g: vbar (P, 3), the velocities' cotangent; with vel also
+    xbar (P, 3), the positions' cotangent, returned as (vbar, xbar)."""
     ni, nj, nk = n3
     pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
     gu, gv, gw = (_f32(a) for a in grads)
     vbar = np.zeros_like(pos)
-    _run("FFAMD_p2g_vjp", [c_int] * 3 + [c_double, c_int, c_void_p, c_double] + [c_void_p] * 4,
-         [ni, nj, nk, dx, len(pos), _fp(pos), radius, _fp(gu), _fp(gv), _fp(gw), _fp(vbar)])
-    return vbar
+    xbar = np.zeros_like(pos) if vel is not None else None
+    v = np.ascontiguousarray(vel, np.float32).reshape(-1, 3) if vel is not None else None
+    _run("FFAMD_p2g_vjp", [c_int] * 3 + [c_double, c_int, c_void_p, c_void_p, c_double] + [c_void_p] * 5,
+         [ni, nj, nk, dx, len(pos), _fp(pos), _fp(v) if v is not None else None, radius, _fp(gu), _fp(gv), _fp(gw),
+          _fp(vbar), _fp(xbar) if xbar is not None else None])
+    return vbar if vel is None else (vbar, xbar)
 
 
 def sample_velocity_vjp(n3, dx, pos, vbar):
@@ -215,3 +228,15 @@ def advect_rk3_vjp(n3, dx, uvw, pos, dt, xbar):
          [ni, nj, nk, dx, _fp(u), _fp(v), _fp(w), len(pos), _fp(pos), dt, _fp(xb), _fp(pbar), _fp(gu), _fp(gv),
           _fp(gw)])
     return pbar, (gu, gv, gw)
+
+
+def sample_velocity_position_vjp(n3, dx, uvw, pos, vbar):
+    """Position cotangent of sample_velocity: xbar (P, 3) = grad V(pos)^T vbar."""
+    ni, nj, nk = n3
+    u, v, w = (_f32(a) for a in uvw)
+    pos = _f32(pos).reshape(-1, 3)
+    vb = _f32(vbar).reshape(-1, 3)
+    xbar = np.zeros_like(pos)
+    _run("FFAMD_sample_velocity_position_vjp", [c_int] * 3 + [c_double] + [c_void_p] * 3 + [c_int] + [c_void_p] * 3,
+         [ni, nj, nk, dx, _fp(u), _fp(v), _fp(w), len(pos), _fp(pos), _fp(vb), _fp(xbar)])
+    return xbar

@@ -215,6 +215,11 @@ void FFAMD_p2g(int isize, int jsize, int ksize, double dx, int np, const float *
                unsigned char *vu, unsigned char *vv, unsigned char *vw, int *err);
 void FFAMD_extrapolate(int wi, int wj, int wk, float *field, const unsigned char *valid,
                        int layers, int *err);
+/* Adjoint of FFAMD_extrapolate (no counterpart in the reference): for the
+ * same flags and layer count the extrapolation is linear in the field, and
+ * this replaces bar (wk x wj x wi) by its transpose applied to bar. */
+void FFAMD_extrapolate_vjp(int wi, int wj, int wk, float *bar, const unsigned char *valid,
+                           int layers, int *err);
 void FFAMD_solid_box_phi(int isize, int jsize, int ksize, double dx, float *phi_nodes, int *err);
 void FFAMD_face_weights(int isize, int jsize, int ksize, const float *phi_nodes, float *wu,
                         float *wv, float *ww, int *err);
@@ -254,9 +259,13 @@ int FFAMD_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, cons
  * three face-grid cotangents grad_u/v/w.  FFAMD_sample_velocity_vjp:
  * trilinear sampling (as FFAMD_sample_velocity) is linear in the face grids;
  * returns the face-grid cotangents u, v, w of the per-point cotangents vbar
- * (np x 3).  Both gather in a fixed order (no float atomics). */
-void FFAMD_p2g_vjp(int isize, int jsize, int ksize, double dx, int np, const float *pos, double radius,
-                   const float *grad_u, const float *grad_v, const float *grad_w, float *vbar, int *err);
+ * (np x 3).  Both gather in a fixed order (no float atomics).  With the
+ * particle velocities vel (np x 3), FFAMD_p2g_vjp also returns xbar (np x 3),
+ * the cotangent of the positions through the transfer weights (vel and xbar
+ * may be null). */
+void FFAMD_p2g_vjp(int isize, int jsize, int ksize, double dx, int np, const float *pos,
+                   const float *vel, double radius, const float *grad_u, const float *grad_v,
+                   const float *grad_w, float *vbar, float *xbar, int *err);
 void FFAMD_sample_velocity_vjp(int isize, int jsize, int ksize, double dx, int np, const float *pos,
                                const float *vbar, float *u, float *v, float *w, int *err);
 
@@ -292,6 +301,47 @@ void FFAMD_mic0_band_shape(int fp64, int *levels, int *tile, int *err);
 void FFAMD_sample_velocity(int isize, int jsize, int ksize, double dx, const float *u,
                            const float *v, const float *w, int np, const float *pos, float *out,
                            int *err);
+
+/* Position cotangent of trilinear sampling (no counterpart in the reference):
+ * xbar (np x 3) = sum_d vbar_d grad V_d(pos) for the face grids u/v/w, inside
+ * the cell each sample reads; 0 outside the domain. */
+void FFAMD_sample_velocity_position_vjp(int isize, int jsize, int ksize, double dx, const float *u,
+                                        const float *v, const float *w, int np, const float *pos,
+                                        const float *vbar, float *xbar, int *err);
+
+/* Differentiable run of substeps (no counterpart in the reference: the
+ * reference's grid is always P2G of its particles and it has no adjoint).
+ * A rollout holds np particles at fixed positions pos (np x 3) whose initial
+ * velocities are the trilinear sample of an initial MAC field, and runs
+ * `substeps` substeps of length dt in the engine's order
+ * (fluidsimulation.cpp:10078-10121; no particle removal), then returns the
+ * particle density on the cells: each cell centre sums the transfer kernel's
+ * weights w(|x_p - c|^2) of the particles within its radius.  The backward
+ * gives the gradient of <density_bar, density> with respect to the initial
+ * field, recomputing substeps from particle checkpoints kept every
+ * checkpoint_every substeps.  The liquid SDF and P2G flags are those the
+ * last forward with freeze = 0 recorded, held fixed (freeze = 1 makes a
+ * forward reuse them, for finite differences of the same map), and the
+ * collision response is taken as the identity.  solid_nodes: nodal solid SDF
+ * ((isize+1)(jsize+1)(ksize+1)) or null for the domain walls; picflip: the
+ * PIC share of the G2P blend; tol/max_iter: every pressure solve, forward
+ * and adjoint (FP64 MIC(0)-PCG). */
+typedef struct FFAMD_Rollout FFAMD_Rollout;
+FFAMD_Rollout *FFAMD_rollout_new(int isize, int jsize, int ksize, double dx, int np,
+                                 const float *pos, const float *solid_nodes, double dt,
+                                 int substeps, int checkpoint_every, double picflip, double gx,
+                                 double gy, double gz, double cfl, double tol, int max_iter,
+                                 int *err);
+void FFAMD_rollout_destroy(FFAMD_Rollout *r);
+void FFAMD_rollout_forward(FFAMD_Rollout *r, const float *u0, const float *v0, const float *w0,
+                           int freeze, float *density, int *err);
+void FFAMD_rollout_backward(FFAMD_Rollout *r, const float *density_bar, float *grad_u,
+                            float *grad_v, float *grad_w, int *err);
+void FFAMD_rollout_positions(FFAMD_Rollout *r, float *pos, int *err);
+/* particles moved by the collision response (summed over the last forward's
+ * substeps) and the pressure iterations of the last forward and backward */
+void FFAMD_rollout_stats(FFAMD_Rollout *r, long long *collided, long long *forward_iterations,
+                         long long *backward_iterations, int *err);
 
 #ifdef __cplusplus
 }

@@ -163,6 +163,74 @@ def test_p2g_vjp_is_oracle_jacobian_transposed():
         assert err <= 1e-6
 
 
+def _p2g64(n3, dx, pos, vel, r, sums=None):
+    """The P2G forward in float64 by brute force over (particle, face) pairs:
+    u_f = sum w v / sum w where sum w > 1e-6, else sum w v, with
+    w = 1 - c1 d^6 + c2 d^4 - c3 d^2 for d < r (velocityadvector.cpp:467-536).
+    sums, when a list, receives the weight sums."""
+    ni, nj, nk = n3
+    pos = pos.astype(np.float64)
+    vel = vel.astype(np.float64)
+    c1, c2, c3 = 4.0 / 9.0 / r ** 6, 17.0 / 9.0 / r ** 4, 22.0 / 9.0 / r ** 2
+    out = []
+    for d, shape in enumerate(_shapes(n3)):
+        k, j, i = np.meshgrid(*(np.arange(s) for s in shape), indexing="ij")
+        fpos = np.stack([i, j, k], -1).reshape(-1, 3).astype(np.float64) * dx
+        fpos[:, [e for e in range(3) if e != d]] += 0.5 * dx
+        d2 = ((fpos[:, None, :] - pos[None, :, :]) ** 2).sum(-1)
+        w = np.where(d2 < r * r, 1.0 - c1 * d2 ** 3 + c2 * d2 ** 2 - c3 * d2, 0.0)
+        S, W = w @ vel[:, d], w.sum(1)
+        if sums is not None:
+            sums.append(W.reshape(shape))
+        out.append(np.where(W > 1e-6, S / np.where(W > 1e-6, W, 1.0), S).reshape(shape))
+    return out
+
+
+def test_p2g_position_vjp_central_differences():
+    """The position cotangent of P2G against central differences of a float64
+    forward, itself pinned to the oracle's float forward."""
+    n = 8
+    n3, dx = (n, n, n), 1.0 / n
+    r = 0.5 * dx * np.sqrt(3.0)
+    pos = _points(n, 300, 21, margin=0.0)
+    rng = np.random.default_rng(22)
+    vel = rng.standard_normal(pos.shape).astype(np.float32)
+    sums = []
+    f64 = _p2g64(n3, dx, pos, vel, r, sums)
+    (u, v, w), _ = O.p2g(n3, dx, pos, vel, r)
+    # the float forward's weights lose ~3e-7 absolute to cancellation near the
+    # cutoff, which a small weight sum turns into a larger error of u
+    top = max(float(np.abs(a).max()) for a in f64)
+    pin = max(float(np.abs(a - b).max()) for a, b in zip(f64, (u, v, w))) / top
+    heavy = max(float(np.abs(np.where(W >= 0.05, a - b, 0)).max()) for a, b, W in zip(f64, (u, v, w), sums)) / top
+    say(f"p2g float64 forward against the oracle: max |diff| / max |u| = {pin:.3g} (weight sum >= 0.05: {heavy:.3g})")
+    assert pin <= 1e-4 and heavy <= 1e-5
+    ub = [rng.standard_normal(s).astype(np.float32) for s in _shapes(n3)]
+
+    def loss(p):
+        return sum(float(np.dot(a.ravel(), b.ravel().astype(np.float64))) for a, b in zip(_p2g64(n3, dx, p, vel, r), ub))
+
+    vbar, xbar = K.p2g_vjp(n3, dx, pos, r, ub, vel=vel)
+    assert np.array_equal(vbar, K.p2g_vjp(n3, dx, pos, r, ub))
+    h = 1e-6 * dx
+    base = pos.astype(np.float64)
+    worst = 0.0
+    scale = float(np.abs(xbar).max())
+    for t in range(48):
+        q, e = int(rng.integers(len(pos))), int(rng.integers(3))
+        a, b = base.copy(), base.copy()
+        a[q, e] += h
+        b[q, e] -= h
+        worst = max(worst, abs((loss(a) - loss(b)) / (2 * h) - float(xbar[q, e])) / scale)
+    dirs = rng.standard_normal(pos.shape)
+    fd = (loss(base + h * dirs) - loss(base - h * dirs)) / (2 * h)
+    ad = float(np.dot(xbar.astype(np.float64).ravel(), dirs.ravel()))
+    say(f"p2g position cotangent: max |fd - adjoint| / max |adjoint| = {worst:.3g}; "
+        f"direction {fd:.6g} vs {ad:.6g}")
+    assert worst <= 1e-4
+    assert abs(fd - ad) <= 1e-4 * np.abs(xbar).astype(np.float64).sum()
+
+
 def test_sample_vjp_is_oracle_jacobian_transposed():
     n = 6
     n3, dx = (n, n, n), 1.0 / n
