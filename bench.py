@@ -275,11 +275,130 @@ def splash_config(n, steps, warmup, tol, mode):
     print(json.dumps(line), flush=True)
 
 
+def adjoint_config(n, substeps, checkpoint_every, fd_faces, eps, tol, dt):
+    """The differentiation configuration: a dam break (block [0, 0.375) x
+    [0, 0.625) x [0, 1), 8 particles per cell outside the walls) whose initial
+    particle velocities are G2P of an initial MAC field (zero: the dam at
+    rest), run for `substeps` substeps of dt.  Loss = squared L2 distance of
+    the particle density after the last substep to a target density from
+    the same run started from a seeded perturbation (smoothed noise,
+    0.2 m/s) of the initial field.  Times the forward (with checkpoints every
+    `checkpoint_every` substeps) and the backward (recomputation + adjoint),
+    then checks the gradient by central differences on `fd_faces` seeded
+    faces (forwards with the recorded masks) and along one smooth direction
+    at three step sizes.  The map has jumps the adjoint does not see (the
+    collision response's 0.1 dx path search; a face losing its last particle
+    in P2G), crossed more often the larger the step; below the jumps the
+    float32 state's rounding sets a noise floor."""
+    import numpy as np
+    from ffengine import kernels, scenes
+    from ffengine.rollout import Rollout
+    n3, dx = (n, n, n), 1.0 / n
+    log("adjoint configuration n=%d, %d substeps, checkpoints every %d" % (n, substeps, checkpoint_every))
+    pos = scenes.dam_break_particles(n, seed=0)
+    solid = kernels.solid_box_phi(n3, dx)
+    c = np.floor(pos / dx).astype(int)
+    free = np.ones(len(pos), bool)
+    for o in range(8):
+        a, b, e = o & 1, (o >> 1) & 1, o >> 2
+        free &= solid[c[:, 2] + e, c[:, 1] + b, c[:, 0] + a] > 0
+    pos = np.ascontiguousarray(pos[free])
+    shapes = [(n, n, n + 1), (n, n + 1, n), (n + 1, n, n)]
+    init = [np.zeros(s, np.float32) for s in shapes]
+    target_init = scenes.noise_mac_field(n, seed=1, amplitude=0.2, correlation=8.0)
+    ro = Rollout(n3, dx, pos, dt=dt, substeps=substeps, checkpoint_every=checkpoint_every, tol=tol, max_iter=5000)
+    target = ro.forward(*target_init).astype(np.float64)
+
+    def loss(rho):
+        return float(((rho.astype(np.float64) - target) ** 2).sum())
+
+    t0 = time.perf_counter()
+    rho = ro.forward(*init)
+    t_fwd = time.perf_counter() - t0
+    moved, fwd_iters, _ = ro.stats()
+    L0 = loss(rho)
+    t0 = time.perf_counter()
+    g = ro.backward((2.0 * (rho.astype(np.float64) - target)).astype(np.float32))
+    t_bwd = time.perf_counter() - t0
+    _, _, bwd_iters = ro.stats()
+    log("  %d particles, loss %.6g, forward %.2fs (%d PCG iterations, %d collision moves), backward %.2fs "
+        "(%d adjoint PCG iterations)" % (len(pos), L0, t_fwd, fwd_iters, moved, t_bwd, bwd_iters))
+    scale = max(float(np.abs(a).max()) for a in g)
+    rng = np.random.default_rng(2)
+    cand = [(d, i) for d in range(3) for i in np.flatnonzero(np.abs(g[d]).ravel() > 0.01 * scale)]
+    picks = [cand[int(q)] for q in rng.choice(len(cand), size=min(fd_faces, len(cand)), replace=False)]
+    steps = [eps]
+    pairs = {e: [] for e in steps}
+    t0 = time.perf_counter()
+    for q, (d, i) in enumerate(picks):
+        for e in steps:
+            plus = [a.copy() for a in init]
+            minus = [a.copy() for a in init]
+            plus[d].ravel()[i] += e
+            minus[d].ravel()[i] -= e
+            h = float(plus[d].ravel()[i]) - float(minus[d].ravel()[i])
+            fd = (loss(ro.forward(*plus, freeze=True)) - loss(ro.forward(*minus, freeze=True))) / h
+            pairs[e].append((fd, float(g[d].ravel()[i])))
+        if q % 8 == 7:
+            log("  central differences: %d of %d faces (%.0fs)" % (q + 1, len(picks), time.perf_counter() - t0))
+    t_fd = time.perf_counter() - t0
+
+    def fd_stats(e):
+        v = np.array(pairs[e])
+        err = np.abs(v[:, 0] - v[:, 1]) / scale
+        rel = np.abs(v[:, 0] - v[:, 1]) / np.maximum(np.abs(v[:, 1]), 1e-300)
+        return {"eps": e, "max_err_over_max_grad": float(err.max()), "median_err_over_max_grad": float(np.median(err)),
+                "median_rel_err": float(np.median(rel)), "sign_agreement": int((np.sign(v[:, 0]) == np.sign(v[:, 1])).sum()),
+                "correlation": float(np.corrcoef(v[:, 0], v[:, 1])[0, 1]) if len(v) > 2 else None}
+    direction = scenes.noise_mac_field(n, seed=3, amplitude=1.0, correlation=8.0)
+    dad = sum(float(np.dot(a.astype(np.float64).ravel(), b.astype(np.float64).ravel())) for a, b in zip(g, direction))
+    dirs = []
+    for tdir in (1e-6, 1e-5, 1e-4):
+        lp = loss(ro.forward(*[a + tdir * b for a, b in zip(init, direction)], freeze=True))
+        mp = ro.stats()[0]
+        lm = loss(ro.forward(*[a - tdir * b for a, b in zip(init, direction)], freeze=True))
+        mm = ro.stats()[0]
+        dfd = (lp - lm) / (2 * tdir)
+        dirs.append({"step": tdir, "fd": dfd, "fd_over_adjoint": dfd / dad, "collision_moves": [mp, mm]})
+    stats = [fd_stats(e) for e in steps]
+    for s in stats:
+        log("  central differences on %d faces, eps %g: error max %.3g, median %.3g of max |grad|, median relative "
+            "%.3g; sign agreement %d/%d; correlation %.4f" % (len(picks), s["eps"], s["max_err_over_max_grad"],
+                                                             s["median_err_over_max_grad"], s["median_rel_err"],
+                                                             s["sign_agreement"], len(picks), s["correlation"]))
+    for r in dirs:
+        log("  direction, step %g: fd %.6g adjoint %.6g ratio %.4f (collision moves %s)"
+            % (r["step"], r["fd"], dad, r["fd_over_adjoint"], r["collision_moves"]))
+    line = {"metric": "differentiable rollout %d^3, %d substeps: seconds for the gradient of the density loss "
+                      "wrt the initial MAC velocity" % (n, substeps),
+            "value": round(t_fwd + t_bwd, 3), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+            "data": "synthetic", "dtype": "f32+f64",
+            "config": {"workload": "dam break %d^3 (%d particles outside the walls), %d substeps of %.6g s, "
+                                   "checkpoints every %d, PCG rel res %g (forward and adjoint)"
+                                   % (n, len(pos), substeps, dt, checkpoint_every, tol),
+                       "grid": n, "particles": len(pos)},
+            "forward_s": round(t_fwd, 3), "backward_s": round(t_bwd, 3),
+            "substeps_per_s_forward": round(substeps / t_fwd, 2),
+            "pcg_iterations": {"forward": fwd_iters, "adjoint": bwd_iters}, "collision_moves": moved,
+            "loss": L0, "grad_max_abs": scale,
+            "central_differences": {"faces": len(picks), "face_pool": "faces with |grad| > 1% of max",
+                                    "steps": stats,
+                                    "direction": {"adjoint": dad, "unit_amplitude_noise_field": True, "steps": dirs}}}
+    ro.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="dam-break", choices=["dam-break", "precision", "splash"],
+    ap.add_argument("--config", default="dam-break", choices=["dam-break", "precision", "splash", "adjoint"],
                     help="dam-break: the 256^3 headline benchmark (default); precision: the 384^3 slab "
-                         "pressure-accuracy comparison; splash: the 512^3 splash")
+                         "pressure-accuracy comparison; splash: the 512^3 splash; adjoint: the 128^3 "
+                         "differentiable rollout with its finite-difference check")
+    ap.add_argument("--substeps", type=int, default=32, help="adjoint: substeps of the rollout")
+    ap.add_argument("--checkpoint-every", type=int, default=8, help="adjoint: checkpoint spacing")
+    ap.add_argument("--fd-faces", type=int, default=64, help="adjoint: faces checked by central differences")
+    ap.add_argument("--fd-eps", type=float, default=1e-3, help="adjoint: central-difference step (m/s)")
+    ap.add_argument("--dt", type=float, default=1.0 / 240.0, help="adjoint: substep length (s)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
@@ -306,6 +425,10 @@ def main():
 
     if args.config == "precision":
         precision_config(args.n or 384, args.tol)
+        return
+    if args.config == "adjoint":
+        adjoint_config(args.n or 128, args.substeps, args.checkpoint_every, args.fd_faces, args.fd_eps, args.tol,
+                       args.dt)
         return
     if args.config == "splash":
         modes = [m for m in args.modes.split(",") if m]

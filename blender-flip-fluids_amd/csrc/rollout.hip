@@ -28,6 +28,22 @@ __global__ void combineKernel(int np, const float *pbarNew, const float *xbarP2G
     }
 }
 
+// P2G normalises a face by its weight sum W when W > 1e-6 and keeps the
+// plain sum S otherwise (velocityadvector.cpp:467-536), a jump when W
+// crosses the threshold.  With the recorded flags held fixed, the decision
+// follows the record instead: S/W on recorded faces (0 when no weight is
+// left), S elsewhere.  Where the decisions agree nothing changes.
+__global__ void freezeNormKernel(float *field, const float *wsum, const uint8_t *rec, size_t n) {
+    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const float W = wsum[f];
+    const bool want = rec[f] != 0, got = W > 1e-6f;
+    if (want && !got)
+        field[f] = W > 0.0f ? field[f] / W : 0.0f;
+    else if (!want && got)
+        field[f] = field[f] * W;
+}
+
 __global__ void countKernel(const uint8_t *a, int n, int *count) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     // one integer atomic per flagged particle: a count, summed in any order
@@ -151,9 +167,10 @@ void Rollout::substep(int k, bool freeze, bool record) {
     p2g(g, _cs, _radius, _u[0], _u[1], _u[2], _valid[0], _valid[1], _valid[2], _s, _ws[0], _ws[1], _ws[2]);
     size_t off = (size_t)k * F;
     for (int d = 0; d < 3; d++) {
-        if (freeze)
+        if (freeze) {
+            freezeNormKernel<<<blocksFor(_nf[d], TPB), TPB, 0, _s>>>(_u[d], _ws[d], _validRec.p + off, _nf[d]);
             copyD(_valid[d].p, _validRec.p + off, _nf[d], _s);
-        else if (record)
+        } else if (record)
             copyD(_validRec.p + off, _valid[d].p, _nf[d], _s);
         off += _nf[d];
     }

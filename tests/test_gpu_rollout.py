@@ -185,6 +185,9 @@ def test_rollout_gradient_independent_of_checkpoints():
 def test_rollout_gradient_central_differences():
     ro, init, target = _rollout_case(substeps=3, checkpoint_every=2)
     rho = ro.forward(*init)
+    # holding the recorded masks fixed changes nothing at the recorded point
+    assert np.array_equal(ro.forward(*init, freeze=True), rho)
+    rho = ro.forward(*init)
     g = ro.backward((2.0 * (rho.astype(np.float64) - target)).astype(np.float32))
     moved, fi, bi = ro.stats()
     say(f"rollout: {ro.np} particles, loss {_loss(rho, target):.6g}, {moved} moved by collisions, "
