@@ -504,6 +504,110 @@ void oracle_p2g(int ni, int nj, int nk, double dx, int np, const float *pos, con
     }
 }
 
+/* VelocityAdvector::_advectGrid for one direction with the APIC producer
+ * (velocityadvector.cpp:74-175, 177-251, 296-465, 543-623).  Block membership
+ * is the FLIP path's (the particle radius plus 1e-6 around each block); in
+ * each member block the particle deposits onto the 8 nodes of its cell in
+ * that block's frame that lie inside the block.  Sums run in ascending
+ * particle index. */
+void oracle_p2g_apic(int ni, int nj, int nk, double dx, int np, const float *pos, const float *vel,
+                     const float *aff, double radius, int dir, float *field, unsigned char *valid) {
+    const int bw = 10;
+    int wi = ni + (dir == 0), wj = nj + (dir == 1), wk = nk + (dir == 2);
+    size_t F = (size_t)wi * wj * wk;
+    std::vector<float> sum(F, 0.0f), wsum(F, 0.0f);
+    int bi = (wi + bw - 1) / bw, bj = (wj + bw - 1) / bw, bk = (wk + bw - 1) / bw;
+    double chunkdx = dx * bw;
+    V3 offset = dir == 0 ? mk(0.0, 0.5 * dx, 0.5 * dx)
+                         : (dir == 1 ? mk(0.5 * dx, 0.0, 0.5 * dx) : mk(0.5 * dx, 0.5 * dx, 0.0));
+    std::vector<char> active((size_t)bi * bj * bk, 0);
+    for (int p = 0; p < np; p++) {
+        V3 x = V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]} - offset;
+        GI g = posToIndex(x, chunkdx);
+        if (inRange(g.i, g.j, g.k, bi, bj, bk)) active[flat(g.i, g.j, g.k, bi, bj)] = 1;
+    }
+    feather26(active, bi, bj, bk);
+    float eps = 1e-6;
+    float sr = radius + eps;
+    float blockdx = chunkdx;
+    GI members[27];
+    for (int p = 0; p < np; p++) {
+        V3 x = V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]} - offset;
+        float velocity = vel[3 * p + dir];
+        V3 affine{aff[9 * p + 3 * dir], aff[9 * p + 3 * dir + 1], aff[9 * p + 3 * dir + 2]};
+        int nm = blockMembers(x, sr, blockdx, bi, bj, bk, active, members);
+        for (int m = 0; m < nm; m++) {
+            GI b = members[m];
+            V3 pl = x - indexToPos(b.i, b.j, b.k, bw * dx);
+            GI g = posToIndex(pl, dx);
+            V3 ipos = divide(pl - indexToPos(g.i, g.j, g.k, dx), (float)dx);
+            float wx[2] = {1.0f - ipos.x, ipos.x}, wy[2] = {1.0f - ipos.y, ipos.y}, wz[2] = {1.0f - ipos.z, ipos.z};
+            for (int q = 0; q < 8; q++) {
+                int oi = q & 1, oj = (q >> 1) & 1, ok = q >> 2;
+                int li = g.i + oi, lj = g.j + oj, lk = g.k + ok;
+                if (li < 0 || lj < 0 || lk < 0 || li >= bw || lj >= bw || lk >= bw) continue;
+                float weight = wx[oi] * wy[oj] * wz[ok];
+                float apicTerm = dot(affine, indexToPos(li, lj, lk, dx) - pl);
+                int fi = b.i * bw + li, fj = b.j * bw + lj, fk = b.k * bw + lk;
+                if (!inRange(fi, fj, fk, wi, wj, wk)) continue;
+                size_t f = flat(fi, fj, fk, wi, wj);
+                sum[f] += weight * (velocity + apicTerm);
+                wsum[f] += weight;
+            }
+        }
+    }
+    for (size_t f = 0; f < F; f++) {
+        float s = sum[f];
+        if (wsum[f] > eps) s /= wsum[f];
+        field[f] = s;
+        valid[f] = wsum[f] > eps;
+    }
+}
+
+/* FluidSimulation::_getIndicesAndGradientWeights (fluidsimulation.cpp:6709-6769)
+ * and _updatePICAPICMarkerParticleVelocitiesThread (6791-6843). */
+void oracle_g2p_apic(int ni, int nj, int nk, double dx, const float *u, const float *v, const float *w, int np,
+                     const float *pos, float *vel, float *aff) {
+    const float *arr[3] = {u, v, w};
+    const int wi[3] = {ni + 1, ni, ni}, wj[3] = {nj, nj + 1, nj}, wk[3] = {nk, nk, nk + 1};
+    float h = 0.5f * dx;
+    float invdx = 1.0f / dx;
+    for (int p = 0; p < np; p++) {
+        V3 x0{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        for (int d = 0; d < 3; d++) {
+            V3 off = d == 0 ? V3{0.0f, h, h} : (d == 1 ? V3{h, 0.0f, h} : V3{h, h, 0.0f});
+            V3 x = x0 - off;
+            GI g = posToIndex(x, dx);
+            V3 ip = divide(x - indexToPos(g.i, g.j, g.k, dx), (float)dx);
+            // the reference's eight gradient weights, factor by factor
+            V3 gw[8] = {
+                V3{-invdx * (1.0f - ip.y) * (1.0f - ip.z), -invdx * (1.0f - ip.x) * (1.0f - ip.z),
+                   -invdx * (1.0f - ip.x) * (1.0f - ip.y)},
+                V3{invdx * (1.0f - ip.y) * (1.0f - ip.z), ip.x * (-invdx) * (1.0f - ip.z),
+                   ip.x * (1.0f - ip.y) * (-invdx)},
+                V3{(-invdx) * ip.y * (1.0f - ip.z), (1.0f - ip.x) * invdx * (1.0f - ip.z),
+                   (1.0f - ip.x) * ip.y * (-invdx)},
+                V3{invdx * ip.y * (1.0f - ip.z), ip.x * invdx * (1.0f - ip.z), ip.x * ip.y * (-invdx)},
+                V3{(-invdx) * (1.0f - ip.y) * ip.z, (1.0f - ip.x) * (-invdx) * ip.z,
+                   (1.0f - ip.x) * (1.0f - ip.y) * invdx},
+                V3{invdx * (1.0f - ip.y) * ip.z, ip.x * (-invdx) * ip.z, ip.x * (1.0f - ip.y) * invdx},
+                V3{(-invdx) * ip.y * ip.z, (1.0f - ip.x) * invdx * ip.z, (1.0f - ip.x) * ip.y * invdx},
+                V3{invdx * ip.y * ip.z, ip.x * invdx * ip.z, ip.x * ip.y * invdx}};
+            V3 a{0.0f, 0.0f, 0.0f};
+            for (int q = 0; q < 8; q++) {
+                int i = g.i + (q & 1), j = g.j + ((q >> 1) & 1), k = g.k + (q >> 2);
+                if (!inRange(i, j, k, wi[d], wj[d], wk[d])) continue;
+                float fv = arr[d][flat(i, j, k, wi[d], wj[d])];
+                a = a + V3{gw[q].x * fv, gw[q].y * fv, gw[q].z * fv};
+            }
+            aff[9 * p + 3 * d] = a.x;
+            aff[9 * p + 3 * d + 1] = a.y;
+            aff[9 * p + 3 * d + 2] = a.z;
+        }
+    }
+    oracle_sample_velocity(ni, nj, nk, dx, u, v, w, np, pos, vel);
+}
+
 /* GridUtils::extrapolateGrid (gridutils.h:95-160) with _initializeStatusGridThread
  * (gridutils.cpp:99-118), _findExtrapolationCells (120-175) and
  * _extrapolateCellsThread (gridutils.h:43-92). */
@@ -1220,12 +1324,14 @@ struct OracleSim {
     int maxPIters = 900;
     int chunks = 1;
     bool extremeRemoval = true;
+    bool apic = false;  // velocity transfer method (fluidsimulation.cpp:2837-2857)
     std::vector<V3> bodyForces;
     // state
     bool initialized = false;
     int frame = 0, stepNumber = 0;
     double frameDt = 0, remaining = 0;
     std::vector<V3> pos, vel;
+    std::vector<float> aff;  // APIC: 9 per particle (AFFINEX, AFFINEY, AFFINEZ), zero on load
     std::vector<std::vector<V3>> loadPos, loadVel;  // one batch per load call
     bool solidGiven = false;
     std::vector<float> solid;         // nodal
@@ -1262,6 +1368,7 @@ struct OracleSim {
                 if (bounds.inside(P[q])) {
                     pos.push_back(P[q]);
                     vel.push_back(Vl[q]);
+                    if (apic) aff.insert(aff.end(), 9, 0.0f);
                 }
             }
         }
@@ -1371,7 +1478,15 @@ struct OracleSim {
         u.assign(nU(), 0.0f); v.assign(nV(), 0.0f); w.assign(nW(), 0.0f);
         vu.assign(nU(), 0); vv.assign(nV(), 0); vw.assign(nW(), 0);
         int layers = (int)std::ceil(std::sqrt(3) * cfl) + 3;  // _extrapolateFluidVelocities 6282-6286
-        if (fluid) {
+        if (fluid && apic) {
+            oracle_p2g_apic(ni, nj, nk, dx, np, P.data(), Vv.data(), aff.data(), liquidRadius(), 0, u.data(),
+                            vu.data());
+            oracle_p2g_apic(ni, nj, nk, dx, np, P.data(), Vv.data(), aff.data(), liquidRadius(), 1, v.data(),
+                            vv.data());
+            oracle_p2g_apic(ni, nj, nk, dx, np, P.data(), Vv.data(), aff.data(), liquidRadius(), 2, w.data(),
+                            vw.data());
+            extrapolateAll(layers);
+        } else if (fluid) {
             oracle_p2g(ni, nj, nk, dx, np, P.data(), Vv.data(), liquidRadius(), 0, u.data(), vu.data());
             oracle_p2g(ni, nj, nk, dx, np, P.data(), Vv.data(), liquidRadius(), 1, v.data(), vv.data());
             oracle_p2g(ni, nj, nk, dx, np, P.data(), Vv.data(), liquidRadius(), 2, w.data(), vw.data());
@@ -1411,14 +1526,20 @@ struct OracleSim {
         // constrain saved then current (6444-6459)
         constrain(su, sv, sw);
         constrain(u, v, w);
-        // G2P (6771-6784)
+        // G2P: FLIP (6771-6784) or APIC (6791-6843)
         Mac cur{ni, nj, nk, dx, u.data(), v.data(), w.data()};
         Mac sav{ni, nj, nk, dx, su.data(), sv.data(), sw.data()};
-        float r1 = (float)picflip, r2 = (float)(1 - picflip);
-        for (size_t p = 0; p < pos.size(); p++) {
-            V3 vPIC = sampleMac(cur, pos[p]);
-            V3 vFLIP = vel[p] + vPIC - sampleMac(sav, pos[p]);
-            vel[p] = r1 * vPIC + r2 * vFLIP;
+        if (apic) {
+            for (size_t p = 0; p < pos.size(); p++) P[3 * p] = pos[p].x, P[3 * p + 1] = pos[p].y, P[3 * p + 2] = pos[p].z;
+            oracle_g2p_apic(ni, nj, nk, dx, u.data(), v.data(), w.data(), np, P.data(), Vv.data(), aff.data());
+            for (size_t p = 0; p < pos.size(); p++) vel[p] = V3{Vv[3 * p], Vv[3 * p + 1], Vv[3 * p + 2]};
+        } else {
+            float r1 = (float)picflip, r2 = (float)(1 - picflip);
+            for (size_t p = 0; p < pos.size(); p++) {
+                V3 vPIC = sampleMac(cur, pos[p]);
+                V3 vFLIP = vel[p] + vPIC - sampleMac(sav, pos[p]);
+                vel[p] = r1 * vPIC + r2 * vFLIP;
+            }
         }
         advance(dt);
     }
@@ -1571,10 +1692,13 @@ struct OracleSim {
             if (!removed[p]) {
                 pos[q] = pos[p];
                 vel[q] = vel[p];
+                if (apic)
+                    for (int e = 0; e < 9; e++) aff[9 * q + e] = aff[9 * p + e];
                 q++;
             }
         pos.resize(q);
         vel.resize(q);
+        if (apic) aff.resize(9 * q);
     }
 };
 
@@ -1591,6 +1715,10 @@ void oracle_sim_add_body_force(OracleSim *s, double fx, double fy, double fz) {
     s->bodyForces.push_back(mk(fx, fy, fz));
 }
 void oracle_sim_set_picflip_ratio(OracleSim *s, double r) { s->picflip = r; }
+void oracle_sim_set_apic(OracleSim *s, int on) { s->apic = on != 0; }
+void oracle_sim_get_affine(OracleSim *s, float *aff) {
+    for (size_t e = 0; e < s->aff.size(); e++) aff[e] = s->aff[e];
+}
 void oracle_sim_set_cfl(OracleSim *s, int cfl) { s->cfl = cfl; }
 void oracle_sim_set_pressure_tolerance(OracleSim *s, double tol) { s->ptol = tol; }
 void oracle_sim_set_max_pressure_iterations(OracleSim *s, int n) { s->maxPIters = n; }

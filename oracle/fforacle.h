@@ -46,6 +46,19 @@ void oracle_p2g(int ni, int nj, int nk, double dx, int np, const float *pos,
                 const float *vel, double radius, int dir, float *field,
                 unsigned char *valid);
 
+/* APIC P2G of one velocity component (velocityadvector.cpp:38-465, 543-623):
+ * trilinear deposits of v + affine . (node - p) to the 8 nodes of the
+ * particle's cell in each 10^3 block it belongs to.  aff is np x 9: the
+ * AFFINEX, AFFINEY and AFFINEZ vectors of each particle. */
+void oracle_p2g_apic(int ni, int nj, int nk, double dx, int np, const float *pos,
+                     const float *vel, const float *aff, double radius, int dir,
+                     float *field, unsigned char *valid);
+
+/* APIC G2P (fluidsimulation.cpp:6709-6769, 6791-6843): vel = the trilinear
+ * sample, aff (np x 9) = the gradients of the three components. */
+void oracle_g2p_apic(int ni, int nj, int nk, double dx, const float *u, const float *v,
+                     const float *w, int np, const float *pos, float *vel, float *aff);
+
 /* Layered extrapolation of one face array (gridutils.h:43-160). */
 void oracle_extrapolate(int wi, int wj, int wk, float *field,
                         const unsigned char *valid, int layers);
@@ -104,6 +117,10 @@ OracleSim *oracle_sim_new(int ni, int nj, int nk, double dx);
 void oracle_sim_destroy(OracleSim *s);
 void oracle_sim_add_body_force(OracleSim *s, double fx, double fy, double fz);
 void oracle_sim_set_picflip_ratio(OracleSim *s, double r);
+/* velocity transfer method: 1 = APIC, 0 = FLIP (the default); set before
+ * initialize.  oracle_sim_get_affine writes 9 floats per particle. */
+void oracle_sim_set_apic(OracleSim *s, int on);
+void oracle_sim_get_affine(OracleSim *s, float *aff);
 void oracle_sim_set_cfl(OracleSim *s, int cfl);
 void oracle_sim_set_pressure_tolerance(OracleSim *s, double tol);
 void oracle_sim_set_max_pressure_iterations(OracleSim *s, int n);

@@ -136,6 +136,78 @@ void refprobe_p2g(int ni, int nj, int nk, double dx, int np, const float *pos, c
     boolOut(valid.validW, vw);
 }
 
+void refprobe_p2g_apic(int ni, int nj, int nk, double dx, int np, const float *pos, const float *vel,
+                       const float *aff, double radius, float *u, float *v, float *w, unsigned char *vu,
+                       unsigned char *vv, unsigned char *vw) {
+    ParticleSystem ps;
+    ps.addAttributeVector3("POSITION");
+    ps.addAttributeVector3("VELOCITY");
+    const char *names[3] = {"AFFINEX", "AFFINEY", "AFFINEZ"};
+    for (int d = 0; d < 3; d++) ps.addAttributeVector3(names[d]);
+    std::vector<vmath::vec3> *Pv, *Vv, *A[3];
+    ps.getAttributeValues("POSITION", Pv);
+    ps.getAttributeValues("VELOCITY", Vv);
+    for (int d = 0; d < 3; d++) ps.getAttributeValues(names[d], A[d]);
+    for (int p = 0; p < np; p++) {
+        Pv->push_back(vmath::vec3(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]));
+        Vv->push_back(vmath::vec3(vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]));
+        for (int d = 0; d < 3; d++)
+            A[d]->push_back(vmath::vec3(aff[9 * p + 3 * d], aff[9 * p + 3 * d + 1], aff[9 * p + 3 * d + 2]));
+    }
+    ps.update();
+    MACVelocityField mac(ni, nj, nk, dx);
+    ValidVelocityComponentGrid valid(ni, nj, nk);
+    VelocityAdvectorParameters params;
+    params.particles = &ps;
+    params.vfield = &mac;
+    params.validVelocities = &valid;
+    params.particleRadius = radius;
+    params.velocityTransferMethod = VelocityAdvectorTransferMethod::APIC;
+    VelocityAdvector adv;
+    adv.advect(params);
+    copyOut(*mac.getArray3dU(), u);
+    copyOut(*mac.getArray3dV(), v);
+    copyOut(*mac.getArray3dW(), w);
+    boolOut(valid.validU, vu);
+    boolOut(valid.validV, vv);
+    boolOut(valid.validW, vw);
+}
+
+/* FluidSimulation::_updatePICAPICMarkerParticleVelocitiesThread on a field
+ * copied into a fresh simulation object: velocities and affine vectors. */
+void refprobe_g2p_apic(int ni, int nj, int nk, double dx, const float *u, const float *v, const float *w, int np,
+                       const float *pos, float *vel, float *aff) {
+    FluidSimulation sim(ni, nj, nk, dx);
+    sim.setVelocityTransferMethodAPIC();
+    sim._initializeParticleSystems();
+    std::vector<vmath::vec3> *P, *V, *A[3];
+    sim._markerParticles.getAttributeValues("POSITION", P);
+    sim._markerParticles.getAttributeValues("VELOCITY", V);
+    for (int p = 0; p < np; p++) {
+        P->push_back(vmath::vec3(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]));
+        V->push_back(vmath::vec3());
+    }
+    sim._markerParticles.update();
+    sim._markerParticles.getAttributeValues("AFFINEX", A[0]);
+    sim._markerParticles.getAttributeValues("AFFINEY", A[1]);
+    sim._markerParticles.getAttributeValues("AFFINEZ", A[2]);
+    sim._MACVelocity = MACVelocityField(ni, nj, nk, dx);
+    copyIn(*sim._MACVelocity.getArray3dU(), u);
+    copyIn(*sim._MACVelocity.getArray3dV(), v);
+    copyIn(*sim._MACVelocity.getArray3dW(), w);
+    sim._updatePICAPICMarkerParticleVelocitiesThread(0, np);
+    for (int p = 0; p < np; p++) {
+        vel[3 * p] = V->at(p).x;
+        vel[3 * p + 1] = V->at(p).y;
+        vel[3 * p + 2] = V->at(p).z;
+        for (int d = 0; d < 3; d++) {
+            aff[9 * p + 3 * d] = A[d]->at(p).x;
+            aff[9 * p + 3 * d + 1] = A[d]->at(p).y;
+            aff[9 * p + 3 * d + 2] = A[d]->at(p).z;
+        }
+    }
+}
+
 void refprobe_extrapolate(int wi, int wj, int wk, float *field, const unsigned char *valid,
                           int layers) {
     Array3d<float> g(wi, wj, wk, 0.0f);

@@ -41,6 +41,8 @@ def oracle():
         L.oracle_sim_last_pressure_error.restype = C.c_double
         L.oracle_sim_add_body_force.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.oracle_sim_set_picflip_ratio.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_sim_set_apic.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_sim_get_affine.argtypes = [C.c_void_p, C.c_void_p]
         L.oracle_sim_set_cfl.argtypes = [C.c_void_p, C.c_int]
         L.oracle_sim_set_pressure_tolerance.argtypes = [C.c_void_p, C.c_double]
         L.oracle_sim_set_max_pressure_iterations.argtypes = [C.c_void_p, C.c_int]
@@ -126,6 +128,35 @@ def p2g(n3, dx, pos, vel, radius):
     return tuple(fields), tuple(valids)
 
 
+def p2g_apic(n3, dx, pos, vel, aff, radius):
+    """APIC P2G; aff (P, 9): the AFFINEX, AFFINEY, AFFINEZ vectors."""
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    vel = _f32(vel).reshape(-1, 3)
+    aff = _f32(aff).reshape(-1, 9)
+    shapes = [(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]
+    fields, valids = [], []
+    for d, s in enumerate(shapes):
+        f = np.zeros(s, np.float32)
+        v = np.zeros(s, np.uint8)
+        oracle().oracle_p2g_apic(ni, nj, nk, C.c_double(dx), len(pos), fp(pos), fp(vel), fp(aff),
+                                 C.c_double(radius), d, fp(f), fp(v))
+        fields.append(f)
+        valids.append(v)
+    return tuple(fields), tuple(valids)
+
+
+def g2p_apic(n3, dx, uvw, pos):
+    """APIC G2P: (velocities (P, 3), affine vectors (P, 9))."""
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    vel = np.zeros_like(pos)
+    aff = np.zeros((len(pos), 9), np.float32)
+    oracle().oracle_g2p_apic(ni, nj, nk, C.c_double(dx), *[fp(_f32(x)) for x in uvw], len(pos), fp(pos), fp(vel),
+                             fp(aff))
+    return vel, aff
+
+
 def extrapolate(field, valid, layers):
     a = np.array(field, np.float32, copy=True, order="C")
     wk, wj, wi = a.shape
@@ -201,10 +232,11 @@ def sample_velocity(n3, dx, uvw, pos):
 class OracleSim:
     """The whole-engine oracle with the FluidSimulation stepping semantics."""
 
-    def __init__(self, n3, dx, gravity=(0.0, -9.81, 0.0), picflip=0.05, cfl=5, tol=1e-9, chunks=1):
+    def __init__(self, n3, dx, gravity=(0.0, -9.81, 0.0), picflip=0.05, cfl=5, tol=1e-9, chunks=1, apic=False):
         L = oracle()
         self.n3, self.dx = n3, dx
         self.h = C.c_void_p(L.oracle_sim_new(*n3, dx))
+        L.oracle_sim_set_apic(self.h, 1 if apic else 0)
         L.oracle_sim_add_body_force(self.h, *gravity)
         L.oracle_sim_set_picflip_ratio(self.h, picflip)
         L.oracle_sim_set_cfl(self.h, cfl)
@@ -234,6 +266,11 @@ class OracleSim:
         v = np.zeros((n, 3), np.float32)
         oracle().oracle_sim_get_particles(self.h, fp(p), fp(v))
         return p, v
+
+    def affine(self):
+        aff = np.zeros((oracle().oracle_sim_num_particles(self.h), 9), np.float32)
+        oracle().oracle_sim_get_affine(self.h, fp(aff))
+        return aff
 
     def substeps(self):
         return oracle().oracle_sim_last_substeps(self.h)
@@ -267,13 +304,15 @@ class ReferenceSim:
     """The reference engine built from its sources, driven through its C ABI
     (fluidsimulation_c.cpp) exactly like the reference's ctypes layer does."""
 
-    def __init__(self, n3, dx, threads=1, gravity=(0.0, -9.81, 0.0), picflip=0.05, cfl=5):
+    def __init__(self, n3, dx, threads=1, gravity=(0.0, -9.81, 0.0), picflip=0.05, cfl=5, apic=False):
         L = C.CDLL(REF_ENGINE)
         self.L = L
         err = C.c_int()
         L.FluidSimulation_new_from_dimensions.restype = C.c_void_p
         L.FluidSimulation_new_from_dimensions.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p]
         self.h = C.c_void_p(L.FluidSimulation_new_from_dimensions(*n3, dx, C.byref(err)))
+        if apic:
+            L.FluidSimulation_set_velocity_transfer_method_APIC(self.h, C.byref(err))
         L.FluidSimulation_set_max_thread_count(self.h, threads, C.byref(err))
         L.FluidSimulation_disable_surface_reconstruction(self.h, C.byref(err))
         L.FluidSimulation_add_body_force(self.h, *[C.c_double(g) for g in gravity], C.byref(err))
@@ -308,6 +347,18 @@ class ReferenceSim:
         self.L.FluidSimulation_get_marker_particle_position_data_range(self.h, 0, n, fp(p), C.byref(err))
         self.L.FluidSimulation_get_marker_particle_velocity_data_range(self.h, 0, n, fp(v), C.byref(err))
         return p, v
+
+    def affine(self):
+        err = C.c_int()
+        n = self.L.FluidSimulation_get_num_marker_particles(self.h, C.byref(err))
+        out = np.zeros((n, 9), np.float32)
+        parts = [np.zeros((n, 3), np.float32) for _ in range(3)]
+        for a, name in zip(parts, ("affinex", "affiney", "affinez")):
+            getattr(self.L, "FluidSimulation_get_marker_particle_%s_data_range" % name)(self.h, 0, n, fp(a),
+                                                                                     C.byref(err))
+        for d in range(3):
+            out[:, 3 * d:3 * d + 3] = parts[d]
+        return out
 
     def solid_phi(self, n3):
         ni, nj, nk = n3
