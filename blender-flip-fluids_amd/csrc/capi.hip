@@ -183,7 +183,13 @@ int FluidSimulation_is_velocity_transfer_method_FLIP(FluidSimulation *o, int *er
     return guarded(err, [&] { return (int)S(o)->isVelocityTransferMethodFLIP(); });
 }
 int FluidSimulation_is_velocity_transfer_method_APIC(FluidSimulation *o, int *err) {
-    return guarded(err, [&] { return (int)!S(o)->isVelocityTransferMethodFLIP(); });
+    return guarded(err, [&] { return (int)S(o)->isVelocityTransferMethodAPIC(); });
+}
+double FluidSimulation_get_PICAPIC_ratio(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getPICAPICRatio(); });
+}
+void FluidSimulation_set_PICAPIC_ratio(FluidSimulation *o, double r, int *err) {
+    guarded(err, [&] { S(o)->setPICAPICRatio(r); });
 }
 double FluidSimulation_get_PICFLIP_ratio(FluidSimulation *o, int *err) {
     return guarded(err, [&] { return S(o)->getPICFLIPRatio(); });
@@ -195,6 +201,19 @@ void FluidSimulation_set_PICFLIP_ratio(FluidSimulation *o, double r, int *err) {
 void FluidSimulation_load_marker_particle_data(FluidSimulation *o, FluidSimulationMarkerParticleData d,
                                                int *err) {
     guarded(err, [&] { S(o)->loadMarkerParticleData(d.size, d.positions, d.velocities); });
+}
+void FluidSimulation_load_marker_particle_affine_data(FluidSimulation *o, FluidSimulationMarkerParticleAffineData d,
+                                                      int *err) {
+    guarded(err, [&] { S(o)->loadMarkerParticleAffineData(d.size, d.affineX, d.affineY, d.affineZ); });
+}
+void FluidSimulation_get_marker_particle_affinex_data_range(FluidSimulation *o, int a, int b, char *data, int *err) {
+    guarded(err, [&] { S(o)->getMarkerParticleAffineDataRange(0, a, b, data); });
+}
+void FluidSimulation_get_marker_particle_affiney_data_range(FluidSimulation *o, int a, int b, char *data, int *err) {
+    guarded(err, [&] { S(o)->getMarkerParticleAffineDataRange(1, a, b, data); });
+}
+void FluidSimulation_get_marker_particle_affinez_data_range(FluidSimulation *o, int a, int b, char *data, int *err) {
+    guarded(err, [&] { S(o)->getMarkerParticleAffineDataRange(2, a, b, data); });
 }
 int FluidSimulation_get_num_marker_particles(FluidSimulation *o, int *err) {
     return guarded(err, [&] { return S(o)->getNumMarkerParticles(); });
@@ -861,6 +880,95 @@ void FFAMD_rollout_stats(FFAMD_Rollout *r, long long *collided, long long *fwd, 
         *collided = R(r)->collided();
         *fwd = R(r)->forwardIterations();
         *bwd = R(r)->backwardIterations();
+    });
+}
+
+void FFAMD_p2g_apic(int ni, int nj, int nk, double dx, int np, const float *pos, const float *vel, const float *aff,
+                    double radius, float *u, float *v, float *w, unsigned char *vu, unsigned char *vv,
+                    unsigned char *vw, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (np < 0) throw std::domain_error("negative particle count");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        const size_t P = (size_t)np;
+        HostSoA p(pos, P, sc.s), vl(vel, P, sc.s);
+        std::vector<DBuf<float> *> a;
+        std::vector<float> c(P);
+        for (int e = 0; e < 9; e++) {
+            a.push_back(new DBuf<float>(P));
+            for (size_t q = 0; q < P; q++) c[q] = aff[9 * q + e];
+            up(a.back()->p, c.data(), P, sc.s);
+            sync(sc.s);
+        }
+        const size_t nf[3] = {(size_t)(ni + 1) * nj * nk, (size_t)ni * (nj + 1) * nk, (size_t)ni * nj * (nk + 1)};
+        DBuf<float> f0(nf[0]), f1(nf[1]), f2(nf[2]);
+        DBuf<uint8_t> b0(nf[0]), b1(nf[1]), b2(nf[2]);
+        const size_t nb = ff::apicBuckets(g), R = ff::apicRecords(np);
+        DBuf<uint8_t> mark(ff::apicBlocks(g)), active(ff::apicBlocks(g));
+        DBuf<int> keys(R), slot(R), perm(R), counts(nb + 1), offsets(nb + 1), scan(ff::scanScratchInts(nb + 1));
+        ff::ApicScratch A{mark, active, keys, slot, perm, counts, offsets, scan};
+        const float *velp[3] = {vl.x, vl.y, vl.z};
+        const float *affp[9];
+        for (int e = 0; e < 9; e++) affp[e] = *a[e];
+        float *fields[3] = {f0, f1, f2};
+        uint8_t *valid[3] = {b0, b1, b2};
+        ff::p2gApic(g, np, p.x, p.y, p.z, velp, affp, radius, fields, valid, A, sc.s);
+        down(u, f0.p, nf[0], sc.s); down(v, f1.p, nf[1], sc.s); down(w, f2.p, nf[2], sc.s);
+        down(vu, b0.p, nf[0], sc.s); down(vv, b1.p, nf[1], sc.s); down(vw, b2.p, nf[2], sc.s);
+        sync(sc.s);
+        for (auto *x : a) delete x;
+    });
+}
+
+void FFAMD_g2p_apic(int ni, int nj, int nk, double dx, const float *u, const float *v, const float *w, int np,
+                    const float *pos, float *vel, float *aff, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (np < 0) throw std::domain_error("negative particle count");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        const size_t P = (size_t)np;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> du(nu), dv(nv), dw(nw);
+        up(du.p, u, nu, sc.s); up(dv.p, v, nv, sc.s); up(dw.p, w, nw, sc.s);
+        std::vector<DBuf<float> *> pa, aa;
+        std::vector<float> c(P);
+        for (int a = 0; a < 6; a++) {
+            pa.push_back(new DBuf<float>(P));
+            for (size_t q = 0; q < P; q++) c[q] = a < 3 ? pos[3 * q + a] : 0.0f;
+            up(pa.back()->p, c.data(), P, sc.s);
+            sync(sc.s);
+        }
+        ff::AffineArrays A;
+        for (int e = 0; e < 9; e++) {
+            aa.push_back(new DBuf<float>(P));
+            A.a[e] = *aa.back();
+        }
+        size_t NN = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
+        DBuf<float> solid(NN);
+        // positive solid SDF everywhere, an empty near-solid grid and a zero
+        // substep: G2P only, no motion and no collision
+        std::vector<float> one(NN, 1.0f);
+        up(solid.p, one.data(), NN, sc.s);
+        DBuf<uint8_t> near(1);
+        ff::hipCheck(hipMemsetAsync(near, 0, 1, sc.s), "memset");
+        ff::Particles pp{*pa[0], *pa[1], *pa[2], *pa[3], *pa[4], *pa[5], np};
+        ff::g2pAdvect(g, pp, ff::MacView{du, dv, dw}, ff::MacView{du, dv, dw}, solid, near, 1, 1, 1, 1.0f, 0.0f, 0.0,
+                      5.0, sc.s, nullptr, &A);
+        std::vector<float> x(P);
+        for (int e = 0; e < 3; e++) {
+            down(x.data(), pa[3 + e]->p, P, sc.s);
+            sync(sc.s);
+            for (size_t q = 0; q < P; q++) vel[3 * q + e] = x[q];
+        }
+        for (int e = 0; e < 9; e++) {
+            down(x.data(), aa[e]->p, P, sc.s);
+            sync(sc.s);
+            for (size_t q = 0; q < P; q++) aff[9 * q + e] = x[q];
+        }
+        for (auto *b : pa) delete b;
+        for (auto *b : aa) delete b;
     });
 }
 

@@ -81,9 +81,14 @@ public:
     void enableSurfaceReconstruction() { _surfaceReconstruction = true; }
     void disableSurfaceReconstruction() { _surfaceReconstruction = false; }
     bool isSurfaceReconstructionEnabled() const { return _surfaceReconstruction; }
-    void setVelocityTransferMethodFLIP() {}
+    // velocity transfer method (fluidsimulation.cpp:2837-2857) and the PIC/APIC
+    // ratio (2877-2893; stored and validated, unused by the transfer itself)
+    void setVelocityTransferMethodFLIP() { _apic = false; }
     void setVelocityTransferMethodAPIC();
-    bool isVelocityTransferMethodFLIP() const { return true; }
+    bool isVelocityTransferMethodFLIP() const { return !_apic; }
+    bool isVelocityTransferMethodAPIC() const { return _apic; }
+    void setPICAPICRatio(double r);
+    double getPICAPICRatio() const { return _picapic; }
 
     // additive settings
     void setPressureSolverTolerance(double tol);
@@ -97,6 +102,10 @@ public:
 
     // particle I/O (fluidsimulation.cpp:3426-3839)
     void loadMarkerParticleData(int n, const char *positions, const char *velocities);
+    // APIC affine vectors of one load batch (fluidsimulation.cpp:3841-3863)
+    void loadMarkerParticleAffineData(int n, const char *ax, const char *ay, const char *az);
+    // affine vectors of particles [start, end) (fluidsimulation.cpp:3455-3501)
+    void getMarkerParticleAffineDataRange(int component, int start, int end, char *out);
     int getNumMarkerParticles() const;
     void getMarkerParticlePositionDataRange(int start, int end, char *out);
     void getMarkerParticleVelocityDataRange(int start, int end, char *out);
@@ -132,6 +141,8 @@ private:
 
     double _cfl = 5.0;
     double _picflip = 0.05;
+    bool _apic = false;
+    double _picapic = 0.0;
     int _minSteps = 1, _maxSteps = 6;
     int _maxPressureIterations = 900;
     double _pressureTolerance = 1e-9;
@@ -146,13 +157,19 @@ private:
     bool _solidGiven = false;
     std::vector<float> _solidHost;
 
-    std::vector<std::vector<float>> _loadPos, _loadVel;
+    std::vector<std::vector<float>> _loadPos, _loadVel, _loadAff;  // _loadAff: 9 floats per particle
 
     // particles (double-buffered for compaction)
     int _capacity = 0;
     DevBuf<float> _pa[6], _pb[6];
     Particles _P{};
     Particles _Q{};
+    // APIC: AFFINEX.xyz, AFFINEY.xyz, AFFINEZ.xyz (SoA), double-buffered like the particles
+    DevBuf<float> _aa[9], _ab[9];
+    AffineArrays _A{}, _B{};
+    DevBuf<uint8_t> _apMark, _apActive;
+    DevBuf<int> _apKeys, _apSlot, _apPerm, _apCounts, _apOffsets, _apScan;
+    ApicScratch _apic_scratch{};
 
     CellSort _cs{};
     DevBuf<int> _keys, _slot, _perm, _counts, _offsets, _keep, _rank, _scan, _maxCount;
@@ -180,6 +197,9 @@ private:
     KernelTimers _timers;
 
     void allocate();
+    // affine arrays and the APIC transfer's scratch for the current capacity
+    // (zero: the particles take zero affine vectors)
+    void allocateAffine(bool zero);
     void uploadParticles();
     double nextTimeStep(double dt);
     void stepFluid(double dt);

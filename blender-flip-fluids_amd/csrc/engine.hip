@@ -107,7 +107,13 @@ void FluidSimulation::setDensity(double d) {
     _density = d;
 }
 void FluidSimulation::setVelocityTransferMethodAPIC() {
-    throw std::runtime_error("Error: the APIC velocity transfer is not available in this engine.\n");
+    _apic = true;
+    if (_initialized && !_A.a[0]) allocateAffine(true);
+}
+void FluidSimulation::setPICAPICRatio(double r) {
+    if (r < 0.0 || r > 1.0)
+        throw std::domain_error("Error: PICAPIC ratio must be in range [0.0, 1.0].\nratio: " + toString(r) + "\n");
+    _picapic = r;
 }
 void FluidSimulation::setPressureSolverTolerance(double tol) {
     if (!(tol > 0.0))
@@ -145,6 +151,39 @@ void FluidSimulation::loadMarkerParticleData(int n, const char *positions, const
     const float *p = (const float *)positions, *v = (const float *)velocities;
     _loadPos.emplace_back(p, p + 3 * (size_t)n);
     _loadVel.emplace_back(v, v + 3 * (size_t)n);
+}
+
+// FluidSimulation::loadMarkerParticleAffineData (fluidsimulation.cpp:3841-3863)
+void FluidSimulation::loadMarkerParticleAffineData(int n, const char *ax, const char *ay, const char *az) {
+    if (n <= 0) return;
+    const float *a[3] = {(const float *)ax, (const float *)ay, (const float *)az};
+    std::vector<float> b(9 * (size_t)n);
+    for (size_t q = 0; q < (size_t)n; q++)
+        for (int d = 0; d < 3; d++)
+            for (int e = 0; e < 3; e++) b[9 * q + 3 * d + e] = a[d][3 * q + e];
+    _loadAff.push_back(std::move(b));
+}
+
+void FluidSimulation::allocateAffine(bool zero) {
+    const size_t P = (size_t)std::max(_capacity, 1);
+    for (int e = 0; e < 9; e++) {
+        _aa[e].alloc(P);
+        _ab[e].alloc(P);
+        _A.a[e] = _aa[e];
+        _B.a[e] = _ab[e];
+        if (zero) hipCheck(hipMemsetAsync(_aa[e], 0, sizeof(float) * P, _stream), "memset");
+    }
+    const size_t nb = apicBuckets(_g), R = apicRecords((int)P);
+    _apMark.alloc(apicBlocks(_g));
+    _apActive.alloc(apicBlocks(_g));
+    _apKeys.alloc(R);
+    _apSlot.alloc(R);
+    _apPerm.alloc(R);
+    _apCounts.alloc(nb + 1);
+    _apOffsets.alloc(nb + 1);
+    _apScan.alloc(scanScratchInts(nb + 1));
+    _apic_scratch = ApicScratch{_apMark, _apActive, _apKeys, _apSlot, _apPerm, _apCounts, _apOffsets, _apScan};
+    if (zero) hipCheck(hipStreamSynchronize(_stream), "sync");
 }
 
 void FluidSimulation::allocate() {
@@ -205,23 +244,34 @@ void FluidSimulation::initialize() {
     // operator[] picks the fragment as (int)(i * (1.0 / E)) in double
     // (fragmentedvector.h:140-151, 294-301), which reads element (c-1)*E at
     // some exact multiples i = c*E.  Particles outside [0, n*dx) are dropped.
-    std::vector<float> hp, hv;
-    const int E = (int)(5000000u / 24u);
-    const double inv = 1.0 / (double)E;
+    // APIC affine vectors load with a batch when every batch has them and the
+    // sizes match (_loadParticles, 5078-5110, and _loadMarkerParticles,
+    // 4918-4919); their queue is a FragmentedVector<MarkerParticleAffine> of
+    // 36-byte elements, read with the same fragment rule.
+    std::vector<float> hp, hv, ha;
+    const int E = (int)(5000000u / 24u), EA = (int)(5000000u / 36u);
+    const double inv = 1.0 / (double)E, invA = 1.0 / (double)EA;
+    const bool affQueue = _loadAff.size() == _loadPos.size();
     Box bounds{V3{0, 0, 0}, _g.ni * _g.dx, _g.nj * _g.dx, _g.nk * _g.dx};
     for (size_t b = 0; b < _loadPos.size(); b++) {
         const std::vector<float> &P = _loadPos[b], &Vl = _loadVel[b];
         size_t n = P.size() / 3;
+        const bool useAff = _apic && affQueue && _loadAff[b].size() == 9 * n;
         for (size_t i = 0; i < n; i++) {
             size_t q = (size_t)((int)((int)i * inv)) * E + (int)i % E;
             V3 x{P[3 * q], P[3 * q + 1], P[3 * q + 2]};
             if (!bounds.inside(x)) continue;
             hp.insert(hp.end(), {x.x, x.y, x.z});
             hv.insert(hv.end(), {Vl[3 * q], Vl[3 * q + 1], Vl[3 * q + 2]});
+            if (_apic) {
+                const size_t qa = (size_t)((int)((int)i * invA)) * EA + (int)i % EA;
+                for (int e = 0; e < 9; e++) ha.push_back(useAff ? _loadAff[b][9 * qa + e] : 0.0f);
+            }
         }
     }
     _loadPos.clear();
     _loadVel.clear();
+    _loadAff.clear();
     int np = (int)(hp.size() / 3);
     _capacity = np;
     allocate();
@@ -235,6 +285,13 @@ void FluidSimulation::initialize() {
     }
     _P = Particles{_pa[0], _pa[1], _pa[2], _pa[3], _pa[4], _pa[5], np};
     _Q = Particles{_pb[0], _pb[1], _pb[2], _pb[3], _pb[4], _pb[5], 0};
+    if (_apic) {
+        allocateAffine(false);
+        for (int e = 0; e < 9; e++) {
+            for (int q = 0; q < np; q++) col[q] = ha[9 * (size_t)q + e];
+            hipCheck(hipMemcpy(_aa[e], col.data(), sizeof(float) * np, hipMemcpyHostToDevice), "upload affine");
+        }
+    }
 
     // static solids: boundary SDF (or the given one), weights, near-solid grid
     if (_solidGiven)
@@ -395,7 +452,16 @@ void FluidSimulation::stepFluid(double dt) {
     hipCheck(hipEventRecord(_ev[1], _stream), "event");
     // advect velocity field: P2G + extrapolation (5623-5661)
     _timers.begin(T_P2G, _stream, 3);
-    p2g(_g, _cs, radius, _u, _v, _w, _vu, _vv, _vw, _stream);
+    if (_apic) {
+        const float *vel[3] = {_P.vx, _P.vy, _P.vz};
+        const float *aff[9];
+        for (int e = 0; e < 9; e++) aff[e] = _A.a[e];
+        float *fields[3] = {_u, _v, _w};
+        uint8_t *valid[3] = {_vu, _vv, _vw};
+        p2gApic(_g, _P.n, _P.px, _P.py, _P.pz, vel, aff, radius, fields, valid, _apic_scratch, _stream);
+    } else {
+        p2g(_g, _cs, radius, _u, _v, _w, _vu, _vv, _vw, _stream);
+    }
     _timers.end(T_P2G, _stream);
     extrapolate(layers);
     // save (5671-5685) and body force (5837-5876)
@@ -436,7 +502,7 @@ void FluidSimulation::stepFluid(double dt) {
     // G2P (6932-6945) + RK3 advection and collision (7853-7890)
     _timers.begin(T_G2P_ADVECT, _stream);
     g2pAdvect(_g, _P, MacView{_u, _v, _w}, MacView{_su, _sv, _sw}, _solid, _near, _ci, _cj, _ck,
-              (float)_picflip, (float)(1 - _picflip), dt, _cfl, _stream);
+              (float)_picflip, (float)(1 - _picflip), dt, _cfl, _stream, nullptr, _apic ? &_A : nullptr);
     _timers.end(T_G2P_ADVECT, _stream);
     hipCheck(hipEventRecord(_ev[6], _stream), "event");
     // removal (7773-7851)
@@ -502,7 +568,12 @@ void FluidSimulation::removeParticles() {
     double maxspeedsq = fmaxspeed * fmaxspeed;
     removalFlags(_g, _P, _solid, _keys, _counts, _slot, _offsets, _perm, _scan, 250, _extremeRemoval,
                  maxspeedsq, _keep, _maxCount, _stream);
+    const int before = _P.n;
     compactParticles(_P, _Q, _keep, _rank, _scan, _stream);
+    if (_apic) {
+        compactAffine(_A, _B, before, _keep, _rank, _stream);
+        std::swap(_A, _B);
+    }
     std::swap(_P, _Q);
     _Q.n = 0;
 }
@@ -526,6 +597,17 @@ void interleave(const float *x, const float *y, const float *z, int start, int e
     }
 }
 }  // namespace
+
+// getMarkerParticleAffine{X,Y,Z}DataRange (fluidsimulation.cpp:3455-3501)
+void FluidSimulation::getMarkerParticleAffineDataRange(int component, int start, int end, char *out) {
+    if (start < 0 || end > getNumMarkerParticles() || start > end)
+        throw std::domain_error("Error: invalid range.\nrange: [" + toString(start) + ", " + toString(end) + "]\n");
+    if (!_apic || !_A.a[0])
+        throw std::runtime_error("Error: affine data exists with the APIC velocity transfer only.\n");
+    if (end > start)
+        interleave(_A.a[3 * component], _A.a[3 * component + 1], _A.a[3 * component + 2], start, end, (float *)out,
+                   _stream);
+}
 
 // getMarkerParticle{Position,Velocity}DataRange (fluidsimulation.cpp:3426-3460)
 void FluidSimulation::getMarkerParticlePositionDataRange(int start, int end, char *out) {

@@ -71,18 +71,49 @@ void liquidSDFPostprocess(Grid g, const float *solidNodes, float *phi, hipStream
 void p2g(Grid g, const CellSort &cs, double radius, float *u, float *v, float *w, uint8_t *vu,
          uint8_t *vv, uint8_t *vw, hipStream_t s, float *wsu = nullptr, float *wsv = nullptr,
          float *wsw = nullptr);
+// stable counting sort of n keys (-1: no bucket) into nb buckets: offsets
+// (nb + 1, exclusive) and perm (the ids of each bucket in ascending order);
+// slot holds n ints, counts nb + 1, scanScratch scanScratchInts(nb + 1)
+void sortKeys(const int *keys, size_t n, size_t nb, int *slot, int *counts, int *offsets, int *perm,
+              int *scanScratch, hipStream_t s);
+// APIC P2G (kernels_apic.hip): per direction, records of (particle, member
+// block) keyed by the particle's cell in the block's frame, sorted, then
+// gathered per face in particle order.  vel: the three velocity components
+// (SoA); aff: AFFINEX.xyz, AFFINEY.xyz, AFFINEZ.xyz (SoA, nine arrays).
+struct ApicScratch {
+    uint8_t *mark, *active;          // apicBlocks(g) bytes each
+    int *keys, *slot, *perm;         // apicRecords(np) ints each
+    int *counts, *offsets;           // apicBuckets(g) + 1 ints each
+    int *scan;                       // scanScratchInts(apicBuckets(g) + 1) ints
+};
+size_t apicRecords(int np);
+size_t apicBuckets(Grid g);
+size_t apicBlocks(Grid g);
+void p2gApic(Grid g, int np, const float *px, const float *py, const float *pz, const float *const vel[3],
+             const float *const aff[9], double radius, float *const field[3], uint8_t *const valid[3],
+             ApicScratch &A, hipStream_t s);
 // max over particles of dot(v, v) (float) as double
 void maxSpeedSq(const Particles &p, double *partial, double *out, hipStream_t s);
 
 // G2P blend + RK3 + collision; writes new positions in place.  moved
 // (optional, P bytes) flags the particles whose position the collision
-// response changed.
+// response changed.  With affine arrays (AFFINEX.xyz, AFFINEY.xyz,
+// AFFINEZ.xyz, SoA) the G2P is APIC's: the velocity becomes the sample and
+// the affine vectors the field's gradients.
 struct MacView {
     const float *u, *v, *w;
 };
+struct AffineArrays {
+    float *a[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
 void g2pAdvect(Grid g, Particles p, MacView cur, MacView saved, const float *solidNodes,
                const uint8_t *nearSolid, int ci, int cj, int ck, float picRatio, float flipRatio,
-               double dt, double cfl, hipStream_t s, uint8_t *moved = nullptr);
+               double dt, double cfl, hipStream_t s, uint8_t *moved = nullptr,
+               const AffineArrays *affine = nullptr);
+// the affine arrays of the particles compactParticles just kept (same keep
+// flags and ranks; n = particle count before the compaction)
+void compactAffine(const AffineArrays &src, const AffineArrays &dst, int n, const int *keep, const int *rank,
+                   hipStream_t s);
 // v = trilinear sample of the MAC field at the positions (SoA)
 void sampleParticles(Grid g, int np, const float *px, const float *py, const float *pz, MacView M, float *vx,
                      float *vy, float *vz, hipStream_t s);

@@ -657,16 +657,62 @@ __device__ V3 resolveCollision(Grid g, const float *sp, NearGrid N, double cfl, 
 
 // G2P (fluidsimulation.cpp:6771-6784), then _RK3 (7616-7623) on the unchanged
 // current field and the collision of the old/new pair (7626-7644).
+// FluidSimulation::_getIndicesAndGradientWeights (fluidsimulation.cpp:6709-6769)
+// and the affine sums of _updatePICAPICMarkerParticleVelocitiesThread
+// (6791-6843): the gradient of component d of the field at p, in the
+// reference's float arithmetic, written to out[3d .. 3d+2]
+__device__ void apicAffine(Grid g, MacView M, V3 p0, float *const *out, int q) {
+    const float h = (float)(0.5f * g.dx);
+    const float invdx = (float)(1.0f / g.dx);
+    for (int d = 0; d < 3; d++) {
+        const FaceArr F = d == 0 ? FaceArr{M.u, g.ni + 1, g.nj, g.nk}
+                                 : (d == 1 ? FaceArr{M.v, g.ni, g.nj + 1, g.nk} : FaceArr{M.w, g.ni, g.nj, g.nk + 1});
+        const V3 off = d == 0 ? V3{0.0f, h, h} : (d == 1 ? V3{h, 0.0f, h} : V3{h, h, 0.0f});
+        const V3 x = sub(p0, off);
+        const I3 c = posToIndex(x, g.dx);
+        const V3 ip = divide3(sub(x, indexToPos(c.i, c.j, c.k, g.dx)), (float)g.dx);
+        const float mx = 1.0f - ip.x, my = 1.0f - ip.y, mz = 1.0f - ip.z, ninv = -invdx;
+        // the eight gradient weights with the reference's factor order
+        const V3 gw[8] = {V3{ninv * my * mz, ninv * mx * mz, ninv * mx * my},
+                          V3{invdx * my * mz, ip.x * ninv * mz, ip.x * my * ninv},
+                          V3{ninv * ip.y * mz, mx * invdx * mz, mx * ip.y * ninv},
+                          V3{invdx * ip.y * mz, ip.x * invdx * mz, ip.x * ip.y * ninv},
+                          V3{ninv * my * ip.z, mx * ninv * ip.z, mx * my * invdx},
+                          V3{invdx * my * ip.z, ip.x * ninv * ip.z, ip.x * my * invdx},
+                          V3{ninv * ip.y * ip.z, mx * invdx * ip.z, mx * ip.y * invdx},
+                          V3{invdx * ip.y * ip.z, ip.x * invdx * ip.z, ip.x * ip.y * invdx}};
+        V3 a{0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int o = 0; o < 8; o++) {
+            const int i = c.i + (o & 1), j = c.j + ((o >> 1) & 1), k = c.k + (o >> 2);
+            if (!inRange(i, j, k, F.wi, F.wj, F.wk)) continue;
+            const float fv = F.a[flat(i, j, k, F.wi, F.wj)];
+            a = add(a, V3{gw[o].x * fv, gw[o].y * fv, gw[o].z * fv});
+        }
+        out[3 * d][q] = a.x;
+        out[3 * d + 1][q] = a.y;
+        out[3 * d + 2][q] = a.z;
+    }
+}
+
 __global__ __launch_bounds__(256) void g2pAdvectKernel(Grid g, Particles p, MacView cur, MacView sav,
                                                        const float *sp, NearGrid N, float r1,
-                                                       float r2, double dt, double cfl, uint8_t *moved) {
+                                                       float r2, double dt, double cfl, uint8_t *moved,
+                                                       AffineArrays aff) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= p.n) return;
     V3 p0{p.px[q], p.py[q], p.pz[q]};
     V3 v0{p.vx[q], p.vy[q], p.vz[q]};
     V3 vPIC = sampleMac(g, cur, p0);
-    V3 vFLIP = sub(add(v0, vPIC), sampleMac(g, sav, p0));
-    V3 vn = add(scale(r1, vPIC), scale(r2, vFLIP));
+    V3 vn;
+    if (aff.a[0]) {
+        // APIC: the particle takes the sampled velocity and the field's gradients
+        vn = vPIC;
+        apicAffine(g, cur, p0, aff.a, q);
+    } else {
+        V3 vFLIP = sub(add(v0, vPIC), sampleMac(g, sav, p0));
+        vn = add(scale(r1, vPIC), scale(r2, vFLIP));
+    }
     p.vx[q] = vn.x;
     p.vy[q] = vn.y;
     p.vz[q] = vn.z;
@@ -805,6 +851,30 @@ void cellSort(Grid g, const Particles &p, CellSort &cs, int *scanScratch, hipStr
     }
 }
 
+namespace {
+__global__ void countKeysKernel(const int *keys, size_t n, int *slot, int *counts) {
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int key = keys[r];
+    // arrival slots are reordered by the segment sort below, so the order
+    // of these integer atomics does not reach the result
+    if (key >= 0) slot[r] = atomicAdd(&counts[key], 1);
+}
+}  // namespace
+
+void sortKeys(const int *keys, size_t n, size_t nb, int *slot, int *counts, int *offsets, int *perm,
+              int *scanScratch, hipStream_t s) {
+    hipCheck(hipMemsetAsync(counts, 0, sizeof(int) * (nb + 1), s), "hipMemsetAsync");
+    if (n > 0)
+        hipLaunchKernelGGL(countKeysKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, keys, n, slot, counts);
+    exclusiveScan(counts, offsets, nb, scanScratch, s);
+    if (n > 0) {
+        hipLaunchKernelGGL(scatterKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, keys, slot, offsets, (int)n,
+                           perm);
+        hipLaunchKernelGGL(segmentSortKernel, dim3(blocksFor(nb, TPB)), dim3(TPB), 0, s, offsets, nb, perm);
+    }
+}
+
 void liquidSDF(Grid g, const CellSort &cs, int np, double radius, float *phi, hipStream_t s) {
     int nbi = (g.ni + BW - 1) / BW, nbj = (g.nj + BW - 1) / BW, nbk = (g.nk + BW - 1) / BW;
     (void)np;
@@ -836,11 +906,12 @@ void maxSpeedSq(const Particles &p, double *partial, double *out, hipStream_t s)
 
 void g2pAdvect(Grid g, Particles p, MacView cur, MacView saved, const float *solidNodes,
                const uint8_t *nearSolid, int ci, int cj, int ck, float r1, float r2, double dt,
-               double cfl, hipStream_t s, uint8_t *moved) {
+               double cfl, hipStream_t s, uint8_t *moved, const AffineArrays *affine) {
     if (p.n == 0) return;
     NearGrid N{nearSolid, ci, cj, ck, 3 * g.dx};
+    const AffineArrays A = affine ? *affine : AffineArrays{};
     hipLaunchKernelGGL(g2pAdvectKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, g, p, cur, saved,
-                       solidNodes, N, r1, r2, dt, cfl, moved);
+                       solidNodes, N, r1, r2, dt, cfl, moved, A);
 }
 
 void speedHistogram(const Particles &p, double stepv, int nbins, int *counts, double *partial,
@@ -888,6 +959,22 @@ void removalFlags(Grid g, const Particles &p, const float *sp, int *keys, int *c
     }
     hipLaunchKernelGGL(removalFlagsKernel, dim3(blocksFor(p.n, TPB)), dim3(TPB), 0, s, p, keys, rank,
                        maxPerCell, extreme ? 1 : 0, maxSpeedSq, keep);
+}
+
+namespace {
+__global__ void compactArraysKernel(AffineArrays src, AffineArrays dst, int n, const int *keep, const int *rank) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n || !keep[q]) return;
+    const int r = rank[q];
+#pragma unroll
+    for (int e = 0; e < 9; e++) dst.a[e][r] = src.a[e][q];
+}
+}  // namespace
+
+void compactAffine(const AffineArrays &src, const AffineArrays &dst, int n, const int *keep, const int *rank,
+                   hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(compactArraysKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, src, dst, n, keep, rank);
 }
 
 void compactParticles(Particles &src, Particles &dst, const int *keep, int *rank, int *scanScratch,

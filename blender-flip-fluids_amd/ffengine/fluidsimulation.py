@@ -28,6 +28,10 @@ class FluidSimulationMarkerParticleData_t(ctypes.Structure):
     _fields_ = [("size", c_int), ("positions", c_char_p), ("velocities", c_char_p)]
 
 
+class FluidSimulationMarkerParticleAffineData_t(ctypes.Structure):
+    _fields_ = [("size", c_int), ("affinex", c_char_p), ("affiney", c_char_p), ("affinez", c_char_p)]
+
+
 class FluidSimulationMeshStats_t(ctypes.Structure):
     _fields_ = [("enabled", c_int), ("vertices", c_int), ("triangles", c_int), ("bytes", c_uint)]
 
@@ -184,6 +188,8 @@ class FluidSimulation(object):
                                         _setter("FluidSimulation_set_max_time_steps_per_frame", c_int))
     PICFLIP_ratio = property(lambda s: _call0(s, "FluidSimulation_get_PICFLIP_ratio", c_double),
                              _setter("FluidSimulation_set_PICFLIP_ratio", c_double))
+    PICAPIC_ratio = property(lambda s: _call0(s, "FluidSimulation_get_PICAPIC_ratio", c_double),
+                             _setter("FluidSimulation_set_PICAPIC_ratio", c_double))
 
     @property
     def enable_surface_reconstruction(self):
@@ -321,6 +327,29 @@ class FluidSimulation(object):
     def get_marker_particle_velocity_data_range(self, start_idx, end_idx):
         return self._get_output_data_range("FluidSimulation_get_marker_particle_velocity_data_range",
                                            start_idx, end_idx, 12)
+
+    def get_marker_particle_affinex_data_range(self, start_idx, end_idx):
+        return self._get_output_data_range("FluidSimulation_get_marker_particle_affinex_data_range",
+                                           start_idx, end_idx, 12)
+
+    def get_marker_particle_affiney_data_range(self, start_idx, end_idx):
+        return self._get_output_data_range("FluidSimulation_get_marker_particle_affiney_data_range",
+                                           start_idx, end_idx, 12)
+
+    def get_marker_particle_affinez_data_range(self, start_idx, end_idx):
+        return self._get_output_data_range("FluidSimulation_get_marker_particle_affinez_data_range",
+                                           start_idx, end_idx, 12)
+
+    def load_marker_particle_affine_data(self, num_particles, affinex_data, affiney_data, affinez_data):
+        bufs = [(c_char * len(d)).from_buffer_copy(d) for d in (affinex_data, affiney_data, affinez_data)]
+        adata = FluidSimulationMarkerParticleAffineData_t()
+        adata.size = c_int(num_particles)
+        adata.affinex = ctypes.cast(bufs[0], c_char_p)
+        adata.affiney = ctypes.cast(bufs[1], c_char_p)
+        adata.affinez = ctypes.cast(bufs[2], c_char_p)
+        f = lib.FluidSimulation_load_marker_particle_affine_data
+        pb.init_lib_func(f, [c_void_p, FluidSimulationMarkerParticleAffineData_t, c_void_p], None)
+        pb.execute_lib_func(f, [self(), adata])
 
     def get_frame_stats_data(self):
         f = lib.FluidSimulation_get_frame_stats_data

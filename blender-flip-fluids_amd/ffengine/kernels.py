@@ -240,3 +240,32 @@ def sample_velocity_position_vjp(n3, dx, uvw, pos, vbar):
     _run("FFAMD_sample_velocity_position_vjp", [c_int] * 3 + [c_double] + [c_void_p] * 3 + [c_int] + [c_void_p] * 3,
          [ni, nj, nk, dx, _fp(u), _fp(v), _fp(w), len(pos), _fp(pos), _fp(vb), _fp(xbar)])
     return xbar
+
+
+def p2g_apic(n3, dx, pos, vel, aff, radius):
+    """APIC P2G: ((u, v, w), (valid_u, valid_v, valid_w)); aff (P, 9) holds the
+    AFFINEX, AFFINEY and AFFINEZ vectors of each particle."""
+    ni, nj, nk = n3
+    pos = _f32(pos).reshape(-1, 3)
+    vel = _f32(vel).reshape(-1, 3)
+    aff = _f32(aff).reshape(-1, 9)
+    u = np.zeros((nk, nj, ni + 1), np.float32)
+    v = np.zeros((nk, nj + 1, ni), np.float32)
+    w = np.zeros((nk + 1, nj, ni), np.float32)
+    vu, vv, vw = (np.zeros(a.shape, np.uint8) for a in (u, v, w))
+    _run("FFAMD_p2g_apic", [c_int] * 3 + [c_double, c_int] + [c_void_p] * 3 + [c_double] + [c_void_p] * 6,
+         [ni, nj, nk, dx, len(pos), _fp(pos), _fp(vel), _fp(aff), radius, _fp(u), _fp(v), _fp(w), _fp(vu), _fp(vv),
+          _fp(vw)])
+    return (u, v, w), (vu, vv, vw)
+
+
+def g2p_apic(n3, dx, uvw, pos):
+    """APIC G2P: (velocities (P, 3), affine vectors (P, 9))."""
+    ni, nj, nk = n3
+    u, v, w = (_f32(a) for a in uvw)
+    pos = _f32(pos).reshape(-1, 3)
+    vel = np.zeros_like(pos)
+    aff = np.zeros((len(pos), 9), np.float32)
+    _run("FFAMD_g2p_apic", [c_int] * 3 + [c_double] + [c_void_p] * 3 + [c_int] + [c_void_p] * 3,
+         [ni, nj, nk, dx, _fp(u), _fp(v), _fp(w), len(pos), _fp(pos), _fp(vel), _fp(aff)])
+    return vel, aff

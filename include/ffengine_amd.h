@@ -39,6 +39,15 @@ typedef struct FluidSimulationMarkerParticleData {
     char *velocities;
 } FluidSimulationMarkerParticleData;
 
+/* FluidSimulationMarkerParticleAffineData (fluidsimulation.h:159-164): packed
+ * float xyz AFFINEX, AFFINEY and AFFINEZ vectors of one load batch. */
+typedef struct FluidSimulationMarkerParticleAffineData {
+    int size;
+    char *affineX;
+    char *affineY;
+    char *affineZ;
+} FluidSimulationMarkerParticleAffineData;
+
 /* FluidSimulationMeshStats / TimingStats / FrameStats (fluidsimulation.h:62-151),
  * same field order and types, returned by value. */
 typedef struct FFMeshStats {
@@ -110,16 +119,30 @@ void FluidSimulation_enable_extreme_velocity_removal(FluidSimulation *obj, int *
 void FluidSimulation_disable_extreme_velocity_removal(FluidSimulation *obj, int *err); /* :2959-2964 */
 int FluidSimulation_is_extreme_velocity_removal_enabled(FluidSimulation *obj, int *err); /* :2966-2971 */
 void FluidSimulation_set_velocity_transfer_method_FLIP(FluidSimulation *obj, int *err); /* :2973-2978 */
-void FluidSimulation_set_velocity_transfer_method_APIC(FluidSimulation *obj, int *err); /* :2980-2985; fails: APIC is not on this path yet */
+void FluidSimulation_set_velocity_transfer_method_APIC(FluidSimulation *obj, int *err); /* :2980-2985 */
 int FluidSimulation_is_velocity_transfer_method_FLIP(FluidSimulation *obj, int *err); /* :2987-2992 */
 int FluidSimulation_is_velocity_transfer_method_APIC(FluidSimulation *obj, int *err); /* :2994-2999 */
 double FluidSimulation_get_PICFLIP_ratio(FluidSimulation *obj, int *err);      /* :3001-3005 */
 void FluidSimulation_set_PICFLIP_ratio(FluidSimulation *obj, double ratio, int *err); /* :3007-3013 */
+double FluidSimulation_get_PICAPIC_ratio(FluidSimulation *obj, int *err);      /* :3014-3018 */
+void FluidSimulation_set_PICAPIC_ratio(FluidSimulation *obj, double ratio, int *err); /* :3020-3026 */
 
 /* ---- particle I/O ---- */
 void FluidSimulation_load_marker_particle_data(FluidSimulation *obj,
                                                FluidSimulationMarkerParticleData data,
                                                int *err);                      /* :4872-4878 */
+void FluidSimulation_load_marker_particle_affine_data(FluidSimulation *obj,
+                                                      FluidSimulationMarkerParticleAffineData data,
+                                                      int *err);               /* :4880-4886 */
+void FluidSimulation_get_marker_particle_affinex_data_range(FluidSimulation *obj, int start_idx,
+                                                            int end_idx, char *data,
+                                                            int *err);         /* :4647-4656 */
+void FluidSimulation_get_marker_particle_affiney_data_range(FluidSimulation *obj, int start_idx,
+                                                            int end_idx, char *data,
+                                                            int *err);         /* :4658-4667 */
+void FluidSimulation_get_marker_particle_affinez_data_range(FluidSimulation *obj, int start_idx,
+                                                            int end_idx, char *data,
+                                                            int *err);         /* :4669-4678 */
 int FluidSimulation_get_num_marker_particles(FluidSimulation *obj, int *err);  /* :3151-3156 */
 unsigned int FluidSimulation_get_marker_particle_position_data_size(FluidSimulation *obj,
                                                                    int *err); /* :3952-3956 */
@@ -301,6 +324,18 @@ void FFAMD_mic0_band_shape(int fp64, int *levels, int *tile, int *err);
 void FFAMD_sample_velocity(int isize, int jsize, int ksize, double dx, const float *u,
                            const float *v, const float *w, int np, const float *pos, float *out,
                            int *err);
+
+/* Kernel-level APIC transfers for parity checks (the engine runs them inside
+ * update()): FFAMD_p2g_apic replaces VelocityAdvector::advect with the APIC
+ * producer (velocityadvector.cpp:38-465, 543-623) and FFAMD_g2p_apic
+ * _updatePICAPICMarkerParticleVelocitiesThread (fluidsimulation.cpp:6709-6769,
+ * 6791-6843).  aff: np x 9 (AFFINEX, AFFINEY, AFFINEZ per particle). */
+void FFAMD_p2g_apic(int isize, int jsize, int ksize, double dx, int np, const float *pos,
+                    const float *vel, const float *aff, double radius, float *u, float *v,
+                    float *w, unsigned char *valid_u, unsigned char *valid_v,
+                    unsigned char *valid_w, int *err);
+void FFAMD_g2p_apic(int isize, int jsize, int ksize, double dx, const float *u, const float *v,
+                    const float *w, int np, const float *pos, float *vel, float *aff, int *err);
 
 /* Position cotangent of trilinear sampling (no counterpart in the reference):
  * xbar (np x 3) = sum_d vbar_d grad V_d(pos) for the face grids u/v/w, inside
