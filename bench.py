@@ -37,9 +37,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_sim(n, pos, solid, mode, tol):
+def make_sim(n, pos, solid, mode, tol, apic=False):
     from ffengine import FluidSimulation
     sim = FluidSimulation(n, n, n, 1.0 / n)
+    if apic:
+        sim.set_velocity_transfer_method_APIC()
     sim.add_body_force(0.0, -9.81, 0.0)
     sim.PICFLIP_ratio = 0.05
     sim.CFL_condition_number = 5
@@ -78,9 +80,9 @@ def enable_timing(sim, on):
     pb.execute_lib_func(f, [sim(), 1 if on else 0])
 
 
-def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt):
+def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt, apic=False):
     import torch
-    sim = make_sim(n, pos, solid, mode, tol)
+    sim = make_sim(n, pos, solid, mode, tol, apic)
     for s in range(warmup):
         sim.update(frame_dt)
         log("  warmup %s %d/%d" % (mode, s + 1, warmup))
@@ -399,6 +401,8 @@ def main():
     ap.add_argument("--fd-faces", type=int, default=64, help="adjoint: faces checked by central differences")
     ap.add_argument("--fd-eps", type=float, default=1e-3, help="adjoint: central-difference step (m/s)")
     ap.add_argument("--dt", type=float, default=1.0 / 240.0, help="adjoint: substep length (s)")
+    ap.add_argument("--transfer", default="flip", choices=["flip", "apic"],
+                    help="dam-break: velocity transfer method (the headline uses FLIP)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
@@ -453,7 +457,8 @@ def main():
     results = {}
     for mode in modes:
         log("mode %s" % mode)
-        results[mode] = run_mode(n, pos, solid, mode, args.tol, args.steps, args.warmup, frame_dt)
+        results[mode] = run_mode(n, pos, solid, mode, args.tol, args.steps, args.warmup, frame_dt,
+                                 args.transfer == "apic")
     # headline: the faster mixed-precision mode, else FP64
     mixed_modes = [m for m in ("mixed-reliable", "mixed", "fp32-preconditioner") if m in results]
     rate = lambda m: results[m]["substeps"] / results[m]["elapsed_s"]  # noqa: E731
@@ -497,8 +502,11 @@ def main():
         "dtype": "f32+f64" if main_mode != "fp64" else "f64",
         "data": "synthetic",
         "config": {"workload": "dam break %d^3 + static sphere obstacle, %d particles, frame 1/240 s, "
-                               "PCG rel res %g" % (n, len(pos), args.tol), "grid": n, "particles": len(pos)},
+                               "PCG rel res %g%s" % (n, len(pos), args.tol,
+                                                     ", APIC transfer" if args.transfer == "apic" else ""),
+                   "grid": n, "particles": len(pos)},
         "mode": main_mode,
+        "transfer": args.transfer,
         "substeps_per_s": {m: round(rate(m), 4) for m in results},
         "fp64_substeps_per_s": round(fp64_rate, 4) if fp64_rate else None,
         "mixed_over_fp64": round(value / fp64_rate, 4) if (fp64_rate and mixed) else None,
