@@ -155,8 +155,8 @@ def test_error_paths():
         sim.PICFLIP_ratio = 1.5
     with pytest.raises(RuntimeError, match="CFL"):
         sim.CFL_condition_number = 0
-    with pytest.raises(RuntimeError, match="APIC"):
-        sim.set_velocity_transfer_method_APIC()
+    with pytest.raises(RuntimeError, match="PICAPIC ratio"):
+        sim.PICAPIC_ratio = -0.1
     pos = scenes.dam_break_particles(n, seed=1)
     sim.load_marker_particle_data(len(pos), pos.tobytes(), np.zeros_like(pos).tobytes())
     sim.initialize()
