@@ -37,11 +37,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_sim(n, pos, solid, mode, tol, apic=False):
+def make_sim(n, pos, solid, mode, tol, apic=False, warm=False):
     from ffengine import FluidSimulation
     sim = FluidSimulation(n, n, n, 1.0 / n)
     if apic:
         sim.set_velocity_transfer_method_APIC()
+    if warm:
+        sim.set_pressure_solver_warm_start(True)
     sim.add_body_force(0.0, -9.81, 0.0)
     sim.PICFLIP_ratio = 0.05
     sim.CFL_condition_number = 5
@@ -80,9 +82,9 @@ def enable_timing(sim, on):
     pb.execute_lib_func(f, [sim(), 1 if on else 0])
 
 
-def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt, apic=False):
+def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt, apic=False, warm=False):
     import torch
-    sim = make_sim(n, pos, solid, mode, tol, apic)
+    sim = make_sim(n, pos, solid, mode, tol, apic, warm)
     for s in range(warmup):
         sim.update(frame_dt)
         log("  warmup %s %d/%d" % (mode, s + 1, warmup))
@@ -403,6 +405,11 @@ def main():
     ap.add_argument("--dt", type=float, default=1.0 / 240.0, help="adjoint: substep length (s)")
     ap.add_argument("--transfer", default="flip", choices=["flip", "apic"],
                     help="dam-break: velocity transfer method (the headline uses FLIP)")
+    ap.add_argument("--no-warm-extra", action="store_true",
+                    help="dam-break: skip the warm-started runs reported beside the headline")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="dam-break: FP64 PCG modes start from the previous substep's pressure (the headline "
+                         "starts from 0, as the reference does)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
@@ -458,7 +465,7 @@ def main():
     for mode in modes:
         log("mode %s" % mode)
         results[mode] = run_mode(n, pos, solid, mode, args.tol, args.steps, args.warmup, frame_dt,
-                                 args.transfer == "apic")
+                                 args.transfer == "apic", args.warm_start)
     # headline: the faster mixed-precision mode, else FP64
     mixed_modes = [m for m in ("mixed-reliable", "mixed", "fp32-preconditioner") if m in results]
     rate = lambda m: results[m]["substeps"] / results[m]["elapsed_s"]  # noqa: E731
@@ -494,6 +501,15 @@ def main():
         log("cpu baseline n=%d threads=%d" % (args.cpu_sample_n, args.cpu_threads))
         cpu = cpu_baseline(args.cpu_sample_n, args.cpu_threads)
 
+    # the same two solvers started from the previous substep's pressure (an
+    # option of this engine; the reference's PCG starts from 0): reported
+    # beside the headline, never as it
+    warm = {}
+    if not args.warm_start and not args.no_warm_extra and args.transfer == "flip":
+        for mode in dict.fromkeys(["fp64", main_mode]):
+            log("warm-started %s" % mode)
+            r = run_mode(n, pos, solid, mode, args.tol, args.steps, args.warmup, frame_dt, False, True)
+            warm[mode] = {"substeps_per_s": round(r["substeps"] / r["elapsed_s"], 4), "pressure_iterations": r["iterations"]}
     line = {
         "metric": "FLIP substeps/s at 256^3, mixed vs FP64 PCG at rel res 1e-8; % HBM peak",
         "value": round(value, 4), "unit": "substeps/s", "n_gpus": 1, "steps": args.steps,
@@ -507,6 +523,8 @@ def main():
                    "grid": n, "particles": len(pos)},
         "mode": main_mode,
         "transfer": args.transfer,
+        "warm_start": bool(args.warm_start),
+        "warm_started_pcg": warm or None,
         "substeps_per_s": {m: round(rate(m), 4) for m in results},
         "fp64_substeps_per_s": round(fp64_rate, 4) if fp64_rate else None,
         "mixed_over_fp64": round(value / fp64_rate, 4) if (fp64_rate and mixed) else None,

@@ -260,6 +260,12 @@ void FluidSimulation_set_pressure_solver_reference_summation(FluidSimulation *o,
 int FluidSimulation_get_pressure_solver_reference_summation(FluidSimulation *o, int *err) {
     return guarded(err, [&] { return S(o)->getPressureSolverReferenceSummation() ? 1 : 0; });
 }
+void FluidSimulation_set_pressure_solver_warm_start(FluidSimulation *o, int on, int *err) {
+    guarded(err, [&] { S(o)->setPressureSolverWarmStart(on != 0); });
+}
+int FluidSimulation_get_pressure_solver_warm_start(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getPressureSolverWarmStart() ? 1 : 0; });
+}
 void FluidSimulation_set_solid_phi(FluidSimulation *o, const float *phi, int *err) {
     guarded(err, [&] { S(o)->setSolidPhi(phi); });
 }

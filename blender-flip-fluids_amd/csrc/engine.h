@@ -98,6 +98,12 @@ public:
     // FP64 solve: dot products in the reference's order for the thread count
     void setPressureSolverReferenceSummation(bool on) { _referenceSummation = on; }
     bool getPressureSolverReferenceSummation() const { return _referenceSummation; }
+    // FP64 PCG modes: start each solve from the previous substep's pressure
+    void setPressureSolverWarmStart(bool on) {
+        _warmStart = on;
+        _ps.setWarmStart(on);
+    }
+    bool getPressureSolverWarmStart() const { return _warmStart; }
     void setSolidPhi(const float *phiNodes);
 
     // particle I/O (fluidsimulation.cpp:3426-3839)
@@ -149,6 +155,7 @@ private:
     PressurePrecision _precision = PressurePrecision::FP64;
     double _innerTolerance = 1e-4;
     bool _referenceSummation = false;
+    bool _warmStart = false;
     int _maxThreads = 1;
     double _density = 20.0;
     bool _extremeRemoval = true;

@@ -300,6 +300,8 @@ void sampleAdjoint(Grid g, int dir, int np, const float *px, const float *py, co
                    hipStream_t s);
 // rhs[r] = cells[rows[r]];  cells[rows[r]] = x[r] (other cells untouched)
 void gatherRows(const PressureSys &S, const double *cells, double *rhs, hipStream_t s);
+// x[r] = (double)cells[rows[r]] (a float cell grid onto the rows)
+void gatherRowsF(const PressureSys &S, const float *cells, double *x, hipStream_t s);
 void scatterRows(const PressureSys &S, const double *x, double *cells, hipStream_t s);
 
 // ---- MIC(0) application, persistent tile wavefront (kernels_mic.hip) ----

@@ -348,6 +348,10 @@ __global__ void gatherRowsKernel(int m, const int *rows, const double *cells, do
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < m) rhs[r] = cells[rows[r]];
 }
+__global__ void gatherRowsFKernel(int m, const int *rows, const float *cells, double *x) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m) x[r] = (double)cells[rows[r]];
+}
 __global__ void scatterRowsKernel(int m, const int *rows, const double *x, double *cells) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < m) cells[rows[r]] = x[r];
@@ -518,6 +522,10 @@ void pressureToGrid(Grid g, const PressureSys &S, const double *x, float *p, hip
 void gatherRows(const PressureSys &S, const double *cells, double *rhs, hipStream_t s) {
     if (S.m == 0) return;
     hipLaunchKernelGGL(gatherRowsKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.rows, cells, rhs);
+}
+void gatherRowsF(const PressureSys &S, const float *cells, double *x, hipStream_t s) {
+    if (S.m == 0) return;
+    hipLaunchKernelGGL(gatherRowsFKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.rows, cells, x);
 }
 void scatterRows(const PressureSys &S, const double *x, double *cells, hipStream_t s) {
     if (S.m == 0) return;

@@ -46,6 +46,10 @@ public:
                          hipEvent_t afterAssembly, KernelTimers *timers = nullptr, int referenceDots = 0,
                          const double *cellRhs = nullptr, double *cellSolution = nullptr);
     const PressureSys &system() const { return _S; }
+    // FP64 PCG (modes 0 and 3): start from the pressure grid's values at
+    // the rows (the previous solve's result) instead of 0.  The stopping
+    // test keeps the zero-start threshold, tolerance x |b|_inf.
+    void setWarmStart(bool on) { _warm = on; }
     const std::vector<int> &levelTable() const { return _hLevelStart; }
     // one preconditioner application z = (L L^T)^-1 r on the current factor
     // (after solve()); FP64 or the FP32 copy.  schedule 0 = one launch per
@@ -76,6 +80,10 @@ private:
     KernelTimers _noTimers;
     int _levelLaunches = 0, _bandLaunches = 0;
     int _refDots = 0;  // thread count of the reference's dot order, or 0 for the tree reductions
+    bool _warm = false;
+    const float *_prevPressure = nullptr;  // the pressure grid at solve() entry
+    // x = 0, r = b; or, warm, x from the pressure grid and r = b - A x
+    void initialGuess(hipStream_t st);
 
     double hostScalar(const double *d, hipStream_t s);
     void enqueueTileRanges(bool fp64, hipStream_t s);

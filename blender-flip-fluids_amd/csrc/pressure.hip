@@ -249,6 +249,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     }
     int iters = 0;
     double resid = -1.0;
+    _prevPressure = pressure;
     bool conv = mode == 0   ? pcgFP64(tol, maxIter, maxAbs, iters, resid, s)
                 : mode == 1 ? pcgMixed(tol, maxIter, maxAbs, innerTol, iters, resid, s)
                 : mode == 2 ? pcgReliable(tol, maxIter, maxAbs, innerTol, iters, resid, s)
@@ -265,13 +266,23 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
 
 // PCGSolver<double>::solve (pcgsolver.h:268-322): x0 = 0, MIC(0), float step
 // lengths, |r|_inf stop test against min(tol * |b|_inf, 1.0)
+void PressureSolver::initialGuess(hipStream_t st) {
+    const int m = _S.m;
+    if (_warm && _prevPressure) {
+        gatherRowsF(_S, _prevPressure, _x, st);
+        residual(_S, _S.rhs, _x, _r, st);
+    } else {
+        hipCheck(hipMemcpyAsync(_r, _S.rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
+        hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
+    }
+}
+
 bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &iters, double &resid,
                              hipStream_t st) {
     const int m = _S.m;
     const int *hs = _hLevelStart.data();
     if (tolFactor < 1e-30) tolFactor = 1e-30;
-    hipCheck(hipMemcpyAsync(_r, _S.rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
-    hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
+    initialGuess(st);
     resid = resid0;
     double tol = tolFactor * resid;
     _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
@@ -336,8 +347,7 @@ bool PressureSolver::pcgPrecond32(double tolFactor, int maxIter, double resid0, 
     const int m = _S.m;
     const int *hs = _hLevelStart.data();
     if (tolFactor < 1e-30) tolFactor = 1e-30;
-    hipCheck(hipMemcpyAsync(_r, _S.rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
-    hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
+    initialGuess(st);
     resid = resid0;
     double tol = tolFactor * resid;
     _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);

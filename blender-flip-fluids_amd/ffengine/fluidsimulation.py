@@ -264,6 +264,16 @@ class FluidSimulation(object):
         pb.init_lib_func(f, [c_void_p, c_int, c_void_p], None)
         pb.execute_lib_func(f, [self(), 1 if enabled else 0])
 
+    def set_pressure_solver_warm_start(self, enabled):
+        """FP64 PCG modes: start each solve from the previous substep's
+        pressure instead of 0 (same stopping threshold)."""
+        f = lib.FluidSimulation_set_pressure_solver_warm_start
+        pb.init_lib_func(f, [c_void_p, c_int, c_void_p], None)
+        pb.execute_lib_func(f, [self(), 1 if enabled else 0])
+
+    def get_pressure_solver_warm_start(self):
+        return bool(_call0(self, "FluidSimulation_get_pressure_solver_warm_start", c_int))
+
     def get_pressure_solver_reference_summation(self):
         return bool(_call0(self, "FluidSimulation_get_pressure_solver_reference_summation", c_int))
 

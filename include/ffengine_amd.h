@@ -194,6 +194,11 @@ int FluidSimulation_get_pressure_solver_precision(FluidSimulation *obj, int *err
 void FluidSimulation_set_pressure_solver_reference_summation(FluidSimulation *obj, int enabled,
                                                             int *err);
 int FluidSimulation_get_pressure_solver_reference_summation(FluidSimulation *obj, int *err);
+/* FP64 PCG (precision modes 0 and 3) started from the previous substep's
+ * pressure at the rows instead of 0 (no counterpart in the reference, whose
+ * PCG starts from 0); the stopping threshold stays tolerance x |b|_inf. */
+void FluidSimulation_set_pressure_solver_warm_start(FluidSimulation *obj, int enabled, int *err);
+int FluidSimulation_get_pressure_solver_warm_start(FluidSimulation *obj, int *err);
 
 /* Nodal solid SDF ((isize+1)*(jsize+1)*(ksize+1) floats, i fastest, negative
  * inside solids) replacing the domain-boundary SDF the engine builds itself.

@@ -166,3 +166,31 @@ def test_error_paths():
         sim.get_marker_particle_position_data_range(0, sim.get_num_marker_particles() + 1)
     # the non-range velocity getter returns positions, like the reference
     assert sim.get_marker_particle_velocity_data() == sim.get_marker_particle_position_data()
+
+
+def test_warm_start_meets_the_same_tolerance():
+    """A PCG warm-started from the previous substep's pressure stops at the
+    same threshold (tolerance x |b|_inf): every solve converges, within the
+    same iteration cap, and the particles stay within a small fraction of a
+    cell of the cold-started run over three frames."""
+    n = 32
+    pos = scenes.dam_break_particles(n, seed=31)
+    cold = make_gpu(n, pos)
+    warm = make_gpu(n, pos)
+    warm.set_pressure_solver_warm_start(True)
+    assert warm.get_pressure_solver_warm_start() and not cold.get_pressure_solver_warm_start()
+    ic, iw = [], []
+    for frame in range(3):
+        cold.update(1.0 / 30.0)
+        warm.update(1.0 / 30.0)
+        sc, sw = cold.get_frame_stats_data(), warm.get_frame_stats_data()
+        assert sc.pressure_solver_success and sw.pressure_solver_success, frame
+        ic.append(sc.pressure_solver_iterations)
+        iw.append(sw.pressure_solver_iterations)
+        pc, _ = gpu_particles(cold)
+        pw, _ = gpu_particles(warm)
+        assert pc.shape == pw.shape, frame
+        d = float(np.abs(pc - pw).max()) * n
+        say(f"warm start frame {frame}: iterations cold {ic[-1]} warm {iw[-1]}, max |dx| = {d:.3g} cells")
+        assert d < 1e-2, frame
+    assert sum(iw[1:]) < sum(ic[1:])
