@@ -10,8 +10,8 @@
 // particle's cell in that block's frame (computed with the reference's own
 // float arithmetic), the records are counting-sorted by key (ascending
 // particle index inside a key), and one thread per face merges its 8 cells'
-// lists in particle order.  The sums therefore run in the reference's order
-// and no float atomics are used.
+// lists in particle order (staged in LDS per quarter block).  The sums
+// therefore run in the reference's order and no float atomics are used.
 #include "kernels.h"
 
 namespace ff {
@@ -114,60 +114,143 @@ __global__ void apicKeysKernel(Grid g, int dir, int np, const float *px, const f
     for (; m < MAX_MEMBERS; m++) keys[(size_t)q * MAX_MEMBERS + m] = -1;
 }
 
-// the APIC producer's deposit of record r onto face (fi, fj, fk) of block b
-// (velocityadvector.cpp:559-611), with the face's sums carried by the caller
+// One workgroup per 10 x 5 x 5 quarter of a block: it stages the records of
+// the 11 x 6 x 6 cells its faces read (36 rows of 11 cells, each row one
+// contiguous run of the sorted records) in LDS with the data the deposit
+// needs, then one thread per face merges its 8 cells' lists in particle
+// order (velocityadvector.cpp:559-611).  A quarter whose records exceed LDS
+// reads them from global memory instead: same merge, same bits.
+constexpr int AT_J = 5, AT_K = 5;                          // quarter of a block in j and k
+constexpr int AT_FACES = BW * AT_J * AT_K;                 // 250
+constexpr int AT_THREADS = 256;
+constexpr int AT_ROWS = (AT_J + 1) * (AT_K + 1);           // 36 rows of cells ci = -1 .. BW-1
+constexpr int AT_CAP = 3968;                               // staged records: 8 arrays, 127 KB
+static_assert(AT_FACES <= AT_THREADS && AT_ROWS <= AT_THREADS, "one face and one row per thread");
+
 template <int dir>
-__global__ void apicFaceKernel(Grid g, const float *px, const float *py, const float *pz, const float *vel,
-                               const float *ax, const float *ay, const float *az, const uint8_t *active,
-                               const int *offsets, const int *perm, float *field, uint8_t *valid) {
+__global__ __launch_bounds__(AT_THREADS) void apicTileKernel(Grid g, const float *px, const float *py, const float *pz,
+                                                             const float *vel, const float *ax, const float *ay,
+                                                             const float *az, const uint8_t *active,
+                                                             const int *offsets, const int *perm, float *field,
+                                                             uint8_t *valid) {
+    __shared__ int sRowStart[AT_ROWS + 1], sRowGlob[AT_ROWS];
+    __shared__ int sQ[AT_CAP];
+    __shared__ float sX[AT_CAP], sY[AT_CAP], sZ[AT_CAP], sV[AT_CAP], sAx[AT_CAP], sAy[AT_CAP], sAz[AT_CAP];
     const FaceDims D = faceDims(g, dir);
-    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= (size_t)D.wi * D.wj * D.wk) return;
-    const int fi = (int)(f % D.wi), fj = (int)((f / D.wi) % D.wj), fk = (int)(f / ((size_t)D.wi * D.wj));
-    const int bi = fi / BW, bj = fj / BW, bk = fk / BW;
-    const int li = fi - bi * BW, lj = fj - bj * BW, lk = fk - bk * BW;
-    const size_t bid = flat(bi, bj, bk, D.bi, D.bj);
+    const int t = threadIdx.x;
+    const int blk = blockIdx.x / 4, quarter = blockIdx.x % 4;
+    const int bi = blk % D.bi, bj = (blk / D.bi) % D.bj, bk = blk / (D.bi * D.bj);
+    const int lj0 = (quarter & 1) * AT_J, lk0 = (quarter >> 1) * AT_K;
+    const size_t bid = (size_t)blk;
+    // this thread's face
+    const int li = t % BW, lj = lj0 + (t / BW) % AT_J, lk = lk0 + t / (BW * AT_J);
+    const int fi = bi * BW + li, fj = bj * BW + lj, fk = bk * BW + lk;
+    const bool mine = t < AT_FACES && fi < D.wi && fj < D.wj && fk < D.wk;
+    const size_t f = mine ? flat(fi, fj, fk, D.wi, D.wj) : 0;
+    if (!active[bid]) {  // workgroup-uniform: the block received no particles
+        if (mine) {
+            field[f] = 0.0f;
+            valid[f] = 0;
+        }
+        return;
+    }
+    const int rowKey0 = (int)(bid * CELLS_PER_BLOCK);
+    auto rowKey = [&](int cj, int ck) { return rowKey0 + LC * ((cj + 1) + LC * (ck + 1)); };  // cell ci = -1
+    int cnt = 0;
+    if (t < AT_ROWS) {
+        const int k0 = rowKey(lj0 - 1 + t % (AT_J + 1), lk0 - 1 + t / (AT_J + 1));
+        const int b = offsets[k0];
+        sRowGlob[t] = b;
+        cnt = offsets[k0 + BW + 1] - b;  // the row's 11 cells are contiguous keys
+        sRowStart[t + 1] = cnt;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        sRowStart[0] = 0;
+        for (int r = 0; r < AT_ROWS; r++) {
+            acc += sRowStart[r + 1];
+            sRowStart[r + 1] = acc;
+        }
+    }
+    __syncthreads();
+    const int total = sRowStart[AT_ROWS];
+    const bool staged = total <= AT_CAP;
+    const V3 off = dirOffset(g, dir);
+    const V3 bpos = indexToPos(bi, bj, bk, BW * g.dx);
+    if (staged) {
+        for (int q = t; q < total; q += AT_THREADS) {
+            int lo = 0, hi = AT_ROWS - 1;  // last row whose start is <= q
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sRowStart[mid] <= q) lo = mid;
+                else hi = mid - 1;
+            }
+            const int p = perm[sRowGlob[lo] + (q - sRowStart[lo])] / MAX_MEMBERS;
+            const V3 pl = sub(sub(V3{px[p], py[p], pz[p]}, off), bpos);
+            sQ[q] = p;
+            sX[q] = pl.x;
+            sY[q] = pl.y;
+            sZ[q] = pl.z;
+            sV[q] = vel[p];
+            sAx[q] = ax[p];
+            sAy[q] = ay[p];
+            sAz[q] = az[p];
+        }
+    }
+    __syncthreads();
+    if (!mine) return;
+    const V3 npos = indexToPos(li, lj, lk, g.dx);
+    // cell o = node - corner o; its list starts at the row's start plus the
+    // cells before it in the row
+    constexpr int NONE = 0x7fffffff;
+    int cur[8], end[8], head[8];
+#pragma unroll
+    for (int o = 0; o < 8; o++) {
+        const int ci = li - (o & 1), cj = lj - ((o >> 1) & 1), ck = lk - (o >> 2);
+        const int row = (cj - (lj0 - 1)) + (AT_J + 1) * (ck - (lk0 - 1));
+        const int key = rowKey(cj, ck) + (ci + 1);
+        const int b = offsets[key], e = offsets[key + 1];
+        cur[o] = staged ? sRowStart[row] + (b - sRowGlob[row]) : b;
+        end[o] = cur[o] + (e - b);
+        head[o] = cur[o] < end[o] ? (staged ? sQ[cur[o]] : perm[cur[o]] / MAX_MEMBERS) : NONE;
+    }
     float sum = 0.0f, wsum = 0.0f;
-    if (active[bid]) {
-        const V3 off = dirOffset(g, dir);
-        const V3 bpos = indexToPos(bi, bj, bk, BW * g.dx);
-        const V3 npos = indexToPos(li, lj, lk, g.dx);
-        // the 8 cells whose stencil holds this node: cell = node - corner
-        int cur[8], end[8];
+    while (true) {
+        int best = 0, pbest = head[0];
 #pragma unroll
-        for (int o = 0; o < 8; o++) {
-            const int ci = li - (o & 1), cj = lj - ((o >> 1) & 1), ck = lk - (o >> 2);
-            const int key = (int)(bid * CELLS_PER_BLOCK) + (ci + 1) + LC * ((cj + 1) + LC * (ck + 1));
-            cur[o] = offsets[key];
-            end[o] = offsets[key + 1];
+        for (int o = 1; o < 8; o++)
+            if (head[o] < pbest) {
+                pbest = head[o];
+                best = o;
+            }
+        if (pbest == NONE) break;
+        int s = 0;
+#pragma unroll
+        for (int o = 0; o < 8; o++)
+            if (o == best) {
+                s = cur[o];
+                cur[o] = s + 1;
+                head[o] = s + 1 < end[o] ? (staged ? sQ[s + 1] : perm[s + 1] / MAX_MEMBERS) : NONE;
+            }
+        V3 pl, a;
+        float v;
+        if (staged) {
+            pl = V3{sX[s], sY[s], sZ[s]};
+            a = V3{sAx[s], sAy[s], sAz[s]};
+            v = sV[s];
+        } else {
+            pl = sub(sub(V3{px[pbest], py[pbest], pz[pbest]}, off), bpos);
+            a = V3{ax[pbest], ay[pbest], az[pbest]};
+            v = vel[pbest];
         }
-        while (true) {
-            int best = -1, rbest = 0x7fffffff;
-#pragma unroll
-            for (int o = 0; o < 8; o++)
-                if (cur[o] < end[o]) {
-                    const int r = perm[cur[o]];
-                    if (r < rbest) {
-                        rbest = r;
-                        best = o;
-                    }
-                }
-            if (best < 0) break;
-#pragma unroll
-            for (int o = 0; o < 8; o++)
-                if (o == best) cur[o]++;
-            const int q = rbest / MAX_MEMBERS;
-            const V3 pl = sub(sub(V3{px[q], py[q], pz[q]}, off), bpos);
-            const I3 c = posToIndex(pl, g.dx);
-            const V3 ip = divide3(sub(pl, indexToPos(c.i, c.j, c.k, g.dx)), (float)g.dx);
-            const int oi = li - c.i, oj = lj - c.j, ok = lk - c.k;
-            const float wx = oi ? ip.x : 1.0f - ip.x, wy = oj ? ip.y : 1.0f - ip.y, wz = ok ? ip.z : 1.0f - ip.z;
-            const float weight = wx * wy * wz;
-            const V3 a{ax[q], ay[q], az[q]};
-            const float apicTerm = dot3(a, sub(npos, pl));
-            sum += weight * (vel[q] + apicTerm);
-            wsum += weight;
-        }
+        const int oi = best & 1, oj = (best >> 1) & 1, ok = best >> 2;
+        const V3 ip = divide3(sub(pl, indexToPos(li - oi, lj - oj, lk - ok, g.dx)), (float)g.dx);
+        const float wx = oi ? ip.x : 1.0f - ip.x, wy = oj ? ip.y : 1.0f - ip.y, wz = ok ? ip.z : 1.0f - ip.z;
+        const float weight = wx * wy * wz;
+        const float apicTerm = dot3(a, sub(npos, pl));
+        sum += weight * (v + apicTerm);
+        wsum += weight;
     }
     const float eps = 1e-6f;
     if (wsum > eps) sum /= wsum;
@@ -216,15 +299,17 @@ void p2gApic(Grid g, int np, const float *px, const float *py, const float *pz, 
                                radius, A.active, A.keys);
         sortKeys(A.keys, R, nb, A.slot, A.counts, A.offsets, A.perm, A.scan, s);
         const float *a3[3] = {aff[3 * dir], aff[3 * dir + 1], aff[3 * dir + 2]};
+        const dim3 grid((unsigned)(4 * nblk)), block(AT_THREADS);
+        (void)nf;
         if (dir == 0)
-            hipLaunchKernelGGL(apicFaceKernel<0>, dim3(blocksFor(nf, TPB)), dim3(TPB), 0, s, g, px, py, pz, vel[0],
-                               a3[0], a3[1], a3[2], A.active, A.offsets, A.perm, field[0], valid[0]);
+            hipLaunchKernelGGL(apicTileKernel<0>, grid, block, 0, s, g, px, py, pz, vel[0], a3[0], a3[1], a3[2],
+                               A.active, A.offsets, A.perm, field[0], valid[0]);
         else if (dir == 1)
-            hipLaunchKernelGGL(apicFaceKernel<1>, dim3(blocksFor(nf, TPB)), dim3(TPB), 0, s, g, px, py, pz, vel[1],
-                               a3[0], a3[1], a3[2], A.active, A.offsets, A.perm, field[1], valid[1]);
+            hipLaunchKernelGGL(apicTileKernel<1>, grid, block, 0, s, g, px, py, pz, vel[1], a3[0], a3[1], a3[2],
+                               A.active, A.offsets, A.perm, field[1], valid[1]);
         else
-            hipLaunchKernelGGL(apicFaceKernel<2>, dim3(blocksFor(nf, TPB)), dim3(TPB), 0, s, g, px, py, pz, vel[2],
-                               a3[0], a3[1], a3[2], A.active, A.offsets, A.perm, field[2], valid[2]);
+            hipLaunchKernelGGL(apicTileKernel<2>, grid, block, 0, s, g, px, py, pz, vel[2], a3[0], a3[1], a3[2],
+                               A.active, A.offsets, A.perm, field[2], valid[2]);
     }
 }
 
