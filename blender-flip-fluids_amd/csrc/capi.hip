@@ -5,6 +5,7 @@
 // (c_bindings/cbindings.h:48-155, cbindings.cpp:35-43).
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -331,12 +332,14 @@ void FFAMD_cell_sort(int ni, int nj, int nk, double dx, int np, const float *pos
 namespace {
 struct SortedParticles {
     ff::CellSort cs;
-    std::vector<DBuf<float> *> f;
-    std::vector<DBuf<int> *> i;
+    std::vector<std::unique_ptr<DBuf<float>>> f;
+    std::vector<std::unique_ptr<DBuf<int>>> i;
     SortedParticles(ff::Grid g, int np, const float *pos, const float *vel, hipStream_t s) {
         size_t N = (size_t)g.ni * g.nj * g.nk, P = (size_t)np;
-        for (int q = 0; q < 12; q++) f.push_back(new DBuf<float>(P));
-        for (int q = 0; q < 6; q++) i.push_back(new DBuf<int>(q == 2 || q == 3 ? N + 1 : (q == 5 ? ff::scanScratchInts(std::max(N + 1, P + 1)) : P)));
+        for (int q = 0; q < 12; q++) f.push_back(std::make_unique<DBuf<float>>(P));
+        for (int q = 0; q < 6; q++)
+            i.push_back(std::make_unique<DBuf<int>>(
+                q == 2 || q == 3 ? N + 1 : (q == 5 ? ff::scanScratchInts(std::max(N + 1, P + 1)) : P)));
         std::vector<float> c(P);
         for (int a = 0; a < 6; a++) {
             const float *src = a < 3 ? pos : vel;
@@ -347,10 +350,6 @@ struct SortedParticles {
         ff::Particles pp{*f[0], *f[1], *f[2], *f[3], *f[4], *f[5], np};
         cs = ff::CellSort{*i[0], *i[1], *i[2], *i[3], *i[4], *f[6], *f[7], *f[8], *f[9], *f[10], *f[11]};
         ff::cellSort(g, pp, cs, *i[5], s);
-    }
-    ~SortedParticles() {
-        for (auto *p : f) delete p;
-        for (auto *p : i) delete p;
     }
 };
 }  // namespace
@@ -474,10 +473,10 @@ void FFAMD_sample_velocity(int ni, int nj, int nk, double dx, const float *u, co
         ff::hipCheck(hipMemsetAsync(zu, 0, sizeof(float) * nu, sc.s), "memset");
         ff::hipCheck(hipMemsetAsync(zv, 0, sizeof(float) * nv, sc.s), "memset");
         ff::hipCheck(hipMemsetAsync(zw, 0, sizeof(float) * nw, sc.s), "memset");
-        std::vector<DBuf<float> *> pa;
+        std::vector<std::unique_ptr<DBuf<float>>> pa;
         std::vector<float> c(P);
         for (int a = 0; a < 6; a++) {
-            pa.push_back(new DBuf<float>(P));
+            pa.push_back(std::make_unique<DBuf<float>>(P));
             for (size_t q = 0; q < P; q++) c[q] = a < 3 ? pos[3 * q + a] : 0.0f;
             up(pa.back()->p, c.data(), P, sc.s);
             sync(sc.s);
@@ -502,7 +501,6 @@ void FFAMD_sample_velocity(int ni, int nj, int nk, double dx, const float *u, co
             out[3 * q + 1] = y[q];
             out[3 * q + 2] = z[q];
         }
-        for (auto *p : pa) delete p;
     });
 }
 
@@ -899,10 +897,10 @@ void FFAMD_p2g_apic(int ni, int nj, int nk, double dx, int np, const float *pos,
         Scratch sc;
         const size_t P = (size_t)np;
         HostSoA p(pos, P, sc.s), vl(vel, P, sc.s);
-        std::vector<DBuf<float> *> a;
+        std::vector<std::unique_ptr<DBuf<float>>> a;
         std::vector<float> c(P);
         for (int e = 0; e < 9; e++) {
-            a.push_back(new DBuf<float>(P));
+            a.push_back(std::make_unique<DBuf<float>>(P));
             for (size_t q = 0; q < P; q++) c[q] = aff[9 * q + e];
             up(a.back()->p, c.data(), P, sc.s);
             sync(sc.s);
@@ -923,7 +921,6 @@ void FFAMD_p2g_apic(int ni, int nj, int nk, double dx, int np, const float *pos,
         down(u, f0.p, nf[0], sc.s); down(v, f1.p, nf[1], sc.s); down(w, f2.p, nf[2], sc.s);
         down(vu, b0.p, nf[0], sc.s); down(vv, b1.p, nf[1], sc.s); down(vw, b2.p, nf[2], sc.s);
         sync(sc.s);
-        for (auto *x : a) delete x;
     });
 }
 
@@ -938,17 +935,17 @@ void FFAMD_g2p_apic(int ni, int nj, int nk, double dx, const float *u, const flo
         size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
         DBuf<float> du(nu), dv(nv), dw(nw);
         up(du.p, u, nu, sc.s); up(dv.p, v, nv, sc.s); up(dw.p, w, nw, sc.s);
-        std::vector<DBuf<float> *> pa, aa;
+        std::vector<std::unique_ptr<DBuf<float>>> pa, aa;
         std::vector<float> c(P);
         for (int a = 0; a < 6; a++) {
-            pa.push_back(new DBuf<float>(P));
+            pa.push_back(std::make_unique<DBuf<float>>(P));
             for (size_t q = 0; q < P; q++) c[q] = a < 3 ? pos[3 * q + a] : 0.0f;
             up(pa.back()->p, c.data(), P, sc.s);
             sync(sc.s);
         }
         ff::AffineArrays A;
         for (int e = 0; e < 9; e++) {
-            aa.push_back(new DBuf<float>(P));
+            aa.push_back(std::make_unique<DBuf<float>>(P));
             A.a[e] = *aa.back();
         }
         size_t NN = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
@@ -973,8 +970,6 @@ void FFAMD_g2p_apic(int ni, int nj, int nk, double dx, const float *u, const flo
             sync(sc.s);
             for (size_t q = 0; q < P; q++) aff[9 * q + e] = x[q];
         }
-        for (auto *b : pa) delete b;
-        for (auto *b : aa) delete b;
     });
 }
 
