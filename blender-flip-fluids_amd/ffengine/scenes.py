@@ -41,6 +41,38 @@ def sphere_phi_nodes(n, center=(0.6, 0.2, 0.5), radius=0.15):
     return phi
 
 
+def sphere_mesh(center=(0.6, 0.2, 0.5), radius=0.15, subdivisions=4):
+    """Closed triangle mesh of a sphere: an icosahedron subdivided
+    `subdivisions` times (20 * 4**subdivisions triangles) with its vertices
+    projected onto the sphere.  Returns (vertices (V, 3) float32, triangles
+    (T, 3) int32)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+             (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    tris = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+            (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+            (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in mid:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                mid[key] = len(verts) - 1
+            return mid[key]
+
+        nxt = []
+        for a, b, c in tris:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nxt += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        tris = nxt
+    v = np.array(verts) * radius + np.asarray(center, np.float64)
+    return v.astype(np.float32), np.array(tris, np.int32)
+
+
 def union_phi(*phis):
     """Union of solids: the pointwise minimum of their signed distances."""
     out = np.array(phis[0], np.float32, copy=True)

@@ -23,6 +23,7 @@
 #include "fforacle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -284,6 +285,468 @@ void oracle_solid_box_phi(int ni, int nj, int nk, double dx, float *phi) {
                 float v = in ? -phi[n] : phi[n];
                 phi[n] = -v;
             }
+}
+
+// ------------------------------------------------- mesh obstacle SDF ----
+namespace {
+
+struct TriMesh {
+    std::vector<V3> v;
+    std::vector<std::array<int, 3>> t;
+};
+
+// AABB(std::vector<vec3>) and AABB(Triangle, vertices): min/max in double,
+// widths padded by 1e-9 (aabb.cpp:54-112)
+Box aabbOf(const V3 *pts, int n) {
+    double minx = pts[0].x, miny = pts[0].y, minz = pts[0].z;
+    double maxx = pts[0].x, maxy = pts[0].y, maxz = pts[0].z;
+    for (int q = 0; q < n; q++) {
+        minx = std::fmin(pts[q].x, minx);
+        miny = std::fmin(pts[q].y, miny);
+        minz = std::fmin(pts[q].z, minz);
+        maxx = std::fmax(pts[q].x, maxx);
+        maxy = std::fmax(pts[q].y, maxy);
+        maxz = std::fmax(pts[q].z, maxz);
+    }
+    const double eps = 1e-9;
+    return Box{mk(minx, miny, minz), maxx - minx + eps, maxy - miny + eps, maxz - minz + eps};
+}
+Box aabbTri(const TriMesh &m, int t) {
+    V3 p[3] = {m.v[m.t[t][0]], m.v[m.t[t][1]], m.v[m.t[t][2]]};
+    return aabbOf(p, 3);
+}
+
+void translate(TriMesh &m, V3 d) {  // TriangleMesh::translate (trianglemesh.cpp:869-873)
+    for (V3 &p : m.v) p = p + d;
+}
+
+V3 cross(V3 a, V3 b) {  // vmath::cross (vmath.h:79-83)
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Collision::lineIntersectsTriangle (collision.cpp:79-118): float
+// Moller-Trumbore; the hit point is p + (float)t * dir.
+bool lineIntersectsTriangle(V3 p, V3 dir, V3 v0, V3 v1, V3 v2, V3 *hit) {
+    V3 e1 = v1 - v0, e2 = v2 - v0;
+    V3 h = cross(dir, e2);
+    float a = dot(e1, h);
+    double eps = 10e-9;
+    if (std::fabs(a) < eps) return false;
+    float f = 1 / a;
+    V3 s = p - v0;
+    float u = f * dot(s, h);
+    if (u < 0.0 || u > 1.0) return false;
+    V3 q = cross(s, e1);
+    float v = f * dot(dir, q);
+    if (v < 0.0 || u + v > 1.0) return false;
+    double t = f * dot(e2, q);
+    *hit = p + (float)t * dir;
+    return true;
+}
+
+// the three draws of one jitter vector: the reference builds
+// vec3(_randomDouble(jit,-jit), x3) and its compiler evaluates the
+// arguments right to left, so z takes the first draw (meshutils.cpp:93-117)
+struct RandSource {
+    const int *r;
+    int n, used;
+    double randMax;
+    double next(double lo, double hi) {
+        if (used >= n) throw std::runtime_error("oracle: rand() values exhausted");
+        return lo + ((double)r[used++] / randMax) * (hi - lo);
+    }
+    V3 jitter(double dx) {
+        double jit = 0.001 * dx;
+        double z = next(jit, -jit), y = next(jit, -jit), x = next(jit, -jit);
+        return mk(x, y, z);
+    }
+};
+
+// MeshUtils::getCellsInsideTriangleMesh (meshutils.cpp:57-224): per column
+// (i, j) the z of every crossing of the +z line through the jittered cell
+// centre (i, j, -1); a cell is inside when the column has an even number of
+// crossings and an odd number of them lie below its centre.
+void cellsInside(const TriMesh &m, int isz, int jsz, int ksz, double dx, RandSource &rs,
+                 std::vector<GI> &cells) {
+    std::vector<std::vector<int>> col((size_t)isz * jsz);
+    for (int t = 0; t < (int)m.t.size(); t++) {
+        Box b = aabbTri(m, t);
+        // Grid3d::getGridIndexBounds(AABB, ...) (grid3d.h:423-436)
+        V3 trans = mk(b.w, b.h, b.d);
+        GI g1 = posToIndex(b.pos, dx), g2 = posToIndex(b.pos + trans, dx);
+        g1.i = std::max(g1.i, 0);
+        g1.j = std::max(g1.j, 0);
+        g2.i = std::min(g2.i, isz - 1);
+        g2.j = std::min(g2.j, jsz - 1);
+        for (int j = g1.j; j <= g2.j; j++)
+            for (int i = g1.i; i <= g2.i; i++) col[(size_t)i + (size_t)isz * j].push_back(t);
+    }
+    V3 jitter = rs.jitter(dx);
+    V3 dir{0.0f, 0.0f, 1.0f};
+    std::vector<std::vector<float>> zs((size_t)isz * jsz);
+    for (int j = 0; j < jsz; j++)
+        for (int i = 0; i < isz; i++) {
+            const std::vector<int> &tl = col[(size_t)i + (size_t)isz * j];
+            if (tl.empty()) continue;
+            V3 gp = indexToCellCenter(i, j, -1, dx) + jitter;
+            for (int t : tl) {
+                V3 hit;
+                if (lineIntersectsTriangle(gp, dir, m.v[m.t[t][0]], m.v[m.t[t][1]], m.v[m.t[t][2]], &hit))
+                    zs[(size_t)i + (size_t)isz * j].push_back(hit.z);
+            }
+        }
+    for (int k = 0; k < ksz; k++)
+        for (int j = 0; j < jsz; j++)
+            for (int i = 0; i < isz; i++) {
+                const std::vector<float> &z = zs[(size_t)i + (size_t)isz * j];
+                if (z.size() % 2 != 0) continue;
+                float zc = indexToCellCenter(i, j, k, dx).z;
+                int less = 0;
+                for (float q : z) less += q < zc;
+                if (less % 2 == 1) cells.push_back(GI{i, j, k});
+            }
+}
+
+// MeshUtils::getGridNodesInsideTriangleMesh, list form (meshutils.cpp:356-380):
+// nodes are the cells of the grid shifted by half a cell.
+void nodesInside(TriMesh m, double dx, RandSource &rs, std::vector<GI> &nodes) {
+    if (m.v.empty()) {
+        // AABB of no points has zero extent (aabb.h:60-63): a 1x1x1 grid with
+        // no triangles, which still draws its jitter and holds no inside cell
+        rs.jitter(dx);
+        return;
+    }
+    translate(m, mk(0.5 * dx, 0.5 * dx, 0.5 * dx));
+    Box b = aabbOf(m.v.data(), (int)m.v.size());
+    GI go = posToIndex(b.pos, dx);
+    V3 off = indexToPos(go.i, go.j, go.k, dx);
+    translate(m, V3{-off.x, -off.y, -off.z});
+    b.pos = b.pos - off;
+    V3 trans = mk(b.w, b.h, b.d);
+    GI g2 = posToIndex(b.pos + trans, dx);
+    std::vector<GI> cells;
+    cellsInside(m, g2.i + 1, g2.j + 1, g2.k + 1, dx, rs, cells);
+    for (GI c : cells) nodes.push_back(GI{c.i + go.i, c.j + go.j, c.k + go.k});
+}
+
+// MeshUtils::_splitIntoMeshIslands (meshutils.cpp:464-573): connected
+// components over shared vertices; island vertices in first-use order.
+std::vector<TriMesh> splitIslands(const TriMesh &m) {
+    int nv = (int)m.v.size();
+    std::vector<std::vector<int>> nb(nv);
+    for (auto &t : m.t) {
+        nb[t[0]].push_back(t[1]); nb[t[0]].push_back(t[2]);
+        nb[t[1]].push_back(t[0]); nb[t[1]].push_back(t[2]);
+        nb[t[2]].push_back(t[0]); nb[t[2]].push_back(t[1]);
+    }
+    std::vector<int> group(nv, -1), stack;
+    std::vector<char> seen(nv, 0);
+    int ng = 0;
+    for (int s = 0; s < nv; s++) {
+        if (seen[s]) continue;
+        stack.assign(1, s);
+        seen[s] = 1;
+        while (!stack.empty()) {
+            int v = stack.back();
+            stack.pop_back();
+            for (int w : nb[v])
+                if (!seen[w]) {
+                    stack.push_back(w);
+                    seen[w] = 1;
+                }
+            group[v] = ng;
+        }
+        ng++;
+    }
+    int maxg = 0;
+    for (int g : group) maxg = std::max(maxg, g);
+    std::vector<TriMesh> isl(nv > 0 ? maxg + 1 : 0);
+    std::vector<int> tr(nv, -1);
+    for (auto t : m.t) {
+        int g = group[t[0]];
+        for (int c = 0; c < 3; c++)
+            if (tr[t[c]] == -1) {
+                tr[t[c]] = (int)isl[g].v.size();
+                isl[g].v.push_back(m.v[t[c]]);
+            }
+        isl[g].t.push_back(t);
+    }
+    for (auto &I : isl)
+        for (auto &t : I.t)
+            for (int c = 0; c < 3; c++) t[c] = tr[t[c]];
+    return isl;
+}
+
+// AABB::getIntersection with a non-empty result (aabb.cpp:170-191)
+bool aabbIntersects(const Box &a, const Box &b) {
+    V3 mn1 = a.pos, mx1 = a.maxPoint(), mn2 = b.pos, mx2 = b.maxPoint();
+    if (mn1.x > mx2.x || mn1.y > mx2.y || mn1.z > mx2.z || mx1.x < mn2.x || mx1.y < mn2.y || mx1.z < mn2.z)
+        return false;
+    double w = std::fmin(mx1.x, mx2.x) - std::fmax(mn1.x, mn2.x);
+    double h = std::fmin(mx1.y, mx2.y) - std::fmax(mn1.y, mn2.y);
+    double d = std::fmin(mx1.z, mx2.z) - std::fmax(mn1.z, mn2.z);
+    return w > 0.0 || h > 0.0 || d > 0.0;
+}
+bool aabbContains(const Box &grid, const Box &b) { return grid.inside(b.pos) && grid.inside(b.maxPoint()); }
+
+// MeshUtils::getGridNodesInsideTriangleMesh, grid form (meshutils.cpp:382-437):
+// a mesh that leaves the grid is split into islands; those inside are
+// tested together, each one crossing the grid border on its own.
+void nodesInsideGrid(const TriMesh &m, int pi, int pj, int pk, double dx, RandSource &rs,
+                     std::vector<char> &inside) {
+    int isz = pi - 1, jsz = pj - 1, ksz = pk - 1;
+    Box grid{V3{0.0f, 0.0f, 0.0f}, isz * dx, jsz * dx, ksz * dx};
+    bool contained = true;
+    for (V3 p : m.v)
+        if (!grid.inside(p)) {
+            contained = false;
+            break;
+        }
+    std::vector<TriMesh> groups;
+    if (contained) {
+        groups.push_back(m);
+    } else {
+        TriMesh in;
+        std::vector<TriMesh> out;
+        for (TriMesh &I : splitIslands(m)) {
+            Box b = aabbOf(I.v.data(), (int)I.v.size());
+            if (aabbContains(grid, b)) {
+                int base = (int)in.v.size();
+                in.v.insert(in.v.end(), I.v.begin(), I.v.end());
+                for (auto t : I.t) in.t.push_back({t[0] + base, t[1] + base, t[2] + base});
+            } else if (aabbIntersects(grid, b)) {
+                out.push_back(I);
+            }
+        }
+        groups.push_back(in);
+        for (TriMesh &o : out) groups.push_back(o);
+    }
+    for (TriMesh &g : groups) {
+        std::vector<GI> nodes;
+        nodesInside(g, dx, rs, nodes);
+        for (GI n : nodes)
+            if (inRange(n.i, n.j, n.k, pi, pj, pk)) inside[flat(n.i, n.j, n.k, pi, pj)] = 1;
+    }
+}
+
+// MeshLevelSet::fastCalculateSignedDistanceField on a level set of
+// isize x jsize x ksize cells at node offset (oi, oj, ok) (meshlevelset.cpp:
+// 572-601): the exact band (multi-threaded form 781-1163: node range from the
+// triangle's AABB +-band, distances in 10^3-node block-local coordinates), then
+// the crossing-count signs (1298-1318).  No propagation beyond the band.
+//
+// singleThreaded: the band as _computeExactBandDistanceFieldSingleThreaded
+// computes it (1165-1234), the form the fracture path's per-island level
+// sets take: node range from int() of the scaled vertex coordinates,
+// clamped to the node grid, distances in the level set's own coordinates.
+void meshLevelSetFast(const TriMesh &m, int isize, int jsize, int ksize, double dx, GI o, int band,
+                      RandSource &rs, std::vector<float> &phi, bool singleThreaded = false) {
+    const int pi = isize + 1, pj = jsize + 1, pk = ksize + 1, bw = 10;
+    float ub = (float)((pi + pj + pk) * dx);  // getDistanceUpperBound (meshlevelset.cpp:769-771)
+    phi.assign((size_t)pi * pj * pk, ub);
+    V3 po = indexToPos(o.i, o.j, o.k, dx);
+    auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    for (int t = 0; singleThreaded && t < (int)m.t.size(); t++) {
+        V3 a = m.v[m.t[t][0]] - po, bb = m.v[m.t[t][1]] - po, c = m.v[m.t[t][2]] - po;
+        double invdx = 1.0 / dx;
+        double fa[3] = {(double)a.x * invdx, (double)a.y * invdx, (double)a.z * invdx};
+        double fb[3] = {(double)bb.x * invdx, (double)bb.y * invdx, (double)bb.z * invdx};
+        double fc[3] = {(double)c.x * invdx, (double)c.y * invdx, (double)c.z * invdx};
+        int lo[3], hi[3], dim[3] = {pi, pj, pk};
+        for (int q = 0; q < 3; q++) {
+            lo[q] = clampi(int(std::fmin(fa[q], std::fmin(fb[q], fc[q]))) - band, 0, dim[q] - 1);
+            hi[q] = clampi(int(std::fmax(fa[q], std::fmax(fb[q], fc[q]))) + band + 1, 0, dim[q] - 1);
+        }
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    V3 gpos = indexToPos(i, j, k, dx);
+                    V3 cp = closestPointOnTriangle(gpos, a, bb, c);
+                    float dist = length(cp - gpos);
+                    size_t n = flat(i, j, k, pi, pj);
+                    if (dist < phi[n]) phi[n] = dist;
+                }
+    }
+    for (int t = 0; !singleThreaded && t < (int)m.t.size(); t++) {
+        Box b = aabbTri(m, t);
+        b.pos = b.pos - po;
+        GI g0 = posToIndex(b.pos, dx), g1 = posToIndex(b.maxPoint(), dx);
+        g0.i -= band; g0.j -= band; g0.k -= band;
+        g1.i += band + 1; g1.j += band + 1; g1.k += band + 1;
+        if (g1.i < 0 || g0.i >= pi || g1.j < 0 || g0.j >= pj || g1.k < 0 || g0.k >= pk) continue;
+        g0.i = std::max(g0.i, 0); g0.j = std::max(g0.j, 0); g0.k = std::max(g0.k, 0);
+        g1.i = std::min(g1.i, isize); g1.j = std::min(g1.j, jsize); g1.k = std::min(g1.k, ksize);
+        V3 a = m.v[m.t[t][0]] - po, bb = m.v[m.t[t][1]] - po, c = m.v[m.t[t][2]] - po;
+        for (int k = g0.k; k <= g1.k; k++)
+            for (int j = g0.j; j <= g1.j; j++)
+                for (int i = g0.i; i <= g1.i; i++) {
+                    int bi = i / bw, bj = j / bw, bk = k / bw;
+                    V3 off = indexToPos(bi, bj, bk, bw * dx);
+                    V3 gpos = indexToPos(i - bi * bw, j - bj * bw, k - bk * bw, dx);
+                    V3 cp = closestPointOnTriangle(gpos, a - off, bb - off, c - off);
+                    float dist = length(cp - gpos);
+                    size_t n = flat(i, j, k, pi, pj);
+                    if (dist < phi[n]) phi[n] = dist;
+                }
+    }
+    TriMesh local = m;
+    translate(local, V3{-po.x, -po.y, -po.z});
+    std::vector<char> inside((size_t)pi * pj * pk, 0);
+    nodesInsideGrid(local, pi, pj, pk, dx, rs, inside);
+    for (size_t n = 0; n < phi.size(); n++)
+        if (inside[n]) phi[n] = -phi[n];
+}
+
+// TriangleMesh::removeExtraneousVertices (trianglemesh.cpp:775-822)
+void removeUnused(TriMesh &m) {
+    std::vector<int> tr(m.v.size(), -1);
+    std::vector<char> used(m.v.size(), 0);
+    for (auto &t : m.t) used[t[0]] = used[t[1]] = used[t[2]] = 1;
+    std::vector<V3> nv;
+    for (size_t q = 0; q < m.v.size(); q++)
+        if (used[q]) {
+            tr[q] = (int)nv.size();
+            nv.push_back(m.v[q]);
+        }
+    m.v = nv;
+    for (auto &t : m.t)
+        for (int c = 0; c < 3; c++) t[c] = tr[t[c]];
+}
+
+// MeshLevelSet::calculateUnion, phi part (meshlevelset.cpp:603-638, 1759-1794)
+void unionInto(std::vector<float> &phi, int pi, int pj, int pk, const std::vector<float> &other, int qi, int qj,
+               int qk, GI o) {
+    for (int k = 0; k < qk; k++)
+        for (int j = 0; j < qj; j++)
+            for (int i = 0; i < qi; i++) {
+                int a = i + o.i, b = j + o.j, c = k + o.k;
+                if (!inRange(a, b, c, pi, pj, pk)) continue;
+                float v = other[flat(i, j, k, qi, qj)];
+                float &p = phi[flat(a, b, c, pi, pj)];
+                if (v < p) p = v;
+            }
+}
+
+// MeshObject::getMeshLevelSet for a static mesh (meshobject.cpp:239-264,
+// 633-701, 738-754): fewer than 25 islands inside the domain -> the whole
+// mesh on its island grid; otherwise each island on its own grid.
+void obstacleLevelSet(TriMesh m, int ni, int nj, int nk, double dx, int band, RandSource &rs,
+                      std::vector<float> &levelset) {
+    // MeshObject::getMesh(frameInterpolation) (meshobject.cpp:158-177): v + f (v - v)
+    // with v - v = +0, so a -0 coordinate becomes +0
+    for (V3 &p : m.v) p = p + V3{0.0f, 0.0f, 0.0f};
+    removeUnused(m);
+    if (m.t.empty()) return;
+    Box grid{V3{0.0f, 0.0f, 0.0f}, ni * dx, nj * dx, nk * dx};
+    std::vector<TriMesh> islands;
+    for (TriMesh &I : splitIslands(m)) {
+        Box b = aabbOf(I.v.data(), (int)I.v.size());
+        if (aabbContains(grid, b) || aabbIntersects(grid, b)) islands.push_back(I);
+    }
+    std::vector<TriMesh> parts;
+    const bool fracture = (int)islands.size() >= 25;
+    if (fracture)
+        // the work queue pops from its back (boundedbuffer.h:119-135): one
+        // producer thread takes the islands last-first, and so draws their
+        // jitters in that order (with more threads the order varies)
+        parts.assign(islands.rbegin(), islands.rend());
+    else
+        parts.push_back(m);
+    for (TriMesh &part : parts) {
+        Box b = aabbOf(part.v.data(), (int)part.v.size());
+        GI g0 = posToIndex(b.pos, dx), g1 = posToIndex(b.maxPoint(), dx);
+        g0.i = (int)std::fmax(g0.i - band, 0); g0.j = (int)std::fmax(g0.j - band, 0); g0.k = (int)std::fmax(g0.k - band, 0);
+        g1.i = (int)std::fmin(g1.i + band + 1, ni - 1); g1.j = (int)std::fmin(g1.j + band + 1, nj - 1);
+        g1.k = (int)std::fmin(g1.k + band + 1, nk - 1);
+        int w = g1.i - g0.i, h = g1.j - g0.j, d = g1.k - g0.k;
+        if (w <= 0 || h <= 0 || d <= 0) continue;
+        std::vector<float> phi;
+        // fracture path: one single-threaded level set per island
+        // (meshobject.cpp:798-839)
+        meshLevelSetFast(part, w, h, d, dx, g0, band, rs, phi, fracture);
+        unionInto(levelset, ni + 1, nj + 1, nk + 1, phi, w + 1, h + 1, d + 1, g0);
+    }
+}
+
+TriMesh meshFrom(int nv, const float *verts, int nt, const int *tris) {
+    TriMesh m;
+    m.v.resize(nv);
+    for (int q = 0; q < nv; q++) m.v[q] = V3{verts[3 * q], verts[3 * q + 1], verts[3 * q + 2]};
+    m.t.resize(nt);
+    for (int q = 0; q < nt; q++) {
+        for (int c = 0; c < 3; c++) {
+            int v = tris[3 * q + c];
+            if (v < 0 || v >= nv) throw std::runtime_error("oracle: triangle index out of range");
+            m.t[q][c] = v;
+        }
+    }
+    return m;
+}
+
+}  // namespace
+
+int oracle_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi, int oj, int ok, int nv,
+                              const float *verts, int nt, const int *tris, int band, const int *rnd, int nrnd,
+                              int rand_max, float *phi_nodes) {
+    try {
+        TriMesh m = meshFrom(nv, verts, nt, tris);
+        RandSource rs{rnd, nrnd, 0, (double)rand_max};
+        std::vector<float> phi;
+        meshLevelSetFast(m, isize, jsize, ksize, dx, GI{oi, oj, ok}, band, rs, phi);
+        std::memcpy(phi_nodes, phi.data(), sizeof(float) * phi.size());
+        return rs.used;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+/* FluidSimulation::_addStaticObjectsToSDF with static mesh obstacles
+ * (fluidsimulation.cpp:5277-5339): the domain box SDF (negated), then the
+ * union of every normal obstacle's level set, then the union of the negated
+ * union of the inversed obstacles' level sets.  The domain box's own sign
+ * test draws one jitter vector (three rand() values) before the obstacles. */
+int oracle_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *verts,
+                            const int *nt, const int *tris, const int *inversed, const int *rnd, int nrnd,
+                            int rand_max, float *phi_nodes) {
+    try {
+        const int band = 3;
+        const size_t nn = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
+        oracle_solid_box_phi(ni, nj, nk, dx, phi_nodes);
+        RandSource rs{rnd, nrnd, 0, (double)rand_max};
+        rs.jitter(dx);  // the box's sign test
+        std::vector<float> solid(phi_nodes, phi_nodes + nn);
+        const float ub = (float)(((ni + 1) + (nj + 1) + (nk + 1)) * dx);
+        std::vector<TriMesh> meshes;
+        size_t vo = 0, to = 0;
+        for (int q = 0; q < nobj; q++) {
+            meshes.push_back(meshFrom(nv[q], verts + 3 * vo, nt[q], tris + 3 * to));
+            vo += nv[q];
+            to += nt[q];
+        }
+        for (int q = 0; q < nobj; q++) {
+            if (inversed[q]) continue;
+            std::vector<float> temp(nn, ub);
+            obstacleLevelSet(meshes[q], ni, nj, nk, dx, band, rs, temp);
+            unionInto(solid, ni + 1, nj + 1, nk + 1, temp, ni + 1, nj + 1, nk + 1, GI{0, 0, 0});
+        }
+        bool anyInv = false;
+        std::vector<float> inv(nn, ub);
+        for (int q = 0; q < nobj; q++) {
+            if (!inversed[q]) continue;
+            anyInv = true;
+            std::vector<float> temp(nn, ub);
+            obstacleLevelSet(meshes[q], ni, nj, nk, dx, band, rs, temp);
+            unionInto(inv, ni + 1, nj + 1, nk + 1, temp, ni + 1, nj + 1, nk + 1, GI{0, 0, 0});
+        }
+        if (anyInv) {
+            for (float &p : inv) p = -p;
+            unionInto(solid, ni + 1, nj + 1, nk + 1, inv, ni + 1, nj + 1, nk + 1, GI{0, 0, 0});
+        }
+        std::memcpy(phi_nodes, solid.data(), sizeof(float) * nn);
+        return rs.used;
+    } catch (const std::exception &) {
+        return -1;
+    }
 }
 
 /* Counting sort by the cell index the engine computes for a particle

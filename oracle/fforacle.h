@@ -27,6 +27,25 @@ extern "C" {
  * box inset by 3*dx+1e-4 (fluidsimulation.cpp:5175-5186, 5277-5287). */
 void oracle_solid_box_phi(int ni, int nj, int nk, double dx, float *phi_nodes);
 
+/* MeshLevelSet::fastCalculateSignedDistanceField on a level set of
+ * isize x jsize x ksize cells at node offset (oi, oj, ok): exact-band
+ * distances (band nodes around each triangle), then crossing-count signs.
+ * rnd[nrnd] are the rand() values the sign test draws (three per jitter
+ * vector; rand_max = RAND_MAX).  phi_nodes: (isize+1)(jsize+1)(ksize+1).
+ * Returns the number of rand() values used, -1 on error. */
+int oracle_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi, int oj, int ok, int nv,
+                              const float *verts, int nt, const int *tris, int band, const int *rnd, int nrnd,
+                              int rand_max, float *phi_nodes);
+
+/* The static solid SDF with nobj mesh obstacles: the domain box, then the
+ * obstacles' level sets (fluidsimulation.cpp:5277-5339).  Obstacle q has
+ * nv[q] vertices and nt[q] triangles, concatenated in verts/tris (triangle
+ * indices local to the obstacle); inversed[q] != 0 for an inversed one.
+ * Returns the number of rand() values used, -1 on error. */
+int oracle_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *verts,
+                            const int *nt, const int *tris, const int *inversed, const int *rnd, int nrnd,
+                            int rand_max, float *phi_nodes);
+
 /* Counting sort of particles by flat cell id of floor(p/dx).  keys[P],
  * counts[N], offsets[N+1] (exclusive scan), perm[P] (stable). */
 void oracle_cell_sort(int ni, int nj, int nk, double dx, int np, const float *pos,

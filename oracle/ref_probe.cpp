@@ -47,6 +47,8 @@
 #include "gridutils.h"
 #include "macvelocityfield.h"
 #include "meshlevelset.h"
+#include "meshobject.h"
+#include "trianglemesh.h"
 #include "particlelevelset.h"
 #include "particlesystem.h"
 #include "pressuresolver.h"
@@ -75,6 +77,18 @@ template <class T>
 void copyIn(Array3d<T> &a, const T *in) {
     std::memcpy(a.getRawArray(), in, sizeof(T) * (size_t)a.width * a.height * a.depth);
 }
+TriangleMesh makeMesh(int nv, const float *verts, int nt, const int *tris) {
+    TriangleMesh m;
+    for (int q = 0; q < nv; q++) m.vertices.push_back(vmath::vec3(verts[3 * q], verts[3 * q + 1], verts[3 * q + 2]));
+    for (int q = 0; q < nt; q++) {
+        Triangle t;
+        t.tri[0] = tris[3 * q];
+        t.tri[1] = tris[3 * q + 1];
+        t.tri[2] = tris[3 * q + 2];
+        m.triangles.push_back(t);
+    }
+    return m;
+}
 void boolOut(Array3d<bool> &a, unsigned char *out) {
     size_t n = (size_t)a.width * a.height * a.depth;
     bool *b = a.getRawArray();
@@ -93,6 +107,46 @@ void refprobe_solid_box_phi(int ni, int nj, int nk, double dx, float *out) {
     MeshLevelSet sdf(ni, nj, nk, dx);
     sim._addStaticObjectsToSDF(0.0, sdf);
     copyOut(*sdf.getPhiArray3d(), out);
+}
+
+/* MeshLevelSet::fastCalculateSignedDistanceField on a level set of
+ * isize x jsize x ksize cells at node offset (oi, oj, ok), after srand(seed)
+ * (meshlevelset.cpp:572-601). */
+void refprobe_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi, int oj, int ok, int nv,
+                                 const float *verts, int nt, const int *tris, int band, unsigned seed,
+                                 float *out) {
+    TriangleMesh m = makeMesh(nv, verts, nt, tris);
+    MeshLevelSet ls(isize, jsize, ksize, dx);
+    ls.setGridOffset(GridIndex(oi, oj, ok));
+    ls.disableVelocityData();
+    srand(seed);
+    ls.fastCalculateSignedDistanceField(m, band);
+    copyOut(*ls.getPhiArray3d(), out);
+}
+
+/* FluidSimulation::_addStaticObjectsToSDF with static mesh obstacles, after
+ * srand(seed) (fluidsimulation.cpp:5277-5339).  The temporary level set is
+ * sized to the domain as _updateSolidLevelSet does (5396-5405). */
+void refprobe_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *verts,
+                               const int *nt, const int *tris, const int *inversed, unsigned seed, float *out) {
+    FluidSimulation sim(ni, nj, nk, dx);
+    sim.initialize();
+    std::vector<std::unique_ptr<MeshObject>> objs;
+    size_t vo = 0, to = 0;
+    for (int q = 0; q < nobj; q++) {
+        objs.emplace_back(new MeshObject(ni, nj, nk, dx));
+        objs.back()->updateMeshStatic(makeMesh(nv[q], verts + 3 * vo, nt[q], tris + 3 * to));
+        if (inversed[q]) objs.back()->inverse();
+        sim.addMeshObstacle(objs.back().get());
+        vo += nv[q];
+        to += nt[q];
+    }
+    sim._tempSolidSDF = MeshLevelSet(ni, nj, nk, dx);
+    MeshLevelSet sdf(ni, nj, nk, dx);
+    srand(seed);
+    sim._addStaticObjectsToSDF(1.0 / 30.0, sdf);
+    copyOut(*sdf.getPhiArray3d(), out);
+    sim.removeMeshObstacles();
 }
 
 void refprobe_liquid_sdf(int ni, int nj, int nk, double dx, int np, const float *pos,

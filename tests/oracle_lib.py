@@ -97,6 +97,78 @@ def solid_box_phi(n3, dx):
     return phi
 
 
+RAND_MAX = 2147483647  # glibc
+
+
+def libc_rand_values(seed, n=96):
+    """The first n values of the C library's rand() after srand(seed): the
+    draws the reference's inside test takes (three per jitter vector)."""
+    libc = C.CDLL(None)
+    libc.srand(C.c_uint(seed))
+    return np.array([libc.rand() for _ in range(n)], np.int32)
+
+
+def libc_srand(seed):
+    C.CDLL(None).srand(C.c_uint(seed))
+
+
+def _mesh_arrays(meshes):
+    verts = np.ascontiguousarray(np.concatenate([_f32(m[0]).reshape(-1, 3) for m in meshes]), np.float32)
+    tris = np.ascontiguousarray(np.concatenate([np.asarray(m[1], np.int32).reshape(-1, 3) for m in meshes]), np.int32)
+    nv = np.array([len(_f32(m[0]).reshape(-1, 3)) for m in meshes], np.int32)
+    nt = np.array([len(np.asarray(m[1]).reshape(-1, 3)) for m in meshes], np.int32)
+    inv = np.array([1 if (len(m) > 2 and m[2]) else 0 for m in meshes], np.int32)
+    return verts, tris, nv, nt, inv
+
+
+def mesh_levelset_fast(cells3, dx, offset, verts, tris, band, seed):
+    """Oracle MeshLevelSet::fastCalculateSignedDistanceField on a level set of
+    cells3 cells at node offset `offset`; returns (phi (k, j, i), rand() draws)."""
+    isz, jsz, ksz = cells3
+    v = _f32(verts).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    r = libc_rand_values(seed)
+    phi = np.zeros((ksz + 1, jsz + 1, isz + 1), np.float32)
+    used = oracle().oracle_mesh_levelset_fast(isz, jsz, ksz, C.c_double(dx), *offset, len(v), fp(v), len(t), fp(t),
+                                              band, fp(r), len(r), RAND_MAX, fp(phi))
+    if used < 0:
+        raise RuntimeError("oracle_mesh_levelset_fast failed")
+    return phi, used
+
+
+def solid_phi_meshes(n3, dx, meshes, seed):
+    """Oracle static solid SDF: domain box plus mesh obstacles, each
+    (vertices, triangles[, inversed]); returns (phi (k, j, i), rand() draws)."""
+    ni, nj, nk = n3
+    verts, tris, nv, nt, inv = _mesh_arrays(meshes)
+    r = libc_rand_values(seed)
+    phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
+    used = oracle().oracle_solid_phi_meshes(ni, nj, nk, C.c_double(dx), len(meshes), fp(nv), fp(verts), fp(nt),
+                                            fp(tris), fp(inv), fp(r), len(r), RAND_MAX, fp(phi))
+    if used < 0:
+        raise RuntimeError("oracle_solid_phi_meshes failed")
+    return phi, used
+
+
+def ref_mesh_levelset_fast(cells3, dx, offset, verts, tris, band, seed):
+    isz, jsz, ksz = cells3
+    v = _f32(verts).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    phi = np.zeros((ksz + 1, jsz + 1, isz + 1), np.float32)
+    probe().refprobe_mesh_levelset_fast(isz, jsz, ksz, C.c_double(dx), *offset, len(v), fp(v), len(t), fp(t), band,
+                                        C.c_uint(seed), fp(phi))
+    return phi
+
+
+def ref_solid_phi_meshes(n3, dx, meshes, seed):
+    ni, nj, nk = n3
+    verts, tris, nv, nt, inv = _mesh_arrays(meshes)
+    phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
+    probe().refprobe_solid_phi_meshes(ni, nj, nk, C.c_double(dx), len(meshes), fp(nv), fp(verts), fp(nt),
+                                      fp(tris), fp(inv), C.c_uint(seed), fp(phi))
+    return phi
+
+
 def liquid_sdf(n3, dx, pos, radius):
     ni, nj, nk = n3
     pos = _f32(pos).reshape(-1, 3)
@@ -300,6 +372,11 @@ class MarkerParticleData(C.Structure):
     _fields_ = [("size", C.c_int), ("positions", C.c_void_p), ("velocities", C.c_void_p)]
 
 
+class TriangleMesh_t(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("triangles", C.c_void_p), ("numVertices", C.c_int),
+                ("numTriangles", C.c_int)]
+
+
 class ReferenceSim:
     """The reference engine built from its sources, driven through its C ABI
     (fluidsimulation_c.cpp) exactly like the reference's ctypes layer does."""
@@ -333,6 +410,28 @@ class ReferenceSim:
     def initialize(self):
         err = C.c_int()
         self.L.FluidSimulation_initialize(self.h, C.byref(err))
+
+    def add_mesh_obstacle(self, n3, dx, verts, tris, inversed=False):
+        """MeshObject_new + update_mesh_static (+ inverse), then
+        FluidSimulation_add_mesh_obstacle (meshobject_c.cpp:38-69)."""
+        L = self.L
+        err = C.c_int()
+        L.MeshObject_new.restype = C.c_void_p
+        L.MeshObject_new.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p]
+        L.MeshObject_update_mesh_static.argtypes = [C.c_void_p, TriangleMesh_t, C.c_void_p]
+        L.MeshObject_inverse.argtypes = [C.c_void_p, C.c_void_p]
+        L.FluidSimulation_add_mesh_obstacle.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        v = _f32(verts).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        self._keep += [v, t]
+        obj = C.c_void_p(L.MeshObject_new(*n3, dx, C.byref(err)))
+        L.MeshObject_update_mesh_static(obj, TriangleMesh_t(fp(v), fp(t), len(v), len(t)), C.byref(err))
+        if inversed:
+            L.MeshObject_inverse(obj, C.byref(err))
+        L.FluidSimulation_add_mesh_obstacle(self.h, obj, C.byref(err))
+        assert err.value == 1
+        self._keep.append(obj)
+        return obj
 
     def update(self, dt):
         err = C.c_int()
