@@ -35,6 +35,11 @@ auto guarded(int *err, F &&f) -> decltype(f()) {
     return decltype(f())();
 }
 
+ff::MeshObstacle *M(MeshObject *o) {
+    if (!o) throw std::runtime_error("Error: null MeshObject handle.\n");
+    return reinterpret_cast<ff::MeshObstacle *>(o);
+}
+
 ff::FluidSimulation *S(FluidSimulation *o) {
     if (!o) throw std::runtime_error("Error: null FluidSimulation handle.\n");
     return reinterpret_cast<ff::FluidSimulation *>(o);
@@ -267,6 +272,145 @@ void FluidSimulation_set_pressure_solver_warm_start(FluidSimulation *o, int on, 
 int FluidSimulation_get_pressure_solver_warm_start(FluidSimulation *o, int *err) {
     return guarded(err, [&] { return S(o)->getPressureSolverWarmStart() ? 1 : 0; });
 }
+// ---- static mesh obstacles ----
+MeshObject *MeshObject_new(int i, int j, int k, double dx, int *err) {
+    return guarded(err, [&] {
+        checkGrid(i, j, k, dx);
+        return reinterpret_cast<MeshObject *>(new ff::MeshObstacle(i, j, k, dx));
+    });
+}
+void MeshObject_destroy(MeshObject *obj) { delete reinterpret_cast<ff::MeshObstacle *>(obj); }
+void MeshObject_update_mesh_static(MeshObject *obj, TriangleMesh_t mesh, int *err) {
+    guarded(err, [&] { M(obj)->updateMeshStatic(mesh.numVertices, mesh.vertices, mesh.numTriangles, mesh.triangles); });
+}
+void MeshObject_update_mesh_animated(MeshObject *obj, TriangleMesh_t, TriangleMesh_t, TriangleMesh_t, int *err) {
+    guarded(err, [&] {
+        M(obj);
+        throw std::runtime_error("Error: animated mesh obstacles are not supported by this engine.\n");
+    });
+}
+void MeshObject_enable(MeshObject *obj, int *err) {
+    guarded(err, [&] { M(obj)->enable(); });
+}
+void MeshObject_disable(MeshObject *obj, int *err) {
+    guarded(err, [&] { M(obj)->disable(); });
+}
+int MeshObject_is_enabled(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return (int)M(obj)->enabled; });
+}
+void MeshObject_inverse(MeshObject *obj, int *err) {
+    guarded(err, [&] { M(obj)->inverse(); });
+}
+int MeshObject_is_inversed(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return (int)M(obj)->inversed; });
+}
+float MeshObject_get_friction(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->friction; });
+}
+void MeshObject_set_friction(MeshObject *obj, float f, int *err) {
+    guarded(err, [&] { M(obj)->setFriction(f); });
+}
+float MeshObject_get_velocity_scale(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->velocityScale; });
+}
+void MeshObject_set_velocity_scale(MeshObject *obj, float scale, int *err) {
+    guarded(err, [&] { M(obj)->velocityScale = scale; });
+}
+float MeshObject_get_whitewater_influence(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->whitewaterInfluence; });
+}
+void MeshObject_set_whitewater_influence(MeshObject *obj, float value, int *err) {
+    guarded(err, [&] { M(obj)->setNonNegative(M(obj)->whitewaterInfluence, value); });
+}
+float MeshObject_get_dust_emission_strength(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->dustEmissionStrength; });
+}
+void MeshObject_set_dust_emission_strength(MeshObject *obj, float value, int *err) {
+    guarded(err, [&] { M(obj)->setNonNegative(M(obj)->dustEmissionStrength, value); });
+}
+float MeshObject_get_sheeting_strength(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->sheetingStrength; });
+}
+void MeshObject_set_sheeting_strength(MeshObject *obj, float value, int *err) {
+    guarded(err, [&] { M(obj)->setNonNegative(M(obj)->sheetingStrength, value); });
+}
+float MeshObject_get_mesh_expansion(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->meshExpansion; });
+}
+void MeshObject_set_mesh_expansion(MeshObject *obj, float ex, int *err) {
+    guarded(err, [&] { M(obj)->meshExpansion = ex; });
+}
+void MeshObject_enable_append_object_velocity(MeshObject *obj, int *err) {
+    guarded(err, [&] { M(obj)->appendObjectVelocity = true; });
+}
+void MeshObject_disable_append_object_velocity(MeshObject *obj, int *err) {
+    guarded(err, [&] { M(obj)->appendObjectVelocity = false; });
+}
+int MeshObject_is_append_object_velocity_enabled(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return (int)M(obj)->appendObjectVelocity; });
+}
+float MeshObject_get_object_velocity_influence(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->objectVelocityInfluence; });
+}
+void MeshObject_set_object_velocity_influence(MeshObject *obj, float value, int *err) {
+    guarded(err, [&] { M(obj)->objectVelocityInfluence = value; });
+}
+int MeshObject_get_priority(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->priority; });
+}
+void MeshObject_set_priority(MeshObject *obj, int n, int *err) {
+    guarded(err, [&] { M(obj)->priority = n; });
+}
+int MeshObject_get_source_id(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->sourceID; });
+}
+void MeshObject_set_source_id(MeshObject *obj, int id, int *err) {
+    guarded(err, [&] { M(obj)->sourceID = id; });
+}
+// the reference declares these four float getters as returning int
+int MeshObject_get_viscosity(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return (int)M(obj)->viscosity; });
+}
+void MeshObject_set_viscosity(MeshObject *obj, float v, int *err) {
+    guarded(err, [&] { M(obj)->viscosity = v; });
+}
+int MeshObject_get_density(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return (int)M(obj)->density; });
+}
+void MeshObject_set_density(MeshObject *obj, float d, int *err) {
+    guarded(err, [&] { M(obj)->density = d; });
+}
+int MeshObject_get_lifetime(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return (int)M(obj)->lifetime; });
+}
+void MeshObject_set_lifetime(MeshObject *obj, float v, int *err) {
+    guarded(err, [&] { M(obj)->lifetime = v; });
+}
+int MeshObject_get_lifetime_variance(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return (int)M(obj)->lifetimeVariance; });
+}
+void MeshObject_set_lifetime_variance(MeshObject *obj, float v, int *err) {
+    guarded(err, [&] { M(obj)->lifetimeVariance = v; });
+}
+FFVector3 MeshObject_get_source_color(MeshObject *obj, int *err) {
+    return guarded(err, [&] { return M(obj)->sourceColor; });
+}
+void MeshObject_set_source_color(MeshObject *obj, double r, double g, double b, int *err) {
+    guarded(err, [&] { M(obj)->sourceColor = FFVector3{(float)r, (float)g, (float)b}; });
+}
+void FluidSimulation_add_mesh_obstacle(FluidSimulation *o, MeshObject *obstacle, int *err) {
+    guarded(err, [&] { S(o)->addMeshObstacle(M(obstacle)); });
+}
+void FluidSimulation_remove_mesh_obstacle(FluidSimulation *o, MeshObject *obstacle, int *err) {
+    guarded(err, [&] { S(o)->removeMeshObstacle(M(obstacle)); });
+}
+void FluidSimulation_remove_mesh_obstacles(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->removeMeshObstacles(); });
+}
+int FFAMD_get_last_solid_rand_draws(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getLastSolidRandDraws(); });
+}
+
 void FluidSimulation_set_solid_phi(FluidSimulation *o, const float *phi, int *err) {
     guarded(err, [&] { S(o)->setSolidPhi(phi); });
 }
@@ -437,6 +581,45 @@ void FFAMD_solid_box_phi(int ni, int nj, int nk, double dx, float *phi, int *err
         ff::solidBoxPhi(ff::Grid{ni, nj, nk, dx}, d, sc.s);
         down(phi, d.p, nn, sc.s);
         sync(sc.s);
+    });
+}
+
+int FFAMD_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi, int oj, int ok, int nv,
+                             const float *vertices, int nt, const int *triangles, float *phi, int *err) {
+    return guarded(err, [&] {
+        checkGrid(isize, jsize, ksize, dx);
+        Scratch sc;
+        size_t nn = (size_t)(isize + 1) * (jsize + 1) * (ksize + 1);
+        DBuf<float> d(nn);
+        ff::ObstacleMesh m{vertices, nv, triangles, nt, false, 0.0f};
+        int draws = ff::meshLevelSetFastDevice(ff::Grid{isize, jsize, ksize, dx}, ff::I3{oi, oj, ok}, m, d, sc.s);
+        down(phi, d.p, nn, sc.s);
+        sync(sc.s);
+        return draws;
+    });
+}
+
+int FFAMD_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *vertices,
+                           const int *nt, const int *triangles, const int *inversed, float *phi, int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (nobj < 0) throw std::invalid_argument("Error: negative obstacle count.\n");
+        Scratch sc;
+        size_t nn = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
+        DBuf<float> d(nn);
+        std::vector<ff::ObstacleMesh> meshes;
+        size_t vo = 0, to = 0;
+        for (int q = 0; q < nobj; q++) {
+            meshes.push_back(ff::ObstacleMesh{vertices + 3 * vo, nv[q], triangles + 3 * to, nt[q], inversed[q] != 0,
+                                              0.0f});
+            vo += (size_t)nv[q];
+            to += (size_t)nt[q];
+        }
+        ff::solidBoxPhi(ff::Grid{ni, nj, nk, dx}, d, sc.s);
+        int draws = ff::meshObstacleSolid(ff::Grid{ni, nj, nk, dx}, meshes.data(), nobj, d, sc.s);
+        down(phi, d.p, nn, sc.s);
+        sync(sc.s);
+        return draws;
     });
 }
 

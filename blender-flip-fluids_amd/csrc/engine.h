@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -37,6 +38,40 @@ struct DevBuf {
 };
 
 void hipCheck(hipError_t e, const char *what);
+
+// A static triangle-mesh obstacle (MeshObject, meshobject.h/.cpp): its mesh
+// in host memory and the reference's per-object settings.  Only enable and
+// disable mark the object's state as changed (meshobject.cpp:306-318), which
+// makes the simulation rebuild its solid SDF.
+class MeshObstacle {
+public:
+    MeshObstacle(int i, int j, int k, double dx) : isize(i), jsize(j), ksize(k), dx(dx) {}
+    // MeshObject::updateMeshStatic (meshobject.cpp:50-59)
+    void updateMeshStatic(int nv, const float *vertices, int nt, const int *triangles);
+    void enable() {
+        if (!enabled) stateChanged = true;
+        enabled = true;
+    }
+    void disable() {
+        if (enabled) stateChanged = true;
+        enabled = false;
+    }
+    void inverse() { inversed = !inversed; }
+    // setters with the reference's clamping (meshobject.cpp:340-387)
+    void setFriction(float f) { friction = std::fmax(std::fmin(f, 1.0f), 0.0f); }
+    void setNonNegative(float &field, float v) { field = std::fmax(v, 0.0f); }
+
+    int isize, jsize, ksize;
+    double dx;
+    std::vector<float> vertices;
+    std::vector<int> triangles;
+    bool enabled = true, inversed = false, stateChanged = false, appendObjectVelocity = false;
+    float friction = 0.0f, velocityScale = 1.0f, whitewaterInfluence = 1.0f, dustEmissionStrength = 1.0f,
+          sheetingStrength = 1.0f, meshExpansion = 0.0f, objectVelocityInfluence = 1.0f;
+    int priority = 0, sourceID = 0;
+    float viscosity = 0.0f, density = 1.0f, lifetime = 0.0f, lifetimeVariance = 0.0f;
+    FFVector3 sourceColor{0.0f, 0.0f, 0.0f};
+};
 
 enum class PressurePrecision { FP64 = 0, MixedRefined = 1, MixedReliable = 2, FP32Preconditioner = 3 };
 
@@ -106,6 +141,14 @@ public:
     bool getPressureSolverWarmStart() const { return _warmStart; }
     void setSolidPhi(const float *phiNodes);
 
+    // static mesh obstacles (fluidsimulation.cpp:2990-3032); the solid SDF is
+    // rebuilt at the next update (fluidsimulation.cpp:5387-5426)
+    void addMeshObstacle(MeshObstacle *obstacle);
+    void removeMeshObstacle(MeshObstacle *obstacle);
+    void removeMeshObstacles();
+    // rand() draws of the last solid SDF rebuild (the sign tests' jitter)
+    int getLastSolidRandDraws() const { return _solidDraws; }
+
     // particle I/O (fluidsimulation.cpp:3426-3839)
     void loadMarkerParticleData(int n, const char *positions, const char *velocities);
     // APIC affine vectors of one load batch (fluidsimulation.cpp:3841-3863)
@@ -163,6 +206,10 @@ private:
     std::vector<FFVector3> _bodyForces;
     bool _solidGiven = false;
     std::vector<float> _solidHost;
+    std::vector<MeshObstacle *> _obstacles;
+    bool _solidUpToDate = false;
+    int _solidDraws = 0;
+    void updateSolidLevelSet();
 
     std::vector<std::vector<float>> _loadPos, _loadVel, _loadAff;  // _loadAff: 9 floats per particle
 

@@ -139,6 +139,72 @@ void FluidSimulation::setSolidPhi(const float *phi) {
     _solidGiven = true;
 }
 
+void MeshObstacle::updateMeshStatic(int nv, const float *v, int nt, const int *t) {
+    if (nv < 0 || nt < 0 || (nv > 0 && !v) || (nt > 0 && !t))
+        throw std::invalid_argument("Error: invalid triangle mesh data.\n");
+    for (int q = 0; q < 3 * nt; q++)
+        if (t[q] < 0 || t[q] >= nv) throw std::out_of_range("Error: triangle vertex index out of range.\n");
+    vertices.assign(v, v + 3 * (size_t)nv);
+    triangles.assign(t, t + 3 * (size_t)nt);
+}
+
+void FluidSimulation::addMeshObstacle(MeshObstacle *obstacle) {
+    if (!obstacle) throw std::invalid_argument("Error: null mesh obstacle.\n");
+    for (MeshObstacle *o : _obstacles)
+        if (o == obstacle) throw std::runtime_error("Error: mesh obstacle has already been added.\n");
+    _obstacles.push_back(obstacle);
+    _solidUpToDate = false;
+}
+
+void FluidSimulation::removeMeshObstacle(MeshObstacle *obstacle) {
+    for (size_t q = 0; q < _obstacles.size(); q++)
+        if (_obstacles[q] == obstacle) {
+            _obstacles.erase(_obstacles.begin() + q);
+            _solidUpToDate = false;
+            return;
+        }
+    throw std::invalid_argument("Error: could not find mesh obstacle to remove.\n");
+}
+
+void FluidSimulation::removeMeshObstacles() {
+    _obstacles.clear();
+    _solidUpToDate = false;
+}
+
+// _updateSolidLevelSet + _addStaticObjectsToSDF + _updateNearSolidGrid
+// (fluidsimulation.cpp:5387-5426, 5277-5339, 5448-5480), run at the start of
+// a substep while particles exist, as the reference's obstacle update is
+// (5550-5570): the objects' state flags are read and cleared, and a stale
+// solid is rebuilt from the domain SDF (or the one set) and the enabled
+// obstacles, with its face weights and near-solid grid.
+void FluidSimulation::updateSolidLevelSet() {
+    for (MeshObstacle *o : _obstacles) {
+        if (o->stateChanged) _solidUpToDate = false;
+        o->stateChanged = false;
+    }
+    if (_solidUpToDate) return;
+    std::vector<ObstacleMesh> meshes;
+    for (MeshObstacle *o : _obstacles) {
+        if (!o->enabled) continue;
+        if (o->friction != 0.0f)
+            throw std::runtime_error("Error: mesh obstacles with friction are not supported by this engine.\n");
+        meshes.push_back(ObstacleMesh{o->vertices.data(), (int)(o->vertices.size() / 3), o->triangles.data(),
+                                      (int)(o->triangles.size() / 3), o->inversed, o->meshExpansion});
+    }
+    if (_solidGiven)
+        hipCheck(hipMemcpyAsync(_solid, _solidHost.data(), sizeof(float) * _solidHost.size(), hipMemcpyHostToDevice,
+                                _stream),
+                 "upload solid SDF");
+    else
+        solidBoxPhi(_g, _solid, _stream);
+    _solidDraws = meshObstacleSolid(_g, meshes.data(), (int)meshes.size(), _solid, _stream);
+    faceWeights(_g, _solid, _wu, _wv, _ww, _stream);
+    int layers = (int)std::ceil((float)_cfl / (float)3);
+    nearSolidGrid(_g, _solid, _near, _nearTmp, _ci, _cj, _ck, layers, _stream);
+    hipCheck(hipStreamSynchronize(_stream), "solid level set");
+    _solidUpToDate = true;
+}
+
 void FluidSimulation::requireInitialized(const char *what) const {
     if (!_initialized)
         throw std::runtime_error(std::string("Error: FluidSimulation must be initialized before ") + what + ".\n");
@@ -309,6 +375,7 @@ void FluidSimulation::initialize() {
     hipCheck(hipMemsetAsync(_w, 0, sizeof(float) * _w.n, _stream), "memset");
     hipCheck(hipStreamSynchronize(_stream), "initialize");
     _initialized = true;
+    _solidUpToDate = false;  // rebuilt, with the obstacles, on the first substep
 }
 
 double FluidSimulation::hostScalar(const double *d) {
@@ -440,6 +507,7 @@ void FluidSimulation::stepFluid(double dt) {
     const double radius = 0.5 * 1.0 * _g.dx * std::sqrt(3.0);  // fluidsimulation.cpp:4351
     const int layers = (int)std::ceil(std::sqrt(3.0) * _cfl) + 3;  // :6282-6286
     if (_P.n == 0) return;
+    updateSolidLevelSet();
     hipCheck(hipEventRecord(_ev[0], _stream), "event");
     // liquid level set (5586-5617)
     _timers.begin(T_CELLSORT, _stream, 5);

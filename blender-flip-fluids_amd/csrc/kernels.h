@@ -15,6 +15,27 @@ namespace ff {
 // throws std::runtime_error naming `what` when e is not hipSuccess
 void hipCheck(hipError_t e, const char *what);
 
+// ---- static mesh obstacles (kernels_mesh.hip) ----
+// A triangle mesh in host memory: 3 floats per vertex, 3 vertex indices per
+// triangle.
+struct ObstacleMesh {
+    const float *vertices;
+    int numVertices;
+    const int *triangles;
+    int numTriangles;
+    bool inversed;
+    float meshExpansion;  // used by the 25-or-more-island path only, which rejects non-zero values
+};
+// Unions the obstacles' signed distance level sets into the nodal solid SDF
+// `solid` (device, (ni+1)(nj+1)(nk+1)), which holds the domain SDF: normal
+// obstacles in order, then the negated union of the inversed ones.  Takes the
+// sign tests' jitter from rand() (three draws per test, the domain box's
+// first) and returns the number of draws.  Synchronizes the stream.
+int meshObstacleSolid(Grid g, const ObstacleMesh *obstacles, int n, float *solid, hipStream_t s);
+// One mesh's fast signed distance level set on `cells` cells at node offset
+// `offset` (device phi, (ni+1)(nj+1)(nk+1)); returns the rand() draws.
+int meshLevelSetFastDevice(Grid cells, I3 offset, const ObstacleMesh &mesh, float *phi, hipStream_t s);
+
 // ---- scan / reductions (kernels_scan.hip) ----
 // Exclusive prefix sum of n int32 values; out may alias in; out[n] = total.
 // scratch must hold scanScratchInts(n) ints.

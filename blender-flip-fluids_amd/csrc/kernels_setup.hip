@@ -7,90 +7,6 @@ namespace ff {
 
 namespace {
 
-__device__ double clampd(double v, double lo, double hi) {
-    if (v < lo) v = lo;
-    if (v > hi) v = hi;
-    return v;
-}
-
-// Collision::findClosestPointOnTriangle (collision.cpp:161-268)
-__device__ V3 closestPointOnTriangle(V3 p0, V3 v0, V3 v1, V3 v2) {
-    V3 edge0 = sub(v1, v0), edge1 = sub(v2, v0), pv = sub(v0, p0);
-    double a = dot3(edge0, edge0), b = dot3(edge0, edge1), c = dot3(edge1, edge1);
-    double d = dot3(edge0, pv), e = dot3(edge1, pv);
-    double det = a * c - b * b;
-    if (fabs(det) < 1e-9) {
-        float feps = 1e-6f;
-        float len1 = len3(sub(v0, v1)), len2 = len3(sub(v1, v2)), len3v = len3(sub(v2, v0));
-        if (len1 < feps && len2 < feps && len3v < feps) return v0;
-        V3 dp, dq;
-        if (len1 < feps) {
-            dp = v0;
-            dq = v2;
-        } else if (len2 < feps) {
-            dp = v1;
-            dq = v0;
-        } else {
-            dp = v2;
-            dq = v1;
-        }
-        float dotnum = dot3(sub(p0, dp), sub(dq, dp));
-        float dotdem = dot3(sub(dq, dp), sub(dq, dp));
-        if (fabsf(dotdem) < feps) return v0;
-        float lambda = dotnum / dotdem;
-        if (lambda <= 0.0f) return dp;
-        if (lambda >= 1.0f) return dq;
-        return add(dp, scale(lambda, sub(dq, dp)));
-    }
-    double s = b * e - c * d, t = b * d - a * e;
-    if (s + t < det) {
-        if (s < 0.0) {
-            if (t < 0.0) {
-                if (d < 0.0) {
-                    s = clampd(-d / a, 0.0, 1.0);
-                    t = 0.0;
-                } else {
-                    s = 0.0;
-                    t = clampd(-e / c, 0.0, 1.0);
-                }
-            } else {
-                s = 0.0;
-                t = clampd(-e / c, 0.0, 1.0);
-            }
-        } else if (t < 0.0) {
-            s = clampd(-d / a, 0.0, 1.0);
-            t = 0.0;
-        } else {
-            double invDet = 1.0 / det;
-            s *= invDet;
-            t *= invDet;
-        }
-    } else {
-        if (s < 0.0) {
-            double tmp0 = b + d, tmp1 = c + e;
-            if (tmp1 > tmp0) {
-                s = clampd((tmp1 - tmp0) / (a - 2 * b + c), 0.0, 1.0);
-                t = 1 - s;
-            } else {
-                t = clampd(-e / c, 0.0, 1.0);
-                s = 0.0;
-            }
-        } else if (t < 0.0) {
-            if (a + d > b + e) {
-                s = clampd((c + e - b - d) / (a - 2 * b + c), 0.0, 1.0);
-                t = 1 - s;
-            } else {
-                s = clampd(-e / c, 0.0, 1.0);
-                t = 0.0;
-            }
-        } else {
-            s = clampd((c + e - b - d) / (a - 2 * b + c), 0.0, 1.0);
-            t = 1.0 - s;
-        }
-    }
-    return add(add(v0, scale((float)s, edge0)), scale((float)t, edge1));
-}
-
 struct BoxMesh {
     V3 v[8];
     int t[12][3];

@@ -7,3 +7,5 @@ loading a from-scratch HIP engine instead of the CPU one.
 """
 from .fluidsimulation import FluidSimulation, FluidSimulationFrameStats_t  # noqa: F401
 from . import kernels, scenes  # noqa: F401
+from .meshobject import MeshObject  # noqa: F401
+from .trianglemesh import TriangleMesh  # noqa: F401

@@ -281,6 +281,23 @@ class FluidSimulation(object):
         return ("fp64", "mixed", "mixed-reliable", "fp32-preconditioner")[
             _call0(self, "FluidSimulation_get_pressure_solver_precision", c_int)]
 
+    def add_mesh_obstacle(self, mesh_object):
+        libfunc = lib.FluidSimulation_add_mesh_obstacle
+        pb.init_lib_func(libfunc, [c_void_p, c_void_p, c_void_p], None)
+        pb.execute_lib_func(libfunc, [self(), mesh_object()])
+
+    def remove_mesh_obstacle(self, mesh_object):
+        libfunc = lib.FluidSimulation_remove_mesh_obstacle
+        pb.init_lib_func(libfunc, [c_void_p, c_void_p, c_void_p], None)
+        pb.execute_lib_func(libfunc, [self(), mesh_object()])
+
+    def remove_mesh_obstacles(self):
+        _call0(self, "FluidSimulation_remove_mesh_obstacles")
+
+    def get_last_solid_rand_draws(self):
+        """rand() values the last solid SDF rebuild drew (sign-test jitter)."""
+        return _call0(self, "FFAMD_get_last_solid_rand_draws", c_int)
+
     @_check_simulation_not_initialized
     def set_solid_phi(self, phi_nodes):
         i, j, k = self._dims

@@ -200,6 +200,69 @@ int FluidSimulation_get_pressure_solver_reference_summation(FluidSimulation *obj
 void FluidSimulation_set_pressure_solver_warm_start(FluidSimulation *obj, int enabled, int *err);
 int FluidSimulation_get_pressure_solver_warm_start(FluidSimulation *obj, int *err);
 
+/* ---- static mesh obstacles ------------------------------------------------
+ * MeshObject (meshobject.h) and its C ABI (c_bindings/meshobject_c.cpp, lines
+ * cited per entry); the mesh is MeshUtils::TriangleMesh_t (meshutils.h:34-39).
+ * Obstacles enter the solid SDF through FluidSimulation_add_mesh_obstacle
+ * (fluidsimulation_c.cpp:3113-3135); the solid is rebuilt at the next
+ * substep when the list or an object's enabled state changed.  Supported:
+ * static meshes, normal and inversed, friction 0.  Animated meshes and
+ * friction fail with an error. */
+typedef struct MeshObject MeshObject;
+typedef struct TriangleMesh_t {
+    float *vertices;
+    int *triangles;
+    int numVertices;
+    int numTriangles;
+} TriangleMesh_t;
+MeshObject *MeshObject_new(int i, int j, int k, double dx, int *err);                 /* :38-50 */
+void MeshObject_destroy(MeshObject *obj);                                               /* :52-54 */
+void MeshObject_update_mesh_static(MeshObject *obj, TriangleMesh_t mesh, int *err);     /* :56-69 */
+void MeshObject_update_mesh_animated(MeshObject *obj, TriangleMesh_t previous, TriangleMesh_t current,
+                                     TriangleMesh_t next, int *err);                   /* :71-88 */
+void MeshObject_enable(MeshObject *obj, int *err);                                      /* :90-94 */
+void MeshObject_disable(MeshObject *obj, int *err);                                     /* :96-100 */
+int MeshObject_is_enabled(MeshObject *obj, int *err);                                   /* :102-106 */
+void MeshObject_inverse(MeshObject *obj, int *err);                                     /* :108-112 */
+int MeshObject_is_inversed(MeshObject *obj, int *err);                                  /* :114-118 */
+float MeshObject_get_friction(MeshObject *obj, int *err);                               /* :120-124 */
+void MeshObject_set_friction(MeshObject *obj, float f, int *err);                       /* :126-130 */
+float MeshObject_get_velocity_scale(MeshObject *obj, int *err);                         /* :132-136 */
+void MeshObject_set_velocity_scale(MeshObject *obj, float scale, int *err);             /* :138-142 */
+float MeshObject_get_whitewater_influence(MeshObject *obj, int *err);                   /* :144-148 */
+void MeshObject_set_whitewater_influence(MeshObject *obj, float value, int *err);       /* :150-154 */
+float MeshObject_get_dust_emission_strength(MeshObject *obj, int *err);                 /* :156-160 */
+void MeshObject_set_dust_emission_strength(MeshObject *obj, float value, int *err);     /* :162-166 */
+float MeshObject_get_sheeting_strength(MeshObject *obj, int *err);                      /* :168-172 */
+void MeshObject_set_sheeting_strength(MeshObject *obj, float value, int *err);          /* :174-178 */
+float MeshObject_get_mesh_expansion(MeshObject *obj, int *err);                         /* :180-184 */
+void MeshObject_set_mesh_expansion(MeshObject *obj, float ex, int *err);                /* :186-192 */
+void MeshObject_enable_append_object_velocity(MeshObject *obj, int *err);               /* :194-198 */
+void MeshObject_disable_append_object_velocity(MeshObject *obj, int *err);              /* :200-204 */
+int MeshObject_is_append_object_velocity_enabled(MeshObject *obj, int *err);            /* :206-210 */
+float MeshObject_get_object_velocity_influence(MeshObject *obj, int *err);              /* :212-216 */
+void MeshObject_set_object_velocity_influence(MeshObject *obj, float value, int *err);  /* :218-224 */
+int MeshObject_get_priority(MeshObject *obj, int *err);                                 /* :226-230 */
+void MeshObject_set_priority(MeshObject *obj, int n, int *err);                         /* :232-236 */
+int MeshObject_get_source_id(MeshObject *obj, int *err);                                /* :238-242 */
+void MeshObject_set_source_id(MeshObject *obj, int id, int *err);                       /* :244-248 */
+int MeshObject_get_viscosity(MeshObject *obj, int *err);                                /* :250-254 */
+void MeshObject_set_viscosity(MeshObject *obj, float v, int *err);                      /* :256-260 */
+int MeshObject_get_density(MeshObject *obj, int *err);                                  /* :262-266 */
+void MeshObject_set_density(MeshObject *obj, float d, int *err);                        /* :268-272 */
+int MeshObject_get_lifetime(MeshObject *obj, int *err);                                 /* :274-278 */
+void MeshObject_set_lifetime(MeshObject *obj, float v, int *err);                       /* :280-284 */
+int MeshObject_get_lifetime_variance(MeshObject *obj, int *err);                        /* :286-290 */
+void MeshObject_set_lifetime_variance(MeshObject *obj, float v, int *err);              /* :292-296 */
+FFVector3 MeshObject_get_source_color(MeshObject *obj, int *err);                       /* :298-303 */
+void MeshObject_set_source_color(MeshObject *obj, double r, double g, double b, int *err); /* :305-312 */
+void FluidSimulation_add_mesh_obstacle(FluidSimulation *obj, MeshObject *obstacle, int *err);    /* fluidsimulation_c.cpp:3113-3119 */
+void FluidSimulation_remove_mesh_obstacle(FluidSimulation *obj, MeshObject *obstacle, int *err); /* :3121-3127 */
+void FluidSimulation_remove_mesh_obstacles(FluidSimulation *obj, int *err);                      /* :3129-3134 */
+/* rand() values the last solid rebuild drew for its sign tests' jitter
+ * (three per test; no counterpart in the reference, for tests). */
+int FFAMD_get_last_solid_rand_draws(FluidSimulation *obj, int *err);
+
 /* Nodal solid SDF ((isize+1)*(jsize+1)*(ksize+1) floats, i fastest, negative
  * inside solids) replacing the domain-boundary SDF the engine builds itself.
  * Solids are static; their face velocity is zero.  Call before initialize. */
@@ -249,6 +312,19 @@ void FFAMD_extrapolate(int wi, int wj, int wk, float *field, const unsigned char
 void FFAMD_extrapolate_vjp(int wi, int wj, int wk, float *bar, const unsigned char *valid,
                            int layers, int *err);
 void FFAMD_solid_box_phi(int isize, int jsize, int ksize, double dx, float *phi_nodes, int *err);
+/* MeshLevelSet::fastCalculateSignedDistanceField (meshlevelset.cpp:572-601)
+ * of one mesh on a level set of isize x jsize x ksize cells at node offset
+ * (oi, oj, ok): exact band of 3 nodes, crossing-count signs with rand()
+ * jitter.  phi_nodes: (isize+1)(jsize+1)(ksize+1).  Returns the rand() draws. */
+int FFAMD_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi, int oj, int ok, int nv,
+                             const float *vertices, int nt, const int *triangles, float *phi_nodes, int *err);
+/* FluidSimulation::_addStaticObjectsToSDF (fluidsimulation.cpp:5277-5339):
+ * the domain box SDF with nobj static mesh obstacles unioned in (obstacle q:
+ * nv[q] vertices, nt[q] triangles, concatenated, indices local to it;
+ * inversed[q] != 0 for an inversed one).  Returns the rand() draws. */
+int FFAMD_solid_phi_meshes(int isize, int jsize, int ksize, double dx, int nobj, const int *nv,
+                           const float *vertices, const int *nt, const int *triangles, const int *inversed,
+                           float *phi_nodes, int *err);
 void FFAMD_face_weights(int isize, int jsize, int ksize, const float *phi_nodes, float *wu,
                         float *wv, float *ww, int *err);
 /* Pressure solve + projection on host grids; mode as in

@@ -1,0 +1,193 @@
+"""Static mesh obstacles on the GPU against the CPU oracle and the reference.
+
+- the fast mesh level set (exact band, crossing-count signs) and the static
+  solid SDF with normal, inversed, several, border-crossing and 30-island
+  obstacles, bit for bit against the oracle (pinned to the reference build by
+  tests/test_oracle_mesh.py), with rand() seeded identically before each;
+- the engine with a MeshObject obstacle: its solid SDF after the first
+  update equals the oracle's, and whole frames equal the reference engine's
+  (built from its sources) bit for bit when both take the same rand() seed;
+- enable/disable and removal rebuild the solid; animated meshes, friction,
+  duplicate adds and unknown removals raise.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from ffengine import FluidSimulation, MeshObject, TriangleMesh
+from ffengine import kernels as K
+from ffengine import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("n,center,radius,sub,cells,off", [
+    (24, (0.5, 0.45, 0.55), 0.2, 3, None, (0, 0, 0)),
+    (40, (0.13, 0.5, 0.5), 0.3, 3, None, (0, 0, 0)),      # crosses the domain border
+    (32, (0.6, 0.2, 0.5), 0.15, 4, (14, 11, 17), (5, 7, 3)),  # offset island grid, leaves it
+])
+def test_mesh_levelset_matches_oracle(n, center, radius, sub, cells, off):
+    dx = 1.0 / n
+    cells = cells or (n, n, n)
+    v, t = scenes.sphere_mesh(center, radius, sub)
+    seed = 17 + n
+    say(f"mesh level set n={n} cells={cells} off={off} triangles={len(t)}")
+    O.libc_srand(seed)
+    g, gd = K.mesh_levelset_fast(cells, dx, off, v, t)
+    o, od = O.mesh_levelset_fast(cells, dx, off, v, t, 3, seed)
+    assert gd == od
+    assert np.array_equal(bits(g), bits(o))
+
+
+def many_spheres(count, seed):
+    rng = np.random.default_rng(seed)
+    verts, tris, base = [], [], 0
+    for _ in range(count):
+        v, t = scenes.sphere_mesh(rng.random(3) * 0.7 + 0.15, 0.03 + 0.03 * rng.random(), 1)
+        verts.append(v)
+        tris.append(t + base)
+        base += len(v)
+    return np.concatenate(verts), np.concatenate(tris)
+
+
+@pytest.mark.parametrize("case", ["one", "two", "inversed", "mixed", "islands", "border"])
+def test_static_solid_matches_oracle(case):
+    n = 36
+    dx = 1.0 / n
+    a = scenes.sphere_mesh((0.55, 0.3, 0.5), 0.17, 3)
+    b = scenes.sphere_mesh((0.3, 0.7, 0.35), 0.12, 2)
+    c = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.4, 2)
+    d = scenes.sphere_mesh((0.95, 0.5, 0.5), 0.2, 2)
+    sets = {"one": [a], "two": [a, b], "inversed": [(c[0], c[1], True)], "mixed": [b, (c[0], c[1], True), a],
+            "islands": [many_spheres(30, 12)], "border": [d, a]}
+    seed = 40 + len(case)
+    say(f"static solid {case}")
+    O.libc_srand(seed)
+    g, gd = K.solid_phi_meshes((n, n, n), dx, sets[case])
+    o, od = O.solid_phi_meshes((n, n, n), dx, sets[case], seed)
+    assert gd == od
+    assert np.array_equal(bits(g), bits(o))
+
+
+def _sim(n, pos, threads=None):
+    sim = FluidSimulation(n, n, n, 1.0 / n)
+    sim.add_body_force(0.0, -9.81, 0.0)
+    sim.PICFLIP_ratio = 0.05
+    sim.CFL_condition_number = 5
+    sim.enable_surface_reconstruction = False
+    if threads:
+        sim.max_thread_count = threads
+        sim.set_pressure_solver_reference_summation(True)
+    sim.load_marker_particle_data(len(pos), pos.tobytes(), np.zeros_like(pos).tobytes())
+    return sim
+
+
+def _obstacle(n, v, t, inversed=False):
+    mo = MeshObject(n, n, n, 1.0 / n)
+    mo.update_mesh_static(TriangleMesh(v, t))
+    mo.inverse = inversed
+    return mo
+
+
+def _state(sim):
+    m = sim.get_num_marker_particles()
+    p = np.frombuffer(sim.get_marker_particle_position_data_range(0, m), np.float32)
+    v = np.frombuffer(sim.get_marker_particle_velocity_data_range(0, m), np.float32)
+    return p, v
+
+
+def test_engine_solid_with_obstacle_matches_oracle_and_rebuilds():
+    n = 32
+    dx = 1.0 / n
+    pos = scenes.dam_break_particles(n, seed=3)
+    v, t = scenes.sphere_mesh((0.6, 0.25, 0.5), 0.15, 3)
+    sim = _sim(n, pos)
+    mo = _obstacle(n, v, t)
+    sim.add_mesh_obstacle(mo)
+    with pytest.raises(RuntimeError):
+        sim.add_mesh_obstacle(mo)
+    sim.initialize()
+    O.libc_srand(5)
+    sim.update(1.0 / 30.0)
+    expect, draws = O.solid_phi_meshes((n, n, n), dx, [(v, t)], 5)
+    say(f"engine solid with obstacle: draws {sim.get_last_solid_rand_draws()} (oracle {draws})")
+    assert sim.get_last_solid_rand_draws() == draws == 6
+    assert np.array_equal(bits(sim.get_solid_sdf()), bits(expect))
+    # disabling marks the object changed: the next update rebuilds without it
+    mo.enable = False
+    sim.update(1.0 / 30.0)
+    assert sim.get_last_solid_rand_draws() == 3
+    assert np.array_equal(bits(sim.get_solid_sdf()), bits(O.solid_box_phi((n, n, n), dx)))
+    mo.enable = True
+    O.libc_srand(5)
+    sim.update(1.0 / 30.0)
+    assert np.array_equal(bits(sim.get_solid_sdf()), bits(expect))
+    sim.remove_mesh_obstacle(mo)
+    with pytest.raises(RuntimeError):
+        sim.remove_mesh_obstacle(mo)
+    sim.update(1.0 / 30.0)
+    assert np.array_equal(bits(sim.get_solid_sdf()), bits(O.solid_box_phi((n, n, n), dx)))
+
+
+def test_unsupported_obstacles_raise():
+    n = 16
+    v, t = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.2, 1)
+    mo = _obstacle(n, v, t)
+    with pytest.raises(RuntimeError):
+        mo.update_mesh_animated(TriangleMesh(v, t), TriangleMesh(v, t), TriangleMesh(v, t))
+    with pytest.raises(ValueError):
+        mo.friction = 1.5
+    mo.friction = 0.25
+    assert abs(mo.friction - 0.25) < 1e-7
+    sim = _sim(n, scenes.dam_break_particles(n, seed=2))
+    sim.add_mesh_obstacle(mo)
+    sim.initialize()
+    with pytest.raises(RuntimeError):
+        sim.update(1.0 / 30.0)
+    bad = MeshObject(n, n, n, 1.0 / n)
+    with pytest.raises(RuntimeError):
+        bad.update_mesh_static(TriangleMesh(v, np.array([[0, 1, len(v)]], np.int32)))
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref reference build absent")
+@pytest.mark.parametrize("case", ["sphere", "inversed_and_sphere"])
+def test_engine_frames_with_obstacles_match_reference(case):
+    n = 32
+    dx = 1.0 / n
+    pos = scenes.dam_break_particles(n, seed=31)
+    a = scenes.sphere_mesh((0.3, 0.3, 0.5), 0.12, 3)  # overlaps the initial fluid
+    c = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.62, 3)  # inversed: liquid only inside it
+    obstacles = [(a, False)] if case == "sphere" else [(c, True), (a, False)]
+    ref = O.ReferenceSim((n, n, n), dx, threads=4)
+    ref.load(pos)
+    for (v, t), inv in obstacles:
+        ref.add_mesh_obstacle((n, n, n), dx, v, t, inversed=inv)
+    ref.initialize()
+    sim = _sim(n, pos, threads=4)
+    keep = [_obstacle(n, v, t, inv) for (v, t), inv in obstacles]
+    for mo in keep:
+        sim.add_mesh_obstacle(mo)
+    sim.initialize()
+    for frame in range(3):
+        O.libc_srand(1234 + frame)
+        ref.update(1.0 / 30.0)
+        O.libc_srand(1234 + frame)
+        sim.update(1.0 / 30.0)
+        pr, vr = ref.particles()
+        pg, vg = _state(sim)
+        same = len(pr) * 3 == len(pg) and np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32))
+        say(f"obstacles {case} frame {frame}: {len(pr)} / {len(pg) // 3} particles, identical {same}")
+        assert len(pr) * 3 == len(pg), frame
+        assert np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32)), frame
+        assert np.array_equal(vr.reshape(-1).view(np.uint32), vg.view(np.uint32)), frame
