@@ -38,7 +38,8 @@ def log(msg):
 
 
 def make_sim(n, pos, solid, mode, tol, apic=False, warm=False):
-    from ffengine import FluidSimulation
+    """solid: a nodal SDF array, a (vertices, triangles) mesh obstacle, or None."""
+    from ffengine import FluidSimulation, MeshObject, TriangleMesh
     sim = FluidSimulation(n, n, n, 1.0 / n)
     if apic:
         sim.set_velocity_transfer_method_APIC()
@@ -50,11 +51,44 @@ def make_sim(n, pos, solid, mode, tol, apic=False, warm=False):
     sim.enable_surface_reconstruction = False
     sim.pressure_solver_tolerance = tol
     sim.set_pressure_solver_precision(mode)
-    if solid is not None:
+    if isinstance(solid, tuple):
+        mo = MeshObject(n, n, n, 1.0 / n)
+        mo.update_mesh_static(TriangleMesh(*solid))
+        sim.add_mesh_obstacle(mo)
+        sim._bench_obstacle = mo  # the simulation holds the object by pointer
+    elif solid is not None:
         sim.set_solid_phi(solid)
     sim.load_marker_particle_data(len(pos), pos.tobytes(), np.zeros_like(pos).tobytes())
     sim.initialize()
     return sim
+
+
+def mesh_obstacle_sdf(n, mesh, cpu, threads, reps=3):
+    """Build time of the static solid SDF (domain box + the mesh obstacle) at
+    n^3: this engine's C ABI call (host preparation, uploads, kernels and the
+    read-back of the nodal grid), and the reference engine's
+    _addStaticObjectsToSDF on `threads` host cores when its build is present."""
+    import oracle_lib
+    from ffengine import kernels
+    dx = 1.0 / n
+    kernels.solid_phi_meshes((n, n, n), dx, [mesh])  # warm-up: first allocation and module load
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        phi, draws = kernels.solid_phi_meshes((n, n, n), dx, [mesh])
+        times.append(time.perf_counter() - t0)
+    out = {"triangles": int(len(mesh[1])), "gpu_ms": round(1e3 * min(times), 3), "rand_draws": int(draws),
+           "cpu_reference_ms": None, "cpu_threads": None, "identical_to_reference": None}
+    if cpu and oracle_lib.reference_available():
+        oracle_lib.probe().refprobe_set_threads(threads)
+        oracle_lib.libc_srand(7)
+        ref, seconds = oracle_lib.ref_solid_phi_meshes((n, n, n), dx, [mesh], 7, timed=True)
+        oracle_lib.probe().refprobe_set_threads(1)
+        oracle_lib.libc_srand(7)
+        mine, _ = kernels.solid_phi_meshes((n, n, n), dx, [mesh])
+        out.update(cpu_reference_ms=round(1e3 * seconds, 3), cpu_threads=threads,
+                   identical_to_reference=bool(np.array_equal(mine.view(np.uint32), ref.view(np.uint32))))
+    return out
 
 
 def kernel_times(sim):
@@ -172,6 +206,8 @@ def cpu_baseline(n_sample, threads, seconds_budget=30.0):
     pos = scenes.dam_break_particles(n_sample, seed=0)
     sim = oracle_lib.ReferenceSim((n_sample,) * 3, dx, threads=threads)
     sim.load(pos)
+    v, t = scenes.sphere_mesh(subdivisions=5)
+    sim.add_mesh_obstacle((n_sample,) * 3, dx, v, t)
     sim.initialize()
     sim.update(1.0 / 240.0)  # warm-up substep (builds the solid SDF)
     t0 = time.perf_counter()
@@ -183,8 +219,9 @@ def cpu_baseline(n_sample, threads, seconds_budget=30.0):
             break
     el = time.perf_counter() - t0
     return {"value": count / el, "unit": "substeps/s", "cores": threads, "kind": "reference",
-            "sample": "reference engine (oracle/_ref, built from its sources) on the %d^3 dam break "
-                      "without the sphere, %d timed substeps of 1/240 s after one warm-up substep" % (n_sample, count)}
+            "sample": "reference engine (oracle/_ref, built from its sources) on the %d^3 dam break with "
+                      "the same sphere MeshObject (%d triangles), %d timed substeps of 1/240 s after one "
+                      "warm-up substep" % (n_sample, len(t), count)}
 
 
 def precision_config(n, tol):
@@ -403,6 +440,9 @@ def main():
     ap.add_argument("--fd-faces", type=int, default=64, help="adjoint: faces checked by central differences")
     ap.add_argument("--fd-eps", type=float, default=1e-3, help="adjoint: central-difference step (m/s)")
     ap.add_argument("--dt", type=float, default=1.0 / 240.0, help="adjoint: substep length (s)")
+    ap.add_argument("--obstacle", default="mesh", choices=["mesh", "phi"],
+                    help="dam-break: the sphere as a static MeshObject (the reference's obstacle path) or as a "
+                         "nodal SDF set directly")
     ap.add_argument("--transfer", default="flip", choices=["flip", "apic"],
                     help="dam-break: velocity transfer method (the headline uses FLIP)")
     ap.add_argument("--no-warm-extra", action="store_true",
@@ -457,7 +497,11 @@ def main():
     del cal_src, cal_dst
     log("scene n=%d" % n)
     pos = scenes.dam_break_particles(n, seed=0)
-    solid = scenes.union_phi(kernels.solid_box_phi((n, n, n), dx), scenes.sphere_phi_nodes(n))
+    if args.obstacle == "mesh":
+        # the sphere as the reference takes it: a static MeshObject obstacle
+        solid = scenes.sphere_mesh(subdivisions=5)
+    else:
+        solid = scenes.union_phi(kernels.solid_box_phi((n, n, n), dx), scenes.sphere_phi_nodes(n))
     modes = [m for m in args.modes.split(",") if m]
     if not set(modes) <= {"fp64", "mixed", "mixed-reliable", "fp32-preconditioner"} or not modes:
         sys.exit("--modes takes fp64, mixed, mixed-reliable and/or fp32-preconditioner")
@@ -496,6 +540,11 @@ def main():
                 "avg_launch_us": round(kt[dom]["ms"] * 1e3 / launches, 3),
                 "algorithmic_bytes_per_launch": int(bytes_ / launches)}
 
+    mesh_sdf = None
+    if args.obstacle == "mesh":
+        log("mesh obstacle SDF build")
+        mesh_sdf = mesh_obstacle_sdf(n, solid, not args.no_cpu, args.cpu_threads)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline n=%d threads=%d" % (args.cpu_sample_n, args.cpu_threads))
@@ -517,8 +566,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if main_mode != "fp64" else "f64",
         "data": "synthetic",
-        "config": {"workload": "dam break %d^3 + static sphere obstacle, %d particles, frame 1/240 s, "
-                               "PCG rel res %g%s" % (n, len(pos), args.tol,
+        "config": {"workload": "dam break %d^3 + static sphere obstacle (%s), %d particles, frame 1/240 s, "
+                               "PCG rel res %g%s" % (n, "icosphere MeshObject, %d triangles" % len(solid[1])
+                                                     if args.obstacle == "mesh" else "nodal SDF", len(pos), args.tol,
                                                      ", APIC transfer" if args.transfer == "apic" else ""),
                    "grid": n, "particles": len(pos)},
         "mode": main_mode,
@@ -531,6 +581,7 @@ def main():
         "pressure_iterations": {m: results[m]["iterations"] for m in results},
         "kernel_ms": {m: {k: round(v["ms"], 3) for k, v in results[m]["kernels"].items()} for m in results},
         "roofline": roofline,
+        "mesh_obstacle_sdf": mesh_sdf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

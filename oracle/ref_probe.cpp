@@ -126,8 +126,9 @@ void refprobe_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int
 
 /* FluidSimulation::_addStaticObjectsToSDF with static mesh obstacles, after
  * srand(seed) (fluidsimulation.cpp:5277-5339).  The temporary level set is
- * sized to the domain as _updateSolidLevelSet does (5396-5405). */
-void refprobe_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *verts,
+ * sized to the domain as _updateSolidLevelSet does (5396-5405).  Returns the
+ * wall-clock seconds of _addStaticObjectsToSDF alone. */
+double refprobe_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *verts,
                                const int *nt, const int *tris, const int *inversed, unsigned seed, float *out) {
     FluidSimulation sim(ni, nj, nk, dx);
     sim.initialize();
@@ -144,9 +145,12 @@ void refprobe_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, cons
     sim._tempSolidSDF = MeshLevelSet(ni, nj, nk, dx);
     MeshLevelSet sdf(ni, nj, nk, dx);
     srand(seed);
+    auto t0 = std::chrono::steady_clock::now();
     sim._addStaticObjectsToSDF(1.0 / 30.0, sdf);
+    double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     copyOut(*sdf.getPhiArray3d(), out);
     sim.removeMeshObstacles();
+    return seconds;
 }
 
 void refprobe_liquid_sdf(int ni, int nj, int nk, double dx, int np, const float *pos,

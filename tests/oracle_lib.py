@@ -160,13 +160,17 @@ def ref_mesh_levelset_fast(cells3, dx, offset, verts, tris, band, seed):
     return phi
 
 
-def ref_solid_phi_meshes(n3, dx, meshes, seed):
+def ref_solid_phi_meshes(n3, dx, meshes, seed, timed=False):
+    """The reference's static solid SDF (domain box + mesh obstacles) after
+    srand(seed); with timed=True also the seconds of the SDF build alone."""
     ni, nj, nk = n3
     verts, tris, nv, nt, inv = _mesh_arrays(meshes)
     phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
-    probe().refprobe_solid_phi_meshes(ni, nj, nk, C.c_double(dx), len(meshes), fp(nv), fp(verts), fp(nt),
-                                      fp(tris), fp(inv), C.c_uint(seed), fp(phi))
-    return phi
+    f = probe().refprobe_solid_phi_meshes
+    f.restype = C.c_double
+    seconds = f(ni, nj, nk, C.c_double(dx), len(meshes), fp(nv), fp(verts), fp(nt), fp(tris), fp(inv),
+                C.c_uint(seed), fp(phi))
+    return (phi, seconds) if timed else phi
 
 
 def liquid_sdf(n3, dx, pos, radius):
