@@ -409,9 +409,17 @@ void FluidSimulation::update(double dt) {
     if (dt < 0.0)
         throw std::domain_error("Error: delta time must be greater than or equal to 0.\ndelta time: " +
                                 toString(dt) + "\n");
-    hipEvent_t t0, t1;
-    hipCheck(hipEventCreate(&t0), "event");
-    hipCheck(hipEventCreate(&t1), "event");
+    // the frame's two timing events, destroyed also when a substep throws
+    struct FrameEvents {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~FrameEvents() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev;
+    hipCheck(hipEventCreate(&ev.a), "event");
+    hipCheck(hipEventCreate(&ev.b), "event");
+    hipEvent_t t0 = ev.a, t1 = ev.b;
     hipCheck(hipEventRecord(t0, _stream), "event");
     double epsdt = 1e-6;
     bool zeroLen = dt < epsdt;
@@ -449,8 +457,6 @@ void FluidSimulation::update(double dt) {
     hipCheck(hipEventSynchronize(t1), "event");
     float total = 0;
     hipCheck(hipEventElapsedTime(&total, t0, t1), "event");
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
 
     FFFrameStats st{};
     st.frame = _currentFrame;
