@@ -61,7 +61,9 @@ void absMaxf(const float *a, size_t n, double *partial, double *out, hipStream_t
 // over a grid-stride loop, then one block folds the partials in a fixed tree;
 // fused kernels that use the same grid produce the same bits.
 constexpr int REDUCE_THREADS = 256;
-void reduceFinish(const double *partial, int nb, double *out, bool isMax, hipStream_t s);
+// outHost (optional): device alias of mapped pinned host memory that also
+// receives the result
+void reduceFinish(const double *partial, int nb, double *out, bool isMax, hipStream_t s, double *outHost = nullptr);
 
 // ---- setup (kernels_setup.hip) ----
 void solidBoxPhi(Grid g, float *phiNodes, hipStream_t s);
@@ -276,13 +278,14 @@ void axpyFloatStepF(float *y, const float *x, size_t n, const double *num, const
 void spmvDot(const PressureSys &S, const double *s, double *q, double *partial, double *sq, hipStream_t st);
 void spmvDotF(const PressureSys &S, const float *s, float *q, double *partial, double *sq, hipStream_t st);
 // a = (float)(num/den); x += a * s; r -= a * q; *rmax = |r|_inf (the absMax order);
-// rf (optional) receives (float)r
+// rf (optional) receives (float)r; rmaxHost (optional) a mapped host copy of *rmax
 void pcgStep(double *x, const double *s, double *r, const double *q, size_t n, const double *num,
-             const double *den, double *partial, double *rmax, hipStream_t st, float *rf = nullptr);
+             const double *den, double *partial, double *rmax, hipStream_t st, float *rf = nullptr,
+             double *rmaxHost = nullptr);
 void pcgStepF(float *x, const float *s, float *r, const float *q, size_t n, const double *num,
-              const double *den, double *partial, double *rmax, hipStream_t st);
+              const double *den, double *partial, double *rmax, hipStream_t st, double *rmaxHost = nullptr);
 void pcgStepMixed(double *x, const float *s, float *r, const float *q, size_t n, const double *num,
-                  const double *den, double *partial, double *rmax, hipStream_t st);
+                  const double *den, double *partial, double *rmax, hipStream_t st, double *rmaxHost = nullptr);
 // z = (double)zf + (double)(float)(num/den) * s   (FP32 zf, FP64 z and s)
 void widenAxpyStep(double *z, const float *zf, const double *s, size_t n, const double *num, const double *den,
                    hipStream_t st);

@@ -475,24 +475,25 @@ void spmvDotF(const PressureSys &S, const float *x, float *y, double *partial, d
 
 template <class TX, class T>
 static void launchPcgStep(TX *x, const T *s, T *r, const T *q, size_t n, const double *num, const double *den,
-                          double *partial, double *rmax, hipStream_t st, float *rf = nullptr) {
+                          double *partial, double *rmax, hipStream_t st, float *rf = nullptr,
+                          double *rmaxHost = nullptr) {
     if (n == 0) return;
     const int nb = reduceBlocks(n);
     hipLaunchKernelGGL((pcgStepKernel<TX, T>), dim3(nb), dim3(REDUCE_THREADS), 0, st, x, s, r, q, n, num, den,
                        partial, rf);
-    reduceFinish(partial, nb, rmax, true, st);
+    reduceFinish(partial, nb, rmax, true, st, rmaxHost);
 }
 void pcgStep(double *x, const double *s, double *r, const double *q, size_t n, const double *num,
-             const double *den, double *partial, double *rmax, hipStream_t st, float *rf) {
-    launchPcgStep<double, double>(x, s, r, q, n, num, den, partial, rmax, st, rf);
+             const double *den, double *partial, double *rmax, hipStream_t st, float *rf, double *rmaxHost) {
+    launchPcgStep<double, double>(x, s, r, q, n, num, den, partial, rmax, st, rf, rmaxHost);
 }
 void pcgStepF(float *x, const float *s, float *r, const float *q, size_t n, const double *num,
-              const double *den, double *partial, double *rmax, hipStream_t st) {
-    launchPcgStep<float, float>(x, s, r, q, n, num, den, partial, rmax, st);
+              const double *den, double *partial, double *rmax, hipStream_t st, double *rmaxHost) {
+    launchPcgStep<float, float>(x, s, r, q, n, num, den, partial, rmax, st, nullptr, rmaxHost);
 }
 void pcgStepMixed(double *x, const float *s, float *r, const float *q, size_t n, const double *num,
-                  const double *den, double *partial, double *rmax, hipStream_t st) {
-    launchPcgStep<double, float>(x, s, r, q, n, num, den, partial, rmax, st);
+                  const double *den, double *partial, double *rmax, hipStream_t st, double *rmaxHost) {
+    launchPcgStep<double, float>(x, s, r, q, n, num, den, partial, rmax, st, nullptr, rmaxHost);
 }
 
 void widenAxpyStep(double *z, const float *zf, const double *s, size_t n, const double *num, const double *den,

@@ -130,9 +130,11 @@ __global__ __launch_bounds__(RED_THREADS) void reduceStage1(const T *a, const TB
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
+// outHost (optional): a mapped pinned host copy of the result, which the
+// host reads after a stream synchronize instead of copying it back
 template <int OP>
 __global__ __launch_bounds__(RED_THREADS) void reduceStage2(const double *partial, int nb,
-                                                            double *out) {
+                                                            double *out, double *outHost) {
     __shared__ double s[RED_THREADS];
     double acc = 0.0;
     for (int i = threadIdx.x; i < nb; i += RED_THREADS) {
@@ -152,23 +154,26 @@ __global__ __launch_bounds__(RED_THREADS) void reduceStage2(const double *partia
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = s[0];
+    if (threadIdx.x == 0) {
+        *out = s[0];
+        if (outHost) *outHost = s[0];
+    }
 }
 
 template <class T, int OP, class TB = T>
 void reduce(const T *a, const TB *b, size_t n, double *partial, double *out, hipStream_t s) {
     int nb = reduceBlocks(n);
     hipLaunchKernelGGL((reduceStage1<T, OP, TB>), dim3(nb), dim3(RED_THREADS), 0, s, a, b, n, partial);
-    hipLaunchKernelGGL((reduceStage2<OP>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+    hipLaunchKernelGGL((reduceStage2<OP>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out, nullptr);
 }
 
 }  // namespace
 
-void reduceFinish(const double *partial, int nb, double *out, bool isMax, hipStream_t s) {
+void reduceFinish(const double *partial, int nb, double *out, bool isMax, hipStream_t s, double *outHost) {
     if (isMax)
-        hipLaunchKernelGGL((reduceStage2<1>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+        hipLaunchKernelGGL((reduceStage2<1>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out, outHost);
     else
-        hipLaunchKernelGGL((reduceStage2<0>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out);
+        hipLaunchKernelGGL((reduceStage2<0>), dim3(1), dim3(RED_THREADS), 0, s, partial, nb, out, outHost);
 }
 
 void dot(const double *a, const double *b, size_t n, double *partial, double *out, hipStream_t s) {

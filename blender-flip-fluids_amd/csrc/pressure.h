@@ -68,6 +68,7 @@ private:
     double *_x = nullptr, *_r = nullptr, *_z = nullptr, *_s = nullptr, *_y = nullptr;
     float *_fr = nullptr, *_fz = nullptr, *_fs = nullptr, *_fd = nullptr, *_fq = nullptr, *_fy = nullptr;
     double *_partial = nullptr, *_scal = nullptr;
+    double *_hostScal = nullptr, *_hostScalDev = nullptr;  // mapped pinned host slot and its device alias
     std::vector<int> _hLevelStart;
     MicSchedule _ms;
     int _maxTiles = 0;
@@ -86,6 +87,7 @@ private:
     void initialGuess(hipStream_t st);
 
     double hostScalar(const double *d, hipStream_t s);
+    double mappedResidual(hipStream_t s);
     void enqueueTileRanges(bool fp64, hipStream_t s);
     // tile schedule of the preconditioner application for this system
     void buildSchedule(bool fp64, hipStream_t s);
