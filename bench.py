@@ -535,6 +535,13 @@ def main():
     if dom == "mic0_apply":
         kname = band_kernel_name(main_mode == "fp64")
         traffic = measured_traffic(kname)
+    # algorithmic HBM bytes / time of every kernel group that has a byte model
+    # (SpMV, the transfers, the preconditioner, ...), as a fraction of 8 TB/s
+    hbm_fraction = {}
+    for k, v in kt.items():
+        b = algorithmic_bytes(k, n, P, M_total, 1, head["substeps"], sum(iters), main_mode)
+        if b is not None and v["ms"] > 0:
+            hbm_fraction[k] = round(b / (v["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "avg_launch_us": round(kt[dom]["ms"] * 1e3 / launches, 3),
@@ -581,6 +588,7 @@ def main():
         "pressure_iterations": {m: results[m]["iterations"] for m in results},
         "kernel_ms": {m: {k: round(v["ms"], 3) for k, v in results[m]["kernels"].items()} for m in results},
         "roofline": roofline,
+        "hbm_fraction_by_group": hbm_fraction,
         "mesh_obstacle_sdf": mesh_sdf,
         "cpu_baseline": cpu,
     }

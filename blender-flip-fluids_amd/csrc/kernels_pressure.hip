@@ -216,16 +216,21 @@ __global__ void backwardLevelKernel(const int *nbr, const T *invdiag, const T *l
 // (A x)[r] with the terms added in ascending column order: -z, -y, -x, diag,
 // +x, +y, +z.  Every load is issued up front (an absent neighbour reads x[r]
 // and its term is skipped by a select), so the row costs one round trip.
+// The -x/-y/-z coefficients are read as the neighbour rows' +x/+y/+z ones:
+// the assembly computes both from the same face by the same expression, so
+// they are the same bits, and those rows were just read by nearby threads
+// (mx, my, mz are kept for the other users of the matrix).
 template <class T>
 __device__ inline T rowProduct(size_t r, const int *nbr, const T *diag, const T *ox, const T *oy, const T *oz,
-                               const T *mx, const T *my, const T *mz, const T *x) {
+                               const T *, const T *, const T *, const T *x) {
     const int *nb = nbr + 6 * r;
     const int n0 = nb[0], n1 = nb[1], n2 = nb[2], n3 = nb[3], n4 = nb[4], n5 = nb[5];
-    const T x0 = x[n0 >= 0 ? (size_t)n0 : r], x1 = x[n1 >= 0 ? (size_t)n1 : r];
-    const T x2 = x[n2 >= 0 ? (size_t)n2 : r], x3 = x[n3 >= 0 ? (size_t)n3 : r];
-    const T x4 = x[n4 >= 0 ? (size_t)n4 : r], x5 = x[n5 >= 0 ? (size_t)n5 : r];
+    const size_t a0 = n0 >= 0 ? (size_t)n0 : r, a2 = n2 >= 0 ? (size_t)n2 : r, a4 = n4 >= 0 ? (size_t)n4 : r;
+    const T x0 = x[a0], x1 = x[n1 >= 0 ? (size_t)n1 : r];
+    const T x2 = x[a2], x3 = x[n3 >= 0 ? (size_t)n3 : r];
+    const T x4 = x[a4], x5 = x[n5 >= 0 ? (size_t)n5 : r];
     const T xr = x[r];
-    const T cmz = mz[r], cmy = my[r], cmx = mx[r], cd = diag[r], cox = ox[r], coy = oy[r], coz = oz[r];
+    const T cmz = oz[a4], cmy = oy[a2], cmx = ox[a0], cd = diag[r], cox = ox[r], coy = oy[r], coz = oz[r];
     T s = 0;
     if (n4 >= 0) s += cmz * x4;
     if (n2 >= 0) s += cmy * x2;
