@@ -208,6 +208,8 @@ private:
     std::vector<float> _solidHost;
     std::vector<MeshObstacle *> _obstacles;
     bool _solidUpToDate = false;
+    bool _frictionActive = false;  // an enabled obstacle has friction: _nodeFric holds the closest-object friction
+    DevBuf<float> _nodeFric;
     int _solidDraws = 0;
     void updateSolidLevelSet();
 

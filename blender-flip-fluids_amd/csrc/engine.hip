@@ -184,20 +184,24 @@ void FluidSimulation::updateSolidLevelSet() {
     }
     if (_solidUpToDate) return;
     std::vector<ObstacleMesh> meshes;
+    bool friction = false;
     for (MeshObstacle *o : _obstacles) {
         if (!o->enabled) continue;
-        if (o->friction != 0.0f)
-            throw std::runtime_error("Error: mesh obstacles with friction are not supported by this engine.\n");
+        friction |= o->friction != 0.0f;
         meshes.push_back(ObstacleMesh{o->vertices.data(), (int)(o->vertices.size() / 3), o->triangles.data(),
-                                      (int)(o->triangles.size() / 3), o->inversed, o->meshExpansion});
+                                      (int)(o->triangles.size() / 3), o->inversed, o->meshExpansion, o->friction});
     }
+    // the closest-object friction grid, kept only while an obstacle has friction
+    _frictionActive = friction;
+    if (friction && _nodeFric.n == 0) _nodeFric.alloc((size_t)(_g.ni + 1) * (_g.nj + 1) * (_g.nk + 1));
     if (_solidGiven)
         hipCheck(hipMemcpyAsync(_solid, _solidHost.data(), sizeof(float) * _solidHost.size(), hipMemcpyHostToDevice,
                                 _stream),
                  "upload solid SDF");
     else
         solidBoxPhi(_g, _solid, _stream);
-    _solidDraws = meshObstacleSolid(_g, meshes.data(), (int)meshes.size(), _solid, _stream);
+    _solidDraws = meshObstacleSolid(_g, meshes.data(), (int)meshes.size(), _solid,
+                                    _frictionActive ? _nodeFric.p : nullptr, _stream);
     faceWeights(_g, _solid, _wu, _wv, _ww, _stream);
     int layers = (int)std::ceil((float)_cfl / (float)3);
     nearSolidGrid(_g, _solid, _near, _nearTmp, _ci, _cj, _ck, layers, _stream);
@@ -565,13 +569,20 @@ void FluidSimulation::stepFluid(double dt) {
         _pErr = (float)_lastErr;
     }
     extrapolate(layers);
-    // constrain saved then current (6444-6459)
-    constrainFaces(_su, _wu, nu, _stream);
-    constrainFaces(_sv, _wv, nv, _stream);
-    constrainFaces(_sw, _ww, nw, _stream);
-    constrainFaces(_u, _wu, nu, _stream);
-    constrainFaces(_v, _wv, nv, _stream);
-    constrainFaces(_w, _ww, nw, _stream);
+    // constrain saved then current (6444-6459); with obstacle friction the
+    // partly open faces blend toward the (zero) solid velocity
+    if (_frictionActive) {
+        float *faces[6] = {_su, _sv, _sw, _u, _v, _w};
+        const float *wts[3] = {_wu, _wv, _ww};
+        for (int q = 0; q < 6; q++) constrainFacesFriction(_g, q % 3, faces[q], wts[q % 3], _nodeFric, _stream);
+    } else {
+        constrainFaces(_su, _wu, nu, _stream);
+        constrainFaces(_sv, _wv, nv, _stream);
+        constrainFaces(_sw, _ww, nw, _stream);
+        constrainFaces(_u, _wu, nu, _stream);
+        constrainFaces(_v, _wv, nv, _stream);
+        constrainFaces(_w, _ww, nw, _stream);
+    }
     hipCheck(hipEventRecord(_ev[5], _stream), "event");
     // G2P (6932-6945) + RK3 advection and collision (7853-7890)
     _timers.begin(T_G2P_ADVECT, _stream);

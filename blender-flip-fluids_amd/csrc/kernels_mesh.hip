@@ -166,18 +166,31 @@ __global__ void negateInsideKernel(float *phi, const uint8_t *inside, size_t n) 
     if (i < n && inside[i]) phi[i] = -phi[i];
 }
 
-// MeshLevelSet::calculateUnion, phi part (meshlevelset.cpp:1759-1794):
-// dst = min(dst, [-]src) over src's nodes at offset o
+// MeshLevelSet::calculateUnion (meshlevelset.cpp:1759-1794): dst = min(dst,
+// [-]src) over src's nodes at offset o.  dstFric (optional) follows the
+// closest mesh object as the friction of that object: a node takes the
+// source's object when the source wins with a smaller magnitude and has a
+// closest triangle there.  The source's object is given by srcFricArr (-1 =
+// none) or, for a mesh level set, by srcFric where the node lies in the
+// exact band (|phi| below the set's upper bound srcUb).
 __global__ void unionKernel(const float *src, int qi, int qj, int qk, I3 o, bool negate, float *dst, int pi, int pj,
-                            int pk) {
+                            int pk, float *dstFric, const float *srcFricArr, float srcFric, float srcUb) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)qi * qj * qk) return;
     const int i = (int)(idx % qi), j = (int)((idx / qi) % qj), k = (int)(idx / ((size_t)qi * qj));
     const int a = i + o.i, b = j + o.j, c = k + o.k;
     if (!inRange(a, b, c, pi, pj, pk)) return;
-    const float v = negate ? -src[idx] : src[idx];
-    float &d = dst[flat(a, b, c, pi, pj)];
-    if (v < d) d = v;
+    const float raw = src[idx];
+    const float v = negate ? -raw : raw;
+    const size_t t = flat(a, b, c, pi, pj);
+    float &d = dst[t];
+    if (v < d) {
+        if (dstFric) {
+            const float sf = srcFricArr ? srcFricArr[idx] : (fabsf(raw) < srcUb ? srcFric : -1.0f);
+            if (sf >= 0.0f && fabsf(v) < fabsf(d)) dstFric[t] = sf;
+        }
+        d = v;
+    }
 }
 
 unsigned blocksFor(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -458,8 +471,9 @@ HMesh hostMesh(const ObstacleMesh &o) {
 }
 
 // MeshObject::getMeshLevelSet for a static mesh, unioned into levelset
-// (meshobject.cpp:239-264, 633-701, 738-839)
-void obstacleLevelSet(const ObstacleMesh &o, Grid g, RandJitter &rj, float *levelset, hipStream_t s) {
+// (meshobject.cpp:239-264, 633-701, 738-839); fric (optional) tracks the
+// closest object's friction beside it
+void obstacleLevelSet(const ObstacleMesh &o, Grid g, RandJitter &rj, float *levelset, float *fric, hipStream_t s) {
     HMesh m = hostMesh(o);
     // getMesh(frameInterpolation): v + f (v - v), which turns -0 into +0
     for (V3 &p : m.v) p = add(p, V3{0.0f, 0.0f, 0.0f});
@@ -507,8 +521,10 @@ void obstacleLevelSet(const ObstacleMesh &o, Grid g, RandJitter &rj, float *leve
         DevArray<float> phi(nn);
         // fracture path: per-island level sets with multi-threading off (meshobject.cpp:798-839)
         levelSetFast(part, w, h, d, g0, g.dx, !fracture, rj, phi.p, s);
+        const float ub = (float)(((w + 1) + (h + 1) + (d + 1)) * g.dx);  // the island set's upper bound
         hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, phi.p, w + 1, h + 1, d + 1, g0,
-                           false, levelset, g.ni + 1, g.nj + 1, g.nk + 1);
+                           false, levelset, g.ni + 1, g.nj + 1, g.nk + 1, fric, (const float *)nullptr, o.friction,
+                           ub);
         hipCheck(hipStreamSynchronize(s), "obstacle union");
     }
 }
@@ -523,24 +539,30 @@ int meshLevelSetFastDevice(Grid cells, I3 offset, const ObstacleMesh &mesh, floa
     return rj.draws;
 }
 
-int meshObstacleSolid(Grid g, const ObstacleMesh *obs, int n, float *solid, hipStream_t s) {
+int meshObstacleSolid(Grid g, const ObstacleMesh *obs, int n, float *solid, float *nodeFriction, hipStream_t s) {
     RandJitter rj;
     rj.next(g.dx);  // the domain box's own sign test draws first (fluidsimulation.cpp:5277-5287)
     const size_t nn = (size_t)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
+    // the domain object (no friction) or none: friction 0
+    if (nodeFriction)
+        hipLaunchKernelGGL(fillFloatKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, nodeFriction, nn, 0.0f);
     for (int q = 0; q < n; q++)
-        if (!obs[q].inversed) obstacleLevelSet(obs[q], g, rj, solid, s);
+        if (!obs[q].inversed) obstacleLevelSet(obs[q], g, rj, solid, nodeFriction, s);
     bool anyInversed = false;
     for (int q = 0; q < n; q++) anyInversed |= obs[q].inversed;
     if (anyInversed) {
         // union of the inversed obstacles' level sets, negated, then unioned
         // into the solid (fluidsimulation.cpp:5321-5338)
-        DevArray<float> inv(nn);
+        DevArray<float> inv(nn), invFric(nodeFriction ? nn : 0);
         const float ub = (float)(((g.ni + 1) + (g.nj + 1) + (g.nk + 1)) * g.dx);
         hipLaunchKernelGGL(fillFloatKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, inv.p, nn, ub);
+        if (nodeFriction)
+            hipLaunchKernelGGL(fillFloatKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, invFric.p, nn, -1.0f);
         for (int q = 0; q < n; q++)
-            if (obs[q].inversed) obstacleLevelSet(obs[q], g, rj, inv.p, s);
+            if (obs[q].inversed) obstacleLevelSet(obs[q], g, rj, inv.p, invFric.p, s);
         hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, inv.p, g.ni + 1, g.nj + 1,
-                           g.nk + 1, I3{0, 0, 0}, true, solid, g.ni + 1, g.nj + 1, g.nk + 1);
+                           g.nk + 1, I3{0, 0, 0}, true, solid, g.ni + 1, g.nj + 1, g.nk + 1, nodeFriction,
+                           (const float *)invFric.p, 0.0f, 0.0f);
         hipCheck(hipStreamSynchronize(s), "inversed obstacles");
     }
     hipCheck(hipGetLastError(), "mesh obstacle kernels");

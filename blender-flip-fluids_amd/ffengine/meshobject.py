@@ -3,8 +3,7 @@
 Same class, property names, validation and errors as the reference's
 src/engine/ffengine/meshobject.py over the MeshObject C ABI
 (c_bindings/meshobject_c.cpp).  This engine takes static meshes, normal or
-inversed, without friction; update_mesh_animated and an enabled obstacle
-with friction raise RuntimeError.
+inversed, with friction; update_mesh_animated raises RuntimeError.
 """
 from ctypes import c_double, c_float, c_int, c_void_p
 

@@ -206,8 +206,9 @@ int FluidSimulation_get_pressure_solver_warm_start(FluidSimulation *obj, int *er
  * Obstacles enter the solid SDF through FluidSimulation_add_mesh_obstacle
  * (fluidsimulation_c.cpp:3113-3135); the solid is rebuilt at the next
  * substep when the list or an object's enabled state changed.  Supported:
- * static meshes, normal and inversed, friction 0.  Animated meshes and
- * friction fail with an error. */
+ * static meshes, normal and inversed, with friction (the velocity
+ * constraint's closest-object friction).  Animated meshes fail with an
+ * error. */
 typedef struct MeshObject MeshObject;
 typedef struct TriangleMesh_t {
     float *vertices;

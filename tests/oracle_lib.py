@@ -415,8 +415,8 @@ class ReferenceSim:
         err = C.c_int()
         self.L.FluidSimulation_initialize(self.h, C.byref(err))
 
-    def add_mesh_obstacle(self, n3, dx, verts, tris, inversed=False):
-        """MeshObject_new + update_mesh_static (+ inverse), then
+    def add_mesh_obstacle(self, n3, dx, verts, tris, inversed=False, friction=0.0):
+        """MeshObject_new + update_mesh_static (+ inverse, friction), then
         FluidSimulation_add_mesh_obstacle (meshobject_c.cpp:38-69)."""
         L = self.L
         err = C.c_int()
@@ -432,6 +432,9 @@ class ReferenceSim:
         L.MeshObject_update_mesh_static(obj, TriangleMesh_t(fp(v), fp(t), len(v), len(t)), C.byref(err))
         if inversed:
             L.MeshObject_inverse(obj, C.byref(err))
+        if friction:
+            L.MeshObject_set_friction.argtypes = [C.c_void_p, C.c_float, C.c_void_p]
+            L.MeshObject_set_friction(obj, friction, C.byref(err))
         L.FluidSimulation_add_mesh_obstacle(self.h, obj, C.byref(err))
         assert err.value == 1
         self._keep.append(obj)

@@ -7,8 +7,10 @@
 - the engine with a MeshObject obstacle: its solid SDF after the first
   update equals the oracle's, and whole frames equal the reference engine's
   (built from its sources) bit for bit when both take the same rand() seed;
-- enable/disable and removal rebuild the solid; animated meshes, friction,
-  duplicate adds and unknown removals raise.
+- enable/disable and removal rebuild the solid; animated meshes, duplicate
+  adds and unknown removals raise;
+- obstacle friction (the constraint's closest-object friction on partly open
+  faces): frames equal the reference engine's bit for bit.
 """
 import sys
 
@@ -93,10 +95,11 @@ def _sim(n, pos, threads=None):
     return sim
 
 
-def _obstacle(n, v, t, inversed=False):
+def _obstacle(n, v, t, inversed=False, friction=0.0):
     mo = MeshObject(n, n, n, 1.0 / n)
     mo.update_mesh_static(TriangleMesh(v, t))
     mo.inverse = inversed
+    mo.friction = friction
     return mo
 
 
@@ -150,32 +153,29 @@ def test_unsupported_obstacles_raise():
         mo.friction = 1.5
     mo.friction = 0.25
     assert abs(mo.friction - 0.25) < 1e-7
-    sim = _sim(n, scenes.dam_break_particles(n, seed=2))
-    sim.add_mesh_obstacle(mo)
-    sim.initialize()
-    with pytest.raises(RuntimeError):
-        sim.update(1.0 / 30.0)
     bad = MeshObject(n, n, n, 1.0 / n)
     with pytest.raises(RuntimeError):
         bad.update_mesh_static(TriangleMesh(v, np.array([[0, 1, len(v)]], np.int32)))
 
 
 @pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref reference build absent")
-@pytest.mark.parametrize("case", ["sphere", "inversed_and_sphere"])
+@pytest.mark.parametrize("case", ["sphere", "inversed_and_sphere", "friction", "inversed_and_sphere_friction"])
 def test_engine_frames_with_obstacles_match_reference(case):
     n = 32
     dx = 1.0 / n
     pos = scenes.dam_break_particles(n, seed=31)
     a = scenes.sphere_mesh((0.3, 0.3, 0.5), 0.12, 3)  # overlaps the initial fluid
     c = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.62, 3)  # inversed: liquid only inside it
-    obstacles = [(a, False)] if case == "sphere" else [(c, True), (a, False)]
+    obstacles = {"sphere": [(a, False, 0.0)], "inversed_and_sphere": [(c, True, 0.0), (a, False, 0.0)],
+                 "friction": [(a, False, 0.5)],
+                 "inversed_and_sphere_friction": [(c, True, 0.3), (a, False, 0.8)]}[case]
     ref = O.ReferenceSim((n, n, n), dx, threads=4)
     ref.load(pos)
-    for (v, t), inv in obstacles:
-        ref.add_mesh_obstacle((n, n, n), dx, v, t, inversed=inv)
+    for (v, t), inv, fr in obstacles:
+        ref.add_mesh_obstacle((n, n, n), dx, v, t, inversed=inv, friction=fr)
     ref.initialize()
     sim = _sim(n, pos, threads=4)
-    keep = [_obstacle(n, v, t, inv) for (v, t), inv in obstacles]
+    keep = [_obstacle(n, v, t, inv, fr) for (v, t), inv, fr in obstacles]
     for mo in keep:
         sim.add_mesh_obstacle(mo)
     sim.initialize()
