@@ -407,6 +407,12 @@ void FluidSimulation_remove_mesh_obstacle(FluidSimulation *o, MeshObject *obstac
 void FluidSimulation_remove_mesh_obstacles(FluidSimulation *o, int *err) {
     guarded(err, [&] { S(o)->removeMeshObstacles(); });
 }
+void FluidSimulation_set_boundary_friction(FluidSimulation *o, double f, int *err) {
+    guarded(err, [&] { S(o)->setBoundaryFriction(f); });
+}
+double FluidSimulation_get_boundary_friction(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getBoundaryFriction(); });
+}
 int FFAMD_get_last_solid_rand_draws(FluidSimulation *o, int *err) {
     return guarded(err, [&] { return S(o)->getLastSolidRandDraws(); });
 }
@@ -591,7 +597,7 @@ int FFAMD_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi,
         Scratch sc;
         size_t nn = (size_t)(isize + 1) * (jsize + 1) * (ksize + 1);
         DBuf<float> d(nn);
-        ff::ObstacleMesh m{vertices, nv, triangles, nt, false, 0.0f, 0.0f};
+        ff::ObstacleMesh m{vertices, nv, triangles, nt, false, 0.0f};
         int draws = ff::meshLevelSetFastDevice(ff::Grid{isize, jsize, ksize, dx}, ff::I3{oi, oj, ok}, m, d, sc.s);
         down(phi, d.p, nn, sc.s);
         sync(sc.s);
@@ -611,12 +617,12 @@ int FFAMD_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const in
         size_t vo = 0, to = 0;
         for (int q = 0; q < nobj; q++) {
             meshes.push_back(ff::ObstacleMesh{vertices + 3 * vo, nv[q], triangles + 3 * to, nt[q], inversed[q] != 0,
-                                              0.0f, 0.0f});
+                                              0.0f});
             vo += (size_t)nv[q];
             to += (size_t)nt[q];
         }
         ff::solidBoxPhi(ff::Grid{ni, nj, nk, dx}, d, sc.s);
-        int draws = ff::meshObstacleSolid(ff::Grid{ni, nj, nk, dx}, meshes.data(), nobj, d, nullptr, sc.s);
+        int draws = ff::meshObstacleSolid(ff::Grid{ni, nj, nk, dx}, meshes.data(), nobj, d, nullptr, false, sc.s);
         down(phi, d.p, nn, sc.s);
         sync(sc.s);
         return draws;

@@ -146,6 +146,10 @@ public:
     void addMeshObstacle(MeshObstacle *obstacle);
     void removeMeshObstacle(MeshObstacle *obstacle);
     void removeMeshObstacles();
+    // FluidSimulation::setBoundaryFriction (fluidsimulation.cpp:2675-2691):
+    // the domain object's friction
+    void setBoundaryFriction(double f);
+    double getBoundaryFriction() const { return _boundaryFriction; }
     // rand() draws of the last solid SDF rebuild (the sign tests' jitter)
     int getLastSolidRandDraws() const { return _solidDraws; }
 
@@ -208,8 +212,16 @@ private:
     std::vector<float> _solidHost;
     std::vector<MeshObstacle *> _obstacles;
     bool _solidUpToDate = false;
-    bool _frictionActive = false;  // an enabled obstacle has friction: _nodeFric holds the closest-object friction
-    DevBuf<float> _nodeFric;
+    // closest mesh object of each solid node (-1 none, 0 domain, q + 1 the
+    // q-th enabled obstacle of the last rebuild) and the objects' friction,
+    // read live each substep as the reference reads getFriction()
+    DevBuf<int> _nodeObj;
+    DevBuf<float> _objFric;
+    std::vector<MeshObstacle *> _solidObjects;
+    std::vector<float> _objFricHost;
+    bool _frictionActive = false;
+    double _boundaryFriction = 0.0;
+    void refreshFriction();
     int _solidDraws = 0;
     void updateSolidLevelSet();
 

@@ -184,29 +184,51 @@ void FluidSimulation::updateSolidLevelSet() {
     }
     if (_solidUpToDate) return;
     std::vector<ObstacleMesh> meshes;
-    bool friction = false;
+    _solidObjects.clear();
     for (MeshObstacle *o : _obstacles) {
         if (!o->enabled) continue;
-        friction |= o->friction != 0.0f;
+        _solidObjects.push_back(o);
         meshes.push_back(ObstacleMesh{o->vertices.data(), (int)(o->vertices.size() / 3), o->triangles.data(),
-                                      (int)(o->triangles.size() / 3), o->inversed, o->meshExpansion, o->friction});
+                                      (int)(o->triangles.size() / 3), o->inversed, o->meshExpansion});
     }
-    // the closest-object friction grid, kept only while an obstacle has friction
-    _frictionActive = friction;
-    if (friction && _nodeFric.n == 0) _nodeFric.alloc((size_t)(_g.ni + 1) * (_g.nj + 1) * (_g.nk + 1));
+    if (_nodeObj.n == 0) _nodeObj.alloc((size_t)(_g.ni + 1) * (_g.nj + 1) * (_g.nk + 1));
     if (_solidGiven)
         hipCheck(hipMemcpyAsync(_solid, _solidHost.data(), sizeof(float) * _solidHost.size(), hipMemcpyHostToDevice,
                                 _stream),
                  "upload solid SDF");
     else
         solidBoxPhi(_g, _solid, _stream);
-    _solidDraws = meshObstacleSolid(_g, meshes.data(), (int)meshes.size(), _solid,
-                                    _frictionActive ? _nodeFric.p : nullptr, _stream);
+    // the domain object holds the domain SDF's band unless a solid SDF was set
+    _solidDraws = meshObstacleSolid(_g, meshes.data(), (int)meshes.size(), _solid, _nodeObj.p, !_solidGiven, _stream);
+    _objFricHost.clear();  // re-upload the table for the new object numbering
     faceWeights(_g, _solid, _wu, _wv, _ww, _stream);
     int layers = (int)std::ceil((float)_cfl / (float)3);
     nearSolidGrid(_g, _solid, _near, _nearTmp, _ci, _cj, _ck, layers, _stream);
     hipCheck(hipStreamSynchronize(_stream), "solid level set");
     _solidUpToDate = true;
+}
+
+// the objects' friction table: [0] the domain object, [q + 1] the q-th
+// enabled obstacle of the last rebuild; uploaded when a value changed
+void FluidSimulation::refreshFriction() {
+    std::vector<float> table(1 + _solidObjects.size());
+    table[0] = _solidGiven ? 0.0f : (float)_boundaryFriction;
+    for (size_t q = 0; q < _solidObjects.size(); q++) table[q + 1] = _solidObjects[q]->friction;
+    _frictionActive = false;
+    for (float f : table) _frictionActive |= f != 0.0f;
+    if (table == _objFricHost) return;
+    if (_objFric.n < table.size()) _objFric.alloc(std::max<size_t>(table.size(), 16));
+    _objFricHost = table;
+    hipCheck(hipMemcpyAsync(_objFric.p, _objFricHost.data(), sizeof(float) * table.size(), hipMemcpyHostToDevice,
+                            _stream),
+             "upload friction table");
+}
+
+void FluidSimulation::setBoundaryFriction(double f) {
+    if (f < 0.0 || f > 1.0)
+        throw std::domain_error("Error: boundary friction must be in range [0.0, 1.0].\nfriction: " + toString(f) +
+                                "\n");
+    _boundaryFriction = f;
 }
 
 void FluidSimulation::requireInitialized(const char *what) const {
@@ -518,6 +540,7 @@ void FluidSimulation::stepFluid(double dt) {
     const int layers = (int)std::ceil(std::sqrt(3.0) * _cfl) + 3;  // :6282-6286
     if (_P.n == 0) return;
     updateSolidLevelSet();
+    refreshFriction();
     hipCheck(hipEventRecord(_ev[0], _stream), "event");
     // liquid level set (5586-5617)
     _timers.begin(T_CELLSORT, _stream, 5);
@@ -574,7 +597,8 @@ void FluidSimulation::stepFluid(double dt) {
     if (_frictionActive) {
         float *faces[6] = {_su, _sv, _sw, _u, _v, _w};
         const float *wts[3] = {_wu, _wv, _ww};
-        for (int q = 0; q < 6; q++) constrainFacesFriction(_g, q % 3, faces[q], wts[q % 3], _nodeFric, _stream);
+        for (int q = 0; q < 6; q++)
+            constrainFacesFriction(_g, q % 3, faces[q], wts[q % 3], _nodeObj, _objFric, _stream);
     } else {
         constrainFaces(_su, _wu, nu, _stream);
         constrainFaces(_sv, _wv, nv, _stream);

@@ -25,17 +25,17 @@ struct ObstacleMesh {
     int numTriangles;
     bool inversed;
     float meshExpansion;  // used by the 25-or-more-island path only, which rejects non-zero values
-    float friction;       // MeshObject::getFriction, in [0, 1]
 };
 // Unions the obstacles' signed distance level sets into the nodal solid SDF
 // `solid` (device, (ni+1)(nj+1)(nk+1)), which holds the domain SDF: normal
 // obstacles in order, then the negated union of the inversed ones.  Takes the
 // sign tests' jitter from rand() (three draws per test, the domain box's
 // first) and returns the number of draws.  Synchronizes the stream.
-// nodeFriction (optional, device, nodal): the friction of each node's
-// closest mesh object in the solid (MeshLevelSet::getClosestMeshObject), 0
-// for the domain object or none.
-int meshObstacleSolid(Grid g, const ObstacleMesh *obstacles, int n, float *solid, float *nodeFriction, hipStream_t s);
+// nodeObject (optional, device, nodal): each node's closest mesh object
+// (MeshLevelSet::getClosestMeshObject): -1 none, 0 the domain object (on
+// the domain SDF's exact band when domainObject), q + 1 obstacle q.
+int meshObstacleSolid(Grid g, const ObstacleMesh *obstacles, int n, float *solid, int *nodeObject, bool domainObject,
+                      hipStream_t s);
 // One mesh's fast signed distance level set on `cells` cells at node offset
 // `offset` (device phi, (ni+1)(nj+1)(nk+1)); returns the rand() draws.
 int meshLevelSetFastDevice(Grid cells, I3 offset, const ObstacleMesh &mesh, float *phi, hipStream_t s);
@@ -189,9 +189,11 @@ void extrapolateLayer(int wi, int wj, int wk, float *field, uint8_t *stat, int l
 void extrapolateAdjoint(int wi, int wj, int wk, float *bar, const uint8_t *stat, int layers, hipStream_t s);
 void addToFaces(float *a, size_t n, float inc, hipStream_t s);
 void constrainFaces(float *a, const float *w, size_t n, hipStream_t s);
-// the same with obstacle friction: f = the mean of the face's four nodes'
-// closest-object friction (fluidsimulation.cpp:6292-6353); dir 0/1/2 = u/v/w
-void constrainFacesFriction(Grid g, int dir, float *a, const float *w, const float *nodeFriction, hipStream_t s);
+// the same with friction: f = the mean of the face's four nodes' closest
+// objects' friction (fluidsimulation.cpp:6292-6353), objFriction[object]
+// (none: 0); dir 0/1/2 = u/v/w
+void constrainFacesFriction(Grid g, int dir, float *a, const float *w, const int *nodeObject, const float *objFriction,
+                            hipStream_t s);
 void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,
                    const float *wv, const float *ww, float *u, float *v, float *w, uint8_t *vu,
                    uint8_t *vv, uint8_t *vw, hipStream_t s);

@@ -117,12 +117,13 @@ __global__ void constrainKernel(float *a, const float *w, size_t n) {
         a[q] = f * sface + (1.0f - f) * a[q];
 }
 
-// _constrainVelocityFieldThread with obstacle friction (fluidsimulation.cpp:
-// 6292-6442): on a partly open face f is 0.25 x the sum, in the reference's
-// order, of the friction of the closest mesh object at the face's four
-// nodes; static solids have face velocity 0.
+// _constrainVelocityFieldThread with friction (fluidsimulation.cpp:6292-6442):
+// on a partly open face f is 0.25 x the sum, in the reference's order, of the
+// friction of the closest mesh object at the face's four nodes (read live
+// from the objects' table, none counting 0); static solids have face
+// velocity 0.
 template <int DIR>
-__global__ void constrainFrictionKernel(Grid g, float *a, const float *w, const float *nf) {
+__global__ void constrainFrictionKernel(Grid g, float *a, const float *w, const int *obj, const float *fr) {
     const int wi = g.ni + (DIR == 0), wj = g.nj + (DIR == 1), wk = g.nk + (DIR == 2);
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (size_t)wi * wj * wk) return;
@@ -136,11 +137,15 @@ __global__ void constrainFrictionKernel(Grid g, float *a, const float *w, const 
         // the two node offsets spanning the face: u (j, k), v (i, k), w (i, j)
         const int ai = DIR == 0 ? 0 : 1, aj = DIR == 0 ? 1 : 0, ak = 0;
         const int bi = 0, bj = DIR == 2 ? 1 : 0, bk = DIR == 2 ? 0 : 1;
+        auto of = [&](int x, int y, int z) {
+            const int m = obj[flat(x, y, z, pi, pj)];
+            return m < 0 ? 0.0f : fr[m];
+        };
         float friction = 0.0f;
-        friction += nf[flat(i, j, k, pi, pj)];
-        friction += nf[flat(i + ai, j + aj, k + ak, pi, pj)];
-        friction += nf[flat(i + bi, j + bj, k + bk, pi, pj)];
-        friction += nf[flat(i + ai + bi, j + aj + bj, k + ak + bk, pi, pj)];
+        friction += of(i, j, k);
+        friction += of(i + ai, j + aj, k + ak);
+        friction += of(i + bi, j + bj, k + bk);
+        friction += of(i + ai + bi, j + aj + bj, k + ak + bk);
         const float f = 0.25f * friction;
         a[q] = f * sface + (1.0f - f) * a[q];
     }
@@ -227,14 +232,15 @@ void constrainFaces(float *a, const float *w, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(constrainKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, a, w, n);
 }
 
-void constrainFacesFriction(Grid g, int dir, float *a, const float *w, const float *nf, hipStream_t s) {
+void constrainFacesFriction(Grid g, int dir, float *a, const float *w, const int *obj, const float *fr,
+                            hipStream_t s) {
     const size_t n = (size_t)(g.ni + (dir == 0)) * (g.nj + (dir == 1)) * (g.nk + (dir == 2));
     if (dir == 0)
-        hipLaunchKernelGGL(constrainFrictionKernel<0>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, nf);
+        hipLaunchKernelGGL(constrainFrictionKernel<0>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr);
     else if (dir == 1)
-        hipLaunchKernelGGL(constrainFrictionKernel<1>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, nf);
+        hipLaunchKernelGGL(constrainFrictionKernel<1>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr);
     else
-        hipLaunchKernelGGL(constrainFrictionKernel<2>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, nf);
+        hipLaunchKernelGGL(constrainFrictionKernel<2>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr);
 }
 
 void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,

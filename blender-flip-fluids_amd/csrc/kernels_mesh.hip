@@ -167,14 +167,13 @@ __global__ void negateInsideKernel(float *phi, const uint8_t *inside, size_t n) 
 }
 
 // MeshLevelSet::calculateUnion (meshlevelset.cpp:1759-1794): dst = min(dst,
-// [-]src) over src's nodes at offset o.  dstFric (optional) follows the
-// closest mesh object as the friction of that object: a node takes the
-// source's object when the source wins with a smaller magnitude and has a
-// closest triangle there.  The source's object is given by srcFricArr (-1 =
-// none) or, for a mesh level set, by srcFric where the node lies in the
-// exact band (|phi| below the set's upper bound srcUb).
+// [-]src) over src's nodes at offset o.  dstObj (optional) follows the
+// closest mesh object: a node takes the source's object when the source wins
+// with a smaller magnitude and has a closest triangle there.  The source's
+// object is given by srcObjArr (-1 = none) or, for a mesh level set, is srcObj
+// where the node lies in the exact band (|phi| below the set's upper bound srcUb).
 __global__ void unionKernel(const float *src, int qi, int qj, int qk, I3 o, bool negate, float *dst, int pi, int pj,
-                            int pk, float *dstFric, const float *srcFricArr, float srcFric, float srcUb) {
+                            int pk, int *dstObj, const int *srcObjArr, int srcObj, float srcUb) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)qi * qj * qk) return;
     const int i = (int)(idx % qi), j = (int)((idx / qi) % qj), k = (int)(idx / ((size_t)qi * qj));
@@ -185,9 +184,9 @@ __global__ void unionKernel(const float *src, int qi, int qj, int qk, I3 o, bool
     const size_t t = flat(a, b, c, pi, pj);
     float &d = dst[t];
     if (v < d) {
-        if (dstFric) {
-            const float sf = srcFricArr ? srcFricArr[idx] : (fabsf(raw) < srcUb ? srcFric : -1.0f);
-            if (sf >= 0.0f && fabsf(v) < fabsf(d)) dstFric[t] = sf;
+        if (dstObj) {
+            const int so = srcObjArr ? srcObjArr[idx] : (fabsf(raw) < srcUb ? srcObj : -1);
+            if (so >= 0 && fabsf(v) < fabsf(d)) dstObj[t] = so;
         }
         d = v;
     }
@@ -471,9 +470,10 @@ HMesh hostMesh(const ObstacleMesh &o) {
 }
 
 // MeshObject::getMeshLevelSet for a static mesh, unioned into levelset
-// (meshobject.cpp:239-264, 633-701, 738-839); fric (optional) tracks the
-// closest object's friction beside it
-void obstacleLevelSet(const ObstacleMesh &o, Grid g, RandJitter &rj, float *levelset, float *fric, hipStream_t s) {
+// (meshobject.cpp:239-264, 633-701, 738-839); obj (optional) tracks the
+// closest object beside it, this obstacle being objId
+void obstacleLevelSet(const ObstacleMesh &o, int objId, Grid g, RandJitter &rj, float *levelset, int *obj,
+                      hipStream_t s) {
     HMesh m = hostMesh(o);
     // getMesh(frameInterpolation): v + f (v - v), which turns -0 into +0
     for (V3 &p : m.v) p = add(p, V3{0.0f, 0.0f, 0.0f});
@@ -523,10 +523,15 @@ void obstacleLevelSet(const ObstacleMesh &o, Grid g, RandJitter &rj, float *leve
         levelSetFast(part, w, h, d, g0, g.dx, !fracture, rj, phi.p, s);
         const float ub = (float)(((w + 1) + (h + 1) + (d + 1)) * g.dx);  // the island set's upper bound
         hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, phi.p, w + 1, h + 1, d + 1, g0,
-                           false, levelset, g.ni + 1, g.nj + 1, g.nk + 1, fric, (const float *)nullptr, o.friction,
-                           ub);
+                           false, levelset, g.ni + 1, g.nj + 1, g.nk + 1, obj, (const int *)nullptr, objId, ub);
         hipCheck(hipStreamSynchronize(s), "obstacle union");
     }
+}
+
+// the domain object (0) on the domain SDF's exact band, none (-1) elsewhere
+__global__ void domainObjectKernel(const float *solid, size_t n, float ub, int domain, int *obj) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) obj[i] = (domain && fabsf(solid[i]) < ub) ? 0 : -1;
 }
 
 }  // namespace
@@ -539,30 +544,33 @@ int meshLevelSetFastDevice(Grid cells, I3 offset, const ObstacleMesh &mesh, floa
     return rj.draws;
 }
 
-int meshObstacleSolid(Grid g, const ObstacleMesh *obs, int n, float *solid, float *nodeFriction, hipStream_t s) {
+int meshObstacleSolid(Grid g, const ObstacleMesh *obs, int n, float *solid, int *nodeObject, bool domainObject,
+                      hipStream_t s) {
     RandJitter rj;
     rj.next(g.dx);  // the domain box's own sign test draws first (fluidsimulation.cpp:5277-5287)
     const size_t nn = (size_t)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
-    // the domain object (no friction) or none: friction 0
-    if (nodeFriction)
-        hipLaunchKernelGGL(fillFloatKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, nodeFriction, nn, 0.0f);
+    const float ubDomain = (float)(((g.ni + 1) + (g.nj + 1) + (g.nk + 1)) * g.dx);
+    if (nodeObject)
+        hipLaunchKernelGGL(domainObjectKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, solid, nn, ubDomain,
+                           domainObject ? 1 : 0, nodeObject);
     for (int q = 0; q < n; q++)
-        if (!obs[q].inversed) obstacleLevelSet(obs[q], g, rj, solid, nodeFriction, s);
+        if (!obs[q].inversed) obstacleLevelSet(obs[q], q + 1, g, rj, solid, nodeObject, s);
     bool anyInversed = false;
     for (int q = 0; q < n; q++) anyInversed |= obs[q].inversed;
     if (anyInversed) {
         // union of the inversed obstacles' level sets, negated, then unioned
         // into the solid (fluidsimulation.cpp:5321-5338)
-        DevArray<float> inv(nn), invFric(nodeFriction ? nn : 0);
-        const float ub = (float)(((g.ni + 1) + (g.nj + 1) + (g.nk + 1)) * g.dx);
-        hipLaunchKernelGGL(fillFloatKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, inv.p, nn, ub);
-        if (nodeFriction)
-            hipLaunchKernelGGL(fillFloatKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, invFric.p, nn, -1.0f);
+        DevArray<float> inv(nn);
+        DevArray<int> invObj(nodeObject ? nn : 0);
+        hipLaunchKernelGGL(fillFloatKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, inv.p, nn, ubDomain);
+        if (nodeObject)
+            hipLaunchKernelGGL(domainObjectKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, inv.p, nn, ubDomain, 0,
+                               invObj.p);
         for (int q = 0; q < n; q++)
-            if (obs[q].inversed) obstacleLevelSet(obs[q], g, rj, inv.p, invFric.p, s);
+            if (obs[q].inversed) obstacleLevelSet(obs[q], q + 1, g, rj, inv.p, invObj.p, s);
         hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, inv.p, g.ni + 1, g.nj + 1,
-                           g.nk + 1, I3{0, 0, 0}, true, solid, g.ni + 1, g.nj + 1, g.nk + 1, nodeFriction,
-                           (const float *)invFric.p, 0.0f, 0.0f);
+                           g.nk + 1, I3{0, 0, 0}, true, solid, g.ni + 1, g.nj + 1, g.nk + 1, nodeObject,
+                           (const int *)invObj.p, 0, 0.0f);
         hipCheck(hipStreamSynchronize(s), "inversed obstacles");
     }
     hipCheck(hipGetLastError(), "mesh obstacle kernels");

@@ -281,6 +281,16 @@ class FluidSimulation(object):
         return ("fp64", "mixed", "mixed-reliable", "fp32-preconditioner")[
             _call0(self, "FluidSimulation_get_pressure_solver_precision", c_int)]
 
+    @property
+    def boundary_friction(self):
+        return _call0(self, "FluidSimulation_get_boundary_friction", c_double)
+
+    @boundary_friction.setter
+    def boundary_friction(self, f):
+        libfunc = lib.FluidSimulation_set_boundary_friction
+        pb.init_lib_func(libfunc, [c_void_p, c_double, c_void_p], None)
+        pb.execute_lib_func(libfunc, [self(), f])
+
     def add_mesh_obstacle(self, mesh_object):
         libfunc = lib.FluidSimulation_add_mesh_obstacle
         pb.init_lib_func(libfunc, [c_void_p, c_void_p, c_void_p], None)

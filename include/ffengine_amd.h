@@ -260,6 +260,8 @@ void MeshObject_set_source_color(MeshObject *obj, double r, double g, double b, 
 void FluidSimulation_add_mesh_obstacle(FluidSimulation *obj, MeshObject *obstacle, int *err);    /* fluidsimulation_c.cpp:3113-3119 */
 void FluidSimulation_remove_mesh_obstacle(FluidSimulation *obj, MeshObject *obstacle, int *err); /* :3121-3127 */
 void FluidSimulation_remove_mesh_obstacles(FluidSimulation *obj, int *err);                      /* :3129-3134 */
+void FluidSimulation_set_boundary_friction(FluidSimulation *obj, double f, int *err);            /* :2813-2817 */
+double FluidSimulation_get_boundary_friction(FluidSimulation *obj, int *err);                   /* :2819-2823 */
 /* rand() values the last solid rebuild drew for its sign tests' jitter
  * (three per test; no counterpart in the reference, for tests). */
 int FFAMD_get_last_solid_rand_draws(FluidSimulation *obj, int *err);

@@ -415,6 +415,17 @@ class ReferenceSim:
         err = C.c_int()
         self.L.FluidSimulation_initialize(self.h, C.byref(err))
 
+    def set_boundary_friction(self, f):
+        err = C.c_int()
+        self.L.FluidSimulation_set_boundary_friction.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+        self.L.FluidSimulation_set_boundary_friction(self.h, f, C.byref(err))
+        assert err.value == 1
+
+    def set_obstacle_friction(self, obj, f):
+        err = C.c_int()
+        self.L.MeshObject_set_friction.argtypes = [C.c_void_p, C.c_float, C.c_void_p]
+        self.L.MeshObject_set_friction(obj, f, C.byref(err))
+
     def add_mesh_obstacle(self, n3, dx, verts, tris, inversed=False, friction=0.0):
         """MeshObject_new + update_mesh_static (+ inverse, friction), then
         FluidSimulation_add_mesh_obstacle (meshobject_c.cpp:38-69)."""

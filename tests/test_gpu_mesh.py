@@ -159,6 +159,44 @@ def test_unsupported_obstacles_raise():
 
 
 @pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref reference build absent")
+def test_boundary_and_live_friction_match_reference():
+    """Boundary friction (the domain object's) and an obstacle's friction
+    changed between frames, which the reference reads live."""
+    n = 32
+    dx = 1.0 / n
+    pos = scenes.dam_break_particles(n, seed=33)
+    v, t = scenes.sphere_mesh((0.3, 0.3, 0.5), 0.12, 3)
+    ref = O.ReferenceSim((n, n, n), dx, threads=4)
+    ref.load(pos)
+    robj = ref.add_mesh_obstacle((n, n, n), dx, v, t)
+    ref.set_boundary_friction(0.4)
+    ref.initialize()
+    sim = _sim(n, pos, threads=4)
+    mo = _obstacle(n, v, t)
+    sim.add_mesh_obstacle(mo)
+    sim.boundary_friction = 0.4
+    assert abs(sim.boundary_friction - 0.4) < 1e-12
+    with pytest.raises(RuntimeError):
+        sim.boundary_friction = 1.5
+    sim.initialize()
+    for frame in range(3):
+        if frame == 1:  # the obstacle's friction changes without any other state change
+            ref.set_obstacle_friction(robj, 0.7)
+            mo.friction = 0.7
+        O.libc_srand(77 + frame)
+        ref.update(1.0 / 30.0)
+        O.libc_srand(77 + frame)
+        sim.update(1.0 / 30.0)
+        pr, vr = ref.particles()
+        pg, vg = _state(sim)
+        same = len(pr) * 3 == len(pg) and np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32))
+        say(f"boundary/live friction frame {frame}: identical {same}")
+        assert len(pr) * 3 == len(pg), frame
+        assert np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32)), frame
+        assert np.array_equal(vr.reshape(-1).view(np.uint32), vg.view(np.uint32)), frame
+
+
+@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref reference build absent")
 @pytest.mark.parametrize("case", ["sphere", "inversed_and_sphere", "friction", "inversed_and_sphere_friction"])
 def test_engine_frames_with_obstacles_match_reference(case):
     n = 32
