@@ -606,7 +606,8 @@ int FFAMD_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi,
 }
 
 int FFAMD_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *vertices,
-                           const int *nt, const int *triangles, const int *inversed, float *phi, int *err) {
+                           const int *nt, const int *triangles, const int *inversed, const float *expansion,
+                           float *phi, int *err) {
     return guarded(err, [&] {
         checkGrid(ni, nj, nk, dx);
         if (nobj < 0) throw std::invalid_argument("Error: negative obstacle count.\n");
@@ -617,7 +618,7 @@ int FFAMD_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const in
         size_t vo = 0, to = 0;
         for (int q = 0; q < nobj; q++) {
             meshes.push_back(ff::ObstacleMesh{vertices + 3 * vo, nv[q], triangles + 3 * to, nt[q], inversed[q] != 0,
-                                              0.0f});
+                                              expansion ? expansion[q] : 0.0f});
             vo += (size_t)nv[q];
             to += (size_t)nt[q];
         }

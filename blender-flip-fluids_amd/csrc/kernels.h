@@ -24,7 +24,7 @@ struct ObstacleMesh {
     const int *triangles;
     int numTriangles;
     bool inversed;
-    float meshExpansion;  // used by the 25-or-more-island path only, which rejects non-zero values
+    float meshExpansion;  // used by the 25-or-more-island path only (the reference ignores it otherwise)
 };
 // Unions the obstacles' signed distance level sets into the nodal solid SDF
 // `solid` (device, (ni+1)(nj+1)(nk+1)), which holds the domain SDF: normal

@@ -499,10 +499,22 @@ void obstacleLevelSet(const ObstacleMesh &o, int objId, Grid g, RandJitter &rj, 
         if (boxContains(domain, b) || boxIntersects(domain, b)) islands.push_back(std::move(I));
     }
     const bool fracture = islands.size() >= 25;  // _numIslandsForFractureOptimizationTrigger
-    // the islands of that path are first expanded about their centroids
-    // (meshobject.cpp:703-736), which this engine does not restate
-    if (fracture && std::fabs(o.meshExpansion) >= 1e-9f)
-        throw std::runtime_error("Error: mesh expansion of obstacles with 25 or more islands is not supported.\n");
+    // that path's islands are first expanded (_expandMeshIslands,
+    // meshobject.cpp:703-736): every vertex moves 0.5 x expansion along its
+    // normalized offset from the island's float centroid
+    if (fracture && std::fabs(o.meshExpansion) >= 1e-9f) {
+        for (HMesh &I : islands) {
+            V3 sum{0.0f, 0.0f, 0.0f};
+            for (const V3 &p : I.v) sum = add(sum, p);
+            const V3 centroid = divide3(sum, (float)I.v.size());
+            const float expval = 0.5f * o.meshExpansion, eps = 1e-9f;
+            for (V3 &p : I.v) {
+                V3 d = sub(p, centroid);
+                if (std::fabs(d.x) < eps && std::fabs(d.y) < eps && std::fabs(d.z) < eps) continue;
+                p = add(p, scale(expval, normalize3(d)));
+            }
+        }
+    }
     std::vector<HMesh> parts;
     if (fracture)  // the island work queue pops from its back (boundedbuffer.h:119-135)
         parts.assign(std::make_move_iterator(islands.rbegin()), std::make_move_iterator(islands.rend()));

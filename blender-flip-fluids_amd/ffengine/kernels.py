@@ -104,18 +104,21 @@ def mesh_levelset_fast(cells3, dx, offset, vertices, triangles):
 
 def solid_phi_meshes(n3, dx, meshes):
     """The domain box SDF with static mesh obstacles (vertices, triangles[,
-    inversed]) unioned in.  Returns (phi (k, j, i) nodal, rand() draws)."""
+    inversed[, expansion]]) unioned in.  Returns (phi (k, j, i) nodal, rand()
+    draws)."""
     ni, nj, nk = n3
     vs = [_f32(m[0]).reshape(-1, 3) for m in meshes]
     ts = [np.ascontiguousarray(m[1], np.int32).reshape(-1, 3) for m in meshes]
     nv = np.array([len(v) for v in vs], np.int32)
     nt = np.array([len(t) for t in ts], np.int32)
     inv = np.array([1 if (len(m) > 2 and m[2]) else 0 for m in meshes], np.int32)
+    exp = np.array([m[3] if len(m) > 3 else 0.0 for m in meshes], np.float32)
     verts = np.ascontiguousarray(np.concatenate(vs) if vs else np.zeros((0, 3), np.float32))
     tris = np.ascontiguousarray(np.concatenate(ts) if ts else np.zeros((0, 3), np.int32))
     phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
-    draws = _run("FFAMD_solid_phi_meshes", [c_int] * 3 + [c_double, c_int] + [c_void_p] * 6,
-                 [ni, nj, nk, dx, len(meshes), _fp(nv), _fp(verts), _fp(nt), _fp(tris), _fp(inv), _fp(phi)], c_int)
+    draws = _run("FFAMD_solid_phi_meshes", [c_int] * 3 + [c_double, c_int] + [c_void_p] * 7,
+                 [ni, nj, nk, dx, len(meshes), _fp(nv), _fp(verts), _fp(nt), _fp(tris), _fp(inv), _fp(exp), _fp(phi)],
+                 c_int)
     return phi, draws
 
 

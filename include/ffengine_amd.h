@@ -324,10 +324,11 @@ int FFAMD_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi,
 /* FluidSimulation::_addStaticObjectsToSDF (fluidsimulation.cpp:5277-5339):
  * the domain box SDF with nobj static mesh obstacles unioned in (obstacle q:
  * nv[q] vertices, nt[q] triangles, concatenated, indices local to it;
- * inversed[q] != 0 for an inversed one).  Returns the rand() draws. */
+ * inversed[q] != 0 for an inversed one; expansion (optional) the objects'
+ * mesh expansion).  Returns the rand() draws. */
 int FFAMD_solid_phi_meshes(int isize, int jsize, int ksize, double dx, int nobj, const int *nv,
                            const float *vertices, const int *nt, const int *triangles, const int *inversed,
-                           float *phi_nodes, int *err);
+                           const float *expansion, float *phi_nodes, int *err);
 void FFAMD_face_weights(int isize, int jsize, int ksize, const float *phi_nodes, float *wu,
                         float *wv, float *ww, int *err);
 /* Pressure solve + projection on host grids; mode as in

@@ -630,8 +630,24 @@ void unionInto(std::vector<float> &phi, int pi, int pj, int pk, const std::vecto
 // MeshObject::getMeshLevelSet for a static mesh (meshobject.cpp:239-264,
 // 633-701, 738-754): fewer than 25 islands inside the domain -> the whole
 // mesh on its island grid; otherwise each island on its own grid.
+// MeshObject::_expandMeshIsland (meshobject.cpp:714-736): every vertex moves
+// 0.5 x expansion along its normalized offset from the float centroid
+void expandIsland(TriMesh &m, float expansion) {
+    if (m.v.empty()) return;
+    V3 sum{0.0f, 0.0f, 0.0f};
+    for (V3 p : m.v) sum = sum + p;
+    V3 centroid = divide(sum, (float)m.v.size());
+    float expval = 0.5f * expansion, eps = 1e-9f;
+    for (V3 &p : m.v) {
+        V3 v = p - centroid;
+        if (std::fabs(v.x) < eps && std::fabs(v.y) < eps && std::fabs(v.z) < eps) continue;
+        v = normalize(v);
+        p = p + expval * v;
+    }
+}
+
 void obstacleLevelSet(TriMesh m, int ni, int nj, int nk, double dx, int band, RandSource &rs,
-                      std::vector<float> &levelset) {
+                      std::vector<float> &levelset, float expansion = 0.0f) {
     // MeshObject::getMesh(frameInterpolation) (meshobject.cpp:158-177): v + f (v - v)
     // with v - v = +0, so a -0 coordinate becomes +0
     for (V3 &p : m.v) p = p + V3{0.0f, 0.0f, 0.0f};
@@ -645,11 +661,16 @@ void obstacleLevelSet(TriMesh m, int ni, int nj, int nk, double dx, int band, Ra
     }
     std::vector<TriMesh> parts;
     const bool fracture = (int)islands.size() >= 25;
-    if (fracture)
-        // the work queue pops from its back (boundedbuffer.h:119-135): one
-        // producer thread takes the islands last-first, and so draws their
-        // jitters in that order (with more threads the order varies)
+    if (fracture) {
+        // islands expanded (_expandMeshIslands, meshobject.cpp:703-712, a
+        // no-op below |expansion| 1e-9); the work queue pops from its back
+        // (boundedbuffer.h:119-135): one producer thread takes the islands
+        // last-first, and so draws their jitters in that order (with more
+        // threads the order varies)
+        if (std::fabs(expansion) >= 1e-9f)
+            for (TriMesh &I : islands) expandIsland(I, expansion);
         parts.assign(islands.rbegin(), islands.rend());
+    }
     else
         parts.push_back(m);
     for (TriMesh &part : parts) {
@@ -706,8 +727,8 @@ int oracle_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi
  * union of the inversed obstacles' level sets.  The domain box's own sign
  * test draws one jitter vector (three rand() values) before the obstacles. */
 int oracle_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *verts,
-                            const int *nt, const int *tris, const int *inversed, const int *rnd, int nrnd,
-                            int rand_max, float *phi_nodes) {
+                            const int *nt, const int *tris, const int *inversed, const float *expansion,
+                            const int *rnd, int nrnd, int rand_max, float *phi_nodes) {
     try {
         const int band = 3;
         const size_t nn = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
@@ -726,7 +747,7 @@ int oracle_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const i
         for (int q = 0; q < nobj; q++) {
             if (inversed[q]) continue;
             std::vector<float> temp(nn, ub);
-            obstacleLevelSet(meshes[q], ni, nj, nk, dx, band, rs, temp);
+            obstacleLevelSet(meshes[q], ni, nj, nk, dx, band, rs, temp, expansion ? expansion[q] : 0.0f);
             unionInto(solid, ni + 1, nj + 1, nk + 1, temp, ni + 1, nj + 1, nk + 1, GI{0, 0, 0});
         }
         bool anyInv = false;
@@ -735,7 +756,7 @@ int oracle_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const i
             if (!inversed[q]) continue;
             anyInv = true;
             std::vector<float> temp(nn, ub);
-            obstacleLevelSet(meshes[q], ni, nj, nk, dx, band, rs, temp);
+            obstacleLevelSet(meshes[q], ni, nj, nk, dx, band, rs, temp, expansion ? expansion[q] : 0.0f);
             unionInto(inv, ni + 1, nj + 1, nk + 1, temp, ni + 1, nj + 1, nk + 1, GI{0, 0, 0});
         }
         if (anyInv) {

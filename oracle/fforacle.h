@@ -40,11 +40,12 @@ int oracle_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi
 /* The static solid SDF with nobj mesh obstacles: the domain box, then the
  * obstacles' level sets (fluidsimulation.cpp:5277-5339).  Obstacle q has
  * nv[q] vertices and nt[q] triangles, concatenated in verts/tris (triangle
- * indices local to the obstacle); inversed[q] != 0 for an inversed one.
+ * indices local to the obstacle); inversed[q] != 0 for an inversed one;
+ * expansion (optional) the objects' mesh expansion.
  * Returns the number of rand() values used, -1 on error. */
 int oracle_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *verts,
-                            const int *nt, const int *tris, const int *inversed, const int *rnd, int nrnd,
-                            int rand_max, float *phi_nodes);
+                            const int *nt, const int *tris, const int *inversed, const float *expansion,
+                            const int *rnd, int nrnd, int rand_max, float *phi_nodes);
 
 /* Counting sort of particles by flat cell id of floor(p/dx).  keys[P],
  * counts[N], offsets[N+1] (exclusive scan), perm[P] (stable). */

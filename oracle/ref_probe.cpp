@@ -129,7 +129,8 @@ void refprobe_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int
  * sized to the domain as _updateSolidLevelSet does (5396-5405).  Returns the
  * wall-clock seconds of _addStaticObjectsToSDF alone. */
 double refprobe_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const int *nv, const float *verts,
-                               const int *nt, const int *tris, const int *inversed, unsigned seed, float *out) {
+                               const int *nt, const int *tris, const int *inversed, const float *expansion,
+                               unsigned seed, float *out) {
     FluidSimulation sim(ni, nj, nk, dx);
     sim.initialize();
     std::vector<std::unique_ptr<MeshObject>> objs;
@@ -138,6 +139,7 @@ double refprobe_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, co
         objs.emplace_back(new MeshObject(ni, nj, nk, dx));
         objs.back()->updateMeshStatic(makeMesh(nv[q], verts + 3 * vo, nt[q], tris + 3 * to));
         if (inversed[q]) objs.back()->inverse();
+        if (expansion) objs.back()->setMeshExpansion(expansion[q]);
         sim.addMeshObstacle(objs.back().get());
         vo += nv[q];
         to += nt[q];

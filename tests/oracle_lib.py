@@ -113,12 +113,14 @@ def libc_srand(seed):
 
 
 def _mesh_arrays(meshes):
+    """meshes: (vertices, triangles[, inversed[, expansion]]) each."""
     verts = np.ascontiguousarray(np.concatenate([_f32(m[0]).reshape(-1, 3) for m in meshes]), np.float32)
     tris = np.ascontiguousarray(np.concatenate([np.asarray(m[1], np.int32).reshape(-1, 3) for m in meshes]), np.int32)
     nv = np.array([len(_f32(m[0]).reshape(-1, 3)) for m in meshes], np.int32)
     nt = np.array([len(np.asarray(m[1]).reshape(-1, 3)) for m in meshes], np.int32)
     inv = np.array([1 if (len(m) > 2 and m[2]) else 0 for m in meshes], np.int32)
-    return verts, tris, nv, nt, inv
+    exp = np.array([m[3] if len(m) > 3 else 0.0 for m in meshes], np.float32)
+    return verts, tris, nv, nt, inv, exp
 
 
 def mesh_levelset_fast(cells3, dx, offset, verts, tris, band, seed):
@@ -140,11 +142,11 @@ def solid_phi_meshes(n3, dx, meshes, seed):
     """Oracle static solid SDF: domain box plus mesh obstacles, each
     (vertices, triangles[, inversed]); returns (phi (k, j, i), rand() draws)."""
     ni, nj, nk = n3
-    verts, tris, nv, nt, inv = _mesh_arrays(meshes)
+    verts, tris, nv, nt, inv, exp = _mesh_arrays(meshes)
     r = libc_rand_values(seed)
     phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
     used = oracle().oracle_solid_phi_meshes(ni, nj, nk, C.c_double(dx), len(meshes), fp(nv), fp(verts), fp(nt),
-                                            fp(tris), fp(inv), fp(r), len(r), RAND_MAX, fp(phi))
+                                            fp(tris), fp(inv), fp(exp), fp(r), len(r), RAND_MAX, fp(phi))
     if used < 0:
         raise RuntimeError("oracle_solid_phi_meshes failed")
     return phi, used
@@ -164,11 +166,11 @@ def ref_solid_phi_meshes(n3, dx, meshes, seed, timed=False):
     """The reference's static solid SDF (domain box + mesh obstacles) after
     srand(seed); with timed=True also the seconds of the SDF build alone."""
     ni, nj, nk = n3
-    verts, tris, nv, nt, inv = _mesh_arrays(meshes)
+    verts, tris, nv, nt, inv, exp = _mesh_arrays(meshes)
     phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
     f = probe().refprobe_solid_phi_meshes
     f.restype = C.c_double
-    seconds = f(ni, nj, nk, C.c_double(dx), len(meshes), fp(nv), fp(verts), fp(nt), fp(tris), fp(inv),
+    seconds = f(ni, nj, nk, C.c_double(dx), len(meshes), fp(nv), fp(verts), fp(nt), fp(tris), fp(inv), fp(exp),
                 C.c_uint(seed), fp(phi))
     return (phi, seconds) if timed else phi
 

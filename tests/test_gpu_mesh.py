@@ -63,7 +63,7 @@ def many_spheres(count, seed):
     return np.concatenate(verts), np.concatenate(tris)
 
 
-@pytest.mark.parametrize("case", ["one", "two", "inversed", "mixed", "islands", "border"])
+@pytest.mark.parametrize("case", ["one", "two", "inversed", "mixed", "islands", "border", "islands_expanded"])
 def test_static_solid_matches_oracle(case):
     n = 36
     dx = 1.0 / n
@@ -72,7 +72,8 @@ def test_static_solid_matches_oracle(case):
     c = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.4, 2)
     d = scenes.sphere_mesh((0.95, 0.5, 0.5), 0.2, 2)
     sets = {"one": [a], "two": [a, b], "inversed": [(c[0], c[1], True)], "mixed": [b, (c[0], c[1], True), a],
-            "islands": [many_spheres(30, 12)], "border": [d, a]}
+            "islands": [many_spheres(30, 12)], "border": [d, a],
+            "islands_expanded": [many_spheres(30, 12) + (True, 0.02)]}
     seed = 40 + len(case)
     say(f"static solid {case}")
     O.libc_srand(seed)

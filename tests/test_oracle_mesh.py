@@ -154,7 +154,8 @@ def many_spheres(count, seed):
 
 @needs_ref
 @pytest.mark.parametrize("meshes,seed", [
-    ("one", 4), ("two", 6), ("inversed", 8), ("mixed", 10), ("islands", 12), ("border", 14)])
+    ("one", 4), ("two", 6), ("inversed", 8), ("mixed", 10), ("islands", 12), ("border", 14),
+    ("islands_expanded", 16), ("whole_expanded", 18)])
 def test_static_solid_matches_reference(meshes, seed):
     n = 36
     dx = 1.0 / n
@@ -163,7 +164,10 @@ def test_static_solid_matches_reference(meshes, seed):
     c = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.4, 2)
     d = scenes.sphere_mesh((0.95, 0.5, 0.5), 0.2, 2)
     sets = {"one": [a], "two": [a, b], "inversed": [(c[0], c[1], True)], "mixed": [b, (c[0], c[1], True), a],
-            "islands": [many_spheres(30, seed)], "border": [d, a]}
+            "islands": [many_spheres(30, seed)], "border": [d, a],
+            # expansion applies to the 25-or-more-island path only
+            "islands_expanded": [many_spheres(30, seed) + (False, 0.02)],
+            "whole_expanded": [(a[0], a[1], False, 0.05)]}
     phi, used = O.solid_phi_meshes((n, n, n), dx, sets[meshes], seed)
     ref = O.ref_solid_phi_meshes((n, n, n), dx, sets[meshes], seed)
     assert used >= 6
