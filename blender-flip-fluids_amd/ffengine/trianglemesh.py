@@ -20,10 +20,11 @@ class TriangleMesh_t(ctypes.Structure):
 
 class TriangleMesh(object):
     def __init__(self, vertices=None, triangles=None):
+        # own copies: translate/scale/apply_transform must not touch the caller's arrays
         self.vertices = np.zeros(0, np.float32) if vertices is None else \
-            np.ascontiguousarray(vertices, np.float32).reshape(-1)
+            np.array(vertices, np.float32, copy=True).reshape(-1)
         self.triangles = np.zeros(0, np.int32) if triangles is None else \
-            np.ascontiguousarray(triangles, np.int32).reshape(-1)
+            np.array(triangles, np.int32, copy=True).reshape(-1)
 
     @classmethod
     def from_bobj(cls, bobj_data):
