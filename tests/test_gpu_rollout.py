@@ -124,26 +124,42 @@ def _ball(n, seed, center=(0.5, 0.55, 0.5), radius=0.22):
     return np.ascontiguousarray(pos[keep])
 
 
-def test_rollout_substep_matches_engine():
+@pytest.mark.parametrize("obstacle", [False, True])
+def test_rollout_substep_matches_engine(obstacle):
+    """One substep of the rollout equals the engine's; with a sphere
+    MeshObject in the engine and the same solid (FFAMD_solid_phi_meshes, same
+    rand() seed) given to the rollout as its nodal solid."""
+    from ffengine import MeshObject, TriangleMesh
     n = 24
     dx = 1.0 / n
     pos = _ball(n, 33)
     init = scenes.noise_mac_field(n, seed=34, amplitude=0.3, correlation=4.0)
     vel = K.sample_velocity((n, n, n), dx, init, pos)
     dt = 1.0 / 120.0
+    sphere = scenes.sphere_mesh((0.5, 0.2, 0.5), 0.1, 2)
     sim = FluidSimulation(n, n, n, dx)
     sim.add_body_force(0.0, -9.81, 0.0)
     sim.PICFLIP_ratio = 0.05
     sim.CFL_condition_number = 5
     sim.enable_surface_reconstruction = False
     sim.pressure_solver_tolerance = 1e-9
+    if obstacle:
+        mo = MeshObject(n, n, n, dx)
+        mo.update_mesh_static(TriangleMesh(*sphere))
+        sim.add_mesh_obstacle(mo)
     sim.load_marker_particle_data(len(pos), pos.tobytes(), vel.tobytes())
     sim.initialize()
+    O.libc_srand(9)
     sim.step(dt)
     m = sim.get_num_marker_particles()
     assert m == len(pos), "the engine removed particles"
     ep = np.frombuffer(sim.get_marker_particle_position_data_range(0, m), np.float32).reshape(-1, 3)
-    ro = Rollout((n, n, n), dx, pos, dt=dt, substeps=1, checkpoint_every=1, tol=1e-9)
+    solid = None
+    if obstacle:
+        O.libc_srand(9)
+        solid, _ = K.solid_phi_meshes((n, n, n), dx, [sphere])
+        assert np.array_equal(solid.view(np.uint32), sim.get_solid_sdf().view(np.uint32))
+    ro = Rollout((n, n, n), dx, pos, dt=dt, substeps=1, checkpoint_every=1, tol=1e-9, solid_nodes=solid)
     ro.forward(*init)
     rp = ro.positions()
     moved, fwd_iters, _ = ro.stats()
