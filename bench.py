@@ -37,10 +37,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_sim(n, pos, solid, mode, tol, apic=False, warm=False):
-    """solid: a nodal SDF array, a (vertices, triangles) mesh obstacle, or None."""
+def make_sim(n, pos, solid, mode, tol, apic=False, warm=False, schedule=None):
+    """solid: a nodal SDF array, a (vertices, triangles) mesh obstacle, or None.
+    schedule: the MIC(0) application's schedule (None: the engine's default)."""
     from ffengine import FluidSimulation, MeshObject, TriangleMesh
     sim = FluidSimulation(n, n, n, 1.0 / n)
+    if schedule is not None:
+        sim.set_pressure_solver_preconditioner_schedule(schedule)
     if apic:
         sim.set_velocity_transfer_method_APIC()
     if warm:
@@ -116,9 +119,9 @@ def enable_timing(sim, on):
     pb.execute_lib_func(f, [sim(), 1 if on else 0])
 
 
-def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt, apic=False, warm=False):
+def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt, apic=False, warm=False, schedule=None):
     import torch
-    sim = make_sim(n, pos, solid, mode, tol, apic, warm)
+    sim = make_sim(n, pos, solid, mode, tol, apic, warm, schedule)
     for s in range(warmup):
         sim.update(frame_dt)
         log("  warmup %s %d/%d" % (mode, s + 1, warmup))
@@ -458,6 +461,8 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=96)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", type=int, default=None, choices=[2, 4],
+                    help="MIC(0) application: 2 = bands of hyperplanes, 4 = line pipeline (default: the engine's)")
     ap.add_argument("--modes", default="fp64,mixed,mixed-reliable,fp32-preconditioner",
                     help="comma list of PCG modes: fp64, mixed (FP32 PCG restarted in FP64 refinement), "
                          "mixed-reliable (FP32 PCG with FP64 reliable updates), fp32-preconditioner (FP64 "
@@ -509,7 +514,7 @@ def main():
     for mode in modes:
         log("mode %s" % mode)
         results[mode] = run_mode(n, pos, solid, mode, args.tol, args.steps, args.warmup, frame_dt,
-                                 args.transfer == "apic", args.warm_start)
+                                 args.transfer == "apic", args.warm_start, args.schedule)
     # headline: the faster mixed-precision mode, else FP64
     mixed_modes = [m for m in ("mixed-reliable", "mixed", "fp32-preconditioner") if m in results]
     rate = lambda m: results[m]["substeps"] / results[m]["elapsed_s"]  # noqa: E731

@@ -272,6 +272,12 @@ void FluidSimulation_set_pressure_solver_warm_start(FluidSimulation *o, int on, 
 int FluidSimulation_get_pressure_solver_warm_start(FluidSimulation *o, int *err) {
     return guarded(err, [&] { return S(o)->getPressureSolverWarmStart() ? 1 : 0; });
 }
+void FluidSimulation_set_pressure_solver_preconditioner_schedule(FluidSimulation *o, int schedule, int *err) {
+    guarded(err, [&] { S(o)->setPressureSolverPreconditionerSchedule(schedule); });
+}
+int FluidSimulation_get_pressure_solver_preconditioner_schedule(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getPressureSolverPreconditionerSchedule(); });
+}
 // ---- static mesh obstacles ----
 MeshObject *MeshObject_new(int i, int j, int k, double dx, int *err) {
     return guarded(err, [&] {
@@ -913,9 +919,9 @@ int FFAMD_mic0_apply(int ni, int nj, int nk, double dx, double dt, const float *
     return guarded(err, [&] {
         checkGrid(ni, nj, nk, dx);
         if (precision != 0 && precision != 1) throw std::domain_error("precision must be 0 (FP64) or 1 (FP32)");
-        if (schedule < 0 || schedule > 3)
-            throw std::domain_error("schedule must be 0 (hyperplanes), 1 (tiles), 2 (bands) or 3 (bands, device "
-                                    "tile check)");
+        if (schedule < 0 || schedule > 4)
+            throw std::domain_error("schedule must be 0 (hyperplanes), 1 (tiles), 2 (bands), 3 (bands, device "
+                                    "tile check) or 4 (line pipeline)");
         ff::Grid g{ni, nj, nk, dx};
         Scratch sc;
         size_t N = (size_t)ni * nj * nk;

@@ -139,6 +139,9 @@ public:
         _ps.setWarmStart(on);
     }
     bool getPressureSolverWarmStart() const { return _warmStart; }
+    // the MIC(0) application's schedule: 2 = bands of hyperplanes, 4 = the line pipeline
+    void setPressureSolverPreconditionerSchedule(int schedule) { _ps.setSchedule(schedule); }
+    int getPressureSolverPreconditionerSchedule() const { return _ps.schedule(); }
     void setSolidPhi(const float *phiNodes);
 
     // static mesh obstacles (fluidsimulation.cpp:2990-3032); the solid SDF is

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "ffdevice.h"
+#include "pencil_types.h"
 
 namespace ff {
 
@@ -336,6 +337,33 @@ void gatherRows(const PressureSys &S, const double *cells, double *rhs, hipStrea
 // x[r] = (double)cells[rows[r]] (a float cell grid onto the rows)
 void gatherRowsF(const PressureSys &S, const float *cells, double *x, hipStream_t s);
 void scatterRows(const PressureSys &S, const double *x, double *cells, hipStream_t s);
+
+// ---- MIC(0) application as a line pipeline (kernels_pencil.hip) ----
+// Pencils of 16 x 16 (j, k) lines along i, one workgroup each, one launch per
+// triangular sweep; neighbour pencils hand results over through epoch-tagged
+// exchange words.  The plan's device buffers are owned by the caller.
+struct PencilPlan {
+    int npj = 0, npk = 0, np = 0, nw = 0;  // pencils along j, k; total; 32-cell words per line
+    int *order = nullptr;                  // np pencils by PJ+PK (device)
+    LineWord *words = nullptr;             // nj*nk lines x nw words of the current system (device)
+    unsigned long long *ex = nullptr;      // pencilExchangeWords(g) words, cleared once (device)
+    unsigned *counter = nullptr;           // ticket counter, cleared once (device)
+    int *error = nullptr;                  // raised by a wait past waitTicks (device)
+    unsigned base = 0, epoch = 0;          // counter value and last epoch used
+    long long waitTicks = 0;               // longest wait in wall-clock ticks
+};
+void pencilShape(Grid g, int &npj, int &npk, int &nw);
+size_t pencilExchangeWords(Grid g);
+size_t pencilLdsBytes(Grid g);  // dynamic LDS per workgroup
+// pencils by PJ+PK, then PJ (the forward sweep's dependency order)
+void pencilOrder(Grid g, std::vector<int> &order);
+// line words of the current system (after pressureRows)
+void pencilLineWords(Grid g, const PressureSys &S, PencilPlan &pp, hipStream_t s);
+// z = (L L^T)^-1 r, y = the forward result (M values, distinct from r and z)
+void pencilApply(Grid g, const PressureSys &S, PencilPlan &pp, const double *r, double *y, double *z,
+                 hipStream_t s);
+void pencilApplyF(Grid g, const PressureSys &S, PencilPlan &pp, const float *r, float *y, float *z,
+                  hipStream_t s);
 
 // ---- MIC(0) application, persistent tile wavefront (kernels_mic.hip) ----
 // Tiles are 16 x 16 x depth cells; the schedule lists the non-empty tiles by

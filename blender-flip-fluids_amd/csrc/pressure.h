@@ -50,12 +50,17 @@ public:
     // the rows (the previous solve's result) instead of 0.  The stopping
     // test keeps the zero-start threshold, tolerance x |b|_inf.
     void setWarmStart(bool on) { _warm = on; }
+    // the solver's MIC(0) application: 2 = bands of hyperplanes, 4 = the
+    // line pipeline (two launches per application)
+    void setSchedule(int schedule);
+    int schedule() const { return _sched; }
     const std::vector<int> &levelTable() const { return _hLevelStart; }
     // one preconditioner application z = (L L^T)^-1 r on the current factor
     // (after solve()); FP64 or the FP32 copy.  schedule 0 = one launch per
     // hyperplane, 1 = tile wavefront (switched off: raises an error),
-    // 2 = bands of hyperplanes (the solver's), 3 = the same bands with the tile
-    // ranges checked on the device instead of passed in the launches.
+    // 2 = bands of hyperplanes, 3 = the same bands with the tile ranges
+    // checked on the device instead of passed in the launches, 4 = the line
+    // pipeline.
     void applyPreconditioner(const double *r, double *z, int schedule, hipStream_t s);
     void applyPreconditionerF(const float *r, float *z, int schedule, hipStream_t s);
     // throws when a preconditioner launch gave up waiting (waits for the stream)
@@ -80,6 +85,12 @@ private:
     KernelTimers *_t = nullptr;
     KernelTimers _noTimers;
     int _levelLaunches = 0, _bandLaunches = 0;
+    PencilPlan _pp;  // line-pipeline buffers (inside _block)
+    int _sched = 2;
+    // z = M^-1 r with the solver's schedule (y: forward result)
+    void precond(const double *r, double *y, double *z, hipStream_t s);
+    void precondF(const float *r, float *y, float *z, hipStream_t s);
+    int applyLaunches() const { return _sched == 4 ? 2 : _bandLaunches; }
     int _refDots = 0;  // thread count of the reference's dot order, or 0 for the tree reductions
     bool _warm = false;
     const float *_prevPressure = nullptr;  // the pressure grid at solve() entry

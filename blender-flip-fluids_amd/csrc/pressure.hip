@@ -53,6 +53,11 @@ void PressureSolver::allocate(Grid g) {
     size_t oRec64[2], oRec32[2];
     for (auto &o : oRec64) o = take(sizeof(MicRec<double>) * M);
     for (auto &o : oRec32) o = take(sizeof(MicRec<float>) * M);
+    pencilShape(g, _pp.npj, _pp.npk, _pp.nw);
+    _pp.np = _pp.npj * _pp.npk;
+    size_t oPOrder = take(4 * (size_t)_pp.np), oPWords = take(sizeof(LineWord) * (size_t)g.nj * g.nk * _pp.nw);
+    const size_t exWords = pencilExchangeWords(g);
+    size_t oPEx = take(8 * exWords), oPCounter = take(4);
     size_t oPart = take(8 * 1024), oScal = take(8 * 16);
     hipCheck(hipMalloc(&_block, bytes), "hipMalloc pressure");
     char *b = (char *)_block;
@@ -94,6 +99,22 @@ void PressureSolver::allocate(Grid g) {
     _y = (double *)(b + oV[4]);
     _fr = (float *)(b + oFV[0]); _fz = (float *)(b + oFV[1]); _fs = (float *)(b + oFV[2]);
     _fd = (float *)(b + oFV[3]); _fq = (float *)(b + oFV[4]); _fy = (float *)(b + oFV[5]);
+    _pp.order = (int *)(b + oPOrder);
+    _pp.words = (LineWord *)(b + oPWords);
+    _pp.ex = (unsigned long long *)(b + oPEx);
+    _pp.counter = (unsigned *)(b + oPCounter);
+    _pp.error = _ms.error;
+    _pp.base = 0;
+    _pp.epoch = 0;
+    {
+        std::vector<int> order;
+        pencilOrder(g, order);
+        hipCheck(hipMemcpy(_pp.order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice),
+                 "H2D pencil order");
+        // epoch 0 marks a word no launch has published; the counter starts at 0
+        hipCheck(hipMemset(_pp.ex, 0, 8 * exWords), "clear exchange words");
+        hipCheck(hipMemset(_pp.counter, 0, 4), "clear ticket counter");
+    }
     _partial = (double *)(b + oPart);
     _scal = (double *)(b + oScal);
     _S.m = 0;
@@ -105,8 +126,13 @@ void PressureSolver::allocate(Grid g) {
     int wallKHz = 0;
     hipCheck(hipDeviceGetAttribute(&wallKHz, hipDeviceAttributeWallClockRate, dev), "device attribute");
     _ms.waitTicks = 2000LL * std::max(wallKHz, 1);  // 2 s
+    _pp.waitTicks = _ms.waitTicks;
     int lds = 0;
     hipCheck(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "device attribute");
+    if ((size_t)lds < pencilLdsBytes(g) + 8192)
+        throw std::runtime_error("the MIC(0) line pipeline needs " + std::to_string(pencilLdsBytes(g) + 8192) +
+                                 " bytes of LDS per workgroup at this grid size; the device offers " +
+                                 std::to_string(lds));
     if ((size_t)lds < micTileLds(true))
         throw std::runtime_error("the MIC(0) tile kernel needs " + std::to_string(micTileLds(true)) +
                                  " bytes of LDS per workgroup; the device offers " + std::to_string(lds));
@@ -156,11 +182,33 @@ void PressureSolver::checkError(hipStream_t s) {
     int e = 0;
     hipCheck(hipMemcpyAsync(&e, _ms.error, sizeof(int), hipMemcpyDeviceToHost, s), "D2H error word");
     hipCheck(hipStreamSynchronize(s), "sync");
-    if (e) throw std::runtime_error("MIC(0) preconditioner: a tile wait exceeded its time limit");
+    if (e) throw std::runtime_error("MIC(0) preconditioner: a wait for a neighbour's results exceeded its time limit");
+}
+
+void PressureSolver::setSchedule(int schedule) {
+    if (schedule != 2 && schedule != 4)
+        throw std::domain_error("the solver's preconditioner schedule must be 2 (bands) or 4 (line pipeline)");
+    _sched = schedule;
+}
+
+void PressureSolver::precond(const double *r, double *y, double *z, hipStream_t s) {
+    if (_sched == 4)
+        pencilApply(_g, _S, _pp, r, y, z, s);
+    else
+        mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, r, y, z, s);
+}
+void PressureSolver::precondF(const float *r, float *y, float *z, hipStream_t s) {
+    if (_sched == 4)
+        pencilApplyF(_g, _S, _pp, r, y, z, s);
+    else
+        mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, r, y, z, s);
 }
 
 void PressureSolver::applyPreconditioner(const double *r, double *z, int schedule, hipStream_t s) {
-    if (schedule == 1) {
+    if (schedule == 4) {
+        pencilLineWords(_g, _S, _pp, s);
+        pencilApply(_g, _S, _pp, r, _y, z, s);
+    } else if (schedule == 1) {
         buildSchedule(true, s);
         micApplyTiles(_S, _ms, r, z, s);
     } else if (schedule == 2 || schedule == 3) {
@@ -175,7 +223,10 @@ void PressureSolver::applyPreconditioner(const double *r, double *z, int schedul
     }
 }
 void PressureSolver::applyPreconditionerF(const float *r, float *z, int schedule, hipStream_t s) {
-    if (schedule == 1) {
+    if (schedule == 4) {
+        pencilLineWords(_g, _S, _pp, s);
+        pencilApplyF(_g, _S, _pp, r, _fy, z, s);
+    } else if (schedule == 1) {
         buildSchedule(false, s);
         micApplyTilesF(_S, _ms, r, z, s);
     } else if (schedule == 2 || schedule == 3) {
@@ -232,6 +283,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     pressureAssemble(_g, dt, u, v, w, wu, wv, ww, phi, _S, s);
     if (cellRhs) gatherRows(_S, cellRhs, _S.rhs, s);
     pressureLevels(_g, _S, scanScratch, s);
+    if (_sched == 4 && m > 0) pencilLineWords(_g, _S, _pp, s);
     _t->end(T_ASSEMBLE, s);
     _hLevelStart.resize(_S.nLevels + 1);
     hipCheck(hipMemcpyAsync(_hLevelStart.data(), _S.levelStart, sizeof(int) * (_S.nLevels + 1),
@@ -249,7 +301,11 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     for (int l = 0; l < _S.nLevels; l++) _levelLaunches += _hLevelStart[l + 1] > _hLevelStart[l];
     mic0BandPlan(_g, _S, _hLevelStart.data(), mode == 0, _hTileLo.data(), _hTileHi.data(), _plan);
     _bandLaunches = (int)_plan.l0.size();
-    if (maxAbs < tol) {
+    // The forward solve skips a right-hand side below the tolerance as an
+    // absolute threshold, as the reference does (pressuresolver.cpp:69-75).  A
+    // given right-hand side (the adjoint's) is solved unless it is exactly 0,
+    // so the solution stays linear in it at every scale.
+    if (cellRhs ? maxAbs == 0.0 : maxAbs < tol) {
         hipCheck(hipMemsetAsync(pressure, 0, sizeof(float) * N, s), "memset");
         res.success = true;
         res.iterations = 0;
@@ -298,8 +354,8 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     mic0Factor(_S, hs, st);
     mic0LowerEntries(_S, true, st);
     _t->end(T_MIC0_FACTOR, st);
-    _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-    mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, _r, _y, _z, st);
+    _t->begin(T_MIC0_APPLY, st, applyLaunches());
+    precond(_r, _y, _z, st);
     _t->end(T_MIC0_APPLY, st);
     double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
     // dot products: the reference's order, or the deterministic tree
@@ -333,8 +389,8 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
             iters = it + 1;
             return true;
         }
-        _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, _r, _y, z, st);
+        _t->begin(T_MIC0_APPLY, st, applyLaunches());
+        precond(_r, _y, z, st);
         _t->end(T_MIC0_APPLY, st);
         dotp(z, _r, rhoNew);
         // beta = rhoNew / rho; z += (float)beta * s; then s <-> z
@@ -365,8 +421,8 @@ bool PressureSolver::pcgPrecond32(double tolFactor, int maxIter, double resid0, 
     mic0ToFloat(_S, st);
     mic0LowerEntries(_S, false, st);
     castToFloat(_r, _fr, m, st);
-    _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-    mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, _fz, st);
+    _t->begin(T_MIC0_APPLY, st, applyLaunches());
+    precondF(_fr, _fy, _fz, st);
     _t->end(T_MIC0_APPLY, st);
     double *rho = _scal + 2, *rhoNew = _scal + 3, *sz = _scal + 4, *rn = _scal + 5;
     dotfd(_fz, _r, m, _partial, rho, st);
@@ -389,8 +445,8 @@ bool PressureSolver::pcgPrecond32(double tolFactor, int maxIter, double resid0, 
             iters = it + 1;
             return true;
         }
-        _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, _fz, st);
+        _t->begin(T_MIC0_APPLY, st, applyLaunches());
+        precondF(_fr, _fy, _fz, st);
         _t->end(T_MIC0_APPLY, st);
         dotfd(_fz, _r, m, _partial, rhoNew, st);
         // beta = rhoNew / rho; z = (double)fz + (float)beta * s; then s <-> z
@@ -430,8 +486,8 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
         absMaxf(_fr, m, _partial, rn, st);
         double innerTol = innerRel * hostScalar(rn, st);
         hipCheck(hipMemsetAsync(_fd, 0, sizeof(float) * m, st), "memset");
-        _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, _fz, st);
+        _t->begin(T_MIC0_APPLY, st, applyLaunches());
+        precondF(_fr, _fy, _fz, st);
         _t->end(T_MIC0_APPLY, st);
         dotf(_fz, _fr, m, _partial, rho, st);
         double hrho = hostScalar(rho, st);
@@ -445,8 +501,8 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
             pcgStepF(_fd, s, _fr, _fq, m, rho, sz, _partial, rn, st, _hostScalDev);
             iters++;
             if (mappedResidual(st) <= innerTol) break;
-            _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-            mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, z, st);
+            _t->begin(T_MIC0_APPLY, st, applyLaunches());
+            precondF(_fr, _fy, z, st);
             _t->end(T_MIC0_APPLY, st);
             dotf(z, _fr, m, _partial, rhoNew, st);
             axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);
@@ -487,8 +543,8 @@ bool PressureSolver::pcgReliable(double tolFactor, int maxIter, double resid0, d
     double sinceUpdate = resid0;
     resid = resid0;
     iters = 0;
-    _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-    mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, _fz, st);
+    _t->begin(T_MIC0_APPLY, st, applyLaunches());
+    precondF(_fr, _fy, _fz, st);
     _t->end(T_MIC0_APPLY, st);
     dotf(_fz, _fr, m, _partial, rho, st);
     double hrho = hostScalar(rho, st);
@@ -513,8 +569,8 @@ bool PressureSolver::pcgReliable(double tolFactor, int maxIter, double resid0, d
             castToFloat(_r, _fr, m, st);
             sinceUpdate = resid;
         }
-        _t->begin(T_MIC0_APPLY, st, _bandLaunches);
-        mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, _fr, _fy, z, st);
+        _t->begin(T_MIC0_APPLY, st, applyLaunches());
+        precondF(_fr, _fy, z, st);
         _t->end(T_MIC0_APPLY, st);
         dotf(z, _fr, m, _partial, rhoNew, st);
         axpyFloatStepF(z, s, m, rhoNew, rho, 1.0, st);

@@ -271,6 +271,16 @@ class FluidSimulation(object):
         pb.init_lib_func(f, [c_void_p, c_int, c_void_p], None)
         pb.execute_lib_func(f, [self(), 1 if enabled else 0])
 
+    def set_pressure_solver_preconditioner_schedule(self, schedule):
+        """Schedule of the MIC(0) application: 2 = bands of hyperplanes,
+        4 = the line pipeline (one launch per triangular sweep); same bits."""
+        f = lib.FluidSimulation_set_pressure_solver_preconditioner_schedule
+        pb.init_lib_func(f, [c_void_p, c_int, c_void_p], None)
+        pb.execute_lib_func(f, [self(), int(schedule)])
+
+    def get_pressure_solver_preconditioner_schedule(self):
+        return int(_call0(self, "FluidSimulation_get_pressure_solver_preconditioner_schedule", c_int))
+
     def get_pressure_solver_warm_start(self):
         return bool(_call0(self, "FluidSimulation_get_pressure_solver_warm_start", c_int))
 

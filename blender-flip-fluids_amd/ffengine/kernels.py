@@ -170,8 +170,8 @@ def mic0_apply(n3, dx, dt, uvw, weights, liquid_phi, r, precision=0, schedule=0)
     """One MIC(0) application z = (L L^T)^-1 r on the pressure system of these
     inputs (row order as pressure_system); precision 0 = FP64, 1 = FP32;
     schedule 0 = one launch per hyperplane, 1 = tile wavefront (switched off),
-    2 = bands of hyperplanes (the solver's), 3 = bands with the tile check on
-    the device."""
+    2 = bands of hyperplanes, 3 = bands with the tile check on the device,
+    4 = the line pipeline (one launch per triangular sweep)."""
     ni, nj, nk = n3
     u, v, w = (_f32(a) for a in uvw)
     wu, wv, ww = (_f32(a) for a in weights)

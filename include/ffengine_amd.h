@@ -199,6 +199,13 @@ int FluidSimulation_get_pressure_solver_reference_summation(FluidSimulation *obj
  * PCG starts from 0); the stopping threshold stays tolerance x |b|_inf. */
 void FluidSimulation_set_pressure_solver_warm_start(FluidSimulation *obj, int enabled, int *err);
 int FluidSimulation_get_pressure_solver_warm_start(FluidSimulation *obj, int *err);
+/* Schedule of the MIC(0) preconditioner application (no counterpart in the
+ * reference, whose triangular solves are serial loops, pcgsolver.h:211-241):
+ * 2 = bands of i+j+k hyperplanes, one launch per band; 4 = the line
+ * pipeline, one launch per triangular sweep.  Both give the same bits. */
+void FluidSimulation_set_pressure_solver_preconditioner_schedule(FluidSimulation *obj, int schedule,
+                                                                 int *err);
+int FluidSimulation_get_pressure_solver_preconditioner_schedule(FluidSimulation *obj, int *err);
 
 /* ---- static mesh obstacles ------------------------------------------------
  * MeshObject (meshobject.h) and its C ABI (c_bindings/meshobject_c.cpp, lines
@@ -355,7 +362,8 @@ int FFAMD_pressure_system(int isize, int jsize, int ksize, double dx, double dt,
  * error, 2 = bands of hyperplanes (the solver's: the launches name the tiles
  * that hold rows of their band), 3 = the same bands with each workgroup
  * reading its tile's hyperplane range on the device (used when the tiles
- * outnumber what a launch can list).  The inputs must give a non-zero
+ * outnumber what a launch can list), 4 = the line pipeline (pencils of
+ * 16 x 16 lines along i, one launch per triangular sweep).  The inputs must give a non-zero
  * divergence.  Returns M. */
 int FFAMD_mic0_apply(int isize, int jsize, int ksize, double dx, double dt, const float *u,
                      const float *v, const float *w, const float *wu, const float *wv,

@@ -1,0 +1,177 @@
+// pencil_emu.cpp -- CPU emulation of the line-pipeline MIC(0) sweeps
+// (blender-flip-fluids_amd/csrc/pencil_sweep.h), thread by thread.
+//
+// Test infrastructure: it compiles the GPU sweep's own body with g++, runs
+// every workgroup as TJ*TK OS threads that meet at a std::barrier wherever the
+// kernel has a workgroup barrier, and the workgroups one after another in
+// launch order (the ticket order), so every exchange word a pencil reads was
+// published earlier.  Built with AddressSanitizer, it checks every access of
+// the sweep's indexing against the buffers' extents.
+//
+// Usage: pencil_emu <input> <output>
+//   input:  int32 ni, nj, nk, m, fp64 (1: double, 0: float), then
+//           int32 cellRow[N], int32 rowOf[N+1], float64 r[m],
+//           float64 recFwd[m][4], float64 recBwd[m][4]  ({inv, a, b, c})
+//   output: float64 z[m]
+#include <algorithm>
+#include <barrier>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+static std::barrier<> *gBarrier = nullptr;
+
+static long long hostClock() {
+    return (long long)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+template <class To, class From>
+static To bitCast(From v) {
+    static_assert(sizeof(To) == sizeof(From), "size");
+    To t;
+    std::memcpy(&t, &v, sizeof(To));
+    return t;
+}
+
+#define FF_PS_DEVICE inline
+#define FF_PS_SHARED static
+#define FF_PS_BARRIER() gBarrier->arrive_and_wait()
+#define FF_PS_LOAD64(p) __atomic_load_n((unsigned long long *)(p), __ATOMIC_RELAXED)
+#define FF_PS_STORE64(p, v) __atomic_store_n((p), (v), __ATOMIC_RELAXED)
+#define FF_PS_LOAD32(p) __atomic_load_n((p), __ATOMIC_RELAXED)
+#define FF_PS_RAISE(p) __atomic_fetch_or((p), 1, __ATOMIC_RELAXED)
+#define FF_PS_TICKET(p) __atomic_fetch_add((p), 1u, __ATOMIC_RELAXED)
+#define FF_PS_CLOCK() hostClock()
+#define FF_PS_PAUSE() std::this_thread::yield()
+#define FF_PS_UNIFORM(x) (x)
+#define FF_PS_F2U(v) bitCast<unsigned>(v)
+#define FF_PS_U2F(u) bitCast<float>(u)
+#define FF_PS_D2U(v) bitCast<unsigned long long>(v)
+#define FF_PS_U2D(u) bitCast<double>(u)
+#include "../../blender-flip-fluids_amd/csrc/pencil_sweep.h"
+
+using ff::LineWord;
+
+template <class T>
+struct Rec {
+    T inv, a, b, c;
+};
+
+// must match kernels_pencil.hip
+constexpr int TJ = 16, TK = 16, C32 = 8, C64 = 4;
+
+template <class T, int C, bool FWD>
+static void sweep(ff::PencilArgs<T, Rec<T>> a, int np, const std::vector<int> &launch) {
+    constexpr int NT = TJ * TK;
+    std::vector<LineWord> sW((size_t)NT * a.nw);
+    std::vector<unsigned> sN((size_t)(TJ + TK) * a.nw);
+    std::barrier<> bar(NT);
+    gBarrier = &bar;
+    for (int w : launch) {
+        (void)w;  // the workgroup's id does not matter: its pencil comes from its ticket
+        std::vector<std::thread> th;
+        th.reserve(NT);
+        for (int t = 0; t < NT; t++)
+            th.emplace_back([&, t] { ff::pencilSweep<T, Rec<T>, TJ, TK, C, FWD>(a, t, sW.data(), sN.data()); });
+        for (auto &x : th) x.join();
+    }
+    (void)np;
+}
+
+template <class T, int C>
+static void apply(int ni, int nj, int nk, int m, const std::vector<int> &cellRow, const std::vector<int> &rowOf,
+                  const std::vector<double> &r64, const std::vector<double> &fwd64, const std::vector<double> &bwd64,
+                  std::vector<double> &z64) {
+    const int npj = (nj + TJ - 1) / TJ, npk = (nk + TK - 1) / TK, np = npj * npk, nw = (ni + 31) / 32;
+    // line words, as lineWordsKernel builds them
+    std::vector<LineWord> words((size_t)nj * nk * nw);
+    for (size_t line = 0; line < (size_t)nj * nk; line++)
+        for (int w = 0; w < nw; w++) {
+            const size_t c0 = line * ni + 32 * (size_t)w;
+            const int cnt = std::min(32, ni - 32 * w);
+            unsigned bits = 0;
+            for (int b = 0; b < cnt; b++) bits |= (cellRow[c0 + b] >= 0 ? 1u : 0u) << b;
+            words[line * nw + w] = LineWord{rowOf[c0], bits};
+        }
+    // pencils by PJ + PK, as pencilOrder lists them
+    std::vector<int> order;
+    for (int d = 0; d <= npj + npk - 2; d++)
+        for (int pj = 0; pj < npj; pj++) {
+            const int pk = d - pj;
+            if (pk >= 0 && pk < npk) order.push_back(pj + pk * npj);
+        }
+    std::vector<T> r(m), y(m), z(m);
+    std::vector<Rec<T>> rf(m), rb(m);
+    for (int q = 0; q < m; q++) {
+        r[q] = (T)r64[q];
+        rf[q] = Rec<T>{(T)fwd64[4 * q], (T)fwd64[4 * q + 1], (T)fwd64[4 * q + 2], (T)fwd64[4 * q + 3]};
+        rb[q] = Rec<T>{(T)bwd64[4 * q], (T)bwd64[4 * q + 1], (T)bwd64[4 * q + 2], (T)bwd64[4 * q + 3]};
+    }
+    std::vector<unsigned long long> ex((size_t)np * 2 * std::max(TJ, TK) * ni * 2, 0ull);
+    unsigned counter = 12345u;  // any start: the launch's base is the counter at its start
+    int error = 0;
+    ff::PencilArgs<T, Rec<T>> a;
+    a.ni = ni; a.nj = nj; a.nk = nk; a.nw = nw; a.npj = npj; a.npk = npk;
+    a.words = words.data();
+    a.order = order.data();
+    a.ex = ex.data();
+    a.counter = &counter;
+    a.error = &error;
+    a.waitTicks = 20000000000LL;  // 20 s
+    std::vector<int> launch(np);
+    for (int q = 0; q < np; q++) launch[q] = q;
+    unsigned epoch = 0;
+    for (int pass = 0; pass < 2; pass++) {  // twice: the second pass sees the first one's stale words
+        a.base = counter;
+        a.epoch = ++epoch;
+        a.in = r.data(); a.out = y.data(); a.rec = rf.data();
+        sweep<T, C, true>(a, np, launch);
+        a.base = counter;
+        a.epoch = ++epoch;
+        a.in = y.data(); a.out = z.data(); a.rec = rb.data();
+        sweep<T, C, false>(a, np, launch);
+    }
+    if (error) throw std::runtime_error("a wait ran out");
+    if (counter != 12345u + 4u * (unsigned)np) throw std::runtime_error("ticket count");
+    for (int q = 0; q < m; q++) z64[q] = (double)z[q];
+}
+
+int main(int argc, char **argv) {
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: pencil_emu <input> <output>\n");
+        return 2;
+    }
+    FILE *f = std::fopen(argv[1], "rb");
+    if (!f) return 2;
+    int hdr[5];
+    if (std::fread(hdr, sizeof(int), 5, f) != 5) return 2;
+    const int ni = hdr[0], nj = hdr[1], nk = hdr[2], m = hdr[3], fp64 = hdr[4];
+    const size_t N = (size_t)ni * nj * nk;
+    std::vector<int> cellRow(N), rowOf(N + 1);
+    std::vector<double> r(m), fwd(4 * (size_t)m), bwd(4 * (size_t)m), z(m);
+    bool ok = std::fread(cellRow.data(), sizeof(int), N, f) == N && std::fread(rowOf.data(), sizeof(int), N + 1, f) == N + 1 &&
+              std::fread(r.data(), sizeof(double), m, f) == (size_t)m &&
+              std::fread(fwd.data(), sizeof(double), 4 * (size_t)m, f) == 4 * (size_t)m &&
+              std::fread(bwd.data(), sizeof(double), 4 * (size_t)m, f) == 4 * (size_t)m;
+    std::fclose(f);
+    if (!ok) return 2;
+    try {
+        if (fp64)
+            apply<double, C64>(ni, nj, nk, m, cellRow, rowOf, r, fwd, bwd, z);
+        else
+            apply<float, C32>(ni, nj, nk, m, cellRow, rowOf, r, fwd, bwd, z);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pencil_emu: %s\n", e.what());
+        return 1;
+    }
+    f = std::fopen(argv[2], "wb");
+    if (!f) return 2;
+    std::fwrite(z.data(), sizeof(double), m, f);
+    std::fclose(f);
+    return 0;
+}

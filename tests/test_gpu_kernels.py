@@ -239,14 +239,16 @@ def _check_mic0_apply(n, schedule):
     assert same_bits(g32.astype(np.float32), _mic_sweeps(sysm, n3, r, np.float32))
 
 
-@pytest.mark.parametrize("schedule", [2, 3, 0], ids=["bands", "bands-device-check", "hyperplanes"])
+@pytest.mark.parametrize("schedule", [4, 2, 3, 0], ids=["line-pipeline", "bands", "bands-device-check",
+                                                     "hyperplanes"])
 @pytest.mark.parametrize("n", CASES + [70])
 def test_mic0_apply_bitexact(n, schedule):
-    """The preconditioner application in bands of hyperplanes (the solver's;
-    also with the tile check on the device, the path of grids whose tiles
-    outnumber a launch's tile mask) and one launch per hyperplane, against the
-    oracle's serial sweeps (FP64) and a hyperplane-vectorised FP32
-    restatement (mixed mode)."""
+    """The preconditioner application as a line pipeline (pencils of lines
+    along i, one launch per sweep), in bands of hyperplanes (also with the tile
+    check on the device, the path of grids whose tiles outnumber a launch's
+    tile mask) and one launch per hyperplane, against the oracle's serial
+    sweeps (FP64) and a hyperplane-vectorised FP32 restatement (mixed
+    mode)."""
     _check_mic0_apply(n, schedule=schedule)
 
 

@@ -244,6 +244,29 @@ def test_rollout_gradient_central_differences():
     ro.close()
 
 
+def test_rollout_gradient_linear_in_the_cotangent():
+    """The adjoint is linear in the density cotangent at every scale.  A
+    cotangent scaled by 2^-40 (an exact scaling; the adjoint's pressure
+    right-hand side then lies far below the solver tolerance taken as an
+    absolute number) gives the gradient scaled by 2^-40, bit for bit: the
+    adjoint's pressure solve is skipped only for a zero right-hand side."""
+    ro, init, target = _rollout_case(substeps=2, checkpoint_every=1)
+    rho = ro.forward(*init)
+    rb = (2.0 * (rho.astype(np.float64) - target)).astype(np.float32)
+    g = ro.backward(rb)
+    scale = np.float32(2.0 ** -40)
+    gs = ro.backward(rb * scale)
+    _, _, adjoint_iters = ro.stats()
+    say(f"rollout cotangent x 2^-40: |grad|_max {max(float(np.abs(a).max()) for a in gs):.6g}, "
+        f"adjoint PCG iterations {adjoint_iters}")
+    assert adjoint_iters > 0
+    for a, b in zip(g, gs):
+        assert np.array_equal((a * scale).view(np.uint32), b.view(np.uint32))
+    zero = ro.backward(np.zeros_like(rb))
+    assert all(not np.any(a) for a in zero)
+    ro.close()
+
+
 def test_rollout_autograd():
     torch = pytest.importorskip("torch")
     from ffengine import autograd as AG
