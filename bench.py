@@ -266,34 +266,14 @@ def precision_config(n, tol):
     print(json.dumps(line), flush=True)
 
 
-def splash_particles(n, seed=0, pool=0.3, center=(0.5, 0.65, 0.5), radius=0.2):
-    """8 jittered particles per cell in the pool (y < pool) and in the cells
-    whose centre lies inside the falling sphere of liquid."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    dx = 1.0 / n
-    c = (np.arange(n) + 0.5) * dx
-    chunks = []
-    for k in range(n):
-        zz, yy, xx = c[k], c[:, None], c[None, :]
-        inside = (yy < pool) | ((xx - center[0]) ** 2 + (yy - center[1]) ** 2 + (zz - center[2]) ** 2 < radius ** 2)
-        jj, ii = np.nonzero(np.broadcast_to(inside, (n, n)))
-        if len(ii) == 0:
-            continue
-        base = np.stack([ii, jj, np.full_like(ii, k)], -1).reshape(-1, 1, 3).astype(np.float32)
-        sub = np.array([[a, b, cc] for cc in (0, 1) for b in (0, 1) for a in (0, 1)], np.float32)[None] * 0.5
-        jit = rng.random((base.shape[0], 8, 3), dtype=np.float32) * 0.5
-        chunks.append(((base + sub + jit) * np.float32(dx)).reshape(-1, 3))
-    return np.concatenate(chunks).astype(np.float32)
-
-
 def splash_config(n, steps, warmup, tol, mode):
     """The large single-GPU configuration: a sphere of liquid (r = 0.2) falling
     into a pool over 30% of the height, 8 particles per liquid cell."""
     import torch
     _, total = torch.cuda.mem_get_info()
     log("splash n=%d: seeding particles" % n)
-    pos = splash_particles(n)
+    from ffengine import scenes
+    pos = scenes.splash_particles(n)
     log("  %d particles" % len(pos))
     res = run_mode(n, pos, None, mode, tol, steps, warmup, 1.0 / 240.0)
     rate = res["substeps"] / res["elapsed_s"]

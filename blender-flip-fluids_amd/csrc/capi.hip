@@ -6,6 +6,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -442,6 +443,9 @@ void FluidSimulation_get_last_pressure_solve_stats(FluidSimulation *o, int *it, 
                                                    int *err) {
     guarded(err, [&] { S(o)->getLastPressureSolveStats(it, e, ok, m); });
 }
+double FluidSimulation_get_last_pressure_solve_rhs_norm(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getLastPressureRhsNorm(); });
+}
 double FluidSimulation_get_last_substep_phase_ms(FluidSimulation *o, int phase, int *err) {
     return guarded(err, [&] { return S(o)->getLastPhaseMilliseconds(phase); });
 }
@@ -560,6 +564,55 @@ void FFAMD_extrapolate(int wi, int wj, int wk, float *field, const unsigned char
         for (int l = 0; l < layers; l++) ff::extrapolateLayer(wi, wj, wk, a, st, l, sc.s);
         down(field, a.p, n, sc.s);
         sync(sc.s);
+    });
+}
+
+// ---- the layered extrapolation without allocation ----
+// Its one scratch buffer, the status byte of every face, lives in this
+// file-scope device array: each GPU the library runs on holds its own copy
+// until the process ends, at the address hipGetSymbolAddress gives on that
+// device (the array's name is only a host-side stand-in).
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;  // faces of one call, at most
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchLock;
+unsigned char *g_scratchAddr[kMaxDevices] = {};
+
+// the scratch array's address on the current device, looked up on the
+// device's first call
+unsigned char *deviceScratch() {
+    int dev = -1;
+    ff::hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratchLock);
+    if (!g_scratchAddr[dev]) {
+        void *p = nullptr;
+        ff::hipCheck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratchAddr[dev] = (unsigned char *)p;
+    }
+    return g_scratchAddr[dev];
+}
+}  // namespace
+
+void FFAMD_extrapolate_noalloc(int wi, int wj, int wk, float *field, const unsigned char *valid, int layers,
+                               void *stream, int *err) {
+    guarded(err, [&] {
+        if (wi < 3 || wj < 3 || wk < 3 || layers < 0 || layers > 250)
+            throw std::domain_error("Error: face array dims must be >= 3 and layers in [0, 250].\n");
+        const size_t n = (size_t)wi * wj * wk;
+        if (n > kNoAllocScratchBytes)
+            throw std::domain_error("Error: FFAMD_extrapolate_noalloc takes at most " +
+                                    std::to_string(kNoAllocScratchBytes) + " faces per call; got " +
+                                    std::to_string(n) + ".\n");
+        if (!field || !valid) throw std::domain_error("Error: null field or flag array.\n");
+        unsigned char *st = deviceScratch();
+        hipStream_t s = (hipStream_t)stream;
+        // every status byte is written by the first kernel: nothing to clear
+        ff::extrapolateInit(wi, wj, wk, valid, st, s);
+        for (int l = 0; l < layers; l++) ff::extrapolateLayer(wi, wj, wk, field, st, l, s);
+        ff::hipCheck(hipGetLastError(), "extrapolation launch");
     });
 }
 

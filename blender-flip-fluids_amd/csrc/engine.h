@@ -176,6 +176,7 @@ public:
     void getVelocityField(float *u, float *v, float *w);
     void getPressureGrid(float *out);
     void getLastPressureSolveStats(int *iterations, double *error, int *success, int *unknowns);
+    double getLastPressureRhsNorm() const { return _lastRhsInf; }
     double getLastPhaseMilliseconds(int phase);
 
     // one substep of dt (for tests and the benchmark; no frame bookkeeping)
@@ -261,6 +262,7 @@ private:
     int _pIters = 0;
     float _pErr = 0.0f;
     int _lastIters = 0, _lastUnknowns = 0, _lastSuccess = 1;
+    double _lastRhsInf = 0.0;
     double _lastErr = 0.0;
     FFFrameStats _stats{};
     double _phaseMs[8] = {0, 0, 0, 0, 0, 0, 0, 0};

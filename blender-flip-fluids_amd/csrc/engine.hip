@@ -642,6 +642,7 @@ bool FluidSimulation::pressureSolve(double dt) {
                                  _scan, _stream, _ev[3], &_timers, _referenceSummation ? _maxThreads : 0);
     _lastIters = r.iterations;
     _lastErr = r.error;
+    _lastRhsInf = r.rhsInf;
     _lastSuccess = r.success ? 1 : 0;
     _lastUnknowns = r.unknowns;
     hipCheck(hipEventRecord(_ev[4], _stream), "event");

@@ -17,6 +17,7 @@ struct PressureResult {
     bool success = true;  // after the acceptable-tolerance rule
     int iterations = 0;
     double error = 0.0;   // |r|_inf at exit (the reference rounds it to float)
+    double rhsInf = 0.0;  // |b|_inf of the right-hand side
     int unknowns = 0;
 };
 

@@ -296,6 +296,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     } else {
         hipCheck(hipStreamSynchronize(s), "sync");
     }
+    res.rhsInf = maxAbs;
     if (afterAssembly) hipCheck(hipEventRecord(afterAssembly, s), "event");
     _levelLaunches = 0;
     for (int l = 0; l < _S.nLevels; l++) _levelLaunches += _hLevelStart[l + 1] > _hLevelStart[l];

@@ -442,7 +442,11 @@ class FluidSimulation(object):
         it, err, ok, m, s = c_int(), c_double(), c_int(), c_int(), c_int()
         f(self(), byref(it), byref(err), byref(ok), byref(m), byref(s))
         pb.check_success(s.value, f.__name__ + " - ")
-        return {"iterations": it.value, "error": err.value, "success": bool(ok.value), "unknowns": m.value}
+        f = lib.FluidSimulation_get_last_pressure_solve_rhs_norm
+        pb.init_lib_func(f, [c_void_p, c_void_p], c_double)
+        rhs = pb.execute_lib_func(f, [self()])
+        return {"iterations": it.value, "error": err.value, "success": bool(ok.value), "unknowns": m.value,
+                "rhs_inf": rhs}
 
     def get_last_substep_phase_ms(self, phase):
         f = lib.FluidSimulation_get_last_substep_phase_ms

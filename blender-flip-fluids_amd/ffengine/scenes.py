@@ -109,3 +109,23 @@ def slab_phi(n, height=0.4):
 def precision_slab(n, seed=0, height=0.4, amplitude=0.5, correlation=8.0):
     """Grid state of the precision configuration: (liquid phi, (u, v, w))."""
     return slab_phi(n, height), noise_mac_field(n, seed, amplitude, correlation)
+
+
+def splash_particles(n, seed=0, pool=0.3, center=(0.5, 0.65, 0.5), radius=0.2):
+    """8 jittered particles per cell in the pool (y < pool) and in the cells
+    whose centre lies inside the falling sphere of liquid."""
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / n
+    c = (np.arange(n) + 0.5) * dx
+    chunks = []
+    for k in range(n):
+        zz, yy, xx = c[k], c[:, None], c[None, :]
+        inside = (yy < pool) | ((xx - center[0]) ** 2 + (yy - center[1]) ** 2 + (zz - center[2]) ** 2 < radius ** 2)
+        jj, ii = np.nonzero(np.broadcast_to(inside, (n, n)))
+        if len(ii) == 0:
+            continue
+        base = np.stack([ii, jj, np.full_like(ii, k)], -1).reshape(-1, 1, 3).astype(np.float32)
+        sub = np.array([[a, b, cc] for cc in (0, 1) for b in (0, 1) for a in (0, 1)], np.float32)[None] * 0.5
+        jit = rng.random((base.shape[0], 8, 3), dtype=np.float32) * 0.5
+        chunks.append(((base + sub + jit) * np.float32(dx)).reshape(-1, 3))
+    return np.concatenate(chunks).astype(np.float32)

@@ -290,6 +290,10 @@ void FluidSimulation_get_pressure_grid_data(FluidSimulation *obj, float *data, i
 void FluidSimulation_get_last_pressure_solve_stats(FluidSimulation *obj, int *iterations,
                                                    double *error, int *success, int *unknowns,
                                                    int *err);
+/* |b|_inf of the last pressure solve's right-hand side (no counterpart in the
+ * reference): with get_last_pressure_solve_stats' |r|_inf it gives the
+ * relative residual the stop test uses. */
+double FluidSimulation_get_last_pressure_solve_rhs_norm(FluidSimulation *obj, int *err);
 /* GPU milliseconds of the last substep's phases: 0 level sets, 1 P2G and
  * extrapolation, 2 pressure system, 3 PCG, 4 projection and constraints,
  * 5 G2P and advection, 6 removal, 7 whole substep. */
@@ -316,6 +320,17 @@ void FFAMD_p2g(int isize, int jsize, int ksize, double dx, int np, const float *
                unsigned char *vu, unsigned char *vv, unsigned char *vw, int *err);
 void FFAMD_extrapolate(int wi, int wj, int wk, float *field, const unsigned char *valid,
                        int layers, int *err);
+/* FFAMD_extrapolate without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under HIP graph capture, tight serving loops and harnesses that forbid
+ * hipMalloc.  field (in place) and valid are DEVICE pointers of wi*wj*wk
+ * entries; the kernels are enqueued on `stream` (a hipStream_t, NULL for the
+ * default stream) and the call returns without waiting.  Its only scratch,
+ * one status byte per face, is a static device array of 64 MiB shared by
+ * every call on a GPU: at most 67108864 faces per call (larger sizes fail
+ * with an error), and calls that overlap on one GPU race. */
+void FFAMD_extrapolate_noalloc(int wi, int wj, int wk, float *field, const unsigned char *valid,
+                               int layers, void *stream, int *err);
 /* Adjoint of FFAMD_extrapolate (no counterpart in the reference): for the
  * same flags and layer count the extrapolation is linear in the field, and
  * this replaces bar (wk x wj x wi) by its transpose applied to bar. */

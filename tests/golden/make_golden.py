@@ -19,6 +19,18 @@ compressed .npz files (no pickled objects) plus one JSON file of checksums:
                    1/30 s: per frame the particle count and the SHA-256 of the
                    position and velocity bytes (the full snapshots would be
                    5.6 MB each).
+  obstacles_n32.json
+                   the 32^3 dam break (ffengine.scenes recipe) with static
+                   mesh obstacles (a sphere, an inversed sphere plus a
+                   sphere, each with and without friction), and with boundary
+                   friction plus an obstacle friction changed between frames:
+                   per frame the particle count and the SHA-256 of positions
+                   and velocities, 4 threads, rand() seeded per frame.
+  savestate_ref_n16.npz
+                   a savestate the reference engine writes through the
+                   add-on's layout after 2 frames of the 16^3 dam break with
+                   random velocities (the .data files and autosave.state as
+                   bytes), and the SHA-256 of its next 2 frames.
 
 Usage:  python tests/golden/make_golden.py [--threads 8] [--skip-c1]
 """
@@ -36,6 +48,10 @@ sys.path.insert(0, os.path.dirname(HERE))
 import ctypes as C  # noqa: E402
 
 import oracle_lib  # noqa: E402
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "blender-flip-fluids_amd"))
+from ffengine import scenes  # noqa: E402  (the scene recipes the GPU tests use)
+from ffengine.savestate import write_savestate  # noqa: E402
 
 fp = oracle_lib.fp
 
@@ -178,6 +194,130 @@ def make_c1(path, threads, frames=60):
     print("wrote", path)
 
 
+# the obstacle scenes of tests/test_gpu_mesh.py: (mesh, inversed, friction) per obstacle
+def obstacle_cases():
+    a = scenes.sphere_mesh((0.3, 0.3, 0.5), 0.12, 3)  # overlaps the initial fluid
+    c = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.62, 3)  # inversed: liquid only inside it
+    return {"sphere": [(a, False, 0.0)], "inversed_and_sphere": [(c, True, 0.0), (a, False, 0.0)],
+            "friction": [(a, False, 0.5)],
+            "inversed_and_sphere_friction": [(c, True, 0.3), (a, False, 0.8)]}
+
+
+def _frame_record(sim):
+    p, v = sim.particles()
+    return {"count": int(len(p)), "positions_sha256": sha(p), "velocities_sha256": sha(v)}
+
+
+def make_obstacles(path, threads=4):
+    n = 32
+    dx = 1.0 / n
+    rec = {"scene": "dam break 32^3 (ffengine.scenes.dam_break_particles), zero velocities, g=(0,-9.81,0), "
+                    "PIC/FLIP 0.05, CFL 5, %d threads, frames of 1/30 s" % threads, "threads": threads,
+           "cases": {}}
+    pos = scenes.dam_break_particles(n, seed=31)
+    for case, obstacles in obstacle_cases().items():
+        ref = oracle_lib.ReferenceSim((n, n, n), dx, threads=threads)
+        ref.load(pos)
+        for (v, t), inv, fr in obstacles:
+            ref.add_mesh_obstacle((n, n, n), dx, v, t, inversed=inv, friction=fr)
+        ref.initialize()
+        frames = []
+        for frame in range(3):
+            oracle_lib.libc_srand(1234 + frame)
+            ref.update(1.0 / 30.0)
+            frames.append(_frame_record(ref))
+        rec["cases"][case] = {"seed": 31, "srand": [1234, 1235, 1236], "initial_positions_sha256": sha(pos),
+                              "frames": frames}
+        print("obstacles", case, [f["count"] for f in frames], flush=True)
+    # boundary friction 0.4 and a sphere whose friction becomes 0.7 before frame 1
+    pos = scenes.dam_break_particles(n, seed=33)
+    v, t = scenes.sphere_mesh((0.3, 0.3, 0.5), 0.12, 3)
+    ref = oracle_lib.ReferenceSim((n, n, n), dx, threads=threads)
+    ref.load(pos)
+    robj = ref.add_mesh_obstacle((n, n, n), dx, v, t)
+    ref.set_boundary_friction(0.4)
+    ref.initialize()
+    frames = []
+    for frame in range(3):
+        if frame == 1:
+            ref.set_obstacle_friction(robj, 0.7)
+        oracle_lib.libc_srand(77 + frame)
+        ref.update(1.0 / 30.0)
+        frames.append(_frame_record(ref))
+    rec["boundary_live_friction"] = {"seed": 33, "boundary_friction": 0.4, "obstacle_friction_from_frame_1": 0.7,
+                                     "srand": [77, 78, 79], "initial_positions_sha256": sha(pos), "frames": frames}
+    with open(path, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    print("wrote", path)
+
+
+class _ReferenceAdapter:
+    """The reference engine's C ABI behind the Python methods the savestate
+    writer calls."""
+
+    def __init__(self, ref, n3, dx):
+        self.ref, self.n3, self.dx = ref, n3, dx
+        ref.L.FluidSimulation_get_current_frame.restype = C.c_int
+
+    def _range(self, name, start, end):
+        out = (C.c_char * (12 * (end - start)))()
+        err = C.c_int()
+        getattr(self.ref.L, name)(self.ref.h, start, end, out, C.byref(err))
+        assert err.value == 1
+        return bytes(out)
+
+    def get_grid_dimensions(self):
+        return self.n3
+
+    def get_cell_size(self):
+        return self.dx
+
+    def is_velocity_transfer_method_APIC(self):
+        return False
+
+    def get_num_marker_particles(self):
+        err = C.c_int()
+        return self.ref.L.FluidSimulation_get_num_marker_particles(self.ref.h, C.byref(err))
+
+    def get_current_frame(self):
+        err = C.c_int()
+        return self.ref.L.FluidSimulation_get_current_frame(self.ref.h, C.byref(err))
+
+    def get_marker_particle_position_data_range(self, start, end):
+        return self._range("FluidSimulation_get_marker_particle_position_data_range", start, end)
+
+    def get_marker_particle_velocity_data_range(self, start, end):
+        return self._range("FluidSimulation_get_marker_particle_velocity_data_range", start, end)
+
+
+def make_savestate(path, threads=4):
+    import tempfile
+    n = 16
+    dx = 1.0 / n
+    pos = scenes.dam_break_particles(n, seed=23)
+    vel = (np.random.default_rng(24).standard_normal(pos.shape) * 0.2).astype(np.float32)
+    ref = oracle_lib.ReferenceSim((n, n, n), dx, threads=threads)
+    ref.load(pos, vel)
+    ref.initialize()
+    for _ in range(2):
+        ref.update(1.0 / 30.0)
+    out = {"n": np.int32(n), "threads": np.int32(threads)}
+    with tempfile.TemporaryDirectory() as d:
+        write_savestate(_ReferenceAdapter(ref, (n, n, n), dx), d, frameno=1)
+        for key, name in (("position", "marker_particle_position.data"),
+                          ("velocity", "marker_particle_velocity.data"), ("state", "autosave.state")):
+            with open(os.path.join(d, name), "rb") as fh:
+                out[key] = np.frombuffer(fh.read(), np.uint8)
+    hashes = []
+    for _ in range(2):
+        ref.update(1.0 / 30.0)
+        r = _frame_record(ref)
+        hashes.append([r["positions_sha256"], r["velocities_sha256"]])
+    out["next_frames_sha256"] = np.array(hashes)
+    np.savez_compressed(path, **out)
+    print("wrote", path)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=8)
@@ -187,6 +327,8 @@ def main():
         sys.exit("the reference build is missing: run `make -C oracle ref` first")
     make_stages(os.path.join(HERE, "stages_n16.npz"))
     make_e2e16(os.path.join(HERE, "e2e_n16.npz"), args.threads)
+    make_obstacles(os.path.join(HERE, "obstacles_n32.json"))
+    make_savestate(os.path.join(HERE, "savestate_ref_n16.npz"))
     if not args.skip_c1:
         make_c1(os.path.join(HERE, "e2e_c1_n64.json"), args.threads)
 

@@ -6,12 +6,17 @@
   tests/test_oracle_mesh.py), with rand() seeded identically before each;
 - the engine with a MeshObject obstacle: its solid SDF after the first
   update equals the oracle's, and whole frames equal the reference engine's
-  (built from its sources) bit for bit when both take the same rand() seed;
+  bit for bit when both take the same rand() seed (the reference's per-frame
+  checksums in tests/golden/obstacles_n32.json);
 - enable/disable and removal rebuild the solid; animated meshes, duplicate
   adds and unknown removals raise;
 - obstacle friction (the constraint's closest-object friction on partly open
-  faces): frames equal the reference engine's bit for bit.
+  faces) and boundary friction: frames equal the reference engine's bit for
+  bit (same fixture).
 """
+import hashlib
+import json
+import os
 import sys
 
 import numpy as np
@@ -22,7 +27,10 @@ from ffengine import FluidSimulation, MeshObject, TriangleMesh
 from ffengine import kernels as K
 from ffengine import scenes
 
+from golden.make_golden import obstacle_cases  # the fixture's obstacle scenes
+
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def say(msg):
@@ -159,74 +167,66 @@ def test_unsupported_obstacles_raise():
         bad.update_mesh_static(TriangleMesh(v, np.array([[0, 1, len(v)]], np.int32)))
 
 
-@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref reference build absent")
+def _fixture():
+    with open(os.path.join(GOLDEN, "obstacles_n32.json")) as fh:
+        return json.load(fh)
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _check_frame(sim, want, what):
+    pg, vg = _state(sim)
+    got = {"count": len(pg) // 3, "positions_sha256": _sha(pg), "velocities_sha256": _sha(vg)}
+    say(f"{what}: {got['count']} particles (reference {want['count']}), identical "
+        f"{got['positions_sha256'] == want['positions_sha256']}")
+    assert got["count"] == want["count"], what
+    assert got["positions_sha256"] == want["positions_sha256"], what
+    assert got["velocities_sha256"] == want["velocities_sha256"], what
+
+
 def test_boundary_and_live_friction_match_reference():
     """Boundary friction (the domain object's) and an obstacle's friction
-    changed between frames, which the reference reads live."""
+    changed between frames, which the reference reads live: every frame's
+    positions and velocities against the reference engine's checksums
+    (tests/golden/obstacles_n32.json, written by make_golden.py)."""
+    rec = _fixture()["boundary_live_friction"]
     n = 32
-    dx = 1.0 / n
-    pos = scenes.dam_break_particles(n, seed=33)
+    pos = scenes.dam_break_particles(n, seed=rec["seed"])
+    assert _sha(pos) == rec["initial_positions_sha256"]
     v, t = scenes.sphere_mesh((0.3, 0.3, 0.5), 0.12, 3)
-    ref = O.ReferenceSim((n, n, n), dx, threads=4)
-    ref.load(pos)
-    robj = ref.add_mesh_obstacle((n, n, n), dx, v, t)
-    ref.set_boundary_friction(0.4)
-    ref.initialize()
-    sim = _sim(n, pos, threads=4)
+    sim = _sim(n, pos, threads=_fixture()["threads"])
     mo = _obstacle(n, v, t)
     sim.add_mesh_obstacle(mo)
-    sim.boundary_friction = 0.4
+    sim.boundary_friction = rec["boundary_friction"]
     assert abs(sim.boundary_friction - 0.4) < 1e-12
     with pytest.raises(RuntimeError):
         sim.boundary_friction = 1.5
     sim.initialize()
-    for frame in range(3):
+    for frame, want in enumerate(rec["frames"]):
         if frame == 1:  # the obstacle's friction changes without any other state change
-            ref.set_obstacle_friction(robj, 0.7)
-            mo.friction = 0.7
-        O.libc_srand(77 + frame)
-        ref.update(1.0 / 30.0)
-        O.libc_srand(77 + frame)
+            mo.friction = rec["obstacle_friction_from_frame_1"]
+        O.libc_srand(rec["srand"][frame])
         sim.update(1.0 / 30.0)
-        pr, vr = ref.particles()
-        pg, vg = _state(sim)
-        same = len(pr) * 3 == len(pg) and np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32))
-        say(f"boundary/live friction frame {frame}: identical {same}")
-        assert len(pr) * 3 == len(pg), frame
-        assert np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32)), frame
-        assert np.array_equal(vr.reshape(-1).view(np.uint32), vg.view(np.uint32)), frame
+        _check_frame(sim, want, f"boundary/live friction frame {frame}")
 
 
-@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref reference build absent")
 @pytest.mark.parametrize("case", ["sphere", "inversed_and_sphere", "friction", "inversed_and_sphere_friction"])
 def test_engine_frames_with_obstacles_match_reference(case):
+    """Whole frames with static mesh obstacles (with and without friction)
+    against the reference engine's per-frame checksums."""
+    fx = _fixture()
+    rec = fx["cases"][case]
     n = 32
-    dx = 1.0 / n
-    pos = scenes.dam_break_particles(n, seed=31)
-    a = scenes.sphere_mesh((0.3, 0.3, 0.5), 0.12, 3)  # overlaps the initial fluid
-    c = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.62, 3)  # inversed: liquid only inside it
-    obstacles = {"sphere": [(a, False, 0.0)], "inversed_and_sphere": [(c, True, 0.0), (a, False, 0.0)],
-                 "friction": [(a, False, 0.5)],
-                 "inversed_and_sphere_friction": [(c, True, 0.3), (a, False, 0.8)]}[case]
-    ref = O.ReferenceSim((n, n, n), dx, threads=4)
-    ref.load(pos)
-    for (v, t), inv, fr in obstacles:
-        ref.add_mesh_obstacle((n, n, n), dx, v, t, inversed=inv, friction=fr)
-    ref.initialize()
-    sim = _sim(n, pos, threads=4)
-    keep = [_obstacle(n, v, t, inv, fr) for (v, t), inv, fr in obstacles]
+    pos = scenes.dam_break_particles(n, seed=rec["seed"])
+    assert _sha(pos) == rec["initial_positions_sha256"]
+    sim = _sim(n, pos, threads=fx["threads"])
+    keep = [_obstacle(n, v, t, inv, fr) for (v, t), inv, fr in obstacle_cases()[case]]
     for mo in keep:
         sim.add_mesh_obstacle(mo)
     sim.initialize()
-    for frame in range(3):
-        O.libc_srand(1234 + frame)
-        ref.update(1.0 / 30.0)
-        O.libc_srand(1234 + frame)
+    for frame, want in enumerate(rec["frames"]):
+        O.libc_srand(rec["srand"][frame])
         sim.update(1.0 / 30.0)
-        pr, vr = ref.particles()
-        pg, vg = _state(sim)
-        same = len(pr) * 3 == len(pg) and np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32))
-        say(f"obstacles {case} frame {frame}: {len(pr)} / {len(pg) // 3} particles, identical {same}")
-        assert len(pr) * 3 == len(pg), frame
-        assert np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32)), frame
-        assert np.array_equal(vr.reshape(-1).view(np.uint32), vg.view(np.uint32)), frame
+        _check_frame(sim, want, f"obstacles {case} frame {frame}")

@@ -3,23 +3,24 @@
 uninterrupted run.
 
 - GPU engine to GPU engine, FLIP and APIC;
-- across engines: a savestate the reference engine (built from its sources,
-  oracle/_ref) writes loads into the GPU engine, and the GPU engine's next
-  frame equals the reference's bit for bit (FP64 PCG with the reference's
-  summation order); skipped where that build is absent.
+- across engines: a savestate the reference engine wrote (a committed
+  fixture) loads into the GPU engine, and the GPU engine's next frames equal
+  the reference's bit for bit (FP64 PCG with the reference's summation
+  order).
 """
-import ctypes as C
+import hashlib
+import os
 import sys
 
 import numpy as np
 import pytest
 
-import oracle_lib as O
 from ffengine import scenes
 from ffengine.fluidsimulation import FluidSimulation
 from ffengine.savestate import load_savestate, read_savestate_info, write_savestate
 
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def say(msg):
@@ -81,69 +82,27 @@ def test_resume_matches_uninterrupted_run(tmp_path, apic):
                 assert getattr(a, g)(0, m) == getattr(b, g)(0, m), frame
 
 
-class _ReferenceAdapter:
-    """The reference engine's C ABI behind the few Python methods the
-    savestate writer uses."""
-
-    def __init__(self, ref, n3, dx):
-        self.ref, self.n3, self.dx = ref, n3, dx
-        L = ref.L
-        L.FluidSimulation_get_current_frame.restype = C.c_int
-
-    def _range(self, name, start, end):
-        out = (C.c_char * (12 * (end - start)))()
-        err = C.c_int()
-        getattr(self.ref.L, name)(self.ref.h, start, end, out, C.byref(err))
-        assert err.value == 1
-        return bytes(out)
-
-    def get_grid_dimensions(self):
-        return self.n3
-
-    def get_cell_size(self):
-        return self.dx
-
-    def is_velocity_transfer_method_APIC(self):
-        return False
-
-    def get_num_marker_particles(self):
-        err = C.c_int()
-        return self.ref.L.FluidSimulation_get_num_marker_particles(self.ref.h, C.byref(err))
-
-    def get_current_frame(self):
-        err = C.c_int()
-        return self.ref.L.FluidSimulation_get_current_frame(self.ref.h, C.byref(err))
-
-    def get_marker_particle_position_data_range(self, start, end):
-        return self._range("FluidSimulation_get_marker_particle_position_data_range", start, end)
-
-    def get_marker_particle_velocity_data_range(self, start, end):
-        return self._range("FluidSimulation_get_marker_particle_velocity_data_range", start, end)
-
-
-@pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref reference build absent")
 def test_reference_savestate_resumes_on_gpu(tmp_path):
-    n = 24
-    dx = 1.0 / n
-    pos = scenes.dam_break_particles(n, seed=23)
-    vel = (np.random.default_rng(24).standard_normal(pos.shape) * 0.2).astype(np.float32)
-    ref = O.ReferenceSim((n, n, n), dx, threads=4)
-    ref.load(pos, vel)
-    ref.initialize()
-    for _ in range(2):
-        ref.update(1.0 / 30.0)
-    info = write_savestate(_ReferenceAdapter(ref, (n, n, n), dx), str(tmp_path), frameno=1)
+    """A savestate the reference engine wrote (tests/golden/savestate_ref_n16.npz,
+    make_golden.py: the 16^3 dam break with random velocities after 2 frames)
+    loads into the GPU engine, whose next 2 frames equal the reference's bit
+    for bit (their SHA-256 in the fixture)."""
+    fx = np.load(os.path.join(GOLDEN, "savestate_ref_n16.npz"))
+    n = int(fx["n"])
+    for key, name in (("position", "marker_particle_position.data"), ("velocity", "marker_particle_velocity.data"),
+                      ("state", "autosave.state")):
+        (tmp_path / name).write_bytes(fx[key].tobytes())
+    info = read_savestate_info(str(tmp_path))
     say(f"reference savestate: frame_id {info['frame_id']}, {info['num_marker_particles']} particles")
-    gpu = _sim(n, threads=4)
+    gpu = _sim(n, threads=int(fx["threads"]))
     load_savestate(gpu, str(tmp_path))
     gpu.initialize()
-    for frame in range(2):
-        ref.update(1.0 / 30.0)
+    assert gpu.get_current_frame() == info["frame_id"] + 1
+    for frame, (want_p, want_v) in enumerate(fx["next_frames_sha256"]):
         gpu.update(1.0 / 30.0)
-        pr, vr = ref.particles()
         pg, vg = _state(gpu)
-        same = np.array_equal(pr.reshape(-1).view(np.uint32), pg.view(np.uint32))
+        same = hashlib.sha256(pg.tobytes()).hexdigest() == want_p
         say(f"reference -> GPU resume frame {frame}: identical {same}")
         assert same, frame
-        assert np.array_equal(vr.reshape(-1).view(np.uint32), vg.view(np.uint32)), frame
-    assert read_savestate_info(str(tmp_path))["marker_particle_position_filedata"] == "marker_particle_position.data"
+        assert hashlib.sha256(vg.tobytes()).hexdigest() == want_v, frame
+    assert info["marker_particle_position_filedata"] == "marker_particle_position.data"
