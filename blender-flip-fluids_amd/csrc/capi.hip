@@ -1128,6 +1128,13 @@ void FFAMD_rollout_positions(FFAMD_Rollout *r, float *pos, int *err) {
     guarded(err, [&] { R(r)->positions(pos); });
 }
 
+void FFAMD_rollout_set_reference_summation(FFAMD_Rollout *r, int threads, int *err) {
+    guarded(err, [&] {
+        if (threads < 0 || threads > 4096) throw std::domain_error("Error: thread count must be in [0, 4096].\n");
+        R(r)->setReferenceSummation(threads);
+    });
+}
+
 void FFAMD_rollout_stats(FFAMD_Rollout *r, long long *collided, long long *fwd, long long *bwd, int *err) {
     guarded(err, [&] {
         *collided = R(r)->collided();

@@ -60,6 +60,10 @@ public:
     long long forwardIterations() const { return _fwdIters; }
     long long backwardIterations() const { return _bwdIters; }
     int particles() const { return _np; }
+    // FP64 dot products of every pressure solve in the reference's summation
+    // order for `threads` threads (blaswrapper.h:27-57), or 0 for the
+    // parallel tree (the default)
+    void setReferenceSummation(int threads) { _refDots = threads < 0 ? 0 : threads; }
 
 private:
     Grid _g;
@@ -71,6 +75,7 @@ private:
     hipStream_t _s = nullptr;
     bool _recorded = false, _haveForward = false;
     long long _collided = 0, _fwdIters = 0, _bwdIters = 0, _iters = 0;
+    int _refDots = 0;
 
     DevBuf<float> _pos0;            // 3P: seeded positions (SoA)
     DevBuf<float> _state;           // 6P: working particle state (px py pz vx vy vz)

@@ -185,7 +185,7 @@ void Rollout::substep(int k, bool freeze, bool record) {
     for (int d = 0; d < 3; d++)
         if (std::fabs(bf[d]) > 1e-6f) addToFaces(_u[d], _nf[d], (float)(bf[d] * _st.dt), _s);
     PressureResult r = _ps.solve(_st.dt, _u[0], _u[1], _u[2], _wt[0], _wt[1], _wt[2], _phi, _st.tol, _st.maxIter, 0,
-                                 1e-4, _pressure, _scan, _s, nullptr);
+                                 1e-4, _pressure, _scan, _s, nullptr, nullptr, _refDots);
     _iters += r.iterations;
     if (!r.success) throw std::runtime_error("rollout: a pressure solve did not converge");
     applyPressure(g, _st.dt, _pressure, _phi, _wt[0], _wt[1], _wt[2], _u[0], _u[1], _u[2], _valid[0], _valid[1],
@@ -273,7 +273,8 @@ void Rollout::adjointSubstep(const float *start) {
     // pressure projection: K ubar + B^T A^-1 (C^T ubar), the same FP64 solve
     projectionAdjointCells(g, _st.dt, _phi, _wt[0], _wt[1], _wt[2], _ub[0], _ub[1], _ub[2], _cellBar, _s);
     PressureResult r = _ps.solve(_st.dt, _zero[0], _zero[1], _zero[2], _wt[0], _wt[1], _wt[2], _phi, _st.tol,
-                                 _st.maxIter, 0, 1e-4, _pressure, _scan, _s, nullptr, nullptr, 0, _cellBar, _lambda);
+                                 _st.maxIter, 0, 1e-4, _pressure, _scan, _s, nullptr, nullptr, _refDots, _cellBar,
+                                 _lambda);
     _bwdIters += r.iterations;  // the adjoint solves only; recomputed forward solves are not counted
     if (!r.success) throw std::runtime_error("rollout: an adjoint pressure solve did not converge");
     projectionAdjointFaces(g, _st.dt, _phi, _wt[0], _wt[1], _wt[2], _ub[0], _ub[1], _ub[2], _lambda, _tmp[0], _tmp[1],

@@ -103,3 +103,9 @@ class Rollout:
         c, f, b = c_longlong(), c_longlong(), c_longlong()
         _call("FFAMD_rollout_stats", [c_void_p] * 4, [self._h, byref(c), byref(f), byref(b)])
         return c.value, f.value, b.value
+
+    def set_reference_summation(self, threads):
+        """FP64 dot products of every pressure solve in the reference's
+        summation order for `threads` threads (blaswrapper.h:27-57); 0 restores
+        the parallel tree."""
+        _call("FFAMD_rollout_set_reference_summation", [c_void_p, c_int], [self._h, int(threads)])

@@ -481,6 +481,12 @@ void FFAMD_rollout_forward(FFAMD_Rollout *r, const float *u0, const float *v0, c
 void FFAMD_rollout_backward(FFAMD_Rollout *r, const float *density_bar, float *grad_u,
                             float *grad_v, float *grad_w, int *err);
 void FFAMD_rollout_positions(FFAMD_Rollout *r, float *pos, int *err);
+/* Dot products of every pressure solve (forward and adjoint) in the
+ * reference's summation order for `threads` threads (blaswrapper.h:27-57),
+ * as FluidSimulation_set_pressure_solver_reference_summation does for the
+ * engine; 0 restores the parallel tree.  With it a forward takes the CPU
+ * oracle's bits. */
+void FFAMD_rollout_set_reference_summation(FFAMD_Rollout *r, int threads, int *err);
 /* particles moved by the collision response (summed over the last forward's
  * substeps) and the pressure iterations of the last forward and backward */
 void FFAMD_rollout_stats(FFAMD_Rollout *r, long long *collided, long long *forward_iterations,

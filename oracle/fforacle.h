@@ -159,6 +159,29 @@ double oracle_sim_last_pressure_error(OracleSim *s);
 void oracle_sim_get_grids(OracleSim *s, float *liquid_phi, float *u, float *v,
                           float *w, float *pressure);
 
+/* ---- the differentiable rollout and its FP64 adjoint (no counterpart in the
+ * reference; the HIP version is FFAMD_rollout_*) ----
+ * substeps substeps of length dt of the engine's substep without particle
+ * removal, from np particles at pos whose velocities are the trilinear sample
+ * of the initial MAC field (u0, v0, w0), then the particle density on the
+ * cells.  forward: freeze = 0 records the liquid SDF and P2G flags of every
+ * substep, freeze = 1 reuses the recorded ones; returns 0, or -1 when a
+ * pressure solve failed.  backward: the gradient of <density_bar, density>
+ * with respect to the initial field of the last recording forward, in
+ * double (masks held fixed, the collision response as the identity);
+ * returns 0 or -1. */
+typedef struct OracleRollout OracleRollout;
+OracleRollout *oracle_rollout_new(int ni, int nj, int nk, double dx, int np, const float *pos,
+                                  const float *solid_nodes, double dt, int substeps, double picflip,
+                                  double gx, double gy, double gz, double cfl, double tol, int max_iter);
+void oracle_rollout_destroy(OracleRollout *r);
+int oracle_rollout_forward(OracleRollout *r, const float *u0, const float *v0, const float *w0,
+                           int freeze, float *density);
+int oracle_rollout_backward(OracleRollout *r, const double *density_bar, double *gu, double *gv,
+                            double *gw);
+void oracle_rollout_positions(OracleRollout *r, float *pos);
+long long oracle_rollout_iterations(OracleRollout *r);
+
 #ifdef __cplusplus
 }
 #endif

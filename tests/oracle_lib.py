@@ -484,3 +484,60 @@ class ReferenceSim:
         out = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
         probe().refprobe_sim_get_solid_phi(self.h, fp(out))
         return out
+
+
+class OracleRollout:
+    """The differentiable rollout on the CPU: the oracle engine's float
+    substeps without particle removal, and the FP64 adjoint of that map
+    (oracle_rollout_* in oracle/fforacle.h)."""
+
+    def __init__(self, n3, dx, pos, dt, substeps, solid=None, picflip=0.05, gravity=(0.0, -9.81, 0.0), cfl=5.0,
+                 tol=1e-9, max_iter=900):
+        L = oracle()
+        L.oracle_rollout_new.restype = C.c_void_p
+        L.oracle_rollout_new.argtypes = ([C.c_int] * 3 + [C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                                          C.c_int] + [C.c_double] * 6 + [C.c_int])
+        L.oracle_rollout_destroy.argtypes = [C.c_void_p]
+        L.oracle_rollout_forward.argtypes = [C.c_void_p] * 4 + [C.c_int, C.c_void_p]
+        L.oracle_rollout_backward.argtypes = [C.c_void_p] * 5
+        L.oracle_rollout_positions.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_rollout_iterations.argtypes = [C.c_void_p]
+        L.oracle_rollout_iterations.restype = C.c_longlong
+        self.L, self.n3, self.dx = L, tuple(n3), dx
+        self.pos = _f32(pos).reshape(-1, 3)
+        sp = None if solid is None else _f32(solid)
+        self._solid = sp
+        self.h = C.c_void_p(L.oracle_rollout_new(*n3, dx, len(self.pos), fp(self.pos),
+                                                 None if sp is None else fp(sp), dt, substeps, picflip, *gravity,
+                                                 cfl, tol, max_iter))
+        ni, nj, nk = n3
+        self.shapes = [(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]
+
+    def forward(self, u0, v0, w0, freeze=False):
+        ni, nj, nk = self.n3
+        f = [_f32(a).reshape(s) for a, s in zip((u0, v0, w0), self.shapes)]
+        rho = np.zeros((nk, nj, ni), np.float32)
+        if self.L.oracle_rollout_forward(self.h, fp(f[0]), fp(f[1]), fp(f[2]), 1 if freeze else 0, fp(rho)) != 0:
+            raise RuntimeError("oracle rollout forward failed")
+        return rho
+
+    def backward(self, density_bar):
+        ni, nj, nk = self.n3
+        rb = np.ascontiguousarray(density_bar, np.float64).reshape(nk, nj, ni)
+        g = [np.zeros(s, np.float64) for s in self.shapes]
+        if self.L.oracle_rollout_backward(self.h, fp(rb), fp(g[0]), fp(g[1]), fp(g[2])) != 0:
+            raise RuntimeError("oracle rollout backward failed")
+        return g
+
+    def positions(self):
+        out = np.zeros_like(self.pos)
+        self.L.oracle_rollout_positions(self.h, fp(out))
+        return out
+
+    def iterations(self):
+        return self.L.oracle_rollout_iterations(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oracle_rollout_destroy(self.h)
+            self.h = None

@@ -1,0 +1,66 @@
+"""Scenes shared by the CPU (oracle against the reference build) and GPU
+(engine against the oracle) tests of one behaviour."""
+import numpy as np
+
+
+def removal_scene(n, seed=0):
+    """A resting block of liquid (8 jittered particles per cell, clear of the
+    walls) plus the two inputs that make _removeMarkerParticles
+    (fluidsimulation.cpp:7773-7851) remove particles of its own accord:
+
+    - 300 particles crowded into one cell of the block (the cap keeps 250,
+      counting in particle order, 7833-7838);
+    - 8 particles in the air above the block moving at 100 m/s, above the
+      speed limit _getMarkerParticleSpeedLimit derives (7723-7771): the
+      histogram's top bin holds more than its removal budget, and the 8 sit
+      within 10% of the fastest speed, so the outlier rule does not lower
+      the limit below 6 CFL steps per frame.
+
+    Returns (positions, velocities, crowd cell, the 8 fast particles' slice)."""
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / n
+    lo, hi = (3, 3, 3), (n // 2, n // 2, n - 3)
+    kk, jj, ii = np.meshgrid(*(np.arange(lo[a], hi[a]) for a in (2, 1, 0)), indexing="ij")
+    base = np.stack([ii, jj, kk], -1).reshape(-1, 1, 3).astype(np.float64)
+    block = ((base + rng.random((base.shape[0], 8, 3))) * dx).reshape(-1, 3)
+    cell = np.array([lo[0] + 2, lo[1] + 2, lo[2] + 3])
+    crowd = (cell + 0.05 + 0.9 * rng.random((300, 3))) * dx
+    fast = (np.array([n // 2 + 2, n // 2 + 3, n // 2]) + 0.5 + 0.8 * (rng.random((8, 3)) - 0.5)) * dx
+    pos = np.concatenate([block, crowd, fast]).astype(np.float32)
+    vel = np.zeros_like(pos)
+    dirs = np.array([0.6, 0.8, 0.0]) + 0.01 * rng.standard_normal((8, 3))
+    vel[-8:] = (100.0 * dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    return pos, vel, tuple(int(c) for c in cell), slice(len(pos) - 8, len(pos))
+
+
+def ball_particles(n, seed, center=(0.5, 0.55, 0.5), radius=0.22):
+    """8 jittered particles per cell in the cells whose centre lies in the
+    ball, float32 (P, 3)."""
+    from ffengine import scenes
+    pos = scenes.dam_break_particles(n, seed=seed, fraction=(1.0, 1.0, 1.0))
+    dx = 1.0 / n
+    cell = (np.floor(pos / dx) + 0.5) * dx
+    keep = np.linalg.norm(cell - np.array(center), axis=1) < radius
+    return np.ascontiguousarray(pos[keep])
+
+
+def differentiation_scene(n, seed=0):
+    """The differentiation configuration's scene at n^3: the dam break block
+    [0, 0.375) x [0, 0.625) x [0, 1), 8 particles per cell, keeping those whose
+    cell lies outside the walls (all 8 nodal solid values positive); the
+    initial MAC field at rest; the target's initial field a seeded smoothed
+    noise of 0.2 m/s.  Returns (positions, initial field, target field)."""
+    import oracle_lib as O
+    from ffengine import scenes
+    dx = 1.0 / n
+    pos = scenes.dam_break_particles(n, seed=seed)
+    solid = O.solid_box_phi((n, n, n), dx).reshape(n + 1, n + 1, n + 1)
+    c = np.floor(pos / dx).astype(int)
+    free = np.ones(len(pos), bool)
+    for o in range(8):
+        a, b, e = o & 1, (o >> 1) & 1, o >> 2
+        free &= solid[c[:, 2] + e, c[:, 1] + b, c[:, 0] + a] > 0
+    shapes = [(n, n, n + 1), (n, n + 1, n), (n + 1, n, n)]
+    init = [np.zeros(s, np.float32) for s in shapes]
+    target_init = scenes.noise_mac_field(n, seed=seed + 1, amplitude=0.2, correlation=8.0)
+    return np.ascontiguousarray(pos[free]), init, target_init

@@ -194,3 +194,40 @@ def test_warm_start_meets_the_same_tolerance():
         say(f"warm start frame {frame}: iterations cold {ic[-1]} warm {iw[-1]}, max |dx| = {d:.3g} cells")
         assert d < 1e-2, frame
     assert sum(iw[1:]) < sum(ic[1:])
+
+
+def test_removal_rules_match_oracle():
+    """_removeMarkerParticles (fluidsimulation.cpp:7773-7851) with both of its
+    own rules firing, against the oracle (itself pinned to the reference build
+    on this scene) bit for bit over two frames: a cell crowded with 300
+    particles, which the 250-per-cell cap thins in particle order, and 8
+    particles at 100 m/s, above the speed limit of
+    _getMarkerParticleSpeedLimit (7723-7771).  No gravity, so the crowd stays
+    in its cell until the cap acts."""
+    from scene_util import removal_scene
+    n = 20
+    dx = 1.0 / n
+    pos, vel, _, _ = removal_scene(n, seed=5)
+    sim = FluidSimulation(n, n, n, dx)
+    sim.PICFLIP_ratio = 0.05
+    sim.CFL_condition_number = 5
+    sim.enable_surface_reconstruction = False
+    sim.max_thread_count = 1
+    sim.set_pressure_solver_reference_summation(True)
+    sim.load_marker_particle_data(len(pos), pos.tobytes(), vel.tobytes())
+    sim.initialize()
+    orc = O.OracleSim((n, n, n), dx, gravity=(0.0, 0.0, 0.0))
+    orc.load(pos, vel)
+    orc.initialize()
+    limit = 6 * 5 * dx * 30.0  # 6 CFL steps per frame at dt = 1/30
+    for frame in range(2):
+        sim.update(1.0 / 30.0)
+        orc.update(1.0 / 30.0)
+        gp, gv = gpu_particles(sim)
+        op, ov = orc.particles()
+        say(f"removal frame {frame}: {len(gp)} particles (oracle {len(op)}, seeded {len(pos)})")
+        assert len(gp) == len(op), frame
+        assert np.array_equal(gp.view(np.uint32), op.view(np.uint32)), frame
+        assert np.array_equal(gv.view(np.uint32), ov.view(np.uint32)), frame
+        assert len(gp) <= len(pos) - 58, frame  # the cap took at least 50, the limit the 8
+        assert float(np.sqrt((gv.astype(np.float64) ** 2).sum(1)).max()) < limit, frame

@@ -166,3 +166,31 @@ def test_engine_apic_matches_reference(n, frames):
         po, vo = orc.particles()
         assert bits_equal(po, pr) and bits_equal(vo, vr), f
         assert bits_equal(orc.affine(), ref.affine()), f
+
+
+def test_removal_matches_reference():
+    """_removeMarkerParticles (fluidsimulation.cpp:7773-7851) with both of its
+    own removal rules firing: a cell crowded with 300 particles (the cap keeps
+    250, counted in particle order) and 8 particles at 100 m/s, above the
+    speed limit of _getMarkerParticleSpeedLimit (7723-7771).  Two frames,
+    oracle against the reference bit for bit."""
+    from scene_util import removal_scene
+    n = 20
+    dx = 1.0 / n
+    pos, vel, _, _ = removal_scene(n, seed=5)
+    # no gravity: the crowd stays in its cell until the cap thins it
+    ref = O.ReferenceSim((n, n, n), dx, threads=1, gravity=(0.0, 0.0, 0.0))
+    orc = O.OracleSim((n, n, n), dx, gravity=(0.0, 0.0, 0.0))
+    for sim in (ref, orc):
+        sim.load(pos, vel)
+        sim.initialize()
+    limit = 6 * 5 * dx * 30.0  # 6 CFL steps per frame, dt = 1/30
+    for f in range(2):
+        ref.update(1.0 / 30.0)
+        orc.update(1.0 / 30.0)
+        pr, vr = ref.particles()
+        po, vo = orc.particles()
+        assert bits_equal(po, pr) and bits_equal(vo, vr), f
+        # both rules fired: the cap took at least 50, the speed limit the 8
+        assert len(po) <= len(pos) - 58, f
+        assert float(np.sqrt((vo.astype(np.float64) ** 2).sum(1)).max()) < limit, f
