@@ -1,10 +1,18 @@
 /*
  * fforacle.cpp -- CPU oracle for the FLIP substep path.
  *
- * TEST INFRASTRUCTURE ONLY: a from-scratch, single-threaded restatement of
- * the reference engine's per-substep algorithms (rlguy/Blender-FLIP-Fluids
- * 1.8.5).  Only tests, the smoke check and the benchmark's CPU-baseline leg
- * load it; the shipped HIP engine never does.
+ * TEST INFRASTRUCTURE ONLY: a from-scratch restatement of the reference
+ * engine's per-substep algorithms (rlguy/Blender-FLIP-Fluids 1.8.5).  Only
+ * tests, the smoke check and the benchmark's CPU-baseline leg load it; the
+ * shipped HIP engine never does.
+ *
+ * Threads (oracle_set_threads, default 1) split only loops whose outputs are
+ * disjoint and whose per-output operation order is the serial one: the
+ * liquid SDF and P2G per 10^3 block (each block's member particles in
+ * ascending index, as the serial loop visits them), SpMV and vector updates
+ * per row, G2P and advection per particle.  The results are therefore the
+ * same bits at any thread count; the MIC(0) factor and sweeps, the dot
+ * products and the removal pass stay serial.
  *
  * Every function names the reference lines it restates.  The restatement
  * keeps the reference's float/double promotion points and operation order
@@ -30,6 +38,8 @@
 #include <limits>
 #include <stdexcept>
 #include <vector>
+
+#include <omp.h>
 
 namespace {
 
@@ -849,6 +859,63 @@ int blockMembers(V3 p, float sr, float blockdx, int bi, int bj, int bk,
     return n;
 }
 
+int g_threads = 1;  // oracle_set_threads
+
+// The member particles of every block, each list in ascending particle
+// index (members(p, out) writes p's member blocks' flat indices and returns
+// their count): threads count their contiguous particle range's memberships per
+// block, the counts are scanned in (block, thread) order and the ranges
+// filled in, so every list holds the serial loop's order.
+struct BlockLists {
+    std::vector<size_t> start;  // nblocks + 1
+    std::vector<int> ids;
+};
+template <class Members>
+BlockLists blockLists(int np, size_t nblocks, Members members) {
+    const int T = g_threads;
+    std::vector<size_t> cnt((size_t)T * nblocks, 0);
+    auto range = [&](int t, int &lo, int &hi) {
+        lo = (int)((long long)np * t / T);
+        hi = (int)((long long)np * (t + 1) / T);
+    };
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; t++) {
+        int lo, hi;
+        range(t, lo, hi);
+        size_t m[27];
+        size_t *c = &cnt[(size_t)t * nblocks];
+        for (int p = lo; p < hi; p++) {
+            int n = members(p, m);
+            for (int q = 0; q < n; q++) c[m[q]]++;
+        }
+    }
+    BlockLists L;
+    L.start.assign(nblocks + 1, 0);
+    std::vector<size_t> at((size_t)T * nblocks);
+    size_t run = 0;
+    for (size_t b = 0; b < nblocks; b++) {
+        L.start[b] = run;
+        for (int t = 0; t < T; t++) {
+            at[(size_t)t * nblocks + b] = run;
+            run += cnt[(size_t)t * nblocks + b];
+        }
+    }
+    L.start[nblocks] = run;
+    L.ids.resize(run);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for (int t = 0; t < T; t++) {
+        int lo, hi;
+        range(t, lo, hi);
+        size_t m[27];
+        size_t *a = &at[(size_t)t * nblocks];
+        for (int p = lo; p < hi; p++) {
+            int n = members(p, m);
+            for (int q = 0; q < n; q++) L.ids[a[m[q]]++] = p;
+        }
+    }
+    return L;
+}
+
 }  // namespace
 
 /* ParticleLevelSet::_computeSignedDistanceFromParticles (particlelevelset.cpp:335-411)
@@ -873,12 +940,21 @@ void oracle_liquid_sdf(int ni, int nj, int nk, double dx, int np, const float *p
     feather26(active, bi, bj, bk);
     float r = radius;
     float sr = 2.0f * r;
-    GI members[27];
-    for (int p = 0; p < np; p++) {
+    const size_t nb = (size_t)bi * bj * bk;
+    BlockLists L = blockLists(np, nb, [&](int p, size_t *out) {
         V3 x{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
-        int nm = blockMembers(x, sr, blockdx, bi, bj, bk, active, members);
-        for (int m = 0; m < nm; m++) {
-            GI b = members[m];
+        GI m[27];
+        int n = blockMembers(x, sr, blockdx, bi, bj, bk, active, m);
+        for (int q = 0; q < n; q++) out[q] = flat(m[q].i, m[q].j, m[q].k, bi, bj);
+        return n;
+    });
+    // each block writes only its own cells, taking its members in ascending index
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 8)
+    for (size_t fb = 0; fb < nb; fb++) {
+        const GI b{(int)(fb % bi), (int)((fb / bi) % bj), (int)(fb / ((size_t)bi * bj))};
+        for (size_t e = L.start[fb]; e < L.start[fb + 1]; e++) {
+            const int p = L.ids[e];
+            V3 x{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
             V3 off = indexToPos(b.i, b.j, b.k, bw * dx);
             V3 pl = x - off;
             GI g0 = posToIndex(V3{pl.x - sr, pl.y - sr, pl.z - sr}, dx);
@@ -964,13 +1040,22 @@ void p2gFlip(int ni, int nj, int nk, double dx, int np, const float *pos, const 
     float coef1 = (4.0f / 9.0f) * (1.0f / (r * r * r * r * r * r));
     float coef2 = (17.0f / 9.0f) * (1.0f / (r * r * r * r));
     float coef3 = (22.0f / 9.0f) * (1.0f / (r * r));
-    GI members[27];
-    for (int p = 0; p < np; p++) {
+    const size_t nb = (size_t)bi * bj * bk;
+    BlockLists L = blockLists(np, nb, [&](int p, size_t *out) {
         V3 x = V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]} - offset;
-        float velocity = vel[3 * p + dir];
-        int nm = blockMembers(x, sr, blockdx, bi, bj, bk, active, members);
-        for (int m = 0; m < nm; m++) {
-            GI b = members[m];
+        GI m[27];
+        int n = blockMembers(x, sr, blockdx, bi, bj, bk, active, m);
+        for (int q = 0; q < n; q++) out[q] = flat(m[q].i, m[q].j, m[q].k, bi, bj);
+        return n;
+    });
+    // each block writes only its own faces, taking its members in ascending index
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 8)
+    for (size_t fb = 0; fb < nb; fb++) {
+        const GI b{(int)(fb % bi), (int)((fb / bi) % bj), (int)(fb / ((size_t)bi * bj))};
+        for (size_t e = L.start[fb]; e < L.start[fb + 1]; e++) {
+            const int p = L.ids[e];
+            V3 x = V3{pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]} - offset;
+            float velocity = vel[3 * p + dir];
             V3 off = indexToPos(b.i, b.j, b.k, bw * dx);
             V3 pl = x - off;
             GI g0 = posToIndex(V3{pl.x - sr, pl.y - sr, pl.z - sr}, dx);
@@ -1473,6 +1558,7 @@ double absMax(const std::vector<double> &x) {
 /* FixedSparseMatrix multiply (sparsematrix.h:174-215): each row sums its
  * entries in ascending column order: -z, -y, -x, diag, +x, +y, +z. */
 void spmv(const PressureSystem &S, const std::vector<double> &x, std::vector<double> &out) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int r = 0; r < S.m; r++) {
         const int *nb = &S.nbr[(size_t)6 * r];
         double s = 0;
@@ -1515,8 +1601,11 @@ bool pcg(PressureSystem &S, double tolFactor, int maxIter, int ncpu, std::vector
         spmv(S, s, z);
         double alpha = rho / blasDot(s, z, ncpu);
         float fa = alpha, fna = -alpha;
-        for (int i = 0; i < m; i++) x[i] += fa * s[i];
-        for (int i = 0; i < m; i++) r[i] += fna * z[i];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int i = 0; i < m; i++) {
+            x[i] += fa * s[i];
+            r[i] += fna * z[i];
+        }
         resid = absMax(r);
         if (resid <= std::min(tol, 1.0)) {
             iters = it + 1;
@@ -1526,6 +1615,7 @@ bool pcg(PressureSystem &S, double tolFactor, int maxIter, int ncpu, std::vector
         double rhoNew = blasDot(z, r, ncpu);
         double beta = rhoNew / rho;
         float fb = beta;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
         for (int i = 0; i < m; i++) z[i] += fb * s[i];
         s.swap(z);
         rho = rhoNew;
@@ -2055,6 +2145,7 @@ struct OracleSim {
             for (size_t p = 0; p < pos.size(); p++) vel[p] = V3{Vv[3 * p], Vv[3 * p + 1], Vv[3 * p + 2]};
         } else {
             float r1 = (float)picflip, r2 = (float)(1 - picflip);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
             for (size_t p = 0; p < pos.size(); p++) {
                 V3 vPIC = sampleMac(cur, pos[p]);
                 V3 vFLIP = vel[p] + vPIC - sampleMac(sav, pos[p]);
@@ -2208,8 +2299,8 @@ struct OracleSim {
         Box boundary = boundaryBox(ni, nj, nk, dx);
         boundary.expand(-0.2f * dx);
         std::vector<V3> out(pos.size());
-        for (size_t p = 0; p < pos.size(); p++) out[p] = rk3(M, pos[p], dt);
-        for (size_t p = 0; p < pos.size(); p++) out[p] = resolveCollision(pos[p], out[p], boundary);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (size_t p = 0; p < pos.size(); p++) out[p] = resolveCollision(pos[p], rk3(M, pos[p], dt), boundary);
         pos = out;
         if (!removal) return;
         // removal uses the frame delta time (7896)
@@ -2217,6 +2308,7 @@ struct OracleSim {
         double maxspeedsq = maxspeed * maxspeed;
         std::vector<int> count(nN(), 0);
         std::vector<char> removed(pos.size(), 0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
         for (size_t p = 0; p < pos.size(); p++) removed[p] = solidAt(pos[p]) < 0.0f;
         for (size_t p = 0; p < pos.size(); p++) {
             if (removed[p]) continue;
@@ -2243,6 +2335,9 @@ struct OracleSim {
         if (apic) aff.resize(9 * q);
     }
 };
+
+void oracle_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int oracle_get_threads(void) { return g_threads; }
 
 OracleSim *oracle_sim_new(int ni, int nj, int nk, double dx) {
     OracleSim *s = new OracleSim();

@@ -132,6 +132,13 @@ void oracle_sample_velocity(int ni, int nj, int nk, double dx, const float *u,
 size_t oracle_load_queue_index(size_t i);
 
 /* ---- whole-engine oracle with the FluidSimulation stepping semantics ---- */
+/* Threads for the loops whose outputs are disjoint and keep the serial
+ * per-output order (liquid SDF and P2G per block, SpMV and vector updates per
+ * row, G2P and advection per particle): the same bits at any count.
+ * Default 1. */
+void oracle_set_threads(int n);
+int oracle_get_threads(void);
+
 typedef struct OracleSim OracleSim;
 OracleSim *oracle_sim_new(int ni, int nj, int nk, double dx);
 void oracle_sim_destroy(OracleSim *s);

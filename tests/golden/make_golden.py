@@ -194,6 +194,47 @@ def make_c1(path, threads, frames=60):
     print("wrote", path)
 
 
+def make_c1_stats(path, threads, frames=60):
+    """Flow observables (scene_util.flow_statistics) of the 64^3 dam break,
+    every frame, from the reference engine (its particle hashes checked
+    against e2e_c1_n64.json on the way) and from the oracle with serial dot
+    products instead of the reference's thread intervals: the second run
+    differs from the first only by the order of the PCG's sums, which is what
+    separates the GPU's tree reductions from the reference too, so its
+    distance from the reference measures how far that alone carries the flow."""
+    from scene_util import flow_statistics
+    n = 64
+    dx = 1.0 / n
+    pos = dam_break(n, seed=0)
+    with open(os.path.join(HERE, "e2e_c1_n64.json")) as fh:
+        hashes = json.load(fh)["frames"]
+    ref = oracle_lib.ReferenceSim((n, n, n), dx, threads=threads)
+    ref.load(pos)
+    ref.initialize()
+    orc = oracle_lib.OracleSim((n, n, n), dx, chunks=1)
+    orc.load(pos)
+    orc.initialize()
+    rec = {"scene": "dam break 64^3 (e2e_c1_n64.json), frames of 1/30 s", "reference_threads": threads,
+           "reference": [], "oracle_serial_dots": []}
+    for f in range(1, frames + 1):
+        t0 = time.time()
+        ref.update(1.0 / 30.0)
+        p, v = ref.particles()
+        h = hashes[f - 1]
+        assert h["frame"] == f and sha(p) == h["positions_sha256"] and sha(v) == h["velocities_sha256"], f
+        rec["reference"].append(flow_statistics(p, v, n))
+        orc.update(1.0 / 30.0)
+        p2, v2 = orc.particles()
+        rec["oracle_serial_dots"].append(flow_statistics(p2, v2, n))
+        a, b = rec["reference"][-1], rec["oracle_serial_dots"][-1]
+        print("frame %d: counts %d/%d centroid diff %.3g cells (%.1fs)" % (
+            f, a["count"], b["count"], max(abs(x - y) for x, y in zip(a["centroid"], b["centroid"])),
+            time.time() - t0), flush=True)
+    with open(path, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    print("wrote", path)
+
+
 # the obstacle scenes of tests/test_gpu_mesh.py: (mesh, inversed, friction) per obstacle
 def obstacle_cases():
     a = scenes.sphere_mesh((0.3, 0.3, 0.5), 0.12, 3)  # overlaps the initial fluid
@@ -322,15 +363,21 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--skip-c1", action="store_true")
+    ap.add_argument("--only-c1-stats", action="store_true",
+                    help="write only e2e_c1_n64_stats.json (the 64^3 flow observables)")
     args = ap.parse_args()
     if not oracle_lib.reference_available():
         sys.exit("the reference build is missing: run `make -C oracle ref` first")
+    if args.only_c1_stats:
+        make_c1_stats(os.path.join(HERE, "e2e_c1_n64_stats.json"), args.threads)
+        return
     make_stages(os.path.join(HERE, "stages_n16.npz"))
     make_e2e16(os.path.join(HERE, "e2e_n16.npz"), args.threads)
     make_obstacles(os.path.join(HERE, "obstacles_n32.json"))
     make_savestate(os.path.join(HERE, "savestate_ref_n16.npz"))
     if not args.skip_c1:
         make_c1(os.path.join(HERE, "e2e_c1_n64.json"), args.threads)
+        make_c1_stats(os.path.join(HERE, "e2e_c1_n64_stats.json"), args.threads)
 
 
 if __name__ == "__main__":

@@ -55,8 +55,15 @@ def oracle():
         L.oracle_sim_get_grids.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.oracle_load_queue_index.restype = C.c_size_t
         L.oracle_load_queue_index.argtypes = [C.c_size_t]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_get_threads.restype = C.c_int
         _oracle = L
     return _oracle
+
+
+def set_threads(n):
+    """Threads of the oracle's disjoint-output loops (same bits at any count)."""
+    oracle().oracle_set_threads(int(n))
 
 
 def reference_available():

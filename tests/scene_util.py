@@ -64,3 +64,19 @@ def differentiation_scene(n, seed=0):
     init = [np.zeros(s, np.float32) for s in shapes]
     target_init = scenes.noise_mac_field(n, seed=seed + 1, amplitude=0.2, correlation=8.0)
     return np.ascontiguousarray(pos[free]), init, target_init
+
+
+def flow_statistics(pos, vel, n, blocks=4):
+    """Per-frame flow observables of a particle state, for comparing runs
+    whose particles diverge individually (reordered sums amplified by the
+    flow) but whose flow must agree: the particle count, the centroid and
+    the mean squared speed (float64 means, in cells and (cells/s)^2), and
+    the fraction of particles in each of blocks^3 equal sub-boxes of the
+    domain (k, j, i order)."""
+    p = np.asarray(pos, np.float64).reshape(-1, 3) * n
+    v = np.asarray(vel, np.float64).reshape(-1, 3) * n
+    b = np.clip(np.floor(p * blocks / n).astype(np.int64), 0, blocks - 1)
+    occ = np.bincount((b[:, 2] * blocks + b[:, 1]) * blocks + b[:, 0], minlength=blocks ** 3)
+    return {"count": int(len(p)), "centroid": [float(c) for c in p.mean(axis=0)],
+            "mean_sq_speed": float((v * v).sum(axis=1).mean()),
+            "occupancy": [float(o) for o in occ / max(len(p), 1)]}

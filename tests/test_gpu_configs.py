@@ -14,8 +14,10 @@ relative residual 1e-8) on the GPU engine.
 - At the configuration's full size, 512^3 (about 359M particles), two
   substeps as size-independent properties: every pressure solve meets
   |r|_inf <= 1e-8 |b|_inf on its FP64 residual; positions and velocities stay
-  finite and inside the domain; the particle count only drops at the first
-  substep (particles seeded inside the walls) and is kept after that.
+  finite and inside the domain; the particle count never grows: the first
+  substep removes the particles seeded inside the walls, and later substeps
+  remove at most one particle in 10^6 (the removal rules: advected into a
+  solid, or above the speed limit).
 """
 import sys
 
@@ -110,4 +112,5 @@ def test_splash_full_size_properties():
     p, v = _particles(sim)
     assert np.isfinite(p).all() and np.isfinite(v).all()
     assert p.min() >= 0.0 and p.max() <= n * dx
-    assert counts[0] <= p0 and counts[1] == counts[0]
+    assert counts[0] <= p0 and counts[1] <= counts[0]
+    assert counts[0] - counts[1] <= 1e-6 * counts[0]
