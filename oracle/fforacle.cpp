@@ -303,6 +303,7 @@ namespace {
 struct TriMesh {
     std::vector<V3> v;
     std::vector<std::array<int, 3>> t;
+    std::vector<V3> vel;  // per-vertex velocities (MeshLevelSet::_vertexVelocities), or empty for zero
 };
 
 // AABB(std::vector<vec3>) and AABB(Triangle, vertices): min/max in double,
@@ -478,6 +479,8 @@ std::vector<TriMesh> splitIslands(const TriMesh &m) {
             if (tr[t[c]] == -1) {
                 tr[t[c]] = (int)isl[g].v.size();
                 isl[g].v.push_back(m.v[t[c]]);
+                // MeshUtils::splitIntoMeshIslands with velocities (meshutils.cpp:574-593)
+                if (!m.vel.empty()) isl[g].vel.push_back(m.vel[t[c]]);
             }
         isl[g].t.push_back(t);
     }
@@ -549,11 +552,16 @@ void nodesInsideGrid(const TriMesh &m, int pi, int pj, int pk, double dx, RandSo
 // computes it (1165-1234), the form the fracture path's per-island level
 // sets take: node range from int() of the scaled vertex coordinates,
 // clamped to the node grid, distances in the level set's own coordinates.
+//
+// closest (optional): each node's closest triangle (-1 outside the band): the
+// first triangle, in index order, at the node's smallest distance.
 void meshLevelSetFast(const TriMesh &m, int isize, int jsize, int ksize, double dx, GI o, int band,
-                      RandSource &rs, std::vector<float> &phi, bool singleThreaded = false) {
+                      RandSource &rs, std::vector<float> &phi, bool singleThreaded = false,
+                      std::vector<int> *closest = nullptr) {
     const int pi = isize + 1, pj = jsize + 1, pk = ksize + 1, bw = 10;
     float ub = (float)((pi + pj + pk) * dx);  // getDistanceUpperBound (meshlevelset.cpp:769-771)
     phi.assign((size_t)pi * pj * pk, ub);
+    if (closest) closest->assign(phi.size(), -1);
     V3 po = indexToPos(o.i, o.j, o.k, dx);
     auto clampi = [](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
     for (int t = 0; singleThreaded && t < (int)m.t.size(); t++) {
@@ -574,7 +582,10 @@ void meshLevelSetFast(const TriMesh &m, int isize, int jsize, int ksize, double 
                     V3 cp = closestPointOnTriangle(gpos, a, bb, c);
                     float dist = length(cp - gpos);
                     size_t n = flat(i, j, k, pi, pj);
-                    if (dist < phi[n]) phi[n] = dist;
+                    if (dist < phi[n]) {
+                        phi[n] = dist;
+                        if (closest) (*closest)[n] = t;
+                    }
                 }
     }
     for (int t = 0; !singleThreaded && t < (int)m.t.size(); t++) {
@@ -596,7 +607,10 @@ void meshLevelSetFast(const TriMesh &m, int isize, int jsize, int ksize, double 
                     V3 cp = closestPointOnTriangle(gpos, a - off, bb - off, c - off);
                     float dist = length(cp - gpos);
                     size_t n = flat(i, j, k, pi, pj);
-                    if (dist < phi[n]) phi[n] = dist;
+                    if (dist < phi[n]) {
+                        phi[n] = dist;
+                        if (closest) (*closest)[n] = t;
+                    }
                 }
     }
     TriMesh local = m;
@@ -612,13 +626,16 @@ void removeUnused(TriMesh &m) {
     std::vector<int> tr(m.v.size(), -1);
     std::vector<char> used(m.v.size(), 0);
     for (auto &t : m.t) used[t[0]] = used[t[1]] = used[t[2]] = 1;
-    std::vector<V3> nv;
+    std::vector<V3> nv, nvel;
     for (size_t q = 0; q < m.v.size(); q++)
         if (used[q]) {
             tr[q] = (int)nv.size();
             nv.push_back(m.v[q]);
+            // MeshObject::getMeshLevelSet erases the removed vertices' velocities (meshobject.cpp:244-248)
+            if (!m.vel.empty()) nvel.push_back(m.vel[q]);
         }
     m.v = nv;
+    m.vel = nvel;
     for (auto &t : m.t)
         for (int c = 0; c < 3; c++) t[c] = tr[t[c]];
 }
@@ -699,6 +716,299 @@ void obstacleLevelSet(TriMesh m, int ni, int nj, int nk, double dx, int band, Ra
     }
 }
 
+float fractionInside4(float bl, float br, float tl, float tr);  // LevelsetUtils::fractionInside, below
+
+// ---- the solid's velocity grids (moving obstacles) ----
+//
+// MeshLevelSet with its velocity data (meshlevelset.h:68-76): phi and the
+// closest triangle per node, the merged mesh with its vertex velocities, and
+// per face the weighted velocity and the weight.  The domain grid has offset
+// (0, 0, 0); an island's level set sits at its grid offset.
+struct SolidLS {
+    int ni = 0, nj = 0, nk = 0;
+    GI off{0, 0, 0};
+    double dx = 0.0;
+    bool vel = true;  // isVelocityDataEnabled
+    std::vector<float> phi;
+    std::vector<int> tri;
+    TriMesh mesh;
+    std::vector<float> fu, fv, fw, wu, wv, ww;
+    // the constructor and reset() (meshlevelset.cpp:710-718)
+    void init(int i, int j, int k, double d, GI o = GI{0, 0, 0}) {
+        ni = i; nj = j; nk = k; dx = d; off = o;
+        reset();
+    }
+    void reset() {
+        mesh = TriMesh();
+        phi.assign((size_t)(ni + 1) * (nj + 1) * (nk + 1), ub());
+        tri.assign(phi.size(), -1);
+        clearVelocity();
+    }
+    void clearVelocity() {  // VelocityDataGrid::reset
+        fu.assign((size_t)(ni + 1) * nj * nk, 0.0f);
+        wu.assign(fu.size(), 0.0f);
+        fv.assign((size_t)ni * (nj + 1) * nk, 0.0f);
+        wv.assign(fv.size(), 0.0f);
+        fw.assign((size_t)ni * nj * (nk + 1), 0.0f);
+        ww.assign(fw.size(), 0.0f);
+    }
+    float ub() const { return (float)((ni + nj + nk + 3) * dx); }  // getDistanceUpperBound
+    float P(int i, int j, int k) const { return phi[flat(i, j, k, ni + 1, nj + 1)]; }
+    V3 posOff() const { return indexToPos(off.i, off.j, off.k, dx); }
+};
+
+// MeshLevelSet::_pointToSegmentVelocity (meshlevelset.cpp:1697-1717)
+V3 segmentVelocity(V3 x0, V3 x1, V3 x2, V3 v1, V3 v2, float *distance) {
+    V3 d = x2 - x1;
+    double m2 = lengthsq(d);
+    float s12 = (float)(dot(x2 - x0, d) / m2);
+    if (s12 < 0)
+        s12 = 0;
+    else if (s12 > 1)
+        s12 = 1;
+    *distance = length(x0 - (s12 * x1 + (1 - s12) * x2));
+    return s12 * v1 + (1 - s12) * v2;
+}
+
+// MeshLevelSet::_pointToTriangleVelocity (meshlevelset.cpp:1568-1639)
+V3 triangleVelocity(const SolidLS &L, V3 x0, int t) {
+    const std::array<int, 3> &T = L.mesh.t[t];
+    V3 v1 = L.mesh.vel[T[0]], v2 = L.mesh.vel[T[1]], v3 = L.mesh.vel[T[2]];
+    float eps = 1e-6f;
+    if (std::fabs(v1.x) < eps && std::fabs(v1.y) < eps && std::fabs(v1.z) < eps && std::fabs(v2.x) < eps &&
+        std::fabs(v2.y) < eps && std::fabs(v2.z) < eps && std::fabs(v3.x) < eps && std::fabs(v3.y) < eps &&
+        std::fabs(v3.z) < eps)
+        return V3{0.0f, 0.0f, 0.0f};
+    V3 po = L.posOff();
+    V3 x1 = L.mesh.v[T[0]] - po, x2 = L.mesh.v[T[1]] - po, x3 = L.mesh.v[T[2]] - po;
+    V3 x13 = x1 - x3, x23 = x2 - x3, x03 = x0 - x3;
+    float m13 = lengthsq(x13), m23 = lengthsq(x23), d = dot(x13, x23);
+    float invdet = 1.0f / std::fmax(m13 * m23 - d * d, 1e-30f);
+    float a = dot(x13, x03), b = dot(x23, x03);
+    float w23 = invdet * (m23 * a - d * b);
+    float w31 = invdet * (m13 * b - d * a);
+    float w12 = 1 - w23 - w31;
+    if (w23 >= 0 && w31 >= 0 && w12 >= 0) return w23 * v1 + w31 * v2 + w12 * v3;
+    float d1, d2;
+    V3 e1, e2;
+    if (w23 > 0) {  // rules out edge 2-3
+        e1 = segmentVelocity(x0, x1, x2, v1, v2, &d1);
+        e2 = segmentVelocity(x0, x1, x3, v1, v3, &d2);
+    } else if (w31 > 0) {  // rules out edge 1-3
+        e1 = segmentVelocity(x0, x1, x2, v1, v2, &d1);
+        e2 = segmentVelocity(x0, x2, x3, v2, v3, &d2);
+    } else {  // rules out edge 1-2
+        e1 = segmentVelocity(x0, x1, x3, v1, v3, &d1);
+        e2 = segmentVelocity(x0, x2, x3, v2, v3, &d2);
+    }
+    return d1 < d2 ? e1 : e2;
+}
+
+// MeshLevelSet::getNearestVelocity (meshlevelset.cpp:168-205): among the
+// closest triangles of the point's cell's 8 nodes, the nearest one's velocity
+V3 nearestVelocity(const SolidLS &L, V3 p) {
+    V3 po = L.posOff();
+    p = p - po;
+    GI g = posToIndex(p, L.dx);
+    const int o8[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+    int nearest = -1;
+    float nearestDist = L.ub();
+    for (int q = 0; q < 8; q++) {
+        int i = g.i + o8[q][0], j = g.j + o8[q][1], k = g.k + o8[q][2];
+        if (!inRange(i, j, k, L.ni + 1, L.nj + 1, L.nk + 1)) continue;
+        int t = L.tri[flat(i, j, k, L.ni + 1, L.nj + 1)];
+        if (t == -1) continue;
+        const std::array<int, 3> &T = L.mesh.t[t];
+        V3 cp = closestPointOnTriangle(p, L.mesh.v[T[0]] - po, L.mesh.v[T[1]] - po, L.mesh.v[T[2]] - po);
+        float d = length(cp - p);
+        if (d < nearestDist) {
+            nearestDist = d;
+            nearest = t;
+        }
+    }
+    if (nearest == -1) return V3{0.0f, 0.0f, 0.0f};
+    return triangleVelocity(L, p, nearest);
+}
+
+// MeshLevelSet::_computeVelocityGrids (meshlevelset.cpp:1320-1425): every face
+// with a positive fraction inside the level set gets weight x the nearest
+// velocity (0 for a mesh whose vertices all stand still) and the weight
+void velocityGrids(SolidLS &L) {
+    L.clearVelocity();
+    bool isStatic = true;
+    for (V3 v : L.mesh.vel)
+        if (std::fabs(v.x) > 0.0f || std::fabs(v.y) > 0.0f || std::fabs(v.z) > 0.0f) {
+            isStatic = false;
+            break;
+        }
+    V3 po = L.posOff();
+    const double dx = L.dx;
+    for (int k = 0; k < L.nk; k++)
+        for (int j = 0; j < L.nj; j++)
+            for (int i = 0; i < L.ni + 1; i++) {
+                float w = fractionInside4(L.P(i, j, k), L.P(i, j + 1, k), L.P(i, j, k + 1), L.P(i, j + 1, k + 1));
+                if (!(w > 0.0f)) continue;
+                V3 v{0.0f, 0.0f, 0.0f};
+                if (!isStatic)
+                    v = nearestVelocity(L, mk((float)i * dx, ((float)j + 0.5) * dx, ((float)k + 0.5) * dx) + po);
+                size_t f = flat(i, j, k, L.ni + 1, L.nj);
+                L.fu[f] = w * v.x;
+                L.wu[f] = w;
+            }
+    for (int k = 0; k < L.nk; k++)
+        for (int j = 0; j < L.nj + 1; j++)
+            for (int i = 0; i < L.ni; i++) {
+                float w = fractionInside4(L.P(i, j, k), L.P(i, j, k + 1), L.P(i + 1, j, k), L.P(i + 1, j, k + 1));
+                if (!(w > 0.0f)) continue;
+                V3 v{0.0f, 0.0f, 0.0f};
+                if (!isStatic)
+                    v = nearestVelocity(L, mk(((float)i + 0.5) * dx, (float)j * dx, ((float)k + 0.5) * dx) + po);
+                size_t f = flat(i, j, k, L.ni, L.nj + 1);
+                L.fv[f] = w * v.y;
+                L.wv[f] = w;
+            }
+    for (int k = 0; k < L.nk + 1; k++)
+        for (int j = 0; j < L.nj; j++)
+            for (int i = 0; i < L.ni; i++) {
+                float w = fractionInside4(L.P(i, j, k), L.P(i, j + 1, k), L.P(i + 1, j, k), L.P(i + 1, j + 1, k));
+                if (!(w > 0.0f)) continue;
+                V3 v{0.0f, 0.0f, 0.0f};
+                if (!isStatic)
+                    v = nearestVelocity(L, mk(((float)i + 0.5) * dx, ((float)j + 0.5) * dx, (float)k * dx) + po);
+                size_t f = flat(i, j, k, L.ni, L.nj);
+                L.fw[f] = w * v.z;
+                L.ww[f] = w;
+            }
+}
+
+// MeshLevelSet::fastCalculateSignedDistanceField with vertex velocities
+// (meshlevelset.cpp:577-601)
+void fastSDF(SolidLS &L, const TriMesh &m, int band, RandSource &rs, bool singleThreaded) {
+    L.mesh = m;
+    if (L.mesh.vel.size() != L.mesh.v.size()) L.mesh.vel.assign(L.mesh.v.size(), V3{0.0f, 0.0f, 0.0f});
+    meshLevelSetFast(m, L.ni, L.nj, L.nk, L.dx, L.off, band, rs, L.phi, singleThreaded, &L.tri);
+    if (L.vel) velocityGrids(L);
+}
+
+// MeshLevelSet::negate (meshlevelset.cpp:702-708)
+void negateLS(SolidLS &L) {
+    for (float &p : L.phi) p = -p;
+    if (L.vel) velocityGrids(L);
+}
+
+// MeshLevelSet::calculateUnion (meshlevelset.cpp:603-638, 1759-1845): the
+// mesh, vertex velocities, phi and closest triangles merged; the velocity data
+// added, on the other grid's border only for faces inside both grids
+void unionLS(SolidLS &L, const SolidLS &O) {
+    const int triOffset = (int)L.mesh.t.size(), vOffset = (int)L.mesh.v.size();
+    L.mesh.v.insert(L.mesh.v.end(), O.mesh.v.begin(), O.mesh.v.end());
+    for (auto t : O.mesh.t) L.mesh.t.push_back({t[0] + vOffset, t[1] + vOffset, t[2] + vOffset});
+    L.mesh.vel.insert(L.mesh.vel.end(), O.mesh.vel.begin(), O.mesh.vel.end());
+    auto onBorder = [](int i, int j, int k, int a, int b, int c) {
+        return i == 0 || j == 0 || k == 0 || i == a - 1 || j == b - 1 || k == c - 1;
+    };
+    for (int k = 0; k < O.nk + 1; k++)
+        for (int j = 0; j < O.nj + 1; j++)
+            for (int i = 0; i < O.ni + 1; i++) {
+                int a = i + O.off.i - L.off.i, b = j + O.off.j - L.off.j, c = k + O.off.k - L.off.k;
+                if (!inRange(a, b, c, L.ni + 1, L.nj + 1, L.nk + 1)) continue;
+                size_t n = flat(a, b, c, L.ni + 1, L.nj + 1);
+                float po = O.P(i, j, k);
+                if (po < L.phi[n]) {
+                    if (std::fabs(po) < std::fabs(L.phi[n])) {
+                        int t = O.tri[flat(i, j, k, O.ni + 1, O.nj + 1)];
+                        if (t != -1) L.tri[n] = t + triOffset;
+                    }
+                    L.phi[n] = po;
+                }
+                if (!L.vel) continue;
+                bool border = onBorder(a, b, c, L.ni + 1, L.nj + 1, L.nk + 1) ||
+                              onBorder(i, j, k, O.ni + 1, O.nj + 1, O.nk + 1);
+                bool uIn = inRange(a, b, c, L.ni + 1, L.nj, L.nk) && inRange(i, j, k, O.ni + 1, O.nj, O.nk);
+                bool vIn = inRange(a, b, c, L.ni, L.nj + 1, L.nk) && inRange(i, j, k, O.ni, O.nj + 1, O.nk);
+                bool wIn = inRange(a, b, c, L.ni, L.nj, L.nk + 1) && inRange(i, j, k, O.ni, O.nj, O.nk + 1);
+                if (!border || uIn) {
+                    size_t f = flat(a, b, c, L.ni + 1, L.nj), g = flat(i, j, k, O.ni + 1, O.nj);
+                    L.fu[f] += O.fu[g];
+                    L.wu[f] += O.wu[g];
+                }
+                if (!border || vIn) {
+                    size_t f = flat(a, b, c, L.ni, L.nj + 1), g = flat(i, j, k, O.ni, O.nj + 1);
+                    L.fv[f] += O.fv[g];
+                    L.wv[f] += O.wv[g];
+                }
+                if (!border || wIn) {
+                    size_t f = flat(a, b, c, L.ni, L.nj), g = flat(i, j, k, O.ni, O.nj);
+                    L.fw[f] += O.fw[g];
+                    L.ww[f] += O.ww[g];
+                }
+            }
+}
+
+// MeshLevelSet::normalizeVelocityGrid (meshlevelset.cpp:640-700, 1739-1757):
+// velocity = field / weight where the weight exceeds 1e-6 (valid), else 0,
+// then 5 extrapolation layers from the valid faces
+void normalizeLS(SolidLS &L, std::vector<float> &u, std::vector<float> &v, std::vector<float> &w) {
+    auto one = [](const std::vector<float> &f, const std::vector<float> &wt, std::vector<float> &out,
+                  std::vector<unsigned char> &valid) {
+        out.assign(f.size(), 0.0f);
+        valid.assign(f.size(), 0);
+        for (size_t q = 0; q < f.size(); q++)
+            if (wt[q] > 1e-6f) {
+                out[q] = f[q] / wt[q];
+                valid[q] = 1;
+            }
+    };
+    std::vector<unsigned char> vu, vv, vw;
+    one(L.fu, L.wu, u, vu);
+    one(L.fv, L.wv, v, vv);
+    one(L.fw, L.ww, w, vw);
+    const int layers = 5;  // _numVelocityExtrapolationLayers (meshlevelset.h:363)
+    oracle_extrapolate(L.ni + 1, L.nj, L.nk, u.data(), vu.data(), layers);
+    oracle_extrapolate(L.ni, L.nj + 1, L.nk, v.data(), vv.data(), layers);
+    oracle_extrapolate(L.ni, L.nj, L.nk + 1, w.data(), vw.data(), layers);
+}
+
+// One obstacle's level set with velocities on the domain grid L (MeshObject::
+// getMeshLevelSet, meshobject.cpp:239-264, with _addMeshIslandsToLevelSet
+// 738-754 or the 25-or-more-island path 756-839): m holds the mesh at the
+// substep's frame progress and its vertex velocities.
+void obstacleLevelSetVel(TriMesh m, int band, RandSource &rs, SolidLS &L, float expansion) {
+    for (V3 &p : m.v) p = p + V3{0.0f, 0.0f, 0.0f};
+    removeUnused(m);
+    if (m.t.empty()) return;
+    const int ni = L.ni, nj = L.nj, nk = L.nk;
+    const double dx = L.dx;
+    Box grid{V3{0.0f, 0.0f, 0.0f}, ni * dx, nj * dx, nk * dx};
+    std::vector<TriMesh> islands;
+    for (TriMesh &I : splitIslands(m)) {
+        Box b = aabbOf(I.v.data(), (int)I.v.size());
+        if (aabbContains(grid, b) || aabbIntersects(grid, b)) islands.push_back(I);
+    }
+    std::vector<TriMesh> parts;
+    const bool fracture = (int)islands.size() >= 25;
+    if (fracture) {
+        if (std::fabs(expansion) >= 1e-9f)
+            for (TriMesh &I : islands) expandIsland(I, expansion);
+        parts.assign(islands.rbegin(), islands.rend());
+    } else {
+        parts.push_back(m);
+    }
+    for (TriMesh &part : parts) {
+        Box b = aabbOf(part.v.data(), (int)part.v.size());
+        GI g0 = posToIndex(b.pos, dx), g1 = posToIndex(b.maxPoint(), dx);
+        g0.i = (int)std::fmax(g0.i - band, 0); g0.j = (int)std::fmax(g0.j - band, 0); g0.k = (int)std::fmax(g0.k - band, 0);
+        g1.i = (int)std::fmin(g1.i + band + 1, ni - 1); g1.j = (int)std::fmin(g1.j + band + 1, nj - 1);
+        g1.k = (int)std::fmin(g1.k + band + 1, nk - 1);
+        int w = g1.i - g0.i, h = g1.j - g0.j, d = g1.k - g0.k;
+        if (w <= 0 || h <= 0 || d <= 0) continue;
+        SolidLS I;
+        I.init(w, h, d, dx, g0);
+        fastSDF(I, part, band, rs, fracture);
+        unionLS(L, I);
+    }
+}
+
 TriMesh meshFrom(int nv, const float *verts, int nt, const int *tris) {
     TriMesh m;
     m.v.resize(nv);
@@ -725,6 +1035,134 @@ int oracle_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi
         std::vector<float> phi;
         meshLevelSetFast(m, isize, jsize, ksize, dx, GI{oi, oj, ok}, band, rs, phi);
         std::memcpy(phi_nodes, phi.data(), sizeof(float) * phi.size());
+        return rs.used;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+
+namespace {
+// One obstacle of the solid update: its meshes (one triangle list) at the
+// previous, current and next frame, and its flags.
+struct MovingObstacle {
+    TriMesh prev, cur, next;
+    bool animated, inversed;
+    float expansion, velocityScale;
+};
+
+// MeshObject::getMesh(frameInterpolation) and getVertexVelocities(dt,
+// frameInterpolation) (meshobject.cpp:61-95, 158-220): the mesh moved from the
+// current toward the next frame's, and each vertex's translation (current
+// minus previous, next minus current) interpolated the same way, times
+// (float)(1 / frame dt) and the velocity scale.  A static object's
+// translations are zero.  One triangle list: the topology never changes.
+TriMesh meshAtProgress(const MovingObstacle &o, float f, double frameDt) {
+    f = std::fmax(0.0f, f);
+    f = std::fmin(1.0f, f);
+    TriMesh m = o.cur;
+    m.vel.assign(m.v.size(), V3{0.0f, 0.0f, 0.0f});
+    for (size_t q = 0; q < m.v.size(); q++) {
+        V3 v1 = o.cur.v[q], v2 = o.next.v[q];
+        m.v[q] = v1 + f * (v2 - v1);
+    }
+    if (!o.animated || frameDt < 1e-10) return m;
+    const float invdt = (float)(1.0 / frameDt);
+    for (size_t q = 0; q < m.v.size(); q++) {
+        V3 p1 = o.cur.v[q] - o.prev.v[q], p2 = o.next.v[q] - o.cur.v[q];
+        V3 t = p1 + f * (p2 - p1);
+        m.vel[q] = (invdt * t);
+        m.vel[q] = o.velocityScale * m.vel[q];
+    }
+    return m;
+}
+
+// FluidSimulation::_updateSolidLevelSet (fluidsimulation.cpp:5387-5426) with
+// the default options (no precomputed static level set, no fracture
+// optimization): the domain's negated box level set with its velocity
+// weights, the static objects (normal, then the negated union of the
+// inversed), then the animated ones (5219-5275), then normalizeVelocityGrid.
+// Returns the solid phi and the normalized solid face velocities.
+void solidUpdate(int ni, int nj, int nk, double dx, const std::vector<MovingObstacle> &obs, double frameDt,
+                 float progress, RandSource &rs, std::vector<float> &phi, std::vector<float> &u,
+                 std::vector<float> &v, std::vector<float> &w) {
+    const int band = 3;
+    SolidLS solid;
+    solid.init(ni, nj, nk, dx);
+    // the domain: fastCalculateSignedDistanceField without velocity data, then
+    // negate, which computes the velocity grids of the negated level set
+    oracle_solid_box_phi(ni, nj, nk, dx, solid.phi.data());
+    rs.jitter(dx);  // the box's sign test
+    solid.mesh.v.assign(8, V3{0.0f, 0.0f, 0.0f});
+    solid.mesh.vel.assign(8, V3{0.0f, 0.0f, 0.0f});
+    velocityGrids(solid);
+    solid.mesh = TriMesh();  // the domain's mesh plays no further part
+    SolidLS temp;
+    temp.init(ni, nj, nk, dx);
+    for (int pass = 0; pass < 2; pass++) {  // static objects, then animated ones
+        const bool animated = pass == 1;
+        for (const MovingObstacle &o : obs) {
+            if (o.animated != animated || o.inversed) continue;
+            temp.reset();
+            obstacleLevelSetVel(meshAtProgress(o, progress, frameDt), band, rs, temp, o.expansion);
+            unionLS(solid, temp);
+        }
+        bool anyInv = false;
+        for (const MovingObstacle &o : obs) anyInv = anyInv || (o.animated == animated && o.inversed);
+        if (!anyInv) continue;
+        SolidLS inv;
+        inv.init(ni, nj, nk, dx);
+        inv.vel = false;
+        for (const MovingObstacle &o : obs) {
+            if (o.animated != animated || !o.inversed) continue;
+            temp.reset();
+            temp.vel = false;
+            obstacleLevelSetVel(meshAtProgress(o, progress, frameDt), band, rs, temp, o.expansion);
+            unionLS(inv, temp);
+        }
+        inv.vel = true;
+        negateLS(inv);
+        unionLS(solid, inv);
+        temp.vel = true;
+    }
+    phi = solid.phi;
+    normalizeLS(solid, u, v, w);
+}
+}  // namespace
+
+/* The solid update with moving obstacles (fluidsimulation.cpp:5219-5426,
+ * meshlevelset.cpp velocity data): objects as flat arrays, one triangle list
+ * per object and three vertex sets (previous, current, next frame); flags bit
+ * 0 animated, bit 1 inversed.  frame_progress is the substep's fraction of the
+ * frame (1 - (remaining + step) / frame dt).  Writes the nodal solid phi and
+ * the normalized, extrapolated solid face velocities.  Returns the rand()
+ * values used, or -1. */
+int oracle_solid_update(int ni, int nj, int nk, double dx, int nobj, const int *nv, const int *nt,
+                        const float *prev, const float *cur, const float *next, const int *tris, const int *flags,
+                        const float *expansion, const float *vscale, double frame_dt, float frame_progress,
+                        const int *rnd, int nrnd, int rand_max, float *phi_nodes, float *us, float *vs, float *ws) {
+    try {
+        std::vector<MovingObstacle> obs;
+        size_t vo = 0, to = 0;
+        for (int q = 0; q < nobj; q++) {
+            MovingObstacle o;
+            o.prev = meshFrom(nv[q], prev + 3 * vo, nt[q], tris + 3 * to);
+            o.cur = meshFrom(nv[q], cur + 3 * vo, nt[q], tris + 3 * to);
+            o.next = meshFrom(nv[q], next + 3 * vo, nt[q], tris + 3 * to);
+            o.animated = (flags[q] & 1) != 0;
+            o.inversed = (flags[q] & 2) != 0;
+            o.expansion = expansion ? expansion[q] : 0.0f;
+            o.velocityScale = vscale ? vscale[q] : 1.0f;
+            obs.push_back(o);
+            vo += nv[q];
+            to += nt[q];
+        }
+        RandSource rs{rnd, nrnd, 0, (double)rand_max};
+        std::vector<float> phi, u, v, w;
+        solidUpdate(ni, nj, nk, dx, obs, frame_dt, frame_progress, rs, phi, u, v, w);
+        std::memcpy(phi_nodes, phi.data(), sizeof(float) * phi.size());
+        std::memcpy(us, u.data(), sizeof(float) * u.size());
+        std::memcpy(vs, v.data(), sizeof(float) * v.size());
+        std::memcpy(ws, w.data(), sizeof(float) * w.size());
         return rs.used;
     } catch (const std::exception &) {
         return -1;

@@ -139,6 +139,16 @@ size_t oracle_load_queue_index(size_t i);
 void oracle_set_threads(int n);
 int oracle_get_threads(void);
 
+/* The solid update with moving mesh obstacles: the nodal solid phi and the
+ * normalized solid face velocities (u: (ni+1) nj nk, v: ni (nj+1) nk, w:
+ * ni nj (nk+1)).  Per object: nv vertices in each of prev/cur/next (3 floats
+ * each), nt triangles (one list), flags bit 0 animated, bit 1 inversed.
+ * Returns the rand() values used, or -1. */
+int oracle_solid_update(int ni, int nj, int nk, double dx, int nobj, const int *nv, const int *nt,
+                        const float *prev, const float *cur, const float *next, const int *tris, const int *flags,
+                        const float *expansion, const float *vscale, double frame_dt, float frame_progress,
+                        const int *rnd, int nrnd, int rand_max, float *phi_nodes, float *us, float *vs, float *ws);
+
 typedef struct OracleSim OracleSim;
 OracleSim *oracle_sim_new(int ni, int nj, int nk, double dx);
 void oracle_sim_destroy(OracleSim *s);

@@ -383,6 +383,23 @@ void refprobe_sim_get_mac(void *s, float *u, float *v, float *w) {
     copyOut(*f->_MACVelocity.getArray3dV(), v);
     copyOut(*f->_MACVelocity.getArray3dW(), w);
 }
+/* The solid update of one substep (_updateSolidLevelSet, fluidsimulation.cpp:
+ * 5387-5426) forced with the given frame timing (frame progress = 1 -
+ * (remaining + step) / frame_dt), then its phi and normalized velocity grids. */
+void refprobe_sim_update_solid(void *s, double frame_dt, double remaining, double step) {
+    FluidSimulation *f = (FluidSimulation *)s;
+    f->_currentFrameDeltaTime = frame_dt;
+    f->_currentFrameDeltaTimeRemaining = remaining;
+    f->_currentFrameTimeStep = step;
+    f->_isSolidLevelSetUpToDate = false;
+    f->_updateSolidLevelSet(frame_dt);
+}
+void refprobe_sim_get_solid_velocity(void *s, float *u, float *v, float *w) {
+    MACVelocityField &m = ((FluidSimulation *)s)->_solidSDF.getVelocityDataGrid()->field;
+    copyOut(*m.getArray3dU(), u);
+    copyOut(*m.getArray3dV(), v);
+    copyOut(*m.getArray3dW(), w);
+}
 int refprobe_sim_last_substeps(void *s) { return ((FluidSimulation *)s)->_outputData.frameData.substeps; }
 int refprobe_sim_pressure_iterations(void *s) { return ((FluidSimulation *)s)->_pressureSolverIterations; }
 

@@ -197,11 +197,11 @@ def make_c1(path, threads, frames=60):
 def make_c1_stats(path, threads, frames=60):
     """Flow observables (scene_util.flow_statistics) of the 64^3 dam break,
     every frame, from the reference engine (its particle hashes checked
-    against e2e_c1_n64.json on the way) and from the oracle with serial dot
-    products instead of the reference's thread intervals: the second run
-    differs from the first only by the order of the PCG's sums, which is what
-    separates the GPU's tree reductions from the reference too, so its
-    distance from the reference measures how far that alone carries the flow."""
+    against e2e_c1_n64.json on the way) and from the oracle started from the
+    same particles with every x coordinate moved by one float ulp: a
+    rounding-level difference, the size of what separates the GPU's tree
+    reductions from the reference's serial sums, so the second run's distance
+    from the first measures how far the flow carries such a difference."""
     from scene_util import flow_statistics
     n = 64
     dx = 1.0 / n
@@ -211,11 +211,13 @@ def make_c1_stats(path, threads, frames=60):
     ref = oracle_lib.ReferenceSim((n, n, n), dx, threads=threads)
     ref.load(pos)
     ref.initialize()
-    orc = oracle_lib.OracleSim((n, n, n), dx, chunks=1)
-    orc.load(pos)
+    nudged = pos.copy()
+    nudged[:, 0] = np.nextafter(nudged[:, 0], np.float32(1.0))
+    orc = oracle_lib.OracleSim((n, n, n), dx)
+    orc.load(nudged)
     orc.initialize()
     rec = {"scene": "dam break 64^3 (e2e_c1_n64.json), frames of 1/30 s", "reference_threads": threads,
-           "reference": [], "oracle_serial_dots": []}
+           "reference": [], "oracle_one_ulp": []}
     for f in range(1, frames + 1):
         t0 = time.time()
         ref.update(1.0 / 30.0)
@@ -225,8 +227,8 @@ def make_c1_stats(path, threads, frames=60):
         rec["reference"].append(flow_statistics(p, v, n))
         orc.update(1.0 / 30.0)
         p2, v2 = orc.particles()
-        rec["oracle_serial_dots"].append(flow_statistics(p2, v2, n))
-        a, b = rec["reference"][-1], rec["oracle_serial_dots"][-1]
+        rec["oracle_one_ulp"].append(flow_statistics(p2, v2, n))
+        a, b = rec["reference"][-1], rec["oracle_one_ulp"][-1]
         print("frame %d: counts %d/%d centroid diff %.3g cells (%.1fs)" % (
             f, a["count"], b["count"], max(abs(x - y) for x, y in zip(a["centroid"], b["centroid"])),
             time.time() - t0), flush=True)

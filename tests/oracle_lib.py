@@ -159,6 +159,39 @@ def solid_phi_meshes(n3, dx, meshes, seed):
     return phi, used
 
 
+def solid_update(n3, dx, objects, frame_dt, progress, seed, nrand=400):
+    """Oracle solid update with moving obstacles.  objects: dicts with prev,
+    cur, next (vertices), tris, and optional animated, inversed, expansion,
+    velocity_scale.  Returns (phi (k, j, i) nodal, (u, v, w) solid face
+    velocities, rand() draws)."""
+    ni, nj, nk = n3
+    prev = np.ascontiguousarray(np.concatenate([_f32(o["prev"]).reshape(-1, 3) for o in objects]), np.float32)
+    cur = np.ascontiguousarray(np.concatenate([_f32(o["cur"]).reshape(-1, 3) for o in objects]), np.float32)
+    nxt = np.ascontiguousarray(np.concatenate([_f32(o["next"]).reshape(-1, 3) for o in objects]), np.float32)
+    tris = np.ascontiguousarray(np.concatenate([np.asarray(o["tris"], np.int32).reshape(-1, 3) for o in objects]),
+                                np.int32)
+    nv = np.array([len(_f32(o["cur"]).reshape(-1, 3)) for o in objects], np.int32)
+    nt = np.array([len(np.asarray(o["tris"]).reshape(-1, 3)) for o in objects], np.int32)
+    flags = np.array([(1 if o.get("animated", True) else 0) | (2 if o.get("inversed") else 0) for o in objects],
+                     np.int32)
+    exp = np.array([o.get("expansion", 0.0) for o in objects], np.float32)
+    vsc = np.array([o.get("velocity_scale", 1.0) for o in objects], np.float32)
+    r = libc_rand_values(seed, nrand)
+    phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
+    u = np.zeros((nk, nj, ni + 1), np.float32)
+    v = np.zeros((nk, nj + 1, ni), np.float32)
+    w = np.zeros((nk + 1, nj, ni), np.float32)
+    L = oracle()
+    L.oracle_solid_update.argtypes = [C.c_int] * 3 + [C.c_double, C.c_int] + [C.c_void_p] * 9 + \
+        [C.c_double, C.c_float, C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 4
+    used = L.oracle_solid_update(ni, nj, nk, dx, len(objects), fp(nv), fp(nt), fp(prev), fp(cur), fp(nxt), fp(tris),
+                                 fp(flags), fp(exp), fp(vsc), frame_dt, progress, fp(r), len(r), RAND_MAX, fp(phi),
+                                 fp(u), fp(v), fp(w))
+    if used < 0:
+        raise RuntimeError("oracle_solid_update failed")
+    return phi, (u, v, w), used
+
+
 def ref_mesh_levelset_fast(cells3, dx, offset, verts, tris, band, seed):
     isz, jsz, ksz = cells3
     v = _f32(verts).reshape(-1, 3)
@@ -434,6 +467,49 @@ class ReferenceSim:
         err = C.c_int()
         self.L.MeshObject_set_friction.argtypes = [C.c_void_p, C.c_float, C.c_void_p]
         self.L.MeshObject_set_friction(obj, f, C.byref(err))
+
+    def add_animated_obstacle(self, n3, dx, prev, cur, nxt, tris, inversed=False, velocity_scale=1.0):
+        """MeshObject_new + update_mesh_animated (+ inverse, velocity scale),
+        then FluidSimulation_add_mesh_obstacle (meshobject_c.cpp:38-88)."""
+        L = self.L
+        err = C.c_int()
+        L.MeshObject_new.restype = C.c_void_p
+        L.MeshObject_new.argtypes = [C.c_int] * 3 + [C.c_double, C.c_void_p]
+        L.MeshObject_update_mesh_animated.argtypes = [C.c_void_p, TriangleMesh_t, TriangleMesh_t, TriangleMesh_t,
+                                                      C.c_void_p]
+        L.MeshObject_inverse.argtypes = [C.c_void_p, C.c_void_p]
+        L.MeshObject_set_velocity_scale.argtypes = [C.c_void_p, C.c_float, C.c_void_p]
+        L.FluidSimulation_add_mesh_obstacle.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        ms = [_f32(a).reshape(-1, 3) for a in (prev, cur, nxt)]
+        self._keep += ms + [t]
+        obj = C.c_void_p(L.MeshObject_new(*n3, dx, C.byref(err)))
+        L.MeshObject_update_mesh_animated(obj, *[TriangleMesh_t(fp(m), fp(t), len(m), len(t)) for m in ms],
+                                          C.byref(err))
+        if inversed:
+            L.MeshObject_inverse(obj, C.byref(err))
+        if velocity_scale != 1.0:
+            L.MeshObject_set_velocity_scale(obj, velocity_scale, C.byref(err))
+        L.FluidSimulation_add_mesh_obstacle(self.h, obj, C.byref(err))
+        assert err.value == 1
+        self._keep.append(obj)
+        return obj
+
+    def update_solid(self, frame_dt, remaining, step):
+        """The reference's solid update forced at the given frame timing."""
+        P = probe()
+        P.refprobe_sim_update_solid.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        P.refprobe_sim_update_solid(self.h, frame_dt, remaining, step)
+
+    def solid_velocity(self, n3):
+        ni, nj, nk = n3
+        u = np.zeros((nk, nj, ni + 1), np.float32)
+        v = np.zeros((nk, nj + 1, ni), np.float32)
+        w = np.zeros((nk + 1, nj, ni), np.float32)
+        P = probe()
+        P.refprobe_sim_get_solid_velocity.argtypes = [C.c_void_p] * 4
+        P.refprobe_sim_get_solid_velocity(self.h, fp(u), fp(v), fp(w))
+        return u, v, w
 
     def add_mesh_obstacle(self, n3, dx, verts, tris, inversed=False, friction=0.0):
         """MeshObject_new + update_mesh_static (+ inverse, friction), then
