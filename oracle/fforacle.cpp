@@ -1783,9 +1783,15 @@ struct PressureSystem {
  * no-ops for zero solid velocity, see the file header) and
  * _calculateMatrixCoefficientsThread (767-867) with SparseMatrix::add/set
  * (sparsematrix.h:72-111; add ignores a -1 column). */
+//
+// us/vs/ws/wc (optional): the solid face velocities and the cell weights (1 -
+// the solid's volume fraction) for the divergence's solid terms
+// (pressuresolver.cpp:652-657); without them the solid is at rest and those
+// terms add +-0.0, which never changes the sum.
 void buildSystem(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
                  const float *w, const float *wu, const float *wv, const float *ww,
-                 const float *phi, PressureSystem &S) {
+                 const float *phi, PressureSystem &S, const float *us = nullptr, const float *vs = nullptr,
+                 const float *ws = nullptr, const float *wc = nullptr) {
     size_t N = (size_t)ni * nj * nk;
     S.key.assign(N, -1);
     S.rows.clear();
@@ -1829,6 +1835,15 @@ void buildSystem(int ni, int nj, int nk, double dx, double dt, const float *u, c
         div += factor * volBottom * V(i, j, k);
         div += -factor * volFront * W(i, j, k + 1);
         div += factor * volBack * W(i, j, k);
+        if (us) {
+            double volCenter = wc[c];
+            div += factor * (volRight - volCenter) * us[flat(i + 1, j, k, ni + 1, nj)];
+            div += -factor * (volLeft - volCenter) * us[flat(i, j, k, ni + 1, nj)];
+            div += factor * (volTop - volCenter) * vs[flat(i, j + 1, k, ni, nj + 1)];
+            div += -factor * (volBottom - volCenter) * vs[flat(i, j, k, ni, nj + 1)];
+            div += factor * (volFront - volCenter) * ws[flat(i, j, k + 1, ni, nj)];
+            div += -factor * (volBack - volCenter) * ws[flat(i, j, k, ni, nj)];
+        }
         S.rhs[r] = div;
 
         // density grid is 1.0 everywhere on the plain path (fluidsimulation.cpp:6214)
@@ -2102,12 +2117,150 @@ void oracle_apply_mic0(int m, const double *diag, const double *ox, const double
 /* PressureSolver::solve (pressuresolver.cpp:52-92) and _solveLinearSystem
  * (869-916): early exit when max|rhs| < tol, PCG, float pressure grid, and the
  * acceptable-tolerance rule (maxIterations reached with |r| < 1.0). */
-int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt, const float *u,
-                          const float *v, const float *w, const float *wu, const float *wv,
-                          const float *ww, const float *phi, double tol, int maxIter,
-                          int nthreads, float *pressure, int *status, double *err) {
+namespace {
+// PressureSolver::_conditionSolidVelocityField (pressuresolver.cpp:161-312):
+// groups of interior liquid cells connected through faces open to the fluid
+// (weight >= 1e-6), never through a border cell; a group of more than one
+// cell none of which has an open face to an air cell is an isolated pocket,
+// and the solid velocities of all its cells' faces become 0.
+void conditionSolidVelocity(int ni, int nj, int nk, const float *phi, const float *wu, const float *wv,
+                            const float *ww, float *us, float *vs, float *ws) {
+    const float eps = 1e-6f;
+    auto U = [&](int i, int j, int k) { return wu[flat(i, j, k, ni + 1, nj)]; };
+    auto V = [&](int i, int j, int k) { return wv[flat(i, j, k, ni, nj + 1)]; };
+    auto W = [&](int i, int j, int k) { return ww[flat(i, j, k, ni, nj)]; };
+    auto L = [&](int i, int j, int k) { return phi[flat(i, j, k, ni, nj)]; };
+    auto border = [&](int i, int j, int k) {
+        return i == 0 || j == 0 || k == 0 || i == ni - 1 || j == nj - 1 || k == nk - 1;
+    };
+    std::vector<char> air((size_t)ni * nj * nk, 0), done((size_t)ni * nj * nk, 0);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                if (border(i, j, k)) continue;
+                if ((U(i, j, k) >= 0.0f + eps && L(i - 1, j, k) >= 0.0f) ||
+                    (U(i + 1, j, k) >= 0.0f + eps && L(i + 1, j, k) >= 0.0f) ||
+                    (V(i, j, k) >= 0.0f + eps && L(i, j - 1, k) >= 0.0f) ||
+                    (V(i, j + 1, k) >= 0.0f + eps && L(i, j + 1, k) >= 0.0f) ||
+                    (W(i, j, k) >= 0.0f + eps && L(i, j, k - 1) >= 0.0f) ||
+                    (W(i, j, k + 1) >= 0.0f + eps && L(i, j, k + 1) >= 0.0f))
+                    air[flat(i, j, k, ni, nj)] = 1;
+            }
+    std::vector<GI> queue, group;
+    for (int k = 1; k < nk - 1; k++)
+        for (int j = 1; j < nj - 1; j++)
+            for (int i = 1; i < ni - 1; i++) {
+                size_t c = flat(i, j, k, ni, nj);
+                if (L(i, j, k) >= 0.0f) {
+                    done[c] = 1;
+                    continue;
+                }
+                if (done[c]) continue;
+                queue.assign(1, GI{i, j, k});
+                done[c] = 1;
+                group.clear();
+                while (!queue.empty()) {
+                    GI g = queue.back();
+                    queue.pop_back();
+                    auto visit = [&](int a, int b, int d, float weight) {
+                        size_t n = flat(a, b, d, ni, nj);
+                        if (!done[n] && L(a, b, d) < 0.0f && weight >= eps) {
+                            done[n] = 1;
+                            if (!border(a, b, d)) queue.push_back(GI{a, b, d});
+                        }
+                    };
+                    visit(g.i - 1, g.j, g.k, U(g.i, g.j, g.k));
+                    visit(g.i + 1, g.j, g.k, U(g.i + 1, g.j, g.k));
+                    visit(g.i, g.j - 1, g.k, V(g.i, g.j, g.k));
+                    visit(g.i, g.j + 1, g.k, V(g.i, g.j + 1, g.k));
+                    visit(g.i, g.j, g.k - 1, W(g.i, g.j, g.k));
+                    visit(g.i, g.j, g.k + 1, W(g.i, g.j, g.k + 1));
+                    group.push_back(g);
+                }
+                if (group.size() == 1) continue;
+                bool isolated = true;
+                for (GI g : group)
+                    if (air[flat(g.i, g.j, g.k, ni, nj)]) {
+                        isolated = false;
+                        break;
+                    }
+                if (!isolated) continue;
+                for (GI g : group) {
+                    us[flat(g.i, g.j, g.k, ni + 1, nj)] = 0.0f;
+                    us[flat(g.i + 1, g.j, g.k, ni + 1, nj)] = 0.0f;
+                    vs[flat(g.i, g.j, g.k, ni, nj + 1)] = 0.0f;
+                    vs[flat(g.i, g.j + 1, g.k, ni, nj + 1)] = 0.0f;
+                    ws[flat(g.i, g.j, g.k, ni, nj)] = 0.0f;
+                    ws[flat(g.i, g.j, g.k + 1, ni, nj)] = 0.0f;
+                }
+            }
+}
+
+// LevelsetUtils volume fractions (levelsetutils.h:60-100, levelsetutils.cpp:
+// 213-258) and MeshLevelSet::_getCellWeight (meshlevelset.cpp:1491-1515)
+void sort4(float &a, float &b, float &c, float &d) {
+    if (a > b) std::swap(a, b);
+    if (c > d) std::swap(c, d);
+    if (a > c) std::swap(a, c);
+    if (b > d) std::swap(b, d);
+    if (b > c) std::swap(b, c);
+}
+float tetFraction(float p0, float p1, float p2, float p3) {
+    return p0 * p0 * p0 / ((p0 - p1) * (p0 - p2) * (p0 - p3));
+}
+float prismFraction(float p0, float p1, float p2, float p3) {
+    float a = p0 / (p0 - p2), b = p0 / (p0 - p3), c = p1 / (p1 - p3), d = p1 / (p1 - p2);
+    return a * b * (1 - d) + b * (1 - c) * d + c * d;
+}
+float volumeFraction4(float p0, float p1, float p2, float p3) {
+    sort4(p0, p1, p2, p3);
+    if (p3 <= 0) return 1;
+    if (p2 <= 0) return 1 - tetFraction(p3, p2, p1, p0);
+    if (p1 <= 0) return prismFraction(p0, p1, p2, p3);
+    if (p0 <= 0) return tetFraction(p0, p1, p2, p3);
+    return 0;
+}
+float volumeFraction8(float p000, float p100, float p010, float p110, float p001, float p101, float p011,
+                      float p111) {
+    return (volumeFraction4(p000, p001, p101, p011) + volumeFraction4(p000, p101, p100, p110) +
+            volumeFraction4(p000, p010, p011, p110) + volumeFraction4(p101, p011, p111, p110) +
+            2 * volumeFraction4(p000, p011, p101, p110) + volumeFraction4(p100, p101, p001, p111) +
+            volumeFraction4(p100, p001, p000, p010) + volumeFraction4(p100, p110, p111, p010) +
+            volumeFraction4(p001, p111, p011, p010) + 2 * volumeFraction4(p100, p111, p001, p010)) /
+           12.0f;
+}
+// the weight grid's cell weights (fluidsimulation.cpp:6140-6147): 1 - the
+// solid's volume fraction in each cell, clamped to [0, 1]
+void cellWeights(int ni, int nj, int nk, const float *solid, std::vector<float> &wc) {
+    wc.assign((size_t)ni * nj * nk, 0.0f);
+    const int pi = ni + 1, pj = nj + 1;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                auto P = [&](int a, int b, int c) { return solid[flat(i + a, j + b, k + c, pi, pj)]; };
+                float p000 = P(0, 0, 0), p001 = P(0, 0, 1), p010 = P(0, 1, 0), p011 = P(0, 1, 1);
+                float p100 = P(1, 0, 0), p101 = P(1, 0, 1), p110 = P(1, 1, 0), p111 = P(1, 1, 1);
+                float weight;
+                if (p000 < 0 && p001 < 0 && p010 < 0 && p011 < 0 && p100 < 0 && p101 < 0 && p110 < 0 && p111 < 0)
+                    weight = 1.0f;
+                else if (p000 >= 0 && p001 >= 0 && p010 >= 0 && p011 >= 0 && p100 >= 0 && p101 >= 0 &&
+                         p110 >= 0 && p111 >= 0)
+                    weight = 0.0f;
+                else
+                    weight = volumeFraction8(p000, p100, p010, p110, p001, p101, p011, p111);
+                wc[flat(i, j, k, ni, nj)] = clampf(1.0f - weight, 0.0f, 1.0f);
+            }
+}
+
+// PressureSolver::solve (pressuresolver.cpp:52-92): the moving solid's
+// velocities (conditioned in place) enter the divergence when us is given
+int pressureSolveImpl(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
+                      const float *w, const float *wu, const float *wv, const float *ww, const float *phi,
+                      double tol, int maxIter, int nthreads, float *pressure, int *status, double *err, float *us,
+                      float *vs, float *ws, const float *wc) {
     PressureSystem S;
-    buildSystem(ni, nj, nk, dx, dt, u, v, w, wu, wv, ww, phi, S);
+    if (us) conditionSolidVelocity(ni, nj, nk, phi, wu, wv, ww, us, vs, ws);
+    buildSystem(ni, nj, nk, dx, dt, u, v, w, wu, wv, ww, phi, S, us, vs, ws, wc);
     size_t N = (size_t)ni * nj * nk;
     for (size_t c = 0; c < N; c++) pressure[c] = 0.0f;
     double maxAbs = 0.0;
@@ -2130,6 +2283,15 @@ int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt, const fl
     status[1] = ret ? 1 : 0;
     err[0] = e;
     return S.m;
+}
+}  // namespace
+
+int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt, const float *u,
+                          const float *v, const float *w, const float *wu, const float *wv,
+                          const float *ww, const float *phi, double tol, int maxIter,
+                          int nthreads, float *pressure, int *status, double *err) {
+    return pressureSolveImpl(ni, nj, nk, dx, dt, u, v, w, wu, wv, ww, phi, tol, maxIter, nthreads, pressure,
+                             status, err, nullptr, nullptr, nullptr, nullptr);
 }
 
 /* PressureSolver::applySolutionToVelocityField (pressuresolver.cpp:94-112) and
@@ -2378,6 +2540,16 @@ struct OracleSim {
     double nsCell = 0;
     std::vector<float> phi, u, v, w, su, sv, sw, pres;
     std::vector<unsigned char> vu, vv, vw;
+    // mesh obstacles with their velocities: the solid is rebuilt at a
+    // substep's start when the list changed or an animated mesh moves
+    // (fluidsimulation.cpp:5387-5426, 5550-5570); the sign tests take their
+    // jitter from the given rand() values in order
+    std::vector<MovingObstacle> movers;
+    bool moversChanged = false;
+    std::vector<int> rnd;
+    size_t rndUsed = 0;
+    double randMax = 2147483647.0;
+    std::vector<float> us, vs, ws, wc;  // solid face velocities, cell weights
     int lastSubsteps = 0, lastIters = 0, pIters = 0;
     bool pSuccess = true;
     double lastErr = 0, pErr = 0;
@@ -2495,8 +2667,32 @@ struct OracleSim {
     }
 
     // FluidSimulation::_stepFluid (fluidsimulation.cpp:10078-10121), plain configuration
+    // MeshObject::_isMeshChanged (meshobject.cpp:893-912)
+    static bool meshChanged(const MovingObstacle &o) {
+        if (!o.animated) return false;
+        if (o.prev.v.size() != o.cur.v.size()) return true;
+        for (size_t q = 0; q < o.prev.v.size(); q++)
+            if (length(o.prev.v[q] - o.cur.v[q]) > 1e-5f) return true;
+        return false;
+    }
+    // _updateSolidLevelSet (fluidsimulation.cpp:5387-5426) at the substep's
+    // frame progress, then the weight grid (6066-6148)
+    void updateMovers(double dt) {
+        bool rebuild = moversChanged;
+        for (const MovingObstacle &o : movers) rebuild = rebuild || meshChanged(o);
+        if (!rebuild) return;
+        moversChanged = false;
+        float progress = 1.0f - (float)(remaining + dt) / (float)frameDt;
+        RandSource rs{rnd.data() + rndUsed, (int)(rnd.size() - rndUsed), 0, randMax};
+        solidUpdate(ni, nj, nk, dx, movers, frameDt, progress, rs, solid, us, vs, ws);
+        rndUsed += rs.used;
+        oracle_face_weights(ni, nj, nk, solid.data(), wu.data(), wv.data(), ww.data());
+        cellWeights(ni, nj, nk, solid.data(), wc);
+    }
+
     void stepFluid(double dt) {
         bool fluid = !pos.empty();
+        if (fluid && !movers.empty()) updateMovers(dt);
         if (fluid) updateNearSolid();
         // liquid level set (5586-5617)
         phi.assign(nN(), 0.0f);
@@ -2553,9 +2749,11 @@ struct OracleSim {
         pres.assign(nN(), 0.0f);
         int status[2];
         double err = 0;
-        oracle_pressure_solve(ni, nj, nk, dx, dt, u.data(), v.data(), w.data(), wu.data(),
-                              wv.data(), ww.data(), phi.data(), ptol, maxPIters, chunks,
-                              pres.data(), status, &err);
+        const bool moving = !movers.empty() && !us.empty();
+        pressureSolveImpl(ni, nj, nk, dx, dt, u.data(), v.data(), w.data(), wu.data(), wv.data(), ww.data(),
+                          phi.data(), ptol, maxPIters, chunks, pres.data(), status, &err,
+                          moving ? us.data() : nullptr, moving ? vs.data() : nullptr, moving ? ws.data() : nullptr,
+                          moving ? wc.data() : nullptr);
         bool success = status[1] != 0;
         if (success)
             oracle_apply_pressure(ni, nj, nk, dx, dt, pres.data(), phi.data(), wu.data(),
@@ -2622,19 +2820,23 @@ struct OracleSim {
 
     // _constrainVelocityFieldThread (fluidsimulation.cpp:6388-6442): domain friction 0.0
     // (fluidsimulation.h:2227), solid face velocity 0 for static solids.
+    // _constrainVelocityFieldThread (fluidsimulation.cpp:6392-6442) without
+    // friction; the solid face velocities of moving obstacles, else 0
     void constrain(std::vector<float> &a, std::vector<float> &b, std::vector<float> &c) {
-        float f = 0.0f, sface = 0.0f;
-        auto one = [&](std::vector<float> &x, const std::vector<float> &wt) {
+        const bool moving = !movers.empty() && !us.empty();
+        float f = 0.0f;
+        auto one = [&](std::vector<float> &x, const std::vector<float> &wt, const std::vector<float> &sv) {
             for (size_t q = 0; q < x.size(); q++) {
+                float sface = moving ? sv[q] : 0.0f;
                 if (wt[q] == 0)
                     x[q] = sface;
                 else if (wt[q] < 1.0f)
                     x[q] = f * sface + (1.0f - f) * x[q];
             }
         };
-        one(a, wu);
-        one(b, wv);
-        one(c, ww);
+        one(a, wu, us);
+        one(b, wv, vs);
+        one(c, ww, ws);
     }
 
     // _RK3 (fluidsimulation.cpp:7616-7623)
@@ -2798,6 +3000,54 @@ void oracle_sim_set_cfl(OracleSim *s, int cfl) { s->cfl = cfl; }
 void oracle_sim_set_pressure_tolerance(OracleSim *s, double tol) { s->ptol = tol; }
 void oracle_sim_set_max_pressure_iterations(OracleSim *s, int n) { s->maxPIters = n; }
 void oracle_sim_set_thread_chunks(OracleSim *s, int n) { s->chunks = n < 1 ? 1 : n; }
+int oracle_sim_add_moving_obstacle(OracleSim *s, int nv, const float *prev, const float *cur, const float *next,
+                                   int nt, const int *tris, int flags, float expansion, float vscale) {
+    try {
+        MovingObstacle o;
+        o.prev = meshFrom(nv, prev, nt, tris);
+        o.cur = meshFrom(nv, cur, nt, tris);
+        o.next = meshFrom(nv, next, nt, tris);
+        o.animated = (flags & 1) != 0;
+        o.inversed = (flags & 2) != 0;
+        o.expansion = expansion;
+        o.velocityScale = vscale;
+        s->movers.push_back(o);
+        s->moversChanged = true;
+        return (int)s->movers.size() - 1;
+    } catch (const std::exception &) {
+        return -1;
+    }
+}
+int oracle_sim_update_moving_obstacle(OracleSim *s, int index, const float *prev, const float *cur,
+                                      const float *next) {
+    if (index < 0 || index >= (int)s->movers.size()) return -1;
+    MovingObstacle &o = s->movers[index];
+    for (size_t q = 0; q < o.cur.v.size(); q++) {
+        o.prev.v[q] = V3{prev[3 * q], prev[3 * q + 1], prev[3 * q + 2]};
+        o.cur.v[q] = V3{cur[3 * q], cur[3 * q + 1], cur[3 * q + 2]};
+        o.next.v[q] = V3{next[3 * q], next[3 * q + 1], next[3 * q + 2]};
+    }
+    return 0;
+}
+void oracle_sim_set_rand_values(OracleSim *s, const int *rnd, int n, int rand_max) {
+    s->rnd.assign(rnd, rnd + n);
+    s->rndUsed = 0;
+    s->randMax = (double)rand_max;
+}
+int oracle_sim_rand_used(OracleSim *s) { return (int)s->rndUsed; }
+void oracle_sim_get_solid_velocity(OracleSim *s, float *u, float *v, float *w) {
+    if (s->us.empty()) {
+        std::fill(u, u + s->nU(), 0.0f);
+        std::fill(v, v + s->nV(), 0.0f);
+        std::fill(w, w + s->nW(), 0.0f);
+        return;
+    }
+    std::copy(s->us.begin(), s->us.end(), u);
+    std::copy(s->vs.begin(), s->vs.end(), v);
+    std::copy(s->ws.begin(), s->ws.end(), w);
+}
+void oracle_sim_get_solid_phi(OracleSim *s, float *phi) { std::copy(s->solid.begin(), s->solid.end(), phi); }
+
 void oracle_sim_set_solid_phi(OracleSim *s, const float *phi) {
     s->solid.assign(phi, phi + s->nNodes());
     s->solidGiven = true;

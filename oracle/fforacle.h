@@ -164,6 +164,18 @@ void oracle_sim_set_max_pressure_iterations(OracleSim *s, int n);
 void oracle_sim_set_thread_chunks(OracleSim *s, int n);
 void oracle_sim_set_solid_phi(OracleSim *s, const float *phi_nodes);
 void oracle_sim_load_particles(OracleSim *s, int np, const float *pos, const float *vel);
+/* Mesh obstacles with velocities (static or animated, normal or inversed:
+ * flags bit 0 animated, bit 1 inversed), one triangle list and three vertex
+ * sets each; replaces the box/set solid by the solid update of every substep
+ * whose meshes move.  The sign tests take the given rand() values in order. */
+int oracle_sim_add_moving_obstacle(OracleSim *s, int nv, const float *prev, const float *cur, const float *next,
+                                   int nt, const int *tris, int flags, float expansion, float vscale);
+int oracle_sim_update_moving_obstacle(OracleSim *s, int index, const float *prev, const float *cur,
+                                      const float *next);
+void oracle_sim_set_rand_values(OracleSim *s, const int *rnd, int n, int rand_max);
+int oracle_sim_rand_used(OracleSim *s);
+void oracle_sim_get_solid_velocity(OracleSim *s, float *u, float *v, float *w);
+void oracle_sim_get_solid_phi(OracleSim *s, float *phi);
 void oracle_sim_initialize(OracleSim *s);
 void oracle_sim_update(OracleSim *s, double frame_dt);
 void oracle_sim_step(OracleSim *s, double dt);

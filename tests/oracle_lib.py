@@ -396,6 +396,50 @@ class OracleSim:
     def pressure_iterations(self):
         return oracle().oracle_sim_last_pressure_iterations(self.h)
 
+    def add_moving_obstacle(self, prev, cur, nxt, tris, animated=True, inversed=False, expansion=0.0,
+                            velocity_scale=1.0):
+        """A mesh obstacle with velocities; returns its index."""
+        L = oracle()
+        L.oracle_sim_add_moving_obstacle.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                     C.c_int, C.c_void_p, C.c_int, C.c_float, C.c_float]
+        ms = [_f32(a).reshape(-1, 3) for a in (prev, cur, nxt)]
+        t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        idx = L.oracle_sim_add_moving_obstacle(self.h, len(ms[1]), fp(ms[0]), fp(ms[1]), fp(ms[2]), len(t), fp(t),
+                                               (1 if animated else 0) | (2 if inversed else 0), expansion,
+                                               velocity_scale)
+        if idx < 0:
+            raise RuntimeError("oracle_sim_add_moving_obstacle failed")
+        return idx
+
+    def update_moving_obstacle(self, idx, prev, cur, nxt):
+        L = oracle()
+        L.oracle_sim_update_moving_obstacle.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        ms = [_f32(a).reshape(-1, 3) for a in (prev, cur, nxt)]
+        if L.oracle_sim_update_moving_obstacle(self.h, idx, *[fp(m) for m in ms]) != 0:
+            raise RuntimeError("oracle_sim_update_moving_obstacle failed")
+
+    def set_rand(self, seed, n=20000):
+        """The sign tests' jitter: the C library's rand() values after srand(seed)."""
+        self._rnd = libc_rand_values(seed, n)
+        L = oracle()
+        L.oracle_sim_set_rand_values.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.oracle_sim_set_rand_values(self.h, fp(self._rnd), len(self._rnd), RAND_MAX)
+
+    def rand_used(self):
+        L = oracle()
+        L.oracle_sim_rand_used.argtypes = [C.c_void_p]
+        return L.oracle_sim_rand_used(self.h)
+
+    def solid_velocity(self):
+        ni, nj, nk = self.n3
+        u = np.zeros((nk, nj, ni + 1), np.float32)
+        v = np.zeros((nk, nj + 1, ni), np.float32)
+        w = np.zeros((nk + 1, nj, ni), np.float32)
+        L = oracle()
+        L.oracle_sim_get_solid_velocity.argtypes = [C.c_void_p] * 4
+        L.oracle_sim_get_solid_velocity(self.h, fp(u), fp(v), fp(w))
+        return u, v, w
+
     def grids(self):
         ni, nj, nk = self.n3
         phi = np.zeros((nk, nj, ni), np.float32)
@@ -494,6 +538,17 @@ class ReferenceSim:
         assert err.value == 1
         self._keep.append(obj)
         return obj
+
+    def update_animated_obstacle(self, obj, prev, cur, nxt, tris):
+        """MeshObject_update_mesh_animated on an obstacle already added."""
+        L = self.L
+        err = C.c_int()
+        t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        ms = [_f32(a).reshape(-1, 3) for a in (prev, cur, nxt)]
+        self._keep += ms + [t]
+        L.MeshObject_update_mesh_animated(obj, *[TriangleMesh_t(fp(m), fp(t), len(m), len(t)) for m in ms],
+                                          C.byref(err))
+        assert err.value == 1
 
     def update_solid(self, frame_dt, remaining, step):
         """The reference's solid update forced at the given frame timing."""

@@ -1,9 +1,10 @@
-"""The pressure rows' neighbour lookup used by the SpMV (csrc/kernels_pressure.hip
-rowProduct) restated on the CPU and checked against the oracle's neighbour
-lists (pressuresolver.cpp:118-159 row order, the 7-point stencil of
+"""The line words' row lookup used by the MIC(0) line pipeline
+(csrc/pencil_sweep.h rowAt, csrc/kernels_pencil.hip lineWordsKernel)
+restated on the CPU and checked against the oracle's neighbour lists
+(pressuresolver.cpp:118-159 row order, the 7-point stencil of
 sparsematrix.h:174-215).
 
-The kernel keeps no per-row neighbour ids.  It finds:
+The lookup needs no per-row neighbour ids.  It finds:
   * the +-x neighbour rows as r -+ 1, present when rows[r -+ 1] is the adjacent
     cell;
   * the +-y/+-z neighbour rows in the line words: for each 32-cell word of a
