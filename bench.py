@@ -119,7 +119,10 @@ def enable_timing(sim, on):
     pb.execute_lib_func(f, [sim(), 1 if on else 0])
 
 
-def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt, apic=False, warm=False, schedule=None):
+def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt, apic=False, warm=False, schedule=None,
+             keep_state=False):
+    """keep_state: also return the particles (positions, velocities) and the
+    nodal solid SDF after the timed steps, for the CPU baseline."""
     import torch
     sim = make_sim(n, pos, solid, mode, tol, apic, warm, schedule)
     for s in range(warmup):
@@ -143,9 +146,14 @@ def run_mode(n, pos, solid, mode, tol, steps, warmup, frame_dt, apic=False, warm
     free, total = torch.cuda.mem_get_info()
     kt = kernel_times(sim)
     nparticles = sim.get_num_marker_particles()
+    state = None
+    if keep_state:
+        p = np.frombuffer(sim.get_marker_particle_position_data_range(0, nparticles), np.float32).reshape(-1, 3)
+        v = np.frombuffer(sim.get_marker_particle_velocity_data_range(0, nparticles), np.float32).reshape(-1, 3)
+        state = (p.copy(), v.copy(), sim.get_solid_sdf().copy())
     del sim
     return {"elapsed_s": elapsed, "substeps": substeps, "iterations": iters, "unknowns": unknowns,
-            "kernels": kt, "particles": nparticles, "device_bytes_used": total - free}
+            "kernels": kt, "particles": nparticles, "device_bytes_used": total - free, "state": state}
 
 
 def algorithmic_bytes(name, n, P, M_total, launches_per_sub, substeps, iters_total, mode):
@@ -198,33 +206,105 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(n_sample, threads, seconds_budget=30.0):
-    """The reference engine (built from its sources into oracle/_ref) on the
-    same scene at n_sample^3, timed on the host cores."""
+CALIBRATION = os.path.join(ROOT, "profiles", "cpu_baseline_dam_break_256.md")
+
+
+def calibration_ratio():
+    """The reference engine's substeps/s over the oracle's, both measured on
+    the same host and thread count on the 256^3 scene (the line
+    'calibration reference_over_port <ratio>' of the committed summary), or
+    None."""
+    if not os.path.isfile(CALIBRATION):
+        return None
+    with open(CALIBRATION) as fh:
+        for line in fh:
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[:2] == ["calibration", "reference_over_port"]:
+                return float(parts[2])
+    return None
+
+
+def cpu_baseline(state, n, tol, threads):
+    """The oracle (this build's CPU restatement, tests/oracle_lib) timed on the
+    host cores for one substep of the same scene: the GPU run's particles and
+    solid SDF after its timed steps.  Its liquid SDF, P2G, SpMV, vector updates,
+    G2P and advection run on `threads` threads (the same bits as serial); the
+    MIC(0) factor and sweeps and the dot products are serial, as in the
+    reference engine's PCG."""
+    import oracle_lib
+    pos, vel, solid = state
+    oracle_lib.set_threads(threads)
+    try:
+        orc = oracle_lib.OracleSim((n, n, n), 1.0 / n, tol=tol)
+        orc.set_solid_phi(solid)
+        orc.load(pos, vel)
+        orc.initialize()
+        t0 = time.perf_counter()
+        orc.step(1.0 / 240.0)
+        el = time.perf_counter() - t0
+    finally:
+        oracle_lib.set_threads(1)
+    ratio = calibration_ratio()
+    out = {"value": round(1.0 / el, 5), "unit": "substeps/s", "cores": threads, "kind": "port",
+           "sample": "the oracle (CPU restatement of the path) for one substep of 1/240 s of the same %d^3 scene "
+                     "(%d particles: the GPU run's state after its timed steps, PCG rel res %g), %.1f s; liquid "
+                     "SDF, P2G, SpMV, vector updates, G2P and advection on %d threads, MIC(0) and dots serial"
+                     % (n, len(pos), tol, el, threads)}
+    if ratio is not None:
+        out["calibration"] = {"reference_over_port": ratio,
+                              "reference_equivalent_substeps_per_s": round(ratio / el, 5),
+                              "source": os.path.relpath(CALIBRATION, ROOT)}
+    return out
+
+
+def cpu_calibration(n, substeps, threads):
+    """No GPU: the reference engine (oracle/_ref, built from its sources) and
+    the oracle on the same host, the same thread count and the same scene (the
+    n^3 dam break with the sphere obstacle, from rest, PCG to the reference's
+    fixed 1e-9): one warm-up substep (builds the solid), then `substeps` timed
+    substeps of 1/240 s each.  Prints one JSON line."""
     import oracle_lib
     from ffengine import scenes
     if not oracle_lib.reference_available():
-        return None
-    dx = 1.0 / n_sample
-    pos = scenes.dam_break_particles(n_sample, seed=0)
-    sim = oracle_lib.ReferenceSim((n_sample,) * 3, dx, threads=threads)
-    sim.load(pos)
-    v, t = scenes.sphere_mesh(subdivisions=5)
-    sim.add_mesh_obstacle((n_sample,) * 3, dx, v, t)
-    sim.initialize()
-    sim.update(1.0 / 240.0)  # warm-up substep (builds the solid SDF)
-    t0 = time.perf_counter()
-    count = 0
-    while True:
-        sim.update(1.0 / 240.0)
-        count += 1
-        if time.perf_counter() - t0 > seconds_budget * 0.5 or count >= 3:
-            break
-    el = time.perf_counter() - t0
-    return {"value": count / el, "unit": "substeps/s", "cores": threads, "kind": "reference",
-            "sample": "reference engine (oracle/_ref, built from its sources) on the %d^3 dam break with "
-                      "the same sphere MeshObject (%d triangles), %d timed substeps of 1/240 s after one "
-                      "warm-up substep" % (n_sample, len(t), count)}
+        sys.exit("the reference build (oracle/_ref) is missing: run `make -C oracle ref`")
+    dx = 1.0 / n
+    pos = scenes.dam_break_particles(n, seed=0)
+    verts, tris = scenes.sphere_mesh(subdivisions=5)
+    out = {}
+    log("reference engine n=%d threads=%d" % (n, threads))
+    ref = oracle_lib.ReferenceSim((n,) * 3, dx, threads=threads)
+    ref.load(pos)
+    ref.add_mesh_obstacle((n,) * 3, dx, verts, tris)
+    ref.initialize()
+    ref.update(1.0 / 240.0)
+    times = []
+    for s in range(substeps):
+        t0 = time.perf_counter()
+        ref.update(1.0 / 240.0)
+        times.append(time.perf_counter() - t0)
+        log("  reference substep %d: %.2f s" % (s + 1, times[-1]))
+    out["reference"] = {"substeps_per_s": len(times) / sum(times), "seconds": [round(t, 3) for t in times]}
+    del ref
+    log("oracle n=%d threads=%d" % (n, threads))
+    oracle_lib.set_threads(threads)
+    orc = oracle_lib.OracleSim((n,) * 3, dx, tol=1e-9)
+    orc.set_solid_phi(oracle_lib.solid_phi_meshes((n,) * 3, dx, [(verts, tris)], 7)[0])
+    orc.load(pos)
+    orc.initialize()
+    orc.step(1.0 / 240.0)
+    times = []
+    for s in range(substeps):
+        t0 = time.perf_counter()
+        orc.step(1.0 / 240.0)
+        times.append(time.perf_counter() - t0)
+        log("  oracle substep %d: %.2f s" % (s + 1, times[-1]))
+    oracle_lib.set_threads(1)
+    out["port"] = {"substeps_per_s": len(times) / sum(times), "seconds": [round(t, 3) for t in times]}
+    ratio = out["reference"]["substeps_per_s"] / out["port"]["substeps_per_s"]
+    line = {"metric": "CPU substeps/s at %d^3: reference engine vs the oracle, same host and threads" % n,
+            "n": n, "threads": threads, "host_cpus": os.cpu_count(), "substeps": substeps,
+            "reference": out["reference"], "port": out["port"], "reference_over_port": round(ratio, 4)}
+    print(json.dumps(line), flush=True)
 
 
 def precision_config(n, tol):
@@ -414,10 +494,12 @@ def adjoint_config(n, substeps, checkpoint_every, fd_faces, eps, tol, dt):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", default="dam-break", choices=["dam-break", "precision", "splash", "adjoint"],
+    ap.add_argument("--config", default="dam-break",
+                    choices=["dam-break", "precision", "splash", "adjoint", "cpu-calibration"],
                     help="dam-break: the 256^3 headline benchmark (default); precision: the 384^3 slab "
                          "pressure-accuracy comparison; splash: the 512^3 splash; adjoint: the 128^3 "
-                         "differentiable rollout with its finite-difference check")
+                         "differentiable rollout with its finite-difference check; cpu-calibration (no GPU, "
+                         "needs the reference build): the reference engine against the oracle on the host")
     ap.add_argument("--substeps", type=int, default=32, help="adjoint: substeps of the rollout")
     ap.add_argument("--checkpoint-every", type=int, default=8, help="adjoint: checkpoint spacing")
     ap.add_argument("--fd-faces", type=int, default=64, help="adjoint: faces checked by central differences")
@@ -438,8 +520,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None, help="grid edge (default 256, 384 or 512 by --config)")
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--cpu-sample-n", type=int, default=96)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="host threads of the CPU baseline (default: min(16, cpu count))")
+    ap.add_argument("--cpu-substeps", type=int, default=3, help="cpu-calibration: timed substeps per engine")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--schedule", type=int, default=None, choices=[2, 4],
                     help="MIC(0) application: 2 = bands of hyperplanes, 4 = line pipeline (default: the engine's)")
@@ -448,6 +531,10 @@ def main():
                          "mixed-reliable (FP32 PCG with FP64 reliable updates), fp32-preconditioner (FP64 "
                          "PCG with the MIC(0) application in FP32)")
     args = ap.parse_args()
+    cpu_threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    if args.config == "cpu-calibration":
+        cpu_calibration(args.n or 256, args.cpu_substeps, cpu_threads)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: the engine keeps a whole simulation on one device "
               "(multi-GPU slab decomposition is design-only); got --gpus %d" % args.gpus, file=sys.stderr)
@@ -491,10 +578,14 @@ def main():
     if not set(modes) <= {"fp64", "mixed", "mixed-reliable", "fp32-preconditioner"} or not modes:
         sys.exit("--modes takes fp64, mixed, mixed-reliable and/or fp32-preconditioner")
     results = {}
-    for mode in modes:
+    for q, mode in enumerate(modes):
         log("mode %s" % mode)
         results[mode] = run_mode(n, pos, solid, mode, args.tol, args.steps, args.warmup, frame_dt,
-                                 args.transfer == "apic", args.warm_start, args.schedule)
+                                 args.transfer == "apic", args.warm_start, args.schedule,
+                                 keep_state=(q == 0 and not args.no_cpu))
+    state = results[modes[0]].pop("state")
+    for r in results.values():
+        r.pop("state", None)
     # headline: the faster mixed-precision mode, else FP64
     mixed_modes = [m for m in ("mixed-reliable", "mixed", "fp32-preconditioner") if m in results]
     rate = lambda m: results[m]["substeps"] / results[m]["elapsed_s"]  # noqa: E731
@@ -535,12 +626,13 @@ def main():
     mesh_sdf = None
     if args.obstacle == "mesh":
         log("mesh obstacle SDF build")
-        mesh_sdf = mesh_obstacle_sdf(n, solid, not args.no_cpu, args.cpu_threads)
+        mesh_sdf = mesh_obstacle_sdf(n, solid, not args.no_cpu, cpu_threads)
 
     cpu = None
-    if not args.no_cpu:
-        log("cpu baseline n=%d threads=%d" % (args.cpu_sample_n, args.cpu_threads))
-        cpu = cpu_baseline(args.cpu_sample_n, args.cpu_threads)
+    if not args.no_cpu and state is not None:
+        log("cpu baseline: the oracle, one substep at n=%d on %d threads" % (n, cpu_threads))
+        cpu = cpu_baseline(state, n, args.tol, cpu_threads)
+    del state
 
     # the same two solvers started from the previous substep's pressure (an
     # option of this engine; the reference's PCG starts from 0): reported

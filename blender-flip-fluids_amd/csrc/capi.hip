@@ -290,10 +290,16 @@ void MeshObject_destroy(MeshObject *obj) { delete reinterpret_cast<ff::MeshObsta
 void MeshObject_update_mesh_static(MeshObject *obj, TriangleMesh_t mesh, int *err) {
     guarded(err, [&] { M(obj)->updateMeshStatic(mesh.numVertices, mesh.vertices, mesh.numTriangles, mesh.triangles); });
 }
-void MeshObject_update_mesh_animated(MeshObject *obj, TriangleMesh_t, TriangleMesh_t, TriangleMesh_t, int *err) {
+void MeshObject_update_mesh_animated(MeshObject *obj, TriangleMesh_t previous, TriangleMesh_t current,
+                                     TriangleMesh_t next, int *err) {
     guarded(err, [&] {
-        M(obj);
-        throw std::runtime_error("Error: animated mesh obstacles are not supported by this engine.\n");
+        ff::MeshObstacle *o = M(obj);
+        if (previous.numVertices != current.numVertices || next.numVertices != current.numVertices ||
+            previous.numTriangles != current.numTriangles || next.numTriangles != current.numTriangles)
+            throw std::runtime_error("Error: animated mesh obstacles must keep their vertex and triangle counts "
+                                     "between frames.\n");
+        o->updateMeshAnimated(current.numVertices, previous.vertices, current.vertices, next.vertices,
+                              current.numTriangles, current.triangles);
     });
 }
 void MeshObject_enable(MeshObject *obj, int *err) {
@@ -429,6 +435,9 @@ void FluidSimulation_set_solid_phi(FluidSimulation *o, const float *phi, int *er
 }
 void FluidSimulation_get_liquid_sdf_data(FluidSimulation *o, float *d, int *err) {
     guarded(err, [&] { S(o)->getLiquidSDF(d); });
+}
+void FluidSimulation_get_solid_velocity_field(FluidSimulation *o, float *u, float *v, float *w, int *err) {
+    guarded(err, [&] { S(o)->getSolidVelocityField(u, v, w); });
 }
 void FluidSimulation_get_solid_sdf_data(FluidSimulation *o, float *d, int *err) {
     guarded(err, [&] { S(o)->getSolidSDF(d); });
@@ -686,6 +695,99 @@ int FFAMD_solid_phi_meshes(int ni, int nj, int nk, double dx, int nobj, const in
         down(phi, d.p, nn, sc.s);
         sync(sc.s);
         return draws;
+    });
+}
+
+int FFAMD_solid_update_moving(int ni, int nj, int nk, double dx, int nobj, const int *nv, const int *nt,
+                              const float *prev, const float *cur, const float *next, const int *triangles,
+                              const int *flags, const float *expansion, const float *velocity_scale,
+                              double frame_dt, float frame_progress, float *phi, float *u, float *v, float *w,
+                              int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (nobj < 0) throw std::invalid_argument("Error: negative obstacle count.\n");
+        const ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        const size_t nn = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
+        const size_t nf[3] = {(size_t)(ni + 1) * nj * nk, (size_t)ni * (nj + 1) * nk, (size_t)ni * nj * (nk + 1)};
+        // the objects as the engine holds them, at the frame progress
+        std::vector<ff::MeshObstacle> objs;
+        objs.reserve(nobj);
+        std::vector<std::vector<float>> pos(nobj), vel(nobj);
+        std::vector<ff::ObstacleMesh> meshes;
+        size_t vo = 0, to = 0;
+        for (int q = 0; q < nobj; q++) {
+            objs.emplace_back(ni, nj, nk, dx);
+            ff::MeshObstacle &o = objs.back();
+            if (flags[q] & 1)
+                o.updateMeshAnimated(nv[q], prev + 3 * vo, cur + 3 * vo, next + 3 * vo, nt[q], triangles + 3 * to);
+            else
+                o.updateMeshStatic(nv[q], cur + 3 * vo, nt[q], triangles + 3 * to);
+            o.inversed = (flags[q] & 2) != 0;
+            o.meshExpansion = expansion ? expansion[q] : 0.0f;
+            o.velocityScale = velocity_scale ? velocity_scale[q] : 1.0f;
+            vo += (size_t)nv[q];
+            to += (size_t)nt[q];
+        }
+        for (int q = 0; q < nobj; q++) {
+            objs[q].meshAtProgress(frame_progress, frame_dt, pos[q], vel[q]);
+            ff::ObstacleMesh m{pos[q].data(), nv[q], objs[q].triangles.data(), nt[q], objs[q].inversed,
+                               objs[q].meshExpansion};
+            m.velocities = vel[q].data();
+            m.animated = objs[q].animated;
+            meshes.push_back(m);
+        }
+        DBuf<float> d(nn), us(nf[0]), vs(nf[1]), ws(nf[2]);
+        DBuf<char> block(ff::solidVelocityBytes(g));
+        const size_t big = std::max(nf[0], std::max(nf[1], nf[2]));
+        DBuf<uint8_t> valid(big), status(big);
+        ff::SolidVelocityData vd = ff::solidVelocityCarve(g, block.p);
+        ff::solidBoxPhi(g, d, sc.s);
+        int draws = ff::meshObstacleSolid(g, meshes.data(), nobj, d, nullptr, false, sc.s, &vd);
+        ff::normalizeSolidVelocity(g, vd, us, vs, ws, valid, status, sc.s);
+        down(phi, d.p, nn, sc.s);
+        down(u, us.p, nf[0], sc.s);
+        down(v, vs.p, nf[1], sc.s);
+        down(w, ws.p, nf[2], sc.s);
+        sync(sc.s);
+        return draws;
+    });
+}
+
+void FFAMD_cell_weights(int ni, int nj, int nk, const float *phi, float *wc, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, 1.0);
+        Scratch sc;
+        const size_t nn = (size_t)(ni + 1) * (nj + 1) * (nk + 1), N = (size_t)ni * nj * nk;
+        DBuf<float> d(nn), c(N);
+        up(d.p, phi, nn, sc.s);
+        ff::cellWeights(ff::Grid{ni, nj, nk, 1.0}, d, c, sc.s);
+        down(wc, c.p, N, sc.s);
+        sync(sc.s);
+    });
+}
+
+void FFAMD_condition_solid_velocity(int ni, int nj, int nk, const float *phi, const float *wu, const float *wv,
+                                    const float *ww, float *us, float *vs, float *ws, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, 1.0);
+        Scratch sc;
+        const size_t N = (size_t)ni * nj * nk;
+        const size_t nf[3] = {(size_t)(ni + 1) * nj * nk, (size_t)ni * (nj + 1) * nk, (size_t)ni * nj * (nk + 1)};
+        DBuf<float> p(N), a(nf[0]), b(nf[1]), c(nf[2]), x(nf[0]), y(nf[1]), z(nf[2]);
+        DBuf<int> scratch(2 * N);
+        up(p.p, phi, N, sc.s);
+        up(a.p, wu, nf[0], sc.s);
+        up(b.p, wv, nf[1], sc.s);
+        up(c.p, ww, nf[2], sc.s);
+        up(x.p, us, nf[0], sc.s);
+        up(y.p, vs, nf[1], sc.s);
+        up(z.p, ws, nf[2], sc.s);
+        ff::conditionSolidVelocity(ff::Grid{ni, nj, nk, 1.0}, p, a, b, c, x, y, z, scratch, sc.s);
+        down(us, x.p, nf[0], sc.s);
+        down(vs, y.p, nf[1], sc.s);
+        down(ws, z.p, nf[2], sc.s);
+        sync(sc.s);
     });
 }
 

@@ -48,6 +48,19 @@ public:
     MeshObstacle(int i, int j, int k, double dx) : isize(i), jsize(j), ksize(k), dx(dx) {}
     // MeshObject::updateMeshStatic (meshobject.cpp:50-59)
     void updateMeshStatic(int nv, const float *vertices, int nt, const int *triangles);
+    // MeshObject::updateMeshAnimated (meshobject.cpp:61-95): the previous,
+    // current and next frame's meshes; the triangles are the current mesh's.
+    // The three must have the same vertex and triangle counts (a mesh whose
+    // topology changes between frames is rejected).
+    void updateMeshAnimated(int nv, const float *previous, const float *current, const float *next, int nt,
+                            const int *triangles);
+    // MeshObject::_isMeshChanged (meshobject.cpp:893-912): animated and some
+    // vertex moved more than 1e-5 from the previous frame's
+    bool isMeshChanged() const;
+    // the mesh at frame progress f and its vertex velocities for a frame of
+    // length frameDt (MeshObject::getMesh(f), getVertexVelocities(dt, f),
+    // meshobject.cpp:158-220): positions and velocities, 3 floats per vertex
+    void meshAtProgress(float f, double frameDt, std::vector<float> &positions, std::vector<float> &velocities) const;
     void enable() {
         if (!enabled) stateChanged = true;
         enabled = true;
@@ -63,8 +76,10 @@ public:
 
     int isize, jsize, ksize;
     double dx;
-    std::vector<float> vertices;
+    std::vector<float> vertices;  // the current frame's
     std::vector<int> triangles;
+    std::vector<float> previousVertices, nextVertices;  // animated objects only
+    bool animated = false;
     bool enabled = true, inversed = false, stateChanged = false, appendObjectVelocity = false;
     float friction = 0.0f, velocityScale = 1.0f, whitewaterInfluence = 1.0f, dustEmissionStrength = 1.0f,
           sheetingStrength = 1.0f, meshExpansion = 0.0f, objectVelocityInfluence = 1.0f;
@@ -173,6 +188,8 @@ public:
     // grid getters for tests and tools (host copies)
     void getLiquidSDF(float *out);
     void getSolidSDF(float *out);
+    // the solid's face velocities of the last rebuild (zero for a static solid)
+    void getSolidVelocityField(float *u, float *v, float *w);
     void getVelocityField(float *u, float *v, float *w);
     void getPressureGrid(float *out);
     void getLastPressureSolveStats(int *iterations, double *error, int *success, int *unknowns);
@@ -227,7 +244,14 @@ private:
     double _boundaryFriction = 0.0;
     void refreshFriction();
     int _solidDraws = 0;
-    void updateSolidLevelSet();
+    void updateSolidLevelSet(double dt);
+    // moving obstacles: the solid's velocity data and face velocities, the
+    // cells' solid weights, and the scratch of their rebuild and conditioning
+    bool _movingSolid = false;  // the last rebuild had an animated obstacle
+    DevBuf<char> _solidVelBlock;
+    DevBuf<float> _us, _vs, _ws, _wc;
+    DevBuf<uint8_t> _svValid, _svStatus;
+    DevBuf<int> _condScratch;
 
     std::vector<std::vector<float>> _loadPos, _loadVel, _loadAff;  // _loadAff: 9 floats per particle
 

@@ -148,6 +148,52 @@ void MeshObstacle::updateMeshStatic(int nv, const float *v, int nt, const int *t
     triangles.assign(t, t + 3 * (size_t)nt);
 }
 
+void MeshObstacle::updateMeshAnimated(int nv, const float *prev, const float *cur, const float *next, int nt,
+                                      const int *t) {
+    if (nv < 0 || nt < 0 || (nv > 0 && (!prev || !cur || !next)) || (nt > 0 && !t))
+        throw std::invalid_argument("Error: invalid triangle mesh data.\n");
+    for (int q = 0; q < 3 * nt; q++)
+        if (t[q] < 0 || t[q] >= nv) throw std::out_of_range("Error: triangle vertex index out of range.\n");
+    previousVertices.assign(prev, prev + 3 * (size_t)nv);
+    vertices.assign(cur, cur + 3 * (size_t)nv);
+    nextVertices.assign(next, next + 3 * (size_t)nv);
+    triangles.assign(t, t + 3 * (size_t)nt);
+    animated = true;
+}
+
+bool MeshObstacle::isMeshChanged() const {
+    if (!animated) return false;
+    if (previousVertices.size() != vertices.size()) return true;
+    const float eps = 1e-5f;
+    for (size_t q = 0; q < vertices.size(); q += 3) {
+        // vmath::length of the float difference (vmath.h): sqrt of the float dot
+        const float dx = previousVertices[q] - vertices[q], dy = previousVertices[q + 1] - vertices[q + 1],
+                    dz = previousVertices[q + 2] - vertices[q + 2];
+        if (std::sqrt(dx * dx + dy * dy + dz * dz) > eps) return true;
+    }
+    return false;
+}
+
+void MeshObstacle::meshAtProgress(float f, double frameDt, std::vector<float> &pos, std::vector<float> &vel) const {
+    f = std::fmax(0.0f, f);
+    f = std::fmin(1.0f, f);
+    const size_t n = vertices.size();
+    pos.resize(n);
+    vel.assign(n, 0.0f);
+    if (!animated) {  // v + f (v - v): -0 becomes +0, as in getMesh(f)
+        for (size_t q = 0; q < n; q++) pos[q] = vertices[q] + f * (vertices[q] - vertices[q]);
+        return;
+    }
+    for (size_t q = 0; q < n; q++) pos[q] = vertices[q] + f * (nextVertices[q] - vertices[q]);
+    if (frameDt < 1e-10) return;
+    const float invdt = (float)(1.0 / frameDt);
+    for (size_t q = 0; q < n; q++) {
+        const float p1 = vertices[q] - previousVertices[q], p2 = nextVertices[q] - vertices[q];
+        const float t = p1 + f * (p2 - p1);
+        vel[q] = velocityScale * (invdt * t);
+    }
+}
+
 void FluidSimulation::addMeshObstacle(MeshObstacle *obstacle) {
     if (!obstacle) throw std::invalid_argument("Error: null mesh obstacle.\n");
     for (MeshObstacle *o : _obstacles)
@@ -171,25 +217,42 @@ void FluidSimulation::removeMeshObstacles() {
     _solidUpToDate = false;
 }
 
-// _updateSolidLevelSet + _addStaticObjectsToSDF + _updateNearSolidGrid
-// (fluidsimulation.cpp:5387-5426, 5277-5339, 5448-5480), run at the start of
-// a substep while particles exist, as the reference's obstacle update is
-// (5550-5570): the objects' state flags are read and cleared, and a stale
-// solid is rebuilt from the domain SDF (or the one set) and the enabled
-// obstacles, with its face weights and near-solid grid.
-void FluidSimulation::updateSolidLevelSet() {
+// _updateSolidLevelSet + _addStaticObjectsToSDF + _addAnimatedObjectsToSolidSDF
+// + _updateNearSolidGrid (fluidsimulation.cpp:5219-5426, 5448-5480), run at the
+// start of a substep of length dt while particles exist, as the reference's
+// obstacle update is (5550-5570): the objects' state flags are read and
+// cleared, and a stale solid -- or one with an enabled animated obstacle whose
+// mesh changed -- is rebuilt from the domain SDF (or the one set) and the
+// enabled obstacles at the substep's frame progress, with its face weights and
+// near-solid grid.  With an animated obstacle the rebuild also makes the
+// solid's face velocities and the cells' solid weights (6066-6148).
+void FluidSimulation::updateSolidLevelSet(double dt) {
+    bool changed = false;
     for (MeshObstacle *o : _obstacles) {
-        if (o->stateChanged) _solidUpToDate = false;
+        if (o->stateChanged || (o->enabled && o->isMeshChanged())) changed = true;
         o->stateChanged = false;
     }
+    if (changed) _solidUpToDate = false;
     if (_solidUpToDate) return;
-    std::vector<ObstacleMesh> meshes;
     _solidObjects.clear();
+    bool anyAnimated = false;
     for (MeshObstacle *o : _obstacles) {
         if (!o->enabled) continue;
         _solidObjects.push_back(o);
-        meshes.push_back(ObstacleMesh{o->vertices.data(), (int)(o->vertices.size() / 3), o->triangles.data(),
-                                      (int)(o->triangles.size() / 3), o->inversed, o->meshExpansion});
+        anyAnimated |= o->animated;
+    }
+    // the substep's progress through the frame (fluidsimulation.cpp:5242-5243)
+    const float progress = 1.0f - (float)(_remaining + dt) / (float)_frameDt;
+    std::vector<std::vector<float>> pos(_solidObjects.size()), vel(_solidObjects.size());
+    std::vector<ObstacleMesh> meshes;
+    for (size_t q = 0; q < _solidObjects.size(); q++) {
+        const MeshObstacle *o = _solidObjects[q];
+        o->meshAtProgress(progress, _frameDt, pos[q], vel[q]);
+        ObstacleMesh m{pos[q].data(), (int)(pos[q].size() / 3), o->triangles.data(), (int)(o->triangles.size() / 3),
+                       o->inversed, o->meshExpansion};
+        m.velocities = vel[q].data();
+        m.animated = o->animated;
+        meshes.push_back(m);
     }
     if (_nodeObj.n == 0) _nodeObj.alloc((size_t)(_g.ni + 1) * (_g.nj + 1) * (_g.nk + 1));
     if (_solidGiven)
@@ -198,8 +261,33 @@ void FluidSimulation::updateSolidLevelSet() {
                  "upload solid SDF");
     else
         solidBoxPhi(_g, _solid, _stream);
+    SolidVelocityData vd{};
+    if (anyAnimated) {
+        const size_t N = (size_t)_g.ni * _g.nj * _g.nk;
+        if (_solidVelBlock.n == 0) {
+            _solidVelBlock.alloc(solidVelocityBytes(_g));
+            _us.alloc(_u.n);
+            _vs.alloc(_v.n);
+            _ws.alloc(_w.n);
+            _wc.alloc(N);
+            const size_t nf = std::max(_u.n, std::max(_v.n, _w.n));
+            _svValid.alloc(nf);
+            _svStatus.alloc(nf);
+            _condScratch.alloc(2 * N);
+        }
+        vd = solidVelocityCarve(_g, _solidVelBlock.p);
+    }
     // the domain object holds the domain SDF's band unless a solid SDF was set
-    _solidDraws = meshObstacleSolid(_g, meshes.data(), (int)meshes.size(), _solid, _nodeObj.p, !_solidGiven, _stream);
+    _solidDraws = meshObstacleSolid(_g, meshes.data(), (int)meshes.size(), _solid, _nodeObj.p, !_solidGiven, _stream,
+                                    anyAnimated ? &vd : nullptr);
+    _movingSolid = anyAnimated;
+    if (anyAnimated) {
+        normalizeSolidVelocity(_g, vd, _us, _vs, _ws, _svValid, _svStatus, _stream);
+        cellWeights(_g, _solid, _wc, _stream);
+        _ps.setSolidTerms(SolidTerms{_us.p, _vs.p, _ws.p, _wc.p});
+    } else {
+        _ps.setSolidTerms(SolidTerms{});
+    }
     _objFricHost.clear();  // re-upload the table for the new object numbering
     faceWeights(_g, _solid, _wu, _wv, _ww, _stream);
     int layers = (int)std::ceil((float)_cfl / (float)3);
@@ -515,6 +603,7 @@ void FluidSimulation::step(double dt) {
     requireInitialized("step");
     if (!(dt > 0.0)) throw std::domain_error("Error: substep length must be greater than 0.\n");
     _frameDt = dt;
+    _remaining = 0.0;  // one substep is the whole frame: progress 0
     stepFluid(dt);
     _stepNumber++;
 }
@@ -539,7 +628,7 @@ void FluidSimulation::stepFluid(double dt) {
     const double radius = 0.5 * 1.0 * _g.dx * std::sqrt(3.0);  // fluidsimulation.cpp:4351
     const int layers = (int)std::ceil(std::sqrt(3.0) * _cfl) + 3;  // :6282-6286
     if (_P.n == 0) return;
-    updateSolidLevelSet();
+    updateSolidLevelSet(dt);
     refreshFriction();
     hipCheck(hipEventRecord(_ev[0], _stream), "event");
     // liquid level set (5586-5617)
@@ -592,20 +681,23 @@ void FluidSimulation::stepFluid(double dt) {
         _pErr = (float)_lastErr;
     }
     extrapolate(layers);
-    // constrain saved then current (6444-6459); with obstacle friction the
-    // partly open faces blend toward the (zero) solid velocity
+    // constrain saved then current (6444-6459): closed faces take the solid's
+    // face velocity (zero for a static solid); with obstacle friction the
+    // partly open faces blend toward it
+    const float *svel[3] = {_movingSolid ? _us.p : nullptr, _movingSolid ? _vs.p : nullptr,
+                            _movingSolid ? _ws.p : nullptr};
     if (_frictionActive) {
         float *faces[6] = {_su, _sv, _sw, _u, _v, _w};
         const float *wts[3] = {_wu, _wv, _ww};
         for (int q = 0; q < 6; q++)
-            constrainFacesFriction(_g, q % 3, faces[q], wts[q % 3], _nodeObj, _objFric, _stream);
+            constrainFacesFriction(_g, q % 3, faces[q], wts[q % 3], _nodeObj, _objFric, _stream, svel[q % 3]);
     } else {
-        constrainFaces(_su, _wu, nu, _stream);
-        constrainFaces(_sv, _wv, nv, _stream);
-        constrainFaces(_sw, _ww, nw, _stream);
-        constrainFaces(_u, _wu, nu, _stream);
-        constrainFaces(_v, _wv, nv, _stream);
-        constrainFaces(_w, _ww, nw, _stream);
+        constrainFaces(_su, _wu, nu, _stream, svel[0]);
+        constrainFaces(_sv, _wv, nv, _stream, svel[1]);
+        constrainFaces(_sw, _ww, nw, _stream, svel[2]);
+        constrainFaces(_u, _wu, nu, _stream, svel[0]);
+        constrainFaces(_v, _wv, nv, _stream, svel[1]);
+        constrainFaces(_w, _ww, nw, _stream, svel[2]);
     }
     hipCheck(hipEventRecord(_ev[5], _stream), "event");
     // G2P (6932-6945) + RK3 advection and collision (7853-7890)
@@ -637,6 +729,10 @@ void FluidSimulation::stepFluid(double dt) {
 
 // PressureSolver::solve + applySolutionToVelocityField gate (fluidsimulation.cpp:6206-6249)
 bool FluidSimulation::pressureSolve(double dt) {
+    // PressureSolver::solve conditions the moving solid's face velocities
+    // first (pressuresolver.cpp:52-56); with a static solid they are all 0
+    if (_movingSolid)
+        conditionSolidVelocity(_g, _phi, _wu, _wv, _ww, _us, _vs, _ws, _condScratch, _stream);
     PressureResult r = _ps.solve(dt, _u, _v, _w, _wu, _wv, _ww, _phi, _pressureTolerance,
                                  _maxPressureIterations, (int)_precision, _innerTolerance, _pressure,
                                  _scan, _stream, _ev[3], &_timers, _referenceSummation ? _maxThreads : 0);
@@ -743,6 +839,20 @@ void FluidSimulation::getLiquidSDF(float *out) {
     hipCheck(hipMemcpyAsync(out, _phi, sizeof(float) * _phi.n, hipMemcpyDeviceToHost, _stream), "D2H");
     hipCheck(hipStreamSynchronize(_stream), "sync");
 }
+void FluidSimulation::getSolidVelocityField(float *u, float *v, float *w) {
+    requireInitialized("getSolidVelocityField");
+    float *out[3] = {u, v, w};
+    const float *src[3] = {_us.p, _vs.p, _ws.p};
+    const size_t n[3] = {_u.n, _v.n, _w.n};
+    for (int d = 0; d < 3; d++) {
+        if (_movingSolid)
+            hipCheck(hipMemcpyAsync(out[d], src[d], sizeof(float) * n[d], hipMemcpyDeviceToHost, _stream), "D2H");
+        else
+            std::fill(out[d], out[d] + n[d], 0.0f);
+    }
+    hipCheck(hipStreamSynchronize(_stream), "solid velocity");
+}
+
 void FluidSimulation::getSolidSDF(float *out) {
     requireInitialized("reading grids");
     hipCheck(hipMemcpyAsync(out, _solid, sizeof(float) * _solid.n, hipMemcpyDeviceToHost, _stream), "D2H");

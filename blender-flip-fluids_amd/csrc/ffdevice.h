@@ -171,6 +171,67 @@ FF_HD V3 closestPointOnTriangle(V3 p0, V3 v0, V3 v1, V3 v2) {
 }
 
 // FluidSimulation::_getBoundaryAABB (fluidsimulation.cpp:5175-5180)
+// LevelsetUtils::fractionInside (levelsetutils.cpp:39-141): the fraction of
+// a segment, or of a square with the four corner values, below zero
+FF_HD float fraction2(float l, float r) {
+    if (l < 0 && r < 0) return 1;
+    if (l < 0 && r >= 0) return l / (l - r);
+    if (l >= 0 && r < 0) return r / (r - l);
+    return 0;
+}
+FF_HD void cycle4(float *a) {
+    float t = a[0];
+    a[0] = a[1];
+    a[1] = a[2];
+    a[2] = a[3];
+    a[3] = t;
+}
+FF_HD float fraction4(float bl, float br, float tl, float tr) {
+    int in = (bl < 0 ? 1 : 0) + (tl < 0 ? 1 : 0) + (br < 0 ? 1 : 0) + (tr < 0 ? 1 : 0);
+    float L[4] = {bl, br, tr, tl};
+    if (in == 4) return 1;
+    if (in == 3) {
+        while (L[0] < 0) cycle4(L);
+        float s0 = 1 - fraction2(L[0], L[3]);
+        float s1 = 1 - fraction2(L[0], L[1]);
+        return 1.0f - 0.5f * s0 * s1;
+    }
+    if (in == 2) {
+        while (L[0] >= 0 || !(L[1] < 0 || L[2] < 0)) cycle4(L);
+        if (L[1] < 0) {
+            float sl = fraction2(L[0], L[3]);
+            float sr = fraction2(L[1], L[2]);
+            return 0.5f * (sl + sr);
+        }
+        float mid = 0.25f * (L[0] + L[1] + L[2] + L[3]);
+        if (mid < 0) {
+            float area = 0;
+            float s1 = 1 - fraction2(L[0], L[3]);
+            float s3 = 1 - fraction2(L[2], L[3]);
+            area += 0.5f * s1 * s3;
+            float s2 = 1 - fraction2(L[2], L[1]);
+            float s0 = 1 - fraction2(L[0], L[1]);
+            area += 0.5f * s0 * s2;
+            return 1.0f - area;
+        }
+        float area = 0;
+        float s0 = fraction2(L[0], L[1]);
+        float s1 = fraction2(L[0], L[3]);
+        area += 0.5f * s0 * s1;
+        float s2 = fraction2(L[2], L[1]);
+        float s3 = fraction2(L[2], L[3]);
+        area += 0.5f * s2 * s3;
+        return area;
+    }
+    if (in == 1) {
+        while (L[0] >= 0) cycle4(L);
+        float s0 = fraction2(L[0], L[3]);
+        float s1 = fraction2(L[0], L[1]);
+        return 0.5f * s0 * s1;
+    }
+    return 0;
+}
+
 FF_HD Box boundaryBox(int ni, int nj, int nk, double dx) {
     Box b{V3{0.0f, 0.0f, 0.0f}, ni * dx, nj * dx, nk * dx};
     b.expand(-3 * dx - 1e-4);

@@ -26,7 +26,24 @@ struct ObstacleMesh {
     int numTriangles;
     bool inversed;
     float meshExpansion;  // used by the 25-or-more-island path only (the reference ignores it otherwise)
+    const float *velocities = nullptr;  // per vertex (3 floats), or null: at rest
+    bool animated = false;  // animated objects are unioned after the static ones
 };
+// The solid's velocity data (MeshLevelSet velocity grids, meshlevelset.cpp:
+// 1320-1425, 603-700): per face the weighted velocity f and the weight w
+// summed over the domain and the obstacles, with the domain-sized scratch of
+// one rebuild (an obstacle's level set and its faces; the union of the
+// inversed obstacles).  All device memory, carved from one block of
+// solidVelocityBytes(g) bytes by solidVelocityCarve.
+struct SolidVelocityData {
+    float *f[3], *w[3];    // accumulated over the solid (u, v, w faces)
+    float *tf[3], *tw[3];  // one obstacle's, or the inversed union's
+    float *tphi, *iphi;    // nodal: one obstacle's level set; the inversed union
+    int *tobj, *iobj;      // nodal: their closest mesh objects
+    int *ttri, *itri;      // nodal: their closest triangles
+};
+size_t solidVelocityBytes(Grid g);
+SolidVelocityData solidVelocityCarve(Grid g, void *block);
 // Unions the obstacles' signed distance level sets into the nodal solid SDF
 // `solid` (device, (ni+1)(nj+1)(nk+1)), which holds the domain SDF: normal
 // obstacles in order, then the negated union of the inversed ones.  Takes the
@@ -35,8 +52,34 @@ struct ObstacleMesh {
 // nodeObject (optional, device, nodal): each node's closest mesh object
 // (MeshLevelSet::getClosestMeshObject): -1 none, 0 the domain object (on
 // the domain SDF's exact band when domainObject), q + 1 obstacle q.
+// velocity (optional): the solid update with velocity data
+// (FluidSimulation::_updateSolidLevelSet, fluidsimulation.cpp:5219-5426): the
+// domain's weights, then the static obstacles and then the animated ones,
+// each group's normal obstacles one by one and then the negated union of its
+// inversed ones, every obstacle built on its own domain-sized level set
+// (meshes and vertex velocities as given, at the substep's frame progress);
+// the velocity data is left in velocity->f / w.
 int meshObstacleSolid(Grid g, const ObstacleMesh *obstacles, int n, float *solid, int *nodeObject, bool domainObject,
-                      hipStream_t s);
+                      hipStream_t s, SolidVelocityData *velocity = nullptr);
+// normalizeVelocityGrid (meshlevelset.cpp:640-700, 1739-1757): velocity =
+// f / w where w exceeds 1e-6, else 0, then 5 extrapolation layers from those
+// faces; writes us, vs, ws (device face arrays).  valid and status: scratch of
+// the largest face array's size.
+void normalizeSolidVelocity(Grid g, const SolidVelocityData &d, float *us, float *vs, float *ws, uint8_t *valid,
+                            uint8_t *status, hipStream_t s);
+// The weight grid's cell weights (fluidsimulation.cpp:6140-6147,
+// MeshLevelSet::_getCellWeight meshlevelset.cpp:1491-1515): 1 - the solid's
+// volume fraction in each cell, clamped to [0, 1] (device, ni nj nk).
+void cellWeights(Grid g, const float *solid, float *wc, hipStream_t s);
+// PressureSolver::_conditionSolidVelocityField (pressuresolver.cpp:161-312):
+// the interior liquid cells (phi < 0) connected through faces of weight
+// >= 1e-6, never through a border cell, form groups; in a group of more than
+// one cell none of which has such a face to an air cell the solid velocities
+// of all its cells' faces become 0 (us, vs, ws in place).  scratch: 2 ni nj nk
+// ints.
+void conditionSolidVelocity(Grid g, const float *phi, const float *wu, const float *wv, const float *ww, float *us,
+                            float *vs, float *ws, int *scratch, hipStream_t s);
+
 // One mesh's fast signed distance level set on `cells` cells at node offset
 // `offset` (device phi, (ni+1)(nj+1)(nk+1)); returns the rand() draws.
 int meshLevelSetFastDevice(Grid cells, I3 offset, const ObstacleMesh &mesh, float *phi, hipStream_t s);
@@ -62,10 +105,43 @@ int referenceDotChunks(size_t n, int threads);
 void dotReference(const double *a, const double *b, size_t n, int threads, double *partial, double *out,
                   hipStream_t s);
 void absMaxf(const float *a, size_t n, double *partial, double *out, hipStream_t s);
-// The reductions above run reduceBlocks(n) blocks of REDUCE_THREADS threads
-// over a grid-stride loop, then one block folds the partials in a fixed tree;
-// fused kernels that use the same grid produce the same bits.
+// The reductions above run reduceBlocks(n) blocks of REDUCE_THREADS threads.
+// Logical block L takes the contiguous span of rows [L*sp, (L+1)*sp),
+// sp = reduceSpan(n, blocks), thread t its rows L*sp + t + q*REDUCE_THREADS in
+// order; the block folds its threads' values (reduceFold), and one block folds
+// the partials in order.  Fused kernels that use the same blocks and spans
+// produce the same bits.
 constexpr int REDUCE_THREADS = 256;
+constexpr int XCDS = 8;  // MI355X: 8 accelerator dies, workgroup b runs on die b % 8
+__host__ __device__ inline size_t reduceSpan(size_t n, unsigned blocks) {
+    const size_t per = (n + blocks - 1) / blocks;
+    return (per + REDUCE_THREADS - 1) / REDUCE_THREADS * REDUCE_THREADS;
+}
+// the logical block of this workgroup: blocks are renumbered so that each die
+// runs one contiguous run of spans (blocks is a multiple of XCDS or below it),
+// and the rows next to a block's rows sit in the same die's L2
+__device__ inline unsigned reduceLogicalBlock() {
+    const unsigned b = blockIdx.x, g = gridDim.x;
+    return g % XCDS == 0 ? (b % XCDS) * (g / XCDS) + b / XCDS : b;
+}
+// fold one double per thread of a REDUCE_THREADS block, OP 0 sum, 1 max, in a
+// fixed order: a wave64 butterfly (every lane ends with the same bits), then
+// the four wave results in wave order.  The result is valid in thread 0.
+template <int OP>
+__device__ inline double reduceFold(double v) {
+    __shared__ double waves[REDUCE_THREADS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = OP == 0 ? v + o : fmax(v, o);
+    }
+    if ((threadIdx.x & 63) == 0) waves[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double acc = waves[0];
+#pragma unroll
+    for (int q = 1; q < REDUCE_THREADS / 64; q++) acc = OP == 0 ? acc + waves[q] : fmax(acc, waves[q]);
+    return acc;
+}
 // outHost (optional): device alias of mapped pinned host memory that also
 // receives the result
 void reduceFinish(const double *partial, int nb, double *out, bool isMax, hipStream_t s, double *outHost = nullptr);
@@ -189,12 +265,15 @@ void extrapolateLayer(int wi, int wj, int wk, float *field, uint8_t *stat, int l
 // `layers` layers left), in place on the face cotangent bar
 void extrapolateAdjoint(int wi, int wj, int wk, float *bar, const uint8_t *stat, int layers, hipStream_t s);
 void addToFaces(float *a, size_t n, float inc, hipStream_t s);
-void constrainFaces(float *a, const float *w, size_t n, hipStream_t s);
+// _constrainVelocityFieldThread (fluidsimulation.cpp:6391-6442) with friction
+// 0: closed faces (weight 0) take the solid's face velocity sv (device, the
+// face array's size; null: a static solid, velocity 0)
+void constrainFaces(float *a, const float *w, size_t n, hipStream_t s, const float *sv = nullptr);
 // the same with friction: f = the mean of the face's four nodes' closest
 // objects' friction (fluidsimulation.cpp:6292-6353), objFriction[object]
-// (none: 0); dir 0/1/2 = u/v/w
+// (none: 0); partly open faces become f sv + (1 - f) a; dir 0/1/2 = u/v/w
 void constrainFacesFriction(Grid g, int dir, float *a, const float *w, const int *nodeObject, const float *objFriction,
-                            hipStream_t s);
+                            hipStream_t s, const float *sv = nullptr);
 void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,
                    const float *wv, const float *ww, float *u, float *v, float *w, uint8_t *vu,
                    uint8_t *vv, uint8_t *vw, hipStream_t s);
@@ -226,14 +305,24 @@ struct PressureSys {
     int *levelStart;  // L+1
     int *levelFill;   // L
     int *levelRows;   // M: rows grouped by i+j+k
+    // row bits of each 32-cell word of every (j, k) line along i, and the row
+    // of the word's first liquid cell (nj*nk lines x nw words; SpMV and the
+    // line-pipeline sweeps find neighbour rows in them)
+    LineWord *words;
+    int nw, ni, nj;
     int m;
     int nLevels;
 };
 void pressureMask(Grid g, const float *phi, int *mask, hipStream_t s);
 void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, int *cellRow, hipStream_t s);
+// the moving solid's terms of the divergence: face velocities and cell weights
+// (all null for a static solid)
+struct SolidTerms {
+    const float *us = nullptr, *vs = nullptr, *ws = nullptr, *wc = nullptr;
+};
 void pressureAssemble(Grid g, double dt, const float *u, const float *v, const float *w,
                       const float *wu, const float *wv, const float *ww, const float *phi,
-                      PressureSys &S, hipStream_t s);
+                      PressureSys &S, hipStream_t s, SolidTerms solid = SolidTerms{});
 void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s);
 // MIC(0) factor and application z = (L L^T)^-1 r; hyperplane-by-hyperplane
 // launches driven by the host copy of the level table

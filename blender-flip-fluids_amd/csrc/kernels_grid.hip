@@ -106,10 +106,10 @@ __global__ void addKernel(float *a, size_t n, float inc) {
 
 // _constrainVelocityFieldThread (fluidsimulation.cpp:6388-6442): domain
 // friction 0 and zero face velocity of static solids
-__global__ void constrainKernel(float *a, const float *w, size_t n) {
+__global__ void constrainKernel(float *a, const float *w, size_t n, const float *sv) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const float f = 0.0f, sface = 0.0f;
+    const float f = 0.0f, sface = sv ? sv[q] : 0.0f;
     float wt = w[q];
     if (wt == 0)
         a[q] = sface;
@@ -120,14 +120,15 @@ __global__ void constrainKernel(float *a, const float *w, size_t n) {
 // _constrainVelocityFieldThread with friction (fluidsimulation.cpp:6292-6442):
 // on a partly open face f is 0.25 x the sum, in the reference's order, of the
 // friction of the closest mesh object at the face's four nodes (read live
-// from the objects' table, none counting 0); static solids have face
-// velocity 0.
+// from the objects' table, none counting 0); the solid's face velocity sv, or
+// 0 for static solids (sv null).
 template <int DIR>
-__global__ void constrainFrictionKernel(Grid g, float *a, const float *w, const int *obj, const float *fr) {
+__global__ void constrainFrictionKernel(Grid g, float *a, const float *w, const int *obj, const float *fr,
+                                        const float *sv) {
     const int wi = g.ni + (DIR == 0), wj = g.nj + (DIR == 1), wk = g.nk + (DIR == 2);
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (size_t)wi * wj * wk) return;
-    const float sface = 0.0f;
+    const float sface = sv ? sv[q] : 0.0f;
     const float wt = w[q];
     if (wt == 0) {
         a[q] = sface;
@@ -228,19 +229,22 @@ void addToFaces(float *a, size_t n, float inc, hipStream_t s) {
     hipLaunchKernelGGL(addKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, a, n, inc);
 }
 
-void constrainFaces(float *a, const float *w, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(constrainKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, a, w, n);
+void constrainFaces(float *a, const float *w, size_t n, hipStream_t s, const float *sv) {
+    hipLaunchKernelGGL(constrainKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, a, w, n, sv);
 }
 
 void constrainFacesFriction(Grid g, int dir, float *a, const float *w, const int *obj, const float *fr,
-                            hipStream_t s) {
+                            hipStream_t s, const float *sv) {
     const size_t n = (size_t)(g.ni + (dir == 0)) * (g.nj + (dir == 1)) * (g.nk + (dir == 2));
     if (dir == 0)
-        hipLaunchKernelGGL(constrainFrictionKernel<0>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr);
+        hipLaunchKernelGGL(constrainFrictionKernel<0>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr,
+                           sv);
     else if (dir == 1)
-        hipLaunchKernelGGL(constrainFrictionKernel<1>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr);
+        hipLaunchKernelGGL(constrainFrictionKernel<1>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr,
+                           sv);
     else
-        hipLaunchKernelGGL(constrainFrictionKernel<2>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr);
+        hipLaunchKernelGGL(constrainFrictionKernel<2>, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, a, w, obj, fr,
+                           sv);
 }
 
 void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,

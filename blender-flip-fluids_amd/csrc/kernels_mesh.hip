@@ -22,10 +22,17 @@
 // the C library's rand(), three draws per test, as in the reference
 // (meshutils.cpp:93-117): a program that seeds rand() identically gets the
 // same jitter from both engines.
+//
+// With velocity data (moving obstacles) every level set also keeps each
+// node's closest triangle and, per face, the weight (the fraction of the face
+// inside the level set) times the velocity of the nearest point of the mesh,
+// interpolated from the vertex velocities (meshlevelset.cpp:168-205,
+// 1320-1425, 1568-1717), and the solid sums these over its parts.
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -46,6 +53,7 @@ constexpr int LS_BLOCK = 10;   // MeshLevelSet::_blockwidth (meshlevelset.h:369)
 struct BandTri {
     V3 a, b, c;
     int lo[3], hi[3];
+    int id;  // the triangle's index in its mesh
 };
 
 // BLOCK: distances in 10^3-node block-local coordinates, the multi-threaded
@@ -74,6 +82,49 @@ __global__ void meshBandKernel(const BandTri *tris, int pi, int pj, double dx, i
         const float dist = len3(sub(closestPointOnTriangle(gpos, a, b, c), gpos));
         atomicMin(phiBits + flat(i, j, k, pi, pj), __float_as_int(dist));
     }
+}
+
+// the band with each node's closest triangle: a 64-bit atomic min over
+// (distance bits, triangle index), so the smallest distance wins and, among
+// equal distances, the lowest triangle index, as the reference's ascending
+// triangle loop with a strict comparison keeps the first
+template <bool BLOCK>
+__global__ void meshBandTriKernel(const BandTri *tris, int pi, int pj, double dx, unsigned long long *keys) {
+    const BandTri t = tris[blockIdx.x];
+    const int ni = t.hi[0] - t.lo[0] + 1, nj = t.hi[1] - t.lo[1] + 1, nk = t.hi[2] - t.lo[2] + 1;
+    const long long count = (long long)ni * nj * nk;
+    for (long long q = threadIdx.x; q < count; q += blockDim.x) {
+        const int i = t.lo[0] + (int)(q % ni);
+        const int j = t.lo[1] + (int)((q / ni) % nj);
+        const int k = t.lo[2] + (int)(q / ((long long)ni * nj));
+        V3 gpos, a = t.a, b = t.b, c = t.c;
+        if (BLOCK) {
+            const int bi = i / LS_BLOCK, bj = j / LS_BLOCK, bk = k / LS_BLOCK;
+            const V3 off = indexToPos(bi, bj, bk, LS_BLOCK * dx);
+            gpos = indexToPos(i - bi * LS_BLOCK, j - bj * LS_BLOCK, k - bk * LS_BLOCK, dx);
+            a = sub(a, off);
+            b = sub(b, off);
+            c = sub(c, off);
+        } else {
+            gpos = indexToPos(i, j, k, dx);
+        }
+        const float dist = len3(sub(closestPointOnTriangle(gpos, a, b, c), gpos));
+        const unsigned long long key = ((unsigned long long)__float_as_uint(dist) << 32) | (unsigned)t.id;
+        atomicMin(keys + flat(i, j, k, pi, pj), key);
+    }
+}
+__global__ void fillKeysKernel(unsigned long long *keys, size_t n, unsigned long long v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = v;
+}
+// keys -> phi (the distance) and the closest triangle (-1: none)
+__global__ void splitKeysKernel(const unsigned long long *keys, size_t n, float *phi, int *tri) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long key = keys[i];
+    phi[i] = __uint_as_float((unsigned)(key >> 32));
+    const unsigned t = (unsigned)(key & 0xffffffffull);
+    tri[i] = t == 0xffffffffu ? -1 : (int)t;
 }
 
 // one triangle of a sign test: vertices in the shifted frame, inclusive column range
@@ -161,6 +212,11 @@ __global__ void fillFloatKernel(float *a, size_t n, float v) {
     if (i < n) a[i] = v;
 }
 
+__global__ void fillIntKernel(int *a, size_t n, int v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ void negateInsideKernel(float *phi, const uint8_t *inside, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && inside[i]) phi[i] = -phi[i];
@@ -172,8 +228,11 @@ __global__ void negateInsideKernel(float *phi, const uint8_t *inside, size_t n) 
 // with a smaller magnitude and has a closest triangle there.  The source's
 // object is given by srcObjArr (-1 = none) or, for a mesh level set, is srcObj
 // where the node lies in the exact band (|phi| below the set's upper bound srcUb).
+// srcTri/dstTri (optional): the closest triangles, merged as the objects are,
+// the source's indices shifted by triOffset (its triangles follow dst's)
 __global__ void unionKernel(const float *src, int qi, int qj, int qk, I3 o, bool negate, float *dst, int pi, int pj,
-                            int pk, int *dstObj, const int *srcObjArr, int srcObj, float srcUb) {
+                            int pk, int *dstObj, const int *srcObjArr, int srcObj, float srcUb,
+                            const int *srcTri = nullptr, int *dstTri = nullptr, int triOffset = 0) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)qi * qj * qk) return;
     const int i = (int)(idx % qi), j = (int)((idx / qi) % qj), k = (int)(idx / ((size_t)qi * qj));
@@ -188,13 +247,169 @@ __global__ void unionKernel(const float *src, int qi, int qj, int qk, I3 o, bool
             const int so = srcObjArr ? srcObjArr[idx] : (fabsf(raw) < srcUb ? srcObj : -1);
             if (so >= 0 && fabsf(v) < fabsf(d)) dstObj[t] = so;
         }
+        if (dstTri && srcTri[idx] != -1 && fabsf(v) < fabsf(d)) dstTri[t] = srcTri[idx] + triOffset;
         d = v;
     }
+}
+
+__global__ void negateKernel(float *phi, size_t n) {  // MeshLevelSet::negate (meshlevelset.cpp:702-708)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) phi[i] = -phi[i];
+}
+
+// ---- the velocity data of a level set (meshlevelset.cpp:168-205, 1320-1425, 1568-1717) ----
+
+// one triangle with its vertices' velocities, in world coordinates
+struct VelTri {
+    V3 a, b, c, va, vb, vc;
+};
+// a level set's view for the nearest velocity: its grid and offset, phi,
+// closest triangles and mesh
+struct VelLS {
+    int ni, nj, nk;
+    I3 off;
+    double dx;
+    float ub;
+    const float *phi;
+    const int *tri;
+    const VelTri *tris;
+    int isStatic;
+};
+
+// _pointToSegmentVelocity (meshlevelset.cpp:1697-1717)
+__device__ V3 segmentVelocity(V3 x0, V3 x1, V3 x2, V3 v1, V3 v2, float *distance) {
+    const V3 d = sub(x2, x1);
+    const double m2 = (double)dot3(d, d);
+    float s12 = (float)((double)dot3(sub(x2, x0), d) / m2);
+    if (s12 < 0)
+        s12 = 0;
+    else if (s12 > 1)
+        s12 = 1;
+    *distance = len3(sub(x0, add(scale(s12, x1), scale(1 - s12, x2))));
+    return add(scale(s12, v1), scale(1 - s12, v2));
+}
+
+// _pointToTriangleVelocity (meshlevelset.cpp:1568-1639), x0 and the vertices
+// in the level set's frame
+__device__ V3 triangleVelocity(const VelTri &T, V3 po, V3 x0) {
+    const V3 v1 = T.va, v2 = T.vb, v3 = T.vc;
+    const float eps = 1e-6f;
+    if (fabsf(v1.x) < eps && fabsf(v1.y) < eps && fabsf(v1.z) < eps && fabsf(v2.x) < eps && fabsf(v2.y) < eps &&
+        fabsf(v2.z) < eps && fabsf(v3.x) < eps && fabsf(v3.y) < eps && fabsf(v3.z) < eps)
+        return V3{0.0f, 0.0f, 0.0f};
+    const V3 x1 = sub(T.a, po), x2 = sub(T.b, po), x3 = sub(T.c, po);
+    const V3 x13 = sub(x1, x3), x23 = sub(x2, x3), x03 = sub(x0, x3);
+    const float m13 = dot3(x13, x13), m23 = dot3(x23, x23), d = dot3(x13, x23);
+    const float invdet = 1.0f / fmaxf(m13 * m23 - d * d, 1e-30f);
+    const float a = dot3(x13, x03), b = dot3(x23, x03);
+    const float w23 = invdet * (m23 * a - d * b);
+    const float w31 = invdet * (m13 * b - d * a);
+    const float w12 = 1 - w23 - w31;
+    if (w23 >= 0 && w31 >= 0 && w12 >= 0) return add(add(scale(w23, v1), scale(w31, v2)), scale(w12, v3));
+    float d1, d2;
+    V3 e1, e2;
+    if (w23 > 0) {
+        e1 = segmentVelocity(x0, x1, x2, v1, v2, &d1);
+        e2 = segmentVelocity(x0, x1, x3, v1, v3, &d2);
+    } else if (w31 > 0) {
+        e1 = segmentVelocity(x0, x1, x2, v1, v2, &d1);
+        e2 = segmentVelocity(x0, x2, x3, v2, v3, &d2);
+    } else {
+        e1 = segmentVelocity(x0, x1, x3, v1, v3, &d1);
+        e2 = segmentVelocity(x0, x2, x3, v2, v3, &d2);
+    }
+    return d1 < d2 ? e1 : e2;
+}
+
+// getNearestVelocity (meshlevelset.cpp:168-205): p in world coordinates
+__device__ V3 nearestVelocity(const VelLS &L, V3 p) {
+    const V3 po = indexToPos(L.off.i, L.off.j, L.off.k, L.dx);
+    p = sub(p, po);
+    const I3 g = posToIndex(p, L.dx);
+    const int o8[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+    int nearest = -1;
+    float best = L.ub;
+    for (int q = 0; q < 8; q++) {
+        const int i = g.i + o8[q][0], j = g.j + o8[q][1], k = g.k + o8[q][2];
+        if (!inRange(i, j, k, L.ni + 1, L.nj + 1, L.nk + 1)) continue;
+        const int t = L.tri[flat(i, j, k, L.ni + 1, L.nj + 1)];
+        if (t == -1) continue;
+        const VelTri T = L.tris[t];
+        const V3 cp = closestPointOnTriangle(p, sub(T.a, po), sub(T.b, po), sub(T.c, po));
+        const float d = len3(sub(cp, p));
+        if (d < best) {
+            best = d;
+            nearest = t;
+        }
+    }
+    if (nearest == -1) return V3{0.0f, 0.0f, 0.0f};
+    return triangleVelocity(L.tris[nearest], po, p);
+}
+
+// _computeVelocityGridThread (meshlevelset.cpp:1320-1374) for one face
+// direction, each face's weight x velocity and weight added into the domain
+// accumulators at the level set's offset (calculateUnion, 1759-1845: the
+// faces inside both arrays)
+template <int DIR>
+__global__ void velocityGridKernel(VelLS L, Grid g, float *df, float *dw) {
+    const int fi = L.ni + (DIR == 0), fj = L.nj + (DIR == 1), fk = L.nk + (DIR == 2);
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)fi * fj * fk) return;
+    const int i = (int)(idx % fi), j = (int)((idx / fi) % fj), k = (int)(idx / ((size_t)fi * fj));
+    const int a = i + L.off.i, b = j + L.off.j, c = k + L.off.k;
+    if (!inRange(a, b, c, g.ni + (DIR == 0), g.nj + (DIR == 1), g.nk + (DIR == 2))) return;
+    const int pi = L.ni + 1, pj = L.nj + 1;
+    auto P = [&](int x, int y, int z) { return L.phi[flat(x, y, z, pi, pj)]; };
+    float weight;
+    if (DIR == 0)
+        weight = fraction4(P(i, j, k), P(i, j + 1, k), P(i, j, k + 1), P(i, j + 1, k + 1));
+    else if (DIR == 1)
+        weight = fraction4(P(i, j, k), P(i, j, k + 1), P(i + 1, j, k), P(i + 1, j, k + 1));
+    else
+        weight = fraction4(P(i, j, k), P(i, j + 1, k), P(i + 1, j, k), P(i + 1, j + 1, k));
+    if (!(weight > 0.0f)) return;
+    float comp = 0.0f;
+    if (!L.isStatic) {
+        const double dx = L.dx;
+        V3 fp;
+        if (DIR == 0)
+            fp = mk((float)i * dx, ((float)j + 0.5) * dx, ((float)k + 0.5) * dx);
+        else if (DIR == 1)
+            fp = mk(((float)i + 0.5) * dx, (float)j * dx, ((float)k + 0.5) * dx);
+        else
+            fp = mk(((float)i + 0.5) * dx, ((float)j + 0.5) * dx, (float)k * dx);
+        const V3 v = nearestVelocity(L, add(fp, indexToPos(L.off.i, L.off.j, L.off.k, dx)));
+        comp = DIR == 0 ? v.x : (DIR == 1 ? v.y : v.z);
+    }
+    const size_t t = flat(a, b, c, g.ni + (DIR == 0), g.nj + (DIR == 1));
+    df[t] += weight * comp;
+    dw[t] += weight;
+}
+
+__global__ void addArraysKernel(float *dst, const float *src, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] += src[i];
+}
+
+// normalizeVelocityGrid's per-face step (meshlevelset.cpp:1739-1757)
+__global__ void normalizeVelocityKernel(const float *f, const float *w, size_t n, float *out, uint8_t *valid) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float wt = w[i];
+    const bool ok = wt > 1e-6f;
+    out[i] = ok ? f[i] / wt : 0.0f;
+    valid[i] = ok ? 1 : 0;
 }
 
 unsigned blocksFor(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 // ------------------------------------------------------------ host ----------
+
+unsigned floatKeyBits(float f) {
+    unsigned u;
+    std::memcpy(&u, &f, sizeof(u));
+    return u;
+}
 
 template <class T>
 struct DevArray {
@@ -212,6 +427,7 @@ struct DevArray {
 
 template <class T>
 void upload(DevArray<T> &d, const std::vector<T> &h, hipStream_t s) {
+    if (h.size() > d.n) throw std::runtime_error("mesh obstacle: upload larger than its device array");
     if (!h.empty())
         hipCheck(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s), "upload mesh data");
 }
@@ -219,6 +435,7 @@ void upload(DevArray<T> &d, const std::vector<T> &h, hipStream_t s) {
 struct HMesh {
     std::vector<V3> v;
     std::vector<std::array<int, 3>> t;
+    std::vector<V3> vel;  // per-vertex velocities, or empty: at rest
 };
 
 // AABB(points) / AABB(Triangle, vertices): double min/max, extents + 1e-9 (aabb.cpp:54-112)
@@ -286,6 +503,8 @@ std::vector<HMesh> splitIslands(const HMesh &m) {
             if (tr[t[c]] < 0) {
                 tr[t[c]] = (int)I.v.size();
                 I.v.push_back(m.v[t[c]]);
+                // the island's vertex velocities follow its vertices (meshutils.cpp:574-593)
+                if (!m.vel.empty()) I.vel.push_back(m.vel[t[c]]);
             }
             t[c] = tr[t[c]];
         }
@@ -367,8 +586,9 @@ void markInside(HMesh m, double dx, int pi, int pj, int pk, RandJitter &rj, uint
 // MeshLevelSet::fastCalculateSignedDistanceField of mesh m on a level set of
 // (isize, jsize, ksize) cells at node offset o (meshlevelset.cpp:572-601);
 // block = the multi-threaded band.  phi: device, (isize+1)(jsize+1)(ksize+1).
+// tri (optional, device, nodes): each node's closest triangle, -1 outside the band.
 void levelSetFast(const HMesh &m, int isize, int jsize, int ksize, I3 o, double dx, bool block, RandJitter &rj,
-                  float *phi, hipStream_t s) {
+                  float *phi, hipStream_t s, int *tri = nullptr) {
     const int pi = isize + 1, pj = jsize + 1, pk = ksize + 1, band = EXACT_BAND;
     const size_t nn = (size_t)pi * pj * pk;
     const float ub = (float)((pi + pj + pk) * dx);  // getDistanceUpperBound (meshlevelset.cpp:769-771)
@@ -378,7 +598,7 @@ void levelSetFast(const HMesh &m, int isize, int jsize, int ksize, I3 o, double 
     bt.reserve(m.t.size());
     for (size_t t = 0; t < m.t.size(); t++) {
         const V3 a = sub(m.v[m.t[t][0]], po), b = sub(m.v[m.t[t][1]], po), c = sub(m.v[m.t[t][2]], po);
-        BandTri T{a, b, c, {0, 0, 0}, {0, 0, 0}};
+        BandTri T{a, b, c, {0, 0, 0}, {0, 0, 0}, (int)t};
         if (block) {  // _initializeTriangleData (meshlevelset.cpp:878-918)
             Box tb = aabbTri(m, t);
             tb.pos = sub(tb.pos, po);
@@ -407,7 +627,24 @@ void levelSetFast(const HMesh &m, int isize, int jsize, int ksize, I3 o, double 
         }
         bt.push_back(T);
     }
-    if (!bt.empty()) {
+    if (tri) {
+        DevArray<unsigned long long> keys(nn);
+        const unsigned long long none = ((unsigned long long)floatKeyBits(ub) << 32) | 0xffffffffull;
+        hipLaunchKernelGGL(fillKeysKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, keys.p, nn, none);
+        if (!bt.empty()) {
+            DevArray<BandTri> dbt(bt.size());
+            upload(dbt, bt, s);
+            if (block)
+                hipLaunchKernelGGL(meshBandTriKernel<true>, dim3((unsigned)bt.size()), dim3(256), 0, s, dbt.p, pi, pj,
+                                   dx, keys.p);
+            else
+                hipLaunchKernelGGL(meshBandTriKernel<false>, dim3((unsigned)bt.size()), dim3(256), 0, s, dbt.p, pi,
+                                   pj, dx, keys.p);
+            hipCheck(hipStreamSynchronize(s), "exact band");
+        }
+        hipLaunchKernelGGL(splitKeysKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, keys.p, nn, phi, tri);
+        hipCheck(hipStreamSynchronize(s), "exact band");
+    } else if (!bt.empty()) {
         DevArray<BandTri> dbt(bt.size());
         upload(dbt, bt, s);
         if (block)
@@ -459,6 +696,10 @@ HMesh hostMesh(const ObstacleMesh &o) {
     m.v.resize(o.numVertices);
     for (int q = 0; q < o.numVertices; q++)
         m.v[q] = V3{o.vertices[3 * q], o.vertices[3 * q + 1], o.vertices[3 * q + 2]};
+    m.vel.assign(o.numVertices, V3{0.0f, 0.0f, 0.0f});
+    if (o.velocities)
+        for (int q = 0; q < o.numVertices; q++)
+            m.vel[q] = V3{o.velocities[3 * q], o.velocities[3 * q + 1], o.velocities[3 * q + 2]};
     m.t.resize(o.numTriangles);
     for (int q = 0; q < o.numTriangles; q++)
         for (int c = 0; c < 3; c++) {
@@ -469,11 +710,25 @@ HMesh hostMesh(const ObstacleMesh &o) {
     return m;
 }
 
-// MeshObject::getMeshLevelSet for a static mesh, unioned into levelset
-// (meshobject.cpp:239-264, 633-701, 738-839); obj (optional) tracks the
-// closest object beside it, this obstacle being objId
+// A domain-sized level set that takes an obstacle's parts with velocity data:
+// phi, closest objects and (optional) closest triangles, indices into `tris`,
+// the triangles of the parts unioned so far; the face accumulators f / w
+// (optional) receive the parts' velocity grids.
+struct VelTarget {
+    float *phi;
+    int *obj;
+    int *tri;
+    std::vector<VelTri> *tris;
+    float *const *f;
+    float *const *w;
+};
+
+// MeshObject::getMeshLevelSet, unioned into levelset (meshobject.cpp:239-264,
+// 633-701, 738-839); obj (optional) tracks the closest object beside it, this
+// obstacle being objId.  With vt, the parts go into vt's level set instead,
+// with their velocity data.
 void obstacleLevelSet(const ObstacleMesh &o, int objId, Grid g, RandJitter &rj, float *levelset, int *obj,
-                      hipStream_t s) {
+                      hipStream_t s, const VelTarget *vt = nullptr) {
     HMesh m = hostMesh(o);
     // getMesh(frameInterpolation): v + f (v - v), which turns -0 into +0
     for (V3 &p : m.v) p = add(p, V3{0.0f, 0.0f, 0.0f});
@@ -481,15 +736,17 @@ void obstacleLevelSet(const ObstacleMesh &o, int objId, Grid g, RandJitter &rj, 
         std::vector<char> isUsed(m.v.size(), 0);
         for (const auto &t : m.t) isUsed[t[0]] = isUsed[t[1]] = isUsed[t[2]] = 1;
         std::vector<int> tr(m.v.size(), -1);
-        std::vector<V3> kept;
+        std::vector<V3> kept, keptVel;
         for (size_t q = 0; q < m.v.size(); q++)
             if (isUsed[q]) {
                 tr[q] = (int)kept.size();
                 kept.push_back(m.v[q]);
+                keptVel.push_back(m.vel[q]);
             }
         for (auto &t : m.t)
             for (int c = 0; c < 3; c++) t[c] = tr[t[c]];
         m.v = kept;
+        m.vel = keptVel;
     }
     if (m.t.empty()) return;
     const Box domain{V3{0.0f, 0.0f, 0.0f}, g.ni * g.dx, g.nj * g.dx, g.nk * g.dx};
@@ -531,11 +788,45 @@ void obstacleLevelSet(const ObstacleMesh &o, int objId, Grid g, RandJitter &rj, 
         if (w <= 0 || h <= 0 || d <= 0) continue;
         const size_t nn = (size_t)(w + 1) * (h + 1) * (d + 1);
         DevArray<float> phi(nn);
-        // fracture path: per-island level sets with multi-threading off (meshobject.cpp:798-839)
-        levelSetFast(part, w, h, d, g0, g.dx, !fracture, rj, phi.p, s);
         const float ub = (float)(((w + 1) + (h + 1) + (d + 1)) * g.dx);  // the island set's upper bound
+        if (!vt) {
+            // fracture path: per-island level sets with multi-threading off (meshobject.cpp:798-839)
+            levelSetFast(part, w, h, d, g0, g.dx, !fracture, rj, phi.p, s);
+            hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, phi.p, w + 1, h + 1, d + 1,
+                               g0, false, levelset, g.ni + 1, g.nj + 1, g.nk + 1, obj, (const int *)nullptr, objId,
+                               ub);
+            hipCheck(hipStreamSynchronize(s), "obstacle union");
+            continue;
+        }
+        // the part's triangles with their vertices' velocities; a part whose
+        // vertices all stand still has zero velocity grids (meshlevelset.cpp:1376-1425)
+        std::vector<VelTri> vts(part.t.size());
+        bool isStatic = true;
+        for (const V3 &v : part.vel)
+            if (std::fabs(v.x) > 0.0f || std::fabs(v.y) > 0.0f || std::fabs(v.z) > 0.0f) isStatic = false;
+        for (size_t t = 0; t < part.t.size(); t++) {
+            const auto &T = part.t[t];
+            vts[t] = VelTri{part.v[T[0]], part.v[T[1]], part.v[T[2]], part.vel[T[0]], part.vel[T[1]], part.vel[T[2]]};
+        }
+        const bool faces = vt->f != nullptr, needTri = vt->tri != nullptr || (faces && !isStatic);
+        DevArray<int> tri(needTri ? nn : 0);
+        levelSetFast(part, w, h, d, g0, g.dx, !fracture, rj, phi.p, s, needTri ? tri.p : nullptr);
+        DevArray<VelTri> dvt(faces && !isStatic ? vts.size() : 0);
+        if (faces) {
+            if (!isStatic) upload(dvt, vts, s);
+            const VelLS L{w, h, d, g0, g.dx, ub, phi.p, tri.p, dvt.p, isStatic ? 1 : 0};
+            hipLaunchKernelGGL(velocityGridKernel<0>, dim3(blocksFor((size_t)(w + 1) * h * d, 256)), dim3(256), 0, s,
+                               L, g, vt->f[0], vt->w[0]);
+            hipLaunchKernelGGL(velocityGridKernel<1>, dim3(blocksFor((size_t)w * (h + 1) * d, 256)), dim3(256), 0, s,
+                               L, g, vt->f[1], vt->w[1]);
+            hipLaunchKernelGGL(velocityGridKernel<2>, dim3(blocksFor((size_t)w * h * (d + 1), 256)), dim3(256), 0, s,
+                               L, g, vt->f[2], vt->w[2]);
+        }
+        const int triOffset = vt->tris ? (int)vt->tris->size() : 0;
         hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, phi.p, w + 1, h + 1, d + 1, g0,
-                           false, levelset, g.ni + 1, g.nj + 1, g.nk + 1, obj, (const int *)nullptr, objId, ub);
+                           false, vt->phi, g.ni + 1, g.nj + 1, g.nk + 1, vt->obj, (const int *)nullptr, objId, ub,
+                           vt->tri ? (const int *)tri.p : nullptr, vt->tri, triOffset);
+        if (vt->tris) vt->tris->insert(vt->tris->end(), vts.begin(), vts.end());
         hipCheck(hipStreamSynchronize(s), "obstacle union");
     }
 }
@@ -556,8 +847,143 @@ int meshLevelSetFastDevice(Grid cells, I3 offset, const ObstacleMesh &mesh, floa
     return rj.draws;
 }
 
+namespace {
+
+size_t faceCount(Grid g, int d) { return (size_t)(g.ni + (d == 0)) * (g.nj + (d == 1)) * (g.nk + (d == 2)); }
+
+void fillF(float *a, size_t n, float v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fillFloatKernel, dim3(blocksFor(n, 256)), dim3(256), 0, s, a, n, v);
+}
+void fillI(int *a, size_t n, int v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fillIntKernel, dim3(blocksFor(n, 256)), dim3(256), 0, s, a, n, v);
+}
+
+// the velocity grids of a domain-sized level set (phi, closest triangles into
+// tris) added into f / w (zeroed first: the level set's own grids)
+void domainVelocityGrids(Grid g, const float *phi, const int *tri, const std::vector<VelTri> &tris, float *const *f,
+                         float *const *w, hipStream_t s) {
+    bool isStatic = true;
+    for (const VelTri &T : tris)
+        for (const V3 &v : {T.va, T.vb, T.vc})
+            if (std::fabs(v.x) > 0.0f || std::fabs(v.y) > 0.0f || std::fabs(v.z) > 0.0f) isStatic = false;
+    DevArray<VelTri> dvt(isStatic ? 0 : tris.size());
+    upload(dvt, isStatic ? std::vector<VelTri>() : tris, s);
+    const float ub = (float)(((g.ni + 1) + (g.nj + 1) + (g.nk + 1)) * g.dx);
+    const VelLS L{g.ni, g.nj, g.nk, I3{0, 0, 0}, g.dx, ub, phi, tri, dvt.p, isStatic ? 1 : 0};
+    for (int d = 0; d < 3; d++) {
+        fillF(f[d], faceCount(g, d), 0.0f, s);
+        fillF(w[d], faceCount(g, d), 0.0f, s);
+    }
+    hipLaunchKernelGGL(velocityGridKernel<0>, dim3(blocksFor(faceCount(g, 0), 256)), dim3(256), 0, s, L, g, f[0], w[0]);
+    hipLaunchKernelGGL(velocityGridKernel<1>, dim3(blocksFor(faceCount(g, 1), 256)), dim3(256), 0, s, L, g, f[1], w[1]);
+    hipLaunchKernelGGL(velocityGridKernel<2>, dim3(blocksFor(faceCount(g, 2), 256)), dim3(256), 0, s, L, g, f[2], w[2]);
+    hipCheck(hipStreamSynchronize(s), "velocity grids");
+}
+
+void addFaces(Grid g, float *const *dst, float *const *src, hipStream_t s) {
+    for (int d = 0; d < 3; d++)
+        hipLaunchKernelGGL(addArraysKernel, dim3(blocksFor(faceCount(g, d), 256)), dim3(256), 0, s, dst[d], src[d],
+                           faceCount(g, d));
+}
+
+// FluidSimulation::_updateSolidLevelSet with velocity data
+// (fluidsimulation.cpp:5219-5426; MeshLevelSet::calculateUnion, negate and
+// the velocity grids, meshlevelset.cpp:577-718, 1320-1425, 1759-1845)
+void solidWithVelocity(Grid g, const ObstacleMesh *obs, int n, float *solid, int *nodeObject, RandJitter &rj,
+                       SolidVelocityData &V, hipStream_t s) {
+    const size_t nn = (size_t)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
+    const float ubDomain = (float)(((g.ni + 1) + (g.nj + 1) + (g.nk + 1)) * g.dx);
+    // the domain: its negated box level set's velocity grids (zero velocity, the faces' weights)
+    domainVelocityGrids(g, solid, nullptr, {}, V.f, V.w, s);
+    for (int pass = 0; pass < 2; pass++) {  // static objects, then animated ones (5387-5426)
+        const bool animated = pass == 1;
+        for (int q = 0; q < n; q++) {
+            if (obs[q].animated != animated || obs[q].inversed) continue;
+            // the object's own level set (_tempSolidSDF), reset, then unioned into the solid
+            fillF(V.tphi, nn, ubDomain, s);
+            fillI(V.tobj, nn, -1, s);
+            for (int d = 0; d < 3; d++) {
+                fillF(V.tf[d], faceCount(g, d), 0.0f, s);
+                fillF(V.tw[d], faceCount(g, d), 0.0f, s);
+            }
+            const VelTarget vt{V.tphi, V.tobj, nullptr, nullptr, V.tf, V.tw};
+            obstacleLevelSet(obs[q], q + 1, g, rj, nullptr, nullptr, s, &vt);
+            hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, V.tphi, g.ni + 1, g.nj + 1,
+                               g.nk + 1, I3{0, 0, 0}, false, solid, g.ni + 1, g.nj + 1, g.nk + 1, nodeObject,
+                               (const int *)V.tobj, 0, 0.0f, nullptr, nullptr, 0);
+            addFaces(g, V.f, V.tf, s);
+            addFaces(g, V.w, V.tw, s);
+        }
+        bool anyInversed = false;
+        for (int q = 0; q < n; q++) anyInversed |= obs[q].animated == animated && obs[q].inversed;
+        if (!anyInversed) continue;
+        // the inversed objects: their union without velocity data, negated
+        // (which computes its velocity grids), then unioned into the solid
+        std::vector<VelTri> invTris, tempTris;
+        fillF(V.iphi, nn, ubDomain, s);
+        fillI(V.iobj, nn, -1, s);
+        fillI(V.itri, nn, -1, s);
+        for (int q = 0; q < n; q++) {
+            if (obs[q].animated != animated || !obs[q].inversed) continue;
+            fillF(V.tphi, nn, ubDomain, s);
+            fillI(V.tobj, nn, -1, s);
+            fillI(V.ttri, nn, -1, s);
+            tempTris.clear();
+            const VelTarget vt{V.tphi, V.tobj, V.ttri, &tempTris, nullptr, nullptr};
+            obstacleLevelSet(obs[q], q + 1, g, rj, nullptr, nullptr, s, &vt);
+            hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, V.tphi, g.ni + 1, g.nj + 1,
+                               g.nk + 1, I3{0, 0, 0}, false, V.iphi, g.ni + 1, g.nj + 1, g.nk + 1, V.iobj,
+                               (const int *)V.tobj, 0, 0.0f, (const int *)V.ttri, V.itri, (int)invTris.size());
+            invTris.insert(invTris.end(), tempTris.begin(), tempTris.end());
+            hipCheck(hipStreamSynchronize(s), "inversed union");
+        }
+        hipLaunchKernelGGL(negateKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, V.iphi, nn);
+        domainVelocityGrids(g, V.iphi, V.itri, invTris, V.tf, V.tw, s);
+        hipLaunchKernelGGL(unionKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, V.iphi, g.ni + 1, g.nj + 1,
+                           g.nk + 1, I3{0, 0, 0}, false, solid, g.ni + 1, g.nj + 1, g.nk + 1, nodeObject,
+                           (const int *)V.iobj, 0, 0.0f, nullptr, nullptr, 0);
+        addFaces(g, V.f, V.tf, s);
+        addFaces(g, V.w, V.tw, s);
+    }
+    hipCheck(hipStreamSynchronize(s), "solid velocity data");
+}
+
+}  // namespace
+
+size_t solidVelocityBytes(Grid g) {
+    const size_t nn = (size_t)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    size_t b = 0;
+    for (int d = 0; d < 3; d++) b += 4 * up(sizeof(float) * faceCount(g, d));
+    return b + 2 * up(sizeof(float) * nn) + 4 * up(sizeof(int) * nn);
+}
+
+SolidVelocityData solidVelocityCarve(Grid g, void *block) {
+    const size_t nn = (size_t)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
+    char *p = (char *)block;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    SolidVelocityData V{};
+    for (int d = 0; d < 3; d++) {
+        V.f[d] = (float *)take(sizeof(float) * faceCount(g, d));
+        V.w[d] = (float *)take(sizeof(float) * faceCount(g, d));
+        V.tf[d] = (float *)take(sizeof(float) * faceCount(g, d));
+        V.tw[d] = (float *)take(sizeof(float) * faceCount(g, d));
+    }
+    V.tphi = (float *)take(sizeof(float) * nn);
+    V.iphi = (float *)take(sizeof(float) * nn);
+    V.tobj = (int *)take(sizeof(int) * nn);
+    V.iobj = (int *)take(sizeof(int) * nn);
+    V.ttri = (int *)take(sizeof(int) * nn);
+    V.itri = (int *)take(sizeof(int) * nn);
+    return V;
+}
+
 int meshObstacleSolid(Grid g, const ObstacleMesh *obs, int n, float *solid, int *nodeObject, bool domainObject,
-                      hipStream_t s) {
+                      hipStream_t s, SolidVelocityData *velocity) {
     RandJitter rj;
     rj.next(g.dx);  // the domain box's own sign test draws first (fluidsimulation.cpp:5277-5287)
     const size_t nn = (size_t)(g.ni + 1) * (g.nj + 1) * (g.nk + 1);
@@ -565,6 +991,11 @@ int meshObstacleSolid(Grid g, const ObstacleMesh *obs, int n, float *solid, int 
     if (nodeObject)
         hipLaunchKernelGGL(domainObjectKernel, dim3(blocksFor(nn, 256)), dim3(256), 0, s, solid, nn, ubDomain,
                            domainObject ? 1 : 0, nodeObject);
+    if (velocity) {
+        solidWithVelocity(g, obs, n, solid, nodeObject, rj, *velocity, s);
+        hipCheck(hipGetLastError(), "mesh obstacle kernels");
+        return rj.draws;
+    }
     for (int q = 0; q < n; q++)
         if (!obs[q].inversed) obstacleLevelSet(obs[q], q + 1, g, rj, solid, nodeObject, s);
     bool anyInversed = false;
@@ -587,6 +1018,21 @@ int meshObstacleSolid(Grid g, const ObstacleMesh *obs, int n, float *solid, int 
     }
     hipCheck(hipGetLastError(), "mesh obstacle kernels");
     return rj.draws;
+}
+
+void normalizeSolidVelocity(Grid g, const SolidVelocityData &d, float *us, float *vs, float *ws, uint8_t *valid,
+                            uint8_t *status, hipStream_t s) {
+    float *out[3] = {us, vs, ws};
+    const int wi[3] = {g.ni + 1, g.ni, g.ni}, wj[3] = {g.nj, g.nj + 1, g.nj}, wk[3] = {g.nk, g.nk, g.nk + 1};
+    const int layers = 5;  // MeshLevelSet::_numVelocityExtrapolationLayers (meshlevelset.h:363)
+    for (int q = 0; q < 3; q++) {
+        const size_t nf = faceCount(g, q);
+        hipLaunchKernelGGL(normalizeVelocityKernel, dim3(blocksFor(nf, 256)), dim3(256), 0, s, d.f[q], d.w[q], nf,
+                           out[q], valid);
+        extrapolateInit(wi[q], wj[q], wk[q], valid, status, s);
+        for (int l = 0; l < layers; l++) extrapolateLayer(wi[q], wj[q], wk[q], out[q], status, l, s);
+    }
+    hipCheck(hipGetLastError(), "normalize solid velocity");
 }
 
 }  // namespace ff

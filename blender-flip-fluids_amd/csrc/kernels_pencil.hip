@@ -31,12 +31,18 @@ namespace {
 // pencil shapes (lines per side) and chunk lengths: FP32 and FP64
 constexpr int PTJ = 16, PTK = 16, PC32 = 8, PC64 = 4;
 
+// dynamic LDS: the staged records (16-byte aligned), inputs and outputs, then
+// the line words and the face lines' row bits
 template <class T, int TJ, int TK, int C, bool FWD>
 __global__ __launch_bounds__(TJ *TK) void pencilSweepKernel(PencilArgs<T, MicRec<T>> a) {
     extern __shared__ unsigned long long pencilShared[];
-    LineWord *sW = (LineWord *)pencilShared;
-    unsigned *sN = (unsigned *)(sW + TJ * TK * a.nw);
-    pencilSweep<T, MicRec<T>, TJ, TK, C, FWD>(a, (int)threadIdx.x, sW, sN);
+    constexpr int NT = TJ * TK;
+    MicRec<T> *sRec = (MicRec<T> *)pencilShared;
+    T *sIn = (T *)(sRec + C * NT);
+    T *sOut = sIn + C * NT;
+    LineWord *sW = (LineWord *)(sOut + C * NT);
+    unsigned *sN = (unsigned *)(sW + NT * a.nw);
+    pencilSweep<T, MicRec<T>, TJ, TK, C, FWD>(a, (int)threadIdx.x, sW, sN, sRec, sIn, sOut);
 }
 
 // one thread per 32-cell word of a line: its row bits and the row of its
@@ -54,7 +60,10 @@ __global__ void lineWordsKernel(Grid g, int nw, const int *cellRow, const int *r
     words[q] = LineWord{rowOf[c0], bits};
 }
 
-size_t pencilLds(int nw) { return (size_t)(PTJ * PTK) * nw * sizeof(LineWord) + (size_t)(PTJ + PTK) * nw * 4; }
+template <class T, int C>
+size_t pencilLds(int nw) {
+    return (size_t)PencilLds<T, MicRec<T>, PTJ, PTK, C>::bytes(nw);
+}
 
 template <class T, int C>
 void pencilSweeps(Grid g, const PressureSys &S, PencilPlan &pp, const T *r, T *y, T *z, const MicRec<T> *recFwd,
@@ -73,7 +82,7 @@ void pencilSweeps(Grid g, const PressureSys &S, PencilPlan &pp, const T *r, T *y
     a.error = pp.error;
     a.waitTicks = pp.waitTicks;
     const dim3 grid(pp.np), block(PTJ * PTK);
-    const size_t lds = pencilLds(pp.nw);
+    const size_t lds = pencilLds<T, C>(pp.nw);
     for (int sweep = 0; sweep < 2; sweep++) {
         a.base = pp.base;
         if (++pp.epoch == 0) pp.epoch = 1;  // 0 tags the cleared buffer
@@ -112,7 +121,8 @@ size_t pencilExchangeWords(Grid g) {
 size_t pencilLdsBytes(Grid g) {
     int npj, npk, nw;
     pencilShape(g, npj, npk, nw);
-    return pencilLds(nw);
+    const size_t a = pencilLds<double, PC64>(nw), b = pencilLds<float, PC32>(nw);
+    return a > b ? a : b;
 }
 
 void pencilOrder(Grid g, std::vector<int> &order) {

@@ -42,12 +42,13 @@ __global__ void rowsKernel(size_t n, const int *mask, const int *rowOf, int *row
 // _calculateNegativeDivergenceVectorThread (pressuresolver.cpp:624-713) and
 // _calculateMatrixCoefficientsThread (767-867).  Static solids have zero face
 // velocity, so the divergence's solid terms add +-0.0 to a sum that is never
-// -0.0 and are left out.  Density is 1 (fluidsimulation.cpp:6214).  An
+// -0.0 and are left out; a moving solid's (T.us given) are added with the
+// cell weight.  Density is 1 (fluidsimulation.cpp:6214).  An
 // off-diagonal entry exists when the neighbour is liquid and a pressure row
 // (SparseMatrix::add ignores -1, sparsematrix.h:94-96).
 __global__ void assembleKernel(Grid g, double dt, const float *u, const float *v, const float *w,
                                const float *wu, const float *wv, const float *ww, const float *phi,
-                               const int *mask, const int *rowOf, PressureSys S) {
+                               const int *mask, const int *rowOf, PressureSys S, SolidTerms T) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= S.m) return;
     const int ni = g.ni, nj = g.nj;
@@ -66,6 +67,15 @@ __global__ void assembleKernel(Grid g, double dt, const float *u, const float *v
     div += factor * volBottom * (double)v[flat(i, j, k, ni, nj + 1)];
     div += -factor * volFront * (double)w[flat(i, j, k + 1, ni, nj)];
     div += factor * volBack * (double)w[flat(i, j, k, ni, nj)];
+    if (T.us) {
+        const double volCenter = T.wc[c];
+        div += factor * (volRight - volCenter) * (double)T.us[flat(i + 1, j, k, ni + 1, nj)];
+        div += -factor * (volLeft - volCenter) * (double)T.us[flat(i, j, k, ni + 1, nj)];
+        div += factor * (volTop - volCenter) * (double)T.vs[flat(i, j + 1, k, ni, nj + 1)];
+        div += -factor * (volBottom - volCenter) * (double)T.vs[flat(i, j, k, ni, nj + 1)];
+        div += factor * (volFront - volCenter) * (double)T.ws[flat(i, j, k + 1, ni, nj)];
+        div += -factor * (volBack - volCenter) * (double)T.ws[flat(i, j, k, ni, nj)];
+    }
     S.rhs[r] = div;
 
     const double rho = (1.0 + 1.0) / 2.0;
@@ -220,17 +230,26 @@ __global__ void backwardLevelKernel(const int *nbr, const T *invdiag, const T *l
 // the assembly computes both from the same face by the same expression, so
 // they are the same bits, and those rows were just read by nearby threads
 // (mx, my, mz are kept for the other users of the matrix).
+// Row r's product with the 7-point matrix, (A x)_r, in the reference's order
+// (sparsematrix.h:174-215 over the row's entries by column: -z, -y, -x,
+// diagonal, +x, +y, +z).  The -x/-y/-z coefficients are the neighbour rows'
+// +x/+y/+z entries (the same bits: both sides assemble -(weight / rho) * dt /
+// dx^2 from the shared face).  A missing neighbour's term is left out, as
+// SparseMatrix leaves out absent entries.
+struct RowTopo {
+    int m;
+    const int *nbr;  // 6 per row: -x, +x, -y, +y, -z, +z rows or -1
+};
+
 template <class T>
-__device__ inline T rowProduct(size_t r, const int *nbr, const T *diag, const T *ox, const T *oy, const T *oz,
-                               const T *, const T *, const T *, const T *x) {
-    const int *nb = nbr + 6 * r;
+__device__ inline T rowProduct(int r, const RowTopo &t, const T *diag, const T *ox, const T *oy, const T *oz,
+                               const T *x) {
+    const int *nb = t.nbr + 6 * (size_t)r;
     const int n0 = nb[0], n1 = nb[1], n2 = nb[2], n3 = nb[3], n4 = nb[4], n5 = nb[5];
-    const size_t a0 = n0 >= 0 ? (size_t)n0 : r, a2 = n2 >= 0 ? (size_t)n2 : r, a4 = n4 >= 0 ? (size_t)n4 : r;
-    const T x0 = x[a0], x1 = x[n1 >= 0 ? (size_t)n1 : r];
-    const T x2 = x[a2], x3 = x[n3 >= 0 ? (size_t)n3 : r];
-    const T x4 = x[a4], x5 = x[n5 >= 0 ? (size_t)n5 : r];
-    const T xr = x[r];
+    const int a0 = n0 >= 0 ? n0 : r, a1 = n1 >= 0 ? n1 : r, a2 = n2 >= 0 ? n2 : r, a3 = n3 >= 0 ? n3 : r,
+              a4 = n4 >= 0 ? n4 : r, a5 = n5 >= 0 ? n5 : r;
     const T cmz = oz[a4], cmy = oy[a2], cmx = ox[a0], cd = diag[r], cox = ox[r], coy = oy[r], coz = oz[r];
+    const T x4 = x[a4], x2 = x[a2], x0 = x[a0], xr = x[r], x1 = x[a1], x3 = x[a3], x5 = x[a5];
     T s = 0;
     if (n4 >= 0) s += cmz * x4;
     if (n2 >= 0) s += cmy * x2;
@@ -242,47 +261,87 @@ __device__ inline T rowProduct(size_t r, const int *nbr, const T *diag, const T 
     return s;
 }
 
-template <class T>
-__global__ void spmvKernel(int m, const int *nbr, const T *diag, const T *ox, const T *oy,
-                           const T *oz, const T *mx, const T *my, const T *mz, const T *x, T *y) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= m) return;
-    y[r] = rowProduct<T>(r, nbr, diag, ox, oy, oz, mx, my, mz, x);
+// one row per thread; blocks renumbered so that each die streams one
+// contiguous eighth of the rows (its neighbour rows then sit in its L2)
+__device__ inline int dieRow() {
+    const unsigned b = blockIdx.x, g = gridDim.x;
+    const unsigned lb = g % XCDS == 0 ? (b % XCDS) * (g / XCDS) + b / XCDS : b;
+    return (int)(lb * blockDim.x + threadIdx.x);
 }
 
-// block sum (OP 0) or max (OP 1) of one double per thread, into partial[block]
-template <int OP>
-__device__ void blockFold(double v, double *partial) {
-    __shared__ double sh[REDUCE_THREADS];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = REDUCE_THREADS / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < (unsigned)off) {
-            if (OP == 0)
-                sh[threadIdx.x] += sh[threadIdx.x + off];
-            else
-                sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
-        }
-        __syncthreads();
+template <class T>
+__global__ void spmvKernel(RowTopo t, const T *diag, const T *ox, const T *oy, const T *oz, const T *x, T *y) {
+    const int r = dieRow();
+    if (r >= t.m) return;
+    y[r] = rowProduct<T>(r, t, diag, ox, oy, oz, x);
+}
+
+// U rows' products at once: the neighbour lists of all U rows are loaded
+// before their values, so a thread keeps U lookups in flight.  Indices of
+// absent neighbours are clamped to the row itself (a valid address) and their
+// terms left out, as in rowProduct.
+template <class T, int U>
+__device__ inline void rowProducts(const int (&r)[U], const RowTopo &t, const T *__restrict__ diag,
+                                   const T *__restrict__ ox, const T *__restrict__ oy, const T *__restrict__ oz,
+                                   const T *__restrict__ x, T (&out)[U]) {
+    int nb[U][6];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int q = 0; q < 6; q++) nb[u][q] = t.nbr[6 * (size_t)r[u] + q];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int q = r[u];
+        const int n0 = nb[u][0], n1 = nb[u][1], n2 = nb[u][2], n3 = nb[u][3], n4 = nb[u][4], n5 = nb[u][5];
+        const int a0 = n0 >= 0 ? n0 : q, a1 = n1 >= 0 ? n1 : q, a2 = n2 >= 0 ? n2 : q, a3 = n3 >= 0 ? n3 : q,
+                  a4 = n4 >= 0 ? n4 : q, a5 = n5 >= 0 ? n5 : q;
+        const T cmz = oz[a4], cmy = oy[a2], cmx = ox[a0], cd = diag[q], cox = ox[q], coy = oy[q], coz = oz[q];
+        const T x4 = x[a4], x2 = x[a2], x0 = x[a0], xr = x[q], x1 = x[a1], x3 = x[a3], x5 = x[a5];
+        T s = 0;
+        if (n4 >= 0) s += cmz * x4;
+        if (n2 >= 0) s += cmy * x2;
+        if (n0 >= 0) s += cmx * x0;
+        s += cd * xr;
+        if (n1 >= 0) s += cox * x1;
+        if (n3 >= 0) s += coy * x3;
+        if (n5 >= 0) s += coz * x5;
+        out[u] = s;
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-// SpMV as spmvKernel, grid-stride, with the per-thread partial of s . q in
-// the order of the standalone dot product
+// SpMV over the rows of the reduction spans, with each thread's partial of
+// s . q in the order of the standalone dot product (its rows in ascending
+// order, SPMV_ROWS at a time)
+constexpr int SPMV_ROWS = 4;
 template <class T>
-__global__ __launch_bounds__(REDUCE_THREADS) void spmvDotKernel(int m, const int *nbr, const T *diag,
-                                                                const T *ox, const T *oy, const T *oz,
-                                                                const T *mx, const T *my, const T *mz,
-                                                                const T *x, T *y, double *partial) {
+__global__ __launch_bounds__(REDUCE_THREADS) void spmvDotKernel(RowTopo t, const T *__restrict__ diag,
+                                                                const T *__restrict__ ox, const T *__restrict__ oy,
+                                                                const T *__restrict__ oz, const T *__restrict__ x,
+                                                                T *__restrict__ y, double *partial) {
+    const unsigned L = reduceLogicalBlock();
+    const size_t span = reduceSpan((size_t)t.m, gridDim.x);
+    const size_t lo = (size_t)L * span, hi = lo + span < (size_t)t.m ? lo + span : (size_t)t.m;
     double acc = 0.0;
-    const size_t stride = (size_t)gridDim.x * REDUCE_THREADS;
-    for (size_t r = (size_t)blockIdx.x * REDUCE_THREADS + threadIdx.x; r < (size_t)m; r += stride) {
-        const T s = rowProduct<T>(r, nbr, diag, ox, oy, oz, mx, my, mz, x);
-        y[r] = s;
-        acc += (double)x[r] * (double)s;
+    for (size_t b = lo + threadIdx.x; b < hi; b += SPMV_ROWS * REDUCE_THREADS) {
+        int r[SPMV_ROWS];
+        bool ok[SPMV_ROWS];
+#pragma unroll
+        for (int u = 0; u < SPMV_ROWS; u++) {
+            const size_t q = b + (size_t)u * REDUCE_THREADS;
+            ok[u] = q < hi;
+            r[u] = (int)(ok[u] ? q : b);
+        }
+        T s[SPMV_ROWS];
+        rowProducts<T, SPMV_ROWS>(r, t, diag, ox, oy, oz, x, s);
+#pragma unroll
+        for (int u = 0; u < SPMV_ROWS; u++)
+            if (ok[u]) {
+                y[r[u]] = s[u];
+                acc += (double)x[r[u]] * (double)s[u];
+            }
     }
-    blockFold<0>(acc, partial);
+    acc = reduceFold<0>(acc);
+    if (threadIdx.x == 0) partial[L] = acc;
 }
 
 // x += a s; r -= a q with a the float-rounded step, and the partial max |r|
@@ -292,26 +351,27 @@ __global__ __launch_bounds__(REDUCE_THREADS) void pcgStepKernel(TX *x, const T *
                                                                 double *partial, float *rf) {
     const double a = *num / *den;
     const float fa = (float)a, fna = (float)(-a);
+    const unsigned L = reduceLogicalBlock();
+    const size_t span = reduceSpan(n, gridDim.x);
+    const size_t lo = (size_t)L * span, hi = lo + span < n ? lo + span : n;
     double acc = 0.0;
-    const size_t stride = (size_t)gridDim.x * REDUCE_THREADS;
-    for (size_t i = (size_t)blockIdx.x * REDUCE_THREADS + threadIdx.x; i < n; i += stride) {
+    for (size_t i = lo + threadIdx.x; i < hi; i += REDUCE_THREADS) {
         x[i] += (TX)fa * (TX)s[i];
         const T ri = r[i] + (T)fna * q[i];
         r[i] = ri;
         if (rf) rf[i] = (float)ri;  // FP32 copy for an FP32 preconditioner
         acc = fmax(acc, fabs((double)ri));
     }
-    blockFold<1>(acc, partial);
+    acc = reduceFold<1>(acc);
+    if (threadIdx.x == 0) partial[L] = acc;
 }
 
 // r = b - A x
-__global__ void residualKernel(int m, const int *nbr, const double *diag, const double *ox,
-                               const double *oy, const double *oz, const double *mx,
-                               const double *my, const double *mz, const double *b,
-                               const double *x, double *r) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m) return;
-    r[q] = b[q] - rowProduct<double>(q, nbr, diag, ox, oy, oz, mx, my, mz, x);
+__global__ void residualKernel(RowTopo t, const double *diag, const double *ox, const double *oy,
+                               const double *oz, const double *b, const double *x, double *r) {
+    const int q = dieRow();
+    if (q >= t.m) return;
+    r[q] = b[q] - rowProduct<double>(q, t, diag, ox, oy, oz, x);
 }
 
 // BLAS::addScaled(float alpha, ...) (blaswrapper.h:128-157): the step length
@@ -388,10 +448,10 @@ void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, int *cel
 
 void pressureAssemble(Grid g, double dt, const float *u, const float *v, const float *w,
                       const float *wu, const float *wv, const float *ww, const float *phi,
-                      PressureSys &S, hipStream_t s) {
+                      PressureSys &S, hipStream_t s, SolidTerms solid) {
     if (S.m == 0) return;
     hipLaunchKernelGGL(assembleKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, g, dt, u, v, w, wu,
-                       wv, ww, phi, S.mask, S.rowOf, S);
+                       wv, ww, phi, S.mask, S.rowOf, S, solid);
 }
 
 void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s) {
@@ -439,15 +499,22 @@ void mic0ApplyF(const PressureSys &S, const int *hs, const float *r, float *z, h
     micApply<float>(S, hs, S.finvdiag, S.flx, S.fly, S.flz, r, z, s);
 }
 
+static RowTopo topo(const PressureSys &S) { return RowTopo{S.m, S.nbr}; }
+// one row per thread over whole runs of XCDS blocks (so dieRow renumbers them)
+static unsigned dieBlocks(size_t m) {
+    const unsigned b = blocksFor(m, TPB);
+    return (b + XCDS - 1) / XCDS * XCDS;
+}
+
 void spmv(const PressureSys &S, const double *x, double *y, hipStream_t s) {
     if (S.m == 0) return;
-    hipLaunchKernelGGL(spmvKernel<double>, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.nbr, S.diag,
-                       S.ox, S.oy, S.oz, S.mx, S.my, S.mz, x, y);
+    hipLaunchKernelGGL(spmvKernel<double>, dim3(dieBlocks(S.m)), dim3(TPB), 0, s, topo(S), S.diag, S.ox, S.oy,
+                       S.oz, x, y);
 }
 void spmvF(const PressureSys &S, const float *x, float *y, hipStream_t s) {
     if (S.m == 0) return;
-    hipLaunchKernelGGL(spmvKernel<float>, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.nbr, S.fdiag,
-                       S.fox, S.foy, S.foz, S.fmx, S.fmy, S.fmz, x, y);
+    hipLaunchKernelGGL(spmvKernel<float>, dim3(dieBlocks(S.m)), dim3(TPB), 0, s, topo(S), S.fdiag, S.fox, S.foy,
+                       S.foz, x, y);
 }
 
 void axpyFloatStep(double *y, const double *x, size_t n, const double *num, const double *den,
@@ -462,20 +529,20 @@ void axpyFloatStepF(float *y, const float *x, size_t n, const double *num, const
 }
 
 template <class T>
-static void launchSpmvDot(const PressureSys &S, const T *diag, const T *ox, const T *oy, const T *oz, const T *mx,
-                          const T *my, const T *mz, const T *x, T *y, double *partial, double *sq, hipStream_t st) {
+static void launchSpmvDot(const PressureSys &S, const T *diag, const T *ox, const T *oy, const T *oz, const T *x,
+                          T *y, double *partial, double *sq, hipStream_t st) {
     const int nb = reduceBlocks(S.m);
-    hipLaunchKernelGGL(spmvDotKernel<T>, dim3(nb), dim3(REDUCE_THREADS), 0, st, S.m, S.nbr, diag, ox, oy, oz, mx,
-                       my, mz, x, y, partial);
+    hipLaunchKernelGGL(spmvDotKernel<T>, dim3(nb), dim3(REDUCE_THREADS), 0, st, topo(S), diag, ox, oy, oz, x, y,
+                       partial);
     reduceFinish(partial, nb, sq, false, st);
 }
 void spmvDot(const PressureSys &S, const double *x, double *y, double *partial, double *sq, hipStream_t st) {
     if (S.m == 0) return;
-    launchSpmvDot<double>(S, S.diag, S.ox, S.oy, S.oz, S.mx, S.my, S.mz, x, y, partial, sq, st);
+    launchSpmvDot<double>(S, S.diag, S.ox, S.oy, S.oz, x, y, partial, sq, st);
 }
 void spmvDotF(const PressureSys &S, const float *x, float *y, double *partial, double *sq, hipStream_t st) {
     if (S.m == 0) return;
-    launchSpmvDot<float>(S, S.fdiag, S.fox, S.foy, S.foz, S.fmx, S.fmy, S.fmz, x, y, partial, sq, st);
+    launchSpmvDot<float>(S, S.fdiag, S.fox, S.foy, S.foz, x, y, partial, sq, st);
 }
 
 template <class TX, class T>
@@ -515,8 +582,8 @@ void axpyMixed(double *x, const float *p, size_t n, const double *num, const dou
 
 void residual(const PressureSys &S, const double *b, const double *x, double *r, hipStream_t s) {
     if (S.m == 0) return;
-    hipLaunchKernelGGL(residualKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S.m, S.nbr, S.diag, S.ox,
-                       S.oy, S.oz, S.mx, S.my, S.mz, b, x, r);
+    hipLaunchKernelGGL(residualKernel, dim3(dieBlocks(S.m)), dim3(TPB), 0, s, topo(S), S.diag, S.ox, S.oy, S.oz, b,
+                       x, r);
 }
 
 void pressureToGrid(Grid g, const PressureSys &S, const double *x, float *p, hipStream_t s) {

@@ -99,6 +99,7 @@ int reduceBlocks(size_t n) {
     size_t b = (n + RED_THREADS * 8 - 1) / (RED_THREADS * 8);
     if (b < 1) b = 1;
     if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
+    if (b >= XCDS) b -= b % XCDS;  // whole runs of blocks per die
     return (int)b;
 }
 
@@ -107,27 +108,18 @@ namespace {
 template <class T, int OP, class TB = T>  // OP 0: sum of a*b, 1: max |a|
 __global__ __launch_bounds__(RED_THREADS) void reduceStage1(const T *a, const TB *b, size_t n,
                                                             double *partial) {
-    __shared__ double s[RED_THREADS];
+    const unsigned L = reduceLogicalBlock();
+    const size_t span = reduceSpan(n, gridDim.x);
+    const size_t lo = (size_t)L * span, hi = lo + span < n ? lo + span : n;
     double acc = 0.0;
-    size_t stride = (size_t)gridDim.x * RED_THREADS;
-    for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += stride) {
+    for (size_t i = lo + threadIdx.x; i < hi; i += RED_THREADS) {
         if (OP == 0)
             acc += (double)a[i] * (double)b[i];
         else
             acc = fmax(acc, fabs((double)a[i]));
     }
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < (unsigned)off) {
-            if (OP == 0)
-                s[threadIdx.x] += s[threadIdx.x + off];
-            else
-                s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + off]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+    acc = reduceFold<OP>(acc);
+    if (threadIdx.x == 0) partial[L] = acc;
 }
 
 // outHost (optional): a mapped pinned host copy of the result, which the
@@ -135,7 +127,6 @@ __global__ __launch_bounds__(RED_THREADS) void reduceStage1(const T *a, const TB
 template <int OP>
 __global__ __launch_bounds__(RED_THREADS) void reduceStage2(const double *partial, int nb,
                                                             double *out, double *outHost) {
-    __shared__ double s[RED_THREADS];
     double acc = 0.0;
     for (int i = threadIdx.x; i < nb; i += RED_THREADS) {
         if (OP == 0)
@@ -143,20 +134,10 @@ __global__ __launch_bounds__(RED_THREADS) void reduceStage2(const double *partia
         else
             acc = fmax(acc, partial[i]);
     }
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < (unsigned)off) {
-            if (OP == 0)
-                s[threadIdx.x] += s[threadIdx.x + off];
-            else
-                s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + off]);
-        }
-        __syncthreads();
-    }
+    acc = reduceFold<OP>(acc);
     if (threadIdx.x == 0) {
-        *out = s[0];
-        if (outHost) *outHost = s[0];
+        *out = acc;
+        if (outHost) *outHost = acc;
     }
 }
 

@@ -1,0 +1,217 @@
+// kernels_solid.hip -- what a moving solid adds to the pressure solve: the
+// cells' solid volume weights and the conditioning of the solid's face
+// velocities around isolated pockets of liquid.
+//
+// Cell weights (fluidsimulation.cpp:6140-6147, MeshLevelSet::_getCellWeight
+// meshlevelset.cpp:1491-1515): one thread per cell evaluates the volume
+// fraction of the solid in the cell from its eight nodal SDF values with the
+// reference's ten-tetrahedron split (levelsetutils.cpp:213-258) in float.
+//
+// Conditioning (PressureSolver::_conditionSolidVelocityField,
+// pressuresolver.cpp:161-312): the reference grows the groups of connected
+// liquid cells one by one with a stack.  The groups are the connected
+// components of a graph (interior liquid cells; an edge where the shared face
+// has weight >= 1e-6), so the GPU finds them with a lock-free union-find in
+// one pass over the cells' +x/+y/+z edges: every root hooks onto the smaller
+// of two roots with a compare-and-swap, so parent indices only decrease and
+// every find and retry ends.  A second pass counts each component's cells and
+// whether any borders air; a third zeroes the faces of the cells of isolated
+// components of more than one cell.
+#include <stdexcept>
+
+#include "kernels.h"
+
+namespace ff {
+
+namespace {
+
+constexpr int TPB = 256;
+unsigned blocksFor(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// LevelsetUtils::volumeFraction of a tetrahedron (levelsetutils.cpp:213-258)
+__device__ void sort4(float &a, float &b, float &c, float &d) {
+    float t;
+    if (a > b) { t = a; a = b; b = t; }
+    if (c > d) { t = c; c = d; d = t; }
+    if (a > c) { t = a; a = c; c = t; }
+    if (b > d) { t = b; b = d; d = t; }
+    if (b > c) { t = b; b = c; c = t; }
+}
+__device__ float tetFraction(float p0, float p1, float p2, float p3) {
+    return p0 * p0 * p0 / ((p0 - p1) * (p0 - p2) * (p0 - p3));
+}
+__device__ float prismFraction(float p0, float p1, float p2, float p3) {
+    const float a = p0 / (p0 - p2), b = p0 / (p0 - p3), c = p1 / (p1 - p3), d = p1 / (p1 - p2);
+    return a * b * (1 - d) + b * (1 - c) * d + c * d;
+}
+__device__ float volumeFraction4(float p0, float p1, float p2, float p3) {
+    sort4(p0, p1, p2, p3);
+    if (p3 <= 0) return 1;
+    if (p2 <= 0) return 1 - tetFraction(p3, p2, p1, p0);
+    if (p1 <= 0) return prismFraction(p0, p1, p2, p3);
+    if (p0 <= 0) return tetFraction(p0, p1, p2, p3);
+    return 0;
+}
+// the cube's ten tetrahedra, the two central ones counted twice, / 12
+// (levelsetutils.h:60-100)
+__device__ float volumeFraction8(float p000, float p100, float p010, float p110, float p001, float p101, float p011,
+                                 float p111) {
+    return (volumeFraction4(p000, p001, p101, p011) + volumeFraction4(p000, p101, p100, p110) +
+            volumeFraction4(p000, p010, p011, p110) + volumeFraction4(p101, p011, p111, p110) +
+            2 * volumeFraction4(p000, p011, p101, p110) + volumeFraction4(p100, p101, p001, p111) +
+            volumeFraction4(p100, p001, p000, p010) + volumeFraction4(p100, p110, p111, p010) +
+            volumeFraction4(p001, p111, p011, p010) + 2 * volumeFraction4(p100, p111, p001, p010)) /
+           12.0f;
+}
+
+__global__ void cellWeightKernel(Grid g, const float *solid, float *wc) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (size_t)g.ni * g.nj * g.nk) return;
+    const int i = (int)(c % g.ni), j = (int)((c / g.ni) % g.nj), k = (int)(c / ((size_t)g.ni * g.nj));
+    const int pi = g.ni + 1, pj = g.nj + 1;
+    auto P = [&](int a, int b, int d) { return solid[flat(i + a, j + b, k + d, pi, pj)]; };
+    const float p000 = P(0, 0, 0), p001 = P(0, 0, 1), p010 = P(0, 1, 0), p011 = P(0, 1, 1);
+    const float p100 = P(1, 0, 0), p101 = P(1, 0, 1), p110 = P(1, 1, 0), p111 = P(1, 1, 1);
+    float weight;
+    if (p000 < 0 && p001 < 0 && p010 < 0 && p011 < 0 && p100 < 0 && p101 < 0 && p110 < 0 && p111 < 0)
+        weight = 1.0f;
+    else if (p000 >= 0 && p001 >= 0 && p010 >= 0 && p011 >= 0 && p100 >= 0 && p101 >= 0 && p110 >= 0 &&
+             p111 >= 0)
+        weight = 0.0f;
+    else
+        weight = volumeFraction8(p000, p100, p010, p110, p001, p101, p011, p111);
+    // clamp(1 - weight, 0, 1) as max(0, min(v, 1))
+    float v = 1.0f - weight;
+    v = 1.0f < v ? 1.0f : v;
+    wc[c] = 0.0f < v ? v : 0.0f;
+}
+
+// ---- conditioning ----
+
+struct CondView {
+    int ni, nj, nk;
+    const float *phi, *wu, *wv, *ww;
+    __device__ bool border(int i, int j, int k) const {
+        return i == 0 || j == 0 || k == 0 || i == ni - 1 || j == nj - 1 || k == nk - 1;
+    }
+    // a member of the graph: an interior liquid cell
+    __device__ bool member(int i, int j, int k) const { return !border(i, j, k) && phi[flat(i, j, k, ni, nj)] < 0.0f; }
+    __device__ float U(int i, int j, int k) const { return wu[flat(i, j, k, ni + 1, nj)]; }
+    __device__ float V(int i, int j, int k) const { return wv[flat(i, j, k, ni, nj + 1)]; }
+    __device__ float W(int i, int j, int k) const { return ww[flat(i, j, k, ni, nj)]; }
+    __device__ float L(int i, int j, int k) const { return phi[flat(i, j, k, ni, nj)]; }
+    // _computeBordersAirGridThread (pressuresolver.cpp:314-360): an open face
+    // (weight >= 1e-6) to an air cell (phi >= 0)
+    __device__ bool bordersAir(int i, int j, int k) const {
+        const float eps = 1e-6f;
+        return (U(i, j, k) >= 0.0f + eps && L(i - 1, j, k) >= 0.0f) ||
+               (U(i + 1, j, k) >= 0.0f + eps && L(i + 1, j, k) >= 0.0f) ||
+               (V(i, j, k) >= 0.0f + eps && L(i, j - 1, k) >= 0.0f) ||
+               (V(i, j + 1, k) >= 0.0f + eps && L(i, j + 1, k) >= 0.0f) ||
+               (W(i, j, k) >= 0.0f + eps && L(i, j, k - 1) >= 0.0f) ||
+               (W(i, j, k + 1) >= 0.0f + eps && L(i, j, k + 1) >= 0.0f);
+    }
+};
+
+__device__ int loadParent(const int *parent, int x) {
+    return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// root of x with path halving; parent[y] <= y for every member y, so the walk
+// ends, and the halving stores only ever lower an entry to an ancestor
+__device__ int findRoot(int *parent, int x) {
+    int p = loadParent(parent, x);
+    while (p != x) {
+        const int gp = loadParent(parent, p);
+        if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+// hook the larger root onto the smaller; a failed swap returns the larger
+// one's new (smaller) parent, so each retry starts lower and the loop ends
+__device__ void unite(int *parent, int a, int b) {
+    int ra = findRoot(parent, a), rb = findRoot(parent, b);
+    while (ra != rb) {
+        if (ra < rb) {
+            const int t = ra;
+            ra = rb;
+            rb = t;
+        }
+        const int old = atomicCAS(parent + ra, ra, rb);
+        if (old == ra) return;
+        ra = findRoot(parent, old);
+        rb = findRoot(parent, rb);
+    }
+}
+
+__global__ void condInitKernel(CondView v, int *parent, int *info) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (size_t)v.ni * v.nj * v.nk) return;
+    const int i = (int)(c % v.ni), j = (int)((c / v.ni) % v.nj), k = (int)(c / ((size_t)v.ni * v.nj));
+    parent[c] = v.member(i, j, k) ? (int)c : -1;
+    info[c] = 0;
+}
+
+__global__ void condUniteKernel(CondView v, int *parent) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (size_t)v.ni * v.nj * v.nk) return;
+    const int i = (int)(c % v.ni), j = (int)((c / v.ni) % v.nj), k = (int)(c / ((size_t)v.ni * v.nj));
+    if (!v.member(i, j, k)) return;
+    const float eps = 1e-6f;
+    if (v.member(i + 1, j, k) && v.U(i + 1, j, k) >= eps) unite(parent, (int)c, (int)flat(i + 1, j, k, v.ni, v.nj));
+    if (v.member(i, j + 1, k) && v.V(i, j + 1, k) >= eps) unite(parent, (int)c, (int)flat(i, j + 1, k, v.ni, v.nj));
+    if (v.member(i, j, k + 1) && v.W(i, j, k + 1) >= eps) unite(parent, (int)c, (int)flat(i, j, k + 1, v.ni, v.nj));
+}
+
+// info[root] = 2 x (cells) | (any cell borders air)
+__global__ void condCountKernel(CondView v, int *parent, int *info) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (size_t)v.ni * v.nj * v.nk) return;
+    const int i = (int)(c % v.ni), j = (int)((c / v.ni) % v.nj), k = (int)(c / ((size_t)v.ni * v.nj));
+    if (!v.member(i, j, k)) return;
+    const int r = findRoot(parent, (int)c);
+    atomicAdd(info + r, 2);
+    if (v.bordersAir(i, j, k)) atomicOr(info + r, 1);
+}
+
+__global__ void condZeroKernel(CondView v, int *parent, const int *info, float *us, float *vs, float *ws) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (size_t)v.ni * v.nj * v.nk) return;
+    const int i = (int)(c % v.ni), j = (int)((c / v.ni) % v.nj), k = (int)(c / ((size_t)v.ni * v.nj));
+    if (!v.member(i, j, k)) return;
+    const int r = findRoot(parent, (int)c);
+    const int in = info[r];
+    if ((in & 1) != 0 || (in >> 1) <= 1) return;
+    const int ni = v.ni, nj = v.nj;
+    us[flat(i, j, k, ni + 1, nj)] = 0.0f;
+    us[flat(i + 1, j, k, ni + 1, nj)] = 0.0f;
+    vs[flat(i, j, k, ni, nj + 1)] = 0.0f;
+    vs[flat(i, j + 1, k, ni, nj + 1)] = 0.0f;
+    ws[flat(i, j, k, ni, nj)] = 0.0f;
+    ws[flat(i, j, k + 1, ni, nj)] = 0.0f;
+}
+
+}  // namespace
+
+void cellWeights(Grid g, const float *solid, float *wc, hipStream_t s) {
+    const size_t n = (size_t)g.ni * g.nj * g.nk;
+    hipLaunchKernelGGL(cellWeightKernel, dim3(blocksFor(n, TPB)), dim3(TPB), 0, s, g, solid, wc);
+}
+
+void conditionSolidVelocity(Grid g, const float *phi, const float *wu, const float *wv, const float *ww, float *us,
+                            float *vs, float *ws, int *scratch, hipStream_t s) {
+    const size_t n = (size_t)g.ni * g.nj * g.nk;
+    if (n >= (size_t)1 << 30) throw std::runtime_error("conditionSolidVelocity: grid too large for int cell counts");
+    int *parent = scratch, *info = scratch + n;
+    const CondView v{g.ni, g.nj, g.nk, phi, wu, wv, ww};
+    const dim3 grid(blocksFor(n, TPB));
+    hipLaunchKernelGGL(condInitKernel, grid, dim3(TPB), 0, s, v, parent, info);
+    hipLaunchKernelGGL(condUniteKernel, grid, dim3(TPB), 0, s, v, parent);
+    hipLaunchKernelGGL(condCountKernel, grid, dim3(TPB), 0, s, v, parent, info);
+    hipLaunchKernelGGL(condZeroKernel, grid, dim3(TPB), 0, s, v, parent, info, us, vs, ws);
+}
+
+}  // namespace ff

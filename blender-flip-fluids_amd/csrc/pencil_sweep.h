@@ -32,15 +32,16 @@
 // every other waiting lane also gives up on) and the launch still finishes;
 // the host turns the word into an error.
 //
-// Latency.  The operands of the next chunk of C steps (rows from the pencil's
-// line words in LDS, then the right-hand side and the MIC(0) record of each
-// row, and the exchange words) are loaded while the current chunk runs, so a
-// step costs one LDS exchange and one barrier.
-//
-// Arithmetic.  Every row performs the reference's operations in the
-// reference's order (forward: subtract the -z, -y, -x terms, then scale by
-// invdiag; backward: +x, +y, +z, then scale), so the result equals the serial
-// sweeps bit for bit.
+// Staging.  The steps run in chunks of C.  A chunk's operands (the
+// right-hand side and the MIC(0) record of each lane's C cells) are loaded
+// one chunk ahead by a coalesced mapping (C consecutive threads take one
+// line's C consecutive cells, whose rows are consecutive), written into LDS
+// step-major, and read back by each lane at its step; the chunk's results go
+// to LDS and out to memory by the same coalesced mapping when the chunk ends.
+// A step therefore touches only LDS and registers (plus the fire-and-forget
+// publication of a face value), and costs one LDS exchange and one barrier.
+// The exchange words a chunk reads are loaded one chunk ahead too and checked
+// when the chunk starts.
 //
 // This header holds only the sweep's body.  Its includer supplies the
 // hardware operations as macros (the HIP kernel file on the GPU, and a
@@ -111,8 +112,19 @@ FF_PS_DEVICE unsigned long long *pencilFace(unsigned long long *ex, int p, int f
     return ex + ((((unsigned long long)p * 2 + face) * TM + line) * ni) * W;
 }
 
+template <class T, class Rec, int TJ, int TK, int C>
+struct PencilLds {
+    static constexpr unsigned long long bytes(int nw) {
+        return (unsigned long long)TJ * TK * nw * sizeof(LineWord) + (unsigned long long)(TJ + TK) * nw * 4 +
+               (unsigned long long)C * TJ * TK * (sizeof(Rec) + 2 * sizeof(T));
+    }
+};
+
+// sW: TJ*TK*nw line words; sN: (TJ+TK)*nw face-line row bits; sRec, sIn, sOut:
+// C*TJ*TK step-major staging (entry q*TJ*TK + lane)
 template <class T, class Rec, int TJ, int TK, int C, bool FWD>
-FF_PS_DEVICE void pencilSweep(const PencilArgs<T, Rec> &a, const int tid, LineWord *sW, unsigned *sN) {
+FF_PS_DEVICE void pencilSweep(const PencilArgs<T, Rec> &a, const int tid, LineWord *sW, unsigned *sN, Rec *sRec,
+                              T *sIn, T *sOut) {
     static_assert(C % 2 == 0, "the LDS plane parity follows the step within a chunk");
     constexpr int NT = TJ * TK, TM = TJ > TK ? TJ : TK;
     constexpr int W = PencilValue<T>::W;
@@ -198,45 +210,90 @@ FF_PS_DEVICE void pencilSweep(const PencilArgs<T, Rec> &a, const int tid, LineWo
     const unsigned *const bitsY = sN + lk * nw;         // the y face line next to this lane's line
     const unsigned *const bitsZ = sN + (TK + lj) * nw;  // the z face line
 
-    struct Chunk {
-        int row[C];
-        T in[C];
-        Rec rec[C];
-        unsigned long long ey[C][W], ez[C][W];
-        unsigned need;  // bit q: the y face value of step q is read; bit C+q: the z face value
+    // the cell of lane `lane` at step t (t counts from lo forward, from hi
+    // backward), and its row (-1: no row, or outside the grid)
+    auto cellOfLane = [&](int lane, int t) {
+        const int o = lane % TJ + lane / TJ;
+        return (FWD ? lo + t : hi - t) - o;
     };
-    auto cellOf = [&](int t) { return (FWD ? lo + t : hi - t) - off; };
-    auto rowAt = [&](int i) {
-        if (!lineIn || i < 0 || i >= ni) return -1;
-        const LineWord w = sW[tid * nw + (i >> 5)];
+    auto rowOfLane = [&](int lane, int i) {
+        const int jl = PJ * TJ + lane % TJ, kl = PK * TK + lane / TJ;
+        if (jl >= nj || kl >= nk || i < 0 || i >= ni) return -1;
+        const LineWord w = sW[lane * nw + (i >> 5)];
         const unsigned bit = 1u << (i & 31);
         return (w.bits & bit) ? w.base + __builtin_popcount(w.bits & (bit - 1u)) : -1;
     };
     auto faceBit = [&](const unsigned *bits, int i) {
         return i >= 0 && i < ni && ((bits[i >> 5] >> (i & 31)) & 1u) != 0u;
     };
-    // loads of a chunk's operands and exchange words (used one chunk later)
-    auto issue = [&](Chunk &ch, int t0) {
-        ch.need = 0u;
+
+    // operands of one chunk in the coalesced mapping: item e = tid + u*NT is
+    // lane e / C's step e % C
+    struct Stage {  // the record's four values kept apart, so they stay in registers
+        int row[C];
+        T in[C], inv[C], ra[C], rb[C], rc[C];
+    };
+    auto load = [&](Stage &st, int t0) {
 #pragma unroll
-        for (int q = 0; q < C; q++) {
-            const int i = cellOf(t0 + q);
-            const int row = rowAt(i);
-            ch.row[q] = row;
+        for (int u = 0; u < C; u++) {
+            const int e = tid + u * NT, lane = e / C, q = e % C;
+            const int row = rowOfLane(lane, cellOfLane(lane, t0 + q));
+            st.row[u] = row;
             // row 0 stands in for a cell without a row (its operands are not used)
             const int rr = row >= 0 ? row : 0;
-            ch.in[q] = a.in[rr];
-            ch.rec[q] = a.rec[rr];
-            if (row >= 0) {
-                if (conY && faceBit(bitsY, i)) {
-                    ch.need |= 1u << q;
+            st.in[u] = a.in[rr];
+            const Rec rec = a.rec[rr];
+            st.inv[u] = rec.inv;
+            st.ra[u] = rec.a;
+            st.rb[u] = rec.b;
+            st.rc[u] = rec.c;
+        }
+    };
+    auto stage = [&](const Stage &st) {
 #pragma unroll
-                    for (int w = 0; w < W; w++) ch.ey[q][w] = FF_PS_LOAD64(nbY + (unsigned long long)i * W + w);
+        for (int u = 0; u < C; u++) {
+            const int e = tid + u * NT, lane = e / C, q = e % C;
+            sIn[q * NT + lane] = st.in[u];
+            Rec rec;
+            rec.inv = st.inv[u];
+            rec.a = st.ra[u];
+            rec.b = st.rb[u];
+            rec.c = st.rc[u];
+            sRec[q * NT + lane] = rec;
+        }
+    };
+    auto writeOut = [&](const int (&rows)[C]) {
+#pragma unroll
+        for (int u = 0; u < C; u++) {
+            const int e = tid + u * NT, lane = e / C, q = e % C;
+            if (rows[u] >= 0) a.out[rows[u]] = sOut[q * NT + lane];
+        }
+    };
+
+    // exchange words of one chunk (this lane's face cells)
+    struct Faces {
+        unsigned long long ey[C][W], ez[C][W];
+        unsigned need;  // bit q: the y face value of step q is read; bit C+q: the z face value
+    };
+    auto fetch = [&](Faces &fc, int t0) {
+        fc.need = 0u;
+#pragma unroll
+        for (int q = 0; q < C; q++)
+#pragma unroll
+            for (int w = 0; w < W; w++) fc.ey[q][w] = fc.ez[q][w] = 0ull;
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+            const int i = cellOfLane(tid, t0 + q);
+            if (rowOfLane(tid, i) >= 0) {
+                if (conY && faceBit(bitsY, i)) {
+                    fc.need |= 1u << q;
+#pragma unroll
+                    for (int w = 0; w < W; w++) fc.ey[q][w] = FF_PS_LOAD64(nbY + (unsigned long long)i * W + w);
                 }
                 if (conZ && faceBit(bitsZ, i)) {
-                    ch.need |= 1u << (C + q);
+                    fc.need |= 1u << (C + q);
 #pragma unroll
-                    for (int w = 0; w < W; w++) ch.ez[q][w] = FF_PS_LOAD64(nbZ + (unsigned long long)i * W + w);
+                    for (int w = 0; w < W; w++) fc.ez[q][w] = FF_PS_LOAD64(nbZ + (unsigned long long)i * W + w);
                 }
             }
         }
@@ -262,12 +319,12 @@ FF_PS_DEVICE void pencilSweep(const PencilArgs<T, Rec> &a, const int tid, LineWo
             }
         }
     };
-    auto verify = [&](Chunk &ch, int t0) {
+    auto verify = [&](Faces &fc, int t0) {
 #pragma unroll
         for (int q = 0; q < C; q++) {
-            const int i = cellOf(t0 + q);
-            if (ch.need & (1u << q)) await(nbY + (unsigned long long)i * W, ch.ey[q]);
-            if (ch.need & (1u << (C + q))) await(nbZ + (unsigned long long)i * W, ch.ez[q]);
+            const int i = cellOfLane(tid, t0 + q);
+            if (fc.need & (1u << q)) await(nbY + (unsigned long long)i * W, fc.ey[q]);
+            if (fc.need & (1u << (C + q))) await(nbZ + (unsigned long long)i * W, fc.ez[q]);
         }
     };
     T prev = T(0);  // this line's result of the previous step (the x neighbour)
@@ -278,67 +335,85 @@ FF_PS_DEVICE void pencilSweep(const PencilArgs<T, Rec> &a, const int tid, LineWo
 #pragma unroll
         for (int q = 0; q < W; q++) FF_PS_STORE64(dst + q, ((unsigned long long)a.epoch << 32) | b[q]);
     };
-    auto run = [&](const Chunk &ch, int t0) {
+    // the C steps of a chunk: LDS and registers only
+    auto run = [&](const Faces &fc, int t0) {
 #pragma unroll
         for (int q = 0; q < C; q++) {
             const int buf = q & 1, pb = buf ^ 1;  // t0 is even
-            const int row = ch.row[q];
+            const int i = cellOfLane(tid, t0 + q);
+            const bool has = rowOfLane(tid, i) >= 0;
             T v = T(0);
-            if (row >= 0) {
-                const Rec rc = ch.rec[q];
-                T y = ch.in[q];
+            if (has) {
+                const Rec rc = sRec[q * NT + tid];
+                T y = sIn[q * NT + tid];
                 if (FWD) {
                     if (lk > 0) {
                         if (sh[pb][tid - TJ]) y -= rc.c * sv[pb][tid - TJ];
-                    } else if (ch.need & (1u << (C + q))) {
-                        y -= rc.c * PencilValue<T>::unpack(ch.ez[q]);
+                    } else if (fc.need & (1u << (C + q))) {
+                        y -= rc.c * PencilValue<T>::unpack(fc.ez[q]);
                     }
                     if (lj > 0) {
                         if (sh[pb][tid - 1]) y -= rc.b * sv[pb][tid - 1];
-                    } else if (ch.need & (1u << q)) {
-                        y -= rc.b * PencilValue<T>::unpack(ch.ey[q]);
+                    } else if (fc.need & (1u << q)) {
+                        y -= rc.b * PencilValue<T>::unpack(fc.ey[q]);
                     }
                     if (prevHas) y -= rc.a * prev;
                 } else {
                     if (prevHas) y -= rc.a * prev;
                     if (lj < TJ - 1) {
                         if (sh[pb][tid + 1]) y -= rc.b * sv[pb][tid + 1];
-                    } else if (ch.need & (1u << q)) {
-                        y -= rc.b * PencilValue<T>::unpack(ch.ey[q]);
+                    } else if (fc.need & (1u << q)) {
+                        y -= rc.b * PencilValue<T>::unpack(fc.ey[q]);
                     }
                     if (lk < TK - 1) {
                         if (sh[pb][tid + TJ]) y -= rc.c * sv[pb][tid + TJ];
-                    } else if (ch.need & (1u << (C + q))) {
-                        y -= rc.c * PencilValue<T>::unpack(ch.ez[q]);
+                    } else if (fc.need & (1u << (C + q))) {
+                        y -= rc.c * PencilValue<T>::unpack(fc.ez[q]);
                     }
                 }
                 v = y * rc.inv;
-                a.out[row] = v;
-                const int i = cellOf(t0 + q);
+                sOut[q * NT + tid] = v;
                 if (pubY) publish(ownY + (unsigned long long)i * W, v);
                 if (pubZ) publish(ownZ + (unsigned long long)i * W, v);
             }
             sv[buf][tid] = v;
-            sh[buf][tid] = row >= 0 ? 1 : 0;
+            sh[buf][tid] = has ? 1 : 0;
             prev = v;
-            prevHas = row >= 0;
+            prevHas = has;
             FF_PS_BARRIER();
         }
     };
 
-    // ---- the chunks, two register sets in turn: a chunk's loads are issued
-    // before the previous chunk runs and checked after it ----
-    Chunk A, B;
-    issue(A, 0);
+    // ---- the chunks: a chunk's operands and exchange words are loaded while
+    // the previous chunk runs, staged and checked when it has ended, then run;
+    // its results go out when the next chunk starts ----
+    Stage S;
+    Faces FA, FB;  // the current chunk's exchange words and the next one's, in turn
+    int outRows[C];
+    auto chunk = [&](int c, Faces &cur, Faces &next) {
+        const int t0 = c * C;
+        // the previous chunk's last barrier has passed: every lane has read its
+        // operands of that chunk and written its results into sOut
+        if (c > 0) writeOut(outRows);
+        stage(S);
+#pragma unroll
+        for (int u = 0; u < C; u++) outRows[u] = S.row[u];
+        verify(cur, t0);
+        FF_PS_BARRIER();
+        if (c + 1 < nc) {
+            load(S, t0 + C);
+            fetch(next, t0 + C);
+        }
+        run(cur, t0);
+    };
+    load(S, 0);
+    fetch(FA, 0);
     for (int c = 0; c < nc; c += 2) {
-        verify(A, c * C);
-        if (c + 1 < nc) issue(B, (c + 1) * C);
-        run(A, c * C);
+        chunk(c, FA, FB);
         if (c + 1 >= nc) break;
-        verify(B, (c + 1) * C);
-        if (c + 2 < nc) issue(A, (c + 2) * C);
-        run(B, (c + 1) * C);
+        chunk(c + 1, FB, FA);
     }
+    writeOut(outRows);
 }
 
 }  // namespace ff

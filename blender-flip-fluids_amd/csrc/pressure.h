@@ -51,6 +51,9 @@ public:
     // the rows (the previous solve's result) instead of 0.  The stopping
     // test keeps the zero-start threshold, tolerance x |b|_inf.
     void setWarmStart(bool on) { _warm = on; }
+    // the moving solid's face velocities and cell weights for the divergence
+    // of the following solves (all null: a static solid)
+    void setSolidTerms(SolidTerms t) { _solid = t; }
     // the solver's MIC(0) application: 2 = bands of hyperplanes, 4 = the
     // line pipeline (two launches per application)
     void setSchedule(int schedule);
@@ -94,6 +97,7 @@ private:
     int applyLaunches() const { return _sched == 4 ? 2 : _bandLaunches; }
     int _refDots = 0;  // thread count of the reference's dot order, or 0 for the tree reductions
     bool _warm = false;
+    SolidTerms _solid{};
     const float *_prevPressure = nullptr;  // the pressure grid at solve() entry
     // x = 0, r = b; or, warm, x from the pressure grid and r = b - A x
     void initialGuess(hipStream_t st);

@@ -101,6 +101,10 @@ void PressureSolver::allocate(Grid g) {
     _fd = (float *)(b + oFV[3]); _fq = (float *)(b + oFV[4]); _fy = (float *)(b + oFV[5]);
     _pp.order = (int *)(b + oPOrder);
     _pp.words = (LineWord *)(b + oPWords);
+    _S.words = _pp.words;
+    _S.nw = _pp.nw;
+    _S.ni = g.ni;
+    _S.nj = g.nj;
     _pp.ex = (unsigned long long *)(b + oPEx);
     _pp.counter = (unsigned *)(b + oPCounter);
     _pp.error = _ms.error;
@@ -129,8 +133,8 @@ void PressureSolver::allocate(Grid g) {
     _pp.waitTicks = _ms.waitTicks;
     int lds = 0;
     hipCheck(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "device attribute");
-    if ((size_t)lds < pencilLdsBytes(g) + 8192)
-        throw std::runtime_error("the MIC(0) line pipeline needs " + std::to_string(pencilLdsBytes(g) + 8192) +
+    if ((size_t)lds < pencilLdsBytes(g) + 8448)
+        throw std::runtime_error("the MIC(0) line pipeline needs " + std::to_string(pencilLdsBytes(g) + 8448) +
                                  " bytes of LDS per workgroup at this grid size; the device offers " +
                                  std::to_string(lds));
     if ((size_t)lds < micTileLds(true))
@@ -280,10 +284,10 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     _S.m = m;
     res.unknowns = m;
     pressureRows(_g, _S.mask, _S.rowOf, _S.rows, _S.cellRow, s);
-    pressureAssemble(_g, dt, u, v, w, wu, wv, ww, phi, _S, s);
+    pressureAssemble(_g, dt, u, v, w, wu, wv, ww, phi, _S, s, _solid);
     if (cellRhs) gatherRows(_S, cellRhs, _S.rhs, s);
     pressureLevels(_g, _S, scanScratch, s);
-    if (_sched == 4 && m > 0) pencilLineWords(_g, _S, _pp, s);
+    if (m > 0) pencilLineWords(_g, _S, _pp, s);  // SpMV and the line pipeline find neighbour rows in them
     _t->end(T_ASSEMBLE, s);
     _hLevelStart.resize(_S.nLevels + 1);
     hipCheck(hipMemcpyAsync(_hLevelStart.data(), _S.levelStart, sizeof(int) * (_S.nLevels + 1),

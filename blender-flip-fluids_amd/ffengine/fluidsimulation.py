@@ -436,6 +436,18 @@ class FluidSimulation(object):
         pb.execute_lib_func(f, [self(), _fp(u), _fp(v), _fp(w)])
         return u, v, w
 
+    def get_solid_velocity_field(self):
+        """The solid's face velocities of the last solid rebuild (moving
+        obstacles; zero for a static solid)."""
+        i, j, k = self._dims
+        u = np.zeros((k, j, i + 1), np.float32)
+        v = np.zeros((k, j + 1, i), np.float32)
+        w = np.zeros((k + 1, j, i), np.float32)
+        f = lib.FluidSimulation_get_solid_velocity_field
+        pb.init_lib_func(f, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p], None)
+        pb.execute_lib_func(f, [self(), _fp(u), _fp(v), _fp(w)])
+        return u, v, w
+
     def get_last_pressure_solve_stats(self):
         f = lib.FluidSimulation_get_last_pressure_solve_stats
         pb.init_lib_func(f, [c_void_p] + [c_void_p] * 5, None)

@@ -4,7 +4,7 @@ Each runs one HIP kernel group of the FLIP substep on host arrays (copied to
 and from the device) -- the parity tests compare them with the CPU oracle on
 identical inputs.  Grids are returned shaped (k, j, i).
 """
-from ctypes import byref, c_double, c_int, c_void_p
+from ctypes import byref, c_double, c_float, c_int, c_void_p
 
 import numpy as np
 
@@ -120,6 +120,54 @@ def solid_phi_meshes(n3, dx, meshes):
                  [ni, nj, nk, dx, len(meshes), _fp(nv), _fp(verts), _fp(nt), _fp(tris), _fp(inv), _fp(exp), _fp(phi)],
                  c_int)
     return phi, draws
+
+
+def solid_update_moving(n3, dx, objects, frame_dt, progress):
+    """The solid update with moving obstacles.  objects: dicts with prev, cur,
+    next (vertices), tris, and optional animated (default True), inversed,
+    expansion, velocity_scale.  Returns (phi (k, j, i) nodal, (u, v, w) solid
+    face velocities, rand() draws)."""
+    ni, nj, nk = n3
+    cat = lambda key: np.ascontiguousarray(np.concatenate([_f32(o[key]).reshape(-1, 3) for o in objects]),
+                                           np.float32)
+    prev, cur, nxt = cat("prev"), cat("cur"), cat("next")
+    tris = np.ascontiguousarray(np.concatenate([np.asarray(o["tris"], np.int32).reshape(-1, 3) for o in objects]),
+                                np.int32)
+    nv = np.array([len(_f32(o["cur"]).reshape(-1, 3)) for o in objects], np.int32)
+    nt = np.array([len(np.asarray(o["tris"]).reshape(-1, 3)) for o in objects], np.int32)
+    flags = np.array([(1 if o.get("animated", True) else 0) | (2 if o.get("inversed") else 0) for o in objects],
+                     np.int32)
+    exp = np.array([o.get("expansion", 0.0) for o in objects], np.float32)
+    vsc = np.array([o.get("velocity_scale", 1.0) for o in objects], np.float32)
+    phi = np.zeros((nk + 1, nj + 1, ni + 1), np.float32)
+    u = np.zeros((nk, nj, ni + 1), np.float32)
+    v = np.zeros((nk, nj + 1, ni), np.float32)
+    w = np.zeros((nk + 1, nj, ni), np.float32)
+    draws = _run("FFAMD_solid_update_moving", [c_int] * 3 + [c_double, c_int] + [c_void_p] * 9 +
+                 [c_double, c_float] + [c_void_p] * 4,
+                 [ni, nj, nk, dx, len(objects), _fp(nv), _fp(nt), _fp(prev), _fp(cur), _fp(nxt), _fp(tris),
+                  _fp(flags), _fp(exp), _fp(vsc), frame_dt, progress, _fp(phi), _fp(u), _fp(v), _fp(w)], c_int)
+    return phi, (u, v, w), draws
+
+
+def cell_weights(n3, phi_nodes):
+    """1 - the solid's volume fraction of each cell, (k, j, i)."""
+    ni, nj, nk = n3
+    p = _f32(phi_nodes)
+    wc = np.zeros((nk, nj, ni), np.float32)
+    _run("FFAMD_cell_weights", [c_int] * 3 + [c_void_p] * 2, [ni, nj, nk, _fp(p), _fp(wc)])
+    return wc
+
+
+def condition_solid_velocity(n3, liquid_phi, weights, solid_velocity):
+    """The solid face velocities (copies) zeroed around isolated liquid pockets."""
+    ni, nj, nk = n3
+    p = _f32(liquid_phi)
+    wu, wv, ww = (_f32(a) for a in weights)
+    us, vs, ws = (np.array(a, np.float32, copy=True) for a in solid_velocity)
+    _run("FFAMD_condition_solid_velocity", [c_int] * 3 + [c_void_p] * 7,
+         [ni, nj, nk, _fp(p), _fp(wu), _fp(wv), _fp(ww), _fp(us), _fp(vs), _fp(ws)])
+    return us, vs, ws
 
 
 def face_weights(n3, phi_nodes):

@@ -1,9 +1,11 @@
-"""MeshObject -- a static triangle-mesh obstacle of the engine.
+"""MeshObject -- a triangle-mesh obstacle of the engine.
 
 Same class, property names, validation and errors as the reference's
 src/engine/ffengine/meshobject.py over the MeshObject C ABI
-(c_bindings/meshobject_c.cpp).  This engine takes static meshes, normal or
-inversed, with friction; update_mesh_animated raises RuntimeError.
+(c_bindings/meshobject_c.cpp).  This engine takes static and animated
+meshes, normal or inversed, with friction.  The three meshes of
+update_mesh_animated must have the same vertex and triangle counts (a
+changing topology raises RuntimeError).
 """
 from ctypes import c_double, c_float, c_int, c_void_p
 

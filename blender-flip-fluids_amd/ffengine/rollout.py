@@ -10,6 +10,11 @@ initial face grids, recomputing substeps from particle checkpoints kept
 every `checkpoint_every` substeps.  The liquid SDF and P2G flags of each
 substep are those of the last forward with freeze=False and are held fixed
 by the backward; the collision response is taken as the identity there.
+
+The solid (the domain walls, or solid_nodes: a nodal solid SDF such as a
+static MeshObject's) is frictionless: the velocity constraint takes friction
+0, the reference's default (fluidsimulation.h:2227).  A solid whose objects
+carry friction is not differentiated: solid_friction != 0 raises ValueError.
 """
 from ctypes import byref, c_double, c_int, c_longlong, c_void_p
 
@@ -38,7 +43,10 @@ def _call(name, argtypes, args, restype=None):
 
 class Rollout:
     def __init__(self, n3, dx, pos, dt=1.0 / 120.0, substeps=1, checkpoint_every=1, picflip=0.05,
-                 gravity=(0.0, -9.81, 0.0), cfl=5, tol=1e-9, max_iter=900, solid_nodes=None):
+                 gravity=(0.0, -9.81, 0.0), cfl=5, tol=1e-9, max_iter=900, solid_nodes=None, solid_friction=0.0):
+        if solid_friction != 0.0:
+            raise ValueError("Rollout: the solid's friction must be 0 (the rollout's velocity constraint and its "
+                             "adjoint are those of a frictionless solid)")
         self.n3 = tuple(int(a) for a in n3)
         self.dx = float(dx)
         self._pos = _f32(pos).reshape(-1, 3)

@@ -207,15 +207,21 @@ void FluidSimulation_set_pressure_solver_preconditioner_schedule(FluidSimulation
                                                                  int *err);
 int FluidSimulation_get_pressure_solver_preconditioner_schedule(FluidSimulation *obj, int *err);
 
-/* ---- static mesh obstacles ------------------------------------------------
+/* ---- mesh obstacles --------------------------------------------------------
  * MeshObject (meshobject.h) and its C ABI (c_bindings/meshobject_c.cpp, lines
  * cited per entry); the mesh is MeshUtils::TriangleMesh_t (meshutils.h:34-39).
  * Obstacles enter the solid SDF through FluidSimulation_add_mesh_obstacle
  * (fluidsimulation_c.cpp:3113-3135); the solid is rebuilt at the next
- * substep when the list or an object's enabled state changed.  Supported:
- * static meshes, normal and inversed, with friction (the velocity
- * constraint's closest-object friction).  Animated meshes fail with an
- * error. */
+ * substep when the list or an object's enabled state changed, and at every
+ * substep while an enabled animated object's mesh moves (its previous and
+ * current frame differ), at the substep's frame progress
+ * (fluidsimulation.cpp:5219-5426).  Static and animated meshes, normal and
+ * inversed, with friction (the velocity constraint's closest-object
+ * friction).  An animated object's solid velocity (from its vertices'
+ * frame-to-frame translations times the velocity scale) enters the pressure
+ * solve and the velocity constraint.  The three meshes of an animated update
+ * must have the same vertex and triangle counts (the reference also accepts
+ * a changing topology; this engine rejects it with an error). */
 typedef struct MeshObject MeshObject;
 typedef struct TriangleMesh_t {
     float *vertices;
@@ -283,6 +289,11 @@ void FluidSimulation_set_solid_phi(FluidSimulation *obj, const float *phi_nodes,
  * v isize*(jsize+1)*ksize, w isize*jsize*(ksize+1). */
 void FluidSimulation_get_liquid_sdf_data(FluidSimulation *obj, float *data, int *err);
 void FluidSimulation_get_solid_sdf_data(FluidSimulation *obj, float *data, int *err);
+/* the solid's face velocities of the last solid rebuild (MeshLevelSet velocity
+ * grids, normalized and extrapolated; zero unless an animated obstacle took
+ * part), u/v/w sized as the velocity field.  No counterpart in the reference's
+ * C ABI: a getter for tests and tools. */
+void FluidSimulation_get_solid_velocity_field(FluidSimulation *obj, float *u, float *v, float *w, int *err);
 void FluidSimulation_get_velocity_field_data(FluidSimulation *obj, float *u, float *v, float *w,
                                              int *err);
 void FluidSimulation_get_pressure_grid_data(FluidSimulation *obj, float *data, int *err);
@@ -351,6 +362,30 @@ int FFAMD_mesh_levelset_fast(int isize, int jsize, int ksize, double dx, int oi,
 int FFAMD_solid_phi_meshes(int isize, int jsize, int ksize, double dx, int nobj, const int *nv,
                            const float *vertices, const int *nt, const int *triangles, const int *inversed,
                            const float *expansion, float *phi_nodes, int *err);
+/* FluidSimulation::_updateSolidLevelSet with moving obstacles
+ * (fluidsimulation.cpp:5219-5426; MeshObject::getMesh(f) and
+ * getVertexVelocities(dt, f), meshobject.cpp:158-220; the MeshLevelSet
+ * velocity data, meshlevelset.cpp:168-205, 577-718, 1320-1425, 1568-1845):
+ * obstacle q has nv[q] vertices in three frames (prev, cur, next; one
+ * concatenated array each) and nt[q] triangles (indices local to it);
+ * flags[q] bit 0 animated, bit 1 inversed; expansion and velocity_scale per
+ * object (optional).  At frame_progress of a frame of frame_dt seconds: the
+ * nodal solid phi and the normalized, extrapolated solid face velocities
+ * u, v, w.  Returns the rand() draws. */
+int FFAMD_solid_update_moving(int isize, int jsize, int ksize, double dx, int nobj, const int *nv,
+                              const int *nt, const float *prev, const float *cur, const float *next,
+                              const int *triangles, const int *flags, const float *expansion,
+                              const float *velocity_scale, double frame_dt, float frame_progress,
+                              float *phi_nodes, float *u, float *v, float *w, int *err);
+/* the cells' solid weights 1 - volume fraction of the solid
+ * (fluidsimulation.cpp:6140-6147, meshlevelset.cpp:1491-1515) */
+void FFAMD_cell_weights(int isize, int jsize, int ksize, const float *phi_nodes, float *wc, int *err);
+/* PressureSolver::_conditionSolidVelocityField (pressuresolver.cpp:161-312):
+ * the solid face velocities us, vs, ws (in place) zeroed around isolated
+ * pockets of the liquid (phi cell-centred) */
+void FFAMD_condition_solid_velocity(int isize, int jsize, int ksize, const float *liquid_phi,
+                                    const float *wu, const float *wv, const float *ww, float *us,
+                                    float *vs, float *ws, int *err);
 void FFAMD_face_weights(int isize, int jsize, int ksize, const float *phi_nodes, float *wu,
                         float *wv, float *ww, int *err);
 /* Pressure solve + projection on host grids; mode as in

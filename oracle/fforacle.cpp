@@ -2294,6 +2294,17 @@ int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt, const fl
                              status, err, nullptr, nullptr, nullptr, nullptr);
 }
 
+void oracle_cell_weights(int ni, int nj, int nk, const float *solid_phi_nodes, float *wc) {
+    std::vector<float> out;
+    cellWeights(ni, nj, nk, solid_phi_nodes, out);
+    std::copy(out.begin(), out.end(), wc);
+}
+
+void oracle_condition_solid_velocity(int ni, int nj, int nk, const float *phi, const float *wu, const float *wv,
+                                     const float *ww, float *us, float *vs, float *ws) {
+    conditionSolidVelocity(ni, nj, nk, phi, wu, wv, ww, us, vs, ws);
+}
+
 /* PressureSolver::applySolutionToVelocityField (pressuresolver.cpp:94-112) and
  * _applyPressureToVelocityFieldThread (1047-1205), surface tension off, density 1. */
 void oracle_apply_pressure(int ni, int nj, int nk, double dx, double dt, const float *p,
@@ -3103,9 +3114,10 @@ void oracle_sim_get_grids(OracleSim *s, float *phi, float *u, float *v, float *w
  * product from its own derivative, evaluated at the forward's states, with
  * the liquid SDF and P2G flags of the recording forward held fixed and the
  * collision response taken as the identity.  The forward's float roundings
- * are not differentiated, the transfer weights and sums are re-evaluated in
- * double, and each pressure solve's A^-1 is applied by an FP64 MIC(0)-CG run
- * to a relative residual of 1e-13. */
+ * are not differentiated; the P2G derivative takes the forward's own float
+ * transfer weights, weight sums and face values (the quantities it divided by
+ * and stored), and each pressure solve's A^-1 is applied by an FP64
+ * MIC(0)-CG run to a relative residual of 1e-13. */
 namespace {
 
 constexpr unsigned char XS_BORDER = 254, XS_UNKNOWN = 255;
@@ -3571,49 +3583,66 @@ struct OracleRollout {
     // P2G's adjoint for the particles x (velocities v): adds to vOut the
     // velocity cotangent sum_f (w_pf / W_f or w_pf) ubar_f and to xOut the
     // position cotangent sum_f ubar_f ((v_p - u_f) / W_f or v_p) w'(d2) (-2)(f - x_p)
+    // The P2G derivative at the forward's own transfer: each particle-face
+    // weight is the float weight the forward summed (p2gFlip's expression in
+    // the face block's frame), and W_f, u_f the float weight sum and value the
+    // forward divided by and stored.  Near-empty faces (W_f a few 1e-5) take
+    // weights from float expressions with heavy cancellation; the derivative
+    // is that of the map the forward actually evaluated, in double.
     void p2gAdjoint(int k, const std::vector<V3> &x, const std::vector<V3> &v, const std::vector<double> *ub,
                     std::vector<double> &xOut, std::vector<double> &vOut) {
         const int ni = sim.ni, nj = sim.nj, nk = sim.nk, np = (int)x.size();
         const double dx = sim.dx, h = 0.5 * dx;
         const Kernel64 Kw(sim.liquidRadius());
         const double r = std::sqrt(Kw.rsq);
+        const int bw = 10;
+        const float rf = (float)sim.liquidRadius();
+        const float rsq = rf * rf;
+        const float coef1 = (4.0f / 9.0f) * (1.0f / (rf * rf * rf * rf * rf * rf));
+        const float coef2 = (17.0f / 9.0f) * (1.0f / (rf * rf * rf * rf));
+        const float coef3 = (22.0f / 9.0f) * (1.0f / (rf * rf));
         const unsigned char *rv = recValid[k].data();
+        std::vector<float> pos(3 * (size_t)np), vel(3 * (size_t)np);
+        for (int p = 0; p < np; p++) {
+            pos[3 * (size_t)p] = x[p].x; pos[3 * (size_t)p + 1] = x[p].y; pos[3 * (size_t)p + 2] = x[p].z;
+            vel[3 * (size_t)p] = v[p].x; vel[3 * (size_t)p + 1] = v[p].y; vel[3 * (size_t)p + 2] = v[p].z;
+        }
         for (int d = 0; d < 3; d++) {
             const FaceDims D = faceDims(ni, nj, nk, d);
             const double off[3] = {d == 0 ? 0.0 : h, d == 1 ? 0.0 : h, d == 2 ? 0.0 : h};
-            // the face sums in double
-            std::vector<double> W(D.n(), 0.0), S(D.n(), 0.0);
-            auto faces = [&](int p, auto &&body) {
+            const V3 offset = mk(off[0], off[1], off[2]);
+            // the forward's float weight sums and values
+            std::vector<float> Wf(D.n()), uf(D.n());
+            std::vector<unsigned char> vf(D.n());
+            p2gFlip(ni, nj, nk, dx, np, pos.data(), vel.data(), sim.liquidRadius(), d, uf.data(), vf.data(),
+                    Wf.data());
+            for (int p = 0; p < np; p++) {
+                const double vd = d == 0 ? v[p].x : (d == 1 ? v[p].y : v[p].z);
                 const double ps[3] = {(double)x[p].x - off[0], (double)x[p].y - off[1], (double)x[p].z - off[2]};
+                const V3 xs = x[p] - offset;
                 int lo[3], hi[3];
                 p2gFaceRange(D, dx, r, ps, lo, hi);
+                double vbAcc = 0.0, xbAcc[3] = {0.0, 0.0, 0.0};
                 for (int fk = lo[2]; fk <= hi[2]; fk++)
                     for (int fj = lo[1]; fj <= hi[1]; fj++)
                         for (int fi = lo[0]; fi <= hi[0]; fi++) {
-                            const double e[3] = {fi * dx - ps[0], fj * dx - ps[1], fk * dx - ps[2]};
-                            const double d2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
-                            if (!(d2 < Kw.rsq)) continue;
-                            body(flat(fi, fj, fk, D.wi, D.wj), d2, e);
+                            // the forward's frame: the face's 10^3 block
+                            const V3 bo = indexToPos(fi / bw, fj / bw, fk / bw, bw * dx);
+                            const V3 pl = xs - bo;
+                            const V3 e = indexToPos(fi - fi / bw * bw, fj - fj / bw * bw, fk - fk / bw * bw, dx) - pl;
+                            const float d2 = dot(e, e);
+                            if (!(d2 < rsq)) continue;
+                            const float wgt = 1.0f - coef1 * d2 * d2 * d2 + coef2 * d2 * d2 - coef3 * d2;
+                            const size_t f = flat(fi, fj, fk, D.wi, D.wj);
+                            const bool norm = rv[f] != 0 && Wf[f] > 0.0f;
+                            const double W = Wf[f];
+                            vbAcc += (norm ? (double)wgt / W : (double)wgt) * ub[d][f];
+                            const double sl = (norm ? (vd - (double)uf[f]) / W : vd) * ub[d][f] * Kw.dw((double)d2) *
+                                              -2.0;
+                            xbAcc[0] += sl * (double)e.x;
+                            xbAcc[1] += sl * (double)e.y;
+                            xbAcc[2] += sl * (double)e.z;
                         }
-            };
-            for (int p = 0; p < np; p++) {
-                const double vd = d == 0 ? v[p].x : (d == 1 ? v[p].y : v[p].z);
-                faces(p, [&](size_t f, double d2, const double *) {
-                    W[f] += Kw.w(d2);
-                    S[f] += Kw.w(d2) * vd;
-                });
-            }
-            for (int p = 0; p < np; p++) {
-                const double vd = d == 0 ? v[p].x : (d == 1 ? v[p].y : v[p].z);
-                double vbAcc = 0.0, xbAcc[3] = {0.0, 0.0, 0.0};
-                faces(p, [&](size_t f, double d2, const double *e) {
-                    const bool norm = rv[f] != 0 && W[f] > 0.0;
-                    const double wgt = Kw.w(d2);
-                    vbAcc += (norm ? wgt / W[f] : wgt) * ub[d][f];
-                    const double uf = norm ? S[f] / W[f] : S[f];
-                    const double sl = (norm ? (vd - uf) / W[f] : vd) * ub[d][f] * Kw.dw(d2) * -2.0;
-                    for (int c = 0; c < 3; c++) xbAcc[c] += sl * e[c];
-                });
                 vOut[3 * (size_t)p + d] += vbAcc;
                 for (int c = 0; c < 3; c++) xOut[3 * (size_t)p + c] += xbAcc[c];
             }

@@ -93,6 +93,15 @@ void oracle_face_weights(int ni, int nj, int nk, const float *solid_phi_nodes,
  * status[1] = success flag (after the acceptable-tolerance rule),
  * err[0] = |r|_inf at exit.  nthreads mimics the reference's BLAS chunking
  * for the FP64 dot products (blaswrapper.h:27-57). */
+/* The weight grid's cell weights: 1 - the solid's volume fraction per cell,
+ * clamped to [0, 1] (fluidsimulation.cpp:6140-6147, meshlevelset.cpp:1491-1515,
+ * levelsetutils.cpp:213-258). */
+void oracle_cell_weights(int ni, int nj, int nk, const float *solid_phi_nodes, float *wc);
+/* PressureSolver::_conditionSolidVelocityField (pressuresolver.cpp:161-312):
+ * the solid face velocities us, vs, ws zeroed in place around isolated
+ * pockets of liquid (phi cell-centred, face weights wu, wv, ww). */
+void oracle_condition_solid_velocity(int ni, int nj, int nk, const float *phi, const float *wu, const float *wv,
+                                     const float *ww, float *us, float *vs, float *ws);
 int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt,
                           const float *u, const float *v, const float *w,
                           const float *wu, const float *wv, const float *ww,

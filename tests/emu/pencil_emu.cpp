@@ -70,6 +70,8 @@ static void sweep(ff::PencilArgs<T, Rec<T>> a, int np, const std::vector<int> &l
     constexpr int NT = TJ * TK;
     std::vector<LineWord> sW((size_t)NT * a.nw);
     std::vector<unsigned> sN((size_t)(TJ + TK) * a.nw);
+    std::vector<Rec<T>> sRec((size_t)C * NT);
+    std::vector<T> sIn((size_t)C * NT), sOut((size_t)C * NT);
     std::barrier<> bar(NT);
     gBarrier = &bar;
     for (int w : launch) {
@@ -77,7 +79,8 @@ static void sweep(ff::PencilArgs<T, Rec<T>> a, int np, const std::vector<int> &l
         std::vector<std::thread> th;
         th.reserve(NT);
         for (int t = 0; t < NT; t++)
-            th.emplace_back([&, t] { ff::pencilSweep<T, Rec<T>, TJ, TK, C, FWD>(a, t, sW.data(), sN.data()); });
+            th.emplace_back([&, t] { ff::pencilSweep<T, Rec<T>, TJ, TK, C, FWD>(a, t, sW.data(), sN.data(), sRec.data(),
+                                                                              sIn.data(), sOut.data()); });
         for (auto &x : th) x.join();
     }
     (void)np;

@@ -291,6 +291,24 @@ def face_weights(n3, phi_nodes):
     return wu, wv, ww
 
 
+def cell_weights(n3, phi_nodes):
+    """1 - the solid's volume fraction of each cell, (k, j, i)."""
+    ni, nj, nk = n3
+    wc = np.zeros((nk, nj, ni), np.float32)
+    oracle().oracle_cell_weights(ni, nj, nk, fp(_f32(phi_nodes)), fp(wc))
+    return wc
+
+
+def condition_solid_velocity(n3, liquid_phi, weights, solid_velocity):
+    """The solid face velocities (copies) zeroed around isolated liquid pockets."""
+    ni, nj, nk = n3
+    wu, wv, ww = (_f32(a) for a in weights)
+    us, vs, ws = (np.array(a, np.float32, copy=True) for a in solid_velocity)
+    oracle().oracle_condition_solid_velocity(ni, nj, nk, fp(_f32(liquid_phi)), fp(wu), fp(wv), fp(ww), fp(us),
+                                             fp(vs), fp(ws))
+    return us, vs, ws
+
+
 def pressure_solve(n3, dx, dt, uvw, weights, liquid_phi, tol=1e-9, max_iter=900, nthreads=1):
     ni, nj, nk = n3
     u, v, w = (_f32(a) for a in uvw)

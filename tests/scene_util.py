@@ -80,3 +80,52 @@ def flow_statistics(pos, vel, n, blocks=4):
     return {"count": int(len(p)), "centroid": [float(c) for c in p.mean(axis=0)],
             "mean_sq_speed": float((v * v).sum(axis=1).mean()),
             "occupancy": [float(o) for o in occ / max(len(p), 1)]}
+
+
+def _moving(center, radius, sub, d, **kw):
+    from ffengine import scenes
+    v, t = scenes.sphere_mesh(center, radius, sub)
+    d = np.asarray(d, np.float32)
+    return dict(prev=v - d, cur=v, next=v + d, tris=t, **kw)
+
+
+def _static(center, radius, sub, **kw):
+    from ffengine import scenes
+    v, t = scenes.sphere_mesh(center, radius, sub)
+    return dict(prev=v, cur=v, next=v, tris=t, animated=False, **kw)
+
+
+def _two_islands(d):
+    from ffengine import scenes
+    a, ta = scenes.sphere_mesh((0.3, 0.3, 0.3), 0.12, 1)
+    b, tb = scenes.sphere_mesh((0.05, 0.7, 0.6), 0.15, 1)  # crosses the x = 0 border
+    v = np.concatenate([a, b])
+    t = np.concatenate([ta, tb + len(a)])
+    d = np.asarray(d, np.float32)
+    return dict(prev=v - d, cur=v, next=v + 2 * d, tris=t)
+
+
+def moving_obstacle_cases():
+    """Obstacle sets of the solid update with moving obstacles: dicts of
+    prev, cur, next vertices, tris and flags (animated, inversed,
+    velocity_scale).  An animated sphere beside a static one; a scaled
+    animated sphere inside an inversed animated container; one animated mesh
+    of two islands, one crossing the domain's border."""
+    return {
+        "animated-and-static": [_moving((0.45, 0.35, 0.5), 0.15, 2, (0.02, 0.01, -0.015)),
+                                _static((0.7, 0.6, 0.4), 0.1, 1)],
+        "scaled-and-inversed": [_moving((0.5, 0.4, 0.5), 0.12, 2, (0.0, -0.03, 0.01), velocity_scale=1.5),
+                                _moving((0.5, 0.5, 0.5), 0.6, 2, (0.01, 0.0, 0.0), inversed=True)],
+        "islands-across-border": [_two_islands((0.015, -0.01, 0.02))],
+    }
+
+
+# frame timings (frame dt, time remaining in the frame after the substep,
+# substep length) of the solid-update cases: progress 0, inside the frame, and
+# clamped at 0
+SOLID_UPDATE_TIMINGS = [(1 / 30, 1 / 30, 1 / 120), (1 / 30, 0.5 / 30, 1 / 240), (1 / 24, 1 / 96, 1 / 96)]
+
+
+def frame_progress(frame_dt, remaining, step):
+    """1 - (float)(remaining + step) / (float)frame_dt, in float."""
+    return float(np.float32(1.0) - np.float32(remaining + step) / np.float32(frame_dt))

@@ -117,6 +117,67 @@ def test_c1_sixty_frames_match_reference_bits():
         assert _sha(v) == fr["velocities_sha256"], fr["frame"]
 
 
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision", ["fp64", "fp32-preconditioner"])
+def test_c1_sixty_frames_flow_matches_reference(precision):
+    """The 64^3 dam break for 60 frames with the solvers the benchmark times:
+    FP64 PCG with the parallel-tree dot products, and FP64 PCG with the FP32
+    MIC(0) application.  Their sums are ordered differently from the
+    reference's, and the flow amplifies such rounding-level differences, so
+    particles cannot be compared one by one after the first frames.  The flow
+    observables (particle count, centroid, mean squared speed, occupancy of
+    4^3 sub-boxes; scene_util.flow_statistics) are compared with the reference
+    engine's every frame (tests/golden/e2e_c1_n64_stats.json).  The fixture
+    also holds the oracle run from the same particles with every x coordinate
+    moved by one ulp: its distance from the reference is how far the flow
+    carries a rounding-level difference.  The GPU runs make such differences
+    at every substep rather than once, so theirs grow a few frames earlier:
+    the tolerance at frame f is 3x the largest one-ulp distance up to frame
+    f + 5, plus a floor (10 particles, 0.01 cells, 1e-4 relative speed, 1e-4
+    occupancy)."""
+    import json
+    import os
+    from scene_util import flow_statistics
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "e2e_c1_n64_stats.json")) as fh:
+        rec = json.load(fh)
+    n = 64
+    sim = make_gpu(n, _golden_dam_break(n, 0), precision=precision, tol=1e-9)
+    frames = list(zip(rec["reference"], rec["oracle_one_ulp"]))
+    env = {"count": [], "centroid": [], "speed": [], "occupancy": []}
+    running = {key: 0.0 for key in env}
+    for ref, ulp in frames:  # the one-ulp spread up to each frame
+        running["count"] = max(running["count"], abs(ref["count"] - ulp["count"]))
+        running["centroid"] = max(running["centroid"],
+                                  max(abs(a - b) for a, b in zip(ref["centroid"], ulp["centroid"])))
+        running["speed"] = max(running["speed"],
+                               abs(ref["mean_sq_speed"] - ulp["mean_sq_speed"]) / ref["mean_sq_speed"])
+        running["occupancy"] = max(running["occupancy"],
+                                   max(abs(a - b) for a, b in zip(ref["occupancy"], ulp["occupancy"])))
+        for key in env:
+            env[key].append(running[key])
+    for f, (ref, ulp) in enumerate(frames):
+        sim.update(1.0 / 30.0)
+        st = sim.get_frame_stats_data()
+        assert st.pressure_solver_success, f + 1
+        p, v = gpu_particles(sim)
+        got = flow_statistics(p, v, n)
+        ahead = min(f + 5, len(frames) - 1)
+        spread = {key: env[key][ahead] for key in env}
+        d_count = abs(got["count"] - ref["count"])
+        d_cent = max(abs(a - b) for a, b in zip(got["centroid"], ref["centroid"]))
+        d_speed = abs(got["mean_sq_speed"] - ref["mean_sq_speed"]) / ref["mean_sq_speed"]
+        d_occ = max(abs(a - b) for a, b in zip(got["occupancy"], ref["occupancy"]))
+        say("C1 %s frame %d: count %d (ref %d), centroid %.3g cells (tol %.3g), speed %.3g (tol %.3g), "
+            "occupancy %.3g (tol %.3g)" % (precision, f + 1, got["count"], ref["count"], d_cent,
+                                           3 * spread["centroid"] + 0.01, d_speed, 3 * spread["speed"] + 1e-4,
+                                           d_occ, 3 * spread["occupancy"] + 1e-4))
+        assert d_count <= 3 * spread["count"] + 10, f + 1
+        assert d_cent <= 3 * spread["centroid"] + 0.01, f + 1
+        assert d_speed <= 3 * spread["speed"] + 1e-4, f + 1
+        assert d_occ <= 3 * spread["occupancy"] + 1e-4, f + 1
+
+
 def test_solid_phi_matches_oracle():
     n = 24
     sim = make_gpu(n, scenes.dam_break_particles(n, seed=1))

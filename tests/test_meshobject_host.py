@@ -72,7 +72,8 @@ def test_mesh_object_validation():
     v, t = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.25, 0)
     with pytest.raises(RuntimeError):
         mo.update_mesh_static(TriangleMesh(v, np.array([[0, 1, len(v)]], np.int32)))
-    with pytest.raises(RuntimeError):
-        mo.update_mesh_animated(TriangleMesh(v, t), TriangleMesh(v, t), TriangleMesh(v, t))
+    mo.update_mesh_animated(TriangleMesh(v, t), TriangleMesh(v, t), TriangleMesh(v, t))
+    with pytest.raises(RuntimeError):  # a topology that changes between frames
+        mo.update_mesh_animated(TriangleMesh(v[:-1], t[:-1]), TriangleMesh(v, t), TriangleMesh(v, t))
     with pytest.raises(RuntimeError):
         MeshObject(0, 8, 8, 0.125)

@@ -14,37 +14,12 @@ import pytest
 
 import oracle_lib as O
 from ffengine import scenes
+from scene_util import SOLID_UPDATE_TIMINGS, frame_progress, moving_obstacle_cases
 
 pytestmark = pytest.mark.skipif(not O.reference_available(), reason="reference build (oracle/_ref) not present")
 
 
-def _moving(center, radius, sub, d, **kw):
-    v, t = scenes.sphere_mesh(center, radius, sub)
-    d = np.asarray(d, np.float32)
-    return dict(prev=v - d, cur=v, next=v + d, tris=t, **kw)
-
-
-def _static(center, radius, sub, **kw):
-    v, t = scenes.sphere_mesh(center, radius, sub)
-    return dict(prev=v, cur=v, next=v, tris=t, animated=False, **kw)
-
-
-def _two_islands(d):
-    a, ta = scenes.sphere_mesh((0.3, 0.3, 0.3), 0.12, 1)
-    b, tb = scenes.sphere_mesh((0.05, 0.7, 0.6), 0.15, 1)  # crosses the x = 0 border
-    v = np.concatenate([a, b])
-    t = np.concatenate([ta, tb + len(a)])
-    d = np.asarray(d, np.float32)
-    return dict(prev=v - d, cur=v, next=v + 2 * d, tris=t)
-
-
-CASES = {
-    "animated-and-static": [_moving((0.45, 0.35, 0.5), 0.15, 2, (0.02, 0.01, -0.015)),
-                            _static((0.7, 0.6, 0.4), 0.1, 1)],
-    "scaled-and-inversed": [_moving((0.5, 0.4, 0.5), 0.12, 2, (0.0, -0.03, 0.01), velocity_scale=1.5),
-                            _moving((0.5, 0.5, 0.5), 0.6, 2, (0.01, 0.0, 0.0), inversed=True)],
-    "islands-across-border": [_two_islands((0.015, -0.01, 0.02))],
-}
+CASES = moving_obstacle_cases()
 
 
 def _reference(n, objects, seed, timings):
@@ -70,11 +45,11 @@ def _reference(n, objects, seed, timings):
 def test_solid_update_matches_reference(case):
     n, seed = 24, 5
     objects = CASES[case]
-    timings = [(1 / 30, 1 / 30, 1 / 120), (1 / 30, 0.5 / 30, 1 / 240), (1 / 24, 1 / 96, 1 / 96)]
+    timings = SOLID_UPDATE_TIMINGS
     want = _reference(n, objects, seed, timings)
     moving = 0.0
     for (frame_dt, remaining, step), (rphi, rvel) in zip(timings, want):
-        progress = float(np.float32(1.0) - np.float32(remaining + step) / np.float32(frame_dt))
+        progress = frame_progress(frame_dt, remaining, step)
         phi, vel, used = O.solid_update((n, n, n), 1.0 / n, objects, frame_dt, progress, seed)
         assert used > 0
         assert np.array_equal(phi.view(np.uint32), rphi.view(np.uint32)), (case, progress)
@@ -84,10 +59,11 @@ def test_solid_update_matches_reference(case):
     assert moving > 0.1  # the obstacles do move
 
 
-def _frames(n, frames, seed, make_objects, both=True):
+def _frames(n, frames, seed, make_objects, inversed=None):
     """The dam break at n^3 with mesh obstacles, frames of 1/30 s through the
     reference engine and the oracle; each frame's obstacle meshes come from
-    make_objects(frame) as (prev, cur, next, tris, animated)."""
+    make_objects(frame) as (prev, cur, next, tris, animated); inversed: the
+    objects' flags (default none)."""
     dx = 1.0 / n
     n3 = (n, n, n)
     pos = scenes.dam_break_particles(n, seed=1)
@@ -96,16 +72,20 @@ def _frames(n, frames, seed, make_objects, both=True):
     orc = O.OracleSim(n3, dx)
     orc.load(pos)
     handles = []
-    for prev, cur, nxt, tris, animated in make_objects(0):
+    objs = make_objects(0)
+    inversed = inversed or [False] * len(objs)
+    for (prev, cur, nxt, tris, animated), inv in zip(objs, inversed):
         if animated:
-            h = ref.add_animated_obstacle(n3, dx, prev, cur, nxt, tris)
+            h = ref.add_animated_obstacle(n3, dx, prev, cur, nxt, tris, inv)
         else:
-            h = ref.add_mesh_obstacle(n3, dx, cur, tris)
-        handles.append((h, orc.add_moving_obstacle(prev, cur, nxt, tris, animated=animated)))
+            h = ref.add_mesh_obstacle(n3, dx, cur, tris, inv)
+        handles.append((h, orc.add_moving_obstacle(prev, cur, nxt, tris, animated=animated, inversed=inv)))
     ref.initialize()
     orc.initialize()
-    O.libc_srand(seed)
+    # the oracle's list of rand() values first: making it draws from the C
+    # library, which is then seeded again for the engine
     orc.set_rand(seed)
+    O.libc_srand(seed)
     out = []
     for f in range(frames):
         if f > 0:
@@ -141,3 +121,26 @@ def test_frames_with_an_animated_obstacle_match_reference():
         assert np.array_equal(rv.view(np.uint32), ov.view(np.uint32)), f
     speed = 0.025 * 30.0
     assert abs(float(np.abs(frames[-1][2][0]).max()) - speed) < 1e-3 * speed
+
+
+def test_frames_with_an_inversed_animated_container_match_reference():
+    """An animated sphere inside an inversed animated container (the liquid
+    held by the container's inside): the inversed group's union, negation and
+    velocity grids on every rebuild.  Bit-identical to the reference engine
+    for 5 frames."""
+    n = 24
+    v, t = scenes.sphere_mesh((0.5, 0.3, 0.5), 0.1, 2)
+    c, tc = scenes.sphere_mesh((0.5, 0.5, 0.5), 0.62, 2)
+    d = np.array([0.0, 0.02, 0.01], np.float32)
+    e = np.array([0.008, 0.0, 0.0], np.float32)
+
+    def objects(f):
+        a, b = v + f * d, c + f * e
+        return [(a - d, a, a + d, t, True), (b - e, b, b + e, tc, True)]
+
+    frames = _frames(n, 5, 7, objects, inversed=[False, True])
+    for f, ((rp, rv), (op, ov), vel) in enumerate(frames):
+        assert len(rp) == len(op), f
+        assert np.array_equal(rp.view(np.uint32), op.view(np.uint32)), f
+        assert np.array_equal(rv.view(np.uint32), ov.view(np.uint32)), f
+    assert max(float(np.abs(a).max()) for a in frames[-1][2]) > 0.2
