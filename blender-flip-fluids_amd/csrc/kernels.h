@@ -327,6 +327,12 @@ void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s);
 // MIC(0) factor and application z = (L L^T)^-1 r; hyperplane-by-hyperplane
 // launches driven by the host copy of the level table
 void mic0Factor(PressureSys &S, const int *hLevelStart, hipStream_t s);
+// the same invdiag in bands of 13 hyperplanes over the sweeps' tiles (host and
+// device tile ranges from mic0BandTileRanges); lfactor entries not included
+void mic0FactorBands(Grid g, const PressureSys &S, const int *hLevelStart, const int *hTileLo, const int *hTileHi,
+                     const int *tileLo, const int *tileHi, hipStream_t s);
+int mic0FactorBandLaunches(const PressureSys &S, const int *hLevelStart);
+void mic0LFactor(PressureSys &S, hipStream_t s);
 void mic0Apply(const PressureSys &S, const int *hLevelStart, const double *r, double *z,
                hipStream_t s);
 void mic0ApplyF(const PressureSys &S, const int *hLevelStart, const float *r, float *z,

@@ -205,6 +205,108 @@ __global__ __launch_bounds__((Band<T, K, BT>::THREADS)) void micBandKernel(Grid 
     }
 }
 
+// factorModifiedIncompleteColesky0 (pcgsolver.h:68-205) in bands of K
+// hyperplanes, with the tiles and halos of the forward sweep above: a row's
+// diagonal needs the invdiag of its -z, -y, -x neighbour rows, all on the
+// previous hyperplane, so the band scheme applies as it is.  One thread
+// follows one line along i; the rows of the previous band level come from
+// LDS, those of the hyperplane before the band (earlier launches) from
+// global memory.  Each row performs mic0LevelKernel's operations in its
+// order, so invdiag is bit-identical to the hyperplane-by-hyperplane factor.
+template <int K, int BT>
+__global__ __launch_bounds__(((BT + K - 1) * (BT + K - 1) + 63) / 64 * 64) void micFactorBandKernel(
+    Grid g, PressureSys S, const int *tileLo, const int *tileHi, int l0, int tilesJ, TileList list) {
+    constexpr int E = BT + K - 1, LINES = E * E;
+    __shared__ double val[2][LINES];
+    __shared__ int rowv[2][LINES];
+    const int ni = g.ni, nj = g.nj, nk = g.nk;
+    int tile = blockIdx.x;
+    if (list.n > 0) {
+        tile = list.id[blockIdx.x];
+    } else if (tileHi[tile] < l0 || tileLo[tile] > l0 + K - 1) {
+        return;
+    }
+    const int J0 = (tile % tilesJ) * BT, K0 = (tile / tilesJ) * BT;
+    const int tid = threadIdx.x;
+    const bool active = tid < LINES;
+    const int ej = tid % E, ek = tid / E;
+    const int j = J0 - (K - 1) + ej, k = K0 - (K - 1) + ek;
+    const bool lineIn = active && j >= 0 && j < nj && k >= 0 && k < nk;
+    // a halo line dj + dk lines from the tile feeds the tile's rows only
+    // through band levels q <= K-1-dj-dk; its later cells are left absent
+    const int dj = K - 1 - ej > 0 ? K - 1 - ej : 0, dk = K - 1 - ek > 0 ? K - 1 - ek : 0;
+    const bool owned = ej >= K - 1 && ek >= K - 1;
+    double prev = 0.0;
+    int prevRow = -1;
+    for (int q = 0; q < K; q++) {
+        const int buf = q & 1, pb = buf ^ 1;
+        const int i = l0 + q - j - k;
+        const bool ok = lineIn && q <= K - 1 - dj - dk && i >= 0 && i < ni;
+        const size_t c = ok ? i + (size_t)ni * (j + (size_t)nj * k) : 0;
+        const int row = ok ? S.cellRow[c] : -1;
+        double inv = 0.0;
+        if (row >= 0) {
+            // the -z, -y, -x neighbour rows and their invdiag; a present row
+            // past the band's first level has both neighbour lines inside the
+            // workgroup (ek, ej >= 1)
+            int rz, ry, rx;
+            double vz = 0.0, vy = 0.0, vx = 0.0;
+            if (q == 0) {
+                rz = k > 0 ? S.cellRow[c - (size_t)ni * nj] : -1;
+                ry = j > 0 ? S.cellRow[c - ni] : -1;
+                rx = i > 0 ? S.cellRow[c - 1] : -1;
+                if (rz >= 0) vz = S.invdiag[rz];
+                if (ry >= 0) vy = S.invdiag[ry];
+                if (rx >= 0) vx = S.invdiag[rx];
+            } else {
+                rz = rowv[pb][tid - E];
+                vz = val[pb][tid - E];
+                ry = rowv[pb][tid - 1];
+                vy = val[pb][tid - 1];
+                rx = prevRow;
+                vx = prev;
+            }
+            double a = S.diag[row];
+            if (a < 1e-9) a = 0;
+            double d = a;
+            const int kr[3] = {rz, ry, rx};
+            const double kv[3] = {vz, vy, vx};
+            const int jslots[3] = {5, 3, 1};
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                const int kk = kr[t];
+                if (kk < 0) continue;
+                if (S.diag[kk] < 1e-9) continue;
+                const int jslot = jslots[t];
+                const int *kb = S.nbr + (size_t)6 * kk;
+                const double ik = kv[t];
+                const double okx = S.ox[kk], oky = S.oy[kk], okz = S.oz[kk];
+                const double Ljk = (jslot == 1 ? okx : (jslot == 3 ? oky : okz)) * ik;
+                d -= Ljk * Ljk;
+                double missing = 0;
+                if (jslot != 1 && kb[1] >= 0) missing += okx * ik;
+                if (jslot != 3 && kb[3] >= 0) missing += oky * ik;
+                if (jslot != 5 && kb[5] >= 0) missing += okz * ik;
+                d -= 0.97 * Ljk * missing;
+            }
+            if (a == 0)
+                inv = d;
+            else if (d < 0.25 * a)
+                inv = 1 / sqrt(a);
+            else
+                inv = 1 / sqrt(d);
+            if (owned) S.invdiag[row] = inv;
+        }
+        if (active) {
+            val[buf][tid] = inv;
+            rowv[buf][tid] = row;
+        }
+        prev = inv;
+        prevRow = row;
+        __syncthreads();
+    }
+}
+
 // L entry of each row's -x/-y/-z neighbour row toward the row (0 when
 // absent), and the band sweeps' records: forward {invdiag, those three},
 // backward {invdiag, the row's own lx, ly, lz}
@@ -381,6 +483,41 @@ void mic0ApplyBandsF(Grid g, const PressureSys &S, const BandPlan &plan, const i
     if (S.m == 0) return;
     if (plan.fp64) throw std::runtime_error("MIC(0) bands: the launch plan was built for FP64");
     bandSweeps<float, K32, BT32>(g, S, plan, r, y, z, S.frecFwd, S.frecBwd, tileLo, tileHi, s);
+}
+
+constexpr int KF = 13;  // factor band depth: 59 launches for the 766 hyperplanes of 256^3
+
+void mic0FactorBands(Grid g, const PressureSys &S, const int *hs, const int *hTileLo, const int *hTileHi,
+                     const int *tileLo, const int *tileHi, hipStream_t s) {
+    if (S.m == 0) return;
+    // the tile ranges are those of the sweeps' tiles (both precisions use BT64)
+    static_assert(BT64 == BT32, "one set of tile ranges serves both sweeps and the factor");
+    const int tilesJ = (g.nj + BT64 - 1) / BT64, tilesK = (g.nk + BT64 - 1) / BT64, nt = tilesJ * tilesK;
+    const bool ranges = hTileLo && hTileHi && nt <= 65535;
+    const dim3 block(((BT64 + KF - 1) * (BT64 + KF - 1) + 63) / 64 * 64);
+    for (int l0 = 0; l0 < S.nLevels; l0 += KF) {
+        const int l1 = l0 + KF < S.nLevels ? l0 + KF : S.nLevels;
+        if (hs[l1] == hs[l0]) continue;
+        TileList tl;
+        tl.n = 0;
+        if (ranges) {
+            int count = 0;
+            for (int t = 0; t < nt; t++) count += !(hTileHi[t] < l0 || hTileLo[t] > l0 + KF - 1);
+            if (count == 0) continue;
+            if (count <= TileList::MAX)
+                for (int t = 0; t < nt; t++)
+                    if (!(hTileHi[t] < l0 || hTileLo[t] > l0 + KF - 1)) tl.id[tl.n++] = (uint16_t)t;
+        }
+        const dim3 grid(tl.n > 0 ? tl.n : nt);
+        hipLaunchKernelGGL((micFactorBandKernel<KF, BT64>), grid, block, 0, s, g, S, tileLo, tileHi, l0, tilesJ, tl);
+    }
+    hipCheck(hipGetLastError(), "MIC(0) factor band launch");
+}
+
+int mic0FactorBandLaunches(const PressureSys &S, const int *hs) {
+    int n = 0;
+    for (int l0 = 0; l0 < S.nLevels; l0 += KF) n += hs[(l0 + KF < S.nLevels ? l0 + KF : S.nLevels)] > hs[l0];
+    return n;
 }
 
 }  // namespace ff

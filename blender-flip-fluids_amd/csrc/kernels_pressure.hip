@@ -472,6 +472,10 @@ void mic0Factor(PressureSys &S, const int *hs, hipStream_t s) {
         hipLaunchKernelGGL(mic0LevelKernel, dim3(blocksFor(cnt, TPB)), dim3(TPB), 0, s, S,
                            S.levelRows + hs[l], cnt);
     }
+    mic0LFactor(S, s);
+}
+
+void mic0LFactor(PressureSys &S, hipStream_t s) {
     if (S.m > 0) hipLaunchKernelGGL(lfactorKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, S);
 }
 

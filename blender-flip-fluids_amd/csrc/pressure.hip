@@ -302,8 +302,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     }
     res.rhsInf = maxAbs;
     if (afterAssembly) hipCheck(hipEventRecord(afterAssembly, s), "event");
-    _levelLaunches = 0;
-    for (int l = 0; l < _S.nLevels; l++) _levelLaunches += _hLevelStart[l + 1] > _hLevelStart[l];
+    _levelLaunches = mic0FactorBandLaunches(_S, _hLevelStart.data());
     mic0BandPlan(_g, _S, _hLevelStart.data(), mode == 0, _hTileLo.data(), _hTileHi.data(), _plan);
     _bandLaunches = (int)_plan.l0.size();
     // The forward solve skips a right-hand side below the tolerance as an
@@ -356,7 +355,8 @@ bool PressureSolver::pcgFP64(double tolFactor, int maxIter, double resid0, int &
     resid = resid0;
     double tol = tolFactor * resid;
     _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
-    mic0Factor(_S, hs, st);
+    mic0FactorBands(_g, _S, hs, _hTileLo.data(), _hTileHi.data(), _tileLo, _tileHi, st);
+    mic0LFactor(_S, st);
     mic0LowerEntries(_S, true, st);
     _t->end(T_MIC0_FACTOR, st);
     _t->begin(T_MIC0_APPLY, st, applyLaunches());
@@ -421,7 +421,8 @@ bool PressureSolver::pcgPrecond32(double tolFactor, int maxIter, double resid0, 
     resid = resid0;
     double tol = tolFactor * resid;
     _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
-    mic0Factor(_S, hs, st);
+    mic0FactorBands(_g, _S, hs, _hTileLo.data(), _hTileHi.data(), _tileLo, _tileHi, st);
+    mic0LFactor(_S, st);
     _t->end(T_MIC0_FACTOR, st);
     mic0ToFloat(_S, st);
     mic0LowerEntries(_S, false, st);
@@ -477,7 +478,8 @@ bool PressureSolver::pcgMixed(double tolFactor, int maxIter, double resid0, doub
     hipCheck(hipMemcpyAsync(_r, _S.rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st), "copy");
     hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
     _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
-    mic0Factor(_S, hs, st);
+    mic0FactorBands(_g, _S, hs, _hTileLo.data(), _hTileHi.data(), _tileLo, _tileHi, st);
+    mic0LFactor(_S, st);
     _t->end(T_MIC0_FACTOR, st);
     mic0ToFloat(_S, st);
     mic0LowerEntries(_S, false, st);
@@ -539,7 +541,8 @@ bool PressureSolver::pcgReliable(double tolFactor, int maxIter, double resid0, d
     const double tol = std::min(tolFactor * resid0, 1.0);
     hipCheck(hipMemsetAsync(_x, 0, sizeof(double) * m, st), "memset");
     _t->begin(T_MIC0_FACTOR, st, _levelLaunches + 2);
-    mic0Factor(_S, hs, st);
+    mic0FactorBands(_g, _S, hs, _hTileLo.data(), _hTileHi.data(), _tileLo, _tileHi, st);
+    mic0LFactor(_S, st);
     _t->end(T_MIC0_FACTOR, st);
     mic0ToFloat(_S, st);
     mic0LowerEntries(_S, false, st);
