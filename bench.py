@@ -379,6 +379,69 @@ def splash_config(n, steps, warmup, tol, mode):
     print(json.dumps(line), flush=True)
 
 
+def moving_config(n, steps, warmup, tol, mode, speed):
+    """The 256^3 dam break with its sphere obstacle (the 20480-triangle
+    icosphere MeshObject) animated: it moves along +z at `speed` m/s, each
+    frame's MeshObject_update_mesh_animated giving the previous, current and
+    next frame's meshes, so the solid (with its face velocities, the cells'
+    solid weights and the pocket conditioning) is rebuilt every substep.  The
+    same run with the sphere static is timed beside it."""
+    import torch
+    from ffengine import FluidSimulation, MeshObject, TriangleMesh, scenes
+    frame_dt = 1.0 / 240.0
+    pos = scenes.dam_break_particles(n, seed=0)
+    v, t = scenes.sphere_mesh(subdivisions=5)
+    d = np.array([0.0, 0.0, speed * frame_dt], np.float32)
+    out = {}
+    for moving in (False, True):
+        sim = FluidSimulation(n, n, n, 1.0 / n)
+        sim.add_body_force(0.0, -9.81, 0.0)
+        sim.PICFLIP_ratio = 0.05
+        sim.CFL_condition_number = 5
+        sim.enable_surface_reconstruction = False
+        sim.pressure_solver_tolerance = tol
+        sim.set_pressure_solver_precision(mode)
+        mo = MeshObject(n, n, n, 1.0 / n)
+        if moving:
+            mo.update_mesh_animated(TriangleMesh(v - d, t), TriangleMesh(v, t), TriangleMesh(v + d, t))
+        else:
+            mo.update_mesh_static(TriangleMesh(v, t))
+        sim.add_mesh_obstacle(mo)
+        sim.load_marker_particle_data(len(pos), pos.tobytes(), np.zeros_like(pos).tobytes())
+        sim.initialize()
+        iters, substeps, elapsed = [], 0, 0.0
+        for f in range(warmup + steps):
+            if moving and f > 0:
+                c = v + f * d
+                mo.update_mesh_animated(TriangleMesh(c - d, t), TriangleMesh(c, t), TriangleMesh(c + d, t))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.update(frame_dt)
+            torch.cuda.synchronize()
+            if f >= warmup:
+                elapsed += time.perf_counter() - t0
+                st = sim.get_frame_stats_data()
+                substeps += st.substeps
+                iters.append(st.pressure_solver_iterations)
+            log("  %s frame %d/%d" % ("moving" if moving else "static", f + 1, warmup + steps))
+        vel = sim.get_solid_velocity_field()
+        out["moving" if moving else "static"] = {
+            "substeps_per_s": round(substeps / elapsed, 4), "ms_per_substep": round(1e3 * elapsed / substeps, 2),
+            "pressure_iterations": iters, "particles": sim.get_num_marker_particles(),
+            "max_solid_speed": round(max(float(np.abs(a).max()) for a in vel), 4)}
+        del sim, mo
+    rate = out["moving"]["substeps_per_s"]
+    line = {"metric": "FLIP substeps/s, dam break %d^3 with an animated sphere obstacle" % n, "value": rate,
+            "unit": "substeps/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True,
+            "data": "synthetic", "dtype": "f32+f64" if mode != "fp64" else "f64", "mode": mode,
+            "config": {"workload": "dam break %d^3, %d particles, the 20480-triangle sphere MeshObject (r 0.15 at "
+                                   "(0.6, 0.2, 0.5)) animated at %g m/s along +z, frame 1/240 s, PCG rel res %g"
+                                   % (n, len(pos), speed, tol), "grid": n, "particles": len(pos)},
+            "moving": out["moving"], "static": out["static"],
+            "rebuild_ms_per_substep": round(out["moving"]["ms_per_substep"] - out["static"]["ms_per_substep"], 2)}
+    print(json.dumps(line), flush=True)
+
+
 def adjoint_config(n, substeps, checkpoint_every, fd_faces, eps, tol, dt):
     """The differentiation configuration: a dam break (block [0, 0.375) x
     [0, 0.625) x [0, 1), 8 particles per cell outside the walls) whose initial
@@ -495,11 +558,13 @@ def adjoint_config(n, substeps, checkpoint_every, fd_faces, eps, tol, dt):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="dam-break",
-                    choices=["dam-break", "precision", "splash", "adjoint", "cpu-calibration"],
+                    choices=["dam-break", "precision", "splash", "adjoint", "moving", "cpu-calibration"],
                     help="dam-break: the 256^3 headline benchmark (default); precision: the 384^3 slab "
                          "pressure-accuracy comparison; splash: the 512^3 splash; adjoint: the 128^3 "
-                         "differentiable rollout with its finite-difference check; cpu-calibration (no GPU, "
+                         "differentiable rollout with its finite-difference check; moving: the 256^3 dam break "
+                         "with its sphere obstacle animated, beside the static run; cpu-calibration (no GPU, "
                          "needs the reference build): the reference engine against the oracle on the host")
+    ap.add_argument("--speed", type=float, default=0.3, help="moving: the sphere's speed along +z (m/s)")
     ap.add_argument("--substeps", type=int, default=32, help="adjoint: substeps of the rollout")
     ap.add_argument("--checkpoint-every", type=int, default=8, help="adjoint: checkpoint spacing")
     ap.add_argument("--fd-faces", type=int, default=64, help="adjoint: faces checked by central differences")
@@ -552,6 +617,11 @@ def main():
     if args.config == "adjoint":
         adjoint_config(args.n or 128, args.substeps, args.checkpoint_every, args.fd_faces, args.fd_eps, args.tol,
                        args.dt)
+        return
+    if args.config == "moving":
+        modes = [m for m in args.modes.split(",") if m]
+        moving_config(args.n or 256, args.steps, args.warmup, args.tol,
+                      modes[0] if len(modes) == 1 else "fp32-preconditioner", args.speed)
         return
     if args.config == "splash":
         modes = [m for m in args.modes.split(",") if m]

@@ -14,9 +14,9 @@
 // has weight >= 1e-6), so the GPU finds them with a lock-free union-find in
 // one pass over the cells' +x/+y/+z edges: every root hooks onto the smaller
 // of two roots with a compare-and-swap, so parent indices only decrease and
-// every find and retry ends.  A second pass counts each component's cells and
-// whether any borders air; a third zeroes the faces of the cells of isolated
-// components of more than one cell.
+// every find and retry ends.  A second pass flags each component that has an
+// edge (more than one cell) and each that borders air; a third zeroes the
+// faces of the cells of isolated components with an edge.
 #include <stdexcept>
 
 #include "kernels.h"
@@ -166,15 +166,25 @@ __global__ void condUniteKernel(CondView v, int *parent) {
     if (v.member(i, j, k + 1) && v.W(i, j, k + 1) >= eps) unite(parent, (int)c, (int)flat(i, j, k + 1, v.ni, v.nj));
 }
 
-// info[root] = 2 x (cells) | (any cell borders air)
-__global__ void condCountKernel(CondView v, int *parent, int *info) {
+// info[root]: bit 1 = the component has an edge (more than one cell), bit 0 =
+// a cell borders air.  Nearly every cell of the liquid body shares one root,
+// so each flag is set with an atomic only while a plain read still finds it
+// clear: once set, the other cells only read it.
+__global__ void condFlagKernel(CondView v, int *parent, int *info) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (size_t)v.ni * v.nj * v.nk) return;
     const int i = (int)(c % v.ni), j = (int)((c / v.ni) % v.nj), k = (int)(c / ((size_t)v.ni * v.nj));
     if (!v.member(i, j, k)) return;
+    const float eps = 1e-6f;
+    const bool edge = (v.member(i + 1, j, k) && v.U(i + 1, j, k) >= eps) ||
+                      (v.member(i, j + 1, k) && v.V(i, j + 1, k) >= eps) ||
+                      (v.member(i, j, k + 1) && v.W(i, j, k + 1) >= eps);
+    const bool air = v.bordersAir(i, j, k);
+    if (!edge && !air) return;
     const int r = findRoot(parent, (int)c);
-    atomicAdd(info + r, 2);
-    if (v.bordersAir(i, j, k)) atomicOr(info + r, 1);
+    const int have = __hip_atomic_load(info + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int want = (edge ? 2 : 0) | (air ? 1 : 0);
+    if ((have & want) != want) atomicOr(info + r, want);
 }
 
 __global__ void condZeroKernel(CondView v, int *parent, const int *info, float *us, float *vs, float *ws) {
@@ -184,7 +194,7 @@ __global__ void condZeroKernel(CondView v, int *parent, const int *info, float *
     if (!v.member(i, j, k)) return;
     const int r = findRoot(parent, (int)c);
     const int in = info[r];
-    if ((in & 1) != 0 || (in >> 1) <= 1) return;
+    if (in != 2) return;  // isolated (no air) and more than one cell
     const int ni = v.ni, nj = v.nj;
     us[flat(i, j, k, ni + 1, nj)] = 0.0f;
     us[flat(i + 1, j, k, ni + 1, nj)] = 0.0f;
@@ -204,13 +214,13 @@ void cellWeights(Grid g, const float *solid, float *wc, hipStream_t s) {
 void conditionSolidVelocity(Grid g, const float *phi, const float *wu, const float *wv, const float *ww, float *us,
                             float *vs, float *ws, int *scratch, hipStream_t s) {
     const size_t n = (size_t)g.ni * g.nj * g.nk;
-    if (n >= (size_t)1 << 30) throw std::runtime_error("conditionSolidVelocity: grid too large for int cell counts");
+    if (n > (size_t)0x7fffffff) throw std::runtime_error("conditionSolidVelocity: grid too large for int cell indices");
     int *parent = scratch, *info = scratch + n;
     const CondView v{g.ni, g.nj, g.nk, phi, wu, wv, ww};
     const dim3 grid(blocksFor(n, TPB));
     hipLaunchKernelGGL(condInitKernel, grid, dim3(TPB), 0, s, v, parent, info);
     hipLaunchKernelGGL(condUniteKernel, grid, dim3(TPB), 0, s, v, parent);
-    hipLaunchKernelGGL(condCountKernel, grid, dim3(TPB), 0, s, v, parent, info);
+    hipLaunchKernelGGL(condFlagKernel, grid, dim3(TPB), 0, s, v, parent, info);
     hipLaunchKernelGGL(condZeroKernel, grid, dim3(TPB), 0, s, v, parent, info, us, vs, ws);
 }
 
