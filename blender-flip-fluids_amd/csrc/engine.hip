@@ -1,4 +1,6 @@
 // engine.hip -- host orchestration of the FLIP substep on one HIP stream.
+#include <cstdlib>
+
 #include "engine.h"
 
 #include <algorithm>
@@ -557,6 +559,9 @@ void FluidSimulation::update(double dt) {
         if (done + step > limit) step = std::fmin(substepTime, _remaining);
         if (_stepNumber == _maxSteps - 1) step = _remaining;
         _remaining -= step;
+        // _stepFluid seeds the C library's rand() with frame + substep
+        // (fluidsimulation.cpp:10079): the obstacles' sign tests draw from it
+        srand((unsigned)(_currentFrame + _stepNumber));
         if (!skipped) {
             stepFluid(step);
             timing[0] += _phaseMs[0] + _phaseMs[1];
@@ -604,6 +609,7 @@ void FluidSimulation::step(double dt) {
     if (!(dt > 0.0)) throw std::domain_error("Error: substep length must be greater than 0.\n");
     _frameDt = dt;
     _remaining = 0.0;  // one substep is the whole frame: progress 0
+    srand((unsigned)(_currentFrame + _stepNumber));  // as _stepFluid does (fluidsimulation.cpp:10079)
     stepFluid(dt);
     _stepNumber++;
 }

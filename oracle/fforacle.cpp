@@ -373,6 +373,38 @@ struct RandSource {
     }
 };
 
+// The C library's rand() after srand(seed), restated so that the oracle can
+// reseed at every substep as FluidSimulation::_stepFluid does
+// (fluidsimulation.cpp:10079, srand(frame + substep)).  This is glibc's
+// default generator (the TYPE_3 additive feedback generator of random_r.c,
+// as published): the state r[0..30] comes from the seed (0 taken as 1) by
+// the Park-Miller minimal standard LCG r[i] = 16807 r[i-1] mod 2^31-1,
+// r[31..33] = r[0..2], then r[i] = r[i-31] + r[i-3] mod 2^32; the first 310
+// sums are discarded and rand() returns each later sum shifted right by one.
+struct GlibcRand {
+    std::vector<uint32_t> r;
+    explicit GlibcRand(unsigned seed) {
+        r.assign(34, 0u);
+        int32_t s0 = (int32_t)seed;
+        if (s0 == 0) s0 = 1;
+        r[0] = (uint32_t)s0;
+        for (int i = 1; i < 31; i++) {
+            const int32_t prev = (int32_t)r[i - 1];
+            const int32_t hi = prev / 127773, lo = prev % 127773;
+            int32_t word = 16807 * lo - 2836 * hi;  // Schrage's product without overflow
+            if (word < 0) word += 2147483647;
+            r[i] = (uint32_t)word;
+        }
+        for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+        for (int i = 34; i < 344; i++) r.push_back(r[i - 31] + r[i - 3]);
+    }
+    int next() {
+        const size_t i = r.size();
+        r.push_back(r[i - 31] + r[i - 3]);
+        return (int)(r.back() >> 1);
+    }
+};
+
 // MeshUtils::getCellsInsideTriangleMesh (meshutils.cpp:57-224): per column
 // (i, j) the z of every crossing of the +z line through the jittered cell
 // centre (i, j, -1); a cell is inside when the column has an even number of
@@ -2557,9 +2589,7 @@ struct OracleSim {
     // jitter from the given rand() values in order
     std::vector<MovingObstacle> movers;
     bool moversChanged = false;
-    std::vector<int> rnd;
-    size_t rndUsed = 0;
-    double randMax = 2147483647.0;
+    size_t rndUsed = 0;  // rand() draws of the solid updates so far
     std::vector<float> us, vs, ws, wc;  // solid face velocities, cell weights
     int lastSubsteps = 0, lastIters = 0, pIters = 0;
     bool pSuccess = true;
@@ -2694,7 +2724,12 @@ struct OracleSim {
         if (!rebuild) return;
         moversChanged = false;
         float progress = 1.0f - (float)(remaining + dt) / (float)frameDt;
-        RandSource rs{rnd.data() + rndUsed, (int)(rnd.size() - rndUsed), 0, randMax};
+        // _stepFluid seeds rand() with frame + substep before the obstacle
+        // update (fluidsimulation.cpp:10079)
+        GlibcRand g((unsigned)(frame + stepNumber));
+        std::vector<int> vals(4096);
+        for (int &x : vals) x = g.next();
+        RandSource rs{vals.data(), (int)vals.size(), 0, 2147483647.0};
         solidUpdate(ni, nj, nk, dx, movers, frameDt, progress, rs, solid, us, vs, ws);
         rndUsed += rs.used;
         oracle_face_weights(ni, nj, nk, solid.data(), wu.data(), wv.data(), ww.data());
@@ -3040,10 +3075,10 @@ int oracle_sim_update_moving_obstacle(OracleSim *s, int index, const float *prev
     }
     return 0;
 }
-void oracle_sim_set_rand_values(OracleSim *s, const int *rnd, int n, int rand_max) {
-    s->rnd.assign(rnd, rnd + n);
-    s->rndUsed = 0;
-    s->randMax = (double)rand_max;
+int oracle_glibc_rand(unsigned seed, int n, int *out) {
+    GlibcRand g(seed);
+    for (int q = 0; q < n; q++) out[q] = g.next();
+    return n;
 }
 int oracle_sim_rand_used(OracleSim *s) { return (int)s->rndUsed; }
 void oracle_sim_get_solid_velocity(OracleSim *s, float *u, float *v, float *w) {
@@ -3076,6 +3111,7 @@ void oracle_sim_initialize(OracleSim *s) { s->initialize(); }
 void oracle_sim_update(OracleSim *s, double dt) { s->update(dt); }
 void oracle_sim_step(OracleSim *s, double dt) {
     s->frameDt = dt;
+    s->remaining = 0.0;  // one substep is the whole frame: progress 0
     s->stepFluid(dt);
     s->stepNumber++;
 }

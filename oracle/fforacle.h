@@ -176,13 +176,18 @@ void oracle_sim_load_particles(OracleSim *s, int np, const float *pos, const flo
 /* Mesh obstacles with velocities (static or animated, normal or inversed:
  * flags bit 0 animated, bit 1 inversed), one triangle list and three vertex
  * sets each; replaces the box/set solid by the solid update of every substep
- * whose meshes move.  The sign tests take the given rand() values in order. */
+ * whose meshes move.  The sign tests draw from rand() seeded with
+ * frame + substep at each substep, as _stepFluid seeds it
+ * (fluidsimulation.cpp:10079), through a restatement of the C library's
+ * generator (oracle_glibc_rand). */
 int oracle_sim_add_moving_obstacle(OracleSim *s, int nv, const float *prev, const float *cur, const float *next,
                                    int nt, const int *tris, int flags, float expansion, float vscale);
 int oracle_sim_update_moving_obstacle(OracleSim *s, int index, const float *prev, const float *cur,
                                       const float *next);
-void oracle_sim_set_rand_values(OracleSim *s, const int *rnd, int n, int rand_max);
 int oracle_sim_rand_used(OracleSim *s);
+/* The first n values of the C library's rand() after srand(seed), from the
+ * oracle's restatement of glibc's generator. */
+int oracle_glibc_rand(unsigned seed, int n, int *out);
 void oracle_sim_get_solid_velocity(OracleSim *s, float *u, float *v, float *w);
 void oracle_sim_get_solid_phi(OracleSim *s, float *phi);
 void oracle_sim_initialize(OracleSim *s);

@@ -119,6 +119,14 @@ def libc_srand(seed):
     C.CDLL(None).srand(C.c_uint(seed))
 
 
+def glibc_rand_values(seed, n=96):
+    """The oracle's restatement of the C library's rand() after srand(seed)."""
+    out = np.zeros(n, np.int32)
+    oracle().oracle_glibc_rand.argtypes = [C.c_uint, C.c_int, C.c_void_p]
+    oracle().oracle_glibc_rand(seed, n, fp(out))
+    return out
+
+
 def _mesh_arrays(meshes):
     """meshes: (vertices, triangles[, inversed[, expansion]]) each."""
     verts = np.ascontiguousarray(np.concatenate([_f32(m[0]).reshape(-1, 3) for m in meshes]), np.float32)
@@ -435,13 +443,6 @@ class OracleSim:
         ms = [_f32(a).reshape(-1, 3) for a in (prev, cur, nxt)]
         if L.oracle_sim_update_moving_obstacle(self.h, idx, *[fp(m) for m in ms]) != 0:
             raise RuntimeError("oracle_sim_update_moving_obstacle failed")
-
-    def set_rand(self, seed, n=20000):
-        """The sign tests' jitter: the C library's rand() values after srand(seed)."""
-        self._rnd = libc_rand_values(seed, n)
-        L = oracle()
-        L.oracle_sim_set_rand_values.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
-        L.oracle_sim_set_rand_values(self.h, fp(self._rnd), len(self._rnd), RAND_MAX)
 
     def rand_used(self):
         L = oracle()

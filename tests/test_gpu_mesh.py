@@ -130,9 +130,15 @@ def test_engine_solid_with_obstacle_matches_oracle_and_rebuilds():
     with pytest.raises(RuntimeError):
         sim.add_mesh_obstacle(mo)
     sim.initialize()
-    O.libc_srand(5)
+    # the engine seeds rand() with frame + substep at every substep, as the
+    # reference's _stepFluid does: the first update's rebuild draws after srand(0)
+    O.libc_srand(4321)
     sim.update(1.0 / 30.0)
-    expect, draws = O.solid_phi_meshes((n, n, n), dx, [(v, t)], 5)
+    st = sim.get_frame_stats_data()
+    if st.substeps == 1:  # the frame's one substep drew the 6 values of its rebuild after srand(0)
+        import ctypes
+        assert ctypes.CDLL(None).rand() == int(O.glibc_rand_values(0, 7)[6])
+    expect, draws = O.solid_phi_meshes((n, n, n), dx, [(v, t)], 0)
     say(f"engine solid with obstacle: draws {sim.get_last_solid_rand_draws()} (oracle {draws})")
     assert sim.get_last_solid_rand_draws() == draws == 6
     assert np.array_equal(bits(sim.get_solid_sdf()), bits(expect))
@@ -142,9 +148,9 @@ def test_engine_solid_with_obstacle_matches_oracle_and_rebuilds():
     assert sim.get_last_solid_rand_draws() == 3
     assert np.array_equal(bits(sim.get_solid_sdf()), bits(O.solid_box_phi((n, n, n), dx)))
     mo.enable = True
-    O.libc_srand(5)
-    sim.update(1.0 / 30.0)
-    assert np.array_equal(bits(sim.get_solid_sdf()), bits(expect))
+    sim.update(1.0 / 30.0)  # frame 2: srand(2)
+    again, _ = O.solid_phi_meshes((n, n, n), dx, [(v, t)], 2)
+    assert np.array_equal(bits(sim.get_solid_sdf()), bits(again))
     sim.remove_mesh_obstacle(mo)
     with pytest.raises(RuntimeError):
         sim.remove_mesh_obstacle(mo)
@@ -207,7 +213,6 @@ def test_boundary_and_live_friction_match_reference():
     for frame, want in enumerate(rec["frames"]):
         if frame == 1:  # the obstacle's friction changes without any other state change
             mo.friction = rec["obstacle_friction_from_frame_1"]
-        O.libc_srand(rec["srand"][frame])
         sim.update(1.0 / 30.0)
         _check_frame(sim, want, f"boundary/live friction frame {frame}")
 
@@ -227,6 +232,5 @@ def test_engine_frames_with_obstacles_match_reference(case):
         sim.add_mesh_obstacle(mo)
     sim.initialize()
     for frame, want in enumerate(rec["frames"]):
-        O.libc_srand(rec["srand"][frame])
         sim.update(1.0 / 30.0)
         _check_frame(sim, want, f"obstacles {case} frame {frame}")

@@ -97,7 +97,7 @@ def _gpu_sim(n, pos):
     return sim
 
 
-def _run_frames(n, frames, seed, make_objects):
+def _run_frames(n, frames, make_objects):
     dx = 1.0 / n
     pos = scenes.dam_break_particles(n, seed=1)
     sim = _gpu_sim(n, pos)
@@ -115,10 +115,9 @@ def _run_frames(n, frames, seed, make_objects):
         objs.append((mo, orc.add_moving_obstacle(prev, cur, nxt, tris, animated=animated, inversed=inversed)))
     sim.initialize()
     orc.initialize()
-    # the oracle's list of rand() values first: making it draws from the C
-    # library, which is then seeded again for the engine
-    orc.set_rand(seed)
-    O.libc_srand(seed)
+    # both engines seed rand() with frame + substep at every substep (the
+    # reference's _stepFluid does, fluidsimulation.cpp:10079); the oracle
+    # through its restatement of the C library's generator
     for f in range(frames):
         if f > 0:
             for (mo, q), (prev, cur, nxt, tris, animated, _) in zip(objs, make_objects(f)):
@@ -155,7 +154,7 @@ def test_engine_frames_with_an_animated_obstacle_match_oracle():
         return [(c - d, c, c + d, t, True, False), (s, s, s, ts, False, False)]
 
     say(f"animated sphere beside a static one, {n}^3")
-    vel = _run_frames(n, 5, 11, objects)
+    vel = _run_frames(n, 5, objects)
     speed = 0.025 * 30.0
     assert abs(float(np.abs(vel[0]).max()) - speed) < 1e-3 * speed
 
@@ -172,4 +171,4 @@ def test_engine_frames_with_an_inversed_animated_container_match_oracle():
         return [(a - d, a, a + d, t, True, False), (b - e, b, b + e, tc, True, True)]
 
     say(f"animated sphere in an inversed animated container, {n}^3")
-    _run_frames(n, 5, 7, objects)
+    _run_frames(n, 5, objects)

@@ -59,7 +59,7 @@ def test_solid_update_matches_reference(case):
     assert moving > 0.1  # the obstacles do move
 
 
-def _frames(n, frames, seed, make_objects, inversed=None):
+def _frames(n, frames, make_objects, inversed=None):
     """The dam break at n^3 with mesh obstacles, frames of 1/30 s through the
     reference engine and the oracle; each frame's obstacle meshes come from
     make_objects(frame) as (prev, cur, next, tris, animated); inversed: the
@@ -82,10 +82,9 @@ def _frames(n, frames, seed, make_objects, inversed=None):
         handles.append((h, orc.add_moving_obstacle(prev, cur, nxt, tris, animated=animated, inversed=inv)))
     ref.initialize()
     orc.initialize()
-    # the oracle's list of rand() values first: making it draws from the C
-    # library, which is then seeded again for the engine
-    orc.set_rand(seed)
-    O.libc_srand(seed)
+    # both engines seed rand() with frame + substep at every substep (the
+    # reference's _stepFluid does, fluidsimulation.cpp:10079); the oracle
+    # through its restatement of the C library's generator
     out = []
     for f in range(frames):
         if f > 0:
@@ -114,7 +113,7 @@ def test_frames_with_an_animated_obstacle_match_reference():
         c = v + f * d
         return [(c - d, c, c + d, t, True), (s, s, s, ts, False)]
 
-    frames = _frames(n, 5, 11, objects)
+    frames = _frames(n, 5, objects)
     for f, ((rp, rv), (op, ov), vel) in enumerate(frames):
         assert len(rp) == len(op), f
         assert np.array_equal(rp.view(np.uint32), op.view(np.uint32)), f
@@ -138,7 +137,7 @@ def test_frames_with_an_inversed_animated_container_match_reference():
         a, b = v + f * d, c + f * e
         return [(a - d, a, a + d, t, True), (b - e, b, b + e, tc, True)]
 
-    frames = _frames(n, 5, 7, objects, inversed=[False, True])
+    frames = _frames(n, 5, objects, inversed=[False, True])
     for f, ((rp, rv), (op, ov), vel) in enumerate(frames):
         assert len(rp) == len(op), f
         assert np.array_equal(rp.view(np.uint32), op.view(np.uint32)), f
