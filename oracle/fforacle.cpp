@@ -2543,7 +2543,644 @@ V3 nodalGradient(const float *g, int pi, int pj, int pk, double dx, V3 p) {
     return V3{dvdx, dvdy, dvdz};
 }
 
+
+// ---- the viscosity solve (ViscositySolver, viscositysolver.cpp) ----
+//
+// One SPD system over the u, v and w faces together (viscositysolver.cpp:
+// 50-79): face states from the solid SDF at the cell centres, control volumes
+// from the liquid SDF sampled on a twice finer grid, a matrix index over the
+// fluid faces with volume, 15-entry rows of the viscous operator, and the
+// reference's PCGSolver<float> with its general-sparse MIC(0) factor.
+
+enum ViscFace : unsigned char { VF_AIR = 0, VF_FLUID = 1, VF_SOLID = 2 };
+
+// SparseMatrix<float> rows (sparsematrix.h:44-112): sorted column lists;
+// set/add ignore a -1 index
+struct SparseRows {
+    std::vector<std::vector<int>> idx;
+    std::vector<std::vector<float>> val;
+    explicit SparseRows(int n = 0) : idx(n), val(n) {}
+    void put(int i, int j, float x, bool add) {
+        if (i == -1 || j == -1) return;
+        std::vector<int> &I = idx[i];
+        std::vector<float> &X = val[i];
+        for (size_t k = 0; k < I.size(); k++) {
+            if (I[k] == j) {
+                if (add)
+                    X[k] += x;
+                else
+                    X[k] = x;
+                return;
+            }
+            if (I[k] > j) {
+                I.insert(I.begin() + k, j);
+                X.insert(X.begin() + k, x);
+                return;
+            }
+        }
+        I.push_back(j);
+        X.push_back(x);
+    }
+};
+
+struct ViscSystem {
+    int ni = 0, nj = 0, nk = 0;
+    std::vector<unsigned char> su, sv, sw;             // face states
+    std::vector<float> center, vu, vv, vw, eu, ev, ew;  // control volumes
+    std::vector<int> table;                            // FaceIndexer index -> row, or -1
+    int m = 0;
+    SparseRows A;
+    std::vector<float> rhs;
+    size_t U(int i, int j, int k) const { return flat(i, j, k, ni + 1, nj); }
+    size_t V(int i, int j, int k) const { return flat(i, j, k, ni, nj + 1); }
+    size_t W(int i, int j, int k) const { return flat(i, j, k, ni, nj); }
+    // FaceIndexer (viscositysolver.h:130-155): u faces, then v, then w
+    int rowU(int i, int j, int k) const { return table[U(i, j, k)]; }
+    int rowV(int i, int j, int k) const { return table[(size_t)(ni + 1) * nj * nk + V(i, j, k)]; }
+    int rowW(int i, int j, int k) const {
+        return table[(size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk + W(i, j, k)];
+    }
+};
+
+// _computeFaceStateGrid (viscositysolver.cpp:102-181) and the control volumes
+// (_computeVolumeGrid, 209-377); liquid (cells) and solid (nodes) SDFs
+void viscosityVolumes(int ni, int nj, int nk, double dx, const float *liquid, const float *solid, ViscSystem &S) {
+    S.ni = ni; S.nj = nj; S.nk = nk;
+    // the solid SDF at the cell centres: the mean of the 8 nodes in the
+    // reference's order (MeshLevelSet::getDistanceAtCellCenter, meshlevelset.cpp:150-160)
+    std::vector<float> sc((size_t)ni * nj * nk);
+    const int pi = ni + 1, pj = nj + 1;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                auto P = [&](int a, int b, int c) { return solid[flat(i + a, j + b, k + c, pi, pj)]; };
+                sc[flat(i, j, k, ni, nj)] = 0.125f * (P(0, 0, 0) + P(1, 0, 0) + P(0, 1, 0) + P(1, 1, 0) +
+                                                      P(0, 0, 1) + P(1, 0, 1) + P(0, 1, 1) + P(1, 1, 1));
+            }
+    auto C = [&](int i, int j, int k) { return sc[flat(i, j, k, ni, nj)]; };
+    S.su.assign((size_t)(ni + 1) * nj * nk, VF_AIR);
+    S.sv.assign((size_t)ni * (nj + 1) * nk, VF_AIR);
+    S.sw.assign((size_t)ni * nj * (nk + 1), VF_AIR);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni + 1; i++) {
+                bool edge = i == 0 || i == ni;
+                S.su[S.U(i, j, k)] = (edge || C(i - 1, j, k) + C(i, j, k) <= 0) ? VF_SOLID : VF_FLUID;
+            }
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj + 1; j++)
+            for (int i = 0; i < ni; i++) {
+                bool edge = j == 0 || j == nj;
+                S.sv[S.V(i, j, k)] = (edge || C(i, j - 1, k) + C(i, j, k) <= 0) ? VF_SOLID : VF_FLUID;
+            }
+    for (int k = 0; k < nk + 1; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                bool edge = k == 0 || k == nk;
+                S.sw[S.W(i, j, k)] = (edge || C(i, j, k - 1) + C(i, j, k) <= 0) ? VF_SOLID : VF_FLUID;
+            }
+    // cells near the liquid: liquid cells and two layers of 6-neighbours, on
+    // an (n+1)^3 grid
+    const int vi = ni + 1, vj = nj + 1, vk = nk + 1;
+    std::vector<char> valid((size_t)vi * vj * vk, 0);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                if (liquid[flat(i, j, k, ni, nj)] < 0) valid[flat(i, j, k, vi, vj)] = 1;
+    for (int layer = 0; layer < 2; layer++) {
+        std::vector<char> next = valid;
+        const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+        for (int k = 0; k < vk; k++)
+            for (int j = 0; j < vj; j++)
+                for (int i = 0; i < vi; i++) {
+                    if (!valid[flat(i, j, k, vi, vj)]) continue;
+                    for (auto &o : nb) {
+                        int a = i + o[0], b = j + o[1], c = k + o[2];
+                        if (inRange(a, b, c, vi, vj, vk)) next[flat(a, b, c, vi, vj)] = 1;
+                    }
+                }
+        valid.swap(next);
+    }
+    // volume fractions of the liquid in the (2n)^3 subcells of the valid cells
+    // (_estimateVolumeFractionsThread, 314-348): the liquid SDF trilinearly
+    // interpolated at each subcell's corners (ParticleLevelSet::trilinearInterpolate,
+    // particlelevelset.cpp:136-138: cell-centred samples)
+    const int si = 2 * ni, sj = 2 * nj, sk = 2 * nk;
+    std::vector<float> sub((size_t)si * sj * sk, 0.0f);
+    const float fdx = (float)dx;  // the solver's float cell width
+    const float sdx = 0.5f * fdx;
+    const V3 start = V3{0.25f * fdx, 0.25f * fdx, 0.25f * fdx};
+    const V3 half = mk(0.5 * dx, 0.5 * dx, 0.5 * dx);
+    auto liquidAt = [&](V3 p) { return nodalTrilinear(liquid, ni, nj, nk, dx, p - half); };
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k < sk; k++)
+        for (int j = 0; j < sj; j++)
+            for (int i = 0; i < si; i++) {
+                if (!valid[flat(i / 2, j / 2, k / 2, vi, vj)]) continue;
+                V3 c = start + V3{i * sdx, j * sdx, k * sdx};
+                float h = 0.5f * sdx;
+                float p000 = liquidAt(c + V3{-h, -h, -h}), p001 = liquidAt(c + V3{-h, -h, +h});
+                float p010 = liquidAt(c + V3{-h, +h, -h}), p011 = liquidAt(c + V3{-h, +h, +h});
+                float p100 = liquidAt(c + V3{+h, -h, -h}), p101 = liquidAt(c + V3{+h, -h, +h});
+                float p110 = liquidAt(c + V3{+h, +h, -h}), p111 = liquidAt(c + V3{+h, +h, +h});
+                sub[flat(i, j, k, si, sj)] = volumeFraction8(p000, p100, p010, p110, p001, p101, p011, p111);
+            }
+    // each grid's volume: the mean of the 8 subcells around its sample point
+    // (_computeVolumeGridThread, 350-377), over cells 1..n-1
+    auto volume = [&](std::vector<float> &out, int wi, int wj, int wk, int oi, int oj, int ok) {
+        out.assign((size_t)wi * wj * wk, 0.0f);
+        for (int k = 1; k < nk; k++)
+            for (int j = 1; j < nj; j++)
+                for (int i = 1; i < ni; i++) {
+                    if (!valid[flat(i, j, k, vi, vj)]) continue;
+                    float acc = 0.0f;
+                    for (int c = 0; c < 2; c++)
+                        for (int b = 0; b < 2; b++)
+                            for (int a = 0; a < 2; a++)
+                                acc += sub[flat(2 * i + oi + a, 2 * j + oj + b, 2 * k + ok + c, si, sj)];
+                    out[flat(i, j, k, wi, wj)] = 0.125f * acc;
+                }
+    };
+    volume(S.center, ni, nj, nk, 0, 0, 0);
+    volume(S.vu, ni + 1, nj, nk, -1, 0, 0);
+    volume(S.vv, ni, nj + 1, nk, 0, -1, 0);
+    volume(S.vw, ni, nj, nk + 1, 0, 0, -1);
+    volume(S.eu, ni, nj + 1, nk + 1, 0, -1, -1);
+    volume(S.ev, ni + 1, nj, nk + 1, -1, 0, -1);
+    volume(S.ew, ni + 1, nj + 1, nk, -1, -1, 0);
+}
+
+// _computeMatrixIndexTable (viscositysolver.cpp:383-473) and the three row
+// assemblies (_initializeLinearSystemThreadU/V/W, 559-830); visc: the
+// (n+1)^3 viscosity grid, uvw the velocity field before the solve
+void viscositySystem(double dx, double dt, const float *visc, const float *u, const float *v, const float *w,
+                     ViscSystem &S) {
+    const int ni = S.ni, nj = S.nj, nk = S.nk;
+    const size_t nU = (size_t)(ni + 1) * nj * nk, nV = (size_t)ni * (nj + 1) * nk, nW = (size_t)ni * nj * (nk + 1);
+    auto CE = [&](int i, int j, int k) { return S.center[flat(i, j, k, ni, nj)]; };
+    auto VU = [&](int i, int j, int k) { return S.vu[flat(i, j, k, ni + 1, nj)]; };
+    auto VV = [&](int i, int j, int k) { return S.vv[flat(i, j, k, ni, nj + 1)]; };
+    auto VW = [&](int i, int j, int k) { return S.vw[flat(i, j, k, ni, nj)]; };
+    auto EU = [&](int i, int j, int k) { return S.eu[flat(i, j, k, ni, nj + 1)]; };
+    auto EV = [&](int i, int j, int k) { return S.ev[flat(i, j, k, ni + 1, nj)]; };
+    auto EW = [&](int i, int j, int k) { return S.ew[flat(i, j, k, ni + 1, nj + 1)]; };
+    auto SU = [&](int i, int j, int k) { return S.su[S.U(i, j, k)]; };
+    auto SV = [&](int i, int j, int k) { return S.sv[S.V(i, j, k)]; };
+    auto SW = [&](int i, int j, int k) { return S.sw[S.W(i, j, k)]; };
+    std::vector<char> in(nU + nV + nW, 0);
+    for (int k = 1; k < nk; k++)
+        for (int j = 1; j < nj; j++)
+            for (int i = 1; i < ni; i++) {
+                if (SU(i, j, k) == VF_FLUID &&
+                    (VU(i, j, k) > 0.0 || CE(i, j, k) > 0.0 || CE(i - 1, j, k) > 0.0 || EW(i, j + 1, k) > 0.0 ||
+                     EW(i, j, k) > 0.0 || EV(i, j, k + 1) > 0.0 || EV(i, j, k) > 0.0))
+                    in[S.U(i, j, k)] = 1;
+            }
+    for (int k = 1; k < nk; k++)
+        for (int j = 1; j < nj; j++)
+            for (int i = 1; i < ni; i++) {
+                if (SV(i, j, k) == VF_FLUID &&
+                    (VV(i, j, k) > 0.0 || EW(i + 1, j, k) > 0.0 || EW(i, j, k) > 0.0 || CE(i, j, k) > 0.0 ||
+                     CE(i, j - 1, k) > 0.0 || EU(i, j, k + 1) > 0.0 || EU(i, j, k) > 0.0))
+                    in[nU + S.V(i, j, k)] = 1;
+            }
+    for (int k = 1; k < nk; k++)
+        for (int j = 1; j < nj; j++)
+            for (int i = 1; i < ni; i++) {
+                if (SW(i, j, k) == VF_FLUID &&
+                    (VW(i, j, k) > 0.0 || EV(i + 1, j, k) > 0.0 || EV(i, j, k) > 0.0 || EU(i, j + 1, k) > 0.0 ||
+                     EU(i, j, k) > 0.0 || CE(i, j, k) > 0.0 || CE(i, j, k - 1) > 0.0))
+                    in[nU + nV + S.W(i, j, k)] = 1;
+            }
+    S.table.assign(nU + nV + nW, -1);
+    S.m = 0;
+    for (size_t q = 0; q < in.size(); q++)
+        if (in[q]) S.table[q] = S.m++;
+    S.A = SparseRows(S.m);
+    S.rhs.assign(S.m, 0.0f);
+    const int vi = ni + 1, vj = nj + 1;
+    auto VS = [&](int i, int j, int k) { return visc[flat(i, j, k, vi, vj)]; };
+    auto Uv = [&](int i, int j, int k) { return u[flat(i, j, k, ni + 1, nj)]; };
+    auto Vv = [&](int i, int j, int k) { return v[flat(i, j, k, ni, nj + 1)]; };
+    auto Wv = [&](int i, int j, int k) { return w[flat(i, j, k, ni, nj)]; };
+    // the solver keeps the cell width and time step as float (viscositysolver.h:36-39)
+    const float fdx = (float)dx, fdt = (float)dt;
+    const float invdx = 1.0f / fdx;
+    const float factor = fdt * invdx * invdx;
+    // the u rows (559-648); the v and w rows are the same operator with the
+    // axes cycled (650-830)
+    for (int k = 1; k < nk; k++)
+        for (int j = 1; j < nj; j++)
+            for (int i = 1; i < ni; i++) {
+                if (SU(i, j, k) != VF_FLUID) continue;
+                const int row = S.rowU(i, j, k);
+                if (row == -1) continue;
+                float vR = VS(i, j, k), vL = VS(i - 1, j, k);
+                float vT = 0.25f * (VS(i - 1, j + 1, k) + VS(i - 1, j, k) + VS(i, j + 1, k) + VS(i, j, k));
+                float vB = 0.25f * (VS(i - 1, j, k) + VS(i - 1, j - 1, k) + VS(i, j, k) + VS(i, j - 1, k));
+                float vF = 0.25f * (VS(i - 1, j, k + 1) + VS(i - 1, j, k) + VS(i, j, k + 1) + VS(i, j, k));
+                float vK = 0.25f * (VS(i - 1, j, k) + VS(i - 1, j, k - 1) + VS(i, j, k) + VS(i, j, k - 1));
+                float fR = 2 * factor * vR * CE(i, j, k), fL = 2 * factor * vL * CE(i - 1, j, k);
+                float fT = factor * vT * EW(i, j + 1, k), fB = factor * vB * EW(i, j, k);
+                float fF = factor * vF * EV(i, j, k + 1), fK = factor * vK * EV(i, j, k);
+                float diag = VU(i, j, k) + fR + fL + fT + fB + fF + fK;
+                S.A.put(row, row, diag, false);
+                struct E { unsigned char st; int col; float coef; float vel; };
+                const E es[14] = {
+                    {SU(i + 1, j, k), S.rowU(i + 1, j, k), -fR, Uv(i + 1, j, k)},
+                    {SU(i - 1, j, k), S.rowU(i - 1, j, k), -fL, Uv(i - 1, j, k)},
+                    {SU(i, j + 1, k), S.rowU(i, j + 1, k), -fT, Uv(i, j + 1, k)},
+                    {SU(i, j - 1, k), S.rowU(i, j - 1, k), -fB, Uv(i, j - 1, k)},
+                    {SU(i, j, k + 1), S.rowU(i, j, k + 1), -fF, Uv(i, j, k + 1)},
+                    {SU(i, j, k - 1), S.rowU(i, j, k - 1), -fK, Uv(i, j, k - 1)},
+                    {SV(i, j + 1, k), S.rowV(i, j + 1, k), -fT, Vv(i, j + 1, k)},
+                    {SV(i - 1, j + 1, k), S.rowV(i - 1, j + 1, k), fT, Vv(i - 1, j + 1, k)},
+                    {SV(i, j, k), S.rowV(i, j, k), fB, Vv(i, j, k)},
+                    {SV(i - 1, j, k), S.rowV(i - 1, j, k), -fB, Vv(i - 1, j, k)},
+                    {SW(i, j, k + 1), S.rowW(i, j, k + 1), -fF, Wv(i, j, k + 1)},
+                    {SW(i - 1, j, k + 1), S.rowW(i - 1, j, k + 1), fF, Wv(i - 1, j, k + 1)},
+                    {SW(i, j, k), S.rowW(i, j, k), fK, Wv(i, j, k)},
+                    {SW(i - 1, j, k), S.rowW(i - 1, j, k), -fK, Wv(i - 1, j, k)}};
+                for (const E &e : es)
+                    if (e.st == VF_FLUID) S.A.put(row, e.col, e.coef, true);
+                float rval = VU(i, j, k) * Uv(i, j, k);
+                for (const E &e : es)
+                    if (e.st == VF_SOLID) rval -= e.coef * e.vel;
+                S.rhs[row] = rval;
+            }
+    for (int k = 1; k < nk; k++)
+        for (int j = 1; j < nj; j++)
+            for (int i = 1; i < ni; i++) {
+                if (SV(i, j, k) != VF_FLUID) continue;
+                const int row = S.rowV(i, j, k);
+                if (row == -1) continue;
+                float vT = VS(i, j, k), vB = VS(i, j - 1, k);
+                float vR = 0.25f * (VS(i, j - 1, k) + VS(i + 1, j - 1, k) + VS(i, j, k) + VS(i + 1, j, k));
+                float vL = 0.25f * (VS(i, j - 1, k) + VS(i - 1, j - 1, k) + VS(i, j, k) + VS(i - 1, j, k));
+                float vF = 0.25f * (VS(i, j - 1, k) + VS(i, j - 1, k + 1) + VS(i, j, k) + VS(i, j, k + 1));
+                float vK = 0.25f * (VS(i, j - 1, k) + VS(i, j - 1, k - 1) + VS(i, j, k) + VS(i, j, k - 1));
+                float fR = factor * vR * EW(i + 1, j, k), fL = factor * vL * EW(i, j, k);
+                float fT = 2 * factor * vT * CE(i, j, k), fB = 2 * factor * vB * CE(i, j - 1, k);
+                float fF = factor * vF * EU(i, j, k + 1), fK = factor * vK * EU(i, j, k);
+                float diag = VV(i, j, k) + fR + fL + fT + fB + fF + fK;
+                S.A.put(row, row, diag, false);
+                struct E { unsigned char st; int col; float coef; float vel; };
+                const E es[14] = {
+                    {SV(i + 1, j, k), S.rowV(i + 1, j, k), -fR, Vv(i + 1, j, k)},
+                    {SV(i - 1, j, k), S.rowV(i - 1, j, k), -fL, Vv(i - 1, j, k)},
+                    {SV(i, j + 1, k), S.rowV(i, j + 1, k), -fT, Vv(i, j + 1, k)},
+                    {SV(i, j - 1, k), S.rowV(i, j - 1, k), -fB, Vv(i, j - 1, k)},
+                    {SV(i, j, k + 1), S.rowV(i, j, k + 1), -fF, Vv(i, j, k + 1)},
+                    {SV(i, j, k - 1), S.rowV(i, j, k - 1), -fK, Vv(i, j, k - 1)},
+                    {SU(i + 1, j, k), S.rowU(i + 1, j, k), -fR, Uv(i + 1, j, k)},
+                    {SU(i + 1, j - 1, k), S.rowU(i + 1, j - 1, k), fR, Uv(i + 1, j - 1, k)},
+                    {SU(i, j, k), S.rowU(i, j, k), fL, Uv(i, j, k)},
+                    {SU(i, j - 1, k), S.rowU(i, j - 1, k), -fL, Uv(i, j - 1, k)},
+                    {SW(i, j, k + 1), S.rowW(i, j, k + 1), -fF, Wv(i, j, k + 1)},
+                    {SW(i, j - 1, k + 1), S.rowW(i, j - 1, k + 1), fF, Wv(i, j - 1, k + 1)},
+                    {SW(i, j, k), S.rowW(i, j, k), fK, Wv(i, j, k)},
+                    {SW(i, j - 1, k), S.rowW(i, j - 1, k), -fK, Wv(i, j - 1, k)}};
+                for (const E &e : es)
+                    if (e.st == VF_FLUID) S.A.put(row, e.col, e.coef, true);
+                float rval = VV(i, j, k) * Vv(i, j, k);
+                for (const E &e : es)
+                    if (e.st == VF_SOLID) rval -= e.coef * e.vel;
+                S.rhs[row] = rval;
+            }
+    for (int k = 1; k < nk; k++)
+        for (int j = 1; j < nj; j++)
+            for (int i = 1; i < ni; i++) {
+                if (SW(i, j, k) != VF_FLUID) continue;
+                const int row = S.rowW(i, j, k);
+                if (row == -1) continue;
+                float vF = VS(i, j, k), vK = VS(i, j, k - 1);
+                float vR = 0.25f * (VS(i, j, k) + VS(i, j, k - 1) + VS(i + 1, j, k) + VS(i + 1, j, k - 1));
+                float vL = 0.25f * (VS(i, j, k) + VS(i, j, k - 1) + VS(i - 1, j, k) + VS(i - 1, j, k - 1));
+                float vT = 0.25f * (VS(i, j, k) + VS(i, j, k - 1) + VS(i, j + 1, k) + VS(i, j + 1, k - 1));
+                float vB = 0.25f * (VS(i, j, k) + VS(i, j, k - 1) + VS(i, j - 1, k) + VS(i, j - 1, k - 1));
+                float fR = factor * vR * EV(i + 1, j, k), fL = factor * vL * EV(i, j, k);
+                float fT = factor * vT * EU(i, j + 1, k), fB = factor * vB * EU(i, j, k);
+                float fF = 2 * factor * vF * CE(i, j, k), fK = 2 * factor * vK * CE(i, j, k - 1);
+                float diag = VW(i, j, k) + fR + fL + fT + fB + fF + fK;
+                S.A.put(row, row, diag, false);
+                struct E { unsigned char st; int col; float coef; float vel; };
+                const E es[14] = {
+                    {SW(i + 1, j, k), S.rowW(i + 1, j, k), -fR, Wv(i + 1, j, k)},
+                    {SW(i - 1, j, k), S.rowW(i - 1, j, k), -fL, Wv(i - 1, j, k)},
+                    {SW(i, j + 1, k), S.rowW(i, j + 1, k), -fT, Wv(i, j + 1, k)},
+                    {SW(i, j - 1, k), S.rowW(i, j - 1, k), -fB, Wv(i, j - 1, k)},
+                    {SW(i, j, k + 1), S.rowW(i, j, k + 1), -fF, Wv(i, j, k + 1)},
+                    {SW(i, j, k - 1), S.rowW(i, j, k - 1), -fK, Wv(i, j, k - 1)},
+                    {SU(i + 1, j, k), S.rowU(i + 1, j, k), -fR, Uv(i + 1, j, k)},
+                    {SU(i + 1, j, k - 1), S.rowU(i + 1, j, k - 1), fR, Uv(i + 1, j, k - 1)},
+                    {SU(i, j, k), S.rowU(i, j, k), fL, Uv(i, j, k)},
+                    {SU(i, j, k - 1), S.rowU(i, j, k - 1), -fL, Uv(i, j, k - 1)},
+                    {SV(i, j + 1, k), S.rowV(i, j + 1, k), -fT, Vv(i, j + 1, k)},
+                    {SV(i, j + 1, k - 1), S.rowV(i, j + 1, k - 1), fT, Vv(i, j + 1, k - 1)},
+                    {SV(i, j, k), S.rowV(i, j, k), fB, Vv(i, j, k)},
+                    {SV(i, j, k - 1), S.rowV(i, j, k - 1), -fB, Vv(i, j, k - 1)}};
+                for (const E &e : es)
+                    if (e.st == VF_FLUID) S.A.put(row, e.col, e.coef, true);
+                float rval = VW(i, j, k) * Wv(i, j, k);
+                for (const E &e : es)
+                    if (e.st == VF_SOLID) rval -= e.coef * e.vel;
+                S.rhs[row] = rval;
+            }
+}
+
+// SparseColumnLowerFactor + factorModifiedIncompleteColesky0 (pcgsolver.h:
+// 27-190) in float: the lower triangle copied column by column, diagonals
+// below 1e-9 taken as 0 (null rows), then the push-form elimination with the
+// dropped fill-in moved onto the diagonal (modification 0.97, pivots below
+// 0.25 x the original diagonal replaced by it).  The pivot's inverse square
+// root is taken in double and rounded once: the template's unqualified sqrt
+// of a float resolves to the C library's double sqrt under the reference's
+// includes.
+struct ViscFactor {
+    std::vector<float> invdiag, value, adiag;
+    std::vector<unsigned> rowindex, colstart;
+};
+void viscFactor(const SparseRows &A, ViscFactor &F) {
+    const unsigned n = (unsigned)A.idx.size();
+    F.invdiag.assign(n, 0.0f);
+    F.adiag.assign(n, 0.0f);
+    F.colstart.assign(n + 1, 0);
+    F.value.clear();
+    F.rowindex.clear();
+    const float eps = 1e-9f;
+    for (unsigned i = 0; i < n; i++) {
+        F.colstart[i] = (unsigned)F.rowindex.size();
+        for (size_t q = 0; q < A.idx[i].size(); q++) {
+            const unsigned c = (unsigned)A.idx[i][q];
+            if (c > i) {
+                F.rowindex.push_back(c);
+                F.value.push_back(A.val[i][q]);
+            } else if (c == i) {
+                float x = A.val[i][q];
+                if (x < eps) x = 0.0f;
+                F.invdiag[i] = F.adiag[i] = x;
+            }
+        }
+    }
+    F.colstart[n] = (unsigned)F.rowindex.size();
+    const float omega = 0.97f, ratio = 0.25f;
+    for (unsigned k = 0; k < n; k++) {
+        if (F.adiag[k] == 0) continue;
+        if (F.invdiag[k] < ratio * F.adiag[k])
+            F.invdiag[k] = (float)(1 / std::sqrt((double)F.adiag[k]));
+        else
+            F.invdiag[k] = (float)(1 / std::sqrt((double)F.invdiag[k]));
+        const unsigned c0 = F.colstart[k], c1 = F.colstart[k + 1];
+        for (unsigned p = c0; p < c1; p++) F.value[p] *= F.invdiag[k];
+        for (unsigned p = c0; p < c1; p++) {
+            const unsigned j = F.rowindex[p];
+            const float mult = F.value[p];
+            float missing = 0;
+            unsigned a = c0;
+            // entries of column k above row j that column j of A lacks
+            size_t b = 0;
+            const std::vector<int> &Aj = A.idx[j];
+            while (a < c1 && F.rowindex[a] < j) {
+                while (b < Aj.size()) {
+                    if ((unsigned)Aj[b] < F.rowindex[a]) {
+                        b++;
+                    } else if ((unsigned)Aj[b] == F.rowindex[a]) {
+                        break;
+                    } else {
+                        missing += F.value[a];
+                        break;
+                    }
+                }
+                a++;
+            }
+            if (a < c1 && F.rowindex[a] == j) F.invdiag[j] -= mult * F.value[a];
+            a++;
+            // below the diagonal: update matching entries, the rest is missing
+            unsigned bb = F.colstart[j];
+            while (a < c1 && bb < F.colstart[j + 1]) {
+                if (F.rowindex[bb] < F.rowindex[a]) {
+                    bb++;
+                } else if (F.rowindex[bb] == F.rowindex[a]) {
+                    F.value[bb] -= mult * F.value[a];
+                    a++;
+                    bb++;
+                } else {
+                    missing += F.value[a];
+                    a++;
+                }
+            }
+            while (a < c1) missing += F.value[a++];
+            F.invdiag[j] -= omega * mult * missing;
+        }
+    }
+}
+
+// solveLower then solveLowerTransposeInPlace (pcgsolver.h:196-229)
+void viscPrecondition(const ViscFactor &F, const std::vector<float> &x, std::vector<float> &out) {
+    const unsigned n = (unsigned)F.invdiag.size();
+    out = x;
+    for (unsigned i = 0; i < n; i++) {
+        out[i] *= F.invdiag[i];
+        for (unsigned q = F.colstart[i]; q < F.colstart[i + 1]; q++) out[F.rowindex[q]] -= F.value[q] * out[i];
+    }
+    for (unsigned i = n; i-- > 0;) {
+        for (unsigned q = F.colstart[i]; q < F.colstart[i + 1]; q++) out[i] -= F.value[q] * out[F.rowindex[q]];
+        out[i] *= F.invdiag[i];
+    }
+}
+
+// BLAS::dot<float> (blaswrapper.h:27-57): a float sum, per thread over
+// 500000-element chunks when more than one thread is allowed
+float viscDot(const std::vector<float> &x, const std::vector<float> &y, int ncpu) {
+    int nt = (int)std::fmin(ncpu, std::ceil((float)x.size() / (float)500000));
+    if (nt <= 1) {
+        float s = 0;
+        for (size_t i = 0; i < x.size(); i++) s += x[i] * y[i];
+        return s;
+    }
+    std::vector<int> iv = splitRange(0, (int)x.size(), nt);
+    float s = 0;
+    for (int t = 0; t < nt; t++) {
+        float part = 0;
+        for (int i = iv[t]; i < iv[t + 1]; i++) part += x[i] * y[i];
+        s += part;
+    }
+    return s;
+}
+float viscAbsMax(const std::vector<float> &x) {
+    float mx = 0;
+    for (float v : x)
+        if (std::fabs(v) > mx) mx = std::fabs(v);
+    return mx;
+}
+// BLAS::addScaled (blaswrapper.h:128-157): y += alpha x with a float alpha
+void viscAddScaled(float alpha, const std::vector<float> &x, std::vector<float> &y) {
+    for (size_t i = 0; i < x.size(); i++) y[i] += alpha * x[i];
+}
+
+// PCGSolver<float>::solve (pcgsolver.h:246-300) with the viscosity solver's
+// parameters (viscositysolver.cpp:832-868): relative tolerance tol on the
+// residual's infinity norm, capped at 1.0, and success also when the
+// iteration limit is reached below an estimated error of 10.
+struct ViscResult {
+    int iterations = 0;
+    float error = 0.0f;
+    bool success = true;
+};
+ViscResult viscPCG(const SparseRows &A, const std::vector<float> &rhs, double tol, int maxIter, int ncpu,
+                   std::vector<float> &x) {
+    ViscResult res;
+    const size_t n = rhs.size();
+    x.assign(n, 0.0f);
+    float tolFactor = (float)tol;
+    if (tolFactor < 1e-30) tolFactor = 1e-30f;
+    const float maxErr = 1.0f;
+    std::vector<float> r = rhs, z(n), s(n);
+    float resid = viscAbsMax(r);
+    if (resid == 0) {
+        res.iterations = 0;
+        res.error = 0.0f;
+        res.success = true;
+        return res;
+    }
+    const double tolAbs = tolFactor * resid;
+    ViscFactor F;
+    viscFactor(A, F);
+    viscPrecondition(F, r, z);
+    double rho = viscDot(z, r, ncpu);
+    if (rho == 0 || rho != rho) {
+        res.iterations = 0;
+        res.error = resid;
+        res.success = false;
+        return res;
+    }
+    s = z;
+    bool ok = false;
+    int it;
+    for (it = 0; it < maxIter; it++) {
+        for (size_t i = 0; i < n; i++) {  // FixedSparseMatrix rows in column order
+            float acc = 0;
+            for (size_t q = 0; q < A.idx[i].size(); q++) acc += A.val[i][q] * s[A.idx[i][q]];
+            z[i] = acc;
+        }
+        const double alpha = rho / viscDot(s, z, ncpu);
+        viscAddScaled((float)alpha, s, x);
+        viscAddScaled((float)-alpha, z, r);
+        resid = viscAbsMax(r);
+        if (resid <= std::min(tolAbs, (double)maxErr)) {
+            ok = true;
+            it++;
+            break;
+        }
+        viscPrecondition(F, r, z);
+        const double rhoNew = viscDot(z, r, ncpu);
+        const double beta = rhoNew / rho;
+        viscAddScaled((float)beta, s, z);
+        s.swap(z);
+        rho = rhoNew;
+    }
+    res.iterations = it;
+    res.error = resid;
+    res.success = ok || (it == maxIter && resid < 10.0);
+    return res;
+}
+
+// ViscositySolver::applyViscosityToVelocityField (viscositysolver.cpp:50-79,
+// 870-905): on success the velocity field is cleared and the matrix faces take
+// the solution; on failure it is left as it was.  Returns the matrix size.
+int viscositySolve(int ni, int nj, int nk, double dx, double dt, const float *liquid, const float *solid,
+                   const float *visc, double tol, int maxIter, int ncpu, float *u, float *v, float *w,
+                   ViscResult &res) {
+    ViscSystem S;
+    viscosityVolumes(ni, nj, nk, dx, liquid, solid, S);
+    viscositySystem(dx, dt, visc, u, v, w, S);
+    res = ViscResult();
+    if (S.m == 0) return 0;
+    std::vector<float> x;
+    res = viscPCG(S.A, S.rhs, tol, maxIter, ncpu, x);
+    if (!res.success) return S.m;
+    const size_t nU = (size_t)(ni + 1) * nj * nk, nV = (size_t)ni * (nj + 1) * nk, nW = (size_t)ni * nj * (nk + 1);
+    std::fill(u, u + nU, 0.0f);
+    std::fill(v, v + nV, 0.0f);
+    std::fill(w, w + nW, 0.0f);
+    for (size_t q = 0; q < nU; q++)
+        if (S.table[q] != -1) u[q] = x[S.table[q]];
+    for (size_t q = 0; q < nV; q++)
+        if (S.table[nU + q] != -1) v[q] = x[S.table[nU + q]];
+    for (size_t q = 0; q < nW; q++)
+        if (S.table[nU + nV + q] != -1) w[q] = x[S.table[nU + nV + q]];
+    return S.m;
+}
 }  // namespace
+
+struct OracleViscSystem {
+    ViscSystem S;
+    int nnz = 0;
+};
+void *oracle_viscosity_system_new(int ni, int nj, int nk, double dx, double dt, const float *liquid_phi,
+                                  const float *solid_phi_nodes, const float *viscosity, const float *u,
+                                  const float *v, const float *w, int *m, int *nnz) {
+    OracleViscSystem *h = new OracleViscSystem();
+    viscosityVolumes(ni, nj, nk, dx, liquid_phi, solid_phi_nodes, h->S);
+    viscositySystem(dx, dt, viscosity, u, v, w, h->S);
+    for (const auto &r : h->S.A.idx) h->nnz += (int)r.size();
+    *m = h->S.m;
+    *nnz = h->nnz;
+    return h;
+}
+void oracle_viscosity_system_get(void *handle, unsigned char *states, float *volumes, int *table, int *rowstart,
+                                 int *cols, float *vals, float *rhs) {
+    const ViscSystem &S = ((OracleViscSystem *)handle)->S;
+    if (states) {
+        states = std::copy(S.su.begin(), S.su.end(), states);
+        states = std::copy(S.sv.begin(), S.sv.end(), states);
+        std::copy(S.sw.begin(), S.sw.end(), states);
+    }
+    if (volumes)
+        for (const std::vector<float> *g : {&S.center, &S.vu, &S.vv, &S.vw, &S.eu, &S.ev, &S.ew})
+            volumes = std::copy(g->begin(), g->end(), volumes);
+    if (table) std::copy(S.table.begin(), S.table.end(), table);
+    if (rowstart) {
+        int q = 0;
+        rowstart[0] = 0;
+        for (int r = 0; r < S.m; r++) {
+            if (cols) std::copy(S.A.idx[r].begin(), S.A.idx[r].end(), cols + q);
+            if (vals) std::copy(S.A.val[r].begin(), S.A.val[r].end(), vals + q);
+            q += (int)S.A.idx[r].size();
+            rowstart[r + 1] = q;
+        }
+    }
+    if (rhs) std::copy(S.rhs.begin(), S.rhs.end(), rhs);
+}
+void oracle_viscosity_system_free(void *handle) { delete (OracleViscSystem *)handle; }
+
+int oracle_viscosity_pcg(int m, const int *rowstart, const int *cols, const float *vals, const float *rhs,
+                         double tol, int max_iter, int nthreads, float *x, int *iterations, float *error) {
+    SparseRows A(m);
+    for (int r = 0; r < m; r++) {
+        A.idx[r].assign(cols + rowstart[r], cols + rowstart[r + 1]);
+        A.val[r].assign(vals + rowstart[r], vals + rowstart[r + 1]);
+    }
+    std::vector<float> b(rhs, rhs + m), sol;
+    ViscResult res = viscPCG(A, b, tol, max_iter, nthreads, sol);
+    std::copy(sol.begin(), sol.end(), x);
+    *iterations = res.iterations;
+    *error = res.error;
+    return res.success ? 1 : 0;
+}
+
+int oracle_viscosity_solve(int ni, int nj, int nk, double dx, double dt, const float *liquid_phi,
+                           const float *solid_phi_nodes, const float *viscosity, double tol, int max_iter,
+                           int nthreads, float *u, float *v, float *w, int *iterations, float *error,
+                           int *success) {
+    ViscResult res;
+    int m = viscositySolve(ni, nj, nk, dx, dt, liquid_phi, solid_phi_nodes, viscosity, tol, max_iter, nthreads, u,
+                           v, w, res);
+    *iterations = res.iterations;
+    *error = res.error;
+    *success = res.success ? 1 : 0;
+    return m;
+}
 
 struct OracleSim {
     int ni, nj, nk;
@@ -2557,6 +3194,12 @@ struct OracleSim {
     int chunks = 1;
     bool extremeRemoval = true;
     bool apic = false;  // velocity transfer method (fluidsimulation.cpp:2837-2857)
+    // constant viscosity (setViscosity, fluidsimulation.cpp:2569-2585): off
+    // until set; the solver's tolerance and iteration limit (fluidsimulation.h:2364-2365)
+    bool viscEnabled = false;
+    double viscosity = 0.0;
+    double viscTol = 1e-4;
+    int viscMaxIters = 900;
     std::vector<V3> bodyForces;
     // Hooks of the differentiable rollout (no counterpart in the reference):
     // particle removal can be switched off; a substep can record its liquid
@@ -2594,6 +3237,11 @@ struct OracleSim {
     int lastSubsteps = 0, lastIters = 0, pIters = 0;
     bool pSuccess = true;
     double lastErr = 0, pErr = 0;
+    // viscosity solve statistics of the frame, kept like the pressure's
+    // (fluidsimulation.cpp:6039-6053)
+    int vIters = 0, lastVIters = 0, vSize = 0;
+    bool vSuccess = true, lastVSuccess = true;
+    float vErr = 0.0f, lastVErr = 0.0f;
 
     size_t nU() const { return (size_t)(ni + 1) * nj * nk; }
     size_t nV() const { return (size_t)ni * (nj + 1) * nk; }
@@ -2704,6 +3352,9 @@ struct OracleSim {
         lastSubsteps = stepNumber;
         lastIters = pIters;
         lastErr = pErr;
+        lastVIters = vIters;
+        lastVErr = vErr;
+        lastVSuccess = vSuccess;
         frame++;
     }
 
@@ -2791,6 +3442,7 @@ struct OracleSim {
         if (std::fabs(bf.x) > beps) for (float &x : u) x += (float)(bf.x * dt);
         if (std::fabs(bf.y) > beps) for (float &x : v) x += (float)(bf.y * dt);
         if (std::fabs(bf.z) > beps) for (float &x : w) x += (float)(bf.z * dt);
+        applyViscosity(dt);
         // pressure (6152-6276)
         pres.assign(nN(), 0.0f);
         int status[2];
@@ -2835,6 +3487,26 @@ struct OracleSim {
             }
         }
         advance(dt);
+    }
+
+    // _applyViscosityToVelocityField (fluidsimulation.cpp:5983-6056) with a
+    // constant viscosity: the field constrained, then the viscosity solve
+    void applyViscosity(double dt) {
+        if (!viscEnabled || pos.empty() || viscosity == 0.0) return;
+        constrain(u, v, w);
+        std::vector<float> grid(nNodes(), (float)viscosity);
+        ViscResult res;
+        vSize = viscositySolve(ni, nj, nk, dx, dt, phi.data(), solid.data(), grid.data(), viscTol, viscMaxIters,
+                               chunks, u.data(), v.data(), w.data(), res);
+        if (stepNumber == 0) {
+            vSuccess = res.success;
+            vIters = res.iterations;
+            vErr = res.error;
+        } else if (vSuccess && (!res.success || res.iterations > vIters)) {
+            vSuccess = res.success;
+            vIters = res.iterations;
+            vErr = res.error;
+        }
     }
 
     // P2G with the recorded flags: S/W where a flag is recorded (0 without
@@ -3046,6 +3718,17 @@ void oracle_sim_set_cfl(OracleSim *s, int cfl) { s->cfl = cfl; }
 void oracle_sim_set_pressure_tolerance(OracleSim *s, double tol) { s->ptol = tol; }
 void oracle_sim_set_max_pressure_iterations(OracleSim *s, int n) { s->maxPIters = n; }
 void oracle_sim_set_thread_chunks(OracleSim *s, int n) { s->chunks = n < 1 ? 1 : n; }
+void oracle_sim_set_viscosity(OracleSim *s, double visc, double tol, int max_iter) {
+    s->viscEnabled = true;
+    s->viscosity = (float)visc;
+    s->viscTol = tol;
+    s->viscMaxIters = max_iter;
+}
+int oracle_sim_last_viscosity_stats(OracleSim *s, int *iterations, float *error) {
+    *iterations = s->lastVIters;
+    *error = s->lastVErr;
+    return s->lastVSuccess ? 1 : 0;
+}
 int oracle_sim_add_moving_obstacle(OracleSim *s, int nv, const float *prev, const float *cur, const float *next,
                                    int nt, const int *tris, int flags, float expansion, float vscale) {
     try {

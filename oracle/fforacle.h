@@ -102,6 +102,33 @@ void oracle_cell_weights(int ni, int nj, int nk, const float *solid_phi_nodes, f
  * pockets of liquid (phi cell-centred, face weights wu, wv, ww). */
 void oracle_condition_solid_velocity(int ni, int nj, int nk, const float *phi, const float *wu, const float *wv,
                                      const float *ww, float *us, float *vs, float *ws);
+/* The viscosity solve (ViscositySolver, viscositysolver.cpp:50-79).  Grids:
+ * liquid_phi cell-centred (ni nj nk), solid_phi_nodes and viscosity on
+ * (ni+1)(nj+1)(nk+1) nodes, u/v/w the MAC field.
+ * system_new builds the face states, the seven control-volume grids, the
+ * matrix index and the linear system, returning the matrix size m and its
+ * nonzero count; system_get copies out (NULL skips): states of the u, v, w
+ * faces concatenated (0 air, 1 fluid, 2 solid), volumes center, U, V, W,
+ * edgeU, edgeV, edgeW concatenated (sizes ni nj nk, (ni+1) nj nk,
+ * ni (nj+1) nk, ni nj (nk+1), ni (nj+1)(nk+1), (ni+1) nj (nk+1),
+ * (ni+1)(nj+1) nk), the face-to-row table (u, v, w faces, -1 when not in the
+ * matrix), the rows as CSR with sorted columns, and the right-hand side.
+ * pcg solves a CSR system with PCGSolver<float> (pcgsolver.h:246-300) and
+ * returns 1 on success.  solve runs the whole stage in place on u, v, w
+ * (viscositysolver.cpp:832-905) and returns m. */
+void *oracle_viscosity_system_new(int ni, int nj, int nk, double dx, double dt, const float *liquid_phi,
+                                  const float *solid_phi_nodes, const float *viscosity, const float *u,
+                                  const float *v, const float *w, int *m, int *nnz);
+void oracle_viscosity_system_get(void *handle, unsigned char *states, float *volumes, int *table, int *rowstart,
+                                 int *cols, float *vals, float *rhs);
+void oracle_viscosity_system_free(void *handle);
+int oracle_viscosity_pcg(int m, const int *rowstart, const int *cols, const float *vals, const float *rhs,
+                         double tol, int max_iter, int nthreads, float *x, int *iterations, float *error);
+int oracle_viscosity_solve(int ni, int nj, int nk, double dx, double dt, const float *liquid_phi,
+                           const float *solid_phi_nodes, const float *viscosity, double tol, int max_iter,
+                           int nthreads, float *u, float *v, float *w, int *iterations, float *error,
+                           int *success);
+
 int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt,
                           const float *u, const float *v, const float *w,
                           const float *wu, const float *wv, const float *ww,
@@ -171,6 +198,12 @@ void oracle_sim_set_cfl(OracleSim *s, int cfl);
 void oracle_sim_set_pressure_tolerance(OracleSim *s, double tol);
 void oracle_sim_set_max_pressure_iterations(OracleSim *s, int n);
 void oracle_sim_set_thread_chunks(OracleSim *s, int n);
+/* setViscosity with a constant value, the viscosity solver's tolerance and
+ * iteration limit (fluidsimulation.cpp:2569-2601, 303-312); the last frame's
+ * viscosity solve statistics: returns success, iterations and error of the
+ * first failing or else the most iterated substep. */
+void oracle_sim_set_viscosity(OracleSim *s, double visc, double tol, int max_iter);
+int oracle_sim_last_viscosity_stats(OracleSim *s, int *iterations, float *error);
 void oracle_sim_set_solid_phi(OracleSim *s, const float *phi_nodes);
 void oracle_sim_load_particles(OracleSim *s, int np, const float *pos, const float *vel);
 /* Mesh obstacles with velocities (static or animated, normal or inversed:

@@ -53,6 +53,7 @@
 #include "particlesystem.h"
 #include "pressuresolver.h"
 #include "velocityadvector.h"
+#include "viscositysolver.h"
 #undef private
 #undef protected
 
@@ -400,7 +401,111 @@ void refprobe_sim_get_solid_velocity(void *s, float *u, float *v, float *w) {
     copyOut(*m.getArray3dV(), v);
     copyOut(*m.getArray3dW(), w);
 }
+/* ViscositySolver stage by stage (viscositysolver.cpp:50-79): the solver's
+ * own face states, volume grids, matrix index and linear system for the
+ * given fields; outputs as oracle_viscosity_system_get (NULL skips).  Returns
+ * the matrix size and writes the nonzero count. */
+namespace {
+struct ViscInputs {
+    MACVelocityField vf;
+    ParticleLevelSet pls;
+    MeshLevelSet solid;
+    Array3d<float> visc;
+    ViscInputs(int ni, int nj, int nk, double dx, const float *liquid, const float *sphi, const float *vg,
+               const float *u, const float *v, const float *w)
+        : vf(ni, nj, nk, dx), pls(ni, nj, nk, dx), solid(ni, nj, nk, dx), visc(ni + 1, nj + 1, nk + 1, 0.0f) {
+        copyIn(*vf.getArray3dU(), u);
+        copyIn(*vf.getArray3dV(), v);
+        copyIn(*vf.getArray3dW(), w);
+        copyIn(pls._phi, liquid);
+        copyIn(*solid.getPhiArray3d(), sphi);
+        copyIn(visc, vg);
+    }
+    ViscositySolverParameters params(double dx, double dt, double tol, int maxIter) {
+        ViscositySolverParameters p;
+        p.cellwidth = dx;
+        p.deltaTime = dt;
+        p.velocityField = &vf;
+        p.liquidSDF = &pls;
+        p.solidSDF = &solid;
+        p.viscosity = &visc;
+        p.errorTolerance = tol;
+        p.maxIterations = maxIter;
+        return p;
+    }
+};
+}  // namespace
+
+int refprobe_viscosity_system(int ni, int nj, int nk, double dx, double dt, const float *liquid,
+                              const float *solid_nodes, const float *viscosity, const float *u, const float *v,
+                              const float *w, int *nnz, unsigned char *states, float *volumes, int *table,
+                              int *rowstart, int *cols, float *vals, float *rhs) {
+    ViscInputs in(ni, nj, nk, dx, liquid, solid_nodes, viscosity, u, v, w);
+    ViscositySolver vs;
+    vs._initialize(in.params(dx, dt, 1e-4, 900));
+    vs._computeFaceStateGrid();
+    vs._computeVolumeGrid();
+    vs._computeMatrixIndexTable();
+    const int m = vs._matrixIndex.matrixSize;
+    SparseMatrixf matrix(m, 15);
+    std::vector<float> b(m, 0.0f);
+    if (m > 0) vs._initializeLinearSystem(matrix, b);
+    int q = 0;
+    for (int r = 0; r < m; r++) q += (int)matrix.index[r].size();
+    *nnz = q;
+    if (states) {
+        for (auto *g : {&vs._state.U, &vs._state.V, &vs._state.W}) {
+            size_t n = (size_t)g->width * g->height * g->depth;
+            for (size_t i = 0; i < n; i++) *states++ = (unsigned char)g->getRawArray()[i];
+        }
+    }
+    if (volumes) {
+        auto &V = vs._volumes;
+        for (Array3d<float> *g : {&V.center, &V.U, &V.V, &V.W, &V.edgeU, &V.edgeV, &V.edgeW}) {
+            copyOut(*g, volumes);
+            volumes += (size_t)g->width * g->height * g->depth;
+        }
+    }
+    if (table) std::copy(vs._matrixIndex.indexTable.begin(), vs._matrixIndex.indexTable.end(), table);
+    if (rowstart) {
+        q = 0;
+        rowstart[0] = 0;
+        for (int r = 0; r < m; r++) {
+            for (size_t e = 0; e < matrix.index[r].size(); e++) {
+                if (cols) cols[q] = (int)matrix.index[r][e];
+                if (vals) vals[q] = matrix.value[r][e];
+                q++;
+            }
+            rowstart[r + 1] = q;
+        }
+    }
+    if (rhs) std::copy(b.begin(), b.end(), rhs);
+    return m;
+}
+
+/* ViscositySolver::applyViscosityToVelocityField on the given fields; u, v, w
+ * updated in place.  Returns 1 on success. */
+int refprobe_viscosity_solve(int ni, int nj, int nk, double dx, double dt, const float *liquid,
+                             const float *solid_nodes, const float *viscosity, double tol, int maxIter, float *u,
+                             float *v, float *w, int *iterations, float *error) {
+    ViscInputs in(ni, nj, nk, dx, liquid, solid_nodes, viscosity, u, v, w);
+    ViscositySolver vs;
+    bool ok = vs.applyViscosityToVelocityField(in.params(dx, dt, tol, maxIter));
+    *iterations = vs.getIterations();
+    *error = vs.getError();
+    copyOut(*in.vf.getArray3dU(), u);
+    copyOut(*in.vf.getArray3dV(), v);
+    copyOut(*in.vf.getArray3dW(), w);
+    return ok ? 1 : 0;
+}
+
 int refprobe_sim_last_substeps(void *s) { return ((FluidSimulation *)s)->_outputData.frameData.substeps; }
 int refprobe_sim_pressure_iterations(void *s) { return ((FluidSimulation *)s)->_pressureSolverIterations; }
+int refprobe_sim_viscosity_stats(void *s, int *iterations, float *error) {
+    FluidSimulation *f = (FluidSimulation *)s;
+    *iterations = f->_viscositySolverIterations;
+    *error = f->_viscositySolverError;
+    return f->_viscositySolverSuccess ? 1 : 0;
+}
 
 }  // extern "C"

@@ -317,6 +317,107 @@ def condition_solid_velocity(n3, liquid_phi, weights, solid_velocity):
     return us, vs, ws
 
 
+def _visc_shapes(n3):
+    ni, nj, nk = n3
+    states = [(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]
+    volumes = [(nk, nj, ni), (nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni), (nk + 1, nj + 1, ni),
+               (nk + 1, nj, ni + 1), (nk, nj + 1, ni + 1)]
+    return states, volumes
+
+
+def _split(flat, shapes):
+    out, q = [], 0
+    for sh in shapes:
+        n = int(np.prod(sh))
+        out.append(flat[q:q + n].reshape(sh))
+        q += n
+    return out
+
+
+VISC_VOLUMES = ("center", "U", "V", "W", "edgeU", "edgeV", "edgeW")
+
+
+def _visc_dict(n3, m, states, volumes, table, rowstart, cols, vals, rhs):
+    sh_s, sh_v = _visc_shapes(n3)
+    return {"m": m, "states": _split(states, sh_s), "volumes": dict(zip(VISC_VOLUMES, _split(volumes, sh_v))),
+            "table": table, "rowstart": rowstart, "cols": cols, "vals": vals, "rhs": rhs}
+
+
+def _visc_buffers(n3, m, nnz):
+    sh_s, sh_v = _visc_shapes(n3)
+    ns = sum(int(np.prod(x)) for x in sh_s)
+    nv = sum(int(np.prod(x)) for x in sh_v)
+    return (np.zeros(ns, np.uint8), np.zeros(nv, np.float32), np.zeros(ns, np.int32), np.zeros(m + 1, np.int32),
+            np.zeros(nnz, np.int32), np.zeros(nnz, np.float32), np.zeros(m, np.float32))
+
+
+def viscosity_system(n3, dx, dt, liquid_phi, solid_nodes, viscosity, uvw):
+    """The oracle's viscosity system: face states, the seven volume grids, the
+    face-to-row table and the CSR matrix with its right-hand side."""
+    L = oracle()
+    L.oracle_viscosity_system_new.restype = C.c_void_p
+    L.oracle_viscosity_system_new.argtypes = [C.c_int] * 3 + [C.c_double] * 2 + [C.c_void_p] * 8
+    L.oracle_viscosity_system_get.argtypes = [C.c_void_p] * 8
+    L.oracle_viscosity_system_free.argtypes = [C.c_void_p]
+    m, nnz = C.c_int(), C.c_int()
+    h = L.oracle_viscosity_system_new(*n3, dx, dt, fp(_f32(liquid_phi)), fp(_f32(solid_nodes)),
+                                      fp(_f32(viscosity)), *[fp(_f32(a)) for a in uvw], C.byref(m), C.byref(nnz))
+    bufs = _visc_buffers(n3, m.value, nnz.value)
+    L.oracle_viscosity_system_get(h, *[fp(b) for b in bufs])
+    L.oracle_viscosity_system_free(h)
+    return _visc_dict(n3, m.value, *bufs)
+
+
+def ref_viscosity_system(n3, dx, dt, liquid_phi, solid_nodes, viscosity, uvw):
+    """The same pieces from the reference's ViscositySolver built from its sources."""
+    P = probe()
+    P.refprobe_viscosity_system.argtypes = [C.c_int] * 3 + [C.c_double] * 2 + [C.c_void_p] * 14
+    args = [fp(_f32(liquid_phi)), fp(_f32(solid_nodes)), fp(_f32(viscosity))] + [fp(_f32(a)) for a in uvw]
+    nnz = C.c_int()
+    m = P.refprobe_viscosity_system(*n3, dx, dt, *args, C.byref(nnz), *([None] * 7))
+    bufs = _visc_buffers(n3, m, nnz.value)
+    P.refprobe_viscosity_system(*n3, dx, dt, *args, C.byref(nnz), *[fp(b) for b in bufs])
+    return _visc_dict(n3, m, *bufs)
+
+
+def viscosity_pcg(system, tol=1e-4, max_iter=900, nthreads=1):
+    """PCGSolver<float> on a CSR system; returns x, iterations, error, success."""
+    L = oracle()
+    L.oracle_viscosity_pcg.argtypes = [C.c_int] + [C.c_void_p] * 4 + [C.c_double, C.c_int, C.c_int] + \
+        [C.c_void_p] * 3
+    m = system["m"]
+    x = np.zeros(m, np.float32)
+    it, err = C.c_int(), C.c_float()
+    ok = L.oracle_viscosity_pcg(m, *[fp(np.ascontiguousarray(system[k])) for k in ("rowstart", "cols", "vals", "rhs")],
+                                tol, max_iter, nthreads, fp(x), C.byref(it), C.byref(err))
+    return x, it.value, err.value, bool(ok)
+
+
+def viscosity_solve(n3, dx, dt, liquid_phi, solid_nodes, viscosity, uvw, tol=1e-4, max_iter=900, nthreads=1):
+    """The whole viscosity stage; returns (u, v, w), iterations, error, success, m."""
+    L = oracle()
+    L.oracle_viscosity_solve.argtypes = [C.c_int] * 3 + [C.c_double] * 2 + [C.c_void_p] * 3 + \
+        [C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 6
+    u, v, w = (np.array(a, np.float32, copy=True) for a in uvw)
+    it, err, ok = C.c_int(), C.c_float(), C.c_int()
+    m = L.oracle_viscosity_solve(*n3, dx, dt, fp(_f32(liquid_phi)), fp(_f32(solid_nodes)), fp(_f32(viscosity)),
+                                 tol, max_iter, nthreads, fp(u), fp(v), fp(w), C.byref(it), C.byref(err),
+                                 C.byref(ok))
+    return (u, v, w), it.value, err.value, bool(ok.value), m
+
+
+def ref_viscosity_solve(n3, dx, dt, liquid_phi, solid_nodes, viscosity, uvw, tol=1e-4, max_iter=900):
+    """ViscositySolver::applyViscosityToVelocityField of the reference build."""
+    P = probe()
+    P.refprobe_viscosity_solve.argtypes = [C.c_int] * 3 + [C.c_double] * 2 + [C.c_void_p] * 3 + \
+        [C.c_double, C.c_int] + [C.c_void_p] * 5
+    u, v, w = (np.array(a, np.float32, copy=True) for a in uvw)
+    it, err = C.c_int(), C.c_float()
+    ok = P.refprobe_viscosity_solve(*n3, dx, dt, fp(_f32(liquid_phi)), fp(_f32(solid_nodes)), fp(_f32(viscosity)),
+                                    tol, max_iter, fp(u), fp(v), fp(w), C.byref(it), C.byref(err))
+    return (u, v, w), it.value, err.value, bool(ok)
+
+
 def pressure_solve(n3, dx, dt, uvw, weights, liquid_phi, tol=1e-9, max_iter=900, nthreads=1):
     ni, nj, nk = n3
     u, v, w = (_f32(a) for a in uvw)
@@ -421,6 +522,19 @@ class OracleSim:
 
     def pressure_iterations(self):
         return oracle().oracle_sim_last_pressure_iterations(self.h)
+
+    def set_viscosity(self, visc, tol=1e-4, max_iter=900):
+        L = oracle()
+        L.oracle_sim_set_viscosity.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.oracle_sim_set_viscosity(self.h, visc, tol, max_iter)
+
+    def viscosity_stats(self):
+        """(success, iterations, error) of the last frame's viscosity solves."""
+        L = oracle()
+        L.oracle_sim_last_viscosity_stats.argtypes = [C.c_void_p] * 3
+        it, err = C.c_int(), C.c_float()
+        ok = L.oracle_sim_last_viscosity_stats(self.h, C.byref(it), C.byref(err))
+        return bool(ok), it.value, err.value
 
     def add_moving_obstacle(self, prev, cur, nxt, tris, animated=True, inversed=False, expansion=0.0,
                             velocity_scale=1.0):
@@ -614,6 +728,26 @@ class ReferenceSim:
         err = C.c_int()
         self.L.FluidSimulation_update(self.h, C.c_double(dt), C.byref(err))
         assert err.value == 1
+
+    def set_viscosity(self, visc, tol=None, max_iter=None):
+        """FluidSimulation_set_viscosity (+ solver tolerance and iteration limit)."""
+        L, err = self.L, C.c_int()
+        L.FluidSimulation_set_viscosity.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+        L.FluidSimulation_set_viscosity(self.h, visc, C.byref(err))
+        assert err.value == 1
+        if tol is not None:
+            L.FluidSimulation_set_viscosity_solver_error_tolerance.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+            L.FluidSimulation_set_viscosity_solver_error_tolerance(self.h, tol, C.byref(err))
+        if max_iter is not None:
+            L.FluidSimulation_set_viscosity_solver_max_iterations.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+            L.FluidSimulation_set_viscosity_solver_max_iterations(self.h, max_iter, C.byref(err))
+
+    def viscosity_stats(self):
+        P = probe()
+        P.refprobe_sim_viscosity_stats.argtypes = [C.c_void_p] * 3
+        it, e = C.c_int(), C.c_float()
+        ok = P.refprobe_sim_viscosity_stats(self.h, C.byref(it), C.byref(e))
+        return bool(ok), it.value, e.value
 
     def particles(self):
         err = C.c_int()

@@ -129,3 +129,25 @@ SOLID_UPDATE_TIMINGS = [(1 / 30, 1 / 30, 1 / 120), (1 / 30, 0.5 / 30, 1 / 240), 
 def frame_progress(frame_dt, remaining, step):
     """1 - (float)(remaining + step) / (float)frame_dt, in float."""
     return float(np.float32(1.0) - np.float32(remaining + step) / np.float32(frame_dt))
+
+
+def viscosity_scene(n, visc, seed=1, obstacle=True, liquid_seed=0):
+    """Inputs of one viscosity solve on an n^3 grid (cell size 1/n, frame
+    step 1/30): the dam-break block's liquid SDF post-processed against the
+    solid, the domain box's nodal SDF (united with a sphere obstacle when
+    obstacle is set), a constant viscosity on the (n+1)^3 nodes and random
+    face velocities.  Returns (n3, dx, dt, liquid_phi, solid_nodes,
+    viscosity, (u, v, w))."""
+    import oracle_lib as O
+    from ffengine import scenes
+    dx = 1.0 / n
+    n3 = (n, n, n)
+    solid = O.solid_box_phi(n3, dx)
+    if obstacle:
+        solid = scenes.union_phi(solid, scenes.sphere_phi_nodes(n, center=(0.6, 0.25, 0.5), radius=0.15))
+    pos = scenes.dam_break_particles(n, seed=liquid_seed)
+    phi = O.liquid_sdf_postprocess(n3, dx, solid, O.liquid_sdf(n3, dx, pos, 0.5 * dx * np.sqrt(3.0)))
+    rng = np.random.default_rng(seed)
+    uvw = tuple(rng.standard_normal(s).astype(np.float32) for s in [(n, n, n + 1), (n, n + 1, n), (n + 1, n, n)])
+    vg = np.full((n + 1, n + 1, n + 1), visc, np.float32)
+    return n3, dx, 1.0 / 30.0, phi, solid, vg, uvw
