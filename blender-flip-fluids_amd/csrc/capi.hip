@@ -205,6 +205,25 @@ void FluidSimulation_set_PICFLIP_ratio(FluidSimulation *o, double r, int *err) {
     guarded(err, [&] { S(o)->setPICFLIPRatio(r); });
 }
 
+int FluidSimulation_get_viscosity_solver_max_iterations(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getViscositySolverMaxIterations(); });
+}
+void FluidSimulation_set_viscosity_solver_max_iterations(FluidSimulation *o, int n, int *err) {
+    guarded(err, [&] { S(o)->setViscositySolverMaxIterations(n); });
+}
+double FluidSimulation_get_viscosity(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getViscosity(); });
+}
+void FluidSimulation_set_viscosity(FluidSimulation *o, double v, int *err) {
+    guarded(err, [&] { S(o)->setViscosity(v); });
+}
+double FluidSimulation_get_viscosity_solver_error_tolerance(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getViscositySolverErrorTolerance(); });
+}
+void FluidSimulation_set_viscosity_solver_error_tolerance(FluidSimulation *o, double tol, int *err) {
+    guarded(err, [&] { S(o)->setViscositySolverErrorTolerance(tol); });
+}
+
 void FluidSimulation_load_marker_particle_data(FluidSimulation *o, FluidSimulationMarkerParticleData d,
                                                int *err) {
     guarded(err, [&] { S(o)->loadMarkerParticleData(d.size, d.positions, d.velocities); });
@@ -454,6 +473,10 @@ void FluidSimulation_get_last_pressure_solve_stats(FluidSimulation *o, int *it, 
 }
 double FluidSimulation_get_last_pressure_solve_rhs_norm(FluidSimulation *o, int *err) {
     return guarded(err, [&] { return S(o)->getLastPressureRhsNorm(); });
+}
+void FluidSimulation_get_last_viscosity_solve_stats(FluidSimulation *o, int *it, double *e, int *ok, int *m,
+                                                    int *err) {
+    guarded(err, [&] { S(o)->getLastViscositySolveStats(it, e, ok, m); });
 }
 double FluidSimulation_get_last_substep_phase_ms(FluidSimulation *o, int phase, int *err) {
     return guarded(err, [&] { return S(o)->getLastPhaseMilliseconds(phase); });
@@ -764,6 +787,80 @@ void FFAMD_cell_weights(int ni, int nj, int nk, const float *phi, float *wc, int
         ff::cellWeights(ff::Grid{ni, nj, nk, 1.0}, d, c, sc.s);
         down(wc, c.p, N, sc.s);
         sync(sc.s);
+    });
+}
+
+namespace {
+// the viscosity stage's inputs on the device
+struct ViscInputs {
+    DBuf<float> liquid, solid, visc, u, v, w;
+    size_t nf[3];
+    ViscInputs(int ni, int nj, int nk, const float *lp, const float *sp, const float *vp, const float *hu,
+               const float *hv, const float *hw, hipStream_t s)
+        : liquid((size_t)ni * nj * nk), solid((size_t)(ni + 1) * (nj + 1) * (nk + 1)),
+          visc((size_t)(ni + 1) * (nj + 1) * (nk + 1)), u((size_t)(ni + 1) * nj * nk), v((size_t)ni * (nj + 1) * nk),
+          w((size_t)ni * nj * (nk + 1)), nf{(size_t)(ni + 1) * nj * nk, (size_t)ni * (nj + 1) * nk,
+                                           (size_t)ni * nj * (nk + 1)} {
+        const size_t nn = (size_t)(ni + 1) * (nj + 1) * (nk + 1);
+        up(liquid.p, lp, (size_t)ni * nj * nk, s);
+        up(solid.p, sp, nn, s);
+        up(visc.p, vp, nn, s);
+        up(u.p, hu, nf[0], s);
+        up(v.p, hv, nf[1], s);
+        up(w.p, hw, nf[2], s);
+    }
+};
+}  // namespace
+
+int FFAMD_viscosity_system(int ni, int nj, int nk, double dx, double dt, const float *liquid_phi,
+                           const float *solid_phi_nodes, const float *viscosity, const float *u, const float *v,
+                           const float *w, int *nnz, unsigned char *states, float *volumes, int *table,
+                           int *rowstart, int *cols, float *vals, float *rhs, int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        Scratch sc;
+        ViscInputs in(ni, nj, nk, liquid_phi, solid_phi_nodes, viscosity, u, v, w, sc.s);
+        ff::ViscositySolver vs;
+        const int m = vs.build(ff::Grid{ni, nj, nk, dx}, dt, in.liquid, in.solid, in.visc, in.u, in.v, in.w, sc.s);
+        ff::ViscosityHostSystem H;
+        vs.copySystem(H, sc.s);
+        *nnz = (int)H.cols.size();
+        if (states) std::copy(H.states.begin(), H.states.end(), states);
+        if (volumes) std::copy(H.volumes.begin(), H.volumes.end(), volumes);
+        if (table) std::copy(H.table.begin(), H.table.end(), table);
+        if (rowstart) std::copy(H.rowstart.begin(), H.rowstart.end(), rowstart);
+        if (cols) std::copy(H.cols.begin(), H.cols.end(), cols);
+        if (vals) std::copy(H.vals.begin(), H.vals.end(), vals);
+        if (rhs) std::copy(H.rhs.begin(), H.rhs.end(), rhs);
+        return m;
+    });
+}
+
+int FFAMD_viscosity_solve(int ni, int nj, int nk, double dx, double dt, const float *liquid_phi,
+                          const float *solid_phi_nodes, const float *viscosity, double tol, int max_iter, float *u,
+                          float *v, float *w, int *iterations, float *error, int *success, float *solution,
+                          int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (max_iter < 0) throw std::domain_error("Error: iteration limit must be >= 0.\n");
+        Scratch sc;
+        ViscInputs in(ni, nj, nk, liquid_phi, solid_phi_nodes, viscosity, u, v, w, sc.s);
+        ff::ViscositySolver vs;
+        const ff::ViscosityResult r = vs.solve(ff::Grid{ni, nj, nk, dx}, dt, in.liquid, in.solid, in.visc, tol,
+                                               max_iter, in.u, in.v, in.w, sc.s);
+        down(u, in.u.p, in.nf[0], sc.s);
+        down(v, in.v.p, in.nf[1], sc.s);
+        down(w, in.w.p, in.nf[2], sc.s);
+        sync(sc.s);
+        if (solution) {
+            std::vector<float> x;
+            vs.copySolution(x, sc.s);
+            std::copy(x.begin(), x.end(), solution);
+        }
+        *iterations = r.iterations;
+        *error = r.error;
+        *success = r.success ? 1 : 0;
+        return r.unknowns;
     });
 }
 

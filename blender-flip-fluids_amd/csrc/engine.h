@@ -19,6 +19,7 @@
 #include "../../include/ffengine_amd.h"
 #include "kernels.h"
 #include "pressure.h"
+#include "viscosity.h"
 #include "timers.h"
 
 namespace ff {
@@ -157,6 +158,18 @@ public:
     // the MIC(0) application's schedule: 2 = bands of hyperplanes, 4 = the line pipeline
     void setPressureSolverPreconditionerSchedule(int schedule) { _ps.setSchedule(schedule); }
     int getPressureSolverPreconditionerSchedule() const { return _ps.schedule(); }
+    // constant viscosity (fluidsimulation.cpp:2565-2601, 303-312): the first
+    // set enables the viscosity solve; a value of 0 keeps it enabled and
+    // skips it.  Negative values and tolerances raise.
+    void setViscosity(double v);
+    double getViscosity() const { return _viscosityValue; }
+    bool isViscosityEnabled() const { return _viscEnabled; }
+    void setViscositySolverErrorTolerance(double tol);
+    double getViscositySolverErrorTolerance() const { return _viscTolerance; }
+    void setViscositySolverMaxIterations(int n) { _viscMaxIterations = n; }
+    int getViscositySolverMaxIterations() const { return _viscMaxIterations; }
+    // the last viscosity solve: iterations, |r|_inf, success, matrix size
+    void getLastViscositySolveStats(int *iterations, double *error, int *success, int *unknowns) const;
     void setSolidPhi(const float *phiNodes);
 
     // static mesh obstacles (fluidsimulation.cpp:2990-3032); the solid SDF is
@@ -278,6 +291,21 @@ private:
     DevBuf<uint8_t> _vu, _vv, _vw, _stu, _stv, _stw;
 
     PressureSolver _ps;
+    // viscosity: the constant value on the (n+1)^3 nodes and its solver
+    bool _viscEnabled = false;
+    double _viscosityValue = 0.0;
+    float _viscosityGridValue = -1.0f;  // the value the node grid holds
+    double _viscTolerance = 1e-4;
+    int _viscMaxIterations = 900;
+    DevBuf<float> _viscNodes;
+    ViscositySolver _visc;
+    bool _vSuccess = true;
+    int _vIters = 0;
+    float _vErr = 0.0f;
+    ViscosityResult _lastVisc{};
+    void applyViscosity(double dt);
+    // the reference's _constrainVelocityField on one MAC field
+    void constrainField(float *u, float *v, float *w);
     DevBuf<double> _partial, _scal;
     DevBuf<int> _bins, _outl;
 
@@ -290,6 +318,7 @@ private:
     double _lastErr = 0.0;
     FFFrameStats _stats{};
     double _phaseMs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double _viscMs = 0;  // the last substep's viscosity stage
     hipEvent_t _ev[9] = {};
     KernelTimers _timers;
 

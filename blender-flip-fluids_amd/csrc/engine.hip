@@ -80,6 +80,26 @@ void FluidSimulation::setPICFLIPRatio(double r) {
         throw std::domain_error("Error: PICFLIP ratio must be in range [0.0, 1.0].\nratio: " + toString(r) + "\n");
     _picflip = r;
 }
+void FluidSimulation::setViscosity(double v) {
+    if (v < 0.0)
+        throw std::domain_error("Error: viscosity must be greater than or equal to 0.\nviscosity: " + toString(v) +
+                                "\n");
+    _viscEnabled = true;
+    _viscosityValue = v;
+}
+void FluidSimulation::setViscositySolverErrorTolerance(double tol) {
+    if (tol < 0.0)
+        throw std::domain_error(
+            "Error: viscosity solver error tolerance must be greater than or equal to 0.\nerror tolerance: " +
+            toString(tol) + "\n");
+    _viscTolerance = tol;
+}
+void FluidSimulation::getLastViscositySolveStats(int *iterations, double *error, int *success, int *unknowns) const {
+    *iterations = _lastVisc.iterations;
+    *error = _lastVisc.error;
+    *success = _lastVisc.success ? 1 : 0;
+    *unknowns = _lastVisc.unknowns;
+}
 void FluidSimulation::setCFLConditionNumber(int n) {
     if (n < 1)
         throw std::domain_error("Error: CFL must be greater than or equal to 1.\nCFL: " + toString(n) + "\n");
@@ -550,6 +570,7 @@ void FluidSimulation::update(double dt) {
     _pIters = 0;
     _pErr = 0.0f;
     double timing[4] = {0, 0, 0, 0};
+    double viscTotal = 0;
     size_t processed = 0;
     double eps = 1e-9;
     do {
@@ -568,6 +589,7 @@ void FluidSimulation::update(double dt) {
             timing[1] += _phaseMs[2] + _phaseMs[3] + _phaseMs[4];
             timing[2] += _phaseMs[5] + _phaseMs[6];
             timing[3] += _phaseMs[7];
+            viscTotal += _viscMs;
         }
         processed += _P.n;
         _stepNumber++;
@@ -590,14 +612,17 @@ void FluidSimulation::update(double dt) {
     st.pressureSolverError = (double)_pErr;
     st.pressureSolverIterations = _pIters;
     st.pressureSolverMaxIterations = _maxPressureIterations;
-    st.viscositySolverEnabled = 0;
-    st.viscositySolverSuccess = 1;
-    st.viscositySolverMaxIterations = 900;
+    st.viscositySolverEnabled = _viscEnabled ? 1 : 0;
+    st.viscositySolverSuccess = _vSuccess ? 1 : 0;
+    st.viscositySolverError = (double)_vErr;
+    st.viscositySolverIterations = _vIters;
+    st.viscositySolverMaxIterations = _viscMaxIterations;
     for (auto &m : st.mesh) m = FFMeshStats{0, -1, -1, 0};
     st.timing.total = total * 1e-3;
     st.timing.advection = timing[0] * 1e-3;
     st.timing.pressure = timing[1] * 1e-3;
     st.timing.particles = timing[2] * 1e-3;
+    st.timing.viscosity = viscTotal * 1e-3;
     _stats = st;
     _updatedOnce = true;
     _currentFrame++;
@@ -669,6 +694,8 @@ void FluidSimulation::stepFluid(double dt) {
     if (std::fabs(bf.x) > beps) addToFaces(_u, nu, (float)(bf.x * dt), _stream);
     if (std::fabs(bf.y) > beps) addToFaces(_v, nv, (float)(bf.y * dt), _stream);
     if (std::fabs(bf.z) > beps) addToFaces(_w, nw, (float)(bf.z * dt), _stream);
+    hipCheck(hipEventRecord(_ev[8], _stream), "event");
+    applyViscosity(dt);
     hipCheck(hipEventRecord(_ev[2], _stream), "event");
     // pressure (6152-6276)
     bool success = pressureSolve(dt);
@@ -687,24 +714,9 @@ void FluidSimulation::stepFluid(double dt) {
         _pErr = (float)_lastErr;
     }
     extrapolate(layers);
-    // constrain saved then current (6444-6459): closed faces take the solid's
-    // face velocity (zero for a static solid); with obstacle friction the
-    // partly open faces blend toward it
-    const float *svel[3] = {_movingSolid ? _us.p : nullptr, _movingSolid ? _vs.p : nullptr,
-                            _movingSolid ? _ws.p : nullptr};
-    if (_frictionActive) {
-        float *faces[6] = {_su, _sv, _sw, _u, _v, _w};
-        const float *wts[3] = {_wu, _wv, _ww};
-        for (int q = 0; q < 6; q++)
-            constrainFacesFriction(_g, q % 3, faces[q], wts[q % 3], _nodeObj, _objFric, _stream, svel[q % 3]);
-    } else {
-        constrainFaces(_su, _wu, nu, _stream, svel[0]);
-        constrainFaces(_sv, _wv, nv, _stream, svel[1]);
-        constrainFaces(_sw, _ww, nw, _stream, svel[2]);
-        constrainFaces(_u, _wu, nu, _stream, svel[0]);
-        constrainFaces(_v, _wv, nv, _stream, svel[1]);
-        constrainFaces(_w, _ww, nw, _stream, svel[2]);
-    }
+    // constrain saved then current (6444-6459)
+    constrainField(_su, _sv, _sw);
+    constrainField(_u, _v, _w);
     hipCheck(hipEventRecord(_ev[5], _stream), "event");
     // G2P (6932-6945) + RK3 advection and collision (7853-7890)
     _timers.begin(T_G2P_ADVECT, _stream);
@@ -720,7 +732,10 @@ void FluidSimulation::stepFluid(double dt) {
     hipCheck(hipEventSynchronize(_ev[7]), "event");
     float ms[7];
     hipCheck(hipEventElapsedTime(&ms[0], _ev[0], _ev[1]), "hipEventElapsedTime");
-    hipCheck(hipEventElapsedTime(&ms[1], _ev[1], _ev[2]), "hipEventElapsedTime");
+    hipCheck(hipEventElapsedTime(&ms[1], _ev[1], _ev[8]), "hipEventElapsedTime");
+    float viscMs = 0;
+    hipCheck(hipEventElapsedTime(&viscMs, _ev[8], _ev[2]), "hipEventElapsedTime");
+    _viscMs = viscMs;
     hipCheck(hipEventElapsedTime(&ms[2], _ev[2], _ev[3]), "hipEventElapsedTime");
     hipCheck(hipEventElapsedTime(&ms[3], _ev[3], _ev[4]), "hipEventElapsedTime");
     hipCheck(hipEventElapsedTime(&ms[4], _ev[4], _ev[5]), "hipEventElapsedTime");
@@ -731,6 +746,57 @@ void FluidSimulation::stepFluid(double dt) {
     for (int q = 0; q < 7; q++) _phaseMs[q] = ms[q];
     _phaseMs[7] = whole;
     _timers.collect();
+}
+
+// _constrainVelocityField (fluidsimulation.cpp:6355-6442): closed faces take
+// the solid's face velocity (zero for a static solid); with obstacle friction
+// the partly open faces blend toward it
+void FluidSimulation::constrainField(float *u, float *v, float *w) {
+    const float *svel[3] = {_movingSolid ? _us.p : nullptr, _movingSolid ? _vs.p : nullptr,
+                            _movingSolid ? _ws.p : nullptr};
+    float *faces[3] = {u, v, w};
+    const float *wts[3] = {_wu, _wv, _ww};
+    const size_t counts[3] = {_u.n, _v.n, _w.n};
+    for (int q = 0; q < 3; q++) {
+        if (_frictionActive)
+            constrainFacesFriction(_g, q, faces[q], wts[q], _nodeObj, _objFric, _stream, svel[q]);
+        else
+            constrainFaces(faces[q], wts[q], counts[q], _stream, svel[q]);
+    }
+}
+
+// _applyViscosityToVelocityField (fluidsimulation.cpp:5983-6056) with a
+// constant viscosity: skipped while the value is 0, else the field is
+// constrained and the viscosity solve runs on it; the frame keeps the first
+// failing, else the most iterated, solve's statistics
+void FluidSimulation::applyViscosity(double dt) {
+    if (!_viscEnabled || _P.n == 0) return;
+    const float value = (float)_viscosityValue;
+    if (value == 0.0f) return;
+    if (_viscosityGridValue != value || _viscNodes.n == 0) {
+        const size_t n = (size_t)(_g.ni + 1) * (_g.nj + 1) * (_g.nk + 1);
+        if (_viscNodes.n != n) _viscNodes.alloc(n);
+        std::vector<float> h(n, value);
+        hipCheck(hipMemcpyAsync(_viscNodes.p, h.data(), sizeof(float) * n, hipMemcpyHostToDevice, _stream),
+                 "viscosity H2D");
+        hipCheck(hipStreamSynchronize(_stream), "sync");
+        _viscosityGridValue = value;
+    }
+    constrainField(_u, _v, _w);
+    _timers.begin(T_VISCOSITY, _stream);
+    ViscosityResult r = _visc.solve(_g, dt, _phi, _solid, _viscNodes, _viscTolerance, _viscMaxIterations, _u, _v,
+                                    _w, _stream);
+    _timers.end(T_VISCOSITY, _stream);
+    _lastVisc = r;
+    if (_stepNumber == 0) {
+        _vSuccess = r.success;
+        _vIters = r.iterations;
+        _vErr = r.error;
+    } else if (_vSuccess && (!r.success || r.iterations > _vIters)) {
+        _vSuccess = r.success;
+        _vIters = r.iterations;
+        _vErr = r.error;
+    }
 }
 
 // PressureSolver::solve + applySolutionToVelocityField gate (fluidsimulation.cpp:6206-6249)
@@ -888,8 +954,8 @@ void FluidSimulation::getKernelTiming(int id, double *ms, long long *launches) c
     *launches = _timers.launches(id);
 }
 double FluidSimulation::getLastPhaseMilliseconds(int phase) {
-    if (phase < 0 || phase > 7) throw std::domain_error("Error: phase must be in [0, 7].\n");
-    return _phaseMs[phase];
+    if (phase < 0 || phase > 8) throw std::domain_error("Error: phase must be in [0, 8].\n");
+    return phase == 8 ? _viscMs : _phaseMs[phase];
 }
 
 }  // namespace ff

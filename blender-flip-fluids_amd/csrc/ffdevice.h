@@ -296,4 +296,59 @@ FF_HD float p2gWeightAt(const P2GWeight &W, float d2) {
     return 1.0f - W.coef1 * d2 * d2 * d2 + W.coef2 * d2 * d2 - W.coef3 * d2;
 }
 
+// ---- volume fractions of level sets ----
+// LevelsetUtils::volumeFraction of a tetrahedron (levelsetutils.cpp:213-258)
+__device__ inline void sort4(float &a, float &b, float &c, float &d) {
+    float t;
+    if (a > b) { t = a; a = b; b = t; }
+    if (c > d) { t = c; c = d; d = t; }
+    if (a > c) { t = a; a = c; c = t; }
+    if (b > d) { t = b; b = d; d = t; }
+    if (b > c) { t = b; b = c; c = t; }
+}
+__device__ inline float tetFraction(float p0, float p1, float p2, float p3) {
+    return p0 * p0 * p0 / ((p0 - p1) * (p0 - p2) * (p0 - p3));
+}
+__device__ inline float prismFraction(float p0, float p1, float p2, float p3) {
+    const float a = p0 / (p0 - p2), b = p0 / (p0 - p3), c = p1 / (p1 - p3), d = p1 / (p1 - p2);
+    return a * b * (1 - d) + b * (1 - c) * d + c * d;
+}
+__device__ inline float volumeFraction4(float p0, float p1, float p2, float p3) {
+    sort4(p0, p1, p2, p3);
+    if (p3 <= 0) return 1;
+    if (p2 <= 0) return 1 - tetFraction(p3, p2, p1, p0);
+    if (p1 <= 0) return prismFraction(p0, p1, p2, p3);
+    if (p0 <= 0) return tetFraction(p0, p1, p2, p3);
+    return 0;
+}
+// the cube's ten tetrahedra, the two central ones counted twice, / 12
+// (levelsetutils.h:60-100)
+__device__ inline float volumeFraction8(float p000, float p100, float p010, float p110, float p001, float p101, float p011,
+                                 float p111) {
+    return (volumeFraction4(p000, p001, p101, p011) + volumeFraction4(p000, p101, p100, p110) +
+            volumeFraction4(p000, p010, p011, p110) + volumeFraction4(p101, p011, p111, p110) +
+            2 * volumeFraction4(p000, p011, p101, p110) + volumeFraction4(p100, p101, p001, p111) +
+            volumeFraction4(p100, p001, p000, p010) + volumeFraction4(p100, p110, p111, p010) +
+            volumeFraction4(p001, p111, p011, p010) + 2 * volumeFraction4(p100, p111, p001, p010)) /
+           12.0f;
+}
+// Interpolation::trilinearInterpolate (interpolation.cpp:72-112) of a float
+// grid of pi x pj x pk samples at spacing dx whose sample (0,0,0) sits at the
+// origin; samples outside the grid count as 0
+__device__ inline float gridTrilinear(const float *a, int pi, int pj, int pk, double dx, V3 p) {
+    const I3 c = posToIndex(p, dx);
+    const V3 gp = indexToPos(c.i, c.j, c.k, dx);
+    const double inv_dx = 1.0 / dx;
+    const double ix = (double)(p.x - gp.x) * inv_dx, iy = (double)(p.y - gp.y) * inv_dx,
+                 iz = (double)(p.z - gp.z) * inv_dx;
+    double pts[8];
+    const int o[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 1}};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int x = c.i + o[q][0], y = c.j + o[q][1], z = c.k + o[q][2];
+        pts[q] = inRange(x, y, z, pi, pj, pk) ? (double)a[flat(x, y, z, pi, pj)] : 0.0;
+    }
+    return (float)trilin8(pts, ix, iy, iz);
+}
+
 }  // namespace ff

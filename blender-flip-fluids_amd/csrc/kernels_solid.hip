@@ -28,42 +28,6 @@ namespace {
 constexpr int TPB = 256;
 unsigned blocksFor(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// LevelsetUtils::volumeFraction of a tetrahedron (levelsetutils.cpp:213-258)
-__device__ void sort4(float &a, float &b, float &c, float &d) {
-    float t;
-    if (a > b) { t = a; a = b; b = t; }
-    if (c > d) { t = c; c = d; d = t; }
-    if (a > c) { t = a; a = c; c = t; }
-    if (b > d) { t = b; b = d; d = t; }
-    if (b > c) { t = b; b = c; c = t; }
-}
-__device__ float tetFraction(float p0, float p1, float p2, float p3) {
-    return p0 * p0 * p0 / ((p0 - p1) * (p0 - p2) * (p0 - p3));
-}
-__device__ float prismFraction(float p0, float p1, float p2, float p3) {
-    const float a = p0 / (p0 - p2), b = p0 / (p0 - p3), c = p1 / (p1 - p3), d = p1 / (p1 - p2);
-    return a * b * (1 - d) + b * (1 - c) * d + c * d;
-}
-__device__ float volumeFraction4(float p0, float p1, float p2, float p3) {
-    sort4(p0, p1, p2, p3);
-    if (p3 <= 0) return 1;
-    if (p2 <= 0) return 1 - tetFraction(p3, p2, p1, p0);
-    if (p1 <= 0) return prismFraction(p0, p1, p2, p3);
-    if (p0 <= 0) return tetFraction(p0, p1, p2, p3);
-    return 0;
-}
-// the cube's ten tetrahedra, the two central ones counted twice, / 12
-// (levelsetutils.h:60-100)
-__device__ float volumeFraction8(float p000, float p100, float p010, float p110, float p001, float p101, float p011,
-                                 float p111) {
-    return (volumeFraction4(p000, p001, p101, p011) + volumeFraction4(p000, p101, p100, p110) +
-            volumeFraction4(p000, p010, p011, p110) + volumeFraction4(p101, p011, p111, p110) +
-            2 * volumeFraction4(p000, p011, p101, p110) + volumeFraction4(p100, p101, p001, p111) +
-            volumeFraction4(p100, p001, p000, p010) + volumeFraction4(p100, p110, p111, p010) +
-            volumeFraction4(p001, p111, p011, p010) + 2 * volumeFraction4(p100, p111, p001, p010)) /
-           12.0f;
-}
-
 __global__ void cellWeightKernel(Grid g, const float *solid, float *wc) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (size_t)g.ni * g.nj * g.nk) return;

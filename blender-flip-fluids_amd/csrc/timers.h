@@ -25,6 +25,7 @@ enum TimerId {
     T_G2P_ADVECT,
     T_REMOVAL,
     T_APPLY_PRESSURE,
+    T_VISCOSITY,
     T_COUNT
 };
 

@@ -96,6 +96,12 @@ def _int_getter(name):
     return get
 
 
+def _require_at_least(value, low):
+    """method_decorators.check_ge_zero / check_ge (src/engine/ffengine/method_decorators.py)."""
+    if isinstance(value, numbers.Real) and value < low:
+        raise ValueError("Value must be greater than or equal to %s" % ("zero" if low == 0 else low))
+
+
 def _setter(name, ctype):
     def set_(self, value):
         f = getattr(lib, name)
@@ -190,6 +196,36 @@ class FluidSimulation(object):
                              _setter("FluidSimulation_set_PICFLIP_ratio", c_double))
     PICAPIC_ratio = property(lambda s: _call0(s, "FluidSimulation_get_PICAPIC_ratio", c_double),
                              _setter("FluidSimulation_set_PICAPIC_ratio", c_double))
+    # constant viscosity and its solver (src/engine/ffengine/fluidsimulation.py
+    # 335-346, 2267-2297): the first set enables the viscosity solve of every
+    # substep; the reference's decorators raise ValueError below 0 (below 1
+    # for the iteration limit)
+    @property
+    def viscosity(self):
+        return _call0(self, "FluidSimulation_get_viscosity", c_double)
+
+    @viscosity.setter
+    def viscosity(self, value):
+        _require_at_least(value, 0)
+        _setter("FluidSimulation_set_viscosity", c_double)(self, value)
+
+    @property
+    def viscosity_solver_error_tolerance(self):
+        return _call0(self, "FluidSimulation_get_viscosity_solver_error_tolerance", c_double)
+
+    @viscosity_solver_error_tolerance.setter
+    def viscosity_solver_error_tolerance(self, value):
+        _require_at_least(value, 0)
+        _setter("FluidSimulation_set_viscosity_solver_error_tolerance", c_double)(self, value)
+
+    @property
+    def viscosity_solver_max_iterations(self):
+        return _int_getter("FluidSimulation_get_viscosity_solver_max_iterations")(self)
+
+    @viscosity_solver_max_iterations.setter
+    def viscosity_solver_max_iterations(self, n):
+        _require_at_least(n, 1)
+        _setter("FluidSimulation_set_viscosity_solver_max_iterations", c_int)(self, int(n))
 
     @property
     def enable_surface_reconstruction(self):
@@ -459,6 +495,14 @@ class FluidSimulation(object):
         rhs = pb.execute_lib_func(f, [self()])
         return {"iterations": it.value, "error": err.value, "success": bool(ok.value), "unknowns": m.value,
                 "rhs_inf": rhs}
+
+    def get_last_viscosity_solve_stats(self):
+        f = lib.FluidSimulation_get_last_viscosity_solve_stats
+        pb.init_lib_func(f, [c_void_p] + [c_void_p] * 5, None)
+        it, err, ok, m, s = c_int(), c_double(), c_int(), c_int(), c_int()
+        f(self(), byref(it), byref(err), byref(ok), byref(m), byref(s))
+        pb.check_success(s.value, f.__name__ + " - ")
+        return {"iterations": it.value, "error": err.value, "success": bool(ok.value), "unknowns": m.value}
 
     def get_last_substep_phase_ms(self, phase):
         f = lib.FluidSimulation_get_last_substep_phase_ms

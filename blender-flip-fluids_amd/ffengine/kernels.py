@@ -150,6 +150,66 @@ def solid_update_moving(n3, dx, objects, frame_dt, progress):
     return phi, (u, v, w), draws
 
 
+def _visc_sizes(n3):
+    ni, nj, nk = n3
+    states = [(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]
+    volumes = [(nk, nj, ni), (nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni), (nk + 1, nj + 1, ni),
+               (nk + 1, nj, ni + 1), (nk, nj + 1, ni + 1)]
+    return states, volumes
+
+
+VISC_VOLUMES = ("center", "U", "V", "W", "edgeU", "edgeV", "edgeW")
+
+
+def _split(flat, shapes):
+    out, q = [], 0
+    for sh in shapes:
+        n = int(np.prod(sh))
+        out.append(flat[q:q + n].reshape(sh))
+        q += n
+    return out
+
+
+def viscosity_system(n3, dx, dt, liquid_phi, solid_nodes, viscosity, uvw):
+    """The viscosity stage's system built on the GPU (FFAMD_viscosity_system):
+    a dict with m, states (u, v, w faces), volumes (by name), table,
+    rowstart, cols, vals and rhs."""
+    ni, nj, nk = n3
+    sh_s, sh_v = _visc_sizes(n3)
+    ins = [_f32(liquid_phi), _f32(solid_nodes), _f32(viscosity)] + [_f32(a) for a in uvw]
+    argt = [c_int] * 3 + [c_double] * 2 + [c_void_p] * 14
+    nnz = c_int()
+    m = _run("FFAMD_viscosity_system", argt, [ni, nj, nk, dx, dt] + [_fp(a) for a in ins] + [byref(nnz)] +
+             [None] * 7, c_int)
+    ns = sum(int(np.prod(x)) for x in sh_s)
+    nv = sum(int(np.prod(x)) for x in sh_v)
+    bufs = [np.zeros(ns, np.uint8), np.zeros(nv, np.float32), np.zeros(ns, np.int32), np.zeros(m + 1, np.int32),
+            np.zeros(nnz.value, np.int32), np.zeros(nnz.value, np.float32), np.zeros(m, np.float32)]
+    _run("FFAMD_viscosity_system", argt, [ni, nj, nk, dx, dt] + [_fp(a) for a in ins] + [byref(nnz)] +
+         [_fp(b) for b in bufs], c_int)
+    states, volumes, table, rowstart, cols, vals, rhs = bufs
+    return {"m": m, "states": _split(states, sh_s), "volumes": dict(zip(VISC_VOLUMES, _split(volumes, sh_v))),
+            "table": table, "rowstart": rowstart, "cols": cols, "vals": vals, "rhs": rhs}
+
+
+def viscosity_solve(n3, dx, dt, liquid_phi, solid_nodes, viscosity, uvw, tol=1e-4, max_iter=900):
+    """The viscosity stage on the GPU (FFAMD_viscosity_solve): returns
+    ((u, v, w), iterations, error, success, m, solution on the rows)."""
+    ni, nj, nk = n3
+    u, v, w = (np.array(a, np.float32, copy=True) for a in uvw)
+    it, err, ok = c_int(), c_float(), c_int()
+    m = _run("FFAMD_viscosity_system", [c_int] * 3 + [c_double] * 2 + [c_void_p] * 14,
+             [ni, nj, nk, dx, dt, _fp(_f32(liquid_phi)), _fp(_f32(solid_nodes)), _fp(_f32(viscosity)), _fp(u),
+              _fp(v), _fp(w), byref(c_int())] + [None] * 7, c_int)
+    x = np.zeros(max(m, 1), np.float32)
+    m2 = _run("FFAMD_viscosity_solve", [c_int] * 3 + [c_double] * 2 + [c_void_p] * 3 + [c_double, c_int] +
+              [c_void_p] * 7,
+              [ni, nj, nk, dx, dt, _fp(_f32(liquid_phi)), _fp(_f32(solid_nodes)), _fp(_f32(viscosity)), tol,
+               max_iter, _fp(u), _fp(v), _fp(w), byref(it), byref(err), byref(ok), _fp(x)], c_int)
+    assert m2 == m
+    return (u, v, w), it.value, err.value, bool(ok.value), m, x[:m]
+
+
 def cell_weights(n3, phi_nodes):
     """1 - the solid's volume fraction of each cell, (k, j, i)."""
     ni, nj, nk = n3

@@ -126,6 +126,17 @@ double FluidSimulation_get_PICFLIP_ratio(FluidSimulation *obj, int *err);      /
 void FluidSimulation_set_PICFLIP_ratio(FluidSimulation *obj, double ratio, int *err); /* :3007-3013 */
 double FluidSimulation_get_PICAPIC_ratio(FluidSimulation *obj, int *err);      /* :3014-3018 */
 void FluidSimulation_set_PICAPIC_ratio(FluidSimulation *obj, double ratio, int *err); /* :3020-3026 */
+/* constant viscosity and its solver (fluidsimulation_c.cpp:333-346, 2728-2754;
+ * fluidsimulation.cpp:303-312, 2565-2601): the first set enables the
+ * viscosity solve after the body forces of every substep; negative values
+ * and tolerances are rejected. */
+int FluidSimulation_get_viscosity_solver_max_iterations(FluidSimulation *obj, int *err);     /* :333-338 */
+void FluidSimulation_set_viscosity_solver_max_iterations(FluidSimulation *obj, int n, int *err); /* :340-346 */
+double FluidSimulation_get_viscosity(FluidSimulation *obj, int *err);                        /* :2728-2732 */
+void FluidSimulation_set_viscosity(FluidSimulation *obj, double v, int *err);                /* :2734-2740 */
+double FluidSimulation_get_viscosity_solver_error_tolerance(FluidSimulation *obj, int *err); /* :2742-2746 */
+void FluidSimulation_set_viscosity_solver_error_tolerance(FluidSimulation *obj, double tol,
+                                                          int *err);                          /* :2748-2754 */
 
 /* ---- particle I/O ---- */
 void FluidSimulation_load_marker_particle_data(FluidSimulation *obj,
@@ -305,15 +316,22 @@ void FluidSimulation_get_last_pressure_solve_stats(FluidSimulation *obj, int *it
  * reference): with get_last_pressure_solve_stats' |r|_inf it gives the
  * relative residual the stop test uses. */
 double FluidSimulation_get_last_pressure_solve_rhs_norm(FluidSimulation *obj, int *err);
-/* GPU milliseconds of the last substep's phases: 0 level sets, 1 P2G and
- * extrapolation, 2 pressure system, 3 PCG, 4 projection and constraints,
- * 5 G2P and advection, 6 removal, 7 whole substep. */
+/* iterations, |r|_inf, success flag and matrix size of the last viscosity
+ * solve (no counterpart in the reference, which logs them) */
+void FluidSimulation_get_last_viscosity_solve_stats(FluidSimulation *obj, int *iterations,
+                                                    double *error, int *success, int *unknowns,
+                                                    int *err);
+/* GPU milliseconds of the last substep's phases: 0 level sets, 1 P2G,
+ * extrapolation and body forces, 2 pressure system, 3 PCG, 4 projection and
+ * constraints, 5 G2P and advection, 6 removal, 7 whole substep, 8 the
+ * viscosity solve. */
 double FluidSimulation_get_last_substep_phase_ms(FluidSimulation *obj, int phase, int *err);
 /* HIP-event timing of kernel groups on the engine's stream (off by default).
  * Enabling resets the counters; ids: 0 cell sort, 1 liquid SDF, 2 P2G,
  * 3 extrapolation, 4 pressure assembly, 5 MIC(0) factor, 6 SpMV, 7 MIC(0)
  * apply (both sweeps), 8 (unused), 9 G2P + advection, 10 removal,
- * 11 pressure gradient.  ms = summed kernel time, launches = kernel count. */
+ * 11 pressure gradient, 12 viscosity solve.  ms = summed kernel time,
+ * launches = kernel count. */
 void FluidSimulation_enable_kernel_timing(FluidSimulation *obj, int on, int *err);
 void FluidSimulation_get_kernel_timing(FluidSimulation *obj, int id, double *ms, long long *launches,
                                        int *err);
@@ -377,6 +395,25 @@ int FFAMD_solid_update_moving(int isize, int jsize, int ksize, double dx, int no
                               const int *triangles, const int *flags, const float *expansion,
                               const float *velocity_scale, double frame_dt, float frame_progress,
                               float *phi_nodes, float *u, float *v, float *w, int *err);
+/* The viscosity stage (ViscositySolver, viscositysolver.cpp:50-79) on host
+ * arrays: liquid_phi cell-centred, solid_phi_nodes and viscosity on the
+ * (n+1)^3 nodes, u/v/w the MAC field.
+ * FFAMD_viscosity_system builds the system and returns the matrix size m,
+ * writing the nonzero count to *nnz; outputs (NULL skips): the face states
+ * of the u, v, w faces (0 air, 1 fluid, 2 solid), the volume grids center,
+ * U, V, W, edgeU, edgeV, edgeW concatenated, the face-to-row table (-1 off
+ * the matrix), the CSR rows (rowstart[m+1], ascending cols, vals) and the
+ * right-hand side.  FFAMD_viscosity_solve runs the stage in place on u, v,
+ * w (cleared and set on the matrix faces when the solve succeeds) and
+ * returns m; solution (NULL skips) receives the m-row solution. */
+int FFAMD_viscosity_system(int isize, int jsize, int ksize, double dx, double dt, const float *liquid_phi,
+                           const float *solid_phi_nodes, const float *viscosity, const float *u, const float *v,
+                           const float *w, int *nnz, unsigned char *states, float *volumes, int *table,
+                           int *rowstart, int *cols, float *vals, float *rhs, int *err);
+int FFAMD_viscosity_solve(int isize, int jsize, int ksize, double dx, double dt, const float *liquid_phi,
+                          const float *solid_phi_nodes, const float *viscosity, double tol, int max_iter, float *u,
+                          float *v, float *w, int *iterations, float *error, int *success, float *solution,
+                          int *err);
 /* the cells' solid weights 1 - volume fraction of the solid
  * (fluidsimulation.cpp:6140-6147, meshlevelset.cpp:1491-1515) */
 void FFAMD_cell_weights(int isize, int jsize, int ksize, const float *phi_nodes, float *wc, int *err);

@@ -2727,6 +2727,21 @@ void viscositySystem(double dx, double dt, const float *visc, const float *u, co
     auto SU = [&](int i, int j, int k) { return S.su[S.U(i, j, k)]; };
     auto SV = [&](int i, int j, int k) { return S.sv[S.V(i, j, k)]; };
     auto SW = [&](int i, int j, int k) { return S.sw[S.W(i, j, k)]; };
+    // A row's neighbour can lie outside its face grid (U(i, nj, k),
+    // U(i, j, nk), V(ni, j, k), V(i, j, nk), W(ni, j, k), W(i, nj, k)).  The
+    // reference then reads the state and matrix index at the wrapped flat
+    // index, which is a border face (never a row) or lies past the array, and
+    // the velocity as 0 (MACVelocityField::U, macvelocityfield.cpp:112-118):
+    // the neighbour adds nothing.  Here it is air.
+    auto stU = [&](int i, int j, int k) -> unsigned char {
+        return inRange(i, j, k, ni + 1, nj, nk) ? SU(i, j, k) : (unsigned char)VF_AIR;
+    };
+    auto stV = [&](int i, int j, int k) -> unsigned char {
+        return inRange(i, j, k, ni, nj + 1, nk) ? SV(i, j, k) : (unsigned char)VF_AIR;
+    };
+    auto stW = [&](int i, int j, int k) -> unsigned char {
+        return inRange(i, j, k, ni, nj, nk + 1) ? SW(i, j, k) : (unsigned char)VF_AIR;
+    };
     std::vector<char> in(nU + nV + nW, 0);
     for (int k = 1; k < nk; k++)
         for (int j = 1; j < nj; j++)
@@ -2767,13 +2782,19 @@ void viscositySystem(double dx, double dt, const float *visc, const float *u, co
     const float fdx = (float)dx, fdt = (float)dt;
     const float invdx = 1.0f / fdx;
     const float factor = fdt * invdx * invdx;
+    auto rU = [&](int i, int j, int k) { return inRange(i, j, k, ni + 1, nj, nk) ? S.rowU(i, j, k) : -1; };
+    auto rV = [&](int i, int j, int k) { return inRange(i, j, k, ni, nj + 1, nk) ? S.rowV(i, j, k) : -1; };
+    auto rW = [&](int i, int j, int k) { return inRange(i, j, k, ni, nj, nk + 1) ? S.rowW(i, j, k) : -1; };
+    auto Uo = [&](int i, int j, int k) { return inRange(i, j, k, ni + 1, nj, nk) ? Uv(i, j, k) : 0.0f; };
+    auto Vo = [&](int i, int j, int k) { return inRange(i, j, k, ni, nj + 1, nk) ? Vv(i, j, k) : 0.0f; };
+    auto Wo = [&](int i, int j, int k) { return inRange(i, j, k, ni, nj, nk + 1) ? Wv(i, j, k) : 0.0f; };
     // the u rows (559-648); the v and w rows are the same operator with the
     // axes cycled (650-830)
     for (int k = 1; k < nk; k++)
         for (int j = 1; j < nj; j++)
             for (int i = 1; i < ni; i++) {
                 if (SU(i, j, k) != VF_FLUID) continue;
-                const int row = S.rowU(i, j, k);
+                const int row = rU(i, j, k);
                 if (row == -1) continue;
                 float vR = VS(i, j, k), vL = VS(i - 1, j, k);
                 float vT = 0.25f * (VS(i - 1, j + 1, k) + VS(i - 1, j, k) + VS(i, j + 1, k) + VS(i, j, k));
@@ -2787,23 +2808,23 @@ void viscositySystem(double dx, double dt, const float *visc, const float *u, co
                 S.A.put(row, row, diag, false);
                 struct E { unsigned char st; int col; float coef; float vel; };
                 const E es[14] = {
-                    {SU(i + 1, j, k), S.rowU(i + 1, j, k), -fR, Uv(i + 1, j, k)},
-                    {SU(i - 1, j, k), S.rowU(i - 1, j, k), -fL, Uv(i - 1, j, k)},
-                    {SU(i, j + 1, k), S.rowU(i, j + 1, k), -fT, Uv(i, j + 1, k)},
-                    {SU(i, j - 1, k), S.rowU(i, j - 1, k), -fB, Uv(i, j - 1, k)},
-                    {SU(i, j, k + 1), S.rowU(i, j, k + 1), -fF, Uv(i, j, k + 1)},
-                    {SU(i, j, k - 1), S.rowU(i, j, k - 1), -fK, Uv(i, j, k - 1)},
-                    {SV(i, j + 1, k), S.rowV(i, j + 1, k), -fT, Vv(i, j + 1, k)},
-                    {SV(i - 1, j + 1, k), S.rowV(i - 1, j + 1, k), fT, Vv(i - 1, j + 1, k)},
-                    {SV(i, j, k), S.rowV(i, j, k), fB, Vv(i, j, k)},
-                    {SV(i - 1, j, k), S.rowV(i - 1, j, k), -fB, Vv(i - 1, j, k)},
-                    {SW(i, j, k + 1), S.rowW(i, j, k + 1), -fF, Wv(i, j, k + 1)},
-                    {SW(i - 1, j, k + 1), S.rowW(i - 1, j, k + 1), fF, Wv(i - 1, j, k + 1)},
-                    {SW(i, j, k), S.rowW(i, j, k), fK, Wv(i, j, k)},
-                    {SW(i - 1, j, k), S.rowW(i - 1, j, k), -fK, Wv(i - 1, j, k)}};
+                    {stU(i + 1, j, k), rU(i + 1, j, k), -fR, Uo(i + 1, j, k)},
+                    {stU(i - 1, j, k), rU(i - 1, j, k), -fL, Uo(i - 1, j, k)},
+                    {stU(i, j + 1, k), rU(i, j + 1, k), -fT, Uo(i, j + 1, k)},
+                    {stU(i, j - 1, k), rU(i, j - 1, k), -fB, Uo(i, j - 1, k)},
+                    {stU(i, j, k + 1), rU(i, j, k + 1), -fF, Uo(i, j, k + 1)},
+                    {stU(i, j, k - 1), rU(i, j, k - 1), -fK, Uo(i, j, k - 1)},
+                    {stV(i, j + 1, k), rV(i, j + 1, k), -fT, Vo(i, j + 1, k)},
+                    {stV(i - 1, j + 1, k), rV(i - 1, j + 1, k), fT, Vo(i - 1, j + 1, k)},
+                    {stV(i, j, k), rV(i, j, k), fB, Vo(i, j, k)},
+                    {stV(i - 1, j, k), rV(i - 1, j, k), -fB, Vo(i - 1, j, k)},
+                    {stW(i, j, k + 1), rW(i, j, k + 1), -fF, Wo(i, j, k + 1)},
+                    {stW(i - 1, j, k + 1), rW(i - 1, j, k + 1), fF, Wo(i - 1, j, k + 1)},
+                    {stW(i, j, k), rW(i, j, k), fK, Wo(i, j, k)},
+                    {stW(i - 1, j, k), rW(i - 1, j, k), -fK, Wo(i - 1, j, k)}};
                 for (const E &e : es)
                     if (e.st == VF_FLUID) S.A.put(row, e.col, e.coef, true);
-                float rval = VU(i, j, k) * Uv(i, j, k);
+                float rval = VU(i, j, k) * Uo(i, j, k);
                 for (const E &e : es)
                     if (e.st == VF_SOLID) rval -= e.coef * e.vel;
                 S.rhs[row] = rval;
@@ -2812,7 +2833,7 @@ void viscositySystem(double dx, double dt, const float *visc, const float *u, co
         for (int j = 1; j < nj; j++)
             for (int i = 1; i < ni; i++) {
                 if (SV(i, j, k) != VF_FLUID) continue;
-                const int row = S.rowV(i, j, k);
+                const int row = rV(i, j, k);
                 if (row == -1) continue;
                 float vT = VS(i, j, k), vB = VS(i, j - 1, k);
                 float vR = 0.25f * (VS(i, j - 1, k) + VS(i + 1, j - 1, k) + VS(i, j, k) + VS(i + 1, j, k));
@@ -2826,23 +2847,23 @@ void viscositySystem(double dx, double dt, const float *visc, const float *u, co
                 S.A.put(row, row, diag, false);
                 struct E { unsigned char st; int col; float coef; float vel; };
                 const E es[14] = {
-                    {SV(i + 1, j, k), S.rowV(i + 1, j, k), -fR, Vv(i + 1, j, k)},
-                    {SV(i - 1, j, k), S.rowV(i - 1, j, k), -fL, Vv(i - 1, j, k)},
-                    {SV(i, j + 1, k), S.rowV(i, j + 1, k), -fT, Vv(i, j + 1, k)},
-                    {SV(i, j - 1, k), S.rowV(i, j - 1, k), -fB, Vv(i, j - 1, k)},
-                    {SV(i, j, k + 1), S.rowV(i, j, k + 1), -fF, Vv(i, j, k + 1)},
-                    {SV(i, j, k - 1), S.rowV(i, j, k - 1), -fK, Vv(i, j, k - 1)},
-                    {SU(i + 1, j, k), S.rowU(i + 1, j, k), -fR, Uv(i + 1, j, k)},
-                    {SU(i + 1, j - 1, k), S.rowU(i + 1, j - 1, k), fR, Uv(i + 1, j - 1, k)},
-                    {SU(i, j, k), S.rowU(i, j, k), fL, Uv(i, j, k)},
-                    {SU(i, j - 1, k), S.rowU(i, j - 1, k), -fL, Uv(i, j - 1, k)},
-                    {SW(i, j, k + 1), S.rowW(i, j, k + 1), -fF, Wv(i, j, k + 1)},
-                    {SW(i, j - 1, k + 1), S.rowW(i, j - 1, k + 1), fF, Wv(i, j - 1, k + 1)},
-                    {SW(i, j, k), S.rowW(i, j, k), fK, Wv(i, j, k)},
-                    {SW(i, j - 1, k), S.rowW(i, j - 1, k), -fK, Wv(i, j - 1, k)}};
+                    {stV(i + 1, j, k), rV(i + 1, j, k), -fR, Vo(i + 1, j, k)},
+                    {stV(i - 1, j, k), rV(i - 1, j, k), -fL, Vo(i - 1, j, k)},
+                    {stV(i, j + 1, k), rV(i, j + 1, k), -fT, Vo(i, j + 1, k)},
+                    {stV(i, j - 1, k), rV(i, j - 1, k), -fB, Vo(i, j - 1, k)},
+                    {stV(i, j, k + 1), rV(i, j, k + 1), -fF, Vo(i, j, k + 1)},
+                    {stV(i, j, k - 1), rV(i, j, k - 1), -fK, Vo(i, j, k - 1)},
+                    {stU(i + 1, j, k), rU(i + 1, j, k), -fR, Uo(i + 1, j, k)},
+                    {stU(i + 1, j - 1, k), rU(i + 1, j - 1, k), fR, Uo(i + 1, j - 1, k)},
+                    {stU(i, j, k), rU(i, j, k), fL, Uo(i, j, k)},
+                    {stU(i, j - 1, k), rU(i, j - 1, k), -fL, Uo(i, j - 1, k)},
+                    {stW(i, j, k + 1), rW(i, j, k + 1), -fF, Wo(i, j, k + 1)},
+                    {stW(i, j - 1, k + 1), rW(i, j - 1, k + 1), fF, Wo(i, j - 1, k + 1)},
+                    {stW(i, j, k), rW(i, j, k), fK, Wo(i, j, k)},
+                    {stW(i, j - 1, k), rW(i, j - 1, k), -fK, Wo(i, j - 1, k)}};
                 for (const E &e : es)
                     if (e.st == VF_FLUID) S.A.put(row, e.col, e.coef, true);
-                float rval = VV(i, j, k) * Vv(i, j, k);
+                float rval = VV(i, j, k) * Vo(i, j, k);
                 for (const E &e : es)
                     if (e.st == VF_SOLID) rval -= e.coef * e.vel;
                 S.rhs[row] = rval;
@@ -2851,7 +2872,7 @@ void viscositySystem(double dx, double dt, const float *visc, const float *u, co
         for (int j = 1; j < nj; j++)
             for (int i = 1; i < ni; i++) {
                 if (SW(i, j, k) != VF_FLUID) continue;
-                const int row = S.rowW(i, j, k);
+                const int row = rW(i, j, k);
                 if (row == -1) continue;
                 float vF = VS(i, j, k), vK = VS(i, j, k - 1);
                 float vR = 0.25f * (VS(i, j, k) + VS(i, j, k - 1) + VS(i + 1, j, k) + VS(i + 1, j, k - 1));
@@ -2865,23 +2886,23 @@ void viscositySystem(double dx, double dt, const float *visc, const float *u, co
                 S.A.put(row, row, diag, false);
                 struct E { unsigned char st; int col; float coef; float vel; };
                 const E es[14] = {
-                    {SW(i + 1, j, k), S.rowW(i + 1, j, k), -fR, Wv(i + 1, j, k)},
-                    {SW(i - 1, j, k), S.rowW(i - 1, j, k), -fL, Wv(i - 1, j, k)},
-                    {SW(i, j + 1, k), S.rowW(i, j + 1, k), -fT, Wv(i, j + 1, k)},
-                    {SW(i, j - 1, k), S.rowW(i, j - 1, k), -fB, Wv(i, j - 1, k)},
-                    {SW(i, j, k + 1), S.rowW(i, j, k + 1), -fF, Wv(i, j, k + 1)},
-                    {SW(i, j, k - 1), S.rowW(i, j, k - 1), -fK, Wv(i, j, k - 1)},
-                    {SU(i + 1, j, k), S.rowU(i + 1, j, k), -fR, Uv(i + 1, j, k)},
-                    {SU(i + 1, j, k - 1), S.rowU(i + 1, j, k - 1), fR, Uv(i + 1, j, k - 1)},
-                    {SU(i, j, k), S.rowU(i, j, k), fL, Uv(i, j, k)},
-                    {SU(i, j, k - 1), S.rowU(i, j, k - 1), -fL, Uv(i, j, k - 1)},
-                    {SV(i, j + 1, k), S.rowV(i, j + 1, k), -fT, Vv(i, j + 1, k)},
-                    {SV(i, j + 1, k - 1), S.rowV(i, j + 1, k - 1), fT, Vv(i, j + 1, k - 1)},
-                    {SV(i, j, k), S.rowV(i, j, k), fB, Vv(i, j, k)},
-                    {SV(i, j, k - 1), S.rowV(i, j, k - 1), -fB, Vv(i, j, k - 1)}};
+                    {stW(i + 1, j, k), rW(i + 1, j, k), -fR, Wo(i + 1, j, k)},
+                    {stW(i - 1, j, k), rW(i - 1, j, k), -fL, Wo(i - 1, j, k)},
+                    {stW(i, j + 1, k), rW(i, j + 1, k), -fT, Wo(i, j + 1, k)},
+                    {stW(i, j - 1, k), rW(i, j - 1, k), -fB, Wo(i, j - 1, k)},
+                    {stW(i, j, k + 1), rW(i, j, k + 1), -fF, Wo(i, j, k + 1)},
+                    {stW(i, j, k - 1), rW(i, j, k - 1), -fK, Wo(i, j, k - 1)},
+                    {stU(i + 1, j, k), rU(i + 1, j, k), -fR, Uo(i + 1, j, k)},
+                    {stU(i + 1, j, k - 1), rU(i + 1, j, k - 1), fR, Uo(i + 1, j, k - 1)},
+                    {stU(i, j, k), rU(i, j, k), fL, Uo(i, j, k)},
+                    {stU(i, j, k - 1), rU(i, j, k - 1), -fL, Uo(i, j, k - 1)},
+                    {stV(i, j + 1, k), rV(i, j + 1, k), -fT, Vo(i, j + 1, k)},
+                    {stV(i, j + 1, k - 1), rV(i, j + 1, k - 1), fT, Vo(i, j + 1, k - 1)},
+                    {stV(i, j, k), rV(i, j, k), fB, Vo(i, j, k)},
+                    {stV(i, j, k - 1), rV(i, j, k - 1), -fB, Vo(i, j, k - 1)}};
                 for (const E &e : es)
                     if (e.st == VF_FLUID) S.A.put(row, e.col, e.coef, true);
-                float rval = VW(i, j, k) * Wv(i, j, k);
+                float rval = VW(i, j, k) * Wo(i, j, k);
                 for (const E &e : es)
                     if (e.st == VF_SOLID) rval -= e.coef * e.vel;
                 S.rhs[row] = rval;
