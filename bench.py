@@ -30,7 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 KERNELS = ["cell_sort", "liquid_sdf", "p2g", "extrapolate", "pressure_assembly", "mic0_factor", "spmv",
-           "mic0_apply", "unused", "g2p_advect", "removal", "apply_pressure"]
+           "mic0_apply", "unused", "g2p_advect", "removal", "apply_pressure", "viscosity"]
 
 
 def log(msg):
@@ -442,6 +442,72 @@ def moving_config(n, steps, warmup, tol, mode, speed):
     print(json.dumps(line), flush=True)
 
 
+def viscous_config(n, steps, warmup, tol, mode, visc):
+    """The 256^3 dam break with its sphere obstacle (the 20480-triangle
+    icosphere MeshObject) and a constant viscosity: every substep runs the
+    viscosity solve (the reference's ViscositySolver system, solved on the
+    GPU by Jacobi-preconditioned CG to the reference's tolerance 1e-4) after
+    the body forces.  Reports the substep rate, the viscosity stage's share
+    and iterations, and the rate of the same run without viscosity."""
+    import torch
+    from ffengine import FluidSimulation, MeshObject, TriangleMesh, scenes
+    frame_dt = 1.0 / 240.0
+    pos = scenes.dam_break_particles(n, seed=0)
+    v, t = scenes.sphere_mesh(subdivisions=5)
+    out = {}
+    for on in (False, True):
+        sim = FluidSimulation(n, n, n, 1.0 / n)
+        sim.add_body_force(0.0, -9.81, 0.0)
+        sim.PICFLIP_ratio = 0.05
+        sim.CFL_condition_number = 5
+        sim.enable_surface_reconstruction = False
+        sim.pressure_solver_tolerance = tol
+        sim.set_pressure_solver_precision(mode)
+        if on:
+            sim.viscosity = visc
+        mo = MeshObject(n, n, n, 1.0 / n)
+        mo.update_mesh_static(TriangleMesh(v, t))
+        sim.add_mesh_obstacle(mo)
+        sim.load_marker_particle_data(len(pos), pos.tobytes(), np.zeros_like(pos).tobytes())
+        sim.initialize()
+        iters, viters, vms, vrows, substeps, elapsed = [], [], [], [], 0, 0.0
+        for f in range(warmup + steps):
+            if f == warmup:
+                enable_timing(sim, True)  # counts the timed frames only
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.update(frame_dt)
+            torch.cuda.synchronize()
+            if f >= warmup:
+                elapsed += time.perf_counter() - t0
+                st = sim.get_frame_stats_data()
+                substeps += st.substeps
+                iters.append(st.pressure_solver_iterations)
+                if on:
+                    vs = sim.get_last_viscosity_solve_stats()
+                    viters.append(vs["iterations"])
+                    vrows.append(vs["unknowns"])
+                    vms.append(round(sim.get_last_substep_phase_ms(8), 2))
+            log("  %s frame %d/%d" % ("viscous" if on else "inviscid", f + 1, warmup + steps))
+        rec = {"substeps_per_s": round(substeps / elapsed, 4), "ms_per_substep": round(1e3 * elapsed / substeps, 2),
+               "pressure_iterations": iters, "particles": sim.get_num_marker_particles()}
+        if on:
+            ms = kernel_times(sim)["viscosity"]["ms"]
+            rec.update({"viscosity_iterations": viters, "viscosity_rows": vrows, "viscosity_ms_last_substep": vms,
+                        "viscosity_ms_per_substep": round(ms / substeps, 2)})
+        out["viscous" if on else "inviscid"] = rec
+        del sim, mo
+    line = {"metric": "FLIP substeps/s, dam break %d^3 with viscosity %g" % (n, visc),
+            "value": out["viscous"]["substeps_per_s"], "unit": "substeps/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "higher_is_better": True, "data": "synthetic",
+            "dtype": "f32+f64" if mode != "fp64" else "f64", "mode": mode,
+            "config": {"workload": "dam break %d^3, %d particles, the 20480-triangle sphere MeshObject (r 0.15 at "
+                                   "(0.6, 0.2, 0.5)), constant viscosity %g (solver tolerance 1e-4), frame 1/240 s, "
+                                   "PCG rel res %g" % (n, len(pos), visc, tol), "grid": n, "particles": len(pos)},
+            "viscous": out["viscous"], "inviscid": out["inviscid"]}
+    print(json.dumps(line), flush=True)
+
+
 def adjoint_config(n, substeps, checkpoint_every, fd_faces, eps, tol, dt):
     """The differentiation configuration: a dam break (block [0, 0.375) x
     [0, 0.625) x [0, 1), 8 particles per cell outside the walls) whose initial
@@ -558,13 +624,16 @@ def adjoint_config(n, substeps, checkpoint_every, fd_faces, eps, tol, dt):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="dam-break",
-                    choices=["dam-break", "precision", "splash", "adjoint", "moving", "cpu-calibration"],
+                    choices=["dam-break", "precision", "splash", "adjoint", "moving", "viscous",
+                             "cpu-calibration"],
                     help="dam-break: the 256^3 headline benchmark (default); precision: the 384^3 slab "
                          "pressure-accuracy comparison; splash: the 512^3 splash; adjoint: the 128^3 "
                          "differentiable rollout with its finite-difference check; moving: the 256^3 dam break "
-                         "with its sphere obstacle animated, beside the static run; cpu-calibration (no GPU, "
+                         "with its sphere obstacle animated, beside the static run; viscous: the 256^3 dam "
+                         "break with a constant viscosity, beside the inviscid run; cpu-calibration (no GPU, "
                          "needs the reference build): the reference engine against the oracle on the host")
     ap.add_argument("--speed", type=float, default=0.3, help="moving: the sphere's speed along +z (m/s)")
+    ap.add_argument("--viscosity", type=float, default=0.5, help="viscous: the constant viscosity")
     ap.add_argument("--substeps", type=int, default=32, help="adjoint: substeps of the rollout")
     ap.add_argument("--checkpoint-every", type=int, default=8, help="adjoint: checkpoint spacing")
     ap.add_argument("--fd-faces", type=int, default=64, help="adjoint: faces checked by central differences")
@@ -622,6 +691,11 @@ def main():
         modes = [m for m in args.modes.split(",") if m]
         moving_config(args.n or 256, args.steps, args.warmup, args.tol,
                       modes[0] if len(modes) == 1 else "fp32-preconditioner", args.speed)
+        return
+    if args.config == "viscous":
+        modes = [m for m in args.modes.split(",") if m]
+        viscous_config(args.n or 256, args.steps, args.warmup, args.tol,
+                       modes[0] if len(modes) == 1 else "fp32-preconditioner", args.viscosity)
         return
     if args.config == "splash":
         modes = [m for m in args.modes.split(",") if m]
