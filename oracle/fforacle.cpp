@@ -1802,6 +1802,291 @@ void oracle_face_weights(int ni, int nj, int nk, const float *sp, float *wu, flo
 
 namespace {
 
+float nodalTrilinear(const float *g, int pi, int pj, int pk, double dx, V3 p);
+double nodalTrilinearD(const float *g, int pi, int pj, int pk, double dx, V3 p);
+
+// ---- surface tension ----
+//
+// ParticleLevelSet::calculateCurvatureGrid (particlelevelset.cpp:196-292): the
+// liquid SDF reinitialized by LevelSetSolver::reinitializeUpwind
+// (levelsetsolver.cpp:82-205, 317-390) over the cells near the surface, the
+// mean curvature of that field (particlelevelset.cpp:692-823), extrapolated
+// 3 layers.  The solver's second buffer is allocated uninitialized in the
+// reference (array3d.h:706): after the first swap its cells outside the
+// solver region are read as neighbours.  Here that buffer starts at 0, which
+// is what a fresh page-mapped allocation holds.
+void curvatureGrid(int ni, int nj, int nk, double dx, const float *phi, std::vector<float> &surface,
+                   std::vector<float> &kgrid) {
+    const size_t N = (size_t)ni * nj * nk;
+    const int band = 3, layers = 3;
+    const float outOfRange = 5.0f;
+    std::vector<unsigned char> valid(N, 0);
+    const float maxSurfaceDist = 2.0f * dx;
+    for (size_t c = 0; c < N; c++)
+        if (std::abs(phi[c]) < maxSurfaceDist) valid[c] = 1;
+    const int bw = 2 * band;
+    const int bi = (ni + bw - 1) / bw, bj = (nj + bw - 1) / bw, bk = (nk + bw - 1) / bw;
+    std::vector<unsigned char> blocks((size_t)bi * bj * bk, 0);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                if (valid[flat(i, j, k, ni, nj)]) blocks[flat(i / bw, j / bw, k / bw, bi, bj)] = 1;
+    {  // GridUtils::featherGrid6 (gridutils.cpp:229-262)
+        std::vector<unsigned char> src = blocks;
+        const int nb[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+        for (int k = 0; k < bk; k++)
+            for (int j = 0; j < bj; j++)
+                for (int i = 0; i < bi; i++) {
+                    if (!src[flat(i, j, k, bi, bj)]) continue;
+                    for (auto &o : nb) {
+                        int a = i + o[0], b = j + o[1], c = k + o[2];
+                        if (inRange(a, b, c, bi, bj, bk)) blocks[flat(a, b, c, bi, bj)] = 1;
+                    }
+                }
+    }
+    size_t numValid = 0;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                if (blocks[flat(i / bw, j / bw, k / bw, bi, bj)]) {
+                    valid[flat(i, j, k, ni, nj)] = 1;
+                    numValid++;
+                }
+    // the solver's cell list is sized numValid and then appended to, so it
+    // starts with numValid copies of cell (0, 0, 0)
+    std::vector<size_t> cells(numValid, 0);
+    for (size_t c = 0; c < N; c++)
+        if (valid[c]) cells.push_back(c);
+    // reinitializeUpwind with max CFL 0.5 and the 0.01 dx stopping threshold
+    const float fdx = (float)dx;
+    const float width = band * dx;
+    auto sign = [&](const std::vector<float> &g, size_t c) {
+        double d = g[c];
+        return (float)(d / std::sqrt(d * d + fdx * fdx));
+    };
+    std::vector<float> in(phi, phi + N);
+    float maxS = -std::numeric_limits<float>::max();
+    for (size_t c = 0; c < N; c++) maxS = std::max(sign(in, c), maxS);
+    const float maxCFL = 0.5f;
+    float dtau = maxCFL * fdx;
+    while (dtau * maxS / fdx > maxCFL) dtau *= 0.5;
+    const int iterations = (int)std::ceil(width / dtau);
+    std::vector<float> bufA = in, bufB(N, 0.0f);
+    std::vector<float> *out = &bufA, *tmp = &bufB;
+    float lastMaxDiff = -1.0f;
+    const float invdx = 1.0f / fdx;
+    auto sq = [](float x) { return x * x; };
+    for (int n = 0; n < iterations; n++) {
+        const std::vector<float> &o = *out;
+        for (size_t c : cells) {
+            const int i = (int)(c % ni), j = (int)((c / ni) % nj), k = (int)(c / ((size_t)ni * nj));
+            const float s = sign(o, c);
+            const int im = i < 1 ? 0 : i - 1, ip = std::min(i + 1, ni - 1);
+            const int jm = j < 1 ? 0 : j - 1, jp = std::min(j + 1, nj - 1);
+            const int km = k < 1 ? 0 : k - 1, kp = std::min(k + 1, nk - 1);
+            const float C0 = o[c];
+            const float dx0 = invdx * (C0 - o[flat(im, j, k, ni, nj)]), dx1 = invdx * (o[flat(ip, j, k, ni, nj)] - C0);
+            const float dy0 = invdx * (C0 - o[flat(i, jm, k, ni, nj)]), dy1 = invdx * (o[flat(i, jp, k, ni, nj)] - C0);
+            const float dz0 = invdx * (C0 - o[flat(i, j, km, ni, nj)]), dz1 = invdx * (o[flat(i, j, kp, ni, nj)] - C0);
+            const float val =
+                C0 -
+                dtau * std::max(s, 0.0f) *
+                    (std::sqrt(sq(std::max(dx0, 0.0f)) + sq(std::min(dx1, 0.0f)) + sq(std::max(dy0, 0.0f)) +
+                               sq(std::min(dy1, 0.0f)) + sq(std::max(dz0, 0.0f)) + sq(std::min(dz1, 0.0f))) -
+                     1.0f) -
+                dtau * std::min(s, 0.0f) *
+                    (std::sqrt(sq(std::min(dx0, 0.0f)) + sq(std::max(dx1, 0.0f)) + sq(std::min(dy0, 0.0f)) +
+                               sq(std::max(dy1, 0.0f)) + sq(std::min(dz0, 0.0f)) + sq(std::max(dz1, 0.0f))) -
+                     1.0f);
+            (*tmp)[c] = val;
+        }
+        float maxDiff = 0;
+        for (size_t c : cells) maxDiff = std::max(std::abs((*tmp)[c] - (*out)[c]), maxDiff);
+        std::swap(tmp, out);
+        if (std::abs(maxDiff - lastMaxDiff) < 0.01f * fdx) break;
+        lastMaxDiff = maxDiff;
+    }
+    surface = *out;
+    const float outDist = outOfRange * dx;
+    for (size_t c = 0; c < N; c++)
+        if (!valid[c]) surface[c] = outDist;
+    // _getValidCurvatureNodes: within (band - 1) dx, with all six neighbours
+    const float upper = (band - 1) * dx;
+    std::vector<unsigned char> near(N, 0), cv(N, 0);
+    for (size_t c = 0; c < N; c++)
+        if (std::abs(surface[c]) < upper) near[c] = 1;
+    for (int k = 1; k < nk - 1; k++)
+        for (int j = 1; j < nj - 1; j++)
+            for (int i = 1; i < ni - 1; i++) {
+                auto T = [&](int a, int b, int e) { return near[flat(a, b, e, ni, nj)] != 0; };
+                if (T(i, j, k) && T(i + 1, j, k) && T(i - 1, j, k) && T(i, j + 1, k) && T(i, j - 1, k) &&
+                    T(i, j, k + 1) && T(i, j, k - 1))
+                    cv[flat(i, j, k, ni, nj)] = 1;
+            }
+    kgrid.assign(N, 0.0f);
+    const float kmax = 1.0f / (float)dx;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                if (!cv[flat(i, j, k, ni, nj)]) continue;
+                if (i == 0 || j == 0 || k == 0 || i == ni - 1 || j == nj - 1 || k == nk - 1) continue;
+                auto P = [&](int a, int b, int e) { return surface[flat(a, b, e, ni, nj)]; };
+                const float x = 0.5f * (P(i + 1, j, k) - P(i - 1, j, k));
+                const float y = 0.5f * (P(i, j + 1, k) - P(i, j - 1, k));
+                const float z = 0.5f * (P(i, j, k + 1) - P(i, j, k - 1));
+                const float xx = P(i + 1, j, k) - 2.0f * P(i, j, k) + P(i - 1, j, k);
+                const float yy = P(i, j + 1, k) - 2.0f * P(i, j, k) + P(i, j - 1, k);
+                const float zz = P(i, j, k + 1) - 2.0f * P(i, j, k) + P(i, j, k - 1);
+                const float xy = 0.25f * (P(i + 1, j + 1, k) - P(i - 1, j + 1, k) - P(i + 1, j - 1, k) + P(i - 1, j - 1, k));
+                const float xz = 0.25f * (P(i + 1, j, k + 1) - P(i - 1, j, k + 1) - P(i + 1, j, k - 1) + P(i - 1, j, k - 1));
+                const float yz = 0.25f * (P(i, j + 1, k + 1) - P(i, j - 1, k + 1) - P(i, j + 1, k - 1) + P(i, j - 1, k - 1));
+                float den = x * x + y * y + z * z;
+                den = (float)std::sqrt((double)(den * den * den));
+                if (den < 1e-9f) continue;
+                float kv = ((xx * (y * y + z * z) + yy * (x * x + z * z) + zz * (x * x + y * y) - 2 * xy * x * y -
+                             2 * xz * x * z - 2 * yz * y * z) /
+                            den) /
+                           dx;
+                kv = std::min(kv, kmax);
+                kv = std::max(kv, -kmax);
+                kgrid[flat(i, j, k, ni, nj)] = kv;
+            }
+    oracle_extrapolate(ni, nj, nk, kgrid.data(), cv.data(), layers);
+}
+
+// PressureSolver::_initializeSurfaceTensionClusterData (pressuresolver.cpp:
+// 314-606) with one thread: 4^3 blocks wholly inside or outside the liquid
+// mark their cells OK; the domain's border cells are BORDER; from each
+// remaining cell beside the surface (flat order) a breadth-first search over
+// same-phase cells takes the status of the first marked cell it meets, or OK
+// once 36 cells are queued, or BAD when it ends below 36.
+enum : char { ST_UNSET = 0, ST_OK_IN = 1, ST_OK_OUT = 2, ST_BAD_IN = 4, ST_BAD_OUT = 8, ST_BORDER = 16 };
+void tensionClusterStatus(int ni, int nj, int nk, const float *phi, std::vector<char> &st) {
+    const int bw = 4, threshold = 36;
+    const int bi = ni / bw, bj = nj / bw, bk = nk / bw;
+    const size_t N = (size_t)ni * nj * nk;
+    std::vector<char> block((size_t)bi * bj * bk, 0);  // bit 0 inside, bit 1 outside
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                const int a = i / bw, b = j / bw, c = k / bw;
+                if (!inRange(a, b, c, bi, bj, bk)) continue;
+                block[flat(a, b, c, bi, bj)] |= phi[flat(i, j, k, ni, nj)] < 0 ? 1 : 2;
+            }
+    st.assign(N, ST_UNSET);
+    auto border = [&](int i, int j, int k) {
+        return i == 0 || j == 0 || k == 0 || i == ni - 1 || j == nj - 1 || k == nk - 1;
+    };
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                const size_t c = flat(i, j, k, ni, nj);
+                if (border(i, j, k)) {
+                    st[c] = ST_BORDER;
+                    continue;
+                }
+                const int a = i / bw, b = j / bw, e = k / bw;
+                if (!inRange(a, b, e, bi, bj, bk)) continue;
+                const char bs = block[flat(a, b, e, bi, bj)];
+                if ((bs & 1) && !(bs & 2))
+                    st[c] = ST_OK_IN;
+                else if (!(bs & 1) && (bs & 2))
+                    st[c] = ST_OK_OUT;
+            }
+    const int nb[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    std::vector<size_t> seeds;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                const size_t c = flat(i, j, k, ni, nj);
+                if (border(i, j, k) || st[c] != ST_UNSET) continue;
+                const bool fl = phi[c] < 0.0f;
+                for (auto &o : nb) {
+                    const float nd = phi[flat(i + o[0], j + o[1], k + o[2], ni, nj)];
+                    if (fl ? nd >= 0.0f : nd < 0.0f) {
+                        seeds.push_back(c);
+                        break;
+                    }
+                }
+            }
+    std::vector<unsigned char> processed(N, 0);
+    std::vector<size_t> queue;
+    for (size_t seed : seeds) {
+        if (st[seed] != ST_UNSET) continue;
+        queue.clear();
+        queue.push_back(seed);
+        size_t head = 0;
+        processed[seed] = 1;
+        const bool fl = phi[seed] < 0.0f;
+        char result = ST_UNSET;
+        bool stop = false;
+        while (head < queue.size()) {
+            const size_t g = queue[head++];
+            const int i = (int)(g % ni), j = (int)((g / ni) % nj), k = (int)(g / ((size_t)ni * nj));
+            for (auto &o : nb) {
+                const size_t n = flat(i + o[0], j + o[1], k + o[2], ni, nj);
+                if (processed[n]) continue;
+                if (!(fl ? phi[n] < 0.0f : phi[n] >= 0.0f)) continue;
+                const char ns = st[n];
+                if (ns == ST_BORDER) continue;
+                if (ns != ST_UNSET) {
+                    result = ns;
+                    stop = true;
+                    break;
+                }
+                queue.push_back(n);
+                processed[n] = 1;
+                if ((int)queue.size() == threshold) result = fl ? ST_OK_IN : ST_OK_OUT;
+            }
+            if (stop) break;
+        }
+        if (result == ST_UNSET && (int)queue.size() < threshold) result = fl ? ST_BAD_IN : ST_BAD_OUT;
+        for (size_t q : queue) {
+            st[q] = result;
+            processed[q] = 0;
+        }
+    }
+}
+
+// PressureSolver::_getSurfaceTensionTerm (pressuresolver.cpp:715-749): the
+// constant times the curvature at the liquid surface's crossing between two
+// cells, or 0 when the pair does not cross it or touches a BAD cluster
+struct Tension {
+    int ni, nj, nk;
+    double dx, k;
+    const float *phi;
+    std::vector<char> status;
+    std::vector<float> surface, curvature;
+    double term(int i1, int j1, int k1, int i2, int j2, int k2) const {
+        const size_t c1 = flat(i1, j1, k1, ni, nj), c2 = flat(i2, j2, k2, ni, nj);
+        const char s = status[c1] | status[c2];
+        if ((s & ST_BAD_IN) || (s & ST_BAD_OUT)) return 0.0;
+        const V3 p1 = indexToCellCenter(i1, j1, k1, dx), p2 = indexToCellCenter(i2, j2, k2, dx);
+        const float phi1 = phi[c1], phi2 = phi[c2];
+        if ((phi1 < 0.0f && phi2 < 0.0f) || (phi1 >= 0.0f && phi2 >= 0.0f)) return 0.0;
+        V3 p;
+        const float eps = 1e-6;
+        if (std::abs(phi2 - phi1) < eps)
+            p = p1 + 0.5f * (p2 - p1);
+        else
+            p = p1 + (phi1 / (phi1 - phi2)) * (p2 - p1);
+        const V3 hdx = mk(0.5 * dx, 0.5 * dx, 0.5 * dx);
+        // Interpolation::trilinearInterpolate returns the double sum (interpolation.cpp:72-112)
+        const double kv = nodalTrilinearD(curvature.data(), ni, nj, nk, dx, p - hdx);
+        return k * kv;
+    }
+};
+void buildTension(int ni, int nj, int nk, double dx, double k, const float *phi, Tension &T) {
+    T.ni = ni;
+    T.nj = nj;
+    T.nk = nk;
+    T.dx = dx;
+    T.k = k;
+    T.phi = phi;
+    tensionClusterStatus(ni, nj, nk, phi, T.status);
+    curvatureGrid(ni, nj, nk, dx, phi, T.surface, T.curvature);
+}
+
 struct PressureSystem {
     int m = 0;
     std::vector<int> rows;                 // flat cell id per row, k,j,i order
@@ -1820,10 +2105,12 @@ struct PressureSystem {
 // the solid's volume fraction) for the divergence's solid terms
 // (pressuresolver.cpp:652-657); without them the solid is at rest and those
 // terms add +-0.0, which never changes the sum.
+// tension (optional): the surface tension terms of the divergence
+// (pressuresolver.cpp:659-710)
 void buildSystem(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
                  const float *w, const float *wu, const float *wv, const float *ww,
                  const float *phi, PressureSystem &S, const float *us = nullptr, const float *vs = nullptr,
-                 const float *ws = nullptr, const float *wc = nullptr) {
+                 const float *ws = nullptr, const float *wc = nullptr, const Tension *tension = nullptr) {
     size_t N = (size_t)ni * nj * nk;
     S.key.assign(N, -1);
     S.rows.clear();
@@ -1875,6 +2162,24 @@ void buildSystem(int ni, int nj, int nk, double dx, double dt, const float *u, c
             div += -factor * (volBottom - volCenter) * vs[flat(i, j, k, ni, nj + 1)];
             div += factor * (volFront - volCenter) * ws[flat(i, j, k + 1, ni, nj)];
             div += -factor * (volBack - volCenter) * ws[flat(i, j, k, ni, nj)];
+        }
+        if (tension) {
+            const double stfactor = dt / (dx * dx);
+            const double pc = PH(i, j, k);
+            auto add = [&](double pn, double vol, int a, int b, int e) {
+                if (pn >= 0.0) {
+                    const double t = tension->term(i, j, k, a, b, e);
+                    double theta = (pc - pn) / (pc + eps);
+                    theta = std::max(-maxtheta, std::min(theta, maxtheta));
+                    div += stfactor * vol * theta * t;
+                }
+            };
+            add(PH(i + 1, j, k), volRight, i + 1, j, k);
+            add(PH(i - 1, j, k), volLeft, i - 1, j, k);
+            add(PH(i, j + 1, k), volTop, i, j + 1, k);
+            add(PH(i, j - 1, k), volBottom, i, j - 1, k);
+            add(PH(i, j, k + 1), volFront, i, j, k + 1);
+            add(PH(i, j, k - 1), volBack, i, j, k - 1);
         }
         S.rhs[r] = div;
 
@@ -2289,10 +2594,10 @@ void cellWeights(int ni, int nj, int nk, const float *solid, std::vector<float> 
 int pressureSolveImpl(int ni, int nj, int nk, double dx, double dt, const float *u, const float *v,
                       const float *w, const float *wu, const float *wv, const float *ww, const float *phi,
                       double tol, int maxIter, int nthreads, float *pressure, int *status, double *err, float *us,
-                      float *vs, float *ws, const float *wc) {
+                      float *vs, float *ws, const float *wc, const Tension *tension = nullptr) {
     PressureSystem S;
     if (us) conditionSolidVelocity(ni, nj, nk, phi, wu, wv, ww, us, vs, ws);
-    buildSystem(ni, nj, nk, dx, dt, u, v, w, wu, wv, ww, phi, S, us, vs, ws, wc);
+    buildSystem(ni, nj, nk, dx, dt, u, v, w, wu, wv, ww, phi, S, us, vs, ws, wc, tension);
     size_t N = (size_t)ni * nj * nk;
     for (size_t c = 0; c < N; c++) pressure[c] = 0.0f;
     double maxAbs = 0.0;
@@ -2326,6 +2631,18 @@ int oracle_pressure_solve(int ni, int nj, int nk, double dx, double dt, const fl
                              status, err, nullptr, nullptr, nullptr, nullptr);
 }
 
+void oracle_curvature_grid(int ni, int nj, int nk, double dx, const float *liquid_phi, float *surface_phi,
+                           float *curvature) {
+    std::vector<float> surf, kg;
+    curvatureGrid(ni, nj, nk, dx, liquid_phi, surf, kg);
+    std::copy(surf.begin(), surf.end(), surface_phi);
+    std::copy(kg.begin(), kg.end(), curvature);
+}
+void oracle_tension_cluster_status(int ni, int nj, int nk, const float *liquid_phi, char *status) {
+    std::vector<char> st;
+    tensionClusterStatus(ni, nj, nk, liquid_phi, st);
+    std::copy(st.begin(), st.end(), status);
+}
 void oracle_cell_weights(int ni, int nj, int nk, const float *solid_phi_nodes, float *wc) {
     std::vector<float> out;
     cellWeights(ni, nj, nk, solid_phi_nodes, out);
@@ -2337,15 +2654,16 @@ void oracle_condition_solid_velocity(int ni, int nj, int nk, const float *phi, c
     conditionSolidVelocity(ni, nj, nk, phi, wu, wv, ww, us, vs, ws);
 }
 
+namespace {
 /* PressureSolver::applySolutionToVelocityField (pressuresolver.cpp:94-112) and
- * _applyPressureToVelocityFieldThread (1047-1205), surface tension off, density 1. */
-void oracle_apply_pressure(int ni, int nj, int nk, double dx, double dt, const float *p,
-                           const float *phi, const float *wu, const float *wv, const float *ww,
-                           float *u, float *v, float *w, unsigned char *vu, unsigned char *vv,
-                           unsigned char *vw) {
+ * _applyPressureToVelocityFieldThread (1047-1205), density 1; with the surface
+ * tension term of the face's cell pair when tension is given. */
+void applyPressure(int ni, int nj, int nk, double dx, double dt, const float *p, const float *phi,
+                   const float *wu, const float *wv, const float *ww, float *u, float *v, float *w,
+                   unsigned char *vu, unsigned char *vv, unsigned char *vw, const Tension *T) {
     float factor = dt / dx;
     float eps = 1e-6;
-    const double maxtheta = 25, tension = 0.0;
+    const double maxtheta = 25;
     auto fluid = [&](int i, int j, int k) { return phi[flat(i, j, k, ni, nj)] < 0.0; };
     auto P = [&](int i, int j, int k) { return p[flat(i, j, k, ni, nj)]; };
     auto PH = [&](int i, int j, int k) { return phi[flat(i, j, k, ni, nj)]; };
@@ -2361,6 +2679,7 @@ void oracle_apply_pressure(int ni, int nj, int nk, double dx, double dt, const f
                 p2 = P(bi, bj, bk);
             } else {
                 float phi1 = PH(ai, aj, ak), phi2 = PH(bi, bj, bk);
+                const double tension = T ? T->term(ai, aj, ak, bi, bj, bk) : 0.0;
                 if (fluid(ai, aj, ak)) {
                     float thetaP = clampT(phi2 / (phi1 + eps));
                     float thetaT = clampT((phi1 - phi2) / (phi1 + eps));
@@ -2413,6 +2732,32 @@ void oracle_apply_pressure(int ni, int nj, int nk, double dx, double dt, const f
                 face(ww[f], lo || hi, i, j, k - 1, i, j, k, w[f], vw[f]);
             }
 }
+}  // namespace
+
+void oracle_apply_pressure(int ni, int nj, int nk, double dx, double dt, const float *p,
+                           const float *phi, const float *wu, const float *wv, const float *ww,
+                           float *u, float *v, float *w, unsigned char *vu, unsigned char *vv,
+                           unsigned char *vw) {
+    applyPressure(ni, nj, nk, dx, dt, p, phi, wu, wv, ww, u, v, w, vu, vv, vw, nullptr);
+}
+
+int oracle_pressure_solve_tension(int ni, int nj, int nk, double dx, double dt, float *u, float *v, float *w,
+                                  const float *wu, const float *wv, const float *ww, const float *liquid_phi,
+                                  double k, double tol, int max_iter, float *pressure, int *status, double *err,
+                                  unsigned char *vu, unsigned char *vv, unsigned char *vw) {
+    Tension T;
+    const Tension *tp = nullptr;
+    if (k > 1e-6) {
+        buildTension(ni, nj, nk, dx, k, liquid_phi, T);
+        tp = &T;
+    }
+    int m = pressureSolveImpl(ni, nj, nk, dx, dt, u, v, w, wu, wv, ww, liquid_phi, tol, max_iter, 1, pressure, status,
+                              err, nullptr, nullptr, nullptr, nullptr, tp);
+    if (status[1])
+        applyPressure(ni, nj, nk, dx, dt, pressure, liquid_phi, wu, wv, ww, u, v, w, vu, vv, vw, tp);
+    return m;
+}
+
 
 namespace {
 
@@ -2502,6 +2847,9 @@ namespace {
 // on the nodal solid SDF, returned through MeshLevelSet::trilinearInterpolate as float
 // (meshlevelset.cpp:261-263).
 float nodalTrilinear(const float *g, int pi, int pj, int pk, double dx, V3 p) {
+    return (float)nodalTrilinearD(g, pi, pj, pk, dx, p);
+}
+double nodalTrilinearD(const float *g, int pi, int pj, int pk, double dx, V3 p) {
     GI c = posToIndex(p, dx);
     V3 gp = indexToPos(c.i, c.j, c.k, dx);
     double inv_dx = 1.0 / dx;
@@ -2513,7 +2861,7 @@ float nodalTrilinear(const float *g, int pi, int pj, int pk, double dx, V3 p) {
         int a = c.i + off[q][0], b = c.j + off[q][1], e = c.k + off[q][2];
         if (inRange(a, b, e, pi, pj, pk)) pts[q] = g[flat(a, b, e, pi, pj)];
     }
-    return (float)trilin8(pts, ix, iy, iz);
+    return trilin8(pts, ix, iy, iz);
 }
 
 // Interpolation::bilinearInterpolate + trilinearInterpolateGradient (interpolation.cpp:189-260)
@@ -3219,6 +3567,10 @@ struct OracleSim {
     // until set; the solver's tolerance and iteration limit (fluidsimulation.h:2364-2365)
     bool viscEnabled = false;
     double viscosity = 0.0;
+    // surface tension (setSurfaceTension, fluidsimulation.cpp:2608-2621;
+    // fluidsimulation.h:2191, 2236)
+    bool tensionEnabled = false, smoothTension = false;
+    double tensionK = 0.0, tensionCondition = 10.0;
     double viscTol = 1e-4;
     int viscMaxIters = 900;
     std::vector<V3> bodyForces;
@@ -3304,7 +3656,12 @@ struct OracleSim {
         initialized = true;
     }
 
-    double liquidRadius() const { return 0.5 * 1.0 * dx * std::sqrt(3.0); }  // fluidsimulation.cpp:4351
+    // fluidsimulation.cpp:4351; scaled by 2 for the smooth surface tension
+    // kernel (5593-5597, 5635-5638, fluidsimulation.h:2236-2239)
+    double liquidRadius() const {
+        const double r = 0.5 * 1.0 * dx * std::sqrt(3.0);
+        return tensionEnabled && smoothTension ? 2.0 * r : r;
+    }
 
     // _updateNearSolidGrid (fluidsimulation.cpp:5436-5480)
     void updateNearSolid() {
@@ -3340,6 +3697,10 @@ struct OracleSim {
         if (!pos.empty()) maxu = std::max(0.0, maxu);
         double eps = 1e-6;
         double timeStep = cfl * dx / (maxu + eps);
+        if (tensionEnabled && !pos.empty()) {  // fluidsimulation.cpp:10248-10251
+            double restriction = std::sqrt(dx * dx * dx) * std::sqrt(1.0 / (tensionK + eps));
+            timeStep = std::min(timeStep, tensionCondition * restriction);
+        }
         int nsub = std::max((int)std::ceil(dt / timeStep), 1);
         return dt / nsub;
     }
@@ -3469,15 +3830,19 @@ struct OracleSim {
         int status[2];
         double err = 0;
         const bool moving = !movers.empty() && !us.empty();
+        // the curvature of the liquid surface and the clusters of the surface
+        // tension terms (fluidsimulation.cpp:5704-5730, pressuresolver.cpp:314-606)
+        Tension T;
+        if (tensionEnabled) buildTension(ni, nj, nk, dx, tensionK, phi.data(), T);
+        const Tension *tp = tensionEnabled ? &T : nullptr;
         pressureSolveImpl(ni, nj, nk, dx, dt, u.data(), v.data(), w.data(), wu.data(), wv.data(), ww.data(),
                           phi.data(), ptol, maxPIters, chunks, pres.data(), status, &err,
                           moving ? us.data() : nullptr, moving ? vs.data() : nullptr, moving ? ws.data() : nullptr,
-                          moving ? wc.data() : nullptr);
+                          moving ? wc.data() : nullptr, tp);
         bool success = status[1] != 0;
         if (success)
-            oracle_apply_pressure(ni, nj, nk, dx, dt, pres.data(), phi.data(), wu.data(),
-                                  wv.data(), ww.data(), u.data(), v.data(), w.data(), vu.data(),
-                                  vv.data(), vw.data());
+            applyPressure(ni, nj, nk, dx, dt, pres.data(), phi.data(), wu.data(), wv.data(), ww.data(), u.data(),
+                          v.data(), w.data(), vu.data(), vv.data(), vw.data(), tp);
         if (stepNumber == 0) {
             pSuccess = success;
             pIters = status[0];
@@ -3744,6 +4109,12 @@ void oracle_sim_set_viscosity(OracleSim *s, double visc, double tol, int max_ite
     s->viscosity = (float)visc;
     s->viscTol = tol;
     s->viscMaxIters = max_iter;
+}
+void oracle_sim_set_surface_tension(OracleSim *s, double k, double condition, int smooth) {
+    s->tensionEnabled = k > 1e-6;
+    s->tensionK = k;
+    s->tensionCondition = condition;
+    s->smoothTension = smooth != 0;
 }
 int oracle_sim_last_viscosity_stats(OracleSim *s, int *iterations, float *error) {
     *iterations = s->lastVIters;

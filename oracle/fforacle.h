@@ -102,6 +102,23 @@ void oracle_cell_weights(int ni, int nj, int nk, const float *solid_phi_nodes, f
  * pockets of liquid (phi cell-centred, face weights wu, wv, ww). */
 void oracle_condition_solid_velocity(int ni, int nj, int nk, const float *phi, const float *wu, const float *wv,
                                      const float *ww, float *us, float *vs, float *ws);
+/* Surface tension.  curvature_grid: ParticleLevelSet::calculateCurvatureGrid
+ * (particlelevelset.cpp:196-292) of the cell-centred liquid phi: the
+ * reinitialized surface phi and the extrapolated curvature (ni nj nk each).
+ * tension_cluster_status: PressureSolver's cluster status per cell
+ * (pressuresolver.cpp:314-606; 1 OK inside, 2 OK outside, 4 BAD inside,
+ * 8 BAD outside, 16 border, 0 unset).  pressure_solve_tension: the pressure
+ * solve with the surface tension constant k (no term when k <= 1e-6) and, on
+ * success, the pressure applied to u, v, w in place (pressuresolver.cpp:
+ * 52-112, 624-749, 1047-1205); returns the row count. */
+void oracle_curvature_grid(int ni, int nj, int nk, double dx, const float *liquid_phi, float *surface_phi,
+                           float *curvature);
+void oracle_tension_cluster_status(int ni, int nj, int nk, const float *liquid_phi, char *status);
+int oracle_pressure_solve_tension(int ni, int nj, int nk, double dx, double dt, float *u, float *v, float *w,
+                                  const float *wu, const float *wv, const float *ww, const float *liquid_phi,
+                                  double k, double tol, int max_iter, float *pressure, int *status, double *err,
+                                  unsigned char *vu, unsigned char *vv, unsigned char *vw);
+
 /* The viscosity solve (ViscositySolver, viscositysolver.cpp:50-79).  Grids:
  * liquid_phi cell-centred (ni nj nk), solid_phi_nodes and viscosity on
  * (ni+1)(nj+1)(nk+1) nodes, u/v/w the MAC field.
@@ -204,6 +221,9 @@ void oracle_sim_set_thread_chunks(OracleSim *s, int n);
  * first failing or else the most iterated substep. */
 void oracle_sim_set_viscosity(OracleSim *s, double visc, double tol, int max_iter);
 int oracle_sim_last_viscosity_stats(OracleSim *s, int *iterations, float *error);
+/* setSurfaceTension (enabled above 1e-6), setSurfaceTensionConditionNumber
+ * and the smooth surface tension kernel (fluidsimulation.cpp:2608-2744) */
+void oracle_sim_set_surface_tension(OracleSim *s, double k, double condition, int smooth);
 void oracle_sim_set_solid_phi(OracleSim *s, const float *phi_nodes);
 void oracle_sim_load_particles(OracleSim *s, int np, const float *pos, const float *vel);
 /* Mesh obstacles with velocities (static or animated, normal or inversed:

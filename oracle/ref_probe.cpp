@@ -401,6 +401,86 @@ void refprobe_sim_get_solid_velocity(void *s, float *u, float *v, float *w) {
     copyOut(*m.getArray3dV(), v);
     copyOut(*m.getArray3dW(), w);
 }
+/* ParticleLevelSet::calculateCurvatureGrid (particlelevelset.cpp:196-292).
+ * The level set solver's second buffer is an uninitialized allocation
+ * (levelsetsolver.cpp:97, array3d.h:706) whose cells outside the solver
+ * region are read after the first swap: what the reference computes depends
+ * on what that memory held.  Callers that compare with the oracle run the
+ * process with every allocation of a page or more on fresh zeroed pages
+ * (MALLOC_MMAP_THRESHOLD_=4096 from the start), so the buffer reads 0. */
+namespace {
+void curvatureOf(int ni, int nj, int nk, double dx, const float *phi, Array3d<float> &surface,
+                 Array3d<float> &kgrid) {
+    ParticleLevelSet pls(ni, nj, nk, dx);
+    copyIn(pls._phi, phi);
+    pls.calculateCurvatureGrid(surface, kgrid);
+}
+}  // namespace
+
+void refprobe_curvature_grid(int ni, int nj, int nk, double dx, const float *phi, float *surface_phi,
+                             float *curvature) {
+    Array3d<float> surface(ni, nj, nk, 0.0f), kgrid(ni, nj, nk, 0.0f);
+    curvatureOf(ni, nj, nk, dx, phi, surface, kgrid);
+    copyOut(surface, surface_phi);
+    copyOut(kgrid, curvature);
+}
+
+/* PressureSolver::solve + applySolutionToVelocityField with surface tension
+ * constant k and the curvature grid of the liquid phi computed as the engine
+ * does (fluidsimulation.cpp:5704-5730, 6206-6249); static solid, density 1.
+ * Writes the cluster status of the solve (pressuresolver.cpp:314-606). */
+int refprobe_pressure_solve_tension(int ni, int nj, int nk, double dx, double dt, float *u, float *v, float *w,
+                                    const float *wu, const float *wv, const float *ww, const float *phi,
+                                    double k, double tol, int maxIter, float *pressure, int *iterations,
+                                    float *error, unsigned char *vu, unsigned char *vv, unsigned char *vw,
+                                    char *cluster) {
+    MACVelocityField fluid(ni, nj, nk, dx), solid(ni, nj, nk, dx);
+    copyIn(*fluid.getArray3dU(), (const float *)u);
+    copyIn(*fluid.getArray3dV(), (const float *)v);
+    copyIn(*fluid.getArray3dW(), (const float *)w);
+    ValidVelocityComponentGrid valid(ni, nj, nk);
+    Array3d<float> liquid(ni, nj, nk, 0.0f);
+    copyIn(liquid, phi);
+    WeightGrid wg(ni, nj, nk);
+    copyIn(wg.U, wu);
+    copyIn(wg.V, wv);
+    copyIn(wg.W, ww);
+    wg.center.fill(1.0f);
+    Array3d<float> surface(ni, nj, nk, 0.0f), kgrid(ni, nj, nk, 0.0f);
+    curvatureOf(ni, nj, nk, dx, phi, surface, kgrid);
+    Array3d<float> pgrid(ni, nj, nk, 0.0f), density(ni, nj, nk, 1.0f);
+    PressureSolverParameters params;
+    params.cellwidth = dx;
+    params.deltaTime = dt;
+    params.tolerance = tol;
+    params.acceptableTolerance = 1.0;
+    params.maxIterations = maxIter;
+    params.velocityFieldFluid = &fluid;
+    params.velocityFieldSolid = &solid;
+    params.validVelocities = &valid;
+    params.liquidSDF = &liquid;
+    params.weightGrid = &wg;
+    params.pressureGrid = &pgrid;
+    params.densityGrid = &density;
+    params.isSurfaceTensionEnabled = k > 1e-6;
+    params.surfaceTensionConstant = k;
+    params.curvatureGrid = &kgrid;
+    PressureSolver ps;
+    bool ok = ps.solve(params);
+    if (ok) ps.applySolutionToVelocityField();
+    copyOut(pgrid, pressure);
+    *iterations = ps.getIterations();
+    *error = ps.getError();
+    copyOut(*fluid.getArray3dU(), u);
+    copyOut(*fluid.getArray3dV(), v);
+    copyOut(*fluid.getArray3dW(), w);
+    boolOut(valid.validU, vu);
+    boolOut(valid.validV, vv);
+    boolOut(valid.validW, vw);
+    if (cluster && params.isSurfaceTensionEnabled) copyOut(ps._surfaceTensionClusterStatus, cluster);
+    return ok ? 1 : 0;
+}
+
 /* ViscositySolver stage by stage (viscositysolver.cpp:50-79): the solver's
  * own face states, volume grids, matrix index and linear system for the
  * given fields; outputs as oracle_viscosity_system_get (NULL skips).  Returns

@@ -418,6 +418,91 @@ def ref_viscosity_solve(n3, dx, dt, liquid_phi, solid_nodes, viscosity, uvw, tol
     return (u, v, w), it.value, err.value, bool(ok)
 
 
+def curvature_grid(n3, dx, liquid_phi):
+    """(surface phi, curvature) of the oracle's calculateCurvatureGrid."""
+    ni, nj, nk = n3
+    surf = np.zeros((nk, nj, ni), np.float32)
+    kg = np.zeros((nk, nj, ni), np.float32)
+    L = oracle()
+    L.oracle_curvature_grid.argtypes = [C.c_int] * 3 + [C.c_double] + [C.c_void_p] * 3
+    L.oracle_curvature_grid(ni, nj, nk, dx, fp(_f32(liquid_phi)), fp(surf), fp(kg))
+    return surf, kg
+
+
+def ref_curvature_grid(n3, dx, liquid_phi):
+    ni, nj, nk = n3
+    surf = np.zeros((nk, nj, ni), np.float32)
+    kg = np.zeros((nk, nj, ni), np.float32)
+    P = probe()
+    P.refprobe_curvature_grid.argtypes = [C.c_int] * 3 + [C.c_double] + [C.c_void_p] * 3
+    P.refprobe_curvature_grid(ni, nj, nk, dx, fp(_f32(liquid_phi)), fp(surf), fp(kg))
+    return surf, kg
+
+
+def tension_cluster_status(n3, liquid_phi):
+    ni, nj, nk = n3
+    st = np.zeros((nk, nj, ni), np.int8)
+    L = oracle()
+    L.oracle_tension_cluster_status.argtypes = [C.c_int] * 3 + [C.c_void_p] * 2
+    L.oracle_tension_cluster_status(ni, nj, nk, fp(_f32(liquid_phi)), fp(st))
+    return st
+
+
+def pressure_solve_tension(n3, dx, dt, uvw, weights, liquid_phi, k, tol=1e-9, max_iter=900):
+    """The oracle's pressure solve with surface tension k, applied: returns
+    (pressure, (u, v, w), (vu, vv, vw), iterations, error, success)."""
+    ni, nj, nk = n3
+    u, v, w = (np.array(a, np.float32, copy=True) for a in uvw)
+    flags = [np.zeros(a.shape, np.uint8) for a in (u, v, w)]
+    p = np.zeros((nk, nj, ni), np.float32)
+    st = (C.c_int * 2)()
+    err = C.c_double()
+    L = oracle()
+    L.oracle_pressure_solve_tension.argtypes = [C.c_int] * 3 + [C.c_double] * 2 + [C.c_void_p] * 7 + \
+        [C.c_double, C.c_double, C.c_int] + [C.c_void_p] * 6
+    L.oracle_pressure_solve_tension(ni, nj, nk, dx, dt, fp(u), fp(v), fp(w), *[fp(_f32(x)) for x in weights],
+                                    fp(_f32(liquid_phi)), k, tol, max_iter, fp(p), st, C.byref(err),
+                                    *[fp(f) for f in flags])
+    return p, (u, v, w), tuple(flags), st[0], err.value, bool(st[1])
+
+
+def ref_pressure_solve_tension(n3, dx, dt, uvw, weights, liquid_phi, k, tol=1e-9, max_iter=900):
+    """The reference's PressureSolver with surface tension k (same returns,
+    plus the cluster status)."""
+    ni, nj, nk = n3
+    u, v, w = (np.array(a, np.float32, copy=True) for a in uvw)
+    flags = [np.zeros(a.shape, np.uint8) for a in (u, v, w)]
+    p = np.zeros((nk, nj, ni), np.float32)
+    cl = np.zeros((nk, nj, ni), np.int8)
+    it, err = C.c_int(), C.c_float()
+    P = probe()
+    P.refprobe_pressure_solve_tension.argtypes = [C.c_int] * 3 + [C.c_double] * 2 + [C.c_void_p] * 7 + \
+        [C.c_double, C.c_double, C.c_int] + [C.c_void_p] * 7
+    ok = P.refprobe_pressure_solve_tension(ni, nj, nk, dx, dt, fp(u), fp(v), fp(w), *[fp(_f32(x)) for x in weights],
+                                           fp(_f32(liquid_phi)), k, tol, max_iter, fp(p), C.byref(it),
+                                           C.byref(err), *[fp(f) for f in flags], fp(cl))
+    return p, (u, v, w), tuple(flags), it.value, err.value, bool(ok), cl
+
+
+def run_reference_isolated(module, func, out_path, **kwargs):
+    """Run module.func(**kwargs) in a fresh interpreter whose allocations of a
+    page or more all come from fresh zeroed pages (MALLOC_MMAP_THRESHOLD_=4096
+    from the start) and save the arrays it returns to out_path (npz).  The
+    reference's level set solver reads a buffer it never initializes
+    (levelsetsolver.cpp:97); in such a process that buffer reads 0, as the
+    oracle assumes, instead of whatever a reused heap block held."""
+    import json
+    import subprocess
+    import sys
+    env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="4096")
+    code = ("import json, sys, numpy as np; sys.path[:0] = %r; import %s as M; "
+            "r = M.%s(**json.loads(%r)); np.savez(%r, **r)"
+            % ([os.path.dirname(os.path.abspath(__file__)), os.path.join(ROOT, "blender-flip-fluids_amd")],
+               module, func, json.dumps(kwargs), out_path))
+    subprocess.run([sys.executable, "-c", code], env=env, check=True, stdout=subprocess.DEVNULL)
+    return dict(np.load(out_path))
+
+
 def pressure_solve(n3, dx, dt, uvw, weights, liquid_phi, tol=1e-9, max_iter=900, nthreads=1):
     ni, nj, nk = n3
     u, v, w = (_f32(a) for a in uvw)
@@ -522,6 +607,11 @@ class OracleSim:
 
     def pressure_iterations(self):
         return oracle().oracle_sim_last_pressure_iterations(self.h)
+
+    def set_surface_tension(self, k, condition=10.0, smooth=False):
+        L = oracle()
+        L.oracle_sim_set_surface_tension.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.oracle_sim_set_surface_tension(self.h, k, condition, 1 if smooth else 0)
 
     def set_viscosity(self, visc, tol=1e-4, max_iter=900):
         L = oracle()
@@ -741,6 +831,20 @@ class ReferenceSim:
         if max_iter is not None:
             L.FluidSimulation_set_viscosity_solver_max_iterations.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
             L.FluidSimulation_set_viscosity_solver_max_iterations(self.h, max_iter, C.byref(err))
+
+    def set_surface_tension(self, k, condition=None, smooth=False):
+        """FluidSimulation_set_surface_tension (+ condition number, smooth kernel)."""
+        L, err = self.L, C.c_int()
+        L.FluidSimulation_set_surface_tension.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+        L.FluidSimulation_set_surface_tension(self.h, k, C.byref(err))
+        assert err.value == 1
+        if condition is not None:
+            L.FluidSimulation_set_surface_tension_condition_number.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+            L.FluidSimulation_set_surface_tension_condition_number(self.h, condition, C.byref(err))
+            assert err.value == 1
+        if smooth:
+            L.FluidSimulation_enable_smooth_surface_tension_kernel(self.h, C.byref(err))
+            assert err.value == 1
 
     def viscosity_stats(self):
         P = probe()
