@@ -224,6 +224,28 @@ void FluidSimulation_set_viscosity_solver_error_tolerance(FluidSimulation *o, do
     guarded(err, [&] { S(o)->setViscositySolverErrorTolerance(tol); });
 }
 
+double FluidSimulation_get_surface_tension(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getSurfaceTension(); });
+}
+void FluidSimulation_set_surface_tension(FluidSimulation *o, double k, int *err) {
+    guarded(err, [&] { S(o)->setSurfaceTension(k); });
+}
+double FluidSimulation_get_surface_tension_condition_number(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->getSurfaceTensionConditionNumber(); });
+}
+void FluidSimulation_set_surface_tension_condition_number(FluidSimulation *o, double n, int *err) {
+    guarded(err, [&] { S(o)->setSurfaceTensionConditionNumber(n); });
+}
+void FluidSimulation_enable_smooth_surface_tension_kernel(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->setSmoothSurfaceTensionKernel(true); });
+}
+void FluidSimulation_disable_smooth_surface_tension_kernel(FluidSimulation *o, int *err) {
+    guarded(err, [&] { S(o)->setSmoothSurfaceTensionKernel(false); });
+}
+int FluidSimulation_is_smooth_surface_tension_kernel_enabled(FluidSimulation *o, int *err) {
+    return guarded(err, [&] { return S(o)->isSmoothSurfaceTensionKernelEnabled() ? 1 : 0; });
+}
+
 void FluidSimulation_load_marker_particle_data(FluidSimulation *o, FluidSimulationMarkerParticleData d,
                                                int *err) {
     guarded(err, [&] { S(o)->loadMarkerParticleData(d.size, d.positions, d.velocities); });
@@ -977,6 +999,89 @@ int FFAMD_pressure_solve(int ni, int nj, int nk, double dx, double dt, float *u,
         ff::hipCheck(hipMemsetAsync(bv, 0, nv, sc.s), "memset");
         ff::hipCheck(hipMemsetAsync(bw, 0, nw, sc.s), "memset");
         if (r.success) ff::applyPressure(g, dt, dp, dphi, a, b, c, du, dv, dw, bu, bv, bw, sc.s);
+        down(pressure, dp.p, N, sc.s);
+        down(u, du.p, nu, sc.s); down(v, dv.p, nv, sc.s); down(w, dw.p, nw, sc.s);
+        down(vu, bu.p, nu, sc.s); down(vv, bv.p, nv, sc.s); down(vw, bw.p, nw, sc.s);
+        sync(sc.s);
+        *iterations = r.iterations;
+        *error = r.error;
+        return r.success ? 1 : 0;
+    });
+}
+
+namespace {
+struct TensionScratch {
+    DBuf<char> block;
+    ff::TensionBuffers B;
+    TensionScratch(ff::Grid g) : block(ff::tensionBytes(g)) { ff::tensionCarve(g, block.p, B); }
+};
+}  // namespace
+
+void FFAMD_curvature_grid(int ni, int nj, int nk, double dx, const float *phi, float *surface_phi, float *curvature,
+                          int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        const size_t N = (size_t)ni * nj * nk;
+        DBuf<float> dphi(N);
+        up(dphi.p, phi, N, sc.s);
+        TensionScratch T(g);
+        ff::curvatureGrid(g, dphi, T.B, sc.s);
+        down(surface_phi, T.B.surface, N, sc.s);
+        down(curvature, T.B.curvature, N, sc.s);
+        sync(sc.s);
+    });
+}
+
+void FFAMD_tension_cluster_status(int ni, int nj, int nk, const float *phi, signed char *status, int *err) {
+    guarded(err, [&] {
+        checkGrid(ni, nj, nk, 1.0);
+        ff::Grid g{ni, nj, nk, 1.0};
+        Scratch sc;
+        const size_t N = (size_t)ni * nj * nk;
+        DBuf<float> dphi(N);
+        up(dphi.p, phi, N, sc.s);
+        TensionScratch T(g);
+        ff::tensionClusterStatus(g, dphi, T.B, sc.s);
+        down((int8_t *)status, T.B.status, N, sc.s);
+        sync(sc.s);
+    });
+}
+
+int FFAMD_pressure_solve_tension(int ni, int nj, int nk, double dx, double dt, float *u, float *v, float *w,
+                                 const float *wu, const float *wv, const float *ww, const float *phi, double k,
+                                 double tol, int maxIter, int reference_threads, float *pressure, int *iterations,
+                                 double *error, unsigned char *vu, unsigned char *vv, unsigned char *vw, int *err) {
+    return guarded(err, [&] {
+        checkGrid(ni, nj, nk, dx);
+        if (reference_threads < 0) throw std::domain_error("thread count must be >= 0");
+        ff::Grid g{ni, nj, nk, dx};
+        Scratch sc;
+        size_t N = (size_t)ni * nj * nk;
+        size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+        DBuf<float> du(nu), dv(nv), dw(nw), a(nu), b(nv), c(nw), dphi(N), dp(N);
+        DBuf<uint8_t> bu(nu), bv(nv), bw(nw);
+        DBuf<int> scan(ff::scanScratchInts(N + 1));
+        up(du.p, (const float *)u, nu, sc.s); up(dv.p, (const float *)v, nv, sc.s); up(dw.p, (const float *)w, nw, sc.s);
+        up(a.p, wu, nu, sc.s); up(b.p, wv, nv, sc.s); up(c.p, ww, nw, sc.s);
+        up(dphi.p, phi, N, sc.s);
+        TensionScratch T(g);
+        ff::TensionView tv{};
+        if (k > 1e-6) {
+            ff::curvatureGrid(g, dphi, T.B, sc.s);
+            ff::tensionClusterStatus(g, dphi, T.B, sc.s);
+            tv = ff::TensionView{T.B.status, T.B.curvature, dphi.p, k};
+        }
+        ff::PressureSolver ps;
+        ps.allocate(g);
+        ps.setTension(tv);
+        ff::PressureResult r = ps.solve(dt, du, dv, dw, a, b, c, dphi, tol, maxIter, 0, 1e-4, dp, scan, sc.s, nullptr,
+                                        nullptr, reference_threads);
+        ff::hipCheck(hipMemsetAsync(bu, 0, nu, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(bv, 0, nv, sc.s), "memset");
+        ff::hipCheck(hipMemsetAsync(bw, 0, nw, sc.s), "memset");
+        if (r.success) ff::applyPressure(g, dt, dp, dphi, a, b, c, du, dv, dw, bu, bv, bw, sc.s, tv);
         down(pressure, dp.p, N, sc.s);
         down(u, du.p, nu, sc.s); down(v, dv.p, nv, sc.s); down(w, dw.p, nw, sc.s);
         down(vu, bu.p, nu, sc.s); down(vv, bv.p, nv, sc.s); down(vw, bw.p, nw, sc.s);

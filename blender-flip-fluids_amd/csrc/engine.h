@@ -168,6 +168,16 @@ public:
     double getViscositySolverErrorTolerance() const { return _viscTolerance; }
     void setViscositySolverMaxIterations(int n) { _viscMaxIterations = n; }
     int getViscositySolverMaxIterations() const { return _viscMaxIterations; }
+    // surface tension (fluidsimulation.cpp:2604-2744): enabled above 1e-6;
+    // the substep restriction's condition number; the smooth kernel, which
+    // doubles the particle radius of the liquid SDF and the transfer, is
+    // stored but a substep with it and surface tension on raises
+    void setSurfaceTension(double k);
+    double getSurfaceTension() const { return _tensionK; }
+    void setSurfaceTensionConditionNumber(double n);
+    double getSurfaceTensionConditionNumber() const { return _tensionCondition; }
+    void setSmoothSurfaceTensionKernel(bool on) { _smoothTension = on; }
+    bool isSmoothSurfaceTensionKernelEnabled() const { return _smoothTension; }
     // the last viscosity solve: iterations, |r|_inf, success, matrix size
     void getLastViscositySolveStats(int *iterations, double *error, int *success, int *unknowns) const;
     void setSolidPhi(const float *phiNodes);
@@ -306,6 +316,12 @@ private:
     void applyViscosity(double dt);
     // the reference's _constrainVelocityField on one MAC field
     void constrainField(float *u, float *v, float *w);
+    // surface tension: settings and the curvature / cluster buffers
+    bool _tensionEnabled = false, _smoothTension = false;
+    double _tensionK = 0.0, _tensionCondition = 10.0;
+    DevBuf<char> _tensionBlock;
+    TensionBuffers _tb{};
+    double liquidRadius() const;
     DevBuf<double> _partial, _scal;
     DevBuf<int> _bins, _outl;
 

@@ -87,6 +87,30 @@ void FluidSimulation::setViscosity(double v) {
     _viscEnabled = true;
     _viscosityValue = v;
 }
+void FluidSimulation::setSurfaceTension(double k) {
+    if (k < 0.0)
+        throw std::domain_error("Error: surface tension must be greater than or equal to 0.\nsurface tension: " +
+                                toString(k) + "\n");
+    const double eps = 1e-6;
+    _tensionEnabled = k > eps;
+    _tensionK = k;
+}
+void FluidSimulation::setSurfaceTensionConditionNumber(double n) {
+    if (n <= 0.0)
+        throw std::domain_error("Error: condition number must be greater than 0.0.\nnumber: " + toString(n) + "\n");
+    _tensionCondition = n;
+}
+// fluidsimulation.cpp:4351, doubled for the smooth surface tension kernel
+// (5593-5597, 5635-5638; fluidsimulation.h:2239)
+double FluidSimulation::liquidRadius() const {
+    const double r = 0.5 * 1.0 * _g.dx * std::sqrt(3.0);
+    // the liquid SDF's block halo and the transfer's 18 neighbour lists
+    // reach particles within the default radius only
+    if (_tensionEnabled && _smoothTension)
+        throw std::runtime_error(
+            "Error: the smooth surface tension kernel (twice the particle radius) is not supported on the GPU.\n");
+    return r;
+}
 void FluidSimulation::setViscositySolverErrorTolerance(double tol) {
     if (tol < 0.0)
         throw std::domain_error(
@@ -535,6 +559,10 @@ double FluidSimulation::nextTimeStep(double dt) {
     if (_P.n > 0) maxu = std::max(0.0, maxu);
     double eps = 1e-6;
     double timeStep = _cfl * _g.dx / (maxu + eps);
+    if (_tensionEnabled && _P.n > 0) {  // fluidsimulation.cpp:10248-10251
+        double restriction = std::sqrt(_g.dx * _g.dx * _g.dx) * std::sqrt(1.0 / (_tensionK + eps));
+        timeStep = std::min(timeStep, _tensionCondition * restriction);
+    }
     int nsub = std::max((int)std::ceil(dt / timeStep), 1);
     return dt / nsub;
 }
@@ -656,7 +684,7 @@ void FluidSimulation::extrapolate(int layers) {
 // FluidSimulation::_stepFluid (fluidsimulation.cpp:10078-10121), plain configuration
 void FluidSimulation::stepFluid(double dt) {
     const size_t nu = _u.n, nv = _v.n, nw = _w.n;
-    const double radius = 0.5 * 1.0 * _g.dx * std::sqrt(3.0);  // fluidsimulation.cpp:4351
+    const double radius = liquidRadius();
     const int layers = (int)std::ceil(std::sqrt(3.0) * _cfl) + 3;  // :6282-6286
     if (_P.n == 0) return;
     updateSolidLevelSet(dt);
@@ -698,10 +726,24 @@ void FluidSimulation::stepFluid(double dt) {
     applyViscosity(dt);
     hipCheck(hipEventRecord(_ev[2], _stream), "event");
     // pressure (6152-6276)
+    // the liquid surface's curvature and the tension terms' clusters
+    // (fluidsimulation.cpp:5704-5730, pressuresolver.cpp:314-606)
+    TensionView tension{};
+    if (_tensionEnabled) {
+        const size_t bytes = tensionBytes(_g);
+        if (_tensionBlock.n < bytes) {
+            _tensionBlock.alloc(bytes);
+            tensionCarve(_g, _tensionBlock.p, _tb);
+        }
+        curvatureGrid(_g, _phi, _tb, _stream);
+        tensionClusterStatus(_g, _phi, _tb, _stream);
+        tension = TensionView{_tb.status, _tb.curvature, _phi, _tensionK};
+    }
+    _ps.setTension(tension);
     bool success = pressureSolve(dt);
     if (success) {
         _timers.begin(T_APPLY_PRESSURE, _stream, 3);
-        applyPressure(_g, dt, _pressure, _phi, _wu, _wv, _ww, _u, _v, _w, _vu, _vv, _vw, _stream);
+        applyPressure(_g, dt, _pressure, _phi, _wu, _wv, _ww, _u, _v, _w, _vu, _vv, _vw, _stream, tension);
         _timers.end(T_APPLY_PRESSURE, _stream);
     }
     if (_stepNumber == 0) {

@@ -335,7 +335,12 @@ __device__ inline float volumeFraction8(float p000, float p100, float p010, floa
 // Interpolation::trilinearInterpolate (interpolation.cpp:72-112) of a float
 // grid of pi x pj x pk samples at spacing dx whose sample (0,0,0) sits at the
 // origin; samples outside the grid count as 0
+__device__ inline double gridTrilinearD(const float *a, int pi, int pj, int pk, double dx, V3 p);
 __device__ inline float gridTrilinear(const float *a, int pi, int pj, int pk, double dx, V3 p) {
+    return (float)gridTrilinearD(a, pi, pj, pk, dx, p);
+}
+// the same, keeping the double sum
+__device__ inline double gridTrilinearD(const float *a, int pi, int pj, int pk, double dx, V3 p) {
     const I3 c = posToIndex(p, dx);
     const V3 gp = indexToPos(c.i, c.j, c.k, dx);
     const double inv_dx = 1.0 / dx;
@@ -348,7 +353,35 @@ __device__ inline float gridTrilinear(const float *a, int pi, int pj, int pk, do
         const int x = c.i + o[q][0], y = c.j + o[q][1], z = c.k + o[q][2];
         pts[q] = inRange(x, y, z, pi, pj, pk) ? (double)a[flat(x, y, z, pi, pj)] : 0.0;
     }
-    return (float)trilin8(pts, ix, iy, iz);
+    return trilin8(pts, ix, iy, iz);
+}
+
+// ---- surface tension ----
+// The liquid surface's curvature grid and cluster status for the pressure
+// solve's tension terms; status null: no surface tension.
+struct TensionView {
+    const int8_t *status = nullptr;    // per cell; bits 4 and 8 mark BAD clusters
+    const float *curvature = nullptr;  // per cell
+    const float *phi = nullptr;        // the liquid SDF, cell-centred
+    double k = 0.0;                    // the surface tension constant
+};
+// PressureSolver::_getSurfaceTensionTerm (pressuresolver.cpp:715-749): k times
+// the curvature, trilinearly interpolated (as a double) at the surface's
+// crossing between the two cells' centres; 0 when both cells are on one
+// side or one of them is in a BAD cluster
+__device__ inline double tensionTerm(Grid g, const TensionView &T, int i1, int j1, int k1, int i2, int j2,
+                                     int k2) {
+    const size_t c1 = flat(i1, j1, k1, g.ni, g.nj), c2 = flat(i2, j2, k2, g.ni, g.nj);
+    const int8_t s = T.status[c1] | T.status[c2];
+    if ((s & 4) || (s & 8)) return 0.0;
+    const float phi1 = T.phi[c1], phi2 = T.phi[c2];
+    if ((phi1 < 0.0f && phi2 < 0.0f) || (phi1 >= 0.0f && phi2 >= 0.0f)) return 0.0;
+    const V3 p1 = indexToCellCenter(i1, j1, k1, g.dx), p2 = indexToCellCenter(i2, j2, k2, g.dx);
+    const float eps = 1e-6f;
+    const V3 p = fabsf(phi2 - phi1) < eps ? add(p1, scale(0.5f, sub(p2, p1)))
+                                         : add(p1, scale(phi1 / (phi1 - phi2), sub(p2, p1)));
+    const V3 hdx = mk(0.5 * g.dx, 0.5 * g.dx, 0.5 * g.dx);
+    return T.k * gridTrilinearD(T.curvature, g.ni, g.nj, g.nk, g.dx, sub(p, hdx));
 }
 
 }  // namespace ff

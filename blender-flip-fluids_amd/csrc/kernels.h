@@ -276,7 +276,33 @@ void constrainFacesFriction(Grid g, int dir, float *a, const float *w, const int
                             hipStream_t s, const float *sv = nullptr);
 void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,
                    const float *wv, const float *ww, float *u, float *v, float *w, uint8_t *vu,
-                   uint8_t *vv, uint8_t *vw, hipStream_t s);
+                   uint8_t *vv, uint8_t *vw, hipStream_t s, TensionView tension = TensionView{});
+
+// ---- surface tension (particlelevelset.cpp:196-292, pressuresolver.cpp:314-606) ----
+// Device buffers of the curvature grid and the cluster status for one grid,
+// carved from one allocation of tensionBytes(g) bytes.
+struct TensionBuffers {
+    float *a = nullptr, *b = nullptr;  // the reinitialization's two fields
+    float *curvature = nullptr;        // N
+    int8_t *status = nullptr;          // N
+    uint8_t *valid = nullptr, *solver = nullptr, *cv = nullptr, *ext = nullptr;  // N
+    int *parent = nullptr, *count = nullptr, *rootFlags = nullptr;             // N
+    uint8_t *blocks = nullptr, *blocks2 = nullptr, *blocks4 = nullptr;
+    float *partial = nullptr;        // per-block maxima
+    int *any = nullptr;              // 1 int
+    const float *surface = nullptr;  // the reinitialized field after curvatureGrid (a or b)
+};
+size_t tensionBytes(Grid g);
+void tensionCarve(Grid g, void *block, TensionBuffers &B);
+// the reinitialized surface field (B.surface) and the extrapolated curvature
+// (B.curvature) of the cell-centred liquid phi; waits for the stream once per
+// reinitialization step (the reference's convergence test)
+void curvatureGrid(Grid g, const float *phi, TensionBuffers &B, hipStream_t s);
+// B.status: 16 on the border, 1 / 2 in 4^3 blocks wholly inside / outside the
+// liquid, 4 / 8 in the BAD clusters (connected same-phase regions of the
+// remaining cells with fewer than 36 cells and no same-phase neighbour in
+// such a block), 0 elsewhere
+void tensionClusterStatus(Grid g, const float *phi, TensionBuffers &B, hipStream_t s);
 
 // ---- pressure (kernels_pressure.hip) ----
 // MIC(0) operands of one row as the band sweeps read them: invdiag and the
@@ -320,9 +346,11 @@ void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, int *cel
 struct SolidTerms {
     const float *us = nullptr, *vs = nullptr, *ws = nullptr, *wc = nullptr;
 };
+// tension: the surface tension terms of the divergence (status null: none)
 void pressureAssemble(Grid g, double dt, const float *u, const float *v, const float *w,
                       const float *wu, const float *wv, const float *ww, const float *phi,
-                      PressureSys &S, hipStream_t s, SolidTerms solid = SolidTerms{});
+                      PressureSys &S, hipStream_t s, SolidTerms solid = SolidTerms{},
+                      TensionView tension = TensionView{});
 void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s);
 // MIC(0) factor and application z = (L L^T)^-1 r; hyperplane-by-hyperplane
 // launches driven by the host copy of the level table

@@ -157,7 +157,7 @@ __global__ void constrainFrictionKernel(Grid g, float *a, const float *w, const 
 // axis are skipped (1058, 1109, 1159) and keep their value.
 template <int DIR>
 __global__ void applyPressureKernel(Grid g, double dt, const float *p, const float *phi,
-                                    const float *wgt, float *a, uint8_t *valid) {
+                                    const float *wgt, float *a, uint8_t *valid, TensionView tv) {
     const int wi = g.ni + (DIR == 0), wj = g.nj + (DIR == 1), wk = g.nk + (DIR == 2);
     size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)wi * wj * wk;
@@ -169,7 +169,7 @@ __global__ void applyPressureKernel(Grid g, double dt, const float *p, const flo
     if (c == 0 || c == nc - 1) return;
     const float factor = (float)(dt / g.dx);
     const float eps = 1e-6f;
-    const double maxtheta = 25, tension = 0.0;
+    const double maxtheta = 25;
     int ai = i - (DIR == 0), aj = j - (DIR == 1), ak = k - (DIR == 2);  // low-side cell
     // isFaceBorderingMaterial (fluidmaterialgrid.cpp:126-146)
     // The reference reads the cell past the last face (c == nc) when that face
@@ -186,6 +186,8 @@ __global__ void applyPressureKernel(Grid g, double dt, const float *p, const flo
             p2 = p[hb];
         } else {
             float phi1 = phi[la], phi2 = phi[hb];
+            // the surface tension term of the pair (pressuresolver.cpp:1076)
+            const double tension = tv.status && c < nc ? tensionTerm(g, tv, ai, aj, ak, i, j, k) : 0.0;
             if (lo) {
                 float thetaP = (float)fmax(-maxtheta, fmin((double)(phi2 / (phi1 + eps)), maxtheta));
                 float thetaT = (float)fmax(-maxtheta, fmin((double)((phi1 - phi2) / (phi1 + eps)), maxtheta));
@@ -249,12 +251,12 @@ void constrainFacesFriction(Grid g, int dir, float *a, const float *w, const int
 
 void applyPressure(Grid g, double dt, const float *p, const float *phi, const float *wu,
                    const float *wv, const float *ww, float *u, float *v, float *w, uint8_t *vu,
-                   uint8_t *vv, uint8_t *vw, hipStream_t s) {
+                   uint8_t *vv, uint8_t *vw, hipStream_t s, TensionView tension) {
     size_t nu = (size_t)(g.ni + 1) * g.nj * g.nk, nv = (size_t)g.ni * (g.nj + 1) * g.nk,
            nw = (size_t)g.ni * g.nj * (g.nk + 1);
-    hipLaunchKernelGGL(applyPressureKernel<0>, dim3(blocksFor(nu, TPB)), dim3(TPB), 0, s, g, dt, p, phi, wu, u, vu);
-    hipLaunchKernelGGL(applyPressureKernel<1>, dim3(blocksFor(nv, TPB)), dim3(TPB), 0, s, g, dt, p, phi, wv, v, vv);
-    hipLaunchKernelGGL(applyPressureKernel<2>, dim3(blocksFor(nw, TPB)), dim3(TPB), 0, s, g, dt, p, phi, ww, w, vw);
+    hipLaunchKernelGGL(applyPressureKernel<0>, dim3(blocksFor(nu, TPB)), dim3(TPB), 0, s, g, dt, p, phi, wu, u, vu, tension);
+    hipLaunchKernelGGL(applyPressureKernel<1>, dim3(blocksFor(nv, TPB)), dim3(TPB), 0, s, g, dt, p, phi, wv, v, vv, tension);
+    hipLaunchKernelGGL(applyPressureKernel<2>, dim3(blocksFor(nw, TPB)), dim3(TPB), 0, s, g, dt, p, phi, ww, w, vw, tension);
 }
 
 }  // namespace ff

@@ -48,7 +48,8 @@ __global__ void rowsKernel(size_t n, const int *mask, const int *rowOf, int *row
 // (SparseMatrix::add ignores -1, sparsematrix.h:94-96).
 __global__ void assembleKernel(Grid g, double dt, const float *u, const float *v, const float *w,
                                const float *wu, const float *wv, const float *ww, const float *phi,
-                               const int *mask, const int *rowOf, PressureSys S, SolidTerms T) {
+                               const int *mask, const int *rowOf, PressureSys S, SolidTerms T,
+                               TensionView tv) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= S.m) return;
     const int ni = g.ni, nj = g.nj;
@@ -75,6 +76,25 @@ __global__ void assembleKernel(Grid g, double dt, const float *u, const float *v
         div += -factor * (volBottom - volCenter) * (double)T.vs[flat(i, j, k, ni, nj + 1)];
         div += factor * (volFront - volCenter) * (double)T.ws[flat(i, j, k + 1, ni, nj)];
         div += -factor * (volBack - volCenter) * (double)T.ws[flat(i, j, k, ni, nj)];
+    }
+    if (tv.status) {  // surface tension (pressuresolver.cpp:659-710)
+        const double stfactor = dt / (g.dx * g.dx);
+        const double pc = phi[c];
+        auto add = [&](int a, int b, int e, double vol) {
+            const double pn = phi[flat(a, b, e, ni, nj)];
+            if (pn >= 0.0) {
+                const double t = tensionTerm(g, tv, i, j, k, a, b, e);
+                double theta = (pc - pn) / (pc + eps);
+                theta = fmax(-maxtheta, fmin(theta, maxtheta));
+                div += stfactor * vol * theta * t;
+            }
+        };
+        add(i + 1, j, k, volRight);
+        add(i - 1, j, k, volLeft);
+        add(i, j + 1, k, volTop);
+        add(i, j - 1, k, volBottom);
+        add(i, j, k + 1, volFront);
+        add(i, j, k - 1, volBack);
     }
     S.rhs[r] = div;
 
@@ -448,10 +468,10 @@ void pressureRows(Grid g, const int *mask, const int *rowOf, int *rows, int *cel
 
 void pressureAssemble(Grid g, double dt, const float *u, const float *v, const float *w,
                       const float *wu, const float *wv, const float *ww, const float *phi,
-                      PressureSys &S, hipStream_t s, SolidTerms solid) {
+                      PressureSys &S, hipStream_t s, SolidTerms solid, TensionView tension) {
     if (S.m == 0) return;
     hipLaunchKernelGGL(assembleKernel, dim3(blocksFor(S.m, TPB)), dim3(TPB), 0, s, g, dt, u, v, w, wu,
-                       wv, ww, phi, S.mask, S.rowOf, S, solid);
+                       wv, ww, phi, S.mask, S.rowOf, S, solid, tension);
 }
 
 void pressureLevels(Grid g, PressureSys &S, int *scanScratch, hipStream_t s) {

@@ -20,6 +20,7 @@
 #include <stdexcept>
 
 #include "kernels.h"
+#include "unionfind.h"
 
 namespace ff {
 
@@ -76,40 +77,6 @@ struct CondView {
                (W(i, j, k + 1) >= 0.0f + eps && L(i, j, k + 1) >= 0.0f);
     }
 };
-
-__device__ int loadParent(const int *parent, int x) {
-    return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// root of x with path halving; parent[y] <= y for every member y, so the walk
-// ends, and the halving stores only ever lower an entry to an ancestor
-__device__ int findRoot(int *parent, int x) {
-    int p = loadParent(parent, x);
-    while (p != x) {
-        const int gp = loadParent(parent, p);
-        if (gp != p) __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = p;
-        p = gp;
-    }
-    return x;
-}
-
-// hook the larger root onto the smaller; a failed swap returns the larger
-// one's new (smaller) parent, so each retry starts lower and the loop ends
-__device__ void unite(int *parent, int a, int b) {
-    int ra = findRoot(parent, a), rb = findRoot(parent, b);
-    while (ra != rb) {
-        if (ra < rb) {
-            const int t = ra;
-            ra = rb;
-            rb = t;
-        }
-        const int old = atomicCAS(parent + ra, ra, rb);
-        if (old == ra) return;
-        ra = findRoot(parent, old);
-        rb = findRoot(parent, rb);
-    }
-}
 
 __global__ void condInitKernel(CondView v, int *parent, int *info) {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

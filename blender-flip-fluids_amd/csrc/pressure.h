@@ -54,6 +54,9 @@ public:
     // the moving solid's face velocities and cell weights for the divergence
     // of the following solves (all null: a static solid)
     void setSolidTerms(SolidTerms t) { _solid = t; }
+    // the surface tension terms of the following solves' divergence (status
+    // null: none)
+    void setTension(TensionView t) { _tension = t; }
     // the solver's MIC(0) application: 2 = bands of hyperplanes, 4 = the
     // line pipeline (two launches per application)
     void setSchedule(int schedule);
@@ -98,6 +101,7 @@ private:
     int _refDots = 0;  // thread count of the reference's dot order, or 0 for the tree reductions
     bool _warm = false;
     SolidTerms _solid{};
+    TensionView _tension{};
     const float *_prevPressure = nullptr;  // the pressure grid at solve() entry
     // x = 0, r = b; or, warm, x from the pressure grid and r = b - A x
     void initialGuess(hipStream_t st);

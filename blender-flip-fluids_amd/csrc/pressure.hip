@@ -284,7 +284,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     _S.m = m;
     res.unknowns = m;
     pressureRows(_g, _S.mask, _S.rowOf, _S.rows, _S.cellRow, s);
-    pressureAssemble(_g, dt, u, v, w, wu, wv, ww, phi, _S, s, _solid);
+    pressureAssemble(_g, dt, u, v, w, wu, wv, ww, phi, _S, s, _solid, _tension);
     if (cellRhs) gatherRows(_S, cellRhs, _S.rhs, s);
     pressureLevels(_g, _S, scanScratch, s);
     if (m > 0) pencilLineWords(_g, _S, _pp, s);  // SpMV and the line pipeline find neighbour rows in them

@@ -209,6 +209,38 @@ class FluidSimulation(object):
         _require_at_least(value, 0)
         _setter("FluidSimulation_set_viscosity", c_double)(self, value)
 
+    # surface tension (src/engine/ffengine/fluidsimulation.py:2293-2305,
+    # 2376-2403)
+    @property
+    def surface_tension(self):
+        return _call0(self, "FluidSimulation_get_surface_tension", c_double)
+
+    @surface_tension.setter
+    def surface_tension(self, value):
+        _require_at_least(value, 0)
+        _setter("FluidSimulation_set_surface_tension", c_double)(self, value)
+
+    @property
+    def surface_tension_condition_number(self):
+        return _call0(self, "FluidSimulation_get_surface_tension_condition_number", c_double)
+
+    @surface_tension_condition_number.setter
+    def surface_tension_condition_number(self, n):
+        if isinstance(n, numbers.Real) and n <= 0.0:
+            raise ValueError("Value must be greater than 0.0")
+        _setter("FluidSimulation_set_surface_tension_condition_number", c_double)(self, n)
+
+    @property
+    def enable_smooth_surface_tension_kernel(self):
+        return bool(_call0(self, "FluidSimulation_is_smooth_surface_tension_kernel_enabled", c_int))
+
+    @enable_smooth_surface_tension_kernel.setter
+    def enable_smooth_surface_tension_kernel(self, on):
+        name = "FluidSimulation_%s_smooth_surface_tension_kernel" % ("enable" if on else "disable")
+        f = getattr(lib, name)
+        pb.init_lib_func(f, [c_void_p, c_void_p], None)
+        pb.execute_lib_func(f, [self()])
+
     @property
     def viscosity_solver_error_tolerance(self):
         return _call0(self, "FluidSimulation_get_viscosity_solver_error_tolerance", c_double)

@@ -210,6 +210,41 @@ def viscosity_solve(n3, dx, dt, liquid_phi, solid_nodes, viscosity, uvw, tol=1e-
     return (u, v, w), it.value, err.value, bool(ok.value), m, x[:m]
 
 
+def curvature_grid(n3, dx, liquid_phi):
+    """(surface phi, curvature) of the liquid SDF on the GPU (FFAMD_curvature_grid)."""
+    ni, nj, nk = n3
+    surf = np.zeros((nk, nj, ni), np.float32)
+    kg = np.zeros((nk, nj, ni), np.float32)
+    _run("FFAMD_curvature_grid", [c_int] * 3 + [c_double] + [c_void_p] * 3,
+         [ni, nj, nk, dx, _fp(_f32(liquid_phi)), _fp(surf), _fp(kg)])
+    return surf, kg
+
+
+def tension_cluster_status(n3, liquid_phi):
+    """The surface tension cluster status per cell (FFAMD_tension_cluster_status)."""
+    ni, nj, nk = n3
+    st = np.zeros((nk, nj, ni), np.int8)
+    _run("FFAMD_tension_cluster_status", [c_int] * 3 + [c_void_p] * 2, [ni, nj, nk, _fp(_f32(liquid_phi)), _fp(st)])
+    return st
+
+
+def pressure_solve_tension(n3, dx, dt, uvw, weights, liquid_phi, k, tol=1e-9, max_iter=900, reference_threads=1):
+    """FP64 PCG pressure solve with surface tension k and the gradient applied
+    (FFAMD_pressure_solve_tension): (pressure, (u, v, w), flags, iterations,
+    error, success)."""
+    ni, nj, nk = n3
+    u, v, w = (np.array(a, np.float32, copy=True) for a in uvw)
+    flags = [np.zeros(a.shape, np.uint8) for a in (u, v, w)]
+    p = np.zeros((nk, nj, ni), np.float32)
+    it, err = c_int(), c_double()
+    ok = _run("FFAMD_pressure_solve_tension", [c_int] * 3 + [c_double] * 2 + [c_void_p] * 7 +
+              [c_double, c_double, c_int, c_int] + [c_void_p] * 6,
+              [ni, nj, nk, dx, dt, _fp(u), _fp(v), _fp(w)] + [_fp(_f32(x)) for x in weights] +
+              [_fp(_f32(liquid_phi)), k, tol, max_iter, reference_threads, _fp(p), byref(it), byref(err)] +
+              [_fp(f) for f in flags], c_int)
+    return p, (u, v, w), tuple(flags), it.value, err.value, bool(ok)
+
+
 def cell_weights(n3, phi_nodes):
     """1 - the solid's volume fraction of each cell, (k, j, i)."""
     ni, nj, nk = n3

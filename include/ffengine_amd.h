@@ -126,6 +126,18 @@ double FluidSimulation_get_PICFLIP_ratio(FluidSimulation *obj, int *err);      /
 void FluidSimulation_set_PICFLIP_ratio(FluidSimulation *obj, double ratio, int *err); /* :3007-3013 */
 double FluidSimulation_get_PICAPIC_ratio(FluidSimulation *obj, int *err);      /* :3014-3018 */
 void FluidSimulation_set_PICAPIC_ratio(FluidSimulation *obj, double ratio, int *err); /* :3020-3026 */
+/* surface tension (fluidsimulation_c.cpp:2757-2770, 2841-2876;
+ * fluidsimulation.cpp:2604-2744): enabled above 1e-6; the condition number of
+ * the substep restriction (> 0); the smooth kernel doubles the particle radius
+ * of the liquid SDF and the velocity transfer. */
+double FluidSimulation_get_surface_tension(FluidSimulation *obj, int *err);               /* :2757-2761 */
+void FluidSimulation_set_surface_tension(FluidSimulation *obj, double k, int *err);       /* :2763-2769 */
+double FluidSimulation_get_surface_tension_condition_number(FluidSimulation *obj, int *err); /* :2841-2847 */
+void FluidSimulation_set_surface_tension_condition_number(FluidSimulation *obj, double n,
+                                                          int *err);                      /* :2849-2855 */
+void FluidSimulation_enable_smooth_surface_tension_kernel(FluidSimulation *obj, int *err);  /* :2857-2862 */
+void FluidSimulation_disable_smooth_surface_tension_kernel(FluidSimulation *obj, int *err); /* :2864-2869 */
+int FluidSimulation_is_smooth_surface_tension_kernel_enabled(FluidSimulation *obj, int *err); /* :2871-2876 */
 /* constant viscosity and its solver (fluidsimulation_c.cpp:333-346, 2728-2754;
  * fluidsimulation.cpp:303-312, 2565-2601): the first set enables the
  * viscosity solve after the body forces of every substep; negative values
@@ -414,6 +426,24 @@ int FFAMD_viscosity_solve(int isize, int jsize, int ksize, double dx, double dt,
                           const float *solid_phi_nodes, const float *viscosity, double tol, int max_iter, float *u,
                           float *v, float *w, int *iterations, float *error, int *success, float *solution,
                           int *err);
+/* Surface tension stages on host arrays (liquid phi cell-centred, ni nj nk):
+ * FFAMD_curvature_grid: ParticleLevelSet::calculateCurvatureGrid
+ * (particlelevelset.cpp:196-292), the reinitialized surface phi and the
+ * extrapolated curvature.  FFAMD_tension_cluster_status: 16 border, 1 / 2
+ * cells of 4^3 blocks wholly inside / outside, 4 / 8 BAD clusters
+ * (pressuresolver.cpp:314-606), 0 elsewhere.  FFAMD_pressure_solve_tension:
+ * the FP64 pressure solve with the tension terms of constant k (none at
+ * k <= 1e-6) and, on success, the gradient applied to u, v, w in place;
+ * reference_threads > 0 takes the dot products in the reference's order for
+ * that thread count.  Returns 1 on success. */
+void FFAMD_curvature_grid(int isize, int jsize, int ksize, double dx, const float *liquid_phi, float *surface_phi,
+                          float *curvature, int *err);
+void FFAMD_tension_cluster_status(int isize, int jsize, int ksize, const float *liquid_phi, signed char *status,
+                                  int *err);
+int FFAMD_pressure_solve_tension(int isize, int jsize, int ksize, double dx, double dt, float *u, float *v, float *w,
+                                 const float *wu, const float *wv, const float *ww, const float *liquid_phi, double k,
+                                 double tol, int max_iter, int reference_threads, float *pressure, int *iterations,
+                                 double *error, unsigned char *vu, unsigned char *vv, unsigned char *vw, int *err);
 /* the cells' solid weights 1 - volume fraction of the solid
  * (fluidsimulation.cpp:6140-6147, meshlevelset.cpp:1491-1515) */
 void FFAMD_cell_weights(int isize, int jsize, int ksize, const float *phi_nodes, float *wc, int *err);
