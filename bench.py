@@ -442,20 +442,23 @@ def moving_config(n, steps, warmup, tol, mode, speed):
     print(json.dumps(line), flush=True)
 
 
-def viscous_config(n, steps, warmup, tol, mode, visc):
+def properties_config(n, steps, warmup, tol, mode, visc, tension):
     """The 256^3 dam break with its sphere obstacle (the 20480-triangle
-    icosphere MeshObject) and a constant viscosity: every substep runs the
-    viscosity solve (the reference's ViscositySolver system, solved on the
-    GPU by Jacobi-preconditioned CG to the reference's tolerance 1e-4) after
-    the body forces.  Reports the substep rate, the viscosity stage's share
-    and iterations, and the rate of the same run without viscosity."""
+    icosphere MeshObject) run three ways: plain, with a constant viscosity
+    (every substep runs the viscosity solve after the body forces: the
+    reference's ViscositySolver system, solved on the GPU by
+    Jacobi-preconditioned CG to the reference's tolerance 1e-4), and with
+    surface tension (every substep computes the curvature grid and the
+    cluster status, and the pressure solve takes the tension terms).
+    Reports each run's substep rate and the stages' shares; the line's
+    value is the viscous run's rate."""
     import torch
     from ffengine import FluidSimulation, MeshObject, TriangleMesh, scenes
     frame_dt = 1.0 / 240.0
     pos = scenes.dam_break_particles(n, seed=0)
     v, t = scenes.sphere_mesh(subdivisions=5)
     out = {}
-    for on in (False, True):
+    for kind in ("plain", "viscous", "tension"):
         sim = FluidSimulation(n, n, n, 1.0 / n)
         sim.add_body_force(0.0, -9.81, 0.0)
         sim.PICFLIP_ratio = 0.05
@@ -463,8 +466,10 @@ def viscous_config(n, steps, warmup, tol, mode, visc):
         sim.enable_surface_reconstruction = False
         sim.pressure_solver_tolerance = tol
         sim.set_pressure_solver_precision(mode)
-        if on:
+        if kind == "viscous":
             sim.viscosity = visc
+        if kind == "tension":
+            sim.surface_tension = tension
         mo = MeshObject(n, n, n, 1.0 / n)
         mo.update_mesh_static(TriangleMesh(v, t))
         sim.add_mesh_obstacle(mo)
@@ -483,28 +488,28 @@ def viscous_config(n, steps, warmup, tol, mode, visc):
                 st = sim.get_frame_stats_data()
                 substeps += st.substeps
                 iters.append(st.pressure_solver_iterations)
-                if on:
+                if kind == "viscous":
                     vs = sim.get_last_viscosity_solve_stats()
                     viters.append(vs["iterations"])
                     vrows.append(vs["unknowns"])
                     vms.append(round(sim.get_last_substep_phase_ms(8), 2))
-            log("  %s frame %d/%d" % ("viscous" if on else "inviscid", f + 1, warmup + steps))
+            log("  %s frame %d/%d" % (kind, f + 1, warmup + steps))
         rec = {"substeps_per_s": round(substeps / elapsed, 4), "ms_per_substep": round(1e3 * elapsed / substeps, 2),
                "pressure_iterations": iters, "particles": sim.get_num_marker_particles()}
-        if on:
-            ms = kernel_times(sim)["viscosity"]["ms"]
+        if kind == "viscous":
             rec.update({"viscosity_iterations": viters, "viscosity_rows": vrows, "viscosity_ms_last_substep": vms,
-                        "viscosity_ms_per_substep": round(ms / substeps, 2)})
-        out["viscous" if on else "inviscid"] = rec
+                        "viscosity_ms_per_substep": round(kernel_times(sim)["viscosity"]["ms"] / substeps, 2)})
+        out[kind] = rec
         del sim, mo
     line = {"metric": "FLIP substeps/s, dam break %d^3 with viscosity %g" % (n, visc),
             "value": out["viscous"]["substeps_per_s"], "unit": "substeps/s", "n_gpus": 1, "steps": steps,
             "warmup": warmup, "higher_is_better": True, "data": "synthetic",
             "dtype": "f32+f64" if mode != "fp64" else "f64", "mode": mode,
             "config": {"workload": "dam break %d^3, %d particles, the 20480-triangle sphere MeshObject (r 0.15 at "
-                                   "(0.6, 0.2, 0.5)), constant viscosity %g (solver tolerance 1e-4), frame 1/240 s, "
-                                   "PCG rel res %g" % (n, len(pos), visc, tol), "grid": n, "particles": len(pos)},
-            "viscous": out["viscous"], "inviscid": out["inviscid"]}
+                                   "(0.6, 0.2, 0.5)), frame 1/240 s, PCG rel res %g; viscous: constant viscosity %g "
+                                   "(solver tolerance 1e-4); tension: surface tension %g"
+                                   % (n, len(pos), tol, visc, tension), "grid": n, "particles": len(pos)},
+            "plain": out["plain"], "viscous": out["viscous"], "tension": out["tension"]}
     print(json.dumps(line), flush=True)
 
 
@@ -624,16 +629,17 @@ def adjoint_config(n, substeps, checkpoint_every, fd_faces, eps, tol, dt):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="dam-break",
-                    choices=["dam-break", "precision", "splash", "adjoint", "moving", "viscous",
+                    choices=["dam-break", "precision", "splash", "adjoint", "moving", "properties",
                              "cpu-calibration"],
                     help="dam-break: the 256^3 headline benchmark (default); precision: the 384^3 slab "
                          "pressure-accuracy comparison; splash: the 512^3 splash; adjoint: the 128^3 "
                          "differentiable rollout with its finite-difference check; moving: the 256^3 dam break "
-                         "with its sphere obstacle animated, beside the static run; viscous: the 256^3 dam "
-                         "break with a constant viscosity, beside the inviscid run; cpu-calibration (no GPU, "
+                         "with its sphere obstacle animated, beside the static run; properties: the 256^3 dam "
+                         "break plain, with a constant viscosity and with surface tension; cpu-calibration (no GPU, "
                          "needs the reference build): the reference engine against the oracle on the host")
     ap.add_argument("--speed", type=float, default=0.3, help="moving: the sphere's speed along +z (m/s)")
-    ap.add_argument("--viscosity", type=float, default=0.5, help="viscous: the constant viscosity")
+    ap.add_argument("--viscosity", type=float, default=0.5, help="properties: the constant viscosity")
+    ap.add_argument("--surface-tension", type=float, default=0.2, help="properties: the surface tension constant")
     ap.add_argument("--substeps", type=int, default=32, help="adjoint: substeps of the rollout")
     ap.add_argument("--checkpoint-every", type=int, default=8, help="adjoint: checkpoint spacing")
     ap.add_argument("--fd-faces", type=int, default=64, help="adjoint: faces checked by central differences")
@@ -692,10 +698,11 @@ def main():
         moving_config(args.n or 256, args.steps, args.warmup, args.tol,
                       modes[0] if len(modes) == 1 else "fp32-preconditioner", args.speed)
         return
-    if args.config == "viscous":
+    if args.config == "properties":
         modes = [m for m in args.modes.split(",") if m]
-        viscous_config(args.n or 256, args.steps, args.warmup, args.tol,
-                       modes[0] if len(modes) == 1 else "fp32-preconditioner", args.viscosity)
+        properties_config(args.n or 256, args.steps, args.warmup, args.tol,
+                          modes[0] if len(modes) == 1 else "fp32-preconditioner", args.viscosity,
+                          args.surface_tension)
         return
     if args.config == "splash":
         modes = [m for m in args.modes.split(",") if m]
