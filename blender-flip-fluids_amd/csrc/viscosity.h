@@ -7,8 +7,9 @@
 // finer grid) and the viscosity on the grid's nodes.  The system is
 // assembled bit for bit as the reference assembles it; the solve is a
 // Jacobi-preconditioned CG in float with deterministic FP64 reductions,
-// stopped by the reference's rule (|r|_inf <= min(tol |b|_inf, 1), success
-// also at the iteration limit below an error of 10).
+// stopped on the device by the reference's rule (|r|_inf <=
+// min(tol |b|_inf, 1), success also at the iteration limit below an error of
+// 10).
 #pragma once
 
 #include <vector>

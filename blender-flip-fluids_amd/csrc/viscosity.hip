@@ -27,6 +27,8 @@
 //                    vectors, FP64 reductions in a fixed order (the same bits
 //                    every run), stopped by the reference's rule
 //                    (pcgsolver.h:268-300, viscositysolver.cpp:832-868)
+//                    checked on the device, the host reading it once per
+//                    batch of iterations
 //   apply            on success the field is cleared and the matrix faces
 //                    take the solution (870-905)
 //
@@ -48,10 +50,11 @@ void hipCheck(hipError_t e, const char *what);
 namespace {
 
 constexpr int TPB = 256;
-constexpr int ELL = 16;  // slots per row: 15 entries at most, one padding
+constexpr int ELL = 16;    // slots per row: 15 entries at most, one padding
+constexpr int BATCH = 8;   // CG iterations per host check of the stopping rule
 constexpr int RED_BLOCKS = 1024;
 enum : unsigned char { VF_AIR = 0, VF_FLUID = 1, VF_SOLID = 2 };
-enum { S_RHO = 0, S_ALPHA, S_BETA, S_RESID, S_COUNT = 8 };
+enum { S_RHO = 0, S_ALPHA, S_BETA, S_RESID, S_DONE, S_ITERS, S_NAN, S_COUNT = 8 };
 
 unsigned blocksFor(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
@@ -236,54 +239,66 @@ __global__ void tableKernel(size_t F, const int *flag, const int *rowOf, int *ta
 }
 
 // ------------------------------------------------------------- assembly ---
-// one off-diagonal candidate: the neighbour face's state and row, the
-// coefficient and the neighbour's velocity
-struct Entry {
-    unsigned char st;
-    int col;
-    float coef, vel;
+// The 14 neighbours of a row, in the order the reference lists them
+// (viscositysolver.cpp:610-650 for u rows, 700-740 for v, 790-830 for w):
+// the face grid (0 u, 1 v, 2 w) and the offset from the row's (i, j, k).
+struct Nb {
+    signed char dir, di, dj, dk;
 };
+__constant__ Nb NB[3][14] = {
+    {{0, 1, 0, 0}, {0, -1, 0, 0}, {0, 0, 1, 0}, {0, 0, -1, 0}, {0, 0, 0, 1}, {0, 0, 0, -1}, {1, 0, 1, 0},
+     {1, -1, 1, 0}, {1, 0, 0, 0}, {1, -1, 0, 0}, {2, 0, 0, 1}, {2, -1, 0, 1}, {2, 0, 0, 0}, {2, -1, 0, 0}},
+    {{1, 1, 0, 0}, {1, -1, 0, 0}, {1, 0, 1, 0}, {1, 0, -1, 0}, {1, 0, 0, 1}, {1, 0, 0, -1}, {0, 1, 0, 0},
+     {0, 1, -1, 0}, {0, 0, 0, 0}, {0, 0, -1, 0}, {2, 0, 0, 1}, {2, 0, -1, 1}, {2, 0, 0, 0}, {2, 0, -1, 0}},
+    {{2, 1, 0, 0}, {2, -1, 0, 0}, {2, 0, 1, 0}, {2, 0, -1, 0}, {2, 0, 0, 1}, {2, 0, 0, -1}, {0, 1, 0, 0},
+     {0, 1, 0, -1}, {0, 0, 0, 0}, {0, 0, 0, -1}, {1, 0, 1, 0}, {1, 0, 1, -1}, {1, 0, 0, 0}, {1, 0, 0, -1}}};
+// The concatenated face index of neighbour e of row (dir, i, j, k), or -1
+// when it lies outside its face grid (U(i, nj, k), U(i, j, nk), V(ni, j, k),
+// V(i, j, nk), W(ni, j, k), W(i, nj, k)).  The reference then reads the
+// state and row at the wrapped flat index, a border face that is never a
+// row or a place past the array, and the velocity as 0
+// (macvelocityfield.cpp:112-118): such a neighbour adds nothing, and here it
+// is air.
+__device__ inline long long nbFaceOf(const Nb n, const Dims &D, int i, int j, int k) {
+    const int a = i + n.di, b = j + n.dj, c = k + n.dk;
+    if (n.dir == 0) return inRange(a, b, c, D.ni + 1, D.nj, D.nk) ? (long long)flat(a, b, c, D.ni + 1, D.nj) : -1;
+    if (n.dir == 1)
+        return inRange(a, b, c, D.ni, D.nj + 1, D.nk) ? (long long)(D.nU + flat(a, b, c, D.ni, D.nj + 1)) : -1;
+    return inRange(a, b, c, D.ni, D.nj, D.nk + 1) ? (long long)(D.nU + D.nV + flat(a, b, c, D.ni, D.nj)) : -1;
+}
+__device__ inline long long nbFace(const Dims &D, int dir, int i, int j, int k, int e) {
+    return nbFaceOf(NB[dir][e], D, i, j, k);
+}
 struct FaceView {
     const unsigned char *state;
     const int *table;
     const float *u, *v, *w, *visc;
     Dims D;
     __device__ float VS(int i, int j, int k) const { return visc[flat(i, j, k, D.ni + 1, D.nj + 1)]; }
-    __device__ float Uv(int i, int j, int k) const { return u[flat(i, j, k, D.ni + 1, D.nj)]; }
-    __device__ float Vv(int i, int j, int k) const { return v[flat(i, j, k, D.ni, D.nj + 1)]; }
-    __device__ float Wv(int i, int j, int k) const { return w[flat(i, j, k, D.ni, D.nj)]; }
-    // A neighbour outside its face grid (U(i, nj, k), U(i, j, nk), V(ni, j, k),
-    // V(i, j, nk), W(ni, j, k), W(i, nj, k)) adds nothing: the reference reads
-    // its state and row at the wrapped flat index, a border face that is
-    // never a row or a place past the array, and its velocity as 0
-    // (macvelocityfield.cpp:112-118).  Here it is air.
-    __device__ Entry eU(int i, int j, int k, float coef) const {
-        if (!inRange(i, j, k, D.ni + 1, D.nj, D.nk)) return Entry{VF_AIR, -1, coef, 0.0f};
-        const size_t f = flat(i, j, k, D.ni + 1, D.nj);
-        return Entry{state[f], table[f], coef, u[f]};
-    }
-    __device__ Entry eV(int i, int j, int k, float coef) const {
-        if (!inRange(i, j, k, D.ni, D.nj + 1, D.nk)) return Entry{VF_AIR, -1, coef, 0.0f};
-        const size_t f = flat(i, j, k, D.ni, D.nj + 1);
-        return Entry{state[D.nU + f], table[D.nU + f], coef, v[f]};
-    }
-    __device__ Entry eW(int i, int j, int k, float coef) const {
-        if (!inRange(i, j, k, D.ni, D.nj, D.nk + 1)) return Entry{VF_AIR, -1, coef, 0.0f};
-        const size_t f = flat(i, j, k, D.ni, D.nj);
-        return Entry{state[D.nU + D.nV + f], table[D.nU + D.nV + f], coef, w[f]};
+    // the velocity of concatenated face f
+    __device__ float vel(long long f) const {
+        if (f < (long long)D.nU) return u[f];
+        if (f < (long long)(D.nU + D.nV)) return v[f - D.nU];
+        return w[f - D.nU - D.nV];
     }
 };
 
+// One thread per row: the row of the viscous operator and its right-hand
+// side in the reference's arithmetic (viscositysolver.cpp:559-830).  The
+// entries (the diagonal and the fluid neighbours that are rows) are sorted
+// by column into a 16-slot ELL block stored slot-major (padding: column =
+// row, value 0); solid neighbours move to the right-hand side in list order.
 __global__ void assembleKernel(FaceView Fv, VolView Vo, float fdx, float fdt, int m, const int *rowFace,
                                int *ellCol, float *ellVal, int *nnz, float *rhs, float *dinv) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= m) return;
     int i, j, k;
-    const int dir = decodeFace(Fv.D, (size_t)rowFace[row], i, j, k);
+    const Dims &D = Fv.D;
+    const int dir = decodeFace(D, (size_t)rowFace[row], i, j, k);
     const float invdx = 1.0f / fdx;
     const float factor = fdt * invdx * invdx;
     auto VS = [&](int a, int b, int c) { return Fv.VS(a, b, c); };
-    Entry es[14];
+    float c[14];
     float diag, rval;
     if (dir == 0) {
         const float vR = VS(i, j, k), vL = VS(i - 1, j, k);
@@ -295,21 +310,9 @@ __global__ void assembleKernel(FaceView Fv, VolView Vo, float fdx, float fdt, in
         const float fT = factor * vT * Vo.EW(i, j + 1, k), fB = factor * vB * Vo.EW(i, j, k);
         const float fF = factor * vF * Vo.EV(i, j, k + 1), fK = factor * vK * Vo.EV(i, j, k);
         diag = Vo.VU(i, j, k) + fR + fL + fT + fB + fF + fK;
-        es[0] = Fv.eU(i + 1, j, k, -fR);
-        es[1] = Fv.eU(i - 1, j, k, -fL);
-        es[2] = Fv.eU(i, j + 1, k, -fT);
-        es[3] = Fv.eU(i, j - 1, k, -fB);
-        es[4] = Fv.eU(i, j, k + 1, -fF);
-        es[5] = Fv.eU(i, j, k - 1, -fK);
-        es[6] = Fv.eV(i, j + 1, k, -fT);
-        es[7] = Fv.eV(i - 1, j + 1, k, fT);
-        es[8] = Fv.eV(i, j, k, fB);
-        es[9] = Fv.eV(i - 1, j, k, -fB);
-        es[10] = Fv.eW(i, j, k + 1, -fF);
-        es[11] = Fv.eW(i - 1, j, k + 1, fF);
-        es[12] = Fv.eW(i, j, k, fK);
-        es[13] = Fv.eW(i - 1, j, k, -fK);
-        rval = Vo.VU(i, j, k) * Fv.Uv(i, j, k);
+        const float cs[14] = {-fR, -fL, -fT, -fB, -fF, -fK, -fT, fT, fB, -fB, -fF, fF, fK, -fK};
+        for (int e = 0; e < 14; e++) c[e] = cs[e];
+        rval = Vo.VU(i, j, k) * Fv.u[flat(i, j, k, D.ni + 1, D.nj)];
     } else if (dir == 1) {
         const float vT = VS(i, j, k), vB = VS(i, j - 1, k);
         const float vR = 0.25f * (VS(i, j - 1, k) + VS(i + 1, j - 1, k) + VS(i, j, k) + VS(i + 1, j, k));
@@ -320,21 +323,9 @@ __global__ void assembleKernel(FaceView Fv, VolView Vo, float fdx, float fdt, in
         const float fT = 2 * factor * vT * Vo.CE(i, j, k), fB = 2 * factor * vB * Vo.CE(i, j - 1, k);
         const float fF = factor * vF * Vo.EU(i, j, k + 1), fK = factor * vK * Vo.EU(i, j, k);
         diag = Vo.VV(i, j, k) + fR + fL + fT + fB + fF + fK;
-        es[0] = Fv.eV(i + 1, j, k, -fR);
-        es[1] = Fv.eV(i - 1, j, k, -fL);
-        es[2] = Fv.eV(i, j + 1, k, -fT);
-        es[3] = Fv.eV(i, j - 1, k, -fB);
-        es[4] = Fv.eV(i, j, k + 1, -fF);
-        es[5] = Fv.eV(i, j, k - 1, -fK);
-        es[6] = Fv.eU(i + 1, j, k, -fR);
-        es[7] = Fv.eU(i + 1, j - 1, k, fR);
-        es[8] = Fv.eU(i, j, k, fL);
-        es[9] = Fv.eU(i, j - 1, k, -fL);
-        es[10] = Fv.eW(i, j, k + 1, -fF);
-        es[11] = Fv.eW(i, j - 1, k + 1, fF);
-        es[12] = Fv.eW(i, j, k, fK);
-        es[13] = Fv.eW(i, j - 1, k, -fK);
-        rval = Vo.VV(i, j, k) * Fv.Vv(i, j, k);
+        const float cs[14] = {-fR, -fL, -fT, -fB, -fF, -fK, -fR, fR, fL, -fL, -fF, fF, fK, -fK};
+        for (int e = 0; e < 14; e++) c[e] = cs[e];
+        rval = Vo.VV(i, j, k) * Fv.v[flat(i, j, k, D.ni, D.nj + 1)];
     } else {
         const float vF = VS(i, j, k), vK = VS(i, j, k - 1);
         const float vR = 0.25f * (VS(i, j, k) + VS(i, j, k - 1) + VS(i + 1, j, k) + VS(i + 1, j, k - 1));
@@ -345,34 +336,20 @@ __global__ void assembleKernel(FaceView Fv, VolView Vo, float fdx, float fdt, in
         const float fT = factor * vT * Vo.EU(i, j + 1, k), fB = factor * vB * Vo.EU(i, j, k);
         const float fF = 2 * factor * vF * Vo.CE(i, j, k), fK = 2 * factor * vK * Vo.CE(i, j, k - 1);
         diag = Vo.VW(i, j, k) + fR + fL + fT + fB + fF + fK;
-        es[0] = Fv.eW(i + 1, j, k, -fR);
-        es[1] = Fv.eW(i - 1, j, k, -fL);
-        es[2] = Fv.eW(i, j + 1, k, -fT);
-        es[3] = Fv.eW(i, j - 1, k, -fB);
-        es[4] = Fv.eW(i, j, k + 1, -fF);
-        es[5] = Fv.eW(i, j, k - 1, -fK);
-        es[6] = Fv.eU(i + 1, j, k, -fR);
-        es[7] = Fv.eU(i + 1, j, k - 1, fR);
-        es[8] = Fv.eU(i, j, k, fL);
-        es[9] = Fv.eU(i, j, k - 1, -fL);
-        es[10] = Fv.eV(i, j + 1, k, -fT);
-        es[11] = Fv.eV(i, j + 1, k - 1, fT);
-        es[12] = Fv.eV(i, j, k, fB);
-        es[13] = Fv.eV(i, j, k - 1, -fB);
-        rval = Vo.VW(i, j, k) * Fv.Wv(i, j, k);
+        const float cs[14] = {-fR, -fL, -fT, -fB, -fF, -fK, -fR, fR, fL, -fL, -fT, fT, fB, -fB};
+        for (int e = 0; e < 14; e++) c[e] = cs[e];
+        rval = Vo.VW(i, j, k) * Fv.w[flat(i, j, k, D.ni, D.nj)];
     }
-    // matrix entries: the diagonal and the fluid neighbours that are rows,
-    // in ascending column order; solid neighbours move to the right-hand side
     int cols[15];
     float vals[15];
     cols[0] = row;
     vals[0] = diag;
     int cnt = 1;
-#pragma unroll
     for (int e = 0; e < 14; e++) {
-        const unsigned char st = es[e].st;
-        if (st == VF_SOLID) rval -= es[e].coef * es[e].vel;
-        const int col = es[e].col;
+        const long long f = nbFace(D, dir, i, j, k, e);
+        const unsigned char st = f >= 0 ? Fv.state[f] : (unsigned char)VF_AIR;
+        if (st == VF_SOLID) rval -= c[e] * Fv.vel(f);
+        const int col = f >= 0 ? Fv.table[f] : -1;
         if (st == VF_FLUID && col >= 0) {
             int p = cnt++;
             while (p > 0 && cols[p - 1] > col) {
@@ -381,12 +358,12 @@ __global__ void assembleKernel(FaceView Fv, VolView Vo, float fdx, float fdt, in
                 p--;
             }
             cols[p] = col;
-            vals[p] = es[e].coef;
+            vals[p] = c[e];
         }
     }
-    for (int s = 0; s < ELL; s++) {
-        ellCol[(size_t)s * m + row] = s < cnt ? cols[s] : row;
-        ellVal[(size_t)s * m + row] = s < cnt ? vals[s] : 0.0f;
+    for (int sl = 0; sl < ELL; sl++) {
+        ellCol[(size_t)sl * m + row] = sl < cnt ? cols[sl] : row;
+        ellVal[(size_t)sl * m + row] = sl < cnt ? vals[sl] : 0.0f;
     }
     nnz[row] = cnt;
     rhs[row] = rval;
@@ -413,6 +390,12 @@ __device__ inline double blockMax(double v, double *sh) {
     }
     return sh[0];
 }
+// The CG iterations run in batches without a host round trip: betaKernel
+// sets scal[S_DONE] when the residual meets the stopping rule (or is NaN)
+// and records the iteration, and every later kernel of the batch returns at
+// once.
+__device__ inline bool finished(const double *scal) { return scal[S_DONE] != 0.0; }
+
 // x = 0, r = b, z = D^-1 r, s = z; partials of |r|_inf and z.r
 __global__ __launch_bounds__(TPB) void initKernel(int m, const float *rhs, const float *dinv, float *x, float *r,
                                                   float *z, float *s, double *pmax, double *pdot) {
@@ -450,11 +433,15 @@ __global__ __launch_bounds__(TPB) void initFinishKernel(int nb, const double *pm
     if (threadIdx.x == 0) {
         scal[S_RESID] = mx;
         scal[S_RHO] = acc;
+        scal[S_DONE] = 0.0;
+        scal[S_ITERS] = 0.0;
+        scal[S_NAN] = 0.0;
     }
 }
 // q = A s and partials of s.q
 __global__ __launch_bounds__(TPB) void spmvKernel(int m, const int *ellCol, const float *ellVal, const float *s,
-                                                  float *qv, double *pdot) {
+                                                  float *qv, const double *scal, double *pdot) {
+    if (finished(scal)) return;
     __shared__ double sh[TPB];
     double acc = 0.0;
     for (size_t q = (size_t)blockIdx.x * TPB + threadIdx.x; q < (size_t)m; q += (size_t)gridDim.x * TPB) {
@@ -468,6 +455,7 @@ __global__ __launch_bounds__(TPB) void spmvKernel(int m, const int *ellCol, cons
     if (threadIdx.x == 0) pdot[blockIdx.x] = acc;
 }
 __global__ __launch_bounds__(TPB) void alphaKernel(int nb, const double *pdot, double *scal) {
+    if (finished(scal)) return;
     __shared__ double sh[TPB];
     double acc = 0.0;
     for (int q = threadIdx.x; q < nb; q += TPB) acc += pdot[q];
@@ -478,6 +466,7 @@ __global__ __launch_bounds__(TPB) void alphaKernel(int nb, const double *pdot, d
 __global__ __launch_bounds__(TPB) void updateKernel(int m, const float *s, const float *qv, const float *dinv,
                                                     float *x, float *r, float *z, const double *scal, double *pmax,
                                                     double *pdot) {
+    if (finished(scal)) return;
     __shared__ double sh[TPB];
     const float a = (float)scal[S_ALPHA];
     double mx = 0.0, acc = 0.0;
@@ -498,7 +487,11 @@ __global__ __launch_bounds__(TPB) void updateKernel(int m, const float *s, const
         pdot[blockIdx.x] = acc;
     }
 }
-__global__ __launch_bounds__(TPB) void betaKernel(int nb, const double *pmax, const double *pdot, double *scal) {
+// the residual's norm, the stopping rule |r|_inf <= min(tol |b|_inf, 1) of
+// iteration it, and beta
+__global__ __launch_bounds__(TPB) void betaKernel(int nb, const double *pmax, const double *pdot, double *scal,
+                                                  double stopAt, int it) {
+    if (finished(scal)) return;
     __shared__ double sh[TPB];
     double mx = 0.0, acc = 0.0;
     for (int q = threadIdx.x; q < nb; q += TPB) {
@@ -512,10 +505,16 @@ __global__ __launch_bounds__(TPB) void betaKernel(int nb, const double *pmax, co
         scal[S_RESID] = mx;
         scal[S_BETA] = acc / scal[S_RHO];
         scal[S_RHO] = acc;
+        if (mx <= stopAt || mx != mx) {
+            scal[S_DONE] = 1.0;
+            scal[S_ITERS] = it + 1;
+            scal[S_NAN] = mx != mx ? 1.0 : 0.0;
+        }
     }
 }
 // s = z + beta s
 __global__ __launch_bounds__(TPB) void directionKernel(int m, const float *z, float *s, const double *scal) {
+    if (finished(scal)) return;
     const float b = (float)scal[S_BETA];
     for (size_t q = (size_t)blockIdx.x * TPB + threadIdx.x; q < (size_t)m; q += (size_t)gridDim.x * TPB)
         s[q] = z[q] + b * s[q];
@@ -590,8 +589,8 @@ void ViscositySolver::allocateRows(int m) {
     _rowBlock = nullptr;
     const int cap = std::max(m + m / 4, 1024);
     const size_t n = (size_t)cap;
-    size_t bytes = 2 * align256(4 * ELL * n) + 2 * align256(4 * n) + 8 * align256(4 * n) +
-                   align256(8 * 2 * RED_BLOCKS) + align256(8 * S_COUNT);
+    size_t bytes = 2 * align256(4 * ELL * n) + 10 * align256(4 * n) + align256(8 * 2 * RED_BLOCKS) +
+                   align256(8 * S_COUNT);
     hipCheck(hipMalloc(&_rowBlock, bytes), "viscosity alloc");
     char *p = (char *)_rowBlock;
     auto take = [&](size_t b) {
@@ -610,7 +609,6 @@ void ViscositySolver::allocateRows(int m) {
     _z = (float *)take(4 * n);
     _s = (float *)take(4 * n);
     _q = (float *)take(4 * n);
-    take(4 * n);  // spare row vector
     _partial = (double *)take(8 * 2 * RED_BLOCKS);
     _scal = (double *)take(8 * S_COUNT);
     _rowCap = cap;
@@ -656,6 +654,7 @@ ViscosityResult ViscositySolver::solve(Grid g, double dt, const float *liquidPhi
     if (m == 0) return res;  // nothing to solve: the field is kept
     const unsigned nb = std::min<unsigned>(blocksFor((size_t)m, TPB), RED_BLOCKS);
     double *pmax = _partial, *pdot = _partial + RED_BLOCKS;
+    const Dims D = dimsOf(g);
     initKernel<<<nb, TPB, 0, s>>>(m, _rhs, _dinv, _x, _r, _z, _s, pmax, pdot);
     initFinishKernel<<<1, TPB, 0, s>>>((int)nb, pmax, pdot, _scal);
     hipCheck(hipMemcpyAsync(_hostScal, _scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s), "viscosity D2H");
@@ -673,33 +672,35 @@ ViscosityResult ViscositySolver::solve(Grid g, double dt, const float *liquidPhi
         res.iterations = 0;
         return res;
     }
-    for (it = 0; it < maxIter && !ok; it++) {
-        spmvKernel<<<nb, TPB, 0, s>>>(m, _ellCol, _ellVal, _s, _q, pdot);
-        alphaKernel<<<1, TPB, 0, s>>>((int)nb, pdot, _scal);
-        updateKernel<<<nb, TPB, 0, s>>>(m, _s, _q, _dinv, _x, _r, _z, _scal, pmax, pdot);
-        betaKernel<<<1, TPB, 0, s>>>((int)nb, pmax, pdot, _scal);
+    const double stopAt = std::min(tolAbs, 1.0);
+    bool nan = false;
+    while (!ok && !nan && it < maxIter) {
+        const int end = std::min(it + BATCH, maxIter);
+        for (int k = it; k < end; k++) {
+            spmvKernel<<<nb, TPB, 0, s>>>(m, _ellCol, _ellVal, _s, _q, _scal, pdot);
+            alphaKernel<<<1, TPB, 0, s>>>((int)nb, pdot, _scal);
+            updateKernel<<<nb, TPB, 0, s>>>(m, _s, _q, _dinv, _x, _r, _z, _scal, pmax, pdot);
+            betaKernel<<<1, TPB, 0, s>>>((int)nb, pmax, pdot, _scal, stopAt, k);
+            directionKernel<<<nb, TPB, 0, s>>>(m, _z, _s, _scal);
+        }
+        hipCheck(hipGetLastError(), "viscosity CG launch");
         hipCheck(hipMemcpyAsync(_hostScal, _scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s),
                  "viscosity D2H");
-        // the next direction does not change the iterate; it is queued
-        // before the residual is read so the device keeps busy
-        directionKernel<<<nb, TPB, 0, s>>>(m, _z, _s, _scal);
         hipCheck(hipStreamSynchronize(s), "viscosity sync");
-        const double resid = _hostScal[S_RESID];
-        res.error = (float)resid;
-        if (resid <= std::min(tolAbs, 1.0)) {
-            ok = true;
-            it++;
-            break;
-        }
-        if (resid != resid) {  // NaN: the reference would run to the limit and fail
-            it = maxIter;
-            break;
+        it = end;
+        res.error = (float)_hostScal[S_RESID];
+        if (_hostScal[S_DONE] != 0.0) {
+            it = (int)_hostScal[S_ITERS];
+            if (_hostScal[S_NAN] != 0.0)
+                nan = true;  // the reference would run to the limit and fail
+            else
+                ok = true;
         }
     }
+    if (nan) it = maxIter;
     res.iterations = it;
-    res.success = ok || (it == maxIter && res.error < 10.0f);
+    res.success = ok || (it == maxIter && !nan && res.error < 10.0f);
     if (!res.success) return res;
-    const Dims D = dimsOf(g);
     scatterKernel<<<blocksFor(_faces, TPB), TPB, 0, s>>>(D, _table, _x, u, v, w);
     hipCheck(hipGetLastError(), "viscosity apply launch");
     return res;
