@@ -664,8 +664,9 @@ def main():
                     help="host threads of the CPU baseline (default: min(16, cpu count))")
     ap.add_argument("--cpu-substeps", type=int, default=3, help="cpu-calibration: timed substeps per engine")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--schedule", type=int, default=None, choices=[2, 4],
-                    help="MIC(0) application: 2 = bands of hyperplanes, 4 = line pipeline (default: the engine's)")
+    ap.add_argument("--schedule", type=int, default=None, choices=[2, 4, 5],
+                    help="MIC(0) application: 2 = bands of hyperplanes, 4 = line pipeline, 5 = the bands "
+                         "replayed from a HIP graph (default: the engine's)")
     ap.add_argument("--modes", default="fp64,mixed,mixed-reliable,fp32-preconditioner",
                     help="comma list of PCG modes: fp64, mixed (FP32 PCG restarted in FP64 refinement), "
                          "mixed-reliable (FP32 PCG with FP64 reliable updates), fp32-preconditioner (FP64 "

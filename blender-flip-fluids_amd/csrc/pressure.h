@@ -58,7 +58,8 @@ public:
     // null: none)
     void setTension(TensionView t) { _tension = t; }
     // the solver's MIC(0) application: 2 = bands of hyperplanes, 4 = the
-    // line pipeline (two launches per application)
+    // line pipeline (two launches per application), 5 = the bands of one
+    // application replayed from a HIP graph captured once per solve
     void setSchedule(int schedule);
     int schedule() const { return _sched; }
     const std::vector<int> &levelTable() const { return _hLevelStart; }
@@ -98,6 +99,18 @@ private:
     void precond(const double *r, double *y, double *z, hipStream_t s);
     void precondF(const float *r, float *y, float *z, hipStream_t s);
     int applyLaunches() const { return _sched == 4 ? 2 : _bandLaunches; }
+    // schedule 5: executable graphs of the band launches for the (r, z)
+    // pairs of the current solve's applications (the PCG's z alternates
+    // between two buffers), dropped when a new system is assembled
+    struct BandGraph {
+        const void *r = nullptr, *z = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    BandGraph _graphs[2];
+    int _graphNext = 0;
+    void dropGraphs();
+    template <class T>
+    void graphApply(const T *r, T *y, T *z, hipStream_t s);
     int _refDots = 0;  // thread count of the reference's dot order, or 0 for the tree reductions
     bool _warm = false;
     SolidTerms _solid{};

@@ -11,6 +11,7 @@ namespace ff {
 void hipCheck(hipError_t e, const char *what);
 
 PressureSolver::~PressureSolver() {
+    dropGraphs();
     if (_block) (void)hipFree(_block);
     if (_hostScal) (void)hipHostFree(_hostScal);
 }
@@ -190,20 +191,70 @@ void PressureSolver::checkError(hipStream_t s) {
 }
 
 void PressureSolver::setSchedule(int schedule) {
-    if (schedule != 2 && schedule != 4)
-        throw std::domain_error("the solver's preconditioner schedule must be 2 (bands) or 4 (line pipeline)");
+    if (schedule != 2 && schedule != 4 && schedule != 5)
+        throw std::domain_error(
+            "the solver's preconditioner schedule must be 2 (bands), 4 (line pipeline) or 5 (bands from a graph)");
     _sched = schedule;
+}
+
+void PressureSolver::dropGraphs() {
+    for (BandGraph &b : _graphs) {
+        if (b.exec) (void)hipGraphExecDestroy(b.exec);
+        b = BandGraph{};
+    }
+    _graphNext = 0;
+}
+
+// The band launches of one application are the same for every application
+// of a solve (same plan, same buffers), so they are recorded once per (r, z)
+// pair by stream capture and replayed with one hipGraphLaunch.  The capture
+// holds kernel launches only.
+template <class T>
+void PressureSolver::graphApply(const T *r, T *y, T *z, hipStream_t s) {
+    if (_S.m == 0) return;
+    BandGraph *slot = nullptr;
+    for (BandGraph &b : _graphs)
+        if (b.exec && b.r == r && b.z == z) slot = &b;
+    if (!slot) {
+        slot = &_graphs[_graphNext];
+        _graphNext = (_graphNext + 1) % 2;
+        if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+        *slot = BandGraph{};
+        hipGraph_t graph = nullptr;
+        hipCheck(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "begin capture");
+        try {
+            if constexpr (sizeof(T) == 8)
+                mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, r, y, z, s);
+            else
+                mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, r, y, z, s);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        hipCheck(hipStreamEndCapture(s, &graph), "end capture");
+        const hipError_t e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hipCheck(e, "instantiate the band graph");
+        slot->r = r;
+        slot->z = z;
+    }
+    hipCheck(hipGraphLaunch(slot->exec, s), "launch the band graph");
 }
 
 void PressureSolver::precond(const double *r, double *y, double *z, hipStream_t s) {
     if (_sched == 4)
         pencilApply(_g, _S, _pp, r, y, z, s);
+    else if (_sched == 5)
+        graphApply(r, y, z, s);
     else
         mic0ApplyBands(_g, _S, _plan, _tileLo, _tileHi, r, y, z, s);
 }
 void PressureSolver::precondF(const float *r, float *y, float *z, hipStream_t s) {
     if (_sched == 4)
         pencilApplyF(_g, _S, _pp, r, y, z, s);
+    else if (_sched == 5)
+        graphApply(r, y, z, s);
     else
         mic0ApplyBandsF(_g, _S, _plan, _tileLo, _tileHi, r, y, z, s);
 }
@@ -305,6 +356,7 @@ PressureResult PressureSolver::solve(double dt, const float *u, const float *v, 
     _levelLaunches = mic0FactorBandLaunches(_S, _hLevelStart.data());
     mic0BandPlan(_g, _S, _hLevelStart.data(), mode == 0, _hTileLo.data(), _hTileHi.data(), _plan);
     _bandLaunches = (int)_plan.l0.size();
+    dropGraphs();  // recorded with the previous system's plan
     // The forward solve skips a right-hand side below the tolerance as an
     // absolute threshold, as the reference does (pressuresolver.cpp:69-75).  A
     // given right-hand side (the adjoint's) is solved unless it is exactly 0,

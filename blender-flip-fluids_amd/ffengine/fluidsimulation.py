@@ -341,7 +341,8 @@ class FluidSimulation(object):
 
     def set_pressure_solver_preconditioner_schedule(self, schedule):
         """Schedule of the MIC(0) application: 2 = bands of hyperplanes,
-        4 = the line pipeline (one launch per triangular sweep); same bits."""
+        4 = the line pipeline (one launch per triangular sweep), 5 = the bands
+        recorded once per solve as a HIP graph and replayed; same bits."""
         f = lib.FluidSimulation_set_pressure_solver_preconditioner_schedule
         pb.init_lib_func(f, [c_void_p, c_int, c_void_p], None)
         pb.execute_lib_func(f, [self(), int(schedule)])

@@ -225,7 +225,9 @@ int FluidSimulation_get_pressure_solver_warm_start(FluidSimulation *obj, int *er
 /* Schedule of the MIC(0) preconditioner application (no counterpart in the
  * reference, whose triangular solves are serial loops, pcgsolver.h:211-241):
  * 2 = bands of i+j+k hyperplanes, one launch per band; 4 = the line
- * pipeline, one launch per triangular sweep.  Both give the same bits. */
+ * pipeline, one launch per triangular sweep; 5 = the bands of schedule 2
+ * recorded once per solve as a HIP graph and replayed per application.  All
+ * give the same bits. */
 void FluidSimulation_set_pressure_solver_preconditioner_schedule(FluidSimulation *obj, int schedule,
                                                                  int *err);
 int FluidSimulation_get_pressure_solver_preconditioner_schedule(FluidSimulation *obj, int *err);
