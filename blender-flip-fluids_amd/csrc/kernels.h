@@ -335,7 +335,7 @@ struct PressureSys {
     // of the word's first liquid cell (nj*nk lines x nw words; SpMV and the
     // line-pipeline sweeps find neighbour rows in them)
     LineWord *words;
-    int nw, ni, nj;
+    int nw, ni, nj, nk;
     int m;
     int nLevels;
 };

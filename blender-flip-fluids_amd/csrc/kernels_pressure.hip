@@ -329,12 +329,75 @@ __device__ inline void rowProducts(const int (&r)[U], const RowTopo &t, const T 
     }
 }
 
+// Neighbour lookup of the fused SpMV: the rows' six neighbour ids (RowTopo,
+// the default) or, built with -DFF_SPMV_WORDS=1, the rows' cells and the line
+// words (WordTopo: 4 B per row instead of 24, one more dependent load; it
+// measured slower at 256^3, 148 ms against 107 ms of SpMV per 10 substeps).
+#ifndef FF_SPMV_WORDS
+#define FF_SPMV_WORDS 0
+#endif
+// The rows' neighbours found from the rows' cells and the line words (no
+// per-row neighbour ids): bit i%32 of word i/32 of line (j, k) marks a
+// pressure cell, whose row is the word's base plus the marked cells below it
+// in the word.  Rows lie on interior cells, so every neighbour cell is inside
+// the grid.
+struct WordTopo {
+    int m, ni, nj, nw;
+    const int *rows;  // flat cell of each row, (k*nj + j)*ni + i
+    const LineWord *words;
+};
+
+__device__ inline int wordRow(const LineWord *__restrict__ words, size_t line, int nw, int i) {
+    const LineWord w = words[line * nw + (i >> 5)];
+    const unsigned bit = 1u << (i & 31);
+    return (w.bits & bit) ? w.base + __builtin_popcount(w.bits & (bit - 1u)) : -1;
+}
+
+template <class T, int U>
+__device__ inline void rowProducts(const int (&r)[U], const WordTopo &t, const T *__restrict__ diag,
+                                    const T *__restrict__ ox, const T *__restrict__ oy, const T *__restrict__ oz,
+                                    const T *__restrict__ x, T (&out)[U]) {
+    int cell[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) cell[u] = t.rows[r[u]];
+    int nb[U][6];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int c = cell[u], i = c % t.ni, line = c / t.ni;
+        const size_t nj = (size_t)t.nj;
+        nb[u][0] = wordRow(t.words, line, t.nw, i - 1);
+        nb[u][1] = wordRow(t.words, line, t.nw, i + 1);
+        nb[u][2] = wordRow(t.words, line - 1, t.nw, i);
+        nb[u][3] = wordRow(t.words, line + 1, t.nw, i);
+        nb[u][4] = wordRow(t.words, line - nj, t.nw, i);
+        nb[u][5] = wordRow(t.words, line + nj, t.nw, i);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int q = r[u];
+        const int n0 = nb[u][0], n1 = nb[u][1], n2 = nb[u][2], n3 = nb[u][3], n4 = nb[u][4], n5 = nb[u][5];
+        const int a0 = n0 >= 0 ? n0 : q, a1 = n1 >= 0 ? n1 : q, a2 = n2 >= 0 ? n2 : q, a3 = n3 >= 0 ? n3 : q,
+                  a4 = n4 >= 0 ? n4 : q, a5 = n5 >= 0 ? n5 : q;
+        const T cmz = oz[a4], cmy = oy[a2], cmx = ox[a0], cd = diag[q], cox = ox[q], coy = oy[q], coz = oz[q];
+        const T x4 = x[a4], x2 = x[a2], x0 = x[a0], xr = x[q], x1 = x[a1], x3 = x[a3], x5 = x[a5];
+        T s = 0;
+        if (n4 >= 0) s += cmz * x4;
+        if (n2 >= 0) s += cmy * x2;
+        if (n0 >= 0) s += cmx * x0;
+        s += cd * xr;
+        if (n1 >= 0) s += cox * x1;
+        if (n3 >= 0) s += coy * x3;
+        if (n5 >= 0) s += coz * x5;
+        out[u] = s;
+    }
+}
+
 // SpMV over the rows of the reduction spans, with each thread's partial of
 // s . q in the order of the standalone dot product (its rows in ascending
 // order, SPMV_ROWS at a time)
 constexpr int SPMV_ROWS = 4;
-template <class T>
-__global__ __launch_bounds__(REDUCE_THREADS) void spmvDotKernel(RowTopo t, const T *__restrict__ diag,
+template <class T, class Topo>
+__global__ __launch_bounds__(REDUCE_THREADS) void spmvDotKernel(Topo t, const T *__restrict__ diag,
                                                                 const T *__restrict__ ox, const T *__restrict__ oy,
                                                                 const T *__restrict__ oz, const T *__restrict__ x,
                                                                 T *__restrict__ y, double *partial) {
@@ -556,8 +619,13 @@ template <class T>
 static void launchSpmvDot(const PressureSys &S, const T *diag, const T *ox, const T *oy, const T *oz, const T *x,
                           T *y, double *partial, double *sq, hipStream_t st) {
     const int nb = reduceBlocks(S.m);
-    hipLaunchKernelGGL(spmvDotKernel<T>, dim3(nb), dim3(REDUCE_THREADS), 0, st, topo(S), diag, ox, oy, oz, x, y,
-                       partial);
+#if FF_SPMV_WORDS
+    WordTopo t{S.m, S.ni, S.nj, S.nw, S.rows, S.words};
+#else
+    RowTopo t = topo(S);
+#endif
+    hipLaunchKernelGGL((spmvDotKernel<T, decltype(t)>), dim3(nb), dim3(REDUCE_THREADS), 0, st, t, diag, ox, oy, oz,
+                       x, y, partial);
     reduceFinish(partial, nb, sq, false, st);
 }
 void spmvDot(const PressureSys &S, const double *x, double *y, double *partial, double *sq, hipStream_t st) {

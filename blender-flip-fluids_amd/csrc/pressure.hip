@@ -106,6 +106,7 @@ void PressureSolver::allocate(Grid g) {
     _S.nw = _pp.nw;
     _S.ni = g.ni;
     _S.nj = g.nj;
+    _S.nk = g.nk;
     _pp.ex = (unsigned long long *)(b + oPEx);
     _pp.counter = (unsigned *)(b + oPCounter);
     _pp.error = _ms.error;
